@@ -1,0 +1,387 @@
+"""MI355X (gfx950) ray-query library: Python mirror of LiteRT's ISceneObject / BVHRT.
+
+The compute path is the C-ABI shared library ``liblrt_hip.so`` built from ``csrc/``
+(hand-written HIP kernels for gfx950 plus the host scene assembly). This module only binds
+it with ctypes; it has no CPU fallback. If the library is missing, every entry point raises.
+
+Reference interface mirrored here:
+  ISceneObject  CrossRT/CrossRT.h:43-175
+  BVHRT         MultiRenderer/BVH2Common.h:66-381
+  CreateSceneRT MultiRenderer/BVH2Common_host.cpp:1594-1600
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblrt_hip.so")
+
+# ---- reference structs as numpy dtypes (byte-compatible, see csrc/lrt_types.h) ----
+HIT_DTYPE = np.dtype([("t", "<f4"), ("primId", "<u4"), ("instId", "<u4"), ("geomId", "<u4"),
+                      ("coords", "<f4", (4,))])                                   # CRT_Hit, 32 B
+NODE_DTYPE = np.dtype([("boxMin", "<f4", (3,)), ("leftOffset", "<u4"), ("boxMax", "<f4", (3,)),
+                       ("escapeIndex", "<u4")])                                    # BVHNode, 32 B
+NODE_PAIR_DTYPE = np.dtype([("left", NODE_DTYPE), ("right", NODE_DTYPE)])         # BVHNodePair, 64 B
+GEOM_DTYPE = np.dtype([("boxMin", "<f4", (4,)), ("boxMax", "<f4", (4,)), ("bvhOffset", "<u4"),
+                       ("type", "<u4"), ("offset", "<u4", (2,)), ("hasChannels", "<u4"),
+                       ("_pad", "<u4", (3,))])                                     # GeomData, 64 B
+INSTANCE_DTYPE = np.dtype([("boxMin", "<f4", (4,)), ("boxMax", "<f4", (4,)), ("geomId", "<u4"),
+                           ("_pad", "<u4", (7,)), ("transform", "<f4", (16,)),
+                           ("transformInv", "<f4", (16,)),
+                           ("transformInvTransposed", "<f4", (16,))])              # InstanceData, 256 B
+FRAME_NODE_DTYPE = np.dtype([("values", "<f4", (8,)), ("offset", "<u4")])         # SdfFrameOctreeNode, 36 B
+SBS_NODE_DTYPE = np.dtype([("pos_xy", "<u4"), ("pos_z_lod_size", "<u4"), ("data_offset", "<u4"),
+                           ("_pad", "<u4")])                                       # SdfSBSNode, 16 B
+SBS_HEADER_DTYPE = np.dtype([("brick_size", "<u4"), ("brick_pad", "<u4"), ("bytes_per_value", "<u4"),
+                             ("aux_data", "<u4")])                                 # SdfSBSHeader, 16 B
+PRESET_DTYPE = np.dtype([("render_mode", "<u4"), ("sdf_node_intersect", "<u4"), ("normal_mode", "<u4"),
+                         ("ray_gen_mode", "<u4"), ("interpolation_mode", "<u4"),
+                         ("representation_mode", "<u4"), ("spp", "<u4"), ("fixed_lod", "<u4"),
+                         ("level_of_detail", "<f4"), ("compact_sdf_eps", "<f4"), ("LOD_distance", "<f4"),
+                         ("channel_render_mode", "<u4"), ("channel_id", "<u4"), ("component_id", "<u4"),
+                         ("CTF_id", "<u4")])                                       # MultiRenderPreset, 60 B
+
+ARRAY_DTYPES = {
+    "nodes": NODE_PAIR_DTYPE, "tlas": NODE_DTYPE, "instances": INSTANCE_DTYPE, "geoms": GEOM_DTYPE,
+    "geom_tags": np.uint32, "start_end": np.uint32, "prim_id_count": np.uint32,
+    "vert_pos": np.float32, "vert_norm": np.float32, "indices": np.uint32, "prim_indices": np.uint32,
+    "orig_nodes": NODE_DTYPE, "frame_nodes": FRAME_NODE_DTYPE, "frame_roots": np.uint32,
+    "sbs_nodes": SBS_NODE_DTYPE, "sbs_data": np.uint32, "sbs_data_f": np.float32, "sbs_roots": np.uint32,
+    "sbs_headers": SBS_HEADER_DTYPE, "preset": PRESET_DTYPE,
+}
+
+# enums (core/render_settings.h, BVH2Common.h:30-43, sdf_simple_shared.h:13-21)
+TYPE_MESH_TRIANGLE, TYPE_SDF_FRAME_OCTREE, TYPE_SDF_SBS, TYPE_SDF_SBS_COL = 0, 3, 6, 11
+SH_TYPE, GEOM_ID_MASK = 24, 0x00FFFFFF
+MISS = 0xFFFFFFFF
+SDF_SBS_NODE_LAYOUT_DX = 1 << 24
+SDF_SBS_NODE_LAYOUT_DX_UV16 = 2 << 24
+SDF_SBS_NODE_LAYOUT_DX_RGB8 = 3 << 24
+SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F = 4 << 24
+SDF_OCTREE_NODE_INTERSECT_ST, SDF_OCTREE_NODE_INTERSECT_ANALYTIC = 0, 1
+SDF_OCTREE_NODE_INTERSECT_NEWTON, SDF_OCTREE_NODE_INTERSECT_BBOX, SDF_OCTREE_NODE_INTERSECT_IT = 2, 3, 4
+MULTI_RENDER_MODE_MASK, MULTI_RENDER_MODE_LAMBERT_NO_TEX = 0, 1
+NORMAL_MODE_GEOMETRY, NORMAL_MODE_VERTEX = 0, 1
+BUILD_HIGH = 4
+
+
+class LrtError(RuntimeError):
+    """Raised when a C-ABI call returns a non-zero lrt_status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"lrt error {code}: {msg}")
+        self.code = code
+
+
+class _Buffer(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("bytes", ctypes.c_size_t)]
+
+
+_lib_handle = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    """Loads liblrt_hip.so; raises if it has not been built (there is no fallback)."""
+    global _lib_handle
+    if _lib_handle is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u32, u64, sz, f32, cp = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t,
+                                     ctypes.c_float, ctypes.c_char_p)
+        i32 = ctypes.c_int
+        pu32 = ctypes.POINTER(ctypes.c_uint32)
+        sigs = {
+            "lrt_last_error": ([], cp), "lrt_version": ([], cp),
+            "lrt_create": ([cp, cp, cp, ctypes.POINTER(vp)], i32),
+            "lrt_destroy": ([vp], i32), "lrt_clear_geom": ([vp], i32), "lrt_clear_scene": ([vp], i32),
+            "lrt_add_triangles": ([vp, vp, sz, sz, vp, vp, sz, u32, pu32], i32),
+            "lrt_add_sdf_frame_octree": ([vp, vp, sz, pu32], i32),
+            "lrt_add_sdf_sbs": ([vp, vp, vp, sz, vp, sz, vp, sz, pu32], i32),
+            "lrt_add_instance": ([vp, u32, vp, pu32], i32),
+            "lrt_update_instance": ([vp, u32, vp], i32),
+            "lrt_commit": ([vp, u32], i32), "lrt_set_preset": ([vp, vp], i32),
+            "lrt_trace_closest_dev": ([vp, vp, vp, vp, u64, vp], i32),
+            "lrt_trace_closest_host": ([vp, vp, vp, vp, u64], i32),
+            "lrt_trace_any_dev": ([vp, vp, vp, vp, u64, vp], i32),
+            "lrt_trace_closest_counted_dev": ([vp, vp, vp, vp, vp, u64, vp], i32),
+            "lrt_query_one": ([vp, vp, vp, vp], i32),
+            "lrt_eye_rays_dev": ([vp, vp, u32, u32, vp, u32, vp, vp, vp], i32),
+            "lrt_set_launch_config": ([vp, i32, i32, i32], i32),
+            "lrt_stack_overflow_count": ([vp, ctypes.POINTER(u64), i32], i32),
+            "lrt_get_array": ([vp, cp, vp, ctypes.POINTER(sz)], i32),
+            "lrt_buffer_free": ([ctypes.POINTER(_Buffer)], None),
+            "lrt_camera_inverse": ([vp, vp, vp, f32, f32, f32, f32, vp, vp], i32),
+            "lrt_gen_icosphere": ([u32, f32, f32, u64, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
+            "lrt_gen_geodesic": ([u32, f32, f32, u64, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
+            "lrt_sdf_spheres_frame_octree": ([vp, sz, u32, i32, ctypes.POINTER(_Buffer)], i32),
+            "lrt_sdf_spheres_sbs": ([vp, sz, u32, vp, i32, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
+            "lrt_sdf_spheres_sbs_indexed": ([vp, sz, u32, u32, u64, i32, ctypes.POINTER(_Buffer),
+                                             ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer), pu32], i32),
+        }
+        for name, (args, res) in sigs.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib_handle = L
+    return _lib_handle
+
+
+def exported_symbols():
+    return ["lrt_last_error", "lrt_version", "lrt_create", "lrt_destroy", "lrt_clear_geom", "lrt_clear_scene",
+            "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_sbs", "lrt_add_instance",
+            "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
+            "lrt_trace_closest_host", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
+            "lrt_set_launch_config", "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
+            "lrt_camera_inverse", "lrt_gen_icosphere", "lrt_gen_geodesic", "lrt_sdf_spheres_frame_octree",
+            "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed"]
+
+
+def _check(code: int):
+    if code != 0:
+        raise LrtError(code, lib().lrt_last_error().decode())
+
+
+def _ptr(a) -> Optional[int]:
+    """Address of a numpy array (host) or a torch tensor (device or host)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return a.data_ptr()
+    raise TypeError(f"unsupported buffer type {type(a)}")
+
+
+def _take(buf: _Buffer, dtype) -> np.ndarray:
+    try:
+        raw = ctypes.string_at(buf.data, buf.bytes) if buf.bytes else b""
+        return np.frombuffer(raw, dtype=dtype).copy()
+    finally:
+        lib().lrt_buffer_free(ctypes.byref(buf))
+
+
+def default_preset() -> np.ndarray:
+    """getDefaultPreset (core/render_settings.h:138-160)."""
+    p = np.zeros((), dtype=PRESET_DTYPE)
+    p["render_mode"] = MULTI_RENDER_MODE_LAMBERT_NO_TEX
+    p["sdf_node_intersect"] = SDF_OCTREE_NODE_INTERSECT_ANALYTIC
+    p["normal_mode"] = NORMAL_MODE_VERTEX
+    p["interpolation_mode"] = 0
+    p["representation_mode"] = 1
+    p["spp"] = 1
+    p["fixed_lod"] = 1
+    p["level_of_detail"] = 16.0
+    p["compact_sdf_eps"] = 0.05
+    p["LOD_distance"] = 10.0
+    return p
+
+
+class SceneRT:
+    """BVHRT behind the C ABI: the ISceneObject methods plus the SDF geometry adders."""
+
+    def __init__(self, impl_name="BVH2Common", build_name="cbvh_embree2", layout_name="SuperTreeletAlignedMerged4"):
+        h = ctypes.c_void_p()
+        _check(lib().lrt_create(impl_name.encode(), build_name.encode(), layout_name.encode(), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib_handle is not None:
+            _lib_handle.lrt_destroy(h)
+            self._h = None
+
+    # ---- ISceneObject (CrossRT/CrossRT.h:43-175) ----
+    def Name(self) -> str:
+        return "BVH2Fat"
+
+    def ClearGeom(self):
+        _check(lib().lrt_clear_geom(self._h))
+
+    def AddGeom_Triangles3f(self, vpos: np.ndarray, indices: np.ndarray, flags: int = BUILD_HIGH,
+                            vnorm: Optional[np.ndarray] = None) -> int:
+        vpos = np.ascontiguousarray(vpos, dtype=np.float32)
+        if vpos.ndim != 2 or vpos.shape[1] not in (3, 4):
+            raise ValueError("vpos must be (n, 3) or (n, 4)")
+        idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+        if vnorm is not None:
+            vnorm = np.ascontiguousarray(vnorm, dtype=np.float32)
+            if vnorm.shape != vpos.shape:
+                raise ValueError("vnorm must have the shape of vpos")
+        out = ctypes.c_uint32()
+        _check(lib().lrt_add_triangles(self._h, _ptr(vpos), vpos.shape[0], vpos.shape[1] * 4, _ptr(vnorm),
+                                       _ptr(idx), idx.size, flags, ctypes.byref(out)))
+        return out.value
+
+    def AddGeom_SdfFrameOctree(self, nodes: np.ndarray) -> int:
+        nodes = np.ascontiguousarray(nodes, dtype=FRAME_NODE_DTYPE)
+        out = ctypes.c_uint32()
+        _check(lib().lrt_add_sdf_frame_octree(self._h, _ptr(nodes), nodes.size, ctypes.byref(out)))
+        return out.value
+
+    def AddGeom_SdfSBS(self, header: np.ndarray, nodes: np.ndarray, values: np.ndarray,
+                       values_f: Optional[np.ndarray] = None) -> int:
+        header = np.ascontiguousarray(header, dtype=SBS_HEADER_DTYPE).reshape(())
+        nodes = np.ascontiguousarray(nodes, dtype=SBS_NODE_DTYPE)
+        values = np.ascontiguousarray(values, dtype=np.uint32)
+        vf = None if values_f is None else np.ascontiguousarray(values_f, dtype=np.float32)
+        out = ctypes.c_uint32()
+        _check(lib().lrt_add_sdf_sbs(self._h, _ptr(header), _ptr(nodes), nodes.size, _ptr(values), values.size,
+                                     _ptr(vf), 0 if vf is None else vf.size, ctypes.byref(out)))
+        return out.value
+
+    def ClearScene(self):
+        _check(lib().lrt_clear_scene(self._h))
+
+    def AddInstance(self, geom_id: int, matrix: Optional[np.ndarray] = None) -> int:
+        """matrix: 4x4 (row-major numpy view of a column-major LiteMath float4x4 is transposed here)."""
+        m = np.eye(4, dtype=np.float32) if matrix is None else np.asarray(matrix, dtype=np.float32)
+        col = np.ascontiguousarray(m.T.reshape(16))
+        out = ctypes.c_uint32()
+        _check(lib().lrt_add_instance(self._h, geom_id, _ptr(col), ctypes.byref(out)))
+        return out.value
+
+    def UpdateInstance(self, inst_id: int, matrix: np.ndarray):
+        col = np.ascontiguousarray(np.asarray(matrix, dtype=np.float32).T.reshape(16))
+        _check(lib().lrt_update_instance(self._h, inst_id, _ptr(col)))
+
+    def CommitScene(self, flags: int = BUILD_HIGH):
+        _check(lib().lrt_commit(self._h, flags))
+
+    def SetPreset(self, preset: np.ndarray):
+        p = np.ascontiguousarray(preset, dtype=PRESET_DTYPE).reshape(())
+        _check(lib().lrt_set_preset(self._h, _ptr(p)))
+
+    def RayQuery_NearestHit(self, pos_near, dir_far) -> np.ndarray:
+        p = np.ascontiguousarray(pos_near, dtype=np.float32).reshape(4)
+        d = np.ascontiguousarray(dir_far, dtype=np.float32).reshape(4)
+        h = np.zeros((), dtype=HIT_DTYPE)
+        _check(lib().lrt_query_one(self._h, _ptr(p), _ptr(d), _ptr(h)))
+        return h
+
+    def RayQuery_AnyHit(self, pos_near, dir_far) -> bool:
+        d = np.asarray(dir_far, dtype=np.float32).reshape(4).copy()
+        d[3] *= -1.0
+        return bool(self.RayQuery_NearestHit(pos_near, d)["geomId"] != MISS)
+
+    # ---- batched trace ----
+    def trace_closest(self, pos, dir, hits, stream: int = 0):
+        """Device buffers (torch tensors): pos/dir float32 (n,4), hits uint8/int32 storage of n*32 bytes."""
+        n = pos.numel() // 4
+        _check(lib().lrt_trace_closest_dev(self._h, _ptr(pos), _ptr(dir), _ptr(hits), n, stream))
+
+    def trace_closest_counted(self, pos, dir, hits, counters, stream: int = 0):
+        """As trace_closest, plus 6 uint32 traversal counters per ray (RayQuery_NearestHitWithRayInfo)."""
+        n = pos.numel() // 4
+        _check(lib().lrt_trace_closest_counted_dev(self._h, _ptr(pos), _ptr(dir), _ptr(hits), _ptr(counters), n,
+                                                   stream))
+
+    def trace_any(self, pos, dir, flags, stream: int = 0):
+        n = pos.numel() // 4
+        _check(lib().lrt_trace_any_dev(self._h, _ptr(pos), _ptr(dir), _ptr(flags), n, stream))
+
+    def trace_closest_host(self, pos: np.ndarray, dir: np.ndarray) -> np.ndarray:
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 4)
+        dir = np.ascontiguousarray(dir, dtype=np.float32).reshape(-1, 4)
+        hits = np.zeros(pos.shape[0], dtype=HIT_DTYPE)
+        _check(lib().lrt_trace_closest_host(self._h, _ptr(pos), _ptr(dir), _ptr(hits), pos.shape[0]))
+        return hits
+
+    def set_launch_config(self, block: int = 256, lds_stack: int = 16, xcd_remap: bool = True):
+        _check(lib().lrt_set_launch_config(self._h, block, lds_stack, int(xcd_remap)))
+
+    def stack_overflow_count(self, reset: bool = False) -> int:
+        v = ctypes.c_uint64()
+        _check(lib().lrt_stack_overflow_count(self._h, ctypes.byref(v), int(reset)))
+        return v.value
+
+    def get_array(self, name: str) -> np.ndarray:
+        n = ctypes.c_size_t()
+        _check(lib().lrt_get_array(self._h, name.encode(), None, ctypes.byref(n)))
+        dt = np.dtype(ARRAY_DTYPES[name])
+        out = np.zeros(n.value // dt.itemsize, dtype=dt)
+        if n.value:
+            _check(lib().lrt_get_array(self._h, name.encode(), _ptr(out), ctypes.byref(n)))
+        return out
+
+
+def CreateSceneRT(impl_name="BVH2Common", build_name="cbvh_embree2", layout_name="SuperTreeletAlignedMerged4"):
+    """CreateSceneRT (BVH2Common_host.cpp:1594-1600): None for an unknown implementation name."""
+    try:
+        return SceneRT(impl_name, build_name, layout_name)
+    except LrtError as e:
+        if e.code == 4:   # LRT_ERR_UNSUPPORTED
+            return None
+        raise
+
+
+# ---- builders and generators (include/lrt_build.h) ----
+def camera_inverse(eye, target, up, fovy_deg, aspect, znear, zfar):
+    """(inverse4x4(proj), inverse4x4(worldView)) as column-major float32[16]."""
+    e, t, u = (np.asarray(v, dtype=np.float32) for v in (eye, target, up))
+    pi = np.zeros(16, np.float32)
+    vi = np.zeros(16, np.float32)
+    _check(lib().lrt_camera_inverse(_ptr(e), _ptr(t), _ptr(u), fovy_deg, aspect, znear, zfar, _ptr(pi), _ptr(vi)))
+    return pi, vi
+
+
+def eye_rays(proj_inv, world_view_inv, width, height, pos, dir, jitter=None, order=1, stream=0):
+    """Primary rays of MultiRenderer::initEyeRay into device tensors pos/dir (n,4) float32."""
+    pi = np.ascontiguousarray(proj_inv, dtype=np.float32)
+    vi = np.ascontiguousarray(world_view_inv, dtype=np.float32)
+    _check(lib().lrt_eye_rays_dev(_ptr(pi), _ptr(vi), width, height, _ptr(jitter), order, _ptr(pos), _ptr(dir),
+                                  stream))
+
+
+def gen_icosphere(subdiv, amplitude=0.0, frequency=4.0, seed=1):
+    vb, ib = _Buffer(), _Buffer()
+    _check(lib().lrt_gen_icosphere(subdiv, amplitude, frequency, seed, ctypes.byref(vb), ctypes.byref(ib)))
+    return _take(vb, np.float32).reshape(-1, 3), _take(ib, np.uint32).reshape(-1, 3)
+
+
+def gen_geodesic(n, amplitude=0.0, frequency=4.0, seed=1):
+    vb, ib = _Buffer(), _Buffer()
+    _check(lib().lrt_gen_geodesic(n, amplitude, frequency, seed, ctypes.byref(vb), ctypes.byref(ib)))
+    return _take(vb, np.float32).reshape(-1, 3), _take(ib, np.uint32).reshape(-1, 3)
+
+
+def sdf_spheres_frame_octree(spheres: np.ndarray, depth: int, threads: int = 0) -> np.ndarray:
+    s = np.ascontiguousarray(spheres, dtype=np.float32).reshape(-1, 4)
+    nb = _Buffer()
+    _check(lib().lrt_sdf_spheres_frame_octree(_ptr(s), s.shape[0], depth, threads, ctypes.byref(nb)))
+    return _take(nb, FRAME_NODE_DTYPE)
+
+
+def sdf_spheres_sbs(spheres: np.ndarray, depth: int, header: np.ndarray, threads: int = 0):
+    s = np.ascontiguousarray(spheres, dtype=np.float32).reshape(-1, 4)
+    h = np.ascontiguousarray(header, dtype=SBS_HEADER_DTYPE).reshape(())
+    nb, vb = _Buffer(), _Buffer()
+    _check(lib().lrt_sdf_spheres_sbs(_ptr(s), s.shape[0], depth, _ptr(h), threads, ctypes.byref(nb),
+                                     ctypes.byref(vb)))
+    return _take(nb, SBS_NODE_DTYPE), _take(vb, np.uint32)
+
+
+def sdf_spheres_sbs_indexed(spheres: np.ndarray, depth: int, brick_size: int, albedo_seed: int = 0,
+                            threads: int = 0):
+    """Indexed fp32 brick set: (header, nodes, values, values_f, n_distance_values)."""
+    s = np.ascontiguousarray(spheres, dtype=np.float32).reshape(-1, 4)
+    nb, vb, fb = _Buffer(), _Buffer(), _Buffer()
+    nd = ctypes.c_uint32()
+    _check(lib().lrt_sdf_spheres_sbs_indexed(_ptr(s), s.shape[0], depth, brick_size, albedo_seed, threads,
+                                             ctypes.byref(nb), ctypes.byref(vb), ctypes.byref(fb), ctypes.byref(nd)))
+    h = np.zeros((), SBS_HEADER_DTYPE)
+    h["brick_size"], h["brick_pad"], h["bytes_per_value"] = brick_size, 0, 4
+    h["aux_data"] = SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F
+    return h, _take(nb, SBS_NODE_DTYPE), _take(vb, np.uint32), _take(fb, np.float32), nd.value

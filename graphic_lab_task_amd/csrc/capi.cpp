@@ -1,0 +1,340 @@
+// extern "C" entry points declared in include/lrt.h.
+#include "../../include/lrt.h"
+
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+#include <string>
+
+#include "host_math.h"
+#include "kernels.h"
+#include "scene.h"
+
+struct lrt_scene {
+  lrt::Scene s;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+lrt_status fail(lrt_status code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+lrt_status hip_fail(hipError_t e, const char* where) {
+  return fail(LRT_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+lrt_status guarded(F&& f) {
+  try {
+    g_err.clear();
+    return f();
+  } catch (const std::invalid_argument& e) {
+    return fail(LRT_ERR_INVALID_ARG, e.what());
+  } catch (const std::exception& e) {
+    return fail(LRT_ERR_HIP, e.what());
+  } catch (...) {
+    return fail(LRT_ERR_HIP, "unknown error");
+  }
+}
+
+lrt::float4x4 to_mat(const float* m16) {
+  lrt::float4x4 m;
+  std::memcpy(m.m, m16, sizeof(m.m));
+  return m;
+}
+
+lrt_status check_device() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(LRT_ERR_NO_DEVICE, "no HIP device available");
+  return LRT_OK;
+}
+
+lrt_status trace_common(lrt_scene* s, const float* pos, const float* dir, lrt_hit* hits, uint32_t* any, uint64_t n,
+                        void* stream, uint32_t* counts = nullptr) {
+  if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+  if (!s->s.committed) return fail(LRT_ERR_NOT_COMMITTED, "scene is not committed");
+  if (n == 0) return LRT_OK;
+  if (!pos || !dir || (!hits && !any)) return fail(LRT_ERR_INVALID_ARG, "null ray or hit buffer");
+  if (n > 0xFFFFFFFFull) return fail(LRT_ERR_INVALID_ARG, "too many rays in one call");
+  lrt::TraceArgs a{};
+  a.pos = reinterpret_cast<const lrt::float4*>(pos);
+  a.dir = reinterpret_cast<const lrt::float4*>(dir);
+  a.out = reinterpret_cast<lrt::CRT_Hit*>(hits);
+  a.any_out = any;
+  a.n = uint32_t(n);
+  a.overflow_count = s->s.d_overflow;
+  a.block = s->s.block;
+  a.lds_stack = s->s.lds_stack;
+  a.xcd_remap = uint32_t(s->s.xcd_remap);
+  a.counts = counts;
+  hipError_t e = lrt::launch_trace(s->s.make_view(), a, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "trace launch");
+  return LRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lrt_last_error(void) { return g_err.c_str(); }
+
+const char* lrt_version(void) { return "lrt-gfx950 0.1"; }
+
+lrt_status lrt_create(const char* impl_name, const char* build_name, const char* layout_name, lrt_scene** out) {
+  return guarded([&]() -> lrt_status {
+    if (!out) return fail(LRT_ERR_INVALID_ARG, "null output pointer");
+    *out = nullptr;
+    const std::string impl = impl_name ? impl_name : "";
+    if (impl != "BVH2Common") return fail(LRT_ERR_UNSUPPORTED, "unknown scene implementation '" + impl + "'");
+    auto* sc = new lrt_scene();
+    sc->s.build_name = build_name ? build_name : "";
+    sc->s.layout_name = layout_name ? layout_name : "";
+    // BuilderPresetsFromString (utils/raytracing/cbvh.cpp:57-69): last digit = prims per leaf
+    const std::string& b = sc->s.build_name;
+    sc->s.prims_in_leaf = 2;
+    if (!b.empty() && b.back() >= '0' && b.back() <= '9') sc->s.prims_in_leaf = b.back() - '0';
+    if (sc->s.prims_in_leaf < 1) sc->s.prims_in_leaf = 1;
+    if (sc->s.prims_in_leaf > 7) sc->s.prims_in_leaf = 7;   // 3-bit leaf count field
+    sc->s.set_preset(lrt::default_preset());
+    *out = sc;
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_destroy(lrt_scene* s) {
+  return guarded([&]() -> lrt_status {
+    delete s;
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_clear_geom(lrt_scene* s) {
+  return guarded([&]() -> lrt_status {
+    if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+    s->s.clear_geom();
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_add_triangles(lrt_scene* s, const float* vpos, size_t n_verts, size_t v_byte_stride, const float* vnorm,
+                             const uint32_t* indices, size_t n_indices, uint32_t build_flags, uint32_t* out_geom_id) {
+  (void)build_flags;
+  return guarded([&]() -> lrt_status {
+    if (!s || !vpos || !indices) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    if (v_byte_stride == 0) v_byte_stride = 12;
+    const uint32_t id = s->s.add_triangles(vpos, n_verts, v_byte_stride, vnorm, indices, n_indices);
+    if (out_geom_id) *out_geom_id = id;
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_add_sdf_frame_octree(lrt_scene* s, const void* nodes, size_t n_nodes, uint32_t* out_geom_id) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !nodes) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    const uint32_t id = s->s.add_sdf_frame_octree(static_cast<const lrt::SdfFrameOctreeNode*>(nodes), n_nodes);
+    if (out_geom_id) *out_geom_id = id;
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_add_sdf_sbs(lrt_scene* s, const void* header, const void* nodes, size_t n_nodes, const uint32_t* values,
+                           size_t n_values, const float* values_f, size_t n_values_f, uint32_t* out_geom_id) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !header || !nodes || !values) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    const uint32_t id = s->s.add_sdf_sbs(*static_cast<const lrt::SdfSBSHeader*>(header),
+                                         static_cast<const lrt::SdfSBSNode*>(nodes), n_nodes, values, n_values,
+                                         values_f, n_values_f);
+    if (out_geom_id) *out_geom_id = id;
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_clear_scene(lrt_scene* s) {
+  return guarded([&]() -> lrt_status {
+    if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+    s->s.clear_scene();
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_add_instance(lrt_scene* s, uint32_t geom_id, const float* matrix16, uint32_t* out_inst_id) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !matrix16) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    const uint32_t id = s->s.add_instance(geom_id, to_mat(matrix16));
+    if (out_inst_id) *out_inst_id = id;
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_update_instance(lrt_scene* s, uint32_t inst_id, const float* matrix16) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !matrix16) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    s->s.update_instance(inst_id, to_mat(matrix16));
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_commit(lrt_scene* s, uint32_t build_flags) {
+  (void)build_flags;
+  return guarded([&]() -> lrt_status {
+    if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+    s->s.build_host();
+    lrt_status st = check_device();
+    if (st != LRT_OK) return fail(st, "host arrays built; no HIP device to upload them to");
+    hipError_t e = s->s.commit();
+    if (e != hipSuccess) return hip_fail(e, "commit");
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_set_preset(lrt_scene* s, const void* preset) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !preset) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    s->s.set_preset(*static_cast<const lrt::MultiRenderPreset*>(preset));
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_trace_closest_dev(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits, uint64_t n,
+                                 void* stream) {
+  return guarded([&]() -> lrt_status { return trace_common(s, pos_near, dir_far, hits, nullptr, n, stream); });
+}
+
+lrt_status lrt_trace_closest_counted_dev(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,
+                                         uint32_t* counters, uint64_t n, void* stream) {
+  return guarded([&]() -> lrt_status {
+    if (!counters) return fail(LRT_ERR_INVALID_ARG, "null counter buffer");
+    return trace_common(s, pos_near, dir_far, hits, nullptr, n, stream, counters);
+  });
+}
+
+lrt_status lrt_trace_any_dev(lrt_scene* s, const float* pos_near, const float* dir_far, uint32_t* flags, uint64_t n,
+                             void* stream) {
+  return guarded([&]() -> lrt_status {
+    if (!flags) return fail(LRT_ERR_INVALID_ARG, "null flag buffer");
+    return trace_common(s, pos_near, dir_far, nullptr, flags, n, stream);
+  });
+}
+
+lrt_status lrt_trace_closest_host(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,
+                                  uint64_t n) {
+  return guarded([&]() -> lrt_status {
+    if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+    if (!s->s.committed) return fail(LRT_ERR_NOT_COMMITTED, "scene is not committed");
+    if (n == 0) return LRT_OK;
+    if (!pos_near || !dir_far || !hits) return fail(LRT_ERR_INVALID_ARG, "null buffer");
+    lrt::DevBuf dp, dd, dh;
+    hipError_t e;
+    if ((e = dp.upload(pos_near, n * 16)) != hipSuccess) return hip_fail(e, "copy rays");
+    if ((e = dd.upload(dir_far, n * 16)) != hipSuccess) return hip_fail(e, "copy rays");
+    if ((e = dh.reserve(n * sizeof(lrt_hit))) != hipSuccess) return hip_fail(e, "alloc hits");
+    lrt_status st = trace_common(s, (const float*)dp.ptr(), (const float*)dd.ptr(), (lrt_hit*)dh.ptr(), nullptr, n,
+                                 nullptr);
+    if (st != LRT_OK) return st;
+    if ((e = hipMemcpy(hits, dh.ptr(), n * sizeof(lrt_hit), hipMemcpyDeviceToHost)) != hipSuccess)
+      return hip_fail(e, "copy hits");
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_query_one(lrt_scene* s, const float* pos_near4, const float* dir_far4, lrt_hit* out) {
+  return lrt_trace_closest_host(s, pos_near4, dir_far4, out, 1);
+}
+
+lrt_status lrt_eye_rays_dev(const float* proj_inv16, const float* world_view_inv16, uint32_t width, uint32_t height,
+                            const float* jitter, uint32_t order, float* pos_near, float* dir_far, void* stream) {
+  return guarded([&]() -> lrt_status {
+    if (!proj_inv16 || !world_view_inv16 || !pos_near || !dir_far) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    if (width == 0 || height == 0 || width > 65535 || height > 65535) return fail(LRT_ERR_INVALID_ARG, "bad viewport");
+    if (order > 1) return fail(LRT_ERR_INVALID_ARG, "order must be 0 or 1");
+    if (order == 1 && (width % 8 != 0 || height % 8 != 0))
+      return fail(LRT_ERR_INVALID_ARG, "packed order needs width and height multiples of 8");
+    lrt::EyeRayParams p{};
+    std::memcpy(p.proj_inv, proj_inv16, sizeof(p.proj_inv));
+    std::memcpy(p.world_view_inv, world_view_inv16, sizeof(p.world_view_inv));
+    p.width = width;
+    p.height = height;
+    p.packed_w = 8 * ((width + 7) / 8);
+    p.packed_h = 8 * ((height + 7) / 8);
+    // spp = 1, sample 0 (MultiRenderer.cpp:163-166): offset (0.5, 0.5)
+    p.d.x = 0.5f;
+    p.d.y = 0.5f;
+    p.order = order;
+    hipError_t e = lrt::launch_eye_rays(p, reinterpret_cast<const lrt::float2*>(jitter),
+                                        reinterpret_cast<lrt::float4*>(pos_near), reinterpret_cast<lrt::float4*>(dir_far),
+                                        reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "eye ray launch");
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_set_launch_config(lrt_scene* s, int block, int lds_stack, int xcd_remap) {
+  return guarded([&]() -> lrt_status {
+    if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+    if (block != 64 && block != 128 && block != 256) return fail(LRT_ERR_INVALID_ARG, "block must be 64, 128 or 256");
+    if (lds_stack != 8 && lds_stack != 16 && lds_stack != 32)
+      return fail(LRT_ERR_INVALID_ARG, "lds_stack must be 8, 16 or 32");
+    s->s.block = block;
+    s->s.lds_stack = lds_stack;
+    s->s.xcd_remap = xcd_remap ? 1 : 0;
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_stack_overflow_count(lrt_scene* s, uint64_t* out, int reset) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !out) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    *out = 0;
+    if (!s->s.d_overflow) return LRT_OK;
+    uint32_t v = 0;
+    hipError_t e = hipMemcpy(&v, s->s.d_overflow, sizeof(v), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "read overflow counter");
+    *out = v;
+    if (reset) {
+      e = hipMemset(s->s.d_overflow, 0, sizeof(uint32_t));
+      if (e != hipSuccess) return hip_fail(e, "reset overflow counter");
+    }
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* bytes) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !name || !bytes) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    const lrt::Scene& c = s->s;
+    const std::string n = name;
+    const void* src = nullptr;
+    size_t sz = 0;
+#define ARR(key, vec)                                   \
+  if (n == key) { src = vec.data(); sz = vec.size() * sizeof(vec[0]); }
+    ARR("nodes", c.nodePairs) ARR("tlas", c.tlas) ARR("instances", c.instances) ARR("geoms", c.geomData)
+    ARR("geom_tags", c.geomTags) ARR("start_end", c.startEnd) ARR("prim_id_count", c.primIdCount)
+    ARR("vert_pos", c.vertPos) ARR("vert_norm", c.vertNorm) ARR("indices", c.indices)
+    ARR("prim_indices", c.primIndices) ARR("orig_nodes", c.origNodes) ARR("frame_nodes", c.frameNodes)
+    ARR("frame_roots", c.frameRoots) ARR("sbs_nodes", c.sbsNodes) ARR("sbs_data", c.sbsData)
+    ARR("sbs_data_f", c.sbsDataF) ARR("sbs_roots", c.sbsRoots) ARR("sbs_headers", c.sbsHeaders)
+#undef ARR
+    if (n == "preset") { src = &c.preset; sz = sizeof(c.preset); }
+    if (!src && sz == 0) {
+      static const char* known[] = {"nodes", "tlas", "instances", "geoms", "geom_tags", "start_end",
+                                    "prim_id_count", "vert_pos", "vert_norm", "indices", "prim_indices",
+                                    "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes", "sbs_data",
+                                    "sbs_data_f", "sbs_roots", "sbs_headers", "preset"};
+      bool ok = false;
+      for (const char* k : known) ok = ok || n == k;
+      if (!ok) return fail(LRT_ERR_INVALID_ARG, "unknown array '" + n + "'");
+    }
+    if (dst) {
+      if (*bytes < sz) return fail(LRT_ERR_INVALID_ARG, "destination too small");
+      if (sz) std::memcpy(dst, src, sz);
+    }
+    *bytes = sz;
+    return LRT_OK;
+  });
+}
+
+}  // extern "C"

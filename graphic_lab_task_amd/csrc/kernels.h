@@ -1,0 +1,34 @@
+// Host-visible launch interface of the gfx950 kernels (no torch types).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "scene_view.h"
+
+namespace lrt {
+
+struct TraceArgs {
+  const float4* pos;        // posAndNear per ray (device)
+  const float4* dir;        // dirAndFar per ray (device)
+  CRT_Hit*      out;        // closest hit per ray (device), may be null for any-hit
+  uint32_t*     any_out;    // any-hit flag per ray (device), null for closest hit
+  uint32_t      n;
+  uint32_t*     overflow_count;  // device counter of rays that hit the stack limit (nullable)
+  int           block;      // 64, 128 or 256
+  int           lds_stack;  // 8, 16 or 32 stack entries kept in LDS
+  uint32_t      xcd_remap;  // 1: contiguous ray range per XCD
+  uint32_t*     counts;     // nullable: 6 counters per ray (RayCounts order), instrumented variant
+};
+
+struct EyeRayParams {
+  float    proj_inv[16];        // inverse4x4(proj), column-major
+  float    world_view_inv[16];  // inverse4x4(worldView), column-major
+  uint32_t width, height;
+  uint32_t packed_w, packed_h;  // padded to multiples of 8 (MultiRenderer_host.cpp:80-81)
+  float2   d;                   // sub-pixel offset when no jitter buffer is given
+  uint32_t order;               // 0: row-major ray index, 1: SuperBlockIndex2DOpt order
+};
+
+hipError_t launch_trace(const SceneView& s, const TraceArgs& a, hipStream_t stream);
+hipError_t launch_eye_rays(const EyeRayParams& prm, const float2* jitter, float4* pos, float4* dir,
+                           hipStream_t stream);
+
+}  // namespace lrt

@@ -1,0 +1,229 @@
+// Byte-compatible restatements of the reference ray-query data types, shared by
+// host assembly code and the gfx950 kernels. Every struct keeps the reference's
+// field order and size so that arrays built here can be handed to reference code
+// (and vice versa) without conversion.
+#pragma once
+#include <cstdint>
+#include <cstddef>
+
+#if defined(__HIPCC__)
+#define LRT_HD __host__ __device__ __forceinline__
+#else
+#define LRT_HD inline
+#endif
+
+namespace lrt {
+
+struct float2 { float x, y; };
+struct float3 { float x, y, z; };
+struct alignas(16) float4 { float x, y, z, w; };
+struct uint2 { uint32_t x, y; };
+
+// Column-major 4x4 matrix, m[c*4 + r] (LiteMath float4x4, LiteMath.h:1931).
+struct alignas(16) float4x4 { float m[16]; };
+
+// CrossRT/CrossRT.h:26-33
+struct CRT_Hit {
+  float    t;
+  uint32_t primId;
+  uint32_t instId;
+  uint32_t geomId;   // (type << 24) | geomId
+  float    coords[4];
+};
+
+// utils/raytracing/cbvh.h:21-27
+struct BVHNode {
+  float3   boxMin;
+  uint32_t leftOffset;   // LEAF_BIT set -> leaf, START/SIZE packed
+  float3   boxMax;
+  uint32_t escapeIndex;
+};
+
+// utils/raytracing/cbvh.h:29-33 (fat node: both children's boxes)
+struct alignas(16) BVHNodePair {
+  BVHNode left;
+  BVHNode right;
+};
+
+// utils/raytracing/intersection_common.h:33-44
+struct GeomData {
+  float4   boxMin;
+  float4   boxMax;
+  uint32_t bvhOffset;
+  uint32_t type;
+  uint2    offset;
+  uint32_t hasChannels;
+  uint32_t _pad[3];
+};
+
+// utils/raytracing/intersection_common.h:47-56
+struct InstanceData {
+  float4   boxMin;
+  float4   boxMax;
+  uint32_t geomId;
+  uint32_t _pad[7];
+  float4x4 transform;
+  float4x4 transformInv;
+  float4x4 transformInvTransposed;
+};
+
+// sdf/simple/sdf_simple_shared.h:23-27
+struct SdfFrameOctreeNode {
+  float    values[8];
+  uint32_t offset;     // children block offset, 0 = leaf
+};
+
+// sdf/simple/sdf_simple_shared.h:39-45
+struct SdfSBSNode {
+  uint32_t pos_xy;          // (x << 16) | y, brick position in its LOD grid
+  uint32_t pos_z_lod_size;  // (z << 16) | lod_size
+  uint32_t data_offset;     // offset (in u32) of the brick's data block
+  uint32_t _pad;
+};
+
+// sdf/simple/sdf_simple_shared.h:49-55
+struct SdfSBSHeader {
+  uint32_t brick_size;
+  uint32_t brick_pad;
+  uint32_t bytes_per_value;
+  uint32_t aux_data;        // node layout (top 8 bits)
+};
+
+// sdf/multi/sdf_multi_shared.h:8-14
+struct SdfMultiOctreeNode {
+  uint32_t data_offset;
+  uint32_t children_offset;
+  uint32_t voxel_count;
+  uint32_t flags;
+};
+
+// core/render_settings.h:116-136
+struct MultiRenderPreset {
+  uint32_t render_mode;
+  uint32_t sdf_node_intersect;
+  uint32_t normal_mode;
+  uint32_t ray_gen_mode;
+  uint32_t interpolation_mode;
+  uint32_t representation_mode;
+  uint32_t spp;
+  uint32_t fixed_lod;
+  float    level_of_detail;
+  float    compact_sdf_eps;
+  float    LOD_distance;
+  uint32_t channel_render_mode;
+  uint32_t channel_id;
+  uint32_t component_id;
+  uint32_t CTF_id;
+};
+
+static_assert(sizeof(CRT_Hit) == 32, "CRT_Hit is 32 B");
+static_assert(sizeof(BVHNode) == 32, "BVHNode is 32 B");
+static_assert(sizeof(BVHNodePair) == 64, "BVHNodePair is 64 B");
+static_assert(sizeof(GeomData) == 64, "GeomData is 64 B");
+static_assert(sizeof(InstanceData) == 256, "InstanceData is 256 B");
+static_assert(sizeof(SdfFrameOctreeNode) == 36, "SdfFrameOctreeNode is 36 B");
+static_assert(sizeof(SdfSBSNode) == 16, "SdfSBSNode is 16 B");
+static_assert(sizeof(SdfSBSHeader) == 16, "SdfSBSHeader is 16 B");
+static_assert(sizeof(SdfMultiOctreeNode) == 16, "SdfMultiOctreeNode is 16 B");
+static_assert(sizeof(MultiRenderPreset) == 60, "MultiRenderPreset is 60 B");
+static_assert(offsetof(InstanceData, transform) == 64, "InstanceData layout");
+static_assert(offsetof(InstanceData, transformInv) == 128, "InstanceData layout");
+static_assert(offsetof(InstanceData, transformInvTransposed) == 192, "InstanceData layout");
+
+// ---- constants (utils/raytracing/raytrace_common.h:35-57, cbvh.h:17-19) ----
+constexpr int      STACK_SIZE  = 80;
+constexpr uint32_t START_MASK  = 0x0FFFFFFFu;
+constexpr uint32_t SIZE_MASK   = 0x70000000u;
+constexpr uint32_t LEAF_BIT    = 0x80000000u;
+constexpr uint32_t LEAF_NORMAL = 0xFFFFFFFFu;
+constexpr uint32_t LEAF_EMPTY  = 0xFFFFFFFDu;
+constexpr uint32_t ESCAPE_ROOT = 0xFFFFFFFEu;
+
+// ---- geometry types and tags (core/render_settings.h:3-42, BVH2Common.h:30-43) ----
+constexpr uint32_t TYPE_MESH_TRIANGLE    = 0;
+constexpr uint32_t TYPE_SDF_FRAME_OCTREE = 3;
+constexpr uint32_t TYPE_SDF_SBS          = 6;
+constexpr uint32_t TYPE_SDF_MULTI_OCTREE = 9;
+constexpr uint32_t TYPE_SDF_SBS_TEX      = 10;
+constexpr uint32_t TYPE_SDF_SBS_COL      = 11;
+constexpr uint32_t SH_TYPE      = 24;
+constexpr uint32_t GEOM_ID_MASK = 0x00FFFFFFu;
+
+constexpr uint32_t TAG_NONE       = 0;
+constexpr uint32_t TAG_TRIANGLE   = 1;
+constexpr uint32_t TAG_SDF_BRICK  = 4;
+constexpr uint32_t TAG_SDF_NODE   = 8;
+constexpr uint32_t TAG_SDF_OCTREE = 15;
+
+// SBS node layouts (sdf/simple/sdf_simple_shared.h:13-21)
+constexpr uint32_t SDF_SBS_NODE_LAYOUT_UNDEFINED        = 0u << 24;
+constexpr uint32_t SDF_SBS_NODE_LAYOUT_DX               = 1u << 24;
+constexpr uint32_t SDF_SBS_NODE_LAYOUT_DX_UV16          = 2u << 24;
+constexpr uint32_t SDF_SBS_NODE_LAYOUT_DX_RGB8          = 3u << 24;
+constexpr uint32_t SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F    = 4u << 24;
+constexpr uint32_t SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN = 5u << 24;
+constexpr uint32_t SDF_SBS_NODE_LAYOUT_MASK             = 0xFF000000u;
+
+constexpr uint32_t INVALID_IDX = 1u << 31;   // sdf/common/sdf_common_shared.h:30
+
+// preset enums (core/render_settings.h:53-106)
+constexpr uint32_t SDF_OCTREE_NODE_INTERSECT_ST       = 0;
+constexpr uint32_t SDF_OCTREE_NODE_INTERSECT_ANALYTIC = 1;
+constexpr uint32_t SDF_OCTREE_NODE_INTERSECT_NEWTON   = 2;
+constexpr uint32_t SDF_OCTREE_NODE_INTERSECT_BBOX     = 3;
+constexpr uint32_t SDF_OCTREE_NODE_INTERSECT_IT       = 4;
+constexpr uint32_t MULTI_RENDER_MODE_MASK             = 0;
+constexpr uint32_t MULTI_RENDER_MODE_LAMBERT_NO_TEX   = 1;
+constexpr uint32_t MULTI_RENDER_MODE_NORMAL           = 8;
+constexpr uint32_t MULTI_RENDER_MODE_ST_ITERATIONS    = 10;
+constexpr uint32_t MULTI_RENDER_MODE_PBR_NO_TEX       = 11;
+constexpr uint32_t MULTI_RENDER_MODE_PHONG_NO_TEX     = 12;
+constexpr uint32_t MULTI_RENDER_MODE_LAMBERT          = 17;
+constexpr uint32_t MULTI_RENDER_MODE_PHONG            = 18;
+constexpr uint32_t MULTI_RENDER_MODE_LOD              = 20;
+constexpr uint32_t MULTI_RENDER_MODE_GEOM             = 7;
+constexpr uint32_t MULTI_RENDER_MODE_PBR              = 21;
+constexpr uint32_t MULTI_RENDER_MODE_CHANNEL          = 23;
+constexpr uint32_t MULTI_RENDER_MODE_INTERSECTION_COUNT = 26;
+constexpr uint32_t MULTI_RENDER_MODE_CHANNEL_LAMBERT  = 27;
+constexpr uint32_t NORMAL_MODE_GEOMETRY     = 0;
+constexpr uint32_t NORMAL_MODE_VERTEX       = 1;
+constexpr uint32_t NORMAL_MODE_SDF_SMOOTHED = 2;
+constexpr uint32_t INTERPOLATION_MODE_TRILINEAR = 0;
+constexpr uint32_t REPRESENTATION_MODE_SURFACE  = 0;
+constexpr uint32_t REPRESENTATION_MODE_VOLUME   = 1;
+
+// core/render_settings.h:138-160
+inline MultiRenderPreset default_preset() {
+  MultiRenderPreset p{};
+  p.render_mode = MULTI_RENDER_MODE_LAMBERT_NO_TEX;
+  p.sdf_node_intersect = SDF_OCTREE_NODE_INTERSECT_ANALYTIC;
+  p.normal_mode = NORMAL_MODE_VERTEX;
+  p.ray_gen_mode = 0;
+  p.interpolation_mode = INTERPOLATION_MODE_TRILINEAR;
+  p.representation_mode = REPRESENTATION_MODE_VOLUME;
+  p.spp = 1;
+  p.fixed_lod = 1;
+  p.level_of_detail = 16.0f;
+  p.compact_sdf_eps = 0.05f;
+  p.LOD_distance = 10.0f;
+  return p;
+}
+
+// MultiRenderer/BVH2Common.cpp:19-32
+LRT_HD bool preset_needs_normal(uint32_t m) {
+  return m == MULTI_RENDER_MODE_LAMBERT_NO_TEX || m == MULTI_RENDER_MODE_NORMAL ||
+         m == MULTI_RENDER_MODE_PHONG_NO_TEX || m == MULTI_RENDER_MODE_LAMBERT ||
+         m == MULTI_RENDER_MODE_LOD || m == MULTI_RENDER_MODE_GEOM ||
+         m == MULTI_RENDER_MODE_PHONG || m == MULTI_RENDER_MODE_CHANNEL ||
+         m == MULTI_RENDER_MODE_PBR || m == MULTI_RENDER_MODE_PBR_NO_TEX ||
+         m == MULTI_RENDER_MODE_CHANNEL_LAMBERT;
+}
+
+LRT_HD uint32_t pack_offset_and_size(uint32_t start, uint32_t size) {
+  return LEAF_BIT | ((size << 28) & SIZE_MASK) | (start & START_MASK);
+}
+LRT_HD uint32_t extract_start(uint32_t v) { return v & START_MASK; }
+LRT_HD uint32_t extract_count(uint32_t v) { return (v & SIZE_MASK) >> 28; }
+
+}  // namespace lrt

@@ -1,0 +1,392 @@
+// Host-side scene assembly: restates the BVHRT host methods
+// (MultiRenderer/BVH2Common_host.cpp) over our own builder, then mirrors every array
+// into device memory at commit.
+#include "scene.h"
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+
+#include "host_math.h"
+
+namespace lrt {
+
+hipError_t DevBuf::reserve(size_t bytes) {
+  if (bytes <= cap_) { used_ = bytes; return hipSuccess; }
+  release();
+  hipError_t e = hipMalloc(&p_, bytes);
+  if (e != hipSuccess) { p_ = nullptr; return e; }
+  cap_ = bytes;
+  used_ = bytes;
+  return hipSuccess;
+}
+
+hipError_t DevBuf::upload(const void* src, size_t bytes) {
+  if (bytes == 0) { used_ = 0; return hipSuccess; }
+  hipError_t e = reserve(bytes);
+  if (e != hipSuccess) return e;
+  return hipMemcpy(p_, src, bytes, hipMemcpyHostToDevice);
+}
+
+void DevBuf::release() {
+  if (p_) (void)hipFree(p_);
+  p_ = nullptr;
+  cap_ = used_ = 0;
+}
+
+Scene::~Scene() {
+  dev.clear();
+  if (d_overflow) (void)hipFree(d_overflow);
+}
+
+namespace {
+
+struct Box4 {
+  float4 mn, mx;
+};
+Box4 empty_box4() {
+  const float inf = std::numeric_limits<float>::infinity();
+  return {make4(inf, inf, inf, inf), make4(-inf, -inf, -inf, -inf)};
+}
+// Box4f::include (LiteMath.h:3422-3426), all four components
+void include4(Box4& b, float4 p) {
+  b.mn = make4(fmin_ref(b.mn.x, p.x), fmin_ref(b.mn.y, p.y), fmin_ref(b.mn.z, p.z), fmin_ref(b.mn.w, p.w));
+  b.mx = make4(fmax_ref(b.mx.x, p.x), fmax_ref(b.mx.y, p.y), fmax_ref(b.mx.z, p.z), fmax_ref(b.mx.w, p.w));
+}
+
+// add_border_nodes_rec for frame octrees (BVH2Common_host.cpp:1348-1383): one box per leaf
+// whose corner values change sign; leftOffset keeps the octree node index.
+void add_border_nodes_rec(const SdfFrameOctreeNode* nodes, size_t n_nodes, std::vector<BVHNode>& out, uint32_t idx,
+                          float3 p, float d) {
+  const uint32_t ofs = nodes[idx].offset;
+  if (ofs != 0 && size_t(ofs) + 8 > n_nodes) throw std::invalid_argument("frame octree child offset out of range");
+  if (ofs == 0) {
+    bool less = false, more = false;
+    for (int i = 0; i < 8; i++) {
+      if (nodes[idx].values[i] <= 0) less = true;
+      else if (nodes[idx].values[i] >= 0) more = true;
+    }
+    if (less && more) {
+      const float3 min_pos = 2.0f * (d * p) - 1.0f;
+      const float3 max_pos = min_pos + 2.0f * d * make3(1, 1, 1);
+      BVHNode n;
+      n.boxMin = min_pos;
+      n.boxMax = max_pos;
+      n.leftOffset = idx;
+      n.escapeIndex = 0;
+      out.push_back(n);
+    }
+  } else {
+    for (int i = 0; i < 8; i++) {
+      const float ch_d = d / 2;
+      const float3 ch_p = 2.0f * p + make3(float((i & 4) >> 2), float((i & 2) >> 1), float(i & 1));
+      add_border_nodes_rec(nodes, n_nodes, out, ofs + i, ch_p, ch_d);
+    }
+  }
+}
+
+}  // namespace
+
+void Scene::clear_scene() {
+  instances.clear();
+  tlas.clear();
+  committed = false;
+}
+
+void Scene::clear_geom() {
+  geomData.clear(); geomTags.clear(); startEnd.clear();
+  vertPos.clear(); vertNorm.clear(); indices.clear(); primIndices.clear(); primIdCount.clear();
+  nodePairs.clear(); origNodes.clear();
+  frameNodes.clear(); frameRoots.clear();
+  sbsNodes.clear(); sbsData.clear(); sbsDataF.clear(); sbsRoots.clear(); sbsHeaders.clear();
+  multiNodes.clear(); multiValues.clear();
+  clear_scene();
+}
+
+// BVHRT::AddGeom_Triangles3f (BVH2Common_host.cpp:307-359) + AppendTreeData (:287-301)
+uint32_t Scene::add_triangles(const float* vpos, size_t n_verts, size_t v_stride_bytes, const float* vnorm,
+                              const uint32_t* idx, size_t n_idx) {
+  if (v_stride_bytes % 4 != 0 || v_stride_bytes < 12) throw std::invalid_argument("vertex stride must be a multiple of 4 and >= 12");
+  if (n_idx % 3 != 0) throw std::invalid_argument("index count must be a multiple of 3");
+  for (size_t i = 0; i < n_idx; ++i)
+    if (idx[i] >= n_verts) throw std::invalid_argument("triangle index out of range");
+  const size_t vStride = v_stride_bytes / 4;
+  const size_t oldSizeVert = vertPos.size();
+  const size_t oldSizeInd = indices.size();
+  vertPos.resize(oldSizeVert + n_verts);
+  vertNorm.resize(oldSizeVert + n_verts);
+  Box4 bbox = empty_box4();
+  for (size_t i = 0; i < n_verts; i++) {
+    const float4 v = make4(vpos[i * vStride + 0], vpos[i * vStride + 1], vpos[i * vStride + 2], 1.0f);
+    vertPos[oldSizeVert + i] = v;
+    vertNorm[oldSizeVert + i] = vnorm ? make4(vnorm[i * vStride + 0], vnorm[i * vStride + 1], vnorm[i * vStride + 2], 1.0f)
+                                      : make4(1.0f, 0.0f, 0.0f, 1.0f);
+    include4(bbox, v);
+  }
+  GeomData g{};
+  g.boxMin = bbox.mn;
+  g.boxMax = bbox.mx;
+  g.offset = uint2{uint32_t(oldSizeInd), uint32_t(oldSizeVert)};
+  g.bvhOffset = uint32_t(nodePairs.size());
+  g.type = TYPE_MESH_TRIANGLE;
+  geomData.push_back(g);
+  geomTags.push_back(TAG_TRIANGLE);
+
+  FatBlas bvh = build_blas_triangles(vertPos.data() + oldSizeVert, n_verts, idx, n_idx, prims_in_leaf);
+  nodePairs.insert(nodePairs.end(), bvh.nodes.begin(), bvh.nodes.end());
+  primIndices.insert(primIndices.end(), bvh.indices.begin(), bvh.indices.end());
+  indices.resize(oldSizeInd + bvh.indices.size() * 3);
+  for (size_t i = 0; i < bvh.indices.size(); i++) {
+    const uint32_t triId = bvh.indices[i];
+    indices[oldSizeInd + 3 * i + 0] = idx[triId * 3 + 0];
+    indices[oldSizeInd + 3 * i + 1] = idx[triId * 3 + 1];
+    indices[oldSizeInd + 3 * i + 2] = idx[triId * 3 + 2];
+  }
+  const size_t oldSize = primIdCount.size();
+  primIdCount.insert(primIdCount.end(), bvh.startCount.begin(), bvh.startCount.end());
+  startEnd.push_back(uint2{uint32_t(oldSize), uint32_t(primIdCount.size())});
+  committed = false;
+  return uint32_t(startEnd.size() - 1);
+}
+
+// BVHRT::AddGeom_AABB (BVH2Common_host.cpp:441-459)
+uint32_t Scene::add_aabb(uint32_t tag, const BVHNode* boxes, size_t n) {
+  FatBlas bvh = build_blas_boxes(boxes, n);
+  nodePairs.insert(nodePairs.end(), bvh.nodes.begin(), bvh.nodes.end());
+  const size_t oldSize = primIdCount.size();
+  primIdCount.resize(oldSize + n);
+  for (size_t i = 0; i < n; i++) primIdCount[oldSize + i] = uint32_t(i);
+  startEnd.push_back(uint2{uint32_t(oldSize), uint32_t(primIdCount.size())});
+  geomTags.push_back(tag);
+  committed = false;
+  return uint32_t(startEnd.size() - 1);
+}
+
+// BVHRT::AddGeom_SdfFrameOctree (BVH2Common_host.cpp:501-534). The reference keeps the leaf
+// boxes of the most recently added octree only (m_origNodes = orig_nodes); here they are
+// appended and GeomData.offset.y holds each octree's first box, which is 0 for the first
+// octree and therefore identical for single-octree scenes.
+uint32_t Scene::add_sdf_frame_octree(const SdfFrameOctreeNode* nodes, size_t n) {
+  if (n == 0 || n >= (1u << 28)) throw std::invalid_argument("frame octree size out of range");
+  std::vector<BVHNode> boxes;
+  add_border_nodes_rec(nodes, n, boxes, 0, make3(0, 0, 0), 1.0f);
+  if (boxes.empty()) throw std::invalid_argument("frame octree has no surface leaves");
+  GeomData g{};
+  g.boxMin = make4(-1, -1, -1, 1);
+  g.boxMax = make4(1, 1, 1, 1);
+  g.offset = uint2{uint32_t(frameRoots.size()), uint32_t(origNodes.size())};
+  g.bvhOffset = uint32_t(nodePairs.size());
+  g.type = TYPE_SDF_FRAME_OCTREE;
+  geomData.push_back(g);
+  const uint32_t n_offset = uint32_t(frameNodes.size());
+  frameRoots.push_back(n_offset);
+  frameNodes.insert(frameNodes.end(), nodes, nodes + n);
+  for (size_t i = n_offset; i < frameNodes.size(); i++) frameNodes[i].offset += n_offset;
+  origNodes.insert(origNodes.end(), boxes.begin(), boxes.end());
+  return add_aabb(TAG_SDF_NODE, boxes.data(), boxes.size());
+}
+
+// BVHRT::AddGeom_SdfSBS (BVH2Common_host.cpp:601-695)
+uint32_t Scene::add_sdf_sbs(const SdfSBSHeader& header, const SdfSBSNode* nodes, size_t n_nodes,
+                            const uint32_t* values, size_t n_values, const float* values_f, size_t n_values_f) {
+  if (n_nodes == 0 || n_values == 0) throw std::invalid_argument("empty sparse brick set");
+  if (header.brick_size < 1 || header.brick_size > 16) throw std::invalid_argument("brick_size must be 1..16");
+  if (header.bytes_per_value != 1 && header.bytes_per_value != 2 && header.bytes_per_value != 4)
+    throw std::invalid_argument("bytes_per_value must be 1, 2 or 4");
+  const uint32_t layout = header.aux_data & SDF_SBS_NODE_LAYOUT_MASK;
+  uint32_t type = TYPE_SDF_SBS;
+  if (layout == SDF_SBS_NODE_LAYOUT_UNDEFINED || layout == SDF_SBS_NODE_LAYOUT_DX) type = TYPE_SDF_SBS;
+  else if (layout == SDF_SBS_NODE_LAYOUT_DX_UV16) type = TYPE_SDF_SBS_TEX;
+  else if (layout == SDF_SBS_NODE_LAYOUT_DX_RGB8 || layout == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F ||
+           layout == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN)
+    type = TYPE_SDF_SBS_COL;
+  else
+    throw std::invalid_argument("unsupported SBS node layout");
+  const bool indexed = layout == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F || layout == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN;
+  if (indexed && (values_f == nullptr || n_values_f == 0)) throw std::invalid_argument("indexed SBS layout needs values_f");
+  // Bounds of every data access the intersector can make (checked on the host once).
+  const uint32_t v_size = header.brick_size + 2 * header.brick_pad + 1;
+  const uint32_t vpi = 4 / header.bytes_per_value;
+  const size_t per_brick = indexed ? size_t(v_size) * v_size * v_size + 8
+                                   : (size_t(v_size) * v_size * v_size + vpi - 1) / vpi +
+                                         ((layout == SDF_SBS_NODE_LAYOUT_DX_RGB8 || layout == SDF_SBS_NODE_LAYOUT_DX_UV16) ? 8 : 0);
+  for (size_t i = 0; i < n_nodes; ++i) {
+    if (size_t(nodes[i].data_offset) + per_brick > n_values) throw std::invalid_argument("brick data out of range");
+    const uint32_t lod = nodes[i].pos_z_lod_size & 0xFFFF;
+    if (lod == 0) throw std::invalid_argument("brick lod_size is 0");
+  }
+  if (indexed) {
+    const size_t v3 = size_t(v_size) * v_size * v_size;
+    for (size_t i = 0; i < n_nodes; ++i) {
+      const uint32_t* w = values + nodes[i].data_offset;
+      for (size_t k = 0; k < v3; ++k)
+        if (w[k] >= n_values_f) throw std::invalid_argument("distance index out of range");
+      for (size_t k = v3; k < v3 + 8; ++k)
+        if (size_t(w[k]) + 3 > n_values_f) throw std::invalid_argument("colour index out of range");
+    }
+  }
+  GeomData g{};
+  g.boxMin = make4(-1, -1, -1, 1);
+  g.boxMax = make4(1, 1, 1, 1);
+  g.offset = uint2{uint32_t(sbsRoots.size()), 0};
+  g.bvhOffset = uint32_t(nodePairs.size());
+  g.type = type;
+  geomData.push_back(g);
+  const uint32_t n_offset = uint32_t(sbsNodes.size());
+  const uint32_t v_offset = uint32_t(sbsData.size());
+  sbsRoots.push_back(n_offset);
+  sbsHeaders.push_back(header);
+  sbsNodes.insert(sbsNodes.end(), nodes, nodes + n_nodes);
+  sbsData.insert(sbsData.end(), values, values + n_values);
+  for (size_t i = n_offset; i < sbsNodes.size(); i++) sbsNodes[i].data_offset += v_offset;
+  if (indexed) {
+    const uint32_t f_offset = uint32_t(sbsDataF.size());
+    sbsDataF.insert(sbsDataF.end(), values_f, values_f + n_values_f);
+    for (size_t i = v_offset; i < sbsData.size(); i++) sbsData[i] += f_offset;
+  }
+  std::vector<BVHNode> boxes(n_nodes);
+  for (size_t i = 0; i < n_nodes; i++) {
+    const float px = float(nodes[i].pos_xy >> 16);
+    const float py = float(nodes[i].pos_xy & 0x0000FFFF);
+    const float pz = float(nodes[i].pos_z_lod_size >> 16);
+    const float sz = float(nodes[i].pos_z_lod_size & 0x0000FFFF);
+    boxes[i].boxMin = make3(-1, -1, -1) + (2.0f * make3(px, py, pz)) / sz;
+    boxes[i].boxMax = boxes[i].boxMin + (2.0f * make3(1, 1, 1)) / sz;
+    boxes[i].leftOffset = 0;
+    boxes[i].escapeIndex = 0;
+  }
+  if (boxes.size() == 1) {  // one-brick edge case (BVH2Common_host.cpp:684-692)
+    boxes.resize(2);
+    boxes[1].boxMin = boxes[0].boxMin - 0.001f * make3(1, 1, 1);
+    boxes[1].boxMax = boxes[0].boxMin;
+  }
+  return add_aabb(TAG_SDF_BRICK, boxes.data(), boxes.size());
+}
+
+// BVHRT::AddInstance (BVH2Common_host.cpp:1264-1301)
+uint32_t Scene::add_instance(uint32_t geom_id, const float4x4& m) {
+  if (geom_id & 0x80000000u) geom_id &= 0x7fffffffu;
+  if (geom_id >= geomData.size()) throw std::invalid_argument("invalid geometry id");
+  const float4 bmin = geomData[geom_id].boxMin, bmax = geomData[geom_id].boxMax;
+  const float4 corners[8] = {
+      make4(bmin.x, bmin.y, bmin.z, 1.0f), make4(bmax.x, bmin.y, bmin.z, 1.0f), make4(bmin.x, bmax.y, bmin.z, 1.0f),
+      make4(bmin.x, bmin.y, bmax.z, 1.0f), make4(bmax.x, bmax.y, bmin.z, 1.0f), make4(bmax.x, bmin.y, bmax.z, 1.0f),
+      make4(bmin.x, bmax.y, bmax.z, 1.0f), make4(bmax.x, bmax.y, bmax.z, 1.0f)};
+  Box4 box = empty_box4();
+  for (int i = 0; i < 8; i++) include4(box, mat_mul4(m.m, corners[i]));
+  InstanceData inst{};
+  inst.boxMin = box.mn;
+  inst.boxMax = box.mx;
+  inst.geomId = geom_id;
+  inst.transform = m;
+  inst.transformInv = inverse4x4(m);
+  inst.transformInvTransposed = transpose4x4(inverse4x4(m));
+  instances.push_back(inst);
+  committed = false;
+  return uint32_t(instances.size() - 1);
+}
+
+// BVHRT::UpdateInstance (BVH2Common_host.cpp:1303-1335). As in the reference, the
+// normal matrix (transformInvTransposed) is left as it was.
+void Scene::update_instance(uint32_t inst_id, const float4x4& m) {
+  if (inst_id >= instances.size()) throw std::invalid_argument("bad instance id");
+  const uint32_t geom_id = instances[inst_id].geomId;
+  const float4 bmin = geomData[geom_id].boxMin, bmax = geomData[geom_id].boxMax;
+  const float4 corners[8] = {
+      make4(bmin.x, bmin.y, bmin.z, 1.0f), make4(bmax.x, bmin.y, bmin.z, 1.0f), make4(bmin.x, bmax.y, bmin.z, 1.0f),
+      make4(bmin.x, bmin.y, bmax.z, 1.0f), make4(bmax.x, bmax.y, bmin.z, 1.0f), make4(bmax.x, bmin.y, bmax.z, 1.0f),
+      make4(bmin.x, bmax.y, bmax.z, 1.0f), make4(bmax.x, bmax.y, bmax.z, 1.0f)};
+  Box4 box = empty_box4();
+  for (int i = 0; i < 8; i++) include4(box, mat_mul4(m.m, corners[i]));
+  instances[inst_id].boxMin = box.mn;
+  instances[inst_id].boxMax = box.mx;
+  instances[inst_id].transform = m;
+  instances[inst_id].transformInv = inverse4x4(m);
+  committed = false;
+}
+
+// BVHRT::SetPreset (BVH2Common_host.cpp:1437-1440)
+void Scene::set_preset(const MultiRenderPreset& p) {
+  preset = p;
+  lod_thr = std::pow(2.0f, preset.level_of_detail) - 0.01f;
+}
+
+uint32_t Scene::leaf_kinds() const {
+  uint32_t k = 0;
+  for (uint32_t t : geomTags) {
+    if (t == TAG_TRIANGLE) k |= LK_TRI;
+    else if (t == TAG_SDF_NODE) k |= LK_FRAME;
+    else if (t == TAG_SDF_BRICK) k |= LK_SBS;
+    else if (t == TAG_SDF_OCTREE) k |= LK_MULTI;
+  }
+  return k;
+}
+
+// BVHRT::CommitScene (BVH2Common_host.cpp:1230-1262), then device upload.
+void Scene::build_host() {
+  if (instances.empty()) throw std::invalid_argument("commit with no instances");
+  std::vector<float4> bmin(instances.size()), bmax(instances.size());
+  for (size_t i = 0; i < instances.size(); ++i) { bmin[i] = instances[i].boxMin; bmax[i] = instances[i].boxMax; }
+  tlas = build_tlas(bmin.data(), bmax.data(), instances.size());
+}
+
+hipError_t Scene::commit() {
+  hipError_t e = hipGetDevice(&device);
+  if (e != hipSuccess) return e;
+#define UP(name, vec)                                                        \
+  e = dev[name].upload(vec.data(), vec.size() * sizeof(vec[0]));             \
+  if (e != hipSuccess) return e;
+  UP("nodes", nodePairs) UP("tlas", tlas) UP("instances", instances) UP("geoms", geomData) UP("geom_tags", geomTags)
+  UP("start_end", startEnd) UP("prim_id_count", primIdCount) UP("vert_pos", vertPos) UP("vert_norm", vertNorm)
+  UP("indices", indices) UP("prim_indices", primIndices) UP("orig_nodes", origNodes) UP("frame_nodes", frameNodes)
+  UP("frame_roots", frameRoots) UP("sbs_nodes", sbsNodes) UP("sbs_data", sbsData) UP("sbs_data_f", sbsDataF)
+  UP("sbs_roots", sbsRoots) UP("sbs_headers", sbsHeaders) UP("multi_nodes", multiNodes) UP("multi_values", multiValues)
+#undef UP
+  if (!d_overflow) {
+    e = hipMalloc(&d_overflow, sizeof(uint32_t));
+    if (e != hipSuccess) return e;
+    e = hipMemset(d_overflow, 0, sizeof(uint32_t));
+    if (e != hipSuccess) return e;
+  }
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) return e;
+  committed = true;
+  return hipSuccess;
+}
+
+SceneView Scene::make_view() const {
+  SceneView v{};
+  auto P = [&](const char* n) -> const void* {
+    auto it = dev.find(n);
+    return (it == dev.end() || it->second.bytes() == 0) ? nullptr : it->second.ptr();
+  };
+  v.nodes = (const BVHNodePair*)P("nodes");
+  v.tlas = (const BVHNode*)P("tlas");
+  v.instances = (const InstanceData*)P("instances");
+  v.geoms = (const GeomData*)P("geoms");
+  v.geom_tags = (const uint32_t*)P("geom_tags");
+  v.start_end = (const uint2*)P("start_end");
+  v.prim_id_count = (const uint32_t*)P("prim_id_count");
+  v.vert_pos = (const float4*)P("vert_pos");
+  v.vert_norm = (const float4*)P("vert_norm");
+  v.indices = (const uint32_t*)P("indices");
+  v.prim_indices = (const uint32_t*)P("prim_indices");
+  v.orig_nodes = (const BVHNode*)P("orig_nodes");
+  v.frame_nodes = (const SdfFrameOctreeNode*)P("frame_nodes");
+  v.frame_roots = (const uint32_t*)P("frame_roots");
+  v.sbs_nodes = (const SdfSBSNode*)P("sbs_nodes");
+  v.sbs_data = (const uint32_t*)P("sbs_data");
+  v.sbs_data_f = (const float*)P("sbs_data_f");
+  v.sbs_roots = (const uint32_t*)P("sbs_roots");
+  v.sbs_headers = (const SdfSBSHeader*)P("sbs_headers");
+  v.multi_nodes = (const SdfMultiOctreeNode*)P("multi_nodes");
+  v.multi_values = (const float*)P("multi_values");
+  v.preset = preset;
+  v.lod_thr = lod_thr;
+  v.leaf_kinds = leaf_kinds();
+  v.n_instances = uint32_t(instances.size());
+  return v;
+}
+
+}  // namespace lrt

@@ -1,0 +1,85 @@
+// Host-side scene assembly (the BVHRT host half) and its device mirror.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "bvh_build.h"
+#include "kernels.h"
+#include "scene_view.h"
+
+namespace lrt {
+
+class DevBuf {
+ public:
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  hipError_t upload(const void* src, size_t bytes);
+  hipError_t reserve(size_t bytes);
+  void release();
+  void* ptr() const { return p_; }
+  size_t bytes() const { return used_; }
+
+ private:
+  void* p_ = nullptr;
+  size_t cap_ = 0;
+  size_t used_ = 0;
+};
+
+// Mirrors the member arrays of BVHRT (MultiRenderer/BVH2Common.h:262-365).
+struct Scene {
+  std::string build_name, layout_name;
+  int prims_in_leaf = 2;
+  MultiRenderPreset preset = default_preset();
+  float lod_thr = 0.0f;
+
+  std::vector<GeomData> geomData;
+  std::vector<uint32_t> geomTags;
+  std::vector<uint2> startEnd;
+  std::vector<float4> vertPos, vertNorm;
+  std::vector<uint32_t> indices, primIndices, primIdCount;
+  std::vector<BVHNodePair> nodePairs;
+  std::vector<BVHNode> origNodes;
+  std::vector<SdfFrameOctreeNode> frameNodes;
+  std::vector<uint32_t> frameRoots;
+  std::vector<SdfSBSNode> sbsNodes;
+  std::vector<uint32_t> sbsData;
+  std::vector<float> sbsDataF;
+  std::vector<uint32_t> sbsRoots;
+  std::vector<SdfSBSHeader> sbsHeaders;
+  std::vector<SdfMultiOctreeNode> multiNodes;
+  std::vector<float> multiValues;
+  std::vector<InstanceData> instances;
+  std::vector<BVHNode> tlas;
+
+  bool committed = false;
+  int device = 0;
+  std::map<std::string, DevBuf> dev;
+  SceneView view{};
+  uint32_t* d_overflow = nullptr;
+  int block = 256, lds_stack = 16, xcd_remap = 1;
+
+  ~Scene();
+
+  void clear_geom();
+  void clear_scene();
+  uint32_t add_triangles(const float* vpos, size_t n_verts, size_t v_stride_bytes, const float* vnorm,
+                         const uint32_t* idx, size_t n_idx);
+  uint32_t add_aabb(uint32_t tag, const BVHNode* boxes, size_t n);
+  uint32_t add_sdf_frame_octree(const SdfFrameOctreeNode* nodes, size_t n);
+  uint32_t add_sdf_sbs(const SdfSBSHeader& header, const SdfSBSNode* nodes, size_t n_nodes, const uint32_t* values,
+                       size_t n_values, const float* values_f, size_t n_values_f);
+  uint32_t add_instance(uint32_t geom_id, const float4x4& m);
+  void update_instance(uint32_t inst_id, const float4x4& m);
+  void build_host();   // TLAS over the instance boxes (host only)
+  hipError_t commit(); // device upload of every array
+  void set_preset(const MultiRenderPreset& p);
+  uint32_t leaf_kinds() const;
+  SceneView make_view() const;
+};
+
+}  // namespace lrt

@@ -1,0 +1,522 @@
+// gfx950 device code for the closest-hit ray query: TLAS walk, fat-node BLAS walk and
+// the leaf intersectors. One ray per lane. Arithmetic follows the reference's operation
+// order (compiled with -ffp-contract=off); see the file:line citations per function.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "lrt_math.h"
+#include "scene_view.h"
+
+namespace lrt {
+
+// ---------------------------------------------------------------------------------------
+// Traversal stack: the first S entries live in LDS (entry-major, one column per lane, so
+// lanes never conflict on a bank), deeper entries spill to a per-lane private array.
+// Depth limit STACK_SIZE=80 as in raytrace_common.h:36; pushes beyond it are dropped and
+// counted instead of writing out of bounds.
+template <int S>
+struct LaneStack {
+  uint32_t* col;      // &lds[threadIdx.x]; entry i at col[i * stride]
+  uint32_t  stride;
+  uint32_t  spill[STACK_SIZE - S];
+  __device__ __forceinline__ void put(int i, uint32_t v) {
+    if (i < S) col[i * stride] = v; else spill[i - S] = v;
+  }
+  __device__ __forceinline__ uint32_t get(int i) const {
+    return i < S ? col[i * stride] : spill[i - S];
+  }
+};
+
+// Per-ray traversal counters of the instrumented kernel variant (the reference's RayInfo
+// counters, BVH2Common.cpp:3403-3571): fat nodes fetched, leaves entered, triangles tested,
+// TLAS nodes fetched, instances entered, SBS voxels stepped.
+struct RayCounts {
+  uint32_t blas_nodes, leaves, prims, tlas_nodes, instances, sdf_voxels;
+};
+
+// Triangle hit whose normal is resolved once the closest hit is final (the normal depends
+// only on the winning triangle and its barycentrics, BVH2Common.cpp:1609-1626).
+struct PendingTri {
+  uint32_t valid, geomId, instId, triId;
+};
+
+__device__ __forceinline__ BVHNodePair load_pair(const BVHNodePair* p) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+  uint4 a = q[0], b = q[1], c = q[2], d = q[3];
+  BVHNodePair r;
+  r.left.boxMin = make3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
+  r.left.leftOffset = a.w;
+  r.left.boxMax = make3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z));
+  r.left.escapeIndex = b.w;
+  r.right.boxMin = make3(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z));
+  r.right.leftOffset = c.w;
+  r.right.boxMax = make3(__uint_as_float(d.x), __uint_as_float(d.y), __uint_as_float(d.z));
+  r.right.escapeIndex = d.w;
+  return r;
+}
+
+__device__ __forceinline__ BVHNode load_node(const BVHNode* p) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+  uint4 a = q[0], b = q[1];
+  BVHNode r;
+  r.boxMin = make3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
+  r.leftOffset = a.w;
+  r.boxMax = make3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z));
+  r.escapeIndex = b.w;
+  return r;
+}
+
+__device__ __forceinline__ void load_mat(const float4x4& m, float* out) {
+  const float4* q = reinterpret_cast<const float4*>(&m);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float4 v = q[i];
+    out[4 * i + 0] = v.x; out[4 * i + 1] = v.y; out[4 * i + 2] = v.z; out[4 * i + 3] = v.w;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Triangles: BVHRT::IntersectAllTrianglesInLeaf (BVH2Common.cpp:1571-1635)
+__device__ __forceinline__ void intersect_tris(const SceneView& s, float3 ray_pos, float3 ray_dir, float tNear,
+                                               uint32_t instId, uint32_t geomId, uint32_t a_start, uint32_t a_count,
+                                               CRT_Hit& hit, PendingTri& pend) {
+  const uint2 off = s.geoms[geomId].offset;
+  for (uint32_t triId = a_start; triId < a_start + a_count; triId++) {
+    const uint32_t base = off.x + triId * 3;
+    const uint32_t A = s.indices[base + 0];
+    const uint32_t B = s.indices[base + 1];
+    const uint32_t C = s.indices[base + 2];
+    const float3 A_pos = xyz(s.vert_pos[off.y + A]);
+    const float3 B_pos = xyz(s.vert_pos[off.y + B]);
+    const float3 C_pos = xyz(s.vert_pos[off.y + C]);
+    const float3 edge1 = B_pos - A_pos;
+    const float3 edge2 = C_pos - A_pos;
+    const float3 pvec = cross3(ray_dir, edge2);
+    const float3 tvec = ray_pos - A_pos;
+    const float3 qvec = cross3(tvec, edge1);
+    const float invDet = 1.0f / dot3(edge1, pvec);
+    const float v = dot3(tvec, pvec) * invDet;
+    const float u = dot3(qvec, ray_dir) * invDet;
+    const float t = dot3(edge2, qvec) * invDet;
+    if (v >= -1e-6f && u >= -1e-6f && (u + v <= 1.0f + 1e-6f) && t > tNear && t < hit.t) {
+      hit.t = t;
+      hit.primId = triId;
+      hit.instId = instId;
+      hit.geomId = geomId | (TYPE_MESH_TRIANGLE << SH_TYPE);
+      hit.coords[0] = u;
+      hit.coords[1] = v;
+      hit.coords[2] = 0.0f;
+      hit.coords[3] = 0.0f;
+      pend.valid = 1; pend.geomId = geomId; pend.instId = instId; pend.triId = triId;
+    }
+  }
+}
+
+// Normal of the final triangle hit (BVH2Common.cpp:1609-1631).
+__device__ __forceinline__ void resolve_tri_normal(const SceneView& s, const PendingTri& pend, CRT_Hit& hit) {
+  if (!pend.valid || !preset_needs_normal(s.preset.render_mode)) return;
+  const uint2 off = s.geoms[pend.geomId].offset;
+  const uint32_t base = off.x + pend.triId * 3;
+  const uint32_t A = s.indices[base + 0], B = s.indices[base + 1], C = s.indices[base + 2];
+  const float u = hit.coords[0], v = hit.coords[1];
+  float3 n = make3(1.0f, 0.0f, 0.0f);
+  if (s.preset.normal_mode == NORMAL_MODE_GEOMETRY) {
+    const float3 A_pos = xyz(s.vert_pos[off.y + A]);
+    const float3 edge1 = xyz(s.vert_pos[off.y + B]) - A_pos;
+    const float3 edge2 = xyz(s.vert_pos[off.y + C]) - A_pos;
+    n = cross3(edge1, edge2);
+  } else if (s.preset.normal_mode == NORMAL_MODE_VERTEX) {
+    const float4 nA = s.vert_norm[off.y + A], nB = s.vert_norm[off.y + B], nC = s.vert_norm[off.y + C];
+    const float wA = 1.0f - u - v;
+    n = make3(nA.x * wA + nB.x * v + u * nC.x, nA.y * wA + nB.y * v + u * nC.y, nA.z * wA + nB.z * v + u * nC.z);
+  }
+  float m[16];
+  load_mat(s.instances[pend.instId].transformInvTransposed, m);
+  n = normalize3(matmul4x3(m, n));
+  const float2 e = encode_normal(n);
+  hit.coords[2] = e.x;
+  hit.coords[3] = e.y;
+}
+
+// ---------------------------------------------------------------------------------------
+// SDF helpers
+// BVHRT::eval_dist_trilinear (BVH2Common.cpp:126-136)
+__device__ __forceinline__ float eval_dist_trilinear(const float v[8], float3 dp) {
+  return (1 - dp.x) * (1 - dp.y) * (1 - dp.z) * v[0] +
+         (1 - dp.x) * (1 - dp.y) * (dp.z) * v[1] +
+         (1 - dp.x) * (dp.y) * (1 - dp.z) * v[2] +
+         (1 - dp.x) * (dp.y) * (dp.z) * v[3] +
+         (dp.x) * (1 - dp.y) * (1 - dp.z) * v[4] +
+         (dp.x) * (1 - dp.y) * (dp.z) * v[5] +
+         (dp.x) * (dp.y) * (1 - dp.z) * v[6] +
+         (dp.x) * (dp.y) * (dp.z) * v[7];
+}
+
+// BVHRT::LocalSurfaceIntersection (BVH2Common.cpp:345-725) for the non-compact SDF types
+// used here (frame octree, SBS, multi octree). `is_surface_node` is always false for them.
+// Returns true when the hit record was updated.
+__device__ bool local_surface_intersection(const SceneView& s, uint32_t type, float3 ray_dir, uint32_t instId,
+                                           uint32_t geomId, const float values[8], uint32_t primId, float d,
+                                           float qNear, float qFar, float2 fNearFar, float3 start_q,
+                                           CRT_Hit& hit) {
+  const float EPS = 1e-6f;
+  const float d_inv = 1.0f / d;
+  float t = qNear;
+  bool early_hit = false;
+  bool hitp = false;
+  unsigned iter = 0;
+  float start_sign = 1;
+  float start_dist = eval_dist_trilinear(values, start_q + t * ray_dir);
+  if (s.preset.representation_mode == REPRESENTATION_MODE_SURFACE) start_sign = sign_ref(start_dist);
+  start_dist *= start_sign;
+  const uint32_t mode = s.preset.sdf_node_intersect;
+
+  if (start_dist <= EPS || mode == SDF_OCTREE_NODE_INTERSECT_BBOX) {
+    early_hit = true;
+    hitp = true;
+  } else if (mode == SDF_OCTREE_NODE_INTERSECT_ST) {
+    const unsigned ST_max_iters = 256;
+    float dist = start_dist;
+    while (t < qFar && dist > EPS && iter < ST_max_iters) {
+      t += dist * d_inv;
+      dist = start_sign * eval_dist_trilinear(values, start_q + t * ray_dir);
+      iter++;
+    }
+    hitp = (dist <= EPS);
+  } else {
+    const float s000 = values[0] * d_inv, s001 = values[1] * d_inv, s010 = values[2] * d_inv,
+                s011 = values[3] * d_inv, s100 = values[4] * d_inv, s101 = values[5] * d_inv,
+                s110 = values[6] * d_inv, s111 = values[7] * d_inv;
+    const float tmp = s101 - s001;
+    const float k0 = s000;
+    const float k1 = s100 - s000;
+    const float k2 = s010 - s000;
+    const float k3 = s110 - s010 - k1;
+    const float k4 = k0 - s001;
+    const float k5 = k1 - tmp;
+    const float k6 = k2 - (s011 - s001);
+    const float k7 = k3 - (s111 - s011 - tmp);
+    const float3 o = start_q;
+    const float3 d3 = ray_dir;
+    const float m0 = o.x * o.y;
+    const float m1 = d3.x * d3.y;
+    const float m2 = o.x * d3.y + o.y * d3.x;
+    const float m3 = k5 * o.z - k1;
+    const float m4 = k6 * o.z - k2;
+    const float m5 = k7 * o.z - k3;
+    const float c0 = (k4 * o.z - k0) + o.x * m3 + o.y * m4 + m0 * m5;
+    const float c1 = d3.x * m3 + d3.y * m4 + m2 * m5 + d3.z * (k4 + k5 * o.x + k6 * o.y + k7 * m0);
+    const float c2 = m1 * m5 + d3.z * (k5 * d3.x + k6 * d3.y + k7 * m2);
+    const float c3 = k7 * m1 * d3.z;
+
+    if (mode == SDF_OCTREE_NODE_INTERSECT_ANALYTIC) {
+      float x1 = 1000, x2 = 1000, x3 = 1000;
+      if (fabs_ref(c3) > 1e-5f) {
+        const float a = c2 / c3, b = c1 / c3, c = c0 / c3;
+        const float Q = (a * a - 3 * b) / 9;
+        const float R = (2 * a * a * a - 9 * a * b + 27 * c) / 54;
+        const float Q3 = Q * Q * Q;
+        if (R * R < Q3) {
+          const float PI = 3.14159265358979323846f;
+          const float theta = acosf(R / sqrtf(Q3));
+          x1 = -2 * sqrtf(Q) * cosf(theta / 3) - a / 3;
+          x2 = -2 * sqrtf(Q) * cosf((theta + 2 * PI) / 3) - a / 3;
+          x3 = -2 * sqrtf(Q) * cosf((theta - 2 * PI) / 3) - a / 3;
+        } else {
+          const float A = -sign_ref(R) * powf(fabs_ref(R) + sqrtf(R * R - Q3), 1.0f / 3.0f);
+          const float B = fabs_ref(A) > EPS ? Q / A : 0;
+          x1 = A + B - a / 3;
+        }
+      } else if (fabs_ref(c2) > 1e-7f) {
+        const float a = c2, b = c1, c = c0;
+        const float D = b * b - 4 * a * c;
+        if (D > 0) {
+          const float q = -0.5f * (b + sign_ref(b) * sqrtf(D));
+          x1 = q / a;
+          if (fabs_ref(q) > EPS) x2 = c / q;
+        }
+      } else if (fabs_ref(c1) > EPS) {
+        x1 = -c0 / c1;
+      }
+      x1 = x1 < 0 ? 1000 : x1;
+      x2 = x2 < 0 ? 1000 : x2;
+      x3 = x3 < 0 ? 1000 : x3;
+      t = fmin_ref(x1, fmin_ref(x2, x3));
+      hitp = (t >= 0 && t <= qFar);
+    } else if (mode == SDF_OCTREE_NODE_INTERSECT_NEWTON) {
+      const float a = 3 * c3, b = 2 * c2, c = c1;
+      const float t0 = 0;
+      float t1 = qFar, t2 = qFar;
+      const float t3 = qFar;
+      const float D = b * b - 4 * a * c;
+      if (D >= 0) {
+        const float q = -0.5f * (b + sign_ref(b) * sqrtf(D));
+        t1 = fabs_ref(a) > EPS ? q / a : t0;
+        t2 = fabs_ref(q) > EPS ? c / q : qFar;
+        const float min_t = fmin_ref(t1, t2);
+        t2 = fmax_ref(t1, t2);
+        t1 = min_t;
+        t1 = clamp_ref(t1, t0, t3);
+        t2 = clamp_ref(t2, t0, t3);
+      }
+      const bool s0 = c0 > 0;
+      const bool s1 = (c0 + t1 * (c1 + t1 * (c2 + t1 * c3))) > 0;
+      const bool s2 = (c0 + t2 * (c1 + t2 * (c2 + t2 * c3))) > 0;
+      const bool s3 = (c0 + t3 * (c1 + t3 * (c2 + t3 * c3))) > 0;
+      float nwt_min = t0, nwt_max = t0;
+      if (s0 != s1) { nwt_min = t0; nwt_max = t1; }
+      else if (s1 != s2) { nwt_min = t1; nwt_max = t2; }
+      else if (s2 != s3) { nwt_min = t2; nwt_max = t3; }
+      if (nwt_min < nwt_max) {
+        const unsigned max_iters = 10;
+        unsigned nwt_iter = 0;
+        float rtn = 0.5f * (nwt_min + nwt_max);
+        float f = 1000;
+        while (nwt_iter < max_iters && fabs_ref(f) >= EPS) {
+          f = c0 + rtn * (c1 + rtn * (c2 + rtn * c3));
+          const float df = c1 + rtn * (2 * c2 + rtn * 3 * c3);
+          const float dx = f / (df + sign_ref(df) * 1e-9f);
+          rtn -= dx;
+          nwt_iter++;
+        }
+        t = rtn;
+        hitp = (t >= 0 && t <= qFar && fabs_ref(f) < EPS);
+      } else {
+        hitp = false;
+      }
+    } else {  // SDF_OCTREE_NODE_INTERSECT_IT
+      const unsigned IT_max_iters = 256;
+      const float k = 2;
+      float e = 0.1f * qFar;
+      const float t_max = fabs_ref(c3) < EPS ? 1e6f : -c2 / (3 * c3);
+      const float df_max = 3 * c3 * t_max * t_max + 2 * c2 * t_max + c1;
+      float dist = start_dist;
+      while (t < qFar && dist > EPS && iter < IT_max_iters) {
+        const float df_1 = 3 * c3 * t * t + 2 * c2 * t + c1;
+        const float df_2 = 3 * c3 * (t + e) * (t + e) + 2 * c2 * (t + e) + c1;
+        float L = (t_max > t && t_max < t + e) ? fmax_ref(df_max, fmax_ref(df_1, df_2)) : fmax_ref(df_1, df_2);
+        L = fmax_ref(L, EPS);
+        const float st = fmin_ref((dist * d_inv) / L, e);
+        t += st;
+        e = k * st;
+        dist = start_sign * eval_dist_trilinear(values, start_q + t * ray_dir);
+        iter++;
+      }
+      hitp = (dist <= EPS);
+    }
+  }
+
+  const float tReal = fNearFar.x + d * t;
+  if (t <= qFar && hitp && tReal < hit.t) {
+    float3 norm = make3(0, 0, 1);
+    if (preset_needs_normal(s.preset.render_mode)) {
+      if (early_hit && mode == SDF_OCTREE_NODE_INTERSECT_BBOX) {
+        norm = make3(0, 0, 0);
+        const float3 v = make3(fmax_ref(start_q.x, 1 - start_q.x), fmax_ref(start_q.y, 1 - start_q.y),
+                               fmax_ref(start_q.z, 1 - start_q.z));
+        int dim = 0;
+        if (v.y > v.x && v.y > v.z) dim = 1;
+        else if (v.z > v.x && v.z > v.y) dim = 2;
+        const float sq = dim == 0 ? start_q.x : (dim == 1 ? start_q.y : start_q.z);
+        const float nv = sq < 0.5f ? -1.0f : 1.0f;
+        if (dim == 0) norm.x = nv; else if (dim == 1) norm.y = nv; else norm.z = nv;
+      } else {
+        const float3 p0 = start_q + t * ray_dir;
+        const float h = 0.001f;
+        const float ddx = (eval_dist_trilinear(values, p0 + make3(h, 0, 0)) -
+                           eval_dist_trilinear(values, p0 + make3(-h, 0, 0))) / (2 * h);
+        const float ddy = (eval_dist_trilinear(values, p0 + make3(0, h, 0)) -
+                           eval_dist_trilinear(values, p0 + make3(0, -h, 0))) / (2 * h);
+        const float ddz = (eval_dist_trilinear(values, p0 + make3(0, 0, h)) -
+                           eval_dist_trilinear(values, p0 + make3(0, 0, -h))) / (2 * h);
+        norm = start_sign * make3(ddx, ddy, ddz);
+      }
+      float m[16];
+      load_mat(s.instances[instId].transformInvTransposed, m);
+      norm = normalize3(matmul4x3(m, norm));
+    }
+    const float2 en = encode_normal(norm);
+    hit.t = tReal;
+    hit.primId = primId;
+    hit.instId = instId;
+    hit.geomId = geomId | (type << SH_TYPE);
+    hit.coords[0] = 0;
+    hit.coords[1] = 0;
+    hit.coords[2] = en.x;
+    hit.coords[3] = en.y;
+    if (s.preset.render_mode == MULTI_RENDER_MODE_ST_ITERATIONS) hit.primId = iter;
+    return true;
+  }
+  return false;
+}
+
+// Frame-octree leaf: BVHRT::OctreeNodeIntersect + RayNodeIntersection (BVH2Common.cpp:727-749, 141-167)
+__device__ __forceinline__ void frame_node_intersect(const SceneView& s, uint32_t type, float3 ray_pos, float3 ray_dir,
+                                                     float tNear, uint32_t instId, uint32_t geomId,
+                                                     uint32_t bvhNodeId, CRT_Hit& hit) {
+  if (type != TYPE_SDF_FRAME_OCTREE) return;
+  const uint2 go = s.geoms[geomId].offset;   // x: octree id, y: first leaf box of this octree
+  const uint32_t sdfId = go.x;
+  const BVHNode on = load_node(s.orig_nodes + go.y + bvhNodeId);
+  const uint32_t primId = on.leftOffset;
+  const uint32_t nodeId = primId + s.frame_roots[sdfId];
+  const float3 min_pos = on.boxMin, max_pos = on.boxMax;
+  const float3 size = max_pos - min_pos;
+  float values[8];
+  const SdfFrameOctreeNode* fn = s.frame_nodes + nodeId;
+#pragma unroll
+  for (int i = 0; i < 8; i++) values[i] = fn->values[i];
+  const float2 fNearFar = ray_box2(ray_pos, safe_inverse(ray_dir), min_pos, max_pos);
+  const float3 start_pos = ray_pos + fNearFar.x * ray_dir;
+  const float d = fmax_ref(size.x, fmax_ref(size.y, size.z));
+  const float3 start_q = (start_pos - min_pos) / d;
+  const float qFar = (fNearFar.y - fNearFar.x) / d;
+  const float qNear = tNear > fNearFar.x ? (tNear - fNearFar.x) / d : 0.0f;
+  if (qNear > 0.0f) return;
+  local_surface_intersection(s, type, ray_dir, instId, geomId, values, primId, d, qNear, qFar, fNearFar, start_q, hit);
+}
+
+// BVHRT::load_distance_values (BVH2Common.cpp:846-882); returns the minimum value.
+__device__ __forceinline__ float sbs_load_values(const SceneView& s, const SdfSBSHeader& h, uint32_t nodeId,
+                                                 float3 voxelPos, uint32_t v_size, float sz_inv, float values[8]) {
+  float vmin = 1e6f;
+  const uint32_t v_off = s.sbs_nodes[nodeId].data_offset;
+  const uint32_t vx = uint32_t(voxelPos.x), vy = uint32_t(voxelPos.y), vz = uint32_t(voxelPos.z);
+  if (h.aux_data == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F || h.aux_data == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const uint32_t px = vx + ((i & 4) >> 2), py = vy + ((i & 2) >> 1), pz = vz + (i & 1);
+      const uint32_t vId = (px + h.brick_pad) * v_size * v_size + (py + h.brick_pad) * v_size + (pz + h.brick_pad);
+      values[i] = s.sbs_data_f[s.sbs_data[v_off + vId]];
+      vmin = fmin_ref(vmin, values[i]);
+    }
+  } else {
+    const uint32_t vals_per_int = 4 / h.bytes_per_value;
+    const uint32_t bits = 8 * h.bytes_per_value;
+    const uint32_t max_val = h.bytes_per_value == 4 ? 0xFFFFFFFFu : ((1u << bits) - 1);
+    const float d_max = 1.73205081f * sz_inv;
+    const float mult = 2 * d_max / float(max_val);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const uint32_t px = vx + ((i & 4) >> 2), py = vy + ((i & 2) >> 1), pz = vz + (i & 1);
+      const uint32_t vId = (px + h.brick_pad) * v_size * v_size + (py + h.brick_pad) * v_size + (pz + h.brick_pad);
+      const uint32_t word = s.sbs_data[v_off + vId / vals_per_int];
+      const uint32_t sh = bits * (vId % vals_per_int);
+      values[i] = -d_max + mult * float((sh < 32 ? (word >> sh) : 0u) & max_val);
+      vmin = fmin_ref(vmin, values[i]);
+    }
+  }
+  return vmin;
+}
+
+// Sparse-brick-set leaf: BVHRT::OctreeBrickIntersect (BVH2Common.cpp:884-1112), trilinear.
+// Returns the number of voxels stepped.
+__device__ uint32_t sbs_brick_intersect(const SceneView& s, uint32_t type, float3 ray_pos, float3 ray_dir, float tNear,
+                                        uint32_t instId, uint32_t geomId, uint32_t bvhNodeId, CRT_Hit& hit) {
+  uint32_t steps = 0;
+  const uint32_t sdfId = s.geoms[geomId].offset.x;
+  const uint32_t primId = bvhNodeId;
+  const uint32_t nodeId = primId + s.sbs_roots[sdfId];
+  const SdfSBSHeader header = s.sbs_headers[sdfId];
+  const uint32_t v_size = header.brick_size + 2 * header.brick_pad + 1;
+  const SdfSBSNode node = s.sbs_nodes[nodeId];
+  const float px = float(node.pos_xy >> 16);
+  const float py = float(node.pos_xy & 0x0000FFFF);
+  const float pz = float(node.pos_z_lod_size >> 16);
+  const float sz = float(node.pos_z_lod_size & 0x0000FFFF);
+  const float sz_inv = 2.0f / sz;
+  const float bs = float(header.brick_size);
+  const float d = 2.0f / (sz * bs);
+  const float3 brick_min_pos = make3(-1, -1, -1) + sz_inv * make3(px, py, pz);
+  const float3 brick_max_pos = brick_min_pos + sz_inv * make3(1, 1, 1);
+  const float3 inv = safe_inverse(ray_dir);
+  float2 brick_fNearFar = ray_box2(ray_pos, inv, brick_min_pos, brick_max_pos);
+  const float old_t = hit.t;
+  const float scale = 0.5f * sz * bs;
+  const float hi = bs - 1e-6f;
+  // A ray crosses at most 3*brick_size-2 voxels of a brick; the cap only ends marches whose
+  // 1e-6 advance is lost to rounding (the reference would loop forever on those).
+  const uint32_t max_steps = 4 * header.brick_size + 8;
+  while (brick_fNearFar.x < brick_fNearFar.y && hit.t == old_t && steps < max_steps) {
+    steps++;
+    const float3 hit_pos = ray_pos + brick_fNearFar.x * ray_dir;
+    const float3 local_pos = (hit_pos - brick_min_pos) * scale;
+    const float3 voxelPos = make3(floorf(clamp_ref(local_pos.x, 1e-6f, hi)), floorf(clamp_ref(local_pos.y, 1e-6f, hi)),
+                                  floorf(clamp_ref(local_pos.z, 1e-6f, hi)));
+    const float3 min_pos = brick_min_pos + d * voxelPos;
+    const float3 max_pos = min_pos + d * make3(1, 1, 1);
+    float values[8];
+    const float vmin = sbs_load_values(s, header, nodeId, voxelPos, v_size, sz_inv, values);
+    const float2 fNearFar = ray_box2(ray_pos, inv, min_pos, max_pos);
+    if (tNear < fNearFar.x && vmin <= 0.0f) {
+      const float3 start_pos = ray_pos + fNearFar.x * ray_dir;
+      const float3 start_q = (start_pos - min_pos) * scale;
+      const float qFar = (fNearFar.y - fNearFar.x) * scale;
+      local_surface_intersection(s, type, ray_dir, instId, geomId, values, primId, d, 0.0f, qFar, fNearFar, start_q,
+                                 hit);
+    }
+    brick_fNearFar.x += fmax_ref(0.0f, fNearFar.y - brick_fNearFar.x) + 1e-6f;
+  }
+  if (hit.t < old_t) {
+    const float3 pos = ray_pos + hit.t * ray_dir;
+    const float3 dp = (pos - brick_min_pos) * (0.5f * sz);
+    const uint32_t aux = header.aux_data;
+    if (aux == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F || aux == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN) {
+      const uint32_t t_off = node.data_offset + v_size * v_size * v_size;
+      float3 col = make3(0, 0, 0);
+      const float w[8] = {(1 - dp.x) * (1 - dp.y) * (1 - dp.z), (1 - dp.x) * (1 - dp.y) * (dp.z),
+                          (1 - dp.x) * (dp.y) * (1 - dp.z),     (1 - dp.x) * (dp.y) * (dp.z),
+                          (dp.x) * (1 - dp.y) * (1 - dp.z),     (dp.x) * (1 - dp.y) * (dp.z),
+                          (dp.x) * (dp.y) * (1 - dp.z),         (dp.x) * (dp.y) * (dp.z)};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t b = s.sbs_data[t_off + i];
+        const float3 c = make3(s.sbs_data_f[b + 0], s.sbs_data_f[b + 1], s.sbs_data_f[b + 2]);
+        col = (i == 0) ? w[0] * c : col + w[i] * c;
+      }
+      col = make3(clamp_ref(floorf(255.0f * col.x + 0.5f), 0.0f, 255.0f),
+                  clamp_ref(floorf(255.0f * col.y + 0.5f), 0.0f, 255.0f),
+                  clamp_ref(floorf(255.0f * col.z + 0.5f), 0.0f, 255.0f));
+      hit.coords[0] = col.x + col.y / 256.0f;
+      hit.coords[1] = col.z;
+    } else if (aux == SDF_SBS_NODE_LAYOUT_DX_RGB8) {
+      const uint32_t vals_per_int = 4 / header.bytes_per_value;
+      const uint32_t t_off = node.data_offset + (v_size * v_size * v_size + vals_per_int - 1) / vals_per_int;
+      const float w[8] = {(1 - dp.x) * (1 - dp.y) * (1 - dp.z), (1 - dp.x) * (1 - dp.y) * (dp.z),
+                          (1 - dp.x) * (dp.y) * (1 - dp.z),     (1 - dp.x) * (dp.y) * (dp.z),
+                          (dp.x) * (1 - dp.y) * (1 - dp.z),     (dp.x) * (1 - dp.y) * (dp.z),
+                          (dp.x) * (dp.y) * (1 - dp.z),         (dp.x) * (dp.y) * (dp.z)};
+      float3 col = make3(0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t c = s.sbs_data[t_off + i];
+        const float3 cf = make3(float((c >> 0) & 0xFF), float((c >> 8) & 0xFF), float((c >> 16) & 0xFF));
+        col = (i == 0) ? w[0] * cf : col + w[i] * cf;
+      }
+      col = make3(clamp_ref(floorf(col.x + 0.5f), 0.0f, 255.0f), clamp_ref(floorf(col.y + 0.5f), 0.0f, 255.0f),
+                  clamp_ref(floorf(col.z + 0.5f), 0.0f, 255.0f));
+      hit.coords[0] = col.x + col.y / 256.0f;
+      hit.coords[1] = col.z;
+    } else if (aux == SDF_SBS_NODE_LAYOUT_DX_UV16) {
+      const uint32_t vals_per_int = 4 / header.bytes_per_value;
+      const uint32_t t_off = node.data_offset + (v_size * v_size * v_size + vals_per_int - 1) / vals_per_int;
+      const float w[8] = {(1 - dp.x) * (1 - dp.y) * (1 - dp.z), (1 - dp.x) * (1 - dp.y) * (dp.z),
+                          (1 - dp.x) * (dp.y) * (1 - dp.z),     (1 - dp.x) * (dp.y) * (dp.z),
+                          (dp.x) * (1 - dp.y) * (1 - dp.z),     (dp.x) * (1 - dp.y) * (dp.z),
+                          (dp.x) * (dp.y) * (1 - dp.z),         (dp.x) * (dp.y) * (dp.z)};
+      float u = 0, v = 0;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t c = s.sbs_data[t_off + i];
+        const float tu = w[i] * (1.5259022e-5f * float(c >> 16));
+        const float tv = w[i] * (1.5259022e-5f * float(c & 0xFFFF));
+        u = (i == 0) ? tu : u + tu;
+        v = (i == 0) ? tv : v + tv;
+      }
+      hit.coords[0] = u;
+      hit.coords[1] = v;
+    }
+  }
+  return steps;
+}
+
+}  // namespace lrt

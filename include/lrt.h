@@ -1,0 +1,152 @@
+/*
+ * lrt.h - C ABI of the MI355X (gfx950) ray-query library.
+ *
+ * This is the drop-in boundary behind LiteRT's ISceneObject / BVHRT interface
+ * (CrossRT/CrossRT.h:43-175, MultiRenderer/BVH2Common.h:66-381). Every entry point
+ * names the reference method it replaces. Signatures use plain pointers, sizes and
+ * the reference's POD structs (restated byte-for-byte in
+ * graphic_lab_task_amd/csrc/lrt_types.h); no C++ or torch types cross the ABI.
+ *
+ * Conventions
+ *  - Every function returns lrt_status (0 = LRT_OK). On failure lrt_last_error()
+ *    returns a message for the calling thread. Nothing throws across the ABI.
+ *  - Build calls (add/commit/instance) need external synchronisation per scene;
+ *    trace calls are thread-safe per committed scene.
+ *  - The library copies every array passed to lrt_add_* (the reference copies
+ *    caller arrays into its own vectors, BVH2Common_host.cpp:317-327, 651-666).
+ *  - Ray and hit buffers of the *_dev calls are device pointers owned by the caller;
+ *    `stream` is a hipStream_t (NULL = default stream).
+ *  - Matrices are float[16], column-major (LiteMath float4x4).
+ */
+#ifndef LRT_H_
+#define LRT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int lrt_status;
+#define LRT_OK 0
+#define LRT_ERR_INVALID_ARG 1
+#define LRT_ERR_HIP 2
+#define LRT_ERR_NOT_COMMITTED 3
+#define LRT_ERR_UNSUPPORTED 4
+#define LRT_ERR_NO_DEVICE 5
+
+typedef struct lrt_scene lrt_scene;
+
+/* Opaque-pointer forms of the reference structs (see lrt_types.h for the layouts). */
+typedef struct lrt_hit { float t; uint32_t primId, instId, geomId; float coords[4]; } lrt_hit;  /* CRT_Hit, CrossRT.h:26-33 */
+
+/* Last error message of the calling thread ("" if none). */
+const char* lrt_last_error(void);
+
+/* Library version string. */
+const char* lrt_version(void);
+
+/* Replaces CreateSceneRT(impl, build, layout) (MultiRenderer/BVH2Common_host.cpp:1594-1600).
+ * impl must be "BVH2Common"; the last digit of build_name sets primitives per leaf
+ * (BuilderPresetsFromString, utils/raytracing/cbvh.cpp:57-69); layout_name is accepted and
+ * ignored as in BuildBVHFat (cbvh_fat.cpp:150-166). Fails with LRT_ERR_UNSUPPORTED on an
+ * unknown impl (the reference returns an empty shared_ptr). */
+lrt_status lrt_create(const char* impl_name, const char* build_name, const char* layout_name, lrt_scene** out);
+
+/* Destroys the scene and frees its device buffers. */
+lrt_status lrt_destroy(lrt_scene* s);
+
+/* Replaces BVHRT::ClearGeom (BVH2Common_host.cpp:128-156). */
+lrt_status lrt_clear_geom(lrt_scene* s);
+
+/* Replaces BVHRT::AddGeom_Triangles3f (BVH2Common_host.cpp:303-359), with the optional
+ * per-vertex normals of the 7-argument overload (BVH2Common.h:93-94). vnorm may be NULL
+ * (then normals are (1,0,0) as in BVH2Common_host.cpp:325); both arrays use v_byte_stride. */
+lrt_status lrt_add_triangles(lrt_scene* s, const float* vpos, size_t n_verts, size_t v_byte_stride,
+                             const float* vnorm, const uint32_t* indices, size_t n_indices, uint32_t build_flags,
+                             uint32_t* out_geom_id);
+
+/* Replaces BVHRT::AddGeom_SdfFrameOctree (BVH2Common_host.cpp:501-534).
+ * nodes: SdfFrameOctreeNode[n_nodes] (36 B each, sdf/simple/sdf_simple_shared.h:23-27). */
+lrt_status lrt_add_sdf_frame_octree(lrt_scene* s, const void* nodes, size_t n_nodes, uint32_t* out_geom_id);
+
+/* Replaces BVHRT::AddGeom_SdfSBS (BVH2Common_host.cpp:601-695).
+ * header: SdfSBSHeader, nodes: SdfSBSNode[n_nodes], values: uint32[n_values],
+ * values_f: float[n_values_f] (indexed layouts only, may be NULL otherwise). */
+lrt_status lrt_add_sdf_sbs(lrt_scene* s, const void* header, const void* nodes, size_t n_nodes,
+                           const uint32_t* values, size_t n_values, const float* values_f, size_t n_values_f,
+                           uint32_t* out_geom_id);
+
+/* Replaces BVHRT::ClearScene (clears instances, keeps geometry). */
+lrt_status lrt_clear_scene(lrt_scene* s);
+
+/* Replaces BVHRT::AddInstance (BVH2Common_host.cpp:1264-1301). */
+lrt_status lrt_add_instance(lrt_scene* s, uint32_t geom_id, const float* matrix16, uint32_t* out_inst_id);
+
+/* Replaces BVHRT::UpdateInstance (BVH2Common_host.cpp:1303-1335); call lrt_commit afterwards. */
+lrt_status lrt_update_instance(lrt_scene* s, uint32_t inst_id, const float* matrix16);
+
+/* Replaces BVHRT::CommitScene (BVH2Common_host.cpp:1230-1262): builds the TLAS and uploads
+ * every array to the current HIP device. */
+lrt_status lrt_commit(lrt_scene* s, uint32_t build_flags);
+
+/* Replaces BVHRT::SetPreset (BVH2Common_host.cpp:1437-1454). preset: MultiRenderPreset
+ * (core/render_settings.h:116-136, 60 B). Takes effect on the next trace call. */
+lrt_status lrt_set_preset(lrt_scene* s, const void* preset);
+
+/* Batched RayQuery_NearestHit (BVH2Common.cpp:3322-3392) over device buffers:
+ * pos_near/dir_far are float4[n] (xyz + tNear / xyz + tFar; tFar <= 0 selects
+ * stop-on-first-hit as in the reference), hits is lrt_hit[n]. Asynchronous on `stream`. */
+lrt_status lrt_trace_closest_dev(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,
+                                 uint64_t n, void* stream);
+
+/* Same over host buffers (copies in, traces, copies out, synchronises). */
+lrt_status lrt_trace_closest_host(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,
+                                  uint64_t n);
+
+/* Instrumented variant of lrt_trace_closest_dev, replacing RayQuery_NearestHitWithRayInfo
+ * (BVH2Common.cpp:3403-3571): also writes 6 uint32 counters per ray to `counters` (device,
+ * 6*n): fat BLAS nodes fetched, leaves entered, triangles tested, TLAS nodes fetched,
+ * instances entered, SBS voxels stepped. Same hits as the plain call. */
+lrt_status lrt_trace_closest_counted_dev(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,
+                                         uint32_t* counters, uint64_t n, void* stream);
+
+/* Batched RayQuery_AnyHit (BVH2Common.cpp:3394-3399): flags[i] = 1 if ray i hits anything. */
+lrt_status lrt_trace_any_dev(lrt_scene* s, const float* pos_near, const float* dir_far, uint32_t* flags,
+                             uint64_t n, void* stream);
+
+/* Single-ray RayQuery_NearestHit for picking (demo_app/plugins/MRFull/execute_commands.cpp:189);
+ * runs the same gfx950 kernel on one ray and synchronises. */
+lrt_status lrt_query_one(lrt_scene* s, const float* pos_near4, const float* dir_far4, lrt_hit* out);
+
+/* Primary rays of MultiRenderer::initEyeRay (MultiRenderer/MultiRenderer.cpp:142-183) for a
+ * width x height viewport (cutting plane disabled): proj_inv = inverse4x4(proj),
+ * world_view_inv = inverse4x4(worldView) (MultiRenderer_host.cpp:104-110). jitter is an
+ * optional device float2[width*height] sub-pixel offset (NULL = pixel centre for spp 1,
+ * sample 0). order 0 writes ray y*width+x, order 1 the SuperBlockIndex2DOpt position
+ * (render_common.h:225-242; width and height must then be multiples of 8). */
+lrt_status lrt_eye_rays_dev(const float* proj_inv16, const float* world_view_inv16, uint32_t width, uint32_t height,
+                            const float* jitter, uint32_t order, float* pos_near, float* dir_far, void* stream);
+
+/* Kernel configuration used by the trace calls: block size (64/128/256), number of traversal
+ * stack entries kept in LDS (8/16/32), XCD-contiguous block mapping (0/1). */
+lrt_status lrt_set_launch_config(lrt_scene* s, int block, int lds_stack, int xcd_remap);
+
+/* Number of rays (since the last reset) whose traversal stack reached the reference limit of
+ * 80 entries (raytrace_common.h:36); the reference does not detect this. */
+lrt_status lrt_stack_overflow_count(lrt_scene* s, uint64_t* out, int reset);
+
+/* Host copies of the committed scene arrays (for checking and inspection). name is one of:
+ * "nodes" (BVHNodePair), "tlas" (BVHNode), "instances" (InstanceData), "geoms" (GeomData),
+ * "geom_tags", "start_end" (uint2), "prim_id_count", "vert_pos" (float4), "vert_norm" (float4),
+ * "indices", "prim_indices", "orig_nodes" (BVHNode), "frame_nodes" (SdfFrameOctreeNode),
+ * "frame_roots", "sbs_nodes", "sbs_data", "sbs_data_f", "sbs_roots", "sbs_headers".
+ * With dst == NULL only *bytes is written. */
+lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LRT_H_ */

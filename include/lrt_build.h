@@ -1,0 +1,67 @@
+/*
+ * lrt_build.h - C ABI of the scene-input builders that sit in front of the ray-query path:
+ * the sparse-octree SDF builder and its converters (sdf/build, sdf/simple), the camera
+ * matrices of MultiRenderer::UpdateCamera, and the synthetic meshes of the benchmark
+ * workloads. Output arrays are library-allocated lrt_buffers released with lrt_buffer_free.
+ */
+#ifndef LRT_BUILD_H_
+#define LRT_BUILD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "lrt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lrt_buffer {
+  void* data;
+  size_t bytes;
+} lrt_buffer;
+
+/* Releases a buffer returned by this API (safe on a zeroed buffer). */
+void lrt_buffer_free(lrt_buffer* b);
+
+/* Camera of MultiRenderer::UpdateCamera (MultiRenderer/MultiRenderer_host.cpp:104-110) for
+ * worldView = lookAt(eye, target, up) and proj = perspectiveMatrix(fovy_deg, aspect, znear, zfar)
+ * (LiteMath.h:3315-3360, called as in render_CLI/offscreen_render.cpp:225-228): writes
+ * inverse4x4(proj) and inverse4x4(worldView), column-major. */
+lrt_status lrt_camera_inverse(const float* eye3, const float* target3, const float* up3, float fovy_deg, float aspect,
+                              float znear, float zfar, float* proj_inv16, float* world_view_inv16);
+
+/* Unit icosphere subdivided `subdiv` times (20*4^subdiv triangles), vertices displaced radially
+ * by (1 + amplitude * noise(frequency * v)) with seeded 3-D value noise. vpos: float3 per
+ * vertex, indices: uint32 triple per triangle. */
+lrt_status lrt_gen_icosphere(uint32_t subdiv, float amplitude, float frequency, uint64_t seed, lrt_buffer* vpos,
+                             lrt_buffer* indices);
+
+/* Geodesic icosahedron of frequency n (20*n^2 triangles), displaced as above. */
+lrt_status lrt_gen_geodesic(uint32_t n, float amplitude, float frequency, uint64_t seed, lrt_buffer* vpos,
+                            lrt_buffer* indices);
+
+/* sdf_converter::create_sdf_frame_octree (sdf/simple/sdf_converter.cpp:265-273) for the union
+ * of spheres (cx,cy,cz,r)[n_spheres]: sdf_to_global_octree with brick size 1, then
+ * global_octree_to_frame_octree. nodes: SdfFrameOctreeNode array. threads <= 0: all cores. */
+lrt_status lrt_sdf_spheres_frame_octree(const float* spheres, size_t n_spheres, uint32_t depth, int threads,
+                                        lrt_buffer* nodes);
+
+/* sdf_converter::create_sdf_SBS (sdf/simple/sdf_converter.cpp:311-322) for the sphere union:
+ * quantised DX / DX_UV16 layouts. header: SdfSBSHeader (brick_size, brick_pad, bytes_per_value,
+ * aux_data). nodes: SdfSBSNode array, values: uint32 array. */
+lrt_status lrt_sdf_spheres_sbs(const float* spheres, size_t n_spheres, uint32_t depth, const void* header, int threads,
+                               lrt_buffer* nodes, lrt_buffer* values);
+
+/* Indexed fp32 sparse brick set (layout ID32F_IRGB32F) for the sphere union: lattice points
+ * shared by bricks are stored once. values_f = [N distances][N albedo RGB] with a seeded
+ * colour field; *n_distance_values = N. */
+lrt_status lrt_sdf_spheres_sbs_indexed(const float* spheres, size_t n_spheres, uint32_t depth, uint32_t brick_size,
+                                       uint64_t albedo_seed, int threads, lrt_buffer* nodes, lrt_buffer* values,
+                                       lrt_buffer* values_f, uint32_t* n_distance_values);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LRT_BUILD_H_ */

@@ -1,0 +1,170 @@
+"""ORACLE - test infrastructure only.
+
+CPU restatement of LiteRT's ray-query path (oracle/oracle.cpp), used as the checker of the
+gfx950 library. Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may load
+it; the product never does.
+
+Two builds of the same source (oracle/Makefile):
+  "exact" (liboracle.so)     FMA contraction off, like the gfx950 kernels: bit-exact twin
+  "fma"   (liboracle_fma.so) the reference's own flags (-O3, contraction on)
+
+Function -> reference -> how it is pinned (tests/test_oracle_cpu.py unless stated):
+  oracle_trace_closest  BVH2Common.cpp:3322-3392, 3239-3320, 47-124, 1571-1635
+      pinned by brute force over all triangles (topology independence), by the closed-form
+      axial-ray distance on an icosphere, by instance-transform invariance, by the
+      stop-on-first-hit / miss-record special cases, and by a planted error (swapped
+      barycentrics) that the brute-force check must catch.
+  oracle_trace_closest  SDF leaves: BVH2Common.cpp:727-749, 141-167, 345-725, 884-1112
+      pinned by the sphere-union closed form (|f(hit)| within a voxel-size bound, normal
+      close to grad f), by cross-format agreement of frame octree vs. SBS of one octree, and
+      by a root check of the analytic cubic in float64 (tests/test_oracle_sdf.py).
+  oracle_trace_any      BVH2Common.cpp:3394-3399   pinned by AnyHit <=> closest primId != ~0
+  oracle_brute_force_tris (ours)                    pinned by an analytic ray/plane case
+  oracle_eye_rays       MultiRenderer.cpp:142-183   pinned by the pinhole closed form
+      (direction of pixel centre from fov/aspect) and the SuperBlockIndex2DOpt permutation
+      property (a bijection onto [0, w*h)).
+  oracle_camera_inverse LiteMath.h:3315-3360, 2070-2154  pinned by M * inverse(M) = I in
+      float64 and the closed-form perspective entries.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+_FILES = {"exact": "liboracle.so", "fma": "liboracle_fma.so"}
+
+HIT_DTYPE = np.dtype([("t", "<f4"), ("primId", "<u4"), ("instId", "<u4"), ("geomId", "<u4"),
+                      ("coords", "<f4", (4,))])
+COUNTS_DTYPE = np.dtype([("blas_nodes", "<u4"), ("leaves", "<u4"), ("prims", "<u4"), ("tlas_nodes", "<u4"),
+                         ("instances", "<u4"), ("sdf_voxels", "<u4")])
+
+_ARRAYS = ["nodes", "tlas", "instances", "geoms", "geom_tags", "start_end", "prim_id_count", "vert_pos",
+           "vert_norm", "indices", "prim_indices", "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes",
+           "sbs_data", "sbs_data_f", "sbs_roots", "sbs_headers"]
+
+
+class OracleScene(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in _ARRAYS] + [("preset", ctypes.c_uint8 * 60)]
+
+
+_libs = {}
+
+
+def available(flavor: str = "exact") -> bool:
+    return os.path.exists(os.path.join(LIB_DIR, _FILES[flavor]))
+
+
+def load(flavor: str = "exact") -> ctypes.CDLL:
+    if flavor not in _libs:
+        path = os.path.join(LIB_DIR, _FILES[flavor])
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library {path} is missing: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        vp, u64, u32, i32, f32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_float
+        L.oracle_flavor.restype = i32
+        L.oracle_trace_closest.argtypes = [ctypes.POINTER(OracleScene), vp, vp, vp, u64, i32, vp]
+        L.oracle_trace_closest.restype = u64
+        L.oracle_trace_any.argtypes = [ctypes.POINTER(OracleScene), vp, vp, vp, u64, i32]
+        L.oracle_trace_any.restype = None
+        L.oracle_brute_force_tris.argtypes = [vp, vp, u64, vp, vp, u64, vp, vp, vp, i32]
+        L.oracle_brute_force_tris.restype = None
+        L.oracle_eye_rays.argtypes = [vp, vp, u32, u32, vp, u32, vp, vp]
+        L.oracle_eye_rays.restype = None
+        L.oracle_camera_inverse.argtypes = [vp, vp, vp, f32, f32, f32, f32, vp, vp]
+        L.oracle_camera_inverse.restype = None
+        L.oracle_inverse4x4.argtypes = [vp, vp]
+        L.oracle_inverse4x4.restype = None
+        _libs[flavor] = L
+    return _libs[flavor]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class Scene:
+    """Holds the host arrays of a committed scene and the OracleScene view over them."""
+
+    def __init__(self, arrays: dict, preset: np.ndarray):
+        self._keep = {}
+        self.view = OracleScene()
+        for name in _ARRAYS:
+            a = arrays.get(name)
+            if a is not None and a.size:
+                a = np.ascontiguousarray(a)
+                self._keep[name] = a
+                setattr(self.view, name, a.ctypes.data)
+            else:
+                setattr(self.view, name, None)
+        raw = np.ascontiguousarray(preset).tobytes()
+        assert len(raw) == 60, "MultiRenderPreset is 60 bytes"
+        ctypes.memmove(self.view.preset, raw, 60)
+
+    @classmethod
+    def from_scene(cls, scene) -> "Scene":
+        """Copies every array out of a product SceneRT (its host-side mirror)."""
+        return cls({n: scene.get_array(n) for n in _ARRAYS}, scene.get_array("preset"))
+
+
+def trace_closest(scene: Scene, pos: np.ndarray, dir: np.ndarray, threads: int = 0, flavor: str = "exact",
+                  counts: bool = False):
+    pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 4)
+    dir = np.ascontiguousarray(dir, dtype=np.float32).reshape(-1, 4)
+    n = pos.shape[0]
+    hits = np.zeros(n, HIT_DTYPE)
+    cnt = np.zeros(n, COUNTS_DTYPE) if counts else None
+    ov = load(flavor).oracle_trace_closest(ctypes.byref(scene.view), _p(pos), _p(dir), _p(hits), n, threads,
+                                           _p(cnt))
+    return (hits, int(ov), cnt) if counts else (hits, int(ov))
+
+
+def trace_any(scene: Scene, pos, dir, threads: int = 0, flavor: str = "exact") -> np.ndarray:
+    pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 4)
+    dir = np.ascontiguousarray(dir, dtype=np.float32).reshape(-1, 4)
+    flags = np.zeros(pos.shape[0], np.uint32)
+    load(flavor).oracle_trace_any(ctypes.byref(scene.view), _p(pos), _p(dir), _p(flags), pos.shape[0], threads)
+    return flags
+
+
+def brute_force_tris(verts: np.ndarray, tris: np.ndarray, pos, dir, threads: int = 0, flavor: str = "exact"):
+    v4 = np.ones((verts.shape[0], 4), np.float32)
+    v4[:, :3] = verts[:, :3]
+    tris = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1)
+    pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 4)
+    dir = np.ascontiguousarray(dir, dtype=np.float32).reshape(-1, 4)
+    n = pos.shape[0]
+    t = np.zeros(n, np.float32)
+    ids = np.zeros(n, np.uint32)
+    t2 = np.zeros(n, np.float32)
+    load(flavor).oracle_brute_force_tris(_p(v4), _p(tris), tris.size // 3, _p(pos), _p(dir), n, _p(t), _p(ids),
+                                         _p(t2), threads)
+    return t, ids, t2
+
+
+def eye_rays(proj_inv, wv_inv, width, height, jitter=None, order=1, flavor: str = "exact"):
+    pi = np.ascontiguousarray(proj_inv, dtype=np.float32)
+    vi = np.ascontiguousarray(wv_inv, dtype=np.float32)
+    jt = None if jitter is None else np.ascontiguousarray(jitter, dtype=np.float32)
+    pos = np.zeros((width * height, 4), np.float32)
+    dir = np.zeros((width * height, 4), np.float32)
+    load(flavor).oracle_eye_rays(_p(pi), _p(vi), width, height, _p(jt), order, _p(pos), _p(dir))
+    return pos, dir
+
+
+def camera_inverse(eye, target, up, fovy, aspect, znear, zfar, flavor: str = "exact"):
+    e, t, u = (np.ascontiguousarray(v, dtype=np.float32) for v in (eye, target, up))
+    pi = np.zeros(16, np.float32)
+    vi = np.zeros(16, np.float32)
+    load(flavor).oracle_camera_inverse(_p(e), _p(t), _p(u), fovy, aspect, znear, zfar, _p(pi), _p(vi))
+    return pi, vi
+
+
+def inverse4x4(m16: np.ndarray, flavor: str = "exact") -> np.ndarray:
+    m = np.ascontiguousarray(m16, dtype=np.float32).reshape(16)
+    out = np.zeros(16, np.float32)
+    load(flavor).oracle_inverse4x4(_p(m), _p(out))
+    return out

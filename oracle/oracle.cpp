@@ -1,0 +1,813 @@
+// ORACLE - test infrastructure only. A CPU restatement of LiteRT's ray-query path used to
+// check the gfx950 library; it is never linked into or called by the product. Only tests/,
+// __graft_entry__.smoke() and bench.py's CPU-baseline leg load it.
+//
+// Independent of the product sources: its own structs and arithmetic, written from the
+// reference files cited per function. Parity status of every function is listed in the
+// header comment of oracle/__init__.py.
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+// ---- LiteMath subset (dependencies/1st-party/LiteMath/LiteMath.h) ----
+struct f2 { float x, y; };
+struct f3 {
+  float x, y, z;
+  f3() : x(0), y(0), z(0) {}
+  f3(float a, float b, float c) : x(a), y(b), z(c) {}
+};
+struct f4 { float x, y, z, w; };
+inline f3 operator+(f3 a, f3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline f3 operator-(f3 a, f3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline f3 operator*(f3 a, f3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline f3 operator/(f3 a, f3 b) { return f3(a.x / b.x, a.y / b.y, a.z / b.z); }
+inline f3 operator*(f3 a, float k) { return f3(a.x * k, a.y * k, a.z * k); }
+inline f3 operator*(float k, f3 a) { return f3(k * a.x, k * a.y, k * a.z); }
+inline f3 operator/(f3 a, float k) { return f3(a.x / k, a.y / k, a.z / k); }
+inline f3 operator-(f3 a, float k) { return f3(a.x - k, a.y - k, a.z - k); }
+inline f3 operator+(f3 a, float k) { return f3(a.x + k, a.y + k, a.z + k); }
+// LiteMath.h:133-138
+inline float lm_sign(float x) { if (x == 0.0f) return 0.0f; else if (x < 0.0f) return -1.0f; else return 1.0f; }
+// LiteMath.h:963-966
+inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline float length(f3 a) { return std::sqrt(dot(a, a)); }
+inline f3 normalize(f3 a) { float lenInv = float(1) / length(a); return a * lenInv; }
+// LiteMath.h:987-992 shuffle_yzx(a*b_yzx - a_yzx*b)
+inline f3 cross(f3 a, f3 b) {
+  const f3 a_yzx(a.y, a.z, a.x), b_yzx(b.y, b.z, b.x);
+  const f3 r = a * b_yzx - a_yzx * b;
+  return f3(r.y, r.z, r.x);
+}
+inline f3 to3(f4 v) { return f3(v.x, v.y, v.z); }
+// LiteMath.h:1813-1819, column-major B
+inline f4 mul(const float* B, f4 V) {
+  f4 r;
+  r.x = V.x * B[0] + V.y * B[4] + V.z * B[8] + V.w * B[12];
+  r.y = V.x * B[1] + V.y * B[5] + V.z * B[9] + V.w * B[13];
+  r.z = V.x * B[2] + V.y * B[6] + V.z * B[10] + V.w * B[14];
+  r.w = V.x * B[3] + V.y * B[7] + V.z * B[11] + V.w * B[15];
+  return r;
+}
+// raytrace_common.h:84-92
+inline f3 matmul3x3(const float* m, f3 v) { return to3(mul(m, f4{v.x, v.y, v.z, 0.0f})); }
+inline f3 matmul4x3(const float* m, f3 v) { return to3(mul(m, f4{v.x, v.y, v.z, 1.0f})); }
+
+// raytrace_common.h:59-67
+inline f3 SafeInverse(f3 d) {
+  const float ooeps = 1.0e-36f;
+  f3 res;
+  res.x = 1.0f / (std::abs(d.x) > ooeps ? d.x : std::abs(ooeps) * lm_sign(d.x));
+  res.y = 1.0f / (std::abs(d.y) > ooeps ? d.y : std::abs(ooeps) * lm_sign(d.y));
+  res.z = 1.0f / (std::abs(d.z) > ooeps ? d.z : std::abs(ooeps) * lm_sign(d.z));
+  return res;
+}
+// raytrace_common.h:69-82
+inline f2 RayBoxIntersection2(f3 o, f3 inv, f3 bmin, f3 bmax) {
+  const float lo = inv.x * (bmin.x - o.x);
+  const float hi = inv.x * (bmax.x - o.x);
+  const float lo1 = inv.y * (bmin.y - o.y);
+  const float hi1 = inv.y * (bmax.y - o.y);
+  const float lo2 = inv.z * (bmin.z - o.z);
+  const float hi2 = inv.z * (bmax.z - o.z);
+  const float tmin = std::max(std::min(lo, hi), std::min(lo1, hi1));
+  const float tmax = std::min(std::max(lo, hi), std::max(lo1, hi1));
+  return f2{std::max(tmin, std::min(lo2, hi2)), std::min(tmax, std::max(lo2, hi2))};
+}
+// intersection_common.h:96-101
+inline f2 encode_normal(f3 v) {
+  const float k = 1.0f / (std::abs(v.x) + std::abs(v.y) + std::abs(v.z));
+  const f2 p{v.x * k, v.y * k};
+  const f2 s{(p.x >= 0.0f) ? +1.0f : -1.0f, (p.y >= 0.0f) ? +1.0f : -1.0f};
+  return (v.z <= 0.0f) ? f2{(1.0f - std::abs(p.y)) * s.x, (1.0f - std::abs(p.x)) * s.y} : p;
+}
+
+// ---- reference layouts (CrossRT.h:26-33, cbvh.h:21-33, intersection_common.h:33-56,
+//      sdf_simple_shared.h:23-55, render_settings.h:116-136) ----
+struct Hit { float t; uint32_t primId, instId, geomId; float coords[4]; };
+struct Node { f3 boxMin; uint32_t leftOffset; f3 boxMax; uint32_t escapeIndex; };
+struct NodePair { Node left, right; };
+struct Geom { f4 boxMin, boxMax; uint32_t bvhOffset, type; uint32_t offset[2]; uint32_t hasChannels, pad[3]; };
+struct Inst { f4 boxMin, boxMax; uint32_t geomId, pad[7]; float transform[16], transformInv[16], transformInvTransposed[16]; };
+struct FrameNode { float values[8]; uint32_t offset; };
+struct SBSNode { uint32_t pos_xy, pos_z_lod_size, data_offset, pad; };
+struct SBSHeader { uint32_t brick_size, brick_pad, bytes_per_value, aux_data; };
+struct Preset {
+  uint32_t render_mode, sdf_node_intersect, normal_mode, ray_gen_mode, interpolation_mode, representation_mode, spp,
+      fixed_lod;
+  float level_of_detail, compact_sdf_eps, LOD_distance;
+  uint32_t channel_render_mode, channel_id, component_id, CTF_id;
+};
+static_assert(sizeof(Hit) == 32 && sizeof(Node) == 32 && sizeof(NodePair) == 64, "layout");
+static_assert(sizeof(Geom) == 64 && sizeof(Inst) == 256 && sizeof(FrameNode) == 36 && sizeof(Preset) == 60, "layout");
+
+constexpr uint32_t LEAF_BIT = 0x80000000u, START_MASK = 0x0FFFFFFFu, SIZE_MASK = 0x70000000u;
+constexpr int STACK_SIZE = 80;
+constexpr uint32_t TAG_NONE = 0, TAG_TRIANGLE = 1, TAG_SDF_BRICK = 4, TAG_SDF_NODE = 8;
+constexpr uint32_t TYPE_MESH_TRIANGLE = 0, TYPE_SDF_FRAME_OCTREE = 3, SH_TYPE = 24, GEOM_ID_MASK = 0x00FFFFFFu;
+constexpr uint32_t LAYOUT_DX_UV16 = 2u << 24, LAYOUT_DX_RGB8 = 3u << 24, LAYOUT_ID32F = 4u << 24, LAYOUT_ID32F_IN = 5u << 24;
+
+}  // namespace
+
+extern "C" {
+
+// Host arrays of a committed scene (the BVHRT members, BVH2Common.h:262-365).
+struct OracleScene {
+  const NodePair* nodes;
+  const Node* tlas;
+  const Inst* instances;
+  const Geom* geoms;
+  const uint32_t* geom_tags;
+  const uint32_t* start_end;     // uint2 per geometry
+  const uint32_t* prim_id_count;
+  const f4* vert_pos;
+  const f4* vert_norm;
+  const uint32_t* indices;
+  const uint32_t* prim_indices;
+  const Node* orig_nodes;
+  const FrameNode* frame_nodes;
+  const uint32_t* frame_roots;
+  const SBSNode* sbs_nodes;
+  const uint32_t* sbs_data;
+  const float* sbs_data_f;
+  const uint32_t* sbs_roots;
+  const SBSHeader* sbs_headers;
+  Preset preset;
+};
+
+// Per-ray traversal counters (the RayInfo idea, BVH2Common.cpp:3403-3571) for the
+// algorithmic-bytes model.
+struct OracleCounts {
+  uint32_t blas_nodes, leaves, prims, tlas_nodes, instances, sdf_voxels;
+};
+
+}  // extern "C"
+
+namespace {
+
+struct Tracer {
+  const OracleScene& s;
+  OracleCounts* cnt;
+
+  // BVH2Common.cpp:19-32
+  bool need_normal() const {
+    const uint32_t m = s.preset.render_mode;
+    return m == 1 || m == 8 || m == 12 || m == 17 || m == 20 || m == 7 || m == 18 || m == 23 || m == 21 || m == 11 ||
+           m == 27;
+  }
+
+  // BVH2Common.cpp:1571-1635
+  void IntersectAllTrianglesInLeaf(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
+                                   uint32_t a_start, uint32_t a_count, Hit* pHit) {
+    const uint32_t* go = s.geoms[geomId].offset;
+    for (uint32_t triId = a_start; triId < a_start + a_count; triId++) {
+      if (cnt) cnt->prims++;
+      const uint32_t A = s.indices[go[0] + triId * 3 + 0];
+      const uint32_t B = s.indices[go[0] + triId * 3 + 1];
+      const uint32_t C = s.indices[go[0] + triId * 3 + 2];
+      const f3 A_pos = to3(s.vert_pos[go[1] + A]);
+      const f3 B_pos = to3(s.vert_pos[go[1] + B]);
+      const f3 C_pos = to3(s.vert_pos[go[1] + C]);
+      const f3 edge1 = B_pos - A_pos;
+      const f3 edge2 = C_pos - A_pos;
+      const f3 pvec = cross(ray_dir, edge2);
+      const f3 tvec = ray_pos - A_pos;
+      const f3 qvec = cross(tvec, edge1);
+      const float invDet = 1.0f / dot(edge1, pvec);
+      const float v = dot(tvec, pvec) * invDet;
+      const float u = dot(qvec, ray_dir) * invDet;
+      const float t = dot(edge2, qvec) * invDet;
+      if (v >= -1e-6f && u >= -1e-6f && (u + v <= 1.0f + 1e-6f) && t > tNear && t < pHit->t) {
+        pHit->t = t;
+        pHit->primId = triId;
+        pHit->instId = instId;
+        pHit->geomId = geomId | (TYPE_MESH_TRIANGLE << SH_TYPE);
+#ifdef ORACLE_PLANT_SWAP_UV
+        // planted mistake, compiled only by the test that proves the checks catch it
+        pHit->coords[0] = v;
+        pHit->coords[1] = u;
+#else
+        pHit->coords[0] = u;
+        pHit->coords[1] = v;
+#endif
+        if (need_normal()) {
+          f3 n(1, 0, 0);
+          if (s.preset.normal_mode == 0) {
+            n = cross(edge1, edge2);
+          } else if (s.preset.normal_mode == 1) {
+            const f4 a = s.vert_norm[go[1] + A], b = s.vert_norm[go[1] + B], c = s.vert_norm[go[1] + C];
+            const float w = 1.0f - u - v;
+            // to_float3(nA*(1-u-v) + nB*v + u*nC)
+            n = f3(a.x * w + b.x * v + u * c.x, a.y * w + b.y * v + u * c.y, a.z * w + b.z * v + u * c.z);
+          }
+          n = normalize(matmul4x3(s.instances[instId].transformInvTransposed, n));
+          const f2 e = encode_normal(n);
+          pHit->coords[2] = e.x;
+          pHit->coords[3] = e.y;
+        } else {
+          pHit->coords[2] = 0;
+          pHit->coords[3] = 0;
+        }
+      }
+    }
+  }
+
+  // BVH2Common.cpp:126-136
+  static float eval_dist_trilinear(const float v[8], f3 dp) {
+    return (1 - dp.x) * (1 - dp.y) * (1 - dp.z) * v[0] + (1 - dp.x) * (1 - dp.y) * (dp.z) * v[1] +
+           (1 - dp.x) * (dp.y) * (1 - dp.z) * v[2] + (1 - dp.x) * (dp.y) * (dp.z) * v[3] +
+           (dp.x) * (1 - dp.y) * (1 - dp.z) * v[4] + (dp.x) * (1 - dp.y) * (dp.z) * v[5] +
+           (dp.x) * (dp.y) * (1 - dp.z) * v[6] + (dp.x) * (dp.y) * (dp.z) * v[7];
+  }
+
+  // BVH2Common.cpp:345-725 (non-compact SDF types, no cutting plane, no tex types)
+  void LocalSurfaceIntersection(uint32_t type, f3 ray_dir, uint32_t instId, uint32_t geomId, const float values[8],
+                                uint32_t primId, float d, float qNear, float qFar, f2 fNearFar, f3 start_q, Hit* pHit) {
+    const float EPS = 1e-6f;
+    const float d_inv = 1.0f / d;
+    float t = qNear;
+    bool early_hit = false, hit = false;
+    unsigned iter = 0;
+    float start_sign = 1;
+    float start_dist = eval_dist_trilinear(values, start_q + t * ray_dir);
+    if (s.preset.representation_mode == 0) start_sign = lm_sign(start_dist);
+    start_dist *= start_sign;
+    const uint32_t mode = s.preset.sdf_node_intersect;
+    if (start_dist <= EPS || mode == 3) {
+      early_hit = true;
+      hit = true;
+    } else if (mode == 0) {
+      float dist = start_dist;
+      while (t < qFar && dist > EPS && iter < 256u) {
+        t += dist * d_inv;
+        dist = start_sign * eval_dist_trilinear(values, start_q + t * ray_dir);
+        iter++;
+      }
+      hit = (dist <= EPS);
+    } else {
+      float s000 = values[0] * d_inv, s001 = values[1] * d_inv, s010 = values[2] * d_inv, s011 = values[3] * d_inv;
+      float s100 = values[4] * d_inv, s101 = values[5] * d_inv, s110 = values[6] * d_inv, s111 = values[7] * d_inv;
+      float tmp = s101 - s001;
+      float k0 = s000, k1 = s100 - s000, k2 = s010 - s000, k3 = s110 - s010 - k1, k4 = k0 - s001, k5 = k1 - tmp;
+      float k6 = k2 - (s011 - s001), k7 = k3 - (s111 - s011 - tmp);
+      f3 o = start_q, d3 = ray_dir;
+      float m0 = o.x * o.y, m1 = d3.x * d3.y, m2 = o.x * d3.y + o.y * d3.x;
+      float m3 = k5 * o.z - k1, m4 = k6 * o.z - k2, m5 = k7 * o.z - k3;
+      float c0 = (k4 * o.z - k0) + o.x * m3 + o.y * m4 + m0 * m5;
+      float c1 = d3.x * m3 + d3.y * m4 + m2 * m5 + d3.z * (k4 + k5 * o.x + k6 * o.y + k7 * m0);
+      float c2 = m1 * m5 + d3.z * (k5 * d3.x + k6 * d3.y + k7 * m2);
+      float c3 = k7 * m1 * d3.z;
+      if (mode == 1) {
+        float x1 = 1000, x2 = 1000, x3 = 1000;
+        const float PI = 3.14159265358979323846f;
+        if (std::abs(c3) > 1e-5f) {
+          float a = c2 / c3, b = c1 / c3, c = c0 / c3;
+          float Q = (a * a - 3 * b) / 9;
+          float R = (2 * a * a * a - 9 * a * b + 27 * c) / 54;
+          float Q3 = Q * Q * Q;
+          if (R * R < Q3) {
+            float theta = std::acos(R / std::sqrt(Q3));
+            x1 = -2 * std::sqrt(Q) * std::cos(theta / 3) - a / 3;
+            x2 = -2 * std::sqrt(Q) * std::cos((theta + 2 * PI) / 3) - a / 3;
+            x3 = -2 * std::sqrt(Q) * std::cos((theta - 2 * PI) / 3) - a / 3;
+          } else {
+            float A = -lm_sign(R) * std::pow(std::abs(R) + std::sqrt(R * R - Q3), 1.0f / 3.0f);
+            float B = std::abs(A) > EPS ? Q / A : 0;
+            x1 = A + B - a / 3;
+          }
+        } else if (std::abs(c2) > 1e-7f) {
+          float a = c2, b = c1, c = c0;
+          float D = b * b - 4 * a * c;
+          if (D > 0) {
+            float q = -0.5f * (b + lm_sign(b) * std::sqrt(D));
+            x1 = q / a;
+            if (std::abs(q) > EPS) x2 = c / q;
+          }
+        } else if (std::abs(c1) > EPS) {
+          x1 = -c0 / c1;
+        }
+        x1 = x1 < 0 ? 1000 : x1;
+        x2 = x2 < 0 ? 1000 : x2;
+        x3 = x3 < 0 ? 1000 : x3;
+        t = std::min(x1, std::min(x2, x3));
+        hit = (t >= 0 && t <= qFar);
+      } else if (mode == 2) {
+        float a = 3 * c3, b = 2 * c2, c = c1;
+        float t0 = 0, t1 = qFar, t2 = qFar, t3 = qFar;
+        float D = b * b - 4 * a * c;
+        if (D >= 0) {
+          float q = -0.5f * (b + lm_sign(b) * std::sqrt(D));
+          t1 = std::abs(a) > EPS ? q / a : t0;
+          t2 = std::abs(q) > EPS ? c / q : qFar;
+          float min_t = std::min(t1, t2);
+          t2 = std::max(t1, t2);
+          t1 = min_t;
+          t1 = std::min(std::max(t0, t1), t3);
+          t2 = std::min(std::max(t0, t2), t3);
+        }
+        bool s0 = c0 > 0, s1 = (c0 + t1 * (c1 + t1 * (c2 + t1 * c3))) > 0;
+        bool s2 = (c0 + t2 * (c1 + t2 * (c2 + t2 * c3))) > 0, s3 = (c0 + t3 * (c1 + t3 * (c2 + t3 * c3))) > 0;
+        float nmin = t0, nmax = t0;
+        if (s0 != s1) { nmin = t0; nmax = t1; }
+        else if (s1 != s2) { nmin = t1; nmax = t2; }
+        else if (s2 != s3) { nmin = t2; nmax = t3; }
+        if (nmin < nmax) {
+          unsigned it = 0;
+          float rtn = 0.5f * (nmin + nmax), f = 1000;
+          while (it < 10u && std::abs(f) >= EPS) {
+            f = c0 + rtn * (c1 + rtn * (c2 + rtn * c3));
+            float df = c1 + rtn * (2 * c2 + rtn * 3 * c3);
+            float dx = f / (df + lm_sign(df) * 1e-9f);
+            rtn -= dx;
+            it++;
+          }
+          t = rtn;
+          hit = (t >= 0 && t <= qFar && std::abs(f) < EPS);
+        } else {
+          hit = false;
+        }
+      } else {
+        const float k = 2;
+        float e = 0.1f * qFar;
+        float t_max = std::abs(c3) < EPS ? 1e6f : -c2 / (3 * c3);
+        float df_max = 3 * c3 * t_max * t_max + 2 * c2 * t_max + c1;
+        float dist = start_dist;
+        while (t < qFar && dist > EPS && iter < 256u) {
+          float df_1 = 3 * c3 * t * t + 2 * c2 * t + c1;
+          float df_2 = 3 * c3 * (t + e) * (t + e) + 2 * c2 * (t + e) + c1;
+          float L = (t_max > t && t_max < t + e) ? std::max(df_max, std::max(df_1, df_2)) : std::max(df_1, df_2);
+          L = std::max(L, EPS);
+          float st = std::min((dist * d_inv) / L, e);
+          t += st;
+          e = k * st;
+          dist = start_sign * eval_dist_trilinear(values, start_q + t * ray_dir);
+          iter++;
+        }
+        hit = (dist <= EPS);
+      }
+    }
+    const float tReal = fNearFar.x + d * t;
+    if (t <= qFar && hit && tReal < pHit->t) {
+      f3 norm(0, 0, 1);
+      if (need_normal()) {
+        if (early_hit && mode == 3) {
+          norm = f3(0, 0, 0);
+          f3 v(std::max(start_q.x, 1 - start_q.x), std::max(start_q.y, 1 - start_q.y), std::max(start_q.z, 1 - start_q.z));
+          int dim = 0;
+          if (v.y > v.x && v.y > v.z) dim = 1;
+          else if (v.z > v.x && v.z > v.y) dim = 2;
+          const float q = dim == 0 ? start_q.x : (dim == 1 ? start_q.y : start_q.z);
+          const float nv = q < 0.5f ? -1.0f : 1.0f;
+          if (dim == 0) norm.x = nv; else if (dim == 1) norm.y = nv; else norm.z = nv;
+        } else {
+          f3 p0 = start_q + t * ray_dir;
+          const float h = 0.001f;
+          float ddx = (eval_dist_trilinear(values, p0 + f3(h, 0, 0)) - eval_dist_trilinear(values, p0 + f3(-h, 0, 0))) / (2 * h);
+          float ddy = (eval_dist_trilinear(values, p0 + f3(0, h, 0)) - eval_dist_trilinear(values, p0 + f3(0, -h, 0))) / (2 * h);
+          float ddz = (eval_dist_trilinear(values, p0 + f3(0, 0, h)) - eval_dist_trilinear(values, p0 + f3(0, 0, -h))) / (2 * h);
+          norm = start_sign * f3(ddx, ddy, ddz);
+        }
+        norm = normalize(matmul4x3(s.instances[instId].transformInvTransposed, norm));
+      }
+      f2 en = encode_normal(norm);
+      pHit->t = tReal;
+      pHit->primId = primId;
+      pHit->instId = instId;
+      pHit->geomId = geomId | (type << SH_TYPE);
+      pHit->coords[0] = 0;
+      pHit->coords[1] = 0;
+      pHit->coords[2] = en.x;
+      pHit->coords[3] = en.y;
+      if (s.preset.render_mode == 10) pHit->primId = iter;
+    }
+  }
+
+  // BVH2Common.cpp:727-749 + 141-167. Leaf boxes of geometry g start at geoms[g].offset[1].
+  void OctreeNodeIntersect(uint32_t type, f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
+                           uint32_t a_start, Hit* pHit) {
+    if (type != TYPE_SDF_FRAME_OCTREE) return;
+    const uint32_t sdfId = s.geoms[geomId].offset[0];
+    const Node& on = s.orig_nodes[s.geoms[geomId].offset[1] + a_start];
+    const uint32_t primId = on.leftOffset;
+    const uint32_t nodeId = primId + s.frame_roots[sdfId];
+    const f3 min_pos = on.boxMin, max_pos = on.boxMax, size = max_pos - min_pos;
+    float values[8];
+    for (int i = 0; i < 8; i++) values[i] = s.frame_nodes[nodeId].values[i];
+    const f2 fNearFar = RayBoxIntersection2(ray_pos, SafeInverse(ray_dir), min_pos, max_pos);
+    const f3 start_pos = ray_pos + fNearFar.x * ray_dir;
+    const float d = std::max(size.x, std::max(size.y, size.z));
+    const f3 start_q = (start_pos - min_pos) / d;
+    const float qFar = (fNearFar.y - fNearFar.x) / d;
+    const float qNear = tNear > fNearFar.x ? (tNear - fNearFar.x) / d : 0.0f;
+    if (qNear > 0.0f) return;
+    LocalSurfaceIntersection(type, ray_dir, instId, geomId, values, primId, d, qNear, qFar, fNearFar, start_q, pHit);
+  }
+
+  // BVH2Common.cpp:846-882
+  float load_distance_values(uint32_t nodeId, f3 voxelPos, uint32_t v_size, float sz_inv, const SBSHeader& h,
+                             float values[8]) {
+    float vmin = 1e6f;
+    const uint32_t v_off = s.sbs_nodes[nodeId].data_offset;
+    for (int i = 0; i < 8; i++) {
+      const uint32_t x = uint32_t(voxelPos.x) + ((i & 4) >> 2), y = uint32_t(voxelPos.y) + ((i & 2) >> 1),
+                     z = uint32_t(voxelPos.z) + (i & 1);
+      // SBS_v_to_i (sdf_common_shared.h:49-52)
+      const uint32_t vId = (x + h.brick_pad) * v_size * v_size + (y + h.brick_pad) * v_size + (z + h.brick_pad);
+      if (h.aux_data == LAYOUT_ID32F || h.aux_data == LAYOUT_ID32F_IN) {
+        values[i] = s.sbs_data_f[s.sbs_data[v_off + vId]];
+      } else {
+        const uint32_t vpi = 4 / h.bytes_per_value, bits = 8 * h.bytes_per_value;
+        const uint32_t max_val = h.bytes_per_value == 4 ? 0xFFFFFFFFu : ((1u << bits) - 1);
+        const float d_max = 1.73205081f * sz_inv;
+        const float mult = 2 * d_max / max_val;
+        const uint32_t sh = bits * (vId % vpi);
+        values[i] = -d_max + mult * ((sh < 32 ? s.sbs_data[v_off + vId / vpi] >> sh : 0u) & max_val);
+      }
+      vmin = std::min(vmin, values[i]);
+    }
+    return vmin;
+  }
+
+  // BVH2Common.cpp:884-1112 (trilinear)
+  void OctreeBrickIntersect(uint32_t type, f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
+                            uint32_t bvhNodeId, Hit* pHit) {
+    const uint32_t sdfId = s.geoms[geomId].offset[0];
+    const uint32_t primId = bvhNodeId;
+    const uint32_t nodeId = primId + s.sbs_roots[sdfId];
+    const SBSHeader header = s.sbs_headers[sdfId];
+    const uint32_t v_size = header.brick_size + 2 * header.brick_pad + 1;
+    const SBSNode& nd = s.sbs_nodes[nodeId];
+    float px = nd.pos_xy >> 16, py = nd.pos_xy & 0x0000FFFF, pz = nd.pos_z_lod_size >> 16,
+          sz = nd.pos_z_lod_size & 0x0000FFFF;
+    float sz_inv = 2.0f / sz;
+    float d = 2.0f / (sz * header.brick_size);
+    f3 brick_min_pos = f3(-1, -1, -1) + sz_inv * f3(px, py, pz);
+    f3 brick_max_pos = brick_min_pos + sz_inv * f3(1, 1, 1);
+    f2 brick_fNearFar = RayBoxIntersection2(ray_pos, SafeInverse(ray_dir), brick_min_pos, brick_max_pos);
+    const float old_t = pHit->t;
+    // step cap: a ray crosses at most 3*brick_size-2 voxels; only rounding-stalled marches hit it
+    uint32_t steps = 0;
+    const uint32_t max_steps = 4 * header.brick_size + 8;
+    while (brick_fNearFar.x < brick_fNearFar.y && pHit->t == old_t && steps < max_steps) {
+      steps++;
+      if (cnt) cnt->sdf_voxels++;
+      f3 hit_pos = ray_pos + brick_fNearFar.x * ray_dir;
+      f3 local_pos = (hit_pos - brick_min_pos) * (0.5f * sz * header.brick_size);
+      const float hi = header.brick_size - 1e-6f;
+      f3 voxelPos(std::floor(std::min(std::max(1e-6f, local_pos.x), hi)), std::floor(std::min(std::max(1e-6f, local_pos.y), hi)),
+                  std::floor(std::min(std::max(1e-6f, local_pos.z), hi)));
+      f3 min_pos = brick_min_pos + d * voxelPos;
+      f3 max_pos = min_pos + d * f3(1, 1, 1);
+      float values[8];
+      float vmin = load_distance_values(nodeId, voxelPos, v_size, sz_inv, header, values);
+      f2 fNearFar = RayBoxIntersection2(ray_pos, SafeInverse(ray_dir), min_pos, max_pos);
+      if (tNear < fNearFar.x && vmin <= 0.0f) {
+        f3 start_pos = ray_pos + fNearFar.x * ray_dir;
+        f3 start_q = (start_pos - min_pos) * (0.5f * sz * header.brick_size);
+        float qFar = (fNearFar.y - fNearFar.x) * (0.5f * sz * header.brick_size);
+        LocalSurfaceIntersection(type, ray_dir, instId, geomId, values, primId, d, 0.0f, qFar, fNearFar, start_q, pHit);
+      }
+      brick_fNearFar.x += std::max(0.0f, fNearFar.y - brick_fNearFar.x) + 1e-6f;
+    }
+    if (pHit->t < old_t) {
+      f3 pos = ray_pos + pHit->t * ray_dir;
+      f3 dp = (pos - brick_min_pos) * (0.5f * sz);
+      const float w[8] = {(1 - dp.x) * (1 - dp.y) * (1 - dp.z), (1 - dp.x) * (1 - dp.y) * (dp.z),
+                          (1 - dp.x) * (dp.y) * (1 - dp.z),     (1 - dp.x) * (dp.y) * (dp.z),
+                          (dp.x) * (1 - dp.y) * (1 - dp.z),     (dp.x) * (1 - dp.y) * (dp.z),
+                          (dp.x) * (dp.y) * (1 - dp.z),         (dp.x) * (dp.y) * (dp.z)};
+      if (header.aux_data == LAYOUT_DX_UV16) {
+        const uint32_t vpi = 4 / header.bytes_per_value;
+        const uint32_t t_off = nd.data_offset + (v_size * v_size * v_size + vpi - 1) / vpi;
+        float u = w[0] * (1.5259022e-5f * float(s.sbs_data[t_off] >> 16));
+        float v = w[0] * (1.5259022e-5f * float(s.sbs_data[t_off] & 0xFFFF));
+        for (int i = 1; i < 8; ++i) {
+          u = u + w[i] * (1.5259022e-5f * float(s.sbs_data[t_off + i] >> 16));
+          v = v + w[i] * (1.5259022e-5f * float(s.sbs_data[t_off + i] & 0xFFFF));
+        }
+        pHit->coords[0] = u;
+        pHit->coords[1] = v;
+      } else if (header.aux_data == LAYOUT_DX_RGB8 || header.aux_data == LAYOUT_ID32F || header.aux_data == LAYOUT_ID32F_IN) {
+        const bool rgb8 = header.aux_data == LAYOUT_DX_RGB8;
+        const uint32_t vpi = 4 / header.bytes_per_value;
+        const uint32_t t_off = rgb8 ? nd.data_offset + (v_size * v_size * v_size + vpi - 1) / vpi
+                                    : nd.data_offset + v_size * v_size * v_size;
+        f3 color;
+        for (int i = 0; i < 8; ++i) {
+          const uint32_t q = s.sbs_data[t_off + i];
+          const f3 c = rgb8 ? f3(float((q >> 0) & 0xFF), float((q >> 8) & 0xFF), float((q >> 16) & 0xFF))
+                            : f3(s.sbs_data_f[q + 0], s.sbs_data_f[q + 1], s.sbs_data_f[q + 2]);
+          color = (i == 0) ? w[0] * c : color + w[i] * c;
+        }
+        const float k = rgb8 ? 1.0f : 255.0f;
+        color = f3(std::min(std::max(0.0f, std::floor(k * color.x + 0.5f)), 255.0f),
+                   std::min(std::max(0.0f, std::floor(k * color.y + 0.5f)), 255.0f),
+                   std::min(std::max(0.0f, std::floor(k * color.z + 0.5f)), 255.0f));
+        pHit->coords[0] = color.x + color.y / 256.0f;
+        pHit->coords[1] = color.z;
+      }
+    }
+  }
+
+  // BVH2Common.cpp:47-124
+  uint32_t IntersectAllPrimitivesInLeaf(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
+                                        uint32_t aabbId, Hit* pHit) {
+    const float tPrev = pHit->t;
+    const uint32_t globalAABBId = s.start_end[2 * geomId] + aabbId;
+    const uint32_t packed = s.prim_id_count[globalAABBId];
+    const uint32_t a_start = packed & START_MASK, a_count = (packed & SIZE_MASK) >> 28;
+    const uint32_t geomTag = s.geom_tags[geomId];
+    if (cnt) cnt->leaves++;
+    switch (geomTag) {
+      case TAG_TRIANGLE: IntersectAllTrianglesInLeaf(ray_pos, ray_dir, tNear, instId, geomId, a_start, a_count, pHit); break;
+      case TAG_SDF_NODE: OctreeNodeIntersect(s.geoms[geomId].type, ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
+      case TAG_SDF_BRICK: OctreeBrickIntersect(s.geoms[geomId].type, ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
+      default: break;
+    }
+    if (s.preset.render_mode == 26) pHit->instId = (pHit->instId == 0xFFFFFFFF) ? 1 : pHit->instId + 1;
+    return (pHit->t >= tPrev) ? TAG_NONE : geomTag;
+  }
+
+  // BVH2Common.cpp:3239-3320. Pushes beyond STACK_SIZE are dropped (the reference writes past
+  // its array); *overflow reports it.
+  void BVH2TraverseF32(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId, bool stopOnFirstHit,
+                       Hit* pHit, bool* overflow) {
+    const uint32_t bvhOffset = s.geoms[geomId].bvhOffset;
+    uint32_t stack[STACK_SIZE];
+    int top = 0;
+    uint32_t leftNodeOffset = 0;
+    bool hitFound = false;
+    const f3 rayDirInv = SafeInverse(ray_dir);
+    while (top >= 0 && !hitFound) {
+      while (top >= 0 && ((leftNodeOffset & LEAF_BIT) == 0)) {
+        if (cnt) cnt->blas_nodes++;
+        const NodePair fatNode = s.nodes[bvhOffset + leftNodeOffset];
+        const uint32_t node0_leftOffset = fatNode.left.leftOffset;
+        const uint32_t node1_leftOffset = fatNode.right.leftOffset;
+        const f2 tm0 = RayBoxIntersection2(ray_pos, rayDirInv, fatNode.left.boxMin, fatNode.left.boxMax);
+        const f2 tm1 = RayBoxIntersection2(ray_pos, rayDirInv, fatNode.right.boxMin, fatNode.right.boxMax);
+        const bool hitChild0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= pHit->t);
+        const bool hitChild1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= pHit->t);
+        leftNodeOffset = hitChild0 ? node0_leftOffset : node1_leftOffset;
+        if (hitChild0 && hitChild1) {
+          leftNodeOffset = (tm0.x <= tm1.x) ? node0_leftOffset : node1_leftOffset;
+          if (top < STACK_SIZE) {
+            stack[top] = (tm0.x <= tm1.x) ? node1_leftOffset : node0_leftOffset;
+            top++;
+          } else if (overflow) {
+            *overflow = true;
+          }
+        }
+        if (!hitChild0 && !hitChild1) {
+          top--;
+          leftNodeOffset = stack[std::max(top, 0)];
+        }
+      }
+      if (top >= 0 && leftNodeOffset != 0xFFFFFFFF) {
+        const float SDF_BIAS = 0.01f;
+        const float tNearSdf = std::max(tNear, SDF_BIAS);
+        const uint32_t hitTag =
+            IntersectAllPrimitivesInLeaf(ray_pos, ray_dir, tNearSdf, instId, geomId, leftNodeOffset & START_MASK, pHit);
+        hitFound = (hitTag != 0) && (hitTag != 1 || stopOnFirstHit);
+      }
+      top--;
+      leftNodeOffset = stack[std::max(top, 0)];
+    }
+  }
+
+  // BVH2Common.cpp:3322-3392
+  Hit RayQuery_NearestHit(f4 posAndNear, f4 dirAndFar, bool* overflow) {
+    bool stopOnFirstHit = (dirAndFar.w <= 0.0f);
+    if (stopOnFirstHit) dirAndFar.w *= -1.0f;
+    const f3 rayDirInv = SafeInverse(to3(dirAndFar));
+    Hit hit;
+    hit.t = dirAndFar.w;
+    hit.primId = hit.instId = hit.geomId = uint32_t(-1);
+    hit.coords[0] = 1.0f;
+    hit.coords[1] = hit.coords[2] = hit.coords[3] = 0.0f;
+    uint32_t nodeIdx = 0;
+    do {
+      uint32_t travFlags = 0, leftOffset = 0;
+      do {
+        if (cnt) cnt->tlas_nodes++;
+        const Node currNode = s.tlas[nodeIdx];
+        const f2 boxHit = RayBoxIntersection2(to3(posAndNear), rayDirInv, currNode.boxMin, currNode.boxMax);
+        const bool intersects = (boxHit.x <= boxHit.y) && (boxHit.y > posAndNear.w) && (boxHit.x < hit.t);
+        travFlags = (currNode.leftOffset & LEAF_BIT) | uint32_t(intersects);
+        leftOffset = currNode.leftOffset;
+        const bool leafOrMiss = (travFlags & LEAF_BIT) != 0 || (travFlags & 0x1) == 0;
+        nodeIdx = leafOrMiss ? currNode.escapeIndex : leftOffset;
+      } while (travFlags != (LEAF_BIT | 0x1) && nodeIdx != 0 && nodeIdx < 0xFFFFFFFE);
+      if (travFlags == (LEAF_BIT | 0x1)) {
+        if (cnt) cnt->instances++;
+        const uint32_t instId = leftOffset & START_MASK;
+        const uint32_t geomId = s.instances[instId].geomId;
+        const f3 ray_pos = matmul4x3(s.instances[instId].transformInv, to3(posAndNear));
+        const f3 ray_dir = matmul3x3(s.instances[instId].transformInv, to3(dirAndFar));
+        BVH2TraverseF32(ray_pos, ray_dir, posAndNear.w, instId, geomId, stopOnFirstHit, &hit, overflow);
+      }
+    } while (nodeIdx < 0xFFFFFFFE && !(stopOnFirstHit && hit.primId != uint32_t(-1)));
+    if (hit.geomId < uint32_t(-1) && ((hit.geomId >> SH_TYPE) == TYPE_MESH_TRIANGLE)) {
+      const uint32_t* go = s.geoms[hit.geomId & GEOM_ID_MASK].offset;
+      hit.primId = s.prim_indices[go[0] / 3 + hit.primId];
+    }
+    return hit;
+  }
+};
+
+// LiteMath.h:2070-2154
+void inverse4x4(const float* a, float* out) {
+  auto m1 = [&](int r, int c) { return a[c * 4 + r]; };
+  float m[16];
+  auto M = [&](int r, int c) -> float& { return m[c * 4 + r]; };
+  float tmp[12];
+  tmp[0] = m1(2, 2) * m1(3, 3); tmp[1] = m1(2, 3) * m1(3, 2); tmp[2] = m1(2, 1) * m1(3, 3);
+  tmp[3] = m1(2, 3) * m1(3, 1); tmp[4] = m1(2, 1) * m1(3, 2); tmp[5] = m1(2, 2) * m1(3, 1);
+  tmp[6] = m1(2, 0) * m1(3, 3); tmp[7] = m1(2, 3) * m1(3, 0); tmp[8] = m1(2, 0) * m1(3, 2);
+  tmp[9] = m1(2, 2) * m1(3, 0); tmp[10] = m1(2, 0) * m1(3, 1); tmp[11] = m1(2, 1) * m1(3, 0);
+  M(0, 0) = tmp[0] * m1(1, 1) + tmp[3] * m1(1, 2) + tmp[4] * m1(1, 3);
+  M(0, 0) -= tmp[1] * m1(1, 1) + tmp[2] * m1(1, 2) + tmp[5] * m1(1, 3);
+  M(1, 0) = tmp[1] * m1(1, 0) + tmp[6] * m1(1, 2) + tmp[9] * m1(1, 3);
+  M(1, 0) -= tmp[0] * m1(1, 0) + tmp[7] * m1(1, 2) + tmp[8] * m1(1, 3);
+  M(2, 0) = tmp[2] * m1(1, 0) + tmp[7] * m1(1, 1) + tmp[10] * m1(1, 3);
+  M(2, 0) -= tmp[3] * m1(1, 0) + tmp[6] * m1(1, 1) + tmp[11] * m1(1, 3);
+  M(3, 0) = tmp[5] * m1(1, 0) + tmp[8] * m1(1, 1) + tmp[11] * m1(1, 2);
+  M(3, 0) -= tmp[4] * m1(1, 0) + tmp[9] * m1(1, 1) + tmp[10] * m1(1, 2);
+  M(0, 1) = tmp[1] * m1(0, 1) + tmp[2] * m1(0, 2) + tmp[5] * m1(0, 3);
+  M(0, 1) -= tmp[0] * m1(0, 1) + tmp[3] * m1(0, 2) + tmp[4] * m1(0, 3);
+  M(1, 1) = tmp[0] * m1(0, 0) + tmp[7] * m1(0, 2) + tmp[8] * m1(0, 3);
+  M(1, 1) -= tmp[1] * m1(0, 0) + tmp[6] * m1(0, 2) + tmp[9] * m1(0, 3);
+  M(2, 1) = tmp[3] * m1(0, 0) + tmp[6] * m1(0, 1) + tmp[11] * m1(0, 3);
+  M(2, 1) -= tmp[2] * m1(0, 0) + tmp[7] * m1(0, 1) + tmp[10] * m1(0, 3);
+  M(3, 1) = tmp[4] * m1(0, 0) + tmp[9] * m1(0, 1) + tmp[10] * m1(0, 2);
+  M(3, 1) -= tmp[5] * m1(0, 0) + tmp[8] * m1(0, 1) + tmp[11] * m1(0, 2);
+  tmp[0] = m1(0, 2) * m1(1, 3); tmp[1] = m1(0, 3) * m1(1, 2); tmp[2] = m1(0, 1) * m1(1, 3);
+  tmp[3] = m1(0, 3) * m1(1, 1); tmp[4] = m1(0, 1) * m1(1, 2); tmp[5] = m1(0, 2) * m1(1, 1);
+  tmp[6] = m1(0, 0) * m1(1, 3); tmp[7] = m1(0, 3) * m1(1, 0); tmp[8] = m1(0, 0) * m1(1, 2);
+  tmp[9] = m1(0, 2) * m1(1, 0); tmp[10] = m1(0, 0) * m1(1, 1); tmp[11] = m1(0, 1) * m1(1, 0);
+  M(0, 2) = tmp[0] * m1(3, 1) + tmp[3] * m1(3, 2) + tmp[4] * m1(3, 3);
+  M(0, 2) -= tmp[1] * m1(3, 1) + tmp[2] * m1(3, 2) + tmp[5] * m1(3, 3);
+  M(1, 2) = tmp[1] * m1(3, 0) + tmp[6] * m1(3, 2) + tmp[9] * m1(3, 3);
+  M(1, 2) -= tmp[0] * m1(3, 0) + tmp[7] * m1(3, 2) + tmp[8] * m1(3, 3);
+  M(2, 2) = tmp[2] * m1(3, 0) + tmp[7] * m1(3, 1) + tmp[10] * m1(3, 3);
+  M(2, 2) -= tmp[3] * m1(3, 0) + tmp[6] * m1(3, 1) + tmp[11] * m1(3, 3);
+  M(3, 2) = tmp[5] * m1(3, 0) + tmp[8] * m1(3, 1) + tmp[11] * m1(3, 2);
+  M(3, 2) -= tmp[4] * m1(3, 0) + tmp[9] * m1(3, 1) + tmp[10] * m1(3, 2);
+  M(0, 3) = tmp[2] * m1(2, 2) + tmp[5] * m1(2, 3) + tmp[1] * m1(2, 1);
+  M(0, 3) -= tmp[4] * m1(2, 3) + tmp[0] * m1(2, 1) + tmp[3] * m1(2, 2);
+  M(1, 3) = tmp[8] * m1(2, 3) + tmp[0] * m1(2, 0) + tmp[7] * m1(2, 2);
+  M(1, 3) -= tmp[6] * m1(2, 2) + tmp[9] * m1(2, 3) + tmp[1] * m1(2, 0);
+  M(2, 3) = tmp[6] * m1(2, 1) + tmp[11] * m1(2, 3) + tmp[3] * m1(2, 0);
+  M(2, 3) -= tmp[10] * m1(2, 3) + tmp[2] * m1(2, 0) + tmp[7] * m1(2, 1);
+  M(3, 3) = tmp[10] * m1(2, 2) + tmp[4] * m1(2, 0) + tmp[9] * m1(2, 1);
+  M(3, 3) -= tmp[8] * m1(2, 1) + tmp[11] * m1(2, 2) + tmp[5] * m1(2, 0);
+  const float k = 1.0f / (m1(0, 0) * M(0, 0) + m1(0, 1) * M(1, 0) + m1(0, 2) * M(2, 0) + m1(0, 3) * M(3, 0));
+  for (int i = 0; i < 16; ++i) out[i] = m[i] * k;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_flavor(void) {
+#ifdef ORACLE_FMA
+  return 1;
+#else
+  return 0;
+#endif
+}
+
+// Batched BVHRT::RayQuery_NearestHit over host arrays; OpenMP over rays
+// (schedule(dynamic,256), as the baseline plan asks). Returns the number of rays whose stack
+// reached the 80-entry limit. counts may be NULL.
+uint64_t oracle_trace_closest(const OracleScene* sc, const float* pos, const float* dir, void* hits, uint64_t n,
+                              int threads, OracleCounts* counts) {
+  uint64_t overflow = 0;
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads) reduction(+ : overflow)
+  for (long i = 0; i < long(n); ++i) {
+    OracleCounts c{};
+    Tracer tr{*sc, counts ? &c : nullptr};
+    bool ov = false;
+    const f4 p{pos[4 * i], pos[4 * i + 1], pos[4 * i + 2], pos[4 * i + 3]};
+    const f4 d{dir[4 * i], dir[4 * i + 1], dir[4 * i + 2], dir[4 * i + 3]};
+    static_cast<Hit*>(hits)[i] = tr.RayQuery_NearestHit(p, d, &ov);
+    if (counts) counts[i] = c;
+    overflow += ov ? 1 : 0;
+  }
+  return overflow;
+}
+
+// BVHRT::RayQuery_AnyHit (BVH2Common.cpp:3394-3399)
+void oracle_trace_any(const OracleScene* sc, const float* pos, const float* dir, uint32_t* flags, uint64_t n,
+                      int threads) {
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+  for (long i = 0; i < long(n); ++i) {
+    Tracer tr{*sc, nullptr};
+    const f4 p{pos[4 * i], pos[4 * i + 1], pos[4 * i + 2], pos[4 * i + 3]};
+    f4 d{dir[4 * i], dir[4 * i + 1], dir[4 * i + 2], dir[4 * i + 3]};
+    d.w *= -1.0f;
+    const Hit h = tr.RayQuery_NearestHit(p, d, nullptr);
+    flags[i] = h.geomId != uint32_t(-1) ? 1u : 0u;
+  }
+}
+
+// Closest hit over every triangle of one mesh with the reference's triangle test
+// (BVH2Common.cpp:1589-1600), no acceleration structure: the topology-independent answer.
+// verts: float4 per vertex, tris: original (not reordered) index triples. tnear as in the
+// BLAS leaf call (max(ray near, 0.01), BVH2Common.cpp:3306-3307). Writes t and the original
+// triangle id (0xFFFFFFFF on miss) and the second-closest t (for the tie band).
+void oracle_brute_force_tris(const float* verts, const uint32_t* tris, uint64_t n_tris, const float* pos,
+                             const float* dir, uint64_t n, float* t_out, uint32_t* id_out, float* t2_out, int threads) {
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+  for (long r = 0; r < long(n); ++r) {
+    const f3 o(pos[4 * r], pos[4 * r + 1], pos[4 * r + 2]);
+    const f3 d(dir[4 * r], dir[4 * r + 1], dir[4 * r + 2]);
+    const float tNear = std::max(pos[4 * r + 3], 0.01f);
+    float best = std::abs(dir[4 * r + 3]), second = best;
+    uint32_t id = 0xFFFFFFFFu;
+    for (uint64_t k = 0; k < n_tris; ++k) {
+      const float* A = verts + 4 * tris[3 * k];
+      const float* B = verts + 4 * tris[3 * k + 1];
+      const float* C = verts + 4 * tris[3 * k + 2];
+      const f3 a(A[0], A[1], A[2]);
+      const f3 e1 = f3(B[0], B[1], B[2]) - a, e2 = f3(C[0], C[1], C[2]) - a;
+      const f3 pvec = cross(d, e2), tvec = o - a, qvec = cross(tvec, e1);
+      const float invDet = 1.0f / dot(e1, pvec);
+      const float v = dot(tvec, pvec) * invDet, u = dot(qvec, d) * invDet, t = dot(e2, qvec) * invDet;
+      if (v >= -1e-6f && u >= -1e-6f && (u + v <= 1.0f + 1e-6f) && t > tNear) {
+        if (t < best) { second = best; best = t; id = uint32_t(k); }
+        else if (t < second) second = t;
+      }
+    }
+    t_out[r] = best;
+    id_out[r] = id;
+    t2_out[r] = second;
+  }
+}
+
+// MultiRenderer::initEyeRay (MultiRenderer.cpp:142-183), cutting plane disabled, spp 1:
+// EyeRayDirNormalized + transform_ray3f (render_common.h:25-55). order 1 places ray (x,y)
+// at SuperBlockIndex2DOpt(x, y, packed width) (render_common.h:225-242).
+void oracle_eye_rays(const float* proj_inv, const float* wv_inv, uint32_t width, uint32_t height, const float* jitter,
+                     uint32_t order, float* pos, float* dir) {
+  const uint32_t pw = 8 * ((width + 7) / 8), ph = 8 * ((height + 7) / 8);
+  for (uint32_t y = 0; y < height; ++y)
+    for (uint32_t x = 0; x < width; ++x) {
+      const uint32_t pix = y * width + x;
+      const f2 dd = jitter ? f2{jitter[2 * pix], jitter[2 * pix + 1]} : f2{0.5f, 0.5f};
+      const float fx = (float(x) + dd.x) / float(pw), fy = (float(y) + dd.y) / float(ph);
+      f4 p{2.0f * fx - 1.0f, -2.0f * fy + 1.0f, 1.0f, 1.0f};
+      p = mul(proj_inv, p);
+      const float w = p.w;
+      p = f4{p.x / w, p.y / w, p.z / w, p.w / w};
+      f3 rd = normalize(to3(p));
+      const f3 zero(0, 0, 0);
+      const f3 o = matmul4x3(wv_inv, zero);
+      const f3 o2 = matmul4x3(wv_inv, zero + 100.0f * rd);
+      rd = normalize(o2 - o);
+      uint32_t idx = pix;
+      if (order == 1) {
+        const uint32_t local = (y & 3u) * 4 + (x & 3u), bx = x >> 2, by = y >> 2;
+        const uint32_t localH = (by & 1u) * 2 + (bx & 1u), wbH = (pw >> 2) >> 1;
+        idx = ((bx >> 1) + (by >> 1) * wbH) * 64 + localH * 16 + local;
+      }
+      pos[4 * idx] = o.x; pos[4 * idx + 1] = o.y; pos[4 * idx + 2] = o.z; pos[4 * idx + 3] = 1e-9f;
+      dir[4 * idx] = rd.x; dir[4 * idx + 1] = rd.y; dir[4 * idx + 2] = rd.z; dir[4 * idx + 3] = 1e9f;
+    }
+}
+
+// MultiRenderer::UpdateCamera inverse matrices for lookAt/perspectiveMatrix
+// (LiteMath.h:3315-3360, MultiRenderer_host.cpp:104-110).
+void oracle_camera_inverse(const float* eye, const float* target, const float* up, float fovy, float aspect,
+                           float znear, float zfar, float* proj_inv, float* wv_inv) {
+  f3 z = normalize(f3(eye[0] - target[0], eye[1] - target[1], eye[2] - target[2]));
+  f3 y(up[0], up[1], up[2]);
+  f3 x = cross(y, z);
+  y = cross(z, x);
+  x = normalize(x);
+  y = normalize(y);
+  const float view[16] = {x.x, y.x, z.x, 0.0f, x.y, y.y, z.y, 0.0f, x.z, y.z, z.z, 0.0f,
+                          -x.x * eye[0] - x.y * eye[1] - x.z * eye[2], -y.x * eye[0] - y.y * eye[1] - y.z * eye[2],
+                          -z.x * eye[0] - z.y * eye[1] - z.z * eye[2], 1.0f};
+  const float ymax = znear * tanf(fovy * 3.14159265358979323846f / 360.0f);
+  const float xmax = ymax * aspect;
+  const float left = -xmax, right = +xmax, bottom = -ymax, top = +ymax;
+  const float temp = 2.0f * znear, temp2 = right - left, temp3 = top - bottom, temp4 = zfar - znear;
+  const float proj[16] = {temp / temp2, 0.0f, 0.0f, 0.0f, 0.0f, temp / temp3, 0.0f, 0.0f,
+                          (right + left) / temp2, (top + bottom) / temp3, (-zfar - znear) / temp4, -1.0f,
+                          0.0f, 0.0f, (-temp * zfar) / temp4, 0.0f};
+  inverse4x4(proj, proj_inv);
+  inverse4x4(view, wv_inv);
+}
+
+void oracle_inverse4x4(const float* m, float* out) { inverse4x4(m, out); }
+
+}  // extern "C"
