@@ -163,7 +163,7 @@ def main():
     g.eye_rays(pi, vi, w, h, pos, dirs, jitter=jt, order=1, stream=sh)
 
     # instrumented launch (untimed): traversal counters for the algorithmic-bytes model
-    counts = torch.zeros((n, 6), dtype=torch.int32, device="cuda")
+    counts = torch.zeros((n, 8), dtype=torch.int32, device="cuda")
     scene.trace_closest_counted(pos, dirs, hits, counts, stream=sh)
     torch.cuda.synchronize()
     cnt = counts.cpu().numpy().view(np.uint32)

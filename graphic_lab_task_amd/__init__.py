@@ -94,6 +94,12 @@ def lib() -> ctypes.CDLL:
     if _lib_handle is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        try:
+            # When torch is present its HIP runtime must be the one the library binds to, so that
+            # device pointers and streams handed over from torch are valid in both.
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp, u32, u64, sz, f32, cp = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t,
                                      ctypes.c_float, ctypes.c_char_p)
@@ -116,6 +122,7 @@ def lib() -> ctypes.CDLL:
             "lrt_query_one": ([vp, vp, vp, vp], i32),
             "lrt_eye_rays_dev": ([vp, vp, u32, u32, vp, u32, vp, vp, vp], i32),
             "lrt_set_launch_config": ([vp, i32, i32, i32], i32),
+            "lrt_set_tail_config": ([vp, u32, u32], i32),
             "lrt_stack_overflow_count": ([vp, ctypes.POINTER(u64), i32], i32),
             "lrt_get_array": ([vp, cp, vp, ctypes.POINTER(sz)], i32),
             "lrt_buffer_free": ([ctypes.POINTER(_Buffer)], None),
@@ -140,7 +147,7 @@ def exported_symbols():
             "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_sbs", "lrt_add_instance",
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
             "lrt_trace_closest_host", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
-            "lrt_set_launch_config", "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
+            "lrt_set_launch_config", "lrt_set_tail_config", "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
             "lrt_camera_inverse", "lrt_gen_icosphere", "lrt_gen_geodesic", "lrt_sdf_spheres_frame_octree",
             "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed"]
 
@@ -283,7 +290,7 @@ class SceneRT:
         _check(lib().lrt_trace_closest_dev(self._h, _ptr(pos), _ptr(dir), _ptr(hits), n, stream))
 
     def trace_closest_counted(self, pos, dir, hits, counters, stream: int = 0):
-        """As trace_closest, plus 6 uint32 traversal counters per ray (RayQuery_NearestHitWithRayInfo)."""
+        """As trace_closest, plus 8 uint32 words per ray: 6 traversal counters, start tick, elapsed ticks."""
         n = pos.numel() // 4
         _check(lib().lrt_trace_closest_counted_dev(self._h, _ptr(pos), _ptr(dir), _ptr(hits), _ptr(counters), n,
                                                    stream))
@@ -301,6 +308,9 @@ class SceneRT:
 
     def set_launch_config(self, block: int = 256, lds_stack: int = 16, xcd_remap: bool = True):
         _check(lib().lrt_set_launch_config(self._h, block, lds_stack, int(xcd_remap)))
+
+    def set_tail_config(self, budget: int = 64, tail_waves: int = 1024):
+        _check(lib().lrt_set_tail_config(self._h, budget, tail_waves))
 
     def stack_overflow_count(self, reset: bool = False) -> int:
         v = ctypes.c_uint64()

@@ -71,7 +71,13 @@ lrt_status trace_common(lrt_scene* s, const float* pos, const float* dir, lrt_hi
   a.lds_stack = s->s.lds_stack;
   a.xcd_remap = uint32_t(s->s.xcd_remap);
   a.counts = counts;
-  hipError_t e = lrt::launch_trace(s->s.make_view(), a, reinterpret_cast<hipStream_t>(stream));
+  hipError_t e = s->s.work_for(stream, size_t(n), a.lds_stack, &a.spill, &a.queue);
+  if (e != hipSuccess) return hip_fail(e, "work buffer");
+  // The cooperative pass reproduces every hit field except the per-leaf call count that the
+  // INTERSECTION_COUNT debug mode accumulates in instId; that mode runs single-pass.
+  a.budget = s->s.preset.render_mode == lrt::MULTI_RENDER_MODE_INTERSECTION_COUNT ? 0u : s->s.budget;
+  a.tail_waves = s->s.tail_waves;
+  e = lrt::launch_trace(s->s.make_view(), a, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "trace launch");
   return LRT_OK;
 }
@@ -281,6 +287,16 @@ lrt_status lrt_set_launch_config(lrt_scene* s, int block, int lds_stack, int xcd
     s->s.block = block;
     s->s.lds_stack = lds_stack;
     s->s.xcd_remap = xcd_remap ? 1 : 0;
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_set_tail_config(lrt_scene* s, uint32_t budget, uint32_t tail_waves) {
+  return guarded([&]() -> lrt_status {
+    if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+    if (tail_waves > 65536) return fail(LRT_ERR_INVALID_ARG, "tail_waves must be <= 65536");
+    s->s.budget = budget;
+    s->s.tail_waves = tail_waves ? tail_waves : 1024u;
     return LRT_OK;
   });
 }

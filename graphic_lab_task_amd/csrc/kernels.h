@@ -5,6 +5,13 @@
 
 namespace lrt {
 
+// Queue of rays handed from pass 1 to the cooperative pass 2 (device memory).
+struct TailQueue {
+  uint32_t  count;   // rays queued by pass 1
+  uint32_t  next;    // next queue entry pass 2 hands out
+  uint32_t* list;    // capacity: rays of the launch
+};
+
 struct TraceArgs {
   const float4* pos;        // posAndNear per ray (device)
   const float4* dir;        // dirAndFar per ray (device)
@@ -16,6 +23,10 @@ struct TraceArgs {
   int           lds_stack;  // 8, 16 or 32 stack entries kept in LDS
   uint32_t      xcd_remap;  // 1: contiguous ray range per XCD
   uint32_t*     counts;     // nullable: 6 counters per ray (RayCounts order), instrumented variant
+  uint32_t*     spill;      // n * (STACK_SIZE - lds_stack) words: stack entries deeper than the LDS part
+  uint32_t      budget;     // pass-1 steps per ray before it is queued for pass 2 (0: single pass)
+  TailQueue*    queue;      // device queue (nullable when budget is 0)
+  uint32_t      tail_waves; // persistent waves of pass 2 (0: default)
 };
 
 struct EyeRayParams {

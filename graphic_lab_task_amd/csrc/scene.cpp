@@ -35,7 +35,30 @@ void DevBuf::release() {
   cap_ = used_ = 0;
 }
 
+hipError_t Scene::work_for(void* stream, size_t n, int lds_stack, uint32_t** spill, TailQueue** queue) {
+  std::lock_guard<std::mutex> lock(spill_mu);
+  DevBuf& b = spill_bufs[stream];
+  size_t& cap = spill_rays[stream];
+  const size_t head = 256;
+  const size_t list_bytes = (n * 4 + 255) / 256 * 256;
+  // spill sized for the smallest LDS part so a later launch config change never overruns it
+  const size_t need = head + list_bytes + n * size_t(STACK_SIZE - 8) * 4;
+  (void)lds_stack;
+  if (cap < n || b.bytes() < need) {
+    hipError_t e = b.reserve(need);
+    if (e != hipSuccess) return e;
+    TailQueue q{0u, 0u, reinterpret_cast<uint32_t*>(static_cast<char*>(b.ptr()) + head)};
+    e = hipMemcpy(b.ptr(), &q, sizeof(q), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    cap = n;
+  }
+  *queue = static_cast<TailQueue*>(b.ptr());
+  *spill = reinterpret_cast<uint32_t*>(static_cast<char*>(b.ptr()) + head + (cap * 4 + 255) / 256 * 256);
+  return hipSuccess;
+}
+
 Scene::~Scene() {
+  spill_bufs.clear();
   dev.clear();
   if (d_overflow) (void)hipFree(d_overflow);
 }
@@ -331,13 +354,109 @@ void Scene::build_host() {
   tlas = build_tlas(bmin.data(), bmax.data(), instances.size());
 }
 
+// Derives the arrays the kernels read from the reference-layout host arrays. The traversal
+// visits the same nodes in the same order and tests the same floats, so hits are unchanged:
+//  * each geometry's pairs are renumbered: its top BFS levels first (they are staged in LDS
+//    for single-instance scenes), then the remaining pairs in their depth-first order;
+//  * triangle leaves hold PackOffsetAndSize(start, count) of their reordered triangles, taken
+//    from m_primIdCount, so a leaf costs no indirection;
+//  * triangles are stored as a vertex soup in leaf order (one 48-byte record per triangle,
+//    vertex indices kept in .w for the normal of the final hit);
+//  * frame-octree leaf boxes carry their 8 corner distances in one 64-byte record.
+void Scene::build_device_layout() {
+  devNodes = nodePairs;
+  // pair ranges per geometry (geometries append their pairs in order)
+  std::vector<std::pair<uint32_t, uint32_t>> ranges(geomData.size());
+  for (size_t g = 0; g < geomData.size(); ++g) {
+    const uint32_t b = geomData[g].bvhOffset;
+    const uint32_t e = g + 1 < geomData.size() ? geomData[g + 1].bvhOffset : uint32_t(nodePairs.size());
+    ranges[g] = {b, e};
+  }
+  for (size_t g = 0; g < geomData.size(); ++g) {
+    const uint32_t b = ranges[g].first, c = ranges[g].second - ranges[g].first;
+    if (c == 0) continue;
+    // new order: BFS over the top levels (up to LDS_TOP_PAIRS pairs), then depth-first order
+    std::vector<uint32_t> order;
+    std::vector<uint8_t> taken(c, 0);
+    std::vector<uint32_t> frontier{0};
+    taken[0] = 1;
+    while (!frontier.empty() && order.size() < size_t(LDS_TOP_PAIRS)) {
+      std::vector<uint32_t> next;
+      for (uint32_t p : frontier) {
+        if (order.size() >= size_t(LDS_TOP_PAIRS)) break;
+        order.push_back(p);
+        for (const BVHNode* ch : {&nodePairs[b + p].left, &nodePairs[b + p].right})
+          if (!(ch->leftOffset & LEAF_BIT) && ch->leftOffset < c) next.push_back(ch->leftOffset);
+      }
+      frontier.swap(next);
+    }
+    for (uint32_t p : order) taken[p] = 1;
+    for (uint32_t p = 0; p < c; ++p)
+      if (!taken[p]) order.push_back(p);
+    std::vector<uint32_t> newIdx(c);
+    for (uint32_t k = 0; k < c; ++k) newIdx[order[k]] = k;
+    const bool tri = geomTags[g] == TAG_TRIANGLE;
+    for (uint32_t k = 0; k < c; ++k) {
+      BVHNodePair pr = nodePairs[b + order[k]];
+      for (BVHNode* ch : {&pr.left, &pr.right}) {
+        if (!(ch->leftOffset & LEAF_BIT)) {
+          ch->leftOffset = newIdx[ch->leftOffset];
+        } else if (tri && ch->leftOffset != LEAF_NORMAL) {
+          const uint32_t w = primIdCount[startEnd[g].x + extract_start(ch->leftOffset)];
+          ch->leftOffset = pack_offset_and_size(extract_start(w), extract_count(w));
+        }
+      }
+      devNodes[b + k] = pr;
+    }
+  }
+  // triangle soup in leaf order
+  triSoup.assign(indices.size(), float4{0, 0, 0, 0});
+  for (size_t g = 0; g < geomData.size(); ++g) {
+    if (geomTags[g] != TAG_TRIANGLE) continue;
+    const uint32_t ib = geomData[g].offset.x, vb = geomData[g].offset.y;
+    size_t next_ib = indices.size();
+    for (size_t h = 0; h < geomData.size(); ++h)
+      if (geomTags[h] == TAG_TRIANGLE && geomData[h].offset.x > ib) next_ib = std::min<size_t>(next_ib, geomData[h].offset.x);
+    for (size_t k = ib; k < next_ib; ++k) {
+      const uint32_t vi = indices[k];
+      const float4 p = vertPos[vb + vi];
+      float4 r;
+      r.x = p.x; r.y = p.y; r.z = p.z;
+      std::memcpy(&r.w, &vi, 4);
+      triSoup[k] = r;
+    }
+  }
+  // frame-octree leaf records
+  frameLeaves.assign(origNodes.size(), FrameLeaf{});
+  for (size_t g = 0; g < geomData.size(); ++g) {
+    if (geomTags[g] != TAG_SDF_NODE || geomData[g].type != TYPE_SDF_FRAME_OCTREE) continue;
+    const uint32_t root = frameRoots[geomData[g].offset.x], ob = geomData[g].offset.y;
+    const uint32_t nb = startEnd[g].y - startEnd[g].x;
+    for (uint32_t k = 0; k < nb; ++k) {
+      const BVHNode& on = origNodes[ob + k];
+      FrameLeaf& fl = frameLeaves[ob + k];
+      fl.boxMin = on.boxMin;
+      fl.boxMax = on.boxMax;
+      for (int j = 0; j < 8; ++j) fl.values[j] = frameNodes[on.leftOffset + root].values[j];
+      fl.primId = on.leftOffset;
+    }
+  }
+  cacheGeom = 0xFFFFFFFFu;
+  cachePairs = 0;
+  if (instances.size() == 1) {
+    cacheGeom = instances[0].geomId;
+    cachePairs = std::min<uint32_t>(LDS_TOP_PAIRS, ranges[cacheGeom].second - ranges[cacheGeom].first);
+  }
+}
+
 hipError_t Scene::commit() {
+  build_device_layout();
   hipError_t e = hipGetDevice(&device);
   if (e != hipSuccess) return e;
 #define UP(name, vec)                                                        \
   e = dev[name].upload(vec.data(), vec.size() * sizeof(vec[0]));             \
   if (e != hipSuccess) return e;
-  UP("nodes", nodePairs) UP("tlas", tlas) UP("instances", instances) UP("geoms", geomData) UP("geom_tags", geomTags)
+  UP("nodes", devNodes) UP("tri_soup", triSoup) UP("frame_leaves", frameLeaves) UP("tlas", tlas) UP("instances", instances) UP("geoms", geomData) UP("geom_tags", geomTags)
   UP("start_end", startEnd) UP("prim_id_count", primIdCount) UP("vert_pos", vertPos) UP("vert_norm", vertNorm)
   UP("indices", indices) UP("prim_indices", primIndices) UP("orig_nodes", origNodes) UP("frame_nodes", frameNodes)
   UP("frame_roots", frameRoots) UP("sbs_nodes", sbsNodes) UP("sbs_data", sbsData) UP("sbs_data_f", sbsDataF)
@@ -363,6 +482,8 @@ SceneView Scene::make_view() const {
   };
   v.nodes = (const BVHNodePair*)P("nodes");
   v.tlas = (const BVHNode*)P("tlas");
+  v.tri_soup = (const float4*)P("tri_soup");
+  v.frame_leaves = (const FrameLeaf*)P("frame_leaves");
   v.instances = (const InstanceData*)P("instances");
   v.geoms = (const GeomData*)P("geoms");
   v.geom_tags = (const uint32_t*)P("geom_tags");
@@ -386,6 +507,8 @@ SceneView Scene::make_view() const {
   v.lod_thr = lod_thr;
   v.leaf_kinds = leaf_kinds();
   v.n_instances = uint32_t(instances.size());
+  v.cache_geom = cacheGeom;
+  v.cache_pairs = cachePairs;
   return v;
 }
 

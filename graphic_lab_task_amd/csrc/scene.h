@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -56,12 +57,26 @@ struct Scene {
   std::vector<InstanceData> instances;
   std::vector<BVHNode> tlas;
 
+  // Device layout derived at commit (same traversal, fewer dependent loads per step).
+  std::vector<BVHNodePair> devNodes;
+  std::vector<float4> triSoup;
+  std::vector<FrameLeaf> frameLeaves;
+  uint32_t cacheGeom = 0xFFFFFFFFu, cachePairs = 0;
+
   bool committed = false;
   int device = 0;
   std::map<std::string, DevBuf> dev;
   SceneView view{};
   uint32_t* d_overflow = nullptr;
   int block = 256, lds_stack = 16, xcd_remap = 1;
+  // Deep-stack spill buffers, one per stream: launches on one stream run in order, so rays of
+  // two launches never share a slot at the same time.
+  std::mutex spill_mu;
+  std::map<void*, DevBuf> spill_bufs;
+  std::map<void*, size_t> spill_rays;
+  uint32_t budget = 64, tail_waves = 1024;
+  // Per-stream work memory for n rays: tail queue (header + list) and deep-stack spill.
+  hipError_t work_for(void* stream, size_t n, int lds_stack, uint32_t** spill, TailQueue** queue);
 
   ~Scene();
 
@@ -76,6 +91,7 @@ struct Scene {
   uint32_t add_instance(uint32_t geom_id, const float4x4& m);
   void update_instance(uint32_t inst_id, const float4x4& m);
   void build_host();   // TLAS over the instance boxes (host only)
+  void build_device_layout();
   hipError_t commit(); // device upload of every array
   void set_preset(const MultiRenderPreset& p);
   uint32_t leaf_kinds() const;

@@ -13,9 +13,24 @@ enum LeafKind : uint32_t {
   LK_MULTI = 1u << 3,   // TAG_SDF_OCTREE (multi-octree walk)
 };
 
+// Frame-octree leaf as the kernels read it: the leaf box, its 8 corner distances and the
+// octree node index, gathered at commit into one 64-byte record per BLAS box.
+struct alignas(16) FrameLeaf {
+  float3   boxMin;
+  float3   boxMax;
+  float    values[8];
+  uint32_t primId;
+  uint32_t _pad;
+};
+static_assert(sizeof(FrameLeaf) == 64, "FrameLeaf is 64 B");
+
+// Number of BLAS pairs (top BFS levels of the cached geometry) the kernels keep in LDS.
+constexpr int LDS_TOP_PAIRS = 63;
+
 struct SceneView {
-  // BVH
-  const BVHNodePair*  nodes;        // m_allNodePairs
+  // BVH, device layout: the reference's fat pairs with leaves holding PackOffsetAndSize(start,
+  // count) directly and each geometry's pairs renumbered so its top BFS levels come first.
+  const BVHNodePair*  nodes;
   const BVHNode*      tlas;         // m_nodesTLAS
   const InstanceData* instances;    // m_instanceData
   const GeomData*     geoms;        // m_geomData
@@ -23,11 +38,13 @@ struct SceneView {
   const uint2*        start_end;    // startEnd
   const uint32_t*     prim_id_count;// m_primIdCount
   // meshes
+  const float4*       tri_soup;     // 3 float4 per leaf-ordered triangle: (A.xyz, idxA), (B.xyz, idxB), (C.xyz, idxC)
   const float4*       vert_pos;     // m_vertPos
   const float4*       vert_norm;    // m_vertNorm
   const uint32_t*     indices;      // m_indices (leaf order)
   const uint32_t*     prim_indices; // m_primIndices
   // frame octree
+  const FrameLeaf*          frame_leaves; // per BLAS box, same index as m_origNodes
   const BVHNode*            orig_nodes;   // m_origNodes
   const SdfFrameOctreeNode* frame_nodes;  // m_SdfFrameOctreeNodes
   const uint32_t*           frame_roots;  // m_SdfFrameOctreeRoots
@@ -45,6 +62,8 @@ struct SceneView {
   float    lod_thr;                 // m_precomputedLoDThr
   uint32_t leaf_kinds;              // LeafKind mask
   uint32_t n_instances;
+  uint32_t cache_geom;              // geometry whose top pairs are staged in LDS (~0u: none)
+  uint32_t cache_pairs;             // number of staged pairs (<= LDS_TOP_PAIRS)
 };
 
 }  // namespace lrt

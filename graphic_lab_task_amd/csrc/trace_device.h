@@ -10,19 +10,23 @@ namespace lrt {
 
 // ---------------------------------------------------------------------------------------
 // Traversal stack: the first S entries live in LDS (entry-major, one column per lane, so
-// lanes never conflict on a bank), deeper entries spill to a per-lane private array.
-// Depth limit STACK_SIZE=80 as in raytrace_common.h:36; pushes beyond it are dropped and
-// counted instead of writing out of bounds.
-template <int S>
+// lanes never conflict on a bank), deeper entries go to this ray's slot of a global spill
+// buffer (only rays deeper than S touch it). Depth limit STACK_SIZE=80 as in
+// raytrace_common.h:36; pushes beyond it are dropped and counted instead of writing out of
+// bounds. The LDS column is an explicit address-space-3 pointer so that every access is a
+// ds_read/ds_write rather than a flat access.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+template <int S, int BLOCK>
 struct LaneStack {
-  uint32_t* col;      // &lds[threadIdx.x]; entry i at col[i * stride]
-  uint32_t  stride;
-  uint32_t  spill[STACK_SIZE - S];
+  lds_u32* col;        // entry i at col[i * BLOCK]
+  uint32_t* spill;     // entry i >= S at spill[i - S]
   __device__ __forceinline__ void put(int i, uint32_t v) {
-    if (i < S) col[i * stride] = v; else spill[i - S] = v;
+    if (i < S) col[i * BLOCK] = v; else spill[i - S] = v;
   }
   __device__ __forceinline__ uint32_t get(int i) const {
-    return i < S ? col[i * stride] : spill[i - S];
+    if (i < S) return col[i * BLOCK];
+    return spill[i - S];
   }
 };
 
@@ -32,6 +36,9 @@ struct LaneStack {
 struct RayCounts {
   uint32_t blas_nodes, leaves, prims, tlas_nodes, instances, sdf_voxels;
 };
+// Words per ray written by the instrumented variant: the 6 counters, then the device wall
+// clock (100 MHz ticks) at ray start (low 32 bits) and the ray's elapsed ticks.
+constexpr int RAY_COUNTERS = 8;
 
 // Triangle hit whose normal is resolved once the closest hit is final (the normal depends
 // only on the winning triangle and its barycentrics, BVH2Common.cpp:1609-1626).
@@ -51,6 +58,26 @@ __device__ __forceinline__ BVHNodePair load_pair(const BVHNodePair* p) {
   r.right.leftOffset = c.w;
   r.right.boxMax = make3(__uint_as_float(d.x), __uint_as_float(d.y), __uint_as_float(d.z));
   r.right.escapeIndex = d.w;
+  return r;
+}
+
+typedef __attribute__((address_space(3))) BVHNodePair lds_pair;
+typedef __attribute__((address_space(3))) const uint32_t lds_cu32;
+
+__device__ __forceinline__ BVHNodePair load_pair_lds(const lds_pair* p) {
+  lds_cu32* q = reinterpret_cast<lds_cu32*>(p);
+  uint32_t w[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) w[k] = q[k];
+  BVHNodePair r;
+  r.left.boxMin = make3(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]));
+  r.left.leftOffset = w[3];
+  r.left.boxMax = make3(__uint_as_float(w[4]), __uint_as_float(w[5]), __uint_as_float(w[6]));
+  r.left.escapeIndex = w[7];
+  r.right.boxMin = make3(__uint_as_float(w[8]), __uint_as_float(w[9]), __uint_as_float(w[10]));
+  r.right.leftOffset = w[11];
+  r.right.boxMax = make3(__uint_as_float(w[12]), __uint_as_float(w[13]), __uint_as_float(w[14]));
+  r.right.escapeIndex = w[15];
   return r;
 }
 
@@ -75,19 +102,18 @@ __device__ __forceinline__ void load_mat(const float4x4& m, float* out) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Triangles: BVHRT::IntersectAllTrianglesInLeaf (BVH2Common.cpp:1571-1635)
+// Triangles: BVHRT::IntersectAllTrianglesInLeaf (BVH2Common.cpp:1571-1635), reading the
+// leaf-ordered vertex soup: triangle triId of geometry geomId is record off.x/3 + triId.
 __device__ __forceinline__ void intersect_tris(const SceneView& s, float3 ray_pos, float3 ray_dir, float tNear,
                                                uint32_t instId, uint32_t geomId, uint32_t a_start, uint32_t a_count,
                                                CRT_Hit& hit, PendingTri& pend) {
-  const uint2 off = s.geoms[geomId].offset;
+  const uint32_t tri0 = s.geoms[geomId].offset.x / 3;
   for (uint32_t triId = a_start; triId < a_start + a_count; triId++) {
-    const uint32_t base = off.x + triId * 3;
-    const uint32_t A = s.indices[base + 0];
-    const uint32_t B = s.indices[base + 1];
-    const uint32_t C = s.indices[base + 2];
-    const float3 A_pos = xyz(s.vert_pos[off.y + A]);
-    const float3 B_pos = xyz(s.vert_pos[off.y + B]);
-    const float3 C_pos = xyz(s.vert_pos[off.y + C]);
+    const float4* q = s.tri_soup + size_t(tri0 + triId) * 3;
+    const float4 a = q[0], b = q[1], c = q[2];
+    const float3 A_pos = xyz(a);
+    const float3 B_pos = xyz(b);
+    const float3 C_pos = xyz(c);
     const float3 edge1 = B_pos - A_pos;
     const float3 edge2 = C_pos - A_pos;
     const float3 pvec = cross3(ray_dir, edge2);
@@ -115,14 +141,15 @@ __device__ __forceinline__ void intersect_tris(const SceneView& s, float3 ray_po
 __device__ __forceinline__ void resolve_tri_normal(const SceneView& s, const PendingTri& pend, CRT_Hit& hit) {
   if (!pend.valid || !preset_needs_normal(s.preset.render_mode)) return;
   const uint2 off = s.geoms[pend.geomId].offset;
-  const uint32_t base = off.x + pend.triId * 3;
-  const uint32_t A = s.indices[base + 0], B = s.indices[base + 1], C = s.indices[base + 2];
+  const float4* q = s.tri_soup + size_t(off.x / 3 + pend.triId) * 3;
+  const float4 qa = q[0], qb = q[1], qc = q[2];
+  const uint32_t A = __float_as_uint(qa.w), B = __float_as_uint(qb.w), C = __float_as_uint(qc.w);
   const float u = hit.coords[0], v = hit.coords[1];
   float3 n = make3(1.0f, 0.0f, 0.0f);
   if (s.preset.normal_mode == NORMAL_MODE_GEOMETRY) {
-    const float3 A_pos = xyz(s.vert_pos[off.y + A]);
-    const float3 edge1 = xyz(s.vert_pos[off.y + B]) - A_pos;
-    const float3 edge2 = xyz(s.vert_pos[off.y + C]) - A_pos;
+    const float3 A_pos = xyz(qa);
+    const float3 edge1 = xyz(qb) - A_pos;
+    const float3 edge2 = xyz(qc) - A_pos;
     n = cross3(edge1, edge2);
   } else if (s.preset.normal_mode == NORMAL_MODE_VERTEX) {
     const float4 nA = s.vert_norm[off.y + A], nB = s.vert_norm[off.y + B], nC = s.vert_norm[off.y + C];
@@ -349,22 +376,20 @@ __device__ bool local_surface_intersection(const SceneView& s, uint32_t type, fl
   return false;
 }
 
-// Frame-octree leaf: BVHRT::OctreeNodeIntersect + RayNodeIntersection (BVH2Common.cpp:727-749, 141-167)
+// Frame-octree leaf: BVHRT::OctreeNodeIntersect + RayNodeIntersection (BVH2Common.cpp:727-749,
+// 141-167), reading the leaf's box, corner values and node index from one FrameLeaf record.
 __device__ __forceinline__ void frame_node_intersect(const SceneView& s, uint32_t type, float3 ray_pos, float3 ray_dir,
                                                      float tNear, uint32_t instId, uint32_t geomId,
                                                      uint32_t bvhNodeId, CRT_Hit& hit) {
   if (type != TYPE_SDF_FRAME_OCTREE) return;
-  const uint2 go = s.geoms[geomId].offset;   // x: octree id, y: first leaf box of this octree
-  const uint32_t sdfId = go.x;
-  const BVHNode on = load_node(s.orig_nodes + go.y + bvhNodeId);
-  const uint32_t primId = on.leftOffset;
-  const uint32_t nodeId = primId + s.frame_roots[sdfId];
-  const float3 min_pos = on.boxMin, max_pos = on.boxMax;
+  const uint2 go = s.geoms[geomId].offset;   // y: first leaf box of this octree
+  const float4* q = reinterpret_cast<const float4*>(s.frame_leaves + go.y + bvhNodeId);
+  const float4 r0 = q[0], r1 = q[1], r2 = q[2], r3 = q[3];
+  const float3 min_pos = make3(r0.x, r0.y, r0.z);
+  const float3 max_pos = make3(r0.w, r1.x, r1.y);
+  const float values[8] = {r1.z, r1.w, r2.x, r2.y, r2.z, r2.w, r3.x, r3.y};
+  const uint32_t primId = __float_as_uint(r3.z);
   const float3 size = max_pos - min_pos;
-  float values[8];
-  const SdfFrameOctreeNode* fn = s.frame_nodes + nodeId;
-#pragma unroll
-  for (int i = 0; i < 8; i++) values[i] = fn->values[i];
   const float2 fNearFar = ray_box2(ray_pos, safe_inverse(ray_dir), min_pos, max_pos);
   const float3 start_pos = ray_pos + fNearFar.x * ray_dir;
   const float d = fmax_ref(size.x, fmax_ref(size.y, size.z));

@@ -1,4 +1,13 @@
 // Closest-hit / any-hit ray query kernels for gfx950 and the primary-ray generator.
+//
+// A trace call runs in two passes (plus a one-block reset):
+//  1. one ray per lane, the reference's sequential walk, with a step budget: the few rays
+//     that need more steps than the budget (grazing rays crossing many boxes) leave their
+//     index in a queue instead of holding their whole wave hostage;
+//  2. a persistent kernel where each wave takes one queued ray and walks its BLAS
+//     cooperatively (trace_coop.h), which returns the same hit in far fewer dependent steps.
+// The instrumented (counted) variant runs pass 1 without a budget.
+#include "trace_coop.h"
 #include "trace_device.h"
 #include "kernels.h"
 
@@ -6,21 +15,27 @@ namespace lrt {
 
 // BVHRT::BVH2TraverseF32 (MultiRenderer/BVH2Common.cpp:3239-3320): stack-based walk over
 // fat nodes, nearer child first, far child pushed; leaves dispatched on the geometry tag.
-template <uint32_t KINDS, int S, bool COUNT>
+// `steps` counts fat-node fetches and leaf visits; past `budget` the walk stops and sets
+// `aborted` (budget 0 = unlimited).
+template <uint32_t KINDS, int S, int BLOCK, bool COUNT>
 __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos, float3 ray_dir, float tNear,
                                               uint32_t instId, uint32_t geomId, bool stopOnFirstHit, CRT_Hit& hit,
-                                              PendingTri& pend, LaneStack<S>& st, uint32_t& overflow,
-                                              RayCounts& cnt) {
+                                              PendingTri& pend, LaneStack<S, BLOCK>& st, uint32_t& overflow,
+                                              RayCounts& cnt, const lds_pair* top_pairs, uint32_t budget,
+                                              uint32_t& steps, bool& aborted) {
   const uint32_t bvhOffset = s.geoms[geomId].bvhOffset;
   const BVHNodePair* nodes = s.nodes + bvhOffset;
+  const uint32_t cached = (top_pairs != nullptr && geomId == s.cache_geom) ? s.cache_pairs : 0u;
   int top = 0;
   uint32_t leftNodeOffset = 0;
   bool hitFound = false;
   const float3 rayDirInv = safe_inverse(ray_dir);
   while (top >= 0 && !hitFound) {
     while (top >= 0 && ((leftNodeOffset & LEAF_BIT) == 0)) {
+      if (budget && ++steps > budget) { aborted = true; return; }
       if (COUNT) cnt.blas_nodes++;
-      const BVHNodePair fat = load_pair(nodes + leftNodeOffset);
+      const BVHNodePair fat = leftNodeOffset < cached ? load_pair_lds(top_pairs + leftNodeOffset)
+                                                      : load_pair(nodes + leftNodeOffset);
       const uint32_t n0 = fat.left.leftOffset;
       const uint32_t n1 = fat.right.leftOffset;
       const float2 tm0 = ray_box2(ray_pos, rayDirInv, fat.left.boxMin, fat.left.boxMax);
@@ -44,13 +59,13 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
       }
     }
     if (top >= 0 && leftNodeOffset != 0xFFFFFFFFu) {
-      // BVHRT::IntersectAllPrimitivesInLeaf (BVH2Common.cpp:47-124)
-      const uint32_t aabbId = extract_start(leftNodeOffset);
+      if (budget && ++steps > budget) { aborted = true; return; }
+      // BVHRT::IntersectAllPrimitivesInLeaf (BVH2Common.cpp:47-124); device-layout leaves carry
+      // PackOffsetAndSize(start, count) themselves
+      const uint32_t a_start = extract_start(leftNodeOffset);
+      const uint32_t a_count = extract_count(leftNodeOffset);
       const float tNearSdf = fmax_ref(tNear, 0.01f);
       const float tPrev = hit.t;
-      const uint32_t packed = s.prim_id_count[s.start_end[geomId].x + aabbId];
-      const uint32_t a_start = extract_start(packed);
-      const uint32_t a_count = extract_count(packed);
       const uint32_t tag = s.geom_tags[geomId];
       if (COUNT) {
         cnt.leaves++;
@@ -76,18 +91,9 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
   }
 }
 
-// BVHRT::RayQuery_NearestHit (MultiRenderer/BVH2Common.cpp:3322-3392): stackless TLAS walk
-// over escape indices, ray moved to instance space (direction not renormalised).
-template <uint32_t KINDS, int S, bool COUNT>
-__device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 posAndNear, float4 dirAndFar,
-                                                     LaneStack<S>& st, uint32_t& overflow, RayCounts& cnt) {
-  bool stopOnFirstHit = (dirAndFar.w <= 0.0f);
-  if (stopOnFirstHit) dirAndFar.w *= -1.0f;
-  const float3 pos = xyz(posAndNear);
-  const float3 dir = xyz(dirAndFar);
-  const float3 rayDirInv = safe_inverse(dir);
+__device__ __forceinline__ CRT_Hit miss_hit(float far) {
   CRT_Hit hit;
-  hit.t = dirAndFar.w;
+  hit.t = far;
   hit.primId = 0xFFFFFFFFu;
   hit.instId = 0xFFFFFFFFu;
   hit.geomId = 0xFFFFFFFFu;
@@ -95,8 +101,37 @@ __device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 
   hit.coords[1] = 0.0f;
   hit.coords[2] = 0.0f;
   hit.coords[3] = 0.0f;
+  return hit;
+}
+
+// Normal of a final triangle hit and the m_primIndices remap (BVH2Common.cpp:3382-3388).
+template <uint32_t KINDS>
+__device__ __forceinline__ void finish_hit(const SceneView& s, const PendingTri& pend, CRT_Hit& hit) {
+  if ((KINDS & LK_TRI) && pend.valid && hit.geomId != 0xFFFFFFFFu && (hit.geomId >> SH_TYPE) == TYPE_MESH_TRIANGLE)
+    resolve_tri_normal(s, pend, hit);
+  if (hit.geomId < 0xFFFFFFFFu && ((hit.geomId >> SH_TYPE) == TYPE_MESH_TRIANGLE)) {
+    const uint2 go = s.geoms[hit.geomId & GEOM_ID_MASK].offset;
+    hit.primId = s.prim_indices[go.x / 3 + hit.primId];
+  }
+}
+
+// BVHRT::RayQuery_NearestHit (MultiRenderer/BVH2Common.cpp:3322-3392): stackless TLAS walk
+// over escape indices, ray moved to instance space (direction not renormalised).
+// COOP = true: the wave holds one ray and walks each BLAS cooperatively.
+template <uint32_t KINDS, int S, int BLOCK, bool COUNT, bool COOP>
+__device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 posAndNear, float4 dirAndFar,
+                                                     LaneStack<S, BLOCK>& st, uint32_t& overflow, RayCounts& cnt,
+                                                     const lds_pair* top_pairs, uint32_t budget, bool& aborted,
+                                                     lds_entry* coop_stack) {
+  bool stopOnFirstHit = (dirAndFar.w <= 0.0f);
+  if (stopOnFirstHit) dirAndFar.w *= -1.0f;
+  const float3 pos = xyz(posAndNear);
+  const float3 dir = xyz(dirAndFar);
+  const float3 rayDirInv = safe_inverse(dir);
+  CRT_Hit hit = miss_hit(dirAndFar.w);
   PendingTri pend;
   pend.valid = 0;
+  uint32_t steps = 0;
   uint32_t nodeIdx = 0;
   do {
     uint32_t travFlags = 0;
@@ -120,17 +155,29 @@ __device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 
       load_mat(inst->transformInv, m);
       const float3 ray_pos = matmul4x3(m, pos);
       const float3 ray_dir = matmul3x3(m, dir);
-      blas_traverse<KINDS, S, COUNT>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId, stopOnFirstHit, hit, pend,
-                                     st, overflow, cnt);
+      if (COOP) {
+        if (!coop_blas<KINDS>(s, coop_stack, ray_pos, ray_dir, posAndNear.w, instId, geomId, stopOnFirstHit, hit,
+                              pend, cnt)) {
+          // frontier too wide or tree deeper than the key: every lane walks the same ray
+          // sequentially on one shared LDS stack
+          LaneStack<STACK_SIZE, 1> seq;
+          seq.col = (lds_u32*)coop_stack;
+          seq.spill = nullptr;
+          uint32_t seq_steps = 0;
+          bool seq_abort = false;
+          blas_traverse<KINDS, STACK_SIZE, 1, false>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId,
+                                                     stopOnFirstHit, hit, pend, seq, overflow, cnt, nullptr, 0u,
+                                                     seq_steps, seq_abort);
+          __syncthreads();
+        }
+      } else {
+        blas_traverse<KINDS, S, BLOCK, COUNT>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId, stopOnFirstHit, hit,
+                                              pend, st, overflow, cnt, top_pairs, budget, steps, aborted);
+        if (aborted) return hit;
+      }
     }
   } while (nodeIdx < 0xFFFFFFFEu && !(stopOnFirstHit && hit.primId != 0xFFFFFFFFu));
-
-  if ((KINDS & LK_TRI) && pend.valid && hit.geomId != 0xFFFFFFFFu && (hit.geomId >> SH_TYPE) == TYPE_MESH_TRIANGLE)
-    resolve_tri_normal(s, pend, hit);
-  if (hit.geomId < 0xFFFFFFFFu && ((hit.geomId >> SH_TYPE) == TYPE_MESH_TRIANGLE)) {
-    const uint2 go = s.geoms[hit.geomId & GEOM_ID_MASK].offset;
-    hit.primId = s.prim_indices[go.x / 3 + hit.primId];
-  }
+  finish_hit<KINDS>(s, pend, hit);
   return hit;
 }
 
@@ -147,33 +194,95 @@ __device__ __forceinline__ void store_hit(CRT_Hit* out, uint32_t i, const CRT_Hi
   o[1] = make4(h.coords[0], h.coords[1], h.coords[2], h.coords[3]);
 }
 
+// Pass 1. `budget` 0: unlimited (no queue).
 template <uint32_t KINDS, int S, int BLOCK, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_trace_closest(SceneView s, const float4* __restrict__ pos,
                                                          const float4* __restrict__ dir, CRT_Hit* __restrict__ out,
                                                          uint32_t* __restrict__ any_out, uint32_t n,
                                                          uint32_t* overflow_count, uint32_t xcd_remap,
-                                                         uint32_t* __restrict__ counts) {
-  __shared__ uint32_t lds[S > 0 ? S * BLOCK : 1];
+                                                         uint32_t* __restrict__ counts, uint32_t* __restrict__ spill,
+                                                         uint32_t budget, TailQueue* queue) {
+  __shared__ uint32_t lds[S * BLOCK];
+  __shared__ BVHNodePair top_pairs[LDS_TOP_PAIRS];
+  // stage the top BFS levels of the cached geometry's BLAS (every thread reaches the barrier)
+  if (s.cache_pairs) {
+    const uint4* src = reinterpret_cast<const uint4*>(s.nodes + s.geoms[s.cache_geom].bvhOffset);
+    uint4* dst = reinterpret_cast<uint4*>(top_pairs);
+    for (uint32_t k = threadIdx.x; k < s.cache_pairs * 4; k += BLOCK) dst[k] = src[k];
+  }
+  __syncthreads();
   const uint32_t b = xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint32_t i = b * BLOCK + threadIdx.x;
   if (i >= n) return;
-  LaneStack<S> st;
-  st.col = lds + threadIdx.x;
-  st.stride = BLOCK;
+  LaneStack<S, BLOCK> st;
+  st.col = (lds_u32*)(lds) + threadIdx.x;
+  st.spill = spill + size_t(i) * (STACK_SIZE - S);
   uint32_t overflow = 0;
   float4 p = pos[i];
   float4 d = dir[i];
   if (any_out) d.w *= -1.0f;   // RayQuery_AnyHit flips the sign of w (BVH2Common.cpp:3394-3399)
   RayCounts cnt{};
-  const CRT_Hit h = ray_query_nearest<KINDS, S, COUNT>(s, p, d, st, overflow, cnt);
+  bool aborted = false;
+  const uint64_t t_start = COUNT ? wall_clock64() : 0;
+  const CRT_Hit h = ray_query_nearest<KINDS, S, BLOCK, COUNT, false>(s, p, d, st, overflow, cnt,
+                                                                     (const lds_pair*)top_pairs,
+                                                                     COUNT ? 0u : budget, aborted, nullptr);
+  if (aborted) {
+    queue->list[atomicAdd(&queue->count, 1u)] = i;
+    return;
+  }
   if (out) store_hit(out, i, h);
   if (COUNT) {
-    uint32_t* c = counts + size_t(i) * 6;
+    const uint64_t t_end = wall_clock64();
+    uint32_t* c = counts + size_t(i) * RAY_COUNTERS;
     c[0] = cnt.blas_nodes; c[1] = cnt.leaves; c[2] = cnt.prims;
     c[3] = cnt.tlas_nodes; c[4] = cnt.instances; c[5] = cnt.sdf_voxels;
+    c[6] = uint32_t(t_start);   // device wall clock (100 MHz) at ray start, low 32 bits
+    c[7] = uint32_t(t_end - t_start);
   }
   if (any_out) any_out[i] = (h.geomId != 0xFFFFFFFFu) ? 1u : 0u;
   if (overflow && overflow_count) atomicAdd(overflow_count, 1u);
+}
+
+// Pass 2: persistent waves; each takes the next queued ray and traces it cooperatively.
+template <uint32_t KINDS>
+__global__ __launch_bounds__(64) void k_trace_tail(SceneView s, const float4* __restrict__ pos,
+                                                   const float4* __restrict__ dir, CRT_Hit* __restrict__ out,
+                                                   uint32_t* __restrict__ any_out, uint32_t* overflow_count,
+                                                   TailQueue* queue) {
+  __shared__ uint32_t coop[COOP_CAP * 4];
+  const uint32_t total = __atomic_load_n(&queue->count, __ATOMIC_RELAXED);
+  for (;;) {
+    uint32_t idx = 0;
+    if (__lane_id() == 0) idx = atomicAdd(&queue->next, 1u);
+    idx = __shfl(idx, 0, 64);
+    if (idx >= total) break;
+    const uint32_t i = queue->list[idx];
+    float4 p = pos[i];
+    float4 d = dir[i];
+    if (any_out) d.w *= -1.0f;
+    LaneStack<1, 64> unused;
+    unused.col = nullptr;
+    unused.spill = nullptr;
+    RayCounts cnt{};
+    uint32_t overflow = 0;
+    bool aborted = false;
+    const CRT_Hit h = ray_query_nearest<KINDS, 1, 64, false, true>(s, p, d, unused, overflow, cnt, nullptr, 0u,
+                                                                   aborted, (lds_entry*)coop);
+    if (__lane_id() == 0) {
+      if (out) store_hit(out, i, h);
+      if (any_out) any_out[i] = (h.geomId != 0xFFFFFFFFu) ? 1u : 0u;
+      if (overflow && overflow_count) atomicAdd(overflow_count, 1u);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_tail_reset(TailQueue* queue) {
+  if (threadIdx.x == 0) {
+    queue->count = 0;
+    queue->next = 0;
+  }
 }
 
 // SuperBlockIndex2DOpt (utils/raytracing/render_common.h:225-242): 4x4 blocks in 2x2 super blocks.
@@ -216,17 +325,27 @@ template <uint32_t KINDS, int S, int BLOCK>
 static hipError_t launch_trace_t(const SceneView& s, const TraceArgs& a, hipStream_t stream) {
   const uint32_t grid = (a.n + BLOCK - 1) / BLOCK;
   if (grid == 0) return hipSuccess;
-  if (a.counts)
+  if (a.counts) {
     hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, true>), dim3(grid), dim3(BLOCK), 0, stream, s, a.pos, a.dir,
-                       a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts);
-  else
-    hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, false>), dim3(grid), dim3(BLOCK), 0, stream, s, a.pos,
-                       a.dir, a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts);
+                       a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts, a.spill, 0u, a.queue);
+    return hipGetLastError();
+  }
+  const bool tail = a.budget != 0 && a.queue != nullptr;
+  if (tail) hipLaunchKernelGGL(k_tail_reset, dim3(1), dim3(64), 0, stream, a.queue);
+  hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, false>), dim3(grid), dim3(BLOCK), 0, stream, s, a.pos, a.dir,
+                     a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts, a.spill, tail ? a.budget : 0u,
+                     a.queue);
+  if (tail) {
+    const uint32_t waves = a.tail_waves ? a.tail_waves : 1024u;
+    hipLaunchKernelGGL((k_trace_tail<KINDS>), dim3(waves), dim3(64), 0, stream, s, a.pos, a.dir, a.out, a.any_out,
+                       a.overflow_count, a.queue);
+  }
   return hipGetLastError();
 }
 
 template <uint32_t KINDS>
 static hipError_t launch_kinds(const SceneView& s, const TraceArgs& a, hipStream_t stream) {
+  if (a.counts) return launch_trace_t<KINDS, 16, 256>(s, a, stream);   // one shape for the counted variant
   const int cfg = a.block * 1000 + a.lds_stack;
   switch (cfg) {
     case 64 * 1000 + 16:  return launch_trace_t<KINDS, 16, 64>(s, a, stream);
@@ -244,12 +363,6 @@ static hipError_t launch_kinds(const SceneView& s, const TraceArgs& a, hipStream
 
 hipError_t launch_trace(const SceneView& s, const TraceArgs& a, hipStream_t stream) {
   const uint32_t k = s.leaf_kinds;
-  if (a.counts) {  // the counting variant has one launch shape
-    if (k == LK_TRI) return launch_trace_t<LK_TRI, 16, 256>(s, a, stream);
-    if (k == LK_FRAME) return launch_trace_t<LK_FRAME, 16, 256>(s, a, stream);
-    if (k == LK_SBS) return launch_trace_t<LK_SBS, 16, 256>(s, a, stream);
-    return launch_trace_t<LK_TRI | LK_FRAME | LK_SBS, 16, 256>(s, a, stream);
-  }
   if (k == LK_TRI) return launch_kinds<LK_TRI>(s, a, stream);
   if (k == LK_FRAME) return launch_kinds<LK_FRAME>(s, a, stream);
   if (k == LK_SBS) return launch_kinds<LK_SBS>(s, a, stream);

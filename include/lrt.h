@@ -106,9 +106,10 @@ lrt_status lrt_trace_closest_host(lrt_scene* s, const float* pos_near, const flo
                                   uint64_t n);
 
 /* Instrumented variant of lrt_trace_closest_dev, replacing RayQuery_NearestHitWithRayInfo
- * (BVH2Common.cpp:3403-3571): also writes 6 uint32 counters per ray to `counters` (device,
- * 6*n): fat BLAS nodes fetched, leaves entered, triangles tested, TLAS nodes fetched,
- * instances entered, SBS voxels stepped. Same hits as the plain call. */
+ * (BVH2Common.cpp:3403-3571): also writes 8 uint32 words per ray to `counters` (device, 8*n):
+ * fat BLAS nodes fetched, leaves entered, triangles tested, TLAS nodes fetched, instances
+ * entered, SBS voxels stepped, then the device wall clock (100 MHz ticks, low 32 bits) when
+ * the ray started and the ray's elapsed ticks. Same hits as the plain call. */
 lrt_status lrt_trace_closest_counted_dev(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,
                                          uint32_t* counters, uint64_t n, void* stream);
 
@@ -132,6 +133,13 @@ lrt_status lrt_eye_rays_dev(const float* proj_inv16, const float* world_view_inv
 /* Kernel configuration used by the trace calls: block size (64/128/256), number of traversal
  * stack entries kept in LDS (8/16/32), XCD-contiguous block mapping (0/1). */
 lrt_status lrt_set_launch_config(lrt_scene* s, int block, int lds_stack, int xcd_remap);
+
+/* Two-pass scheme of the trace calls: rays still unfinished after `budget` traversal steps
+ * (fat-node fetches plus leaf visits) in the one-ray-per-lane pass are finished by a second
+ * pass where `tail_waves` persistent waves each walk one ray cooperatively; both passes
+ * return the hit of the reference's sequential walk. budget 0 = single pass. Default 64/1024.
+ * No counterpart in the reference (its CPU loop has no wave tail). */
+lrt_status lrt_set_tail_config(lrt_scene* s, uint32_t budget, uint32_t tail_waves);
 
 /* Number of rays (since the last reset) whose traversal stack reached the reference limit of
  * 80 entries (raytrace_common.h:36); the reference does not detect this. */
