@@ -25,6 +25,9 @@ Function -> reference -> how it is pinned (tests/test_oracle_cpu.py unless state
       property (a bijection onto [0, w*h)).
   oracle_camera_inverse LiteMath.h:3315-3360, 2070-2154  pinned by M * inverse(M) = I in
       float64 and the closed-form perspective entries.
+  oracle_local_surface  BVH2Common.cpp:345-725 (one voxel)  pinned by float64 root finding on
+      the trilinear polynomial along the ray (first sign change), for every intersection mode
+      (tests/test_oracle_sdf.py).
 """
 from __future__ import annotations
 
@@ -78,6 +81,8 @@ def load(flavor: str = "exact") -> ctypes.CDLL:
         L.oracle_camera_inverse.restype = None
         L.oracle_inverse4x4.argtypes = [vp, vp]
         L.oracle_inverse4x4.restype = None
+        L.oracle_local_surface.argtypes = [vp, vp, f32, f32, f32, f32, vp, vp, vp]
+        L.oracle_local_surface.restype = i32
         _libs[flavor] = L
     return _libs[flavor]
 
@@ -168,3 +173,15 @@ def inverse4x4(m16: np.ndarray, flavor: str = "exact") -> np.ndarray:
     out = np.zeros(16, np.float32)
     load(flavor).oracle_inverse4x4(_p(m), _p(out))
     return out
+
+
+def local_surface(preset: np.ndarray, values, d: float, q_near: float, q_far: float, f_near: float, start_q, ray_dir,
+                  flavor: str = "exact"):
+    """One LocalSurfaceIntersection on a voxel of size d (identity instance); returns the hit record."""
+    raw = np.ascontiguousarray(preset)
+    v = np.ascontiguousarray(values, dtype=np.float32).reshape(8)
+    sq = np.ascontiguousarray(start_q, dtype=np.float32).reshape(3)
+    rd = np.ascontiguousarray(ray_dir, dtype=np.float32).reshape(3)
+    h = np.zeros((), HIT_DTYPE)
+    load(flavor).oracle_local_surface(_p(raw), _p(v), d, q_near, q_far, f_near, _p(sq), _p(rd), _p(h))
+    return h

@@ -255,6 +255,9 @@ struct Tracer {
       float tmp = s101 - s001;
       float k0 = s000, k1 = s100 - s000, k2 = s010 - s000, k3 = s110 - s010 - k1, k4 = k0 - s001, k5 = k1 - tmp;
       float k6 = k2 - (s011 - s001), k7 = k3 - (s111 - s011 - tmp);
+#ifdef ORACLE_PLANT_DROP_K7
+      k7 = 0.0f;   // planted mistake (the xyz term of the trilinear), compiled only by the test that catches it
+#endif
       f3 o = start_q, d3 = ray_dir;
       float m0 = o.x * o.y, m1 = d3.x * d3.y, m2 = o.x * d3.y + o.y * d3.x;
       float m3 = k5 * o.z - k1, m4 = k6 * o.z - k2, m5 = k7 * o.z - k3;
@@ -809,5 +812,32 @@ void oracle_camera_inverse(const float* eye, const float* target, const float* u
 }
 
 void oracle_inverse4x4(const float* m, float* out) { inverse4x4(m, out); }
+
+}  // extern "C"
+
+extern "C" {
+
+// One LocalSurfaceIntersection call (BVH2Common.cpp:345-725) on a unit voxel, with an identity
+// instance, for the float64 root checks: `values` are the 8 corner distances, start_q the entry
+// point in voxel coordinates, d the voxel size. Writes the hit record (t = fNear + d * t_local
+// when hit, else far = 1e9) and returns 1 on a hit.
+int oracle_local_surface(const void* preset, const float* values, float d, float qNear, float qFar, float fNear,
+                         const float* start_q, const float* ray_dir, void* hit_out) {
+  OracleScene sc{};
+  std::memcpy(&sc.preset, preset, sizeof(Preset));
+  Inst inst{};
+  for (int i = 0; i < 4; ++i) {
+    inst.transform[5 * i] = 1.0f;
+    inst.transformInv[5 * i] = 1.0f;
+    inst.transformInvTransposed[5 * i] = 1.0f;
+  }
+  sc.instances = &inst;
+  Tracer tr{sc, nullptr};
+  Hit h{1e9f, uint32_t(-1), uint32_t(-1), uint32_t(-1), {1, 0, 0, 0}};
+  tr.LocalSurfaceIntersection(TYPE_SDF_FRAME_OCTREE, f3(ray_dir[0], ray_dir[1], ray_dir[2]), 0, 0, values, 0, d, qNear,
+                              qFar, f2{fNear, 0.0f}, f3(start_q[0], start_q[1], start_q[2]), &h);
+  std::memcpy(hit_out, &h, sizeof(h));
+  return h.primId != uint32_t(-1) ? 1 : 0;
+}
 
 }  // extern "C"
