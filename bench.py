@@ -80,8 +80,15 @@ def algorithmic_bytes(cnt, hits, kind):
     return b
 
 
+def strided(a, k):
+    """Every n/k-th ray: a sample spread over the whole image (rays are in screen-tile order)."""
+    step = max(1, a.shape[0] // k)
+    return np.ascontiguousarray(a[::step][:k])
+
+
 def cpu_baseline(scene, pos, dir, threads):
-    """Oracle (reference flags) on the host cores over the same rays; median of 5 after 1 warm-up."""
+    """Oracle (reference flags) on the host cores over an evenly strided sample of the same rays;
+    median of 5 after 1 warm-up; plus a 1-thread run (the reference loop is serial)."""
     import oracle
     if not oracle.available("fma"):
         return None
@@ -93,17 +100,19 @@ def cpu_baseline(scene, pos, dir, threads):
         oracle.trace_closest(sc, p, d, threads=th, flavor="fma")
         return time.perf_counter() - t
 
-    first = run(pos[: min(n, 65536)], dir[: min(n, 65536)], threads)
-    est_full = first * n / min(n, 65536)
-    sample = n if est_full * 6 < 20.0 else max(65536, int(n * 20.0 / (6 * est_full)))
+    probe = min(n, 65536)
+    first = run(strided(pos, probe), strided(dir, probe), threads)
+    est_full = first * n / probe
+    sample = n if est_full * 6 < 20.0 else max(probe, int(n * 20.0 / (6 * est_full)))
     sample = min(sample, n)
-    p, d = pos[:sample], dir[:sample]
+    p, d = strided(pos, sample), strided(dir, sample)
     run(p, d, threads)
     times = sorted(run(p, d, threads) for _ in range(5))
-    mrays = sample / times[2] / 1e6
-    s1 = min(sample, 65536)
-    run(p[:s1], d[:s1], 1)
-    t1 = sorted(run(p[:s1], d[:s1], 1) for _ in range(3))[1]
+    mrays = p.shape[0] / times[2] / 1e6
+    s1 = min(p.shape[0], 32768)
+    p1, d1 = strided(p, s1), strided(d, s1)
+    run(p1, d1, 1)
+    t1 = sorted(run(p1, d1, 1) for _ in range(3))[1]
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as fh:
@@ -114,18 +123,22 @@ def cpu_baseline(scene, pos, dir, threads):
     except OSError:
         pass
     return {"value": round(mrays, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"first {sample} of {n} rays of the same workload, median of 5 after 1 warm-up, "
-                      f"OpenMP dynamic(256); oracle built with the reference flags (-O3, FMA contraction)",
-            "value_1thread": round(s1 / t1 / 1e6, 3), "cpu_model": cpu_model}
+            "sample": f"{p.shape[0]} of {n} rays of the same workload (every {max(1, n // p.shape[0])}th), median "
+                      f"of 5 after 1 warm-up, OpenMP dynamic(256); oracle built with the reference flags (-O3, FMA "
+                      f"contraction); 1-thread value on {p1.shape[0]} of them",
+            "value_1thread": round(p1.shape[0] / t1 / 1e6, 3), "cpu_model": cpu_model}
 
 
 def load_traffic(workload):
-    path = os.path.join(ROOT, "profiles", f"pmc_{workload}.json")
+    """HBM bytes per step from the committed counter summary (profiles/traffic_<workload>.txt)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        d = json.load(fh)
-    return d.get("hbm_bytes_per_launch")
+        for line in fh:
+            if line.startswith("hbm_bytes_per_step:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def main():
@@ -139,6 +152,11 @@ def main():
     ap.add_argument("--lds-stack", type=int, default=16)
     ap.add_argument("--xcd-remap", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the given launch configuration")
+    ap.add_argument("--budget", type=int, default=64, help="tail step budget when not autotuning")
+    ap.add_argument("--tail-waves", type=int, default=1024, help="tail waves when not autotuning")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device copy (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the sharded multi-GPU path is "
@@ -152,6 +170,7 @@ def main():
     torch.cuda.set_device(0)
     scene, w, h, jitter, desc, kind, build_s = build_workload(args.workload, threads)
     scene.set_launch_config(args.block, args.lds_stack, bool(args.xcd_remap))
+    scene.set_tail_config(args.budget, args.tail_waves)
     n = w * h
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
@@ -169,6 +188,17 @@ def main():
     cnt = counts.cpu().numpy().view(np.uint32)
     hits_np = hits.cpu().numpy().view(g.HIT_DTYPE).reshape(-1)
     alg_bytes = algorithmic_bytes(cnt, hits_np, kind)
+
+    tune = None
+    if not args.no_autotune:
+        tune = scene.autotune(pos, dirs, hits, stream=sh)
+    if args.copy_calibration:
+        src = torch.empty(64 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)   # one elementwise kernel: reads 256 MiB, writes 256 MiB
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         scene.trace_closest(pos, dirs, hits, stream=sh)
@@ -188,9 +218,12 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, pos.cpu().numpy(), dirs.cpu().numpy(), threads)
 
+    launch = ({k: tune[k] for k in ("block", "lds_stack", "xcd_remap", "budget", "tail_waves")} if tune else
+              {"block": args.block, "lds_stack": args.lds_stack, "xcd_remap": args.xcd_remap, "budget": args.budget,
+               "tail_waves": args.tail_waves})
     desc.update({"hit_fraction": round(float((hits_np["primId"] != g.MISS).mean()), 4),
-                 "scene_build_s": round(build_s, 2), "block": args.block, "lds_stack": args.lds_stack,
-                 "xcd_remap": args.xcd_remap, "stack_overflow_rays": overflow,
+                 "scene_build_s": round(build_s, 2), "launch": launch, "autotuned": tune is not None,
+                 "stack_overflow_rays": overflow,
                  "mean_fat_nodes_per_ray": round(float(cnt[:, 0].mean()), 2),
                  "algorithmic_bytes_per_ray": round(alg_bytes / n, 1)})
     out = {

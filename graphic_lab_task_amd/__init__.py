@@ -77,6 +77,12 @@ class LrtError(RuntimeError):
         self.code = code
 
 
+class _TuneResult(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int32), ("lds_stack", ctypes.c_int32), ("xcd_remap", ctypes.c_int32),
+                ("budget", ctypes.c_uint32), ("tail_waves", ctypes.c_uint32), ("best_ms", ctypes.c_float),
+                ("configs_tried", ctypes.c_uint32)]
+
+
 class _Buffer(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("bytes", ctypes.c_size_t)]
 
@@ -123,6 +129,7 @@ def lib() -> ctypes.CDLL:
             "lrt_eye_rays_dev": ([vp, vp, u32, u32, vp, u32, vp, vp, vp], i32),
             "lrt_set_launch_config": ([vp, i32, i32, i32], i32),
             "lrt_set_tail_config": ([vp, u32, u32], i32),
+            "lrt_autotune": ([vp, vp, vp, vp, u64, vp, vp], i32),
             "lrt_stack_overflow_count": ([vp, ctypes.POINTER(u64), i32], i32),
             "lrt_get_array": ([vp, cp, vp, ctypes.POINTER(sz)], i32),
             "lrt_buffer_free": ([ctypes.POINTER(_Buffer)], None),
@@ -130,7 +137,7 @@ def lib() -> ctypes.CDLL:
             "lrt_gen_icosphere": ([u32, f32, f32, u64, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
             "lrt_gen_geodesic": ([u32, f32, f32, u64, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
             "lrt_sdf_spheres_frame_octree": ([vp, sz, u32, i32, ctypes.POINTER(_Buffer)], i32),
-            "lrt_sdf_spheres_sbs": ([vp, sz, u32, vp, i32, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
+            "lrt_sdf_spheres_sbs": ([vp, sz, u32, vp, u64, i32, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
             "lrt_sdf_spheres_sbs_indexed": ([vp, sz, u32, u32, u64, i32, ctypes.POINTER(_Buffer),
                                              ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer), pu32], i32),
         }
@@ -147,7 +154,7 @@ def exported_symbols():
             "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_sbs", "lrt_add_instance",
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
             "lrt_trace_closest_host", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
-            "lrt_set_launch_config", "lrt_set_tail_config", "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
+            "lrt_set_launch_config", "lrt_set_tail_config", "lrt_autotune", "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
             "lrt_camera_inverse", "lrt_gen_icosphere", "lrt_gen_geodesic", "lrt_sdf_spheres_frame_octree",
             "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed"]
 
@@ -295,6 +302,13 @@ class SceneRT:
         _check(lib().lrt_trace_closest_counted_dev(self._h, _ptr(pos), _ptr(dir), _ptr(hits), _ptr(counters), n,
                                                    stream))
 
+    def autotune(self, pos, dir, hits, stream: int = 0) -> dict:
+        """Times the launch configurations on this device ray batch and keeps the fastest."""
+        n = pos.numel() // 4
+        r = _TuneResult()
+        _check(lib().lrt_autotune(self._h, _ptr(pos), _ptr(dir), _ptr(hits), n, stream, ctypes.byref(r)))
+        return {f: getattr(r, f) for f, _ in _TuneResult._fields_}
+
     def trace_any(self, pos, dir, flags, stream: int = 0):
         n = pos.numel() // 4
         _check(lib().lrt_trace_any_dev(self._h, _ptr(pos), _ptr(dir), _ptr(flags), n, stream))
@@ -374,11 +388,11 @@ def sdf_spheres_frame_octree(spheres: np.ndarray, depth: int, threads: int = 0) 
     return _take(nb, FRAME_NODE_DTYPE)
 
 
-def sdf_spheres_sbs(spheres: np.ndarray, depth: int, header: np.ndarray, threads: int = 0):
+def sdf_spheres_sbs(spheres: np.ndarray, depth: int, header: np.ndarray, threads: int = 0, colour_seed: int = 0):
     s = np.ascontiguousarray(spheres, dtype=np.float32).reshape(-1, 4)
     h = np.ascontiguousarray(header, dtype=SBS_HEADER_DTYPE).reshape(())
     nb, vb = _Buffer(), _Buffer()
-    _check(lib().lrt_sdf_spheres_sbs(_ptr(s), s.shape[0], depth, _ptr(h), threads, ctypes.byref(nb),
+    _check(lib().lrt_sdf_spheres_sbs(_ptr(s), s.shape[0], depth, _ptr(h), colour_seed, threads, ctypes.byref(nb),
                                      ctypes.byref(vb)))
     return _take(nb, SBS_NODE_DTYPE), _take(vb, np.uint32)
 

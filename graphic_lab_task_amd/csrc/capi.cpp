@@ -9,6 +9,7 @@
 #include "host_math.h"
 #include "kernels.h"
 #include "scene.h"
+#include "tune.h"
 
 struct lrt_scene {
   lrt::Scene s;
@@ -297,6 +298,47 @@ lrt_status lrt_set_tail_config(lrt_scene* s, uint32_t budget, uint32_t tail_wave
     if (tail_waves > 65536) return fail(LRT_ERR_INVALID_ARG, "tail_waves must be <= 65536");
     s->s.budget = budget;
     s->s.tail_waves = tail_waves ? tail_waves : 1024u;
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits, uint64_t n,
+                        void* stream, lrt_tune_result* out) {
+  return guarded([&]() -> lrt_status {
+    if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+    if (!s->s.committed) return fail(LRT_ERR_NOT_COMMITTED, "scene is not committed");
+    if (!pos_near || !dir_far || !hits || n == 0) return fail(LRT_ERR_INVALID_ARG, "autotune needs a non-empty ray batch");
+    lrt::Scene& sc = s->s;
+    const lrt::TuneConfig start{sc.block, sc.lds_stack, sc.xcd_remap, sc.budget, sc.tail_waves};
+    auto apply = [&](const lrt::TuneConfig& c) {
+      sc.block = c.block;
+      sc.lds_stack = c.lds_stack;
+      sc.xcd_remap = c.xcd_remap;
+      sc.budget = c.budget;
+      sc.tail_waves = c.tail_waves;
+    };
+    lrt_status st = LRT_OK;
+    auto launch = [&](const lrt::TuneConfig& c) -> hipError_t {
+      apply(c);
+      st = trace_common(s, pos_near, dir_far, hits, nullptr, n, stream);
+      return st == LRT_OK ? hipSuccess : hipErrorLaunchFailure;
+    };
+    lrt::TuneResult r{};
+    const hipError_t e = lrt::autotune(launch, reinterpret_cast<hipStream_t>(stream), start, 3, &r);
+    if (e != hipSuccess) {
+      apply(start);
+      return st != LRT_OK ? st : hip_fail(e, "autotune");
+    }
+    apply(r.best);
+    if (out) {
+      out->block = r.best.block;
+      out->lds_stack = r.best.lds_stack;
+      out->xcd_remap = r.best.xcd_remap;
+      out->budget = r.best.budget;
+      out->tail_waves = r.best.tail_waves;
+      out->best_ms = r.best_ms;
+      out->configs_tried = r.tried;
+    }
     return LRT_OK;
   });
 }

@@ -37,8 +37,9 @@ void sdf_to_global_octree(const SphereUnion& sdf, uint32_t depth, uint32_t brick
 // global_octree_to_frame_octree (sdf/simple/sdf_converter.cpp:53-65)
 void global_octree_to_frame_octree(const GlobalOctree& g, std::vector<SdfFrameOctreeNode>& out);
 // global_octree_to_SBS (sdf/simple/sdf_converter.cpp:156-225), quantised DX layout
-void global_octree_to_sbs_dx(const GlobalOctree& g, const SdfSBSHeader& header, std::vector<SdfSBSNode>& nodes,
-                             std::vector<uint32_t>& values);
+// colour_seed != 0 fills the per-corner words of the DX_UV16 / DX_RGB8 layouts from a seeded field.
+void global_octree_to_sbs_dx(const GlobalOctree& g, const SdfSBSHeader& header, uint64_t colour_seed,
+                             std::vector<SdfSBSNode>& nodes, std::vector<uint32_t>& values);
 // Indexed fp32 layout (ID32F_IRGB32F): shared lattice points stored once in values_f, addressed by
 // v_size^3 indices per brick, followed by 8 colour indices per brick (brick corners, 3 floats each).
 // values_f = [N distances][N albedo RGB triples]; the albedo is a seeded smooth colour field.

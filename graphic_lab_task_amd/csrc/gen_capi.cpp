@@ -99,8 +99,8 @@ lrt_status lrt_sdf_spheres_frame_octree(const float* spheres, size_t n_spheres, 
   });
 }
 
-lrt_status lrt_sdf_spheres_sbs(const float* spheres, size_t n_spheres, uint32_t depth, const void* header, int threads,
-                               lrt_buffer* nodes, lrt_buffer* values) {
+lrt_status lrt_sdf_spheres_sbs(const float* spheres, size_t n_spheres, uint32_t depth, const void* header,
+                               uint64_t colour_seed, int threads, lrt_buffer* nodes, lrt_buffer* values) {
   if (!spheres || n_spheres == 0 || !header || !nodes || !values) return LRT_ERR_INVALID_ARG;
   return guarded([&]() -> lrt_status {
     const lrt::SdfSBSHeader h = *static_cast<const lrt::SdfSBSHeader*>(header);
@@ -109,7 +109,7 @@ lrt_status lrt_sdf_spheres_sbs(const float* spheres, size_t n_spheres, uint32_t 
     lrt::sdf_to_global_octree(lrt::SphereUnion{spheres, n_spheres}, depth, h.brick_size, h.brick_pad, threads, g);
     std::vector<lrt::SdfSBSNode> n;
     std::vector<uint32_t> v;
-    lrt::global_octree_to_sbs_dx(g, h, n, v);
+    lrt::global_octree_to_sbs_dx(g, h, colour_seed, n, v);
     lrt_status st = to_buffer(n, nodes);
     if (st != LRT_OK) return st;
     return to_buffer(v, values);

@@ -158,7 +158,16 @@ void global_octree_to_frame_octree(const GlobalOctree& g, std::vector<SdfFrameOc
 
 namespace {
 
-void sbs_dx_rec(const GlobalOctree& g, const SdfSBSHeader& h, std::vector<SdfSBSNode>& nodes,
+// Seeded smooth field in [0,1] at a brick corner (value noise), channel c.
+double corner_field(uint32_t lod, uint32_t px, uint32_t py, uint32_t pz, uint32_t corner, uint64_t seed, uint64_t c) {
+  const double s = 2.0 / lod;
+  const double x = -1.0 + s * (px + ((corner & 4) ? 1 : 0));
+  const double y = -1.0 + s * (py + ((corner & 2) ? 1 : 0));
+  const double z = -1.0 + s * (pz + ((corner & 1) ? 1 : 0));
+  return std::min(1.0, std::max(0.0, 0.5 + 0.45 * value_noise3(2.0 * x, 2.0 * y, 2.0 * z, seed * 3 + c)));
+}
+
+void sbs_dx_rec(const GlobalOctree& g, const SdfSBSHeader& h, uint64_t colour_seed, std::vector<SdfSBSNode>& nodes,
                 std::vector<uint32_t>& values, uint32_t idx, uint32_t lod, uint32_t px, uint32_t py, uint32_t pz) {
   const GONode& n = g.nodes[idx];
   if (n.offset == 0) {
@@ -170,7 +179,9 @@ void sbs_dx_rec(const GlobalOctree& g, const SdfSBSHeader& h, std::vector<SdfSBS
     const uint32_t bits = 8 * h.bytes_per_value;
     const uint32_t max_val = h.bytes_per_value == 4 ? 0xFFFFFFFFu : ((1u << bits) - 1);
     const uint32_t vpi = 4 / h.bytes_per_value;
-    const uint32_t texs = (h.aux_data & SDF_SBS_NODE_LAYOUT_MASK) == SDF_SBS_NODE_LAYOUT_DX_UV16 ? 32 : 0;
+    const uint32_t layout = h.aux_data & SDF_SBS_NODE_LAYOUT_MASK;
+    // DX_UV16 reserves 32 words (8 used), as the reference converter does; DX_RGB8 needs 8
+    const uint32_t texs = layout == SDF_SBS_NODE_LAYOUT_DX_UV16 ? 32 : (layout == SDF_SBS_NODE_LAYOUT_DX_RGB8 ? 8 : 0);
     const uint32_t dist_size = (v3 + vpi - 1) / vpi;
     SdfSBSNode sn{};
     sn.data_offset = off;
@@ -184,9 +195,23 @@ void sbs_dx_rec(const GlobalOctree& g, const SdfSBSHeader& h, std::vector<SdfSBS
       c = std::min(c, max_val);
       values[off + i / vpi] |= c << (bits * (i % vpi) % 32);
     }
+    // per-corner texture coordinates (u16|v16) or colour (r8|g8<<8|b8<<16); zero without a seed
+    if (colour_seed != 0 && layout == SDF_SBS_NODE_LAYOUT_DX_UV16) {
+      for (uint32_t k = 0; k < 8; ++k) {
+        const uint32_t u = uint32_t(65535.0 * corner_field(lod, px, py, pz, k, colour_seed, 0));
+        const uint32_t v = uint32_t(65535.0 * corner_field(lod, px, py, pz, k, colour_seed, 1));
+        values[off + dist_size + k] = (u << 16) | (v & 0xFFFFu);
+      }
+    } else if (colour_seed != 0 && layout == SDF_SBS_NODE_LAYOUT_DX_RGB8) {
+      for (uint32_t k = 0; k < 8; ++k) {
+        uint32_t w = 0;
+        for (uint32_t c = 0; c < 3; ++c) w |= uint32_t(255.0 * corner_field(lod, px, py, pz, k, colour_seed, c) + 0.5) << (8 * c);
+        values[off + dist_size + k] = w;
+      }
+    }
   } else {
     for (uint32_t i = 0; i < 8; i++)
-      sbs_dx_rec(g, h, nodes, values, n.offset + i, 2 * lod, 2 * px + ((i & 4) >> 2), 2 * py + ((i & 2) >> 1),
+      sbs_dx_rec(g, h, colour_seed, nodes, values, n.offset + i, 2 * lod, 2 * px + ((i & 4) >> 2), 2 * py + ((i & 2) >> 1),
                  2 * pz + (i & 1));
   }
 }
@@ -209,13 +234,13 @@ void collect_bricks(const GlobalOctree& g, std::vector<BrickRef>& out, uint32_t 
 }  // namespace
 
 // global_octree_to_SBS_rec (sdf/simple/sdf_converter.cpp:156-215)
-void global_octree_to_sbs_dx(const GlobalOctree& g, const SdfSBSHeader& header, std::vector<SdfSBSNode>& nodes,
-                             std::vector<uint32_t>& values) {
+void global_octree_to_sbs_dx(const GlobalOctree& g, const SdfSBSHeader& header, uint64_t colour_seed,
+                             std::vector<SdfSBSNode>& nodes, std::vector<uint32_t>& values) {
   if (header.brick_size != g.brick_size || header.brick_pad != g.brick_pad)
     throw std::invalid_argument("header brick size/pad differ from the octree's");
   nodes.clear();
   values.clear();
-  sbs_dx_rec(g, header, nodes, values, 0, 1, 0, 0, 0);
+  sbs_dx_rec(g, header, colour_seed, nodes, values, 0, 1, 0, 0, 0);
 }
 
 // Indexed fp32 conversion (our replacement for SBS_col_to_SBS_ind, sdf_converter_exotic.cpp:432-518,

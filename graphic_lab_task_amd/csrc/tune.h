@@ -1,0 +1,28 @@
+// Launch-configuration autotuner for the trace kernels. Every configuration returns identical
+// hit records, so tuning on the caller's own rays is safe: it only changes the time.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+
+namespace lrt {
+
+struct TuneConfig {
+  int block, lds_stack, xcd_remap;
+  uint32_t budget, tail_waves;
+};
+
+struct TuneResult {
+  TuneConfig best;
+  float best_ms;
+  uint32_t tried;
+};
+
+// Two stages, each candidate timed with HIP events on `stream` (one warm-up launch, then the
+// minimum of `reps`): (1) block size x LDS stack entries with the single-pass walk, (2) step
+// budget x tail waves of the two-pass scheme on the best shape from (1).
+hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, hipStream_t stream,
+                    const TuneConfig& start, int reps, TuneResult* out);
+
+}  // namespace lrt

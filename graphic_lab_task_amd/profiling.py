@@ -1,0 +1,123 @@
+"""Summaries of rocprofv3 output for the trace kernels (text, for the profiles/ folder).
+
+  python -m graphic_lab_task_amd.profiling stats <dir> --command "<profiled command>"
+      per-kernel calls / average / median / min duration from <dir>/*kernel_trace.csv
+  python -m graphic_lab_task_amd.profiling traffic <fetch_dir> <write_dir> --workload W --command "..."
+      HBM bytes per trace call from two --pmc runs (FETCH_SIZE, then WRITE_SIZE, KiB per
+      dispatch); both counters are scaled by what they read for the 256 MiB elementwise copy
+      (torch.neg) that `bench.py --copy-calibration` runs in the same process: known bytes
+      (256 MiB read + 256 MiB written) / counted bytes.
+"""
+import argparse
+import csv
+import glob
+import os
+import statistics
+import sys
+from collections import defaultdict
+
+TRACE_KERNELS = ("k_trace_closest", "k_trace_tail", "k_tail_reset")
+COPY_BYTES = 2 * (64 << 20) * 4          # read + write of the 256 MiB float copy
+
+
+def _rows(directory, suffix):
+    files = sorted(glob.glob(os.path.join(directory, "**", f"*{suffix}"), recursive=True))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {directory}")
+    with open(files[0], newline="") as fh:
+        return list(csv.DictReader(fh))
+
+
+def _col(row, *names):
+    low = {k.lower(): k for k in row}
+    for n in names:
+        if n.lower() in low:
+            return row[low[n.lower()]]
+    raise KeyError(names)
+
+
+def short_name(name):
+    for k in TRACE_KERNELS:
+        if k in name:
+            return name[name.index(k):].split("(")[0]
+    return name.split("(")[0][:80]
+
+
+def kernel_stats(directory):
+    by = defaultdict(list)
+    for r in _rows(directory, "kernel_trace.csv"):
+        name = _col(r, "Kernel_Name", "KernelName", "Name")
+        dur = int(_col(r, "End_Timestamp", "End")) - int(_col(r, "Start_Timestamp", "Start"))
+        by[short_name(name)].append(dur / 1e3)
+    return by
+
+
+def cmd_stats(a):
+    by = kernel_stats(a.dir)
+    out = [f"# rocprofv3 kernel trace summary", f"command: {a.command}", "",
+           f"{'kernel':70s} {'calls':>6s} {'avg_us':>10s} {'median_us':>10s} {'min_us':>10s} {'total_us':>12s}"]
+    for name, d in sorted(by.items(), key=lambda kv: -sum(kv[1])):
+        out.append(f"{name:70s} {len(d):6d} {statistics.mean(d):10.2f} {statistics.median(d):10.2f} "
+                   f"{min(d):10.2f} {sum(d):12.1f}")
+    print("\n".join(out))
+
+
+def _counter_per_dispatch(directory, counter):
+    """{short kernel name: [value per dispatch]} for one counter (summed over its instances)."""
+    per = defaultdict(float)
+    names = {}
+    for r in _rows(directory, "counter_collection.csv"):
+        if _col(r, "Counter_Name") != counter:
+            continue
+        did = _col(r, "Dispatch_Id")
+        per[did] += float(_col(r, "Counter_Value"))
+        names[did] = short_name(_col(r, "Kernel_Name"))
+    by = defaultdict(list)
+    for did, v in per.items():
+        by[names[did]].append(v)
+    return by
+
+
+def cmd_traffic(a):
+    fetch = _counter_per_dispatch(a.fetch_dir, "FETCH_SIZE")
+    write = _counter_per_dispatch(a.write_dir, "WRITE_SIZE")
+    copy_names = [k for k in fetch if "neg" in k.lower()]
+    if not copy_names:
+        raise SystemExit("no neg kernel in the FETCH_SIZE run (run bench.py with --copy-calibration)")
+    ck = copy_names[0]
+    kib_copy = statistics.median(fetch[ck]) + statistics.median(write.get(ck, [0.0]))
+    scale = COPY_BYTES / (kib_copy * 1024.0)
+    lines = [f"# HBM traffic per trace call from rocprofv3 counters", f"command: {a.command}",
+             f"copy reference kernel: {ck}: FETCH+WRITE {kib_copy:.1f} KiB per dispatch for {COPY_BYTES} bytes "
+             f"-> scale {scale:.4f}", ""]
+    total = 0.0
+    for k in TRACE_KERNELS:
+        f = [v for name, vs in fetch.items() if name.startswith(k) for v in vs]
+        w = [v for name, vs in write.items() if name.startswith(k) for v in vs]
+        if not f:
+            continue
+        fb = statistics.median(f) * 1024.0 * scale
+        wb = statistics.median(w) * 1024.0 * scale if w else 0.0
+        total += fb + wb
+        lines.append(f"{k:20s} dispatches {len(f):4d}  fetch {fb / 1e6:10.3f} MB  write {wb / 1e6:10.3f} MB (median)")
+    lines += ["", f"workload: {a.workload}", f"hbm_bytes_per_step: {total:.0f}"]
+    print("\n".join(lines))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("dir")
+    s.add_argument("--command", required=True)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_dir")
+    t.add_argument("write_dir")
+    t.add_argument("--workload", required=True)
+    t.add_argument("--command", required=True)
+    a = ap.parse_args(argv)
+    {"stats": cmd_stats, "traffic": cmd_traffic}[a.cmd](a)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

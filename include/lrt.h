@@ -141,6 +141,21 @@ lrt_status lrt_set_launch_config(lrt_scene* s, int block, int lds_stack, int xcd
  * No counterpart in the reference (its CPU loop has no wave tail). */
 lrt_status lrt_set_tail_config(lrt_scene* s, uint32_t budget, uint32_t tail_waves);
 
+/* Result of lrt_autotune. */
+typedef struct lrt_tune_result {
+  int32_t block, lds_stack, xcd_remap;
+  uint32_t budget, tail_waves;
+  float best_ms;           /* fastest time of one closest-hit call on the tuning batch */
+  uint32_t configs_tried;
+} lrt_tune_result;
+
+/* Chooses the launch configuration (block size, LDS stack entries, step budget, tail waves) by
+ * timing closest-hit calls on the caller's device ray batch (`hits` is overwritten with the
+ * batch's hit records), and keeps the fastest on the scene. Every configuration returns
+ * identical records. No counterpart in the reference (its CPU loop has nothing to tune). */
+lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits, uint64_t n,
+                        void* stream, lrt_tune_result* out);
+
 /* Number of rays (since the last reset) whose traversal stack reached the reference limit of
  * 80 entries (raytrace_common.h:36); the reference does not detect this. */
 lrt_status lrt_stack_overflow_count(lrt_scene* s, uint64_t* out, int reset);

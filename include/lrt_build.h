@@ -48,10 +48,12 @@ lrt_status lrt_sdf_spheres_frame_octree(const float* spheres, size_t n_spheres, 
                                         lrt_buffer* nodes);
 
 /* sdf_converter::create_sdf_SBS (sdf/simple/sdf_converter.cpp:311-322) for the sphere union:
- * quantised DX / DX_UV16 layouts. header: SdfSBSHeader (brick_size, brick_pad, bytes_per_value,
- * aux_data). nodes: SdfSBSNode array, values: uint32 array. */
-lrt_status lrt_sdf_spheres_sbs(const float* spheres, size_t n_spheres, uint32_t depth, const void* header, int threads,
-                               lrt_buffer* nodes, lrt_buffer* values);
+ * quantised DX / DX_UV16 / DX_RGB8 layouts. header: SdfSBSHeader (brick_size, brick_pad,
+ * bytes_per_value, aux_data). colour_seed != 0 fills the 8 per-corner texture (u16|v16) or
+ * colour (r8|g8|b8) words from a seeded smooth field; 0 leaves them zero, as the reference
+ * converter does without a texture-coordinate channel. nodes: SdfSBSNode array, values: uint32. */
+lrt_status lrt_sdf_spheres_sbs(const float* spheres, size_t n_spheres, uint32_t depth, const void* header,
+                               uint64_t colour_seed, int threads, lrt_buffer* nodes, lrt_buffer* values);
 
 /* Indexed fp32 sparse brick set (layout ID32F_IRGB32F) for the sphere union: lattice points
  * shared by bricks are stored once. values_f = [N distances][N albedo RGB] with a seeded

@@ -298,3 +298,58 @@ def test_tail_pass_matches_single_pass(lrt, torch_dev, budget, waves):
         torch_dev.cuda.synchronize()
         assert raw_equal(got, base), f"{name}: {np.sum(got['primId'] != base['primId'])} id mismatches"
         assert torch_dev.equal(flags, flags0), name
+
+
+def _sbs_scene(lrt, kind, depth, bs, bpv=1):
+    sp = sphere_set()
+    s = lrt.SceneRT()
+    if kind == "indexed":
+        h, n, v, vf, nd = lrt.sdf_spheres_sbs_indexed(sp, depth, bs, 1)
+        s.AddInstance(s.AddGeom_SdfSBS(h, n, v, vf))
+    else:
+        layout = {"dx": lrt.SDF_SBS_NODE_LAYOUT_DX, "rgb8": lrt.SDF_SBS_NODE_LAYOUT_DX_RGB8,
+                  "uv16": lrt.SDF_SBS_NODE_LAYOUT_DX_UV16}[kind]
+        h = np.zeros((), lrt.SBS_HEADER_DTYPE)
+        h["brick_size"], h["brick_pad"], h["bytes_per_value"], h["aux_data"] = bs, 0, bpv, layout
+        n, v = lrt.sdf_spheres_sbs(sp, depth, h, colour_seed=7)
+        s.AddInstance(s.AddGeom_SdfSBS(h, n, v))
+    s.CommitScene()
+    return s
+
+
+@pytest.mark.parametrize("kind,depth,bs,bpv", [("indexed", 5, 8, 4), ("dx", 5, 8, 1), ("dx", 4, 8, 2),
+                                               ("rgb8", 4, 4, 1), ("uv16", 6, 2, 1), ("indexed", 8, 1, 4)])
+def test_sdf_brick_set_parity(lrt, orc, torch_dev, kind, depth, bs, bpv):
+    """Sparse brick sets (voxel march inside each brick, BVH2Common.cpp:884-1112), the 256^3
+    8^3-brick configuration among them."""
+    _say(f"sbs {kind} depth={depth} brick={bs} bytes={bpv}")
+    s = _sbs_scene(lrt, kind, depth, bs, bpv)
+    jit = np.random.default_rng(5).random((256 * 256, 2), dtype=np.float32)
+    pi, vi, pos, dirs = dev_rays(torch_dev, lrt, 256, 256, jitter=jit)
+    got = gpu_trace(torch_dev, s, pos, dirs)
+    p, d = pos.cpu().numpy(), dirs.cpu().numpy()
+    want, ov = orc.trace_closest(orc.Scene.from_scene(s), p, d)
+    r = sdf_compare(got, want, p, d)
+    _say(str(r))
+    assert r["hits"] > 0.1 * r["n"]
+    assert r["hit_flip"] <= 2e-5 * r["n"] + 2
+    assert r["id_diff_beyond_tie"] <= 2e-5 * r["n"] + 2
+    assert r["t_rel_gt_1e-5"] <= 2e-5 * r["n"] + 2
+    same_t = (got["t"] == want["t"]) & (got["primId"] != MISS)
+    assert np.mean(same_t[got["primId"] != MISS]) > 0.5
+    # colour / texture coordinates follow the hit point: equal where t is equal
+    if kind in ("indexed", "rgb8", "uv16"):
+        assert np.array_equal(got["coords"][same_t, :2], want["coords"][same_t, :2])
+
+
+def test_autotune_keeps_results(lrt, orc, torch_dev):
+    _say("autotune")
+    v, f, s = mesh_scene(lrt, 5, 0.05)
+    pi, vi, pos, dirs = dev_rays(torch_dev, lrt, 256, 256)
+    base = gpu_trace(torch_dev, s, pos, dirs)
+    hits = torch_dev.zeros((pos.shape[0], 8), dtype=torch_dev.int32, device="cuda")
+    r = s.autotune(pos, dirs, hits)
+    torch_dev.cuda.synchronize()
+    assert r["configs_tried"] >= 10 and r["best_ms"] > 0 and r["block"] in (64, 128, 256)
+    assert raw_equal(hits.cpu().numpy().view(lrt.HIT_DTYPE).reshape(-1), base)
+    assert raw_equal(gpu_trace(torch_dev, s, pos, dirs), base)

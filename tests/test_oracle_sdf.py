@@ -306,3 +306,39 @@ def test_brick_colour_is_trilinear_albedo(lrt, orc, camera_batch):
     b = hits["coords"][hit, 1]
     got = np.stack([r, g, b], 1)
     assert np.abs(got - want).max() <= 1.0 and np.mean(got == want) > 0.97
+
+
+@pytest.mark.parametrize("layout", ["rgb8", "uv16"])
+def test_quantised_brick_colour_and_texcoords(lrt, orc, camera_batch, layout):
+    """DX_RGB8: 8-bit corner colours, trilinear, rounded (coords[0] = r + g/256, coords[1] = b);
+    DX_UV16: 16-bit corner texture coordinates, trilinear (coords = u, v)
+    (BVH2Common.cpp:1035-1109)."""
+    sp = sphere_set()
+    pos, dir = camera_batch
+    bs, bpv = 4, 1
+    h = np.zeros((), lrt.SBS_HEADER_DTYPE)
+    h["brick_size"], h["brick_pad"], h["bytes_per_value"] = bs, 0, bpv
+    h["aux_data"] = lrt.SDF_SBS_NODE_LAYOUT_DX_RGB8 if layout == "rgb8" else lrt.SDF_SBS_NODE_LAYOUT_DX_UV16
+    n, v = lrt.sdf_spheres_sbs(sp, 5, h, colour_seed=7)
+    s = host_scene(lrt, lambda s: s.AddInstance(s.AddGeom_SdfSBS(h, n, v)))
+    hits = orc.trace_closest(orc.Scene.from_scene(s), pos, dir)[0]
+    hit = hits["primId"] != MISS
+    assert hit.sum() > 1000
+    nodes = n[hits["primId"][hit]]
+    lod = (nodes["pos_z_lod_size"] & 0xFFFF).astype(np.float64)
+    bp = np.stack([nodes["pos_xy"] >> 16, nodes["pos_xy"] & 0xFFFF, nodes["pos_z_lod_size"] >> 16], 1).astype(np.float64)
+    bmin = -1.0 + (2.0 / lod)[:, None] * bp
+    p = pos[hit, :3].astype(np.float64) + hits["t"][hit, None].astype(np.float64) * dir[hit, :3]
+    w = np.stack([trilinear(np.eye(8)[c], (p - bmin) * (0.5 * lod)[:, None]) for c in range(8)], 1)
+    dist_words = ((bs + 1) ** 3 + 4 // bpv - 1) // (4 // bpv)
+    words = np.stack([v[nodes["data_offset"] + dist_words + c] for c in range(8)], 1).astype(np.uint64)
+    if layout == "rgb8":
+        rgb = np.stack([(words >> (8 * ch)) & 0xFF for ch in range(3)], 2).astype(np.float64)
+        want = np.clip(np.floor(np.einsum("hc,hck->hk", w, rgb) + 0.5), 0, 255)
+        r = np.floor(hits["coords"][hit, 0])
+        got = np.stack([r, np.round((hits["coords"][hit, 0] - r) * 256.0), hits["coords"][hit, 1]], 1)
+        assert np.abs(got - want).max() <= 1.0 and np.mean(got == want) > 0.97
+    else:
+        uv = np.stack([(words >> 16) & 0xFFFF, words & 0xFFFF], 2).astype(np.float64) / 65535.0
+        want = np.einsum("hc,hck->hk", w, uv)
+        assert np.abs(hits["coords"][hit, :2] - want).max() < 2e-3
