@@ -130,6 +130,11 @@ def lib() -> ctypes.CDLL:
             "lrt_set_launch_config": ([vp, i32, i32, i32], i32),
             "lrt_set_tail_config": ([vp, u32, u32], i32),
             "lrt_autotune": ([vp, vp, vp, vp, u64, vp, vp], i32),
+            "lrt_sbs_param_layout": ([vp, u32, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u64)], i32),
+            "lrt_sbs_forward_dev": ([vp, u32, vp, vp, vp, u64, vp, vp], i32),
+            "lrt_sbs_backward_dev": ([vp, u32, vp, vp, vp, u64, vp, vp, vp], i32),
+            "lrt_sbs_l2_backward_dev": ([vp, u32, vp, vp, vp, u64, vp, vp, vp, vp], i32),
+            "lrt_sbs_recompute_misses": ([vp, ctypes.POINTER(u64), i32], i32),
             "lrt_stack_overflow_count": ([vp, ctypes.POINTER(u64), i32], i32),
             "lrt_get_array": ([vp, cp, vp, ctypes.POINTER(sz)], i32),
             "lrt_buffer_free": ([ctypes.POINTER(_Buffer)], None),
@@ -154,7 +159,9 @@ def exported_symbols():
             "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_sbs", "lrt_add_instance",
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
             "lrt_trace_closest_host", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
-            "lrt_set_launch_config", "lrt_set_tail_config", "lrt_autotune", "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
+            "lrt_set_launch_config", "lrt_set_tail_config", "lrt_autotune", "lrt_sbs_param_layout",
+            "lrt_sbs_forward_dev", "lrt_sbs_backward_dev", "lrt_sbs_l2_backward_dev", "lrt_sbs_recompute_misses",
+            "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
             "lrt_camera_inverse", "lrt_gen_icosphere", "lrt_gen_geodesic", "lrt_sdf_spheres_frame_octree",
             "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed"]
 
@@ -308,6 +315,31 @@ class SceneRT:
         r = _TuneResult()
         _check(lib().lrt_autotune(self._h, _ptr(pos), _ptr(dir), _ptr(hits), n, stream, ctypes.byref(r)))
         return {f: getattr(r, f) for f, _ in _TuneResult._fields_}
+
+    # ---- differentiable sparse brick set ----
+    def sbs_param_layout(self, geom_id: int):
+        """(f_offset, n_dist, n_params): the geometry's slice of values_f holds n_dist distances
+        then n_dist RGB albedos; parameter arrays span the scene's whole values_f (n_params)."""
+        off, nd, npar = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        _check(lib().lrt_sbs_param_layout(self._h, geom_id, ctypes.byref(off), ctypes.byref(nd), ctypes.byref(npar)))
+        return off.value, nd.value, npar.value
+
+    def sbs_forward(self, geom_id, params, pos, dir, colour, stream: int = 0):
+        _check(lib().lrt_sbs_forward_dev(self._h, geom_id, _ptr(params), _ptr(pos), _ptr(dir), pos.numel() // 4,
+                                         _ptr(colour), stream))
+
+    def sbs_backward(self, geom_id, params, pos, dir, dL_dcolour, grad, stream: int = 0):
+        _check(lib().lrt_sbs_backward_dev(self._h, geom_id, _ptr(params), _ptr(pos), _ptr(dir), pos.numel() // 4,
+                                          _ptr(dL_dcolour), _ptr(grad), stream))
+
+    def sbs_l2_backward(self, geom_id, params, pos, dir, target, grad, loss=None, stream: int = 0):
+        _check(lib().lrt_sbs_l2_backward_dev(self._h, geom_id, _ptr(params), _ptr(pos), _ptr(dir), pos.numel() // 4,
+                                             _ptr(target), _ptr(grad), _ptr(loss), stream))
+
+    def sbs_recompute_misses(self, reset: bool = False) -> int:
+        v = ctypes.c_uint64()
+        _check(lib().lrt_sbs_recompute_misses(self._h, ctypes.byref(v), int(reset)))
+        return v.value
 
     def trace_any(self, pos, dir, flags, stream: int = 0):
         n = pos.numel() // 4

@@ -83,6 +83,75 @@ lrt_status trace_common(lrt_scene* s, const float* pos, const float* dir, lrt_hi
   return LRT_OK;
 }
 
+// Resolves the differentiable brick-set geometry: indexed layout whose values_f slice holds
+// n_dist distances then n_dist RGB albedos (4 n_dist floats).
+lrt_status diff_geometry(lrt_scene* s, uint32_t geom_id, uint32_t* f_offset, uint32_t* n_dist) {
+  if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+  lrt::Scene& sc = s->s;
+  if (geom_id >= sc.geomData.size() || sc.geomTags[geom_id] != lrt::TAG_SDF_BRICK)
+    return fail(LRT_ERR_INVALID_ARG, "geometry is not a sparse brick set");
+  const uint32_t sdf = sc.geomData[geom_id].offset.x;
+  const lrt::uint2 sl = sc.sbsFSlice[sdf];
+  if (sl.y == 0 || sl.y % 4 != 0)
+    return fail(LRT_ERR_INVALID_ARG, "brick set needs the indexed fp32 layout with 4*n_dist values_f entries "
+                                     "(distances, then RGB albedo per lattice point)");
+  // every distance index of the geometry's bricks must address the distance part of its slice
+  // (checked once per geometry; the kernels derive albedo addresses from these indices)
+  if (sc.sbsDiffChecked.size() <= sdf) sc.sbsDiffChecked.resize(sdf + 1, 0);
+  if (sc.sbsDiffChecked[sdf] == 0) {
+    const lrt::SdfSBSHeader& h = sc.sbsHeaders[sdf];
+    const uint32_t v_size = h.brick_size + 2 * h.brick_pad + 1, v3 = v_size * v_size * v_size;
+    const size_t first = sc.sbsRoots[sdf];
+    const size_t last = sdf + 1 < sc.sbsRoots.size() ? sc.sbsRoots[sdf + 1] : sc.sbsNodes.size();
+    bool ok = true;
+    for (size_t b = first; b < last && ok; ++b)
+      for (uint32_t k = 0; k < v3 && ok; ++k) {
+        const uint32_t idx = sc.sbsData[sc.sbsNodes[b].data_offset + k];
+        ok = idx >= sl.x && idx < sl.x + sl.y / 4;
+      }
+    sc.sbsDiffChecked[sdf] = ok ? 1 : 2;
+  }
+  if (sc.sbsDiffChecked[sdf] != 1)
+    return fail(LRT_ERR_INVALID_ARG, "brick distance indices must address the first n_dist values_f entries");
+  if (f_offset) *f_offset = sl.x;
+  if (n_dist) *n_dist = sl.y / 4;
+  return LRT_OK;
+}
+
+lrt_status sbs_diff_common(lrt_scene* s, uint32_t geom_id, const float* params, const float* pos, const float* dir,
+                           uint64_t n, float* colour, const float* dL_dcolour, const float* target, float* grad,
+                           float* loss, void* stream) {
+  uint32_t f_offset = 0, n_dist = 0;
+  lrt_status st = diff_geometry(s, geom_id, &f_offset, &n_dist);
+  if (st != LRT_OK) return st;
+  if (!s->s.committed) return fail(LRT_ERR_NOT_COMMITTED, "scene is not committed");
+  if (n == 0) return LRT_OK;
+  if (!params || !pos || !dir) return fail(LRT_ERR_INVALID_ARG, "null parameter or ray buffer");
+  if (!colour && !grad) return fail(LRT_ERR_INVALID_ARG, "null output buffer");
+  if (n > 0xFFFFFFFFull) return fail(LRT_ERR_INVALID_ARG, "too many rays in one call");
+  lrt::SbsDiffArgs a{};
+  a.pos = reinterpret_cast<const lrt::float4*>(pos);
+  a.dir = reinterpret_cast<const lrt::float4*>(dir);
+  a.n = uint32_t(n);
+  a.params = params;
+  a.geom_id = geom_id;
+  a.f_offset = f_offset;
+  a.n_dist = n_dist;
+  a.colour = reinterpret_cast<lrt::float4*>(colour);
+  a.dL_dcolour = reinterpret_cast<const lrt::float4*>(dL_dcolour);
+  a.target = reinterpret_cast<const lrt::float4*>(target);
+  a.grad = grad;
+  a.loss = loss;
+  a.counters = s->s.d_overflow ? s->s.d_overflow + 1 : nullptr;
+  a.xcd_remap = uint32_t(s->s.xcd_remap);
+  lrt::TailQueue* q = nullptr;
+  hipError_t e = s->s.work_for(stream, size_t(n), 16, &a.spill, &q);
+  if (e != hipSuccess) return hip_fail(e, "work buffer");
+  e = lrt::launch_sbs_diff(s->s.make_view(), a, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "differentiable brick-set launch");
+  return LRT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -338,6 +407,65 @@ lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_fa
       out->tail_waves = r.best.tail_waves;
       out->best_ms = r.best_ms;
       out->configs_tried = r.tried;
+    }
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_sbs_param_layout(lrt_scene* s, uint32_t geom_id, uint32_t* f_offset, uint32_t* n_dist,
+                                uint64_t* n_params) {
+  return guarded([&]() -> lrt_status {
+    uint32_t off = 0, nd = 0;
+    lrt_status st = diff_geometry(s, geom_id, &off, &nd);
+    if (st != LRT_OK) return st;
+    if (f_offset) *f_offset = off;
+    if (n_dist) *n_dist = nd;
+    if (n_params) *n_params = s->s.sbsDataF.size();
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_sbs_forward_dev(lrt_scene* s, uint32_t geom_id, const float* params, const float* pos_near,
+                               const float* dir_far, uint64_t n, float* colour, void* stream) {
+  return guarded([&]() -> lrt_status {
+    if (!colour) return fail(LRT_ERR_INVALID_ARG, "null colour buffer");
+    return sbs_diff_common(s, geom_id, params, pos_near, dir_far, n, colour, nullptr, nullptr, nullptr, nullptr,
+                           stream);
+  });
+}
+
+lrt_status lrt_sbs_backward_dev(lrt_scene* s, uint32_t geom_id, const float* params, const float* pos_near,
+                                const float* dir_far, uint64_t n, const float* dL_dcolour, float* dL_dparams,
+                                void* stream) {
+  return guarded([&]() -> lrt_status {
+    if (!dL_dcolour || !dL_dparams) return fail(LRT_ERR_INVALID_ARG, "null gradient buffer");
+    return sbs_diff_common(s, geom_id, params, pos_near, dir_far, n, nullptr, dL_dcolour, nullptr, dL_dparams,
+                           nullptr, stream);
+  });
+}
+
+lrt_status lrt_sbs_l2_backward_dev(lrt_scene* s, uint32_t geom_id, const float* params, const float* pos_near,
+                                   const float* dir_far, uint64_t n, const float* target, float* dL_dparams,
+                                   float* loss, void* stream) {
+  return guarded([&]() -> lrt_status {
+    if (!target || !dL_dparams) return fail(LRT_ERR_INVALID_ARG, "null target or gradient buffer");
+    return sbs_diff_common(s, geom_id, params, pos_near, dir_far, n, nullptr, nullptr, target, dL_dparams, loss,
+                           stream);
+  });
+}
+
+lrt_status lrt_sbs_recompute_misses(lrt_scene* s, uint64_t* out, int reset) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !out) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    *out = 0;
+    if (!s->s.d_overflow) return LRT_OK;
+    uint32_t v = 0;
+    hipError_t e = hipMemcpy(&v, s->s.d_overflow + 1, sizeof(v), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "counter read");
+    *out = v;
+    if (reset) {
+      e = hipMemset(s->s.d_overflow + 1, 0, sizeof(uint32_t));
+      if (e != hipSuccess) return hip_fail(e, "counter reset");
     }
     return LRT_OK;
   });

@@ -1,0 +1,147 @@
+// Differentiable sparse-brick-set rendering: forward colour and the adjoint of an image loss
+// with respect to the brick set's distance values and per-lattice-point albedo.
+//
+// The forward image is C(ray) = trilinear albedo of the hit voxel at the hit point (miss: 0).
+// The backward recomputes the hit (same traversal, then a second march over the hit brick
+// that records the voxel; nothing is stored between the passes) and accumulates
+//   dL/dA[k][c] += G_c * w_k(q)
+//   dL/ds[j]    += -(sum_c G_c * dot(grad C_c(q), dir)) * w_j(q) / dot(grad f(q), dir)
+// where q is the hit point in voxel coordinates, w the trilinear weights, f the trilinear
+// distance and G = dL/dC (given, or 2(C - T) for the fused L2 loss): the hit point moves along
+// the ray when a corner distance changes (implicit function theorem on f(q(t)) = 0).
+#include "kernels.h"
+#include "traverse.h"
+
+namespace lrt {
+
+namespace {
+
+__device__ __forceinline__ void trilinear_weights(float3 q, float w[8], float3 gw[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const float wx = (i & 4) ? q.x : 1.0f - q.x, dx = (i & 4) ? 1.0f : -1.0f;
+    const float wy = (i & 2) ? q.y : 1.0f - q.y, dy = (i & 2) ? 1.0f : -1.0f;
+    const float wz = (i & 1) ? q.z : 1.0f - q.z, dz = (i & 1) ? 1.0f : -1.0f;
+    w[i] = wx * wy * wz;
+    gw[i] = make3(dx * wy * wz, wx * dy * wz, wx * wy * dz);
+  }
+}
+
+// Recomputes the voxel that produced hit `h` of the ray (p, d): one march over the hit brick
+// with the same arithmetic as the traversal; returns false unless it lands on the same t.
+__device__ bool recompute_hit_voxel(const SceneView& s, uint32_t diff_geom, float4 p, float4 d, const CRT_Hit& h,
+                                    SbsVoxelRec& rec, float3& ray_dir_local) {
+  if (h.geomId == 0xFFFFFFFFu) return false;
+  const uint32_t geomId = h.geomId & GEOM_ID_MASK;
+  if (geomId != diff_geom) return false;
+  float m[16];
+  load_mat(s.instances[h.instId].transformInv, m);
+  const float3 ray_pos = matmul4x3(m, xyz(p));
+  const float3 ray_dir = matmul3x3(m, xyz(d));
+  CRT_Hit hh = miss_hit(fabs_ref(d.w));
+  sbs_brick_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, fmax_ref(p.w, 0.01f), h.instId, geomId, h.primId, hh,
+                      &rec);
+  ray_dir_local = ray_dir;
+  return __float_as_uint(hh.t) == __float_as_uint(h.t);
+}
+
+template <int MODE>   // 0 forward, 1 backward with dL/dC, 2 backward of the L2 loss against a target
+__device__ __forceinline__ void diff_ray(const SceneView& s, const SbsDiffArgs& a, uint32_t i, const CRT_Hit& h) {
+  const float4 p = a.pos[i];
+  const float4 d = a.dir[i];
+  SbsVoxelRec rec;
+  float3 dir_l;
+  const bool hit = recompute_hit_voxel(s, a.geom_id, p, d, h, rec, dir_l);
+  if (h.geomId != 0xFFFFFFFFu && (h.geomId & GEOM_ID_MASK) == a.geom_id && !hit && a.counters)
+    atomicAdd(a.counters, 1u);
+  float3 C = make3(0, 0, 0);
+  float w[8];
+  float3 gw[8];
+  float3 q = make3(0, 0, 0);
+  if (hit) {
+    q = rec.start_q + rec.t_local * dir_l;
+    trilinear_weights(q, w, gw);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t ab = a.f_offset + a.n_dist + 3u * (rec.idx[k] - a.f_offset);
+      C = C + w[k] * make3(a.params[ab], a.params[ab + 1], a.params[ab + 2]);
+    }
+  }
+  if (MODE == 0) {
+    a.colour[i] = make4(C.x, C.y, C.z, hit ? 1.0f : 0.0f);
+    return;
+  }
+  float3 G;
+  if (MODE == 1) {
+    G = xyz(a.dL_dcolour[i]);
+  } else {
+    const float3 T = xyz(a.target[i]);
+    const float3 e = C - T;
+    G = 2.0f * e;
+    if (a.loss) atomicAdd(a.loss, dot3(e, e));
+  }
+  if (!hit) return;
+  float3 gf = make3(0, 0, 0);
+  float gCd = 0.0f;   // sum_c G_c * dot(grad C_c, dir)
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t ab = a.f_offset + a.n_dist + 3u * (rec.idx[k] - a.f_offset);
+    const float3 A = make3(a.params[ab], a.params[ab + 1], a.params[ab + 2]);
+    gf = gf + a.params[rec.idx[k]] * gw[k];
+    gCd += dot3(G, A) * dot3(gw[k], dir_l);
+    unsafeAtomicAdd(a.grad + ab + 0, G.x * w[k]);
+    unsafeAtomicAdd(a.grad + ab + 1, G.y * w[k]);
+    unsafeAtomicAdd(a.grad + ab + 2, G.z * w[k]);
+  }
+  const float den = dot3(gf, dir_l);
+  if (fabs_ref(den) > 1e-20f) {
+    const float k0 = -gCd / den;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) unsafeAtomicAdd(a.grad + rec.idx[k], k0 * w[k]);
+  }
+}
+
+template <int MODE, int S, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sbs_diff(SceneView s, SbsDiffArgs a) {
+  __shared__ uint32_t lds[S * BLOCK];
+  __shared__ BVHNodePair top_pairs[LDS_TOP_PAIRS];
+  if (s.cache_pairs) {
+    const uint4* src = reinterpret_cast<const uint4*>(s.nodes + s.geoms[s.cache_geom].bvhOffset);
+    uint4* dst = reinterpret_cast<uint4*>(top_pairs);
+    for (uint32_t k = threadIdx.x; k < s.cache_pairs * 4; k += BLOCK) dst[k] = src[k];
+  }
+  __syncthreads();
+  const uint32_t b = a.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const uint32_t i = b * BLOCK + threadIdx.x;
+  if (i >= a.n) return;
+  LaneStack<S, BLOCK> st;
+  st.col = (lds_u32*)(lds) + threadIdx.x;
+  st.spill = a.spill + size_t(i) * (STACK_SIZE - S);
+  uint32_t overflow = 0;
+  RayCounts cnt{};
+  bool aborted = false;
+  const CRT_Hit h = ray_query_nearest<LK_SBS, S, BLOCK, false, false>(s, a.pos[i], a.dir[i], st, overflow, cnt,
+                                                                      (const lds_pair*)top_pairs, 0u, aborted, nullptr);
+  diff_ray<MODE>(s, a, i, h);
+}
+
+template <int MODE>
+hipError_t launch_mode(const SceneView& s, const SbsDiffArgs& a, hipStream_t stream) {
+  constexpr int S = 16, BLOCK = 128;
+  const uint32_t grid = (a.n + BLOCK - 1) / BLOCK;
+  hipLaunchKernelGGL((k_sbs_diff<MODE, S, BLOCK>), dim3(grid), dim3(BLOCK), 0, stream, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_sbs_diff(const SceneView& s, const SbsDiffArgs& a, hipStream_t stream) {
+  if (a.n == 0) return hipSuccess;
+  SceneView v = s;
+  v.sbs_data_f = a.params;   // distances come from the caller's parameter array
+  if (a.colour) return launch_mode<0>(v, a, stream);
+  if (a.dL_dcolour) return launch_mode<1>(v, a, stream);
+  return launch_mode<2>(v, a, stream);
+}
+
+}  // namespace lrt

@@ -38,7 +38,30 @@ struct EyeRayParams {
   uint32_t order;               // 0: row-major ray index, 1: SuperBlockIndex2DOpt order
 };
 
+// Differentiable brick-set rendering (one launch = one of: forward colour, backward with a
+// given dL/dC, backward of the L2 loss against a target). The brick set uses an indexed fp32
+// layout whose values_f slice [f_offset, f_offset + 4 n_dist) holds n_dist distances followed
+// by n_dist RGB albedos; `params` replaces the scene's values_f for the launch.
+struct SbsDiffArgs {
+  const float4* pos;
+  const float4* dir;
+  uint32_t      n;
+  const float*  params;       // device values_f (distances and albedo), scene-sized
+  uint32_t      geom_id;      // the differentiable geometry (hits on others render 0, no gradient)
+  uint32_t      f_offset;     // first values_f entry of the geometry
+  uint32_t      n_dist;       // distance values of the geometry
+  float4*       colour;       // forward: (r, g, b, hit) per ray
+  const float4* dL_dcolour;   // backward: upstream gradient per ray (rgb used)
+  const float4* target;       // fused L2 backward: target colour per ray
+  float*        grad;         // backward: dL/dparams, accumulated (scene-sized)
+  float*        loss;         // fused L2: sum over rays of |C - T|^2, accumulated (nullable)
+  uint32_t*     counters;     // nullable: [0] += rays whose recomputed hit differed from the walk
+  uint32_t*     spill;        // n * (STACK_SIZE - 16) words
+  uint32_t      xcd_remap;
+};
+
 hipError_t launch_trace(const SceneView& s, const TraceArgs& a, hipStream_t stream);
+hipError_t launch_sbs_diff(const SceneView& s, const SbsDiffArgs& a, hipStream_t stream);
 hipError_t launch_eye_rays(const EyeRayParams& prm, const float2* jitter, float4* pos, float4* dir,
                            hipStream_t stream);
 

@@ -122,7 +122,7 @@ void Scene::clear_geom() {
   vertPos.clear(); vertNorm.clear(); indices.clear(); primIndices.clear(); primIdCount.clear();
   nodePairs.clear(); origNodes.clear();
   frameNodes.clear(); frameRoots.clear();
-  sbsNodes.clear(); sbsData.clear(); sbsDataF.clear(); sbsRoots.clear(); sbsHeaders.clear();
+  sbsNodes.clear(); sbsData.clear(); sbsDataF.clear(); sbsRoots.clear(); sbsHeaders.clear(); sbsFSlice.clear(); sbsDiffChecked.clear();
   multiNodes.clear(); multiValues.clear();
   clear_scene();
 }
@@ -267,6 +267,9 @@ uint32_t Scene::add_sdf_sbs(const SdfSBSHeader& header, const SdfSBSNode* nodes,
     const uint32_t f_offset = uint32_t(sbsDataF.size());
     sbsDataF.insert(sbsDataF.end(), values_f, values_f + n_values_f);
     for (size_t i = v_offset; i < sbsData.size(); i++) sbsData[i] += f_offset;
+    sbsFSlice.push_back({f_offset, uint32_t(n_values_f)});
+  } else {
+    sbsFSlice.push_back({0u, 0u});
   }
   std::vector<BVHNode> boxes(n_nodes);
   for (size_t i = 0; i < n_nodes; i++) {
@@ -463,9 +466,9 @@ hipError_t Scene::commit() {
   UP("sbs_roots", sbsRoots) UP("sbs_headers", sbsHeaders) UP("multi_nodes", multiNodes) UP("multi_values", multiValues)
 #undef UP
   if (!d_overflow) {
-    e = hipMalloc(&d_overflow, sizeof(uint32_t));
+    e = hipMalloc(&d_overflow, 2 * sizeof(uint32_t));   // [0] stack overflows, [1] adjoint recompute misses
     if (e != hipSuccess) return e;
-    e = hipMemset(d_overflow, 0, sizeof(uint32_t));
+    e = hipMemset(d_overflow, 0, 2 * sizeof(uint32_t));
     if (e != hipSuccess) return e;
   }
   e = hipDeviceSynchronize();

@@ -50,6 +50,8 @@ struct Scene {
   std::vector<SdfSBSNode> sbsNodes;
   std::vector<uint32_t> sbsData;
   std::vector<float> sbsDataF;
+  std::vector<uint2> sbsFSlice;     // per brick-set geometry: (first values_f entry, count)
+  std::vector<uint8_t> sbsDiffChecked;  // per brick-set geometry: 0 unchecked, 1 usable by the adjoint, 2 not
   std::vector<uint32_t> sbsRoots;
   std::vector<SdfSBSHeader> sbsHeaders;
   std::vector<SdfMultiOctreeNode> multiNodes;

@@ -180,11 +180,12 @@ __device__ __forceinline__ float eval_dist_trilinear(const float v[8], float3 dp
 
 // BVHRT::LocalSurfaceIntersection (BVH2Common.cpp:345-725) for the non-compact SDF types
 // used here (frame octree, SBS, multi octree). `is_surface_node` is always false for them.
-// Returns true when the hit record was updated.
+// Returns true when the hit record was updated; then *t_local (if given) receives the hit
+// parameter in voxel units along ray_dir from start_q.
 __device__ bool local_surface_intersection(const SceneView& s, uint32_t type, float3 ray_dir, uint32_t instId,
                                            uint32_t geomId, const float values[8], uint32_t primId, float d,
                                            float qNear, float qFar, float2 fNearFar, float3 start_q,
-                                           CRT_Hit& hit) {
+                                           CRT_Hit& hit, float* t_local = nullptr) {
   const float EPS = 1e-6f;
   const float d_inv = 1.0f / d;
   float t = qNear;
@@ -371,6 +372,7 @@ __device__ bool local_surface_intersection(const SceneView& s, uint32_t type, fl
     hit.coords[2] = en.x;
     hit.coords[3] = en.y;
     if (s.preset.render_mode == MULTI_RENDER_MODE_ST_ITERATIONS) hit.primId = iter;
+    if (t_local) *t_local = t;
     return true;
   }
   return false;
@@ -433,10 +435,20 @@ __device__ __forceinline__ float sbs_load_values(const SceneView& s, const SdfSB
   return vmin;
 }
 
+// The voxel whose surface produced a brick-set hit: the value indices of its 8 corners (into
+// the scene's values_f, indexed layouts), the entry point in voxel units, and the hit
+// parameter along the ray direction from there.
+struct SbsVoxelRec {
+  uint32_t idx[8];
+  float3 start_q;
+  float t_local;
+};
+
 // Sparse-brick-set leaf: BVHRT::OctreeBrickIntersect (BVH2Common.cpp:884-1112), trilinear.
-// Returns the number of voxels stepped.
+// Returns the number of voxels stepped. `rec` (indexed layouts only) records the hit voxel.
 __device__ uint32_t sbs_brick_intersect(const SceneView& s, uint32_t type, float3 ray_pos, float3 ray_dir, float tNear,
-                                        uint32_t instId, uint32_t geomId, uint32_t bvhNodeId, CRT_Hit& hit) {
+                                        uint32_t instId, uint32_t geomId, uint32_t bvhNodeId, CRT_Hit& hit,
+                                        SbsVoxelRec* rec = nullptr) {
   uint32_t steps = 0;
   const uint32_t sdfId = s.geoms[geomId].offset.x;
   const uint32_t primId = bvhNodeId;
@@ -476,8 +488,20 @@ __device__ uint32_t sbs_brick_intersect(const SceneView& s, uint32_t type, float
       const float3 start_pos = ray_pos + fNearFar.x * ray_dir;
       const float3 start_q = (start_pos - min_pos) * scale;
       const float qFar = (fNearFar.y - fNearFar.x) * scale;
-      local_surface_intersection(s, type, ray_dir, instId, geomId, values, primId, d, 0.0f, qFar, fNearFar, start_q,
-                                 hit);
+      float t_local = 0.0f;
+      const bool got = local_surface_intersection(s, type, ray_dir, instId, geomId, values, primId, d, 0.0f, qFar,
+                                                  fNearFar, start_q, hit, rec ? &t_local : nullptr);
+      if (rec && got) {
+        const uint32_t vx = uint32_t(voxelPos.x), vy = uint32_t(voxelPos.y), vz = uint32_t(voxelPos.z);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const uint32_t vId = (vx + ((i & 4) >> 2) + header.brick_pad) * v_size * v_size +
+                               (vy + ((i & 2) >> 1) + header.brick_pad) * v_size + (vz + (i & 1) + header.brick_pad);
+          rec->idx[i] = s.sbs_data[node.data_offset + vId];
+        }
+        rec->start_q = start_q;
+        rec->t_local = t_local;
+      }
     }
     brick_fNearFar.x += fmax_ref(0.0f, fNearFar.y - brick_fNearFar.x) + 1e-6f;
   }

@@ -1,0 +1,191 @@
+// Ray traversal device code shared by the trace and differentiable kernels: the reference's
+// sequential BLAS walk, its TLAS walk and the hit finishing, plus small launch helpers.
+#pragma once
+#include "trace_coop.h"
+#include "trace_device.h"
+
+namespace lrt {
+
+
+// BVHRT::BVH2TraverseF32 (MultiRenderer/BVH2Common.cpp:3239-3320): stack-based walk over
+// fat nodes, nearer child first, far child pushed; leaves dispatched on the geometry tag.
+// `steps` counts fat-node fetches and leaf visits; past `budget` the walk stops and sets
+// `aborted` (budget 0 = unlimited).
+template <uint32_t KINDS, int S, int BLOCK, bool COUNT>
+__device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos, float3 ray_dir, float tNear,
+                                              uint32_t instId, uint32_t geomId, bool stopOnFirstHit, CRT_Hit& hit,
+                                              PendingTri& pend, LaneStack<S, BLOCK>& st, uint32_t& overflow,
+                                              RayCounts& cnt, const lds_pair* top_pairs, uint32_t budget,
+                                              uint32_t& steps, bool& aborted) {
+  const uint32_t bvhOffset = s.geoms[geomId].bvhOffset;
+  const BVHNodePair* nodes = s.nodes + bvhOffset;
+  const uint32_t cached = (top_pairs != nullptr && geomId == s.cache_geom) ? s.cache_pairs : 0u;
+  int top = 0;
+  uint32_t leftNodeOffset = 0;
+  bool hitFound = false;
+  const float3 rayDirInv = safe_inverse(ray_dir);
+  while (top >= 0 && !hitFound) {
+    while (top >= 0 && ((leftNodeOffset & LEAF_BIT) == 0)) {
+      if (budget && ++steps > budget) { aborted = true; return; }
+      if (COUNT) cnt.blas_nodes++;
+      const BVHNodePair fat = leftNodeOffset < cached ? load_pair_lds(top_pairs + leftNodeOffset)
+                                                      : load_pair(nodes + leftNodeOffset);
+      const uint32_t n0 = fat.left.leftOffset;
+      const uint32_t n1 = fat.right.leftOffset;
+      const float2 tm0 = ray_box2(ray_pos, rayDirInv, fat.left.boxMin, fat.left.boxMax);
+      const float2 tm1 = ray_box2(ray_pos, rayDirInv, fat.right.boxMin, fat.right.boxMax);
+      const bool hit0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= hit.t);
+      const bool hit1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= hit.t);
+      leftNodeOffset = hit0 ? n0 : n1;
+      if (hit0 && hit1) {
+        const bool first0 = tm0.x <= tm1.x;
+        leftNodeOffset = first0 ? n0 : n1;
+        if (top < STACK_SIZE) {
+          st.put(top, first0 ? n1 : n0);
+          top++;
+        } else {
+          overflow = 1;
+        }
+      }
+      if (!hit0 && !hit1) {
+        top--;
+        leftNodeOffset = st.get(top > 0 ? top : 0);
+      }
+    }
+    if (top >= 0 && leftNodeOffset != 0xFFFFFFFFu) {
+      if (budget && ++steps > budget) { aborted = true; return; }
+      // BVHRT::IntersectAllPrimitivesInLeaf (BVH2Common.cpp:47-124); device-layout leaves carry
+      // PackOffsetAndSize(start, count) themselves
+      const uint32_t a_start = extract_start(leftNodeOffset);
+      const uint32_t a_count = extract_count(leftNodeOffset);
+      const float tNearSdf = fmax_ref(tNear, 0.01f);
+      const float tPrev = hit.t;
+      const uint32_t tag = s.geom_tags[geomId];
+      if (COUNT) {
+        cnt.leaves++;
+        if (tag == TAG_TRIANGLE) cnt.prims += a_count;
+      }
+      if ((KINDS & LK_TRI) && tag == TAG_TRIANGLE) {
+        intersect_tris(s, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, a_count, hit, pend);
+      } else if ((KINDS & LK_FRAME) && tag == TAG_SDF_NODE) {
+        frame_node_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
+        if (hit.t < tPrev) pend.valid = 0;
+      } else if ((KINDS & LK_SBS) && tag == TAG_SDF_BRICK) {
+        cnt.sdf_voxels += sbs_brick_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId,
+                                              a_start, hit);
+        if (hit.t < tPrev) pend.valid = 0;
+      }
+      if (s.preset.render_mode == MULTI_RENDER_MODE_INTERSECTION_COUNT)
+        hit.instId = (hit.instId == 0xFFFFFFFFu) ? 1 : hit.instId + 1;
+      const uint32_t hitTag = (hit.t >= tPrev) ? TAG_NONE : tag;
+      hitFound = (hitTag != TAG_NONE) && (hitTag != TAG_TRIANGLE || stopOnFirstHit);
+    }
+    top--;
+    leftNodeOffset = st.get(top > 0 ? top : 0);
+  }
+}
+
+__device__ __forceinline__ CRT_Hit miss_hit(float far) {
+  CRT_Hit hit;
+  hit.t = far;
+  hit.primId = 0xFFFFFFFFu;
+  hit.instId = 0xFFFFFFFFu;
+  hit.geomId = 0xFFFFFFFFu;
+  hit.coords[0] = 1.0f;
+  hit.coords[1] = 0.0f;
+  hit.coords[2] = 0.0f;
+  hit.coords[3] = 0.0f;
+  return hit;
+}
+
+// Normal of a final triangle hit and the m_primIndices remap (BVH2Common.cpp:3382-3388).
+template <uint32_t KINDS>
+__device__ __forceinline__ void finish_hit(const SceneView& s, const PendingTri& pend, CRT_Hit& hit) {
+  if ((KINDS & LK_TRI) && pend.valid && hit.geomId != 0xFFFFFFFFu && (hit.geomId >> SH_TYPE) == TYPE_MESH_TRIANGLE)
+    resolve_tri_normal(s, pend, hit);
+  if (hit.geomId < 0xFFFFFFFFu && ((hit.geomId >> SH_TYPE) == TYPE_MESH_TRIANGLE)) {
+    const uint2 go = s.geoms[hit.geomId & GEOM_ID_MASK].offset;
+    hit.primId = s.prim_indices[go.x / 3 + hit.primId];
+  }
+}
+
+// BVHRT::RayQuery_NearestHit (MultiRenderer/BVH2Common.cpp:3322-3392): stackless TLAS walk
+// over escape indices, ray moved to instance space (direction not renormalised).
+// COOP = true: the wave holds one ray and walks each BLAS cooperatively.
+template <uint32_t KINDS, int S, int BLOCK, bool COUNT, bool COOP>
+__device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 posAndNear, float4 dirAndFar,
+                                                     LaneStack<S, BLOCK>& st, uint32_t& overflow, RayCounts& cnt,
+                                                     const lds_pair* top_pairs, uint32_t budget, bool& aborted,
+                                                     lds_entry* coop_stack) {
+  bool stopOnFirstHit = (dirAndFar.w <= 0.0f);
+  if (stopOnFirstHit) dirAndFar.w *= -1.0f;
+  const float3 pos = xyz(posAndNear);
+  const float3 dir = xyz(dirAndFar);
+  const float3 rayDirInv = safe_inverse(dir);
+  CRT_Hit hit = miss_hit(dirAndFar.w);
+  PendingTri pend;
+  pend.valid = 0;
+  uint32_t steps = 0;
+  uint32_t nodeIdx = 0;
+  do {
+    uint32_t travFlags = 0;
+    uint32_t leftOffset = 0;
+    do {
+      if (COUNT) cnt.tlas_nodes++;
+      const BVHNode cur = load_node(s.tlas + nodeIdx);
+      const float2 bh = ray_box2(pos, rayDirInv, cur.boxMin, cur.boxMax);
+      const bool intersects = (bh.x <= bh.y) && (bh.y > posAndNear.w) && (bh.x < hit.t);
+      travFlags = (cur.leftOffset & LEAF_BIT) | uint32_t(intersects);
+      leftOffset = cur.leftOffset;
+      const bool leaf_or_miss = (travFlags & LEAF_BIT) != 0 || (travFlags & 0x1) == 0;
+      nodeIdx = leaf_or_miss ? cur.escapeIndex : leftOffset;
+    } while (travFlags != (LEAF_BIT | 0x1) && nodeIdx != 0 && nodeIdx < 0xFFFFFFFEu);
+    if (travFlags == (LEAF_BIT | 0x1)) {
+      if (COUNT) cnt.instances++;
+      const uint32_t instId = extract_start(leftOffset);
+      const InstanceData* inst = s.instances + instId;
+      const uint32_t geomId = inst->geomId;
+      float m[16];
+      load_mat(inst->transformInv, m);
+      const float3 ray_pos = matmul4x3(m, pos);
+      const float3 ray_dir = matmul3x3(m, dir);
+      if (COOP) {
+        if (!coop_blas<KINDS>(s, coop_stack, ray_pos, ray_dir, posAndNear.w, instId, geomId, stopOnFirstHit, hit,
+                              pend, cnt)) {
+          // frontier too wide or tree deeper than the key: every lane walks the same ray
+          // sequentially on one shared LDS stack
+          LaneStack<STACK_SIZE, 1> seq;
+          seq.col = (lds_u32*)coop_stack;
+          seq.spill = nullptr;
+          uint32_t seq_steps = 0;
+          bool seq_abort = false;
+          blas_traverse<KINDS, STACK_SIZE, 1, false>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId,
+                                                     stopOnFirstHit, hit, pend, seq, overflow, cnt, nullptr, 0u,
+                                                     seq_steps, seq_abort);
+          __syncthreads();
+        }
+      } else {
+        blas_traverse<KINDS, S, BLOCK, COUNT>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId, stopOnFirstHit, hit,
+                                              pend, st, overflow, cnt, top_pairs, budget, steps, aborted);
+        if (aborted) return hit;
+      }
+    }
+  } while (nodeIdx < 0xFFFFFFFEu && !(stopOnFirstHit && hit.primId != 0xFFFFFFFFu));
+  finish_hit<KINDS>(s, pend, hit);
+  return hit;
+}
+
+// Logical block index so that each XCD (blocks are dealt round-robin to the 8 XCDs) works
+// on one contiguous range of rays and keeps its own L2 warm with the nodes those rays touch.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t x = b & 7u, q = nb >> 3, r = nb & 7u;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+__device__ __forceinline__ void store_hit(CRT_Hit* out, uint32_t i, const CRT_Hit& h) {
+  float4* o = reinterpret_cast<float4*>(out + i);
+  o[0] = make4(h.t, __uint_as_float(h.primId), __uint_as_float(h.instId), __uint_as_float(h.geomId));
+  o[1] = make4(h.coords[0], h.coords[1], h.coords[2], h.coords[3]);
+}
+
+}  // namespace lrt

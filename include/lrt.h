@@ -141,6 +141,32 @@ lrt_status lrt_set_launch_config(lrt_scene* s, int block, int lds_stack, int xcd
  * No counterpart in the reference (its CPU loop has no wave tail). */
 lrt_status lrt_set_tail_config(lrt_scene* s, uint32_t budget, uint32_t tail_waves);
 
+/* ---- Differentiable sparse brick set (no counterpart in the reference) ----
+ * Geometry: an indexed fp32 brick set (layout ID32F_IRGB32F) whose values_f holds n_dist
+ * distance values followed by n_dist RGB albedos, one per lattice point (4*n_dist floats);
+ * the bricks' index arrays address the distances. Rendering: colour = trilinear albedo of the
+ * hit voxel at the hit point of the reference brick march (BVH2Common.cpp:884-1112); 0 on a
+ * miss. `params` is a device array of the scene's whole values_f (lrt_get_array "sbs_data_f"
+ * size) and replaces the committed copy for the call; gradients have the same size and
+ * layout and are ACCUMULATED (zero them first). Ray buffers as in lrt_trace_closest_dev. The
+ * backward passes recompute every hit (no state is kept between calls). */
+lrt_status lrt_sbs_param_layout(lrt_scene* s, uint32_t geom_id, uint32_t* f_offset, uint32_t* n_dist,
+                                uint64_t* n_params);
+/* colour: float4 (r, g, b, hit 0/1) per ray. */
+lrt_status lrt_sbs_forward_dev(lrt_scene* s, uint32_t geom_id, const float* params, const float* pos_near,
+                               const float* dir_far, uint64_t n, float* colour, void* stream);
+/* dL_dcolour: float4 per ray (rgb used) -> dL_dparams += J^T dL_dcolour. */
+lrt_status lrt_sbs_backward_dev(lrt_scene* s, uint32_t geom_id, const float* params, const float* pos_near,
+                                const float* dir_far, uint64_t n, const float* dL_dcolour, float* dL_dparams,
+                                void* stream);
+/* L = sum over rays |colour - target|^2 (target: float4 per ray, rgb used): dL_dparams += dL/dparams,
+ * *loss += L (loss may be NULL); recomputes the colour itself, so no forward call is needed. */
+lrt_status lrt_sbs_l2_backward_dev(lrt_scene* s, uint32_t geom_id, const float* params, const float* pos_near,
+                                   const float* dir_far, uint64_t n, const float* target, float* dL_dparams,
+                                   float* loss, void* stream);
+/* Rays whose recomputed hit voxel did not reproduce the walk's hit (expected 0). */
+lrt_status lrt_sbs_recompute_misses(lrt_scene* s, uint64_t* out, int reset);
+
 /* Result of lrt_autotune. */
 typedef struct lrt_tune_result {
   int32_t block, lds_stack, xcd_remap;

@@ -150,9 +150,24 @@ struct OracleCounts {
 
 namespace {
 
+// The voxel of the latest brick-set hit (indexed layouts): value indices of its corners, entry
+// point in voxel units, direction, local hit parameter and the hit t it produced.
+struct VoxelRec {
+  uint32_t hit;
+  uint32_t idx[8];
+  float start_q[3];
+  float dir[3];
+  float t_local;
+  float t;
+  float pad;
+};
+static_assert(sizeof(VoxelRec) == 72, "layout");
+
 struct Tracer {
   const OracleScene& s;
   OracleCounts* cnt;
+  VoxelRec* rec = nullptr;     // records the voxel of each brick-set hit (test support)
+  float last_t_local = 0.0f;   // local hit parameter of the latest LocalSurfaceIntersection hit
 
   // BVH2Common.cpp:19-32
   bool need_normal() const {
@@ -378,6 +393,7 @@ struct Tracer {
         norm = normalize(matmul4x3(s.instances[instId].transformInvTransposed, norm));
       }
       f2 en = encode_normal(norm);
+      last_t_local = t;
       pHit->t = tReal;
       pHit->primId = primId;
       pHit->instId = instId;
@@ -473,7 +489,22 @@ struct Tracer {
         f3 start_pos = ray_pos + fNearFar.x * ray_dir;
         f3 start_q = (start_pos - min_pos) * (0.5f * sz * header.brick_size);
         float qFar = (fNearFar.y - fNearFar.x) * (0.5f * sz * header.brick_size);
+        const float t_before = pHit->t;
         LocalSurfaceIntersection(type, ray_dir, instId, geomId, values, primId, d, 0.0f, qFar, fNearFar, start_q, pHit);
+        if (rec && pHit->t < t_before && (header.aux_data == LAYOUT_ID32F || header.aux_data == LAYOUT_ID32F_IN)) {
+          rec->hit = 1;
+          for (int i = 0; i < 8; i++) {
+            const uint32_t x = uint32_t(voxelPos.x) + ((i & 4) >> 2), y = uint32_t(voxelPos.y) + ((i & 2) >> 1),
+                           z = uint32_t(voxelPos.z) + (i & 1);
+            const uint32_t vId = (x + header.brick_pad) * v_size * v_size + (y + header.brick_pad) * v_size +
+                                 (z + header.brick_pad);
+            rec->idx[i] = s.sbs_data[nd.data_offset + vId];
+          }
+          rec->start_q[0] = start_q.x; rec->start_q[1] = start_q.y; rec->start_q[2] = start_q.z;
+          rec->dir[0] = ray_dir.x; rec->dir[1] = ray_dir.y; rec->dir[2] = ray_dir.z;
+          rec->t_local = last_t_local;
+          rec->t = pHit->t;
+        }
       }
       brick_fNearFar.x += std::max(0.0f, fNearFar.y - brick_fNearFar.x) + 1e-6f;
     }
@@ -838,6 +869,29 @@ int oracle_local_surface(const void* preset, const float* values, float d, float
                               qFar, f2{fNear, 0.0f}, f3(start_q[0], start_q[1], start_q[2]), &h);
   std::memcpy(hit_out, &h, sizeof(h));
   return h.primId != uint32_t(-1) ? 1 : 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Closest hit plus the voxel record of brick-set hits (VoxelRec per ray; hit = 0 when the final
+// hit is not from a recorded voxel), for the float64 adjoint checks.
+void oracle_sbs_hit_voxels(const OracleScene* sc, const float* pos, const float* dir, uint64_t n, int threads,
+                           void* hits, void* recs) {
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+  for (long i = 0; i < long(n); ++i) {
+    VoxelRec r{};
+    Tracer tr{*sc, nullptr};
+    tr.rec = &r;
+    const f4 p{pos[4 * i], pos[4 * i + 1], pos[4 * i + 2], pos[4 * i + 3]};
+    const f4 d{dir[4 * i], dir[4 * i + 1], dir[4 * i + 2], dir[4 * i + 3]};
+    const Hit h = tr.RayQuery_NearestHit(p, d, nullptr);
+    if (r.hit && (h.primId == uint32_t(-1) || r.t != h.t)) r.hit = 0;
+    static_cast<Hit*>(hits)[i] = h;
+    static_cast<VoxelRec*>(recs)[i] = r;
+  }
 }
 
 }  // extern "C"
