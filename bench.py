@@ -7,6 +7,15 @@ Prints one JSON line (see DESIGN.md for the metric, the roofline model and the C
 Workloads (synthetic, seeded):
   sdf_octree_d9_1024  64-sphere union, frame octree depth 9, 1024x1024 jittered primary rays
   icosphere7_512      icosphere subdivided 7x (327,680 tris), radial value noise 0.05, 512x512 rays
+  tlas_4096x32_4m     two-level BVH: 4,096 instances (random rotation, scale U[0.5,2], translation
+                      U[0,100]^3, seed 8) of 32 noise-displaced geodesic spheres with 20 n^2
+                      triangles, n log-uniform in [5,100] (seed 7); 4,194,304 cosine-hemisphere
+                      secondary rays leaving random instance surfaces (seed 9)
+  sbs_batch_64x4x128  64 differentiable brick-set scenes (64-sphere unions, seeds 1000+k) with grid
+                      resolutions drawn from {32,64,128,256}^3 (seed 10), 8^3 bricks, 4 ring views
+                      of 128x128 each; target = each scene rendered with inverted albedo; a step =
+                      one batched run (forward + L2 adjoint of all 4.2 M rays, one persistent
+                      kernel, HIP-graph replay); also timed: the same work as per-scene launches
   sbs_diff_256_8x256  differentiable brick set: 64-sphere union (seed 4) on a 256^3 grid of 8^3
                       fp32 bricks with per-lattice-point RGB albedo; 8 ring views of 256x256; a
                       step = forward render + adjoint of the L2 loss against the image of a second
@@ -48,6 +57,77 @@ def spheres(seed=2, n=64):
     return np.concatenate([c, r], axis=1).astype(np.float32)
 
 
+def rotation(q):
+    a, b, c, d = q / np.linalg.norm(q)
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+def tlas_scene(n_inst=4096, n_mesh=32):
+    """The batched two-level scene: (scene, meshes [(verts, tris)], instance matrices (row-major))."""
+    import graphic_lab_task_amd as g
+    rng = np.random.default_rng(7)
+    freqs = np.exp(rng.uniform(np.log(5), np.log(100), n_mesh)).round().astype(int)
+    scene = g.SceneRT()
+    meshes = []
+    for k, nf in enumerate(freqs):
+        v, f = g.gen_geodesic(int(nf), 0.05, 4.0, 100 + k)
+        meshes.append((v, f))
+        scene.AddGeom_Triangles3f(v, f)
+    rng = np.random.default_rng(8)
+    mats = np.zeros((n_inst, 4, 4), np.float32)
+    for i in range(n_inst):
+        m = np.eye(4)
+        m[:3, :3] = rotation(rng.normal(size=4)) * rng.uniform(0.5, 2.0)
+        m[:3, 3] = rng.uniform(0, 100, 3)
+        mats[i] = m
+        scene.AddInstance(i % n_mesh, mats[i])
+    try:
+        scene.CommitScene()
+    except g.LrtError as e:   # no device (host-side checks): the host arrays are built
+        if e.code != 5:
+            raise
+    return scene, meshes, mats
+
+
+def hemisphere_rays(meshes, mats, n, seed=9):
+    """Rays leaving random points of random instance surfaces, cosine-distributed about the
+    geometric normal; origin offset 1e-3 along the normal, near 1e-4, far 1e9."""
+    rng = np.random.default_rng(seed)
+    n_mesh = len(meshes)
+    inst = rng.integers(0, mats.shape[0], n)
+    pos = np.zeros((n, 4), np.float32)
+    dirs = np.zeros((n, 4), np.float32)
+    for k in range(n_mesh):
+        sel = np.nonzero(inst % n_mesh == k)[0]
+        if sel.size == 0:
+            continue
+        v, f = meshes[k]
+        v = v[:, :3].astype(np.float64)
+        tri = f[rng.integers(0, f.shape[0], sel.size)]
+        A, B, C = v[tri[:, 0]], v[tri[:, 1]], v[tri[:, 2]]
+        r1, r2 = np.sqrt(rng.random(sel.size))[:, None], rng.random(sel.size)[:, None]
+        p = (1 - r1) * A + r1 * (1 - r2) * B + r1 * r2 * C
+        nrm = np.cross(B - A, C - A)
+        M = mats[inst[sel]].astype(np.float64)
+        pw = np.einsum("nij,nj->ni", M[:, :3, :3], p) + M[:, :3, 3]
+        nw = np.einsum("nij,nj->ni", np.linalg.inv(M[:, :3, :3]).transpose(0, 2, 1), nrm)
+        nw /= np.linalg.norm(nw, axis=1, keepdims=True)
+        t1 = np.where(np.abs(nw[:, :1]) > 0.9, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+        t1 = np.cross(nw, t1)
+        t1 /= np.linalg.norm(t1, axis=1, keepdims=True)
+        t2 = np.cross(nw, t1)
+        u1, u2 = rng.random(sel.size)[:, None], rng.random(sel.size)[:, None]
+        rr, phi = np.sqrt(u1), 2 * np.pi * u2
+        d = rr * np.cos(phi) * t1 + rr * np.sin(phi) * t2 + np.sqrt(1 - u1) * nw
+        pos[sel, :3] = pw + 1e-3 * nw
+        pos[sel, 3] = 1e-4
+        dirs[sel, :3] = d / np.linalg.norm(d, axis=1, keepdims=True)
+        dirs[sel, 3] = 1e9
+    return pos, dirs
+
+
 def build_workload(name, threads):
     """Returns (scene, width, height, jitter or None, description dict, leaf kind)."""
     import graphic_lab_task_amd as g
@@ -71,7 +151,98 @@ def build_workload(name, threads):
                 "surface_leaves": int(scene.get_array("orig_nodes").size), "rays": 1024 * 1024,
                 "width": 1024, "height": 1024}
         return scene, 1024, 1024, jitter, desc, "frame", time.time() - t0
+    if name == "tlas_4096x32_4m":
+        scene, meshes, mats = tlas_scene()
+        pos, dirs = hemisphere_rays(meshes, mats, 4 * 1024 * 1024)
+        desc = {"workload": name, "instances": int(mats.shape[0]), "meshes": len(meshes),
+                "triangles_unique": int(sum(f.shape[0] for _, f in meshes)),
+                "triangles_instanced": int(sum(meshes[i % len(meshes)][1].shape[0] for i in range(mats.shape[0]))),
+                "rays": int(pos.shape[0]), "ray_kind": "cosine-hemisphere secondary"}
+        return scene, (pos, dirs), None, None, desc, "tri", time.time() - t0
     raise SystemExit(f"unknown workload {name}")
+
+
+def sbs_batch_scenes(torch, g, n_scenes=64, res=128, views=4, threads=0, seed=10, stream=0):
+    """Items (dicts for graphic_lab_task_amd.SbsBatch) of the batched differentiable workload."""
+    import graphic_lab_task_amd.diff as gd
+    grids = np.random.default_rng(seed).choice([32, 64, 128, 256], n_scenes)
+    pos, dirs = gd.ring_rays(torch, res, res, count=views, stream=stream)
+    items = []
+    for k, gr in enumerate(grids):
+        depth = int(np.log2(gr // 8))
+        h, nodes, vals, vf, nd = g.sdf_spheres_sbs_indexed(spheres(1000 + k), depth, 8, 1 + k, threads)
+        s = g.SceneRT()
+        gid = s.AddGeom_SdfSBS(h, nodes, vals, vf)
+        s.AddInstance(gid)
+        s.CommitScene()
+        params = torch.from_numpy(vf).cuda()
+        tparams = params.clone()
+        tparams[nd:] = 1.0 - tparams[nd:]
+        target = torch.zeros((pos.shape[0], 4), dtype=torch.float32, device="cuda")
+        s.sbs_forward(gid, tparams, pos, dirs, target, stream=stream)
+        items.append({"scene": s, "geom_id": gid, "params": params, "grad": torch.zeros_like(params),
+                      "loss": torch.zeros(1, dtype=torch.float32, device="cuda"), "pos": pos, "dir": dirs,
+                      "target": target, "colour": torch.zeros_like(target), "grid": int(gr), "n_dist": int(nd)})
+    return items
+
+
+def run_batch(args, torch, g):
+    """Batched differentiable workload: one persistent launch per step vs. per-scene launches."""
+    t0 = time.time()
+    threads = min(16, len(os.sched_getaffinity(0)))
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    items = sbs_batch_scenes(torch, g, threads=threads, stream=sh)
+    build_s = time.time() - t0
+    n = sum(it["pos"].shape[0] for it in items)
+    batch = g.SbsBatch(items)
+
+    def plain():
+        for it in items:
+            it["grad"].zero_()
+            it["loss"].zero_()
+            it["scene"].sbs_forward(it["geom_id"], it["params"], it["pos"], it["dir"], it["colour"], stream=sh)
+            it["scene"].sbs_l2_backward(it["geom_id"], it["params"], it["pos"], it["dir"], it["target"], it["grad"],
+                                        it["loss"], stream=sh)
+
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(steps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    ms_plain = timed(plain, max(3, args.steps // 5))
+    ms_eager = timed(lambda: batch.run(sh), args.steps)
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(stream)
+    with torch.cuda.stream(side):
+        batch.run(side.cuda_stream)            # warm-up on the capture stream
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph, stream=side):
+            batch.run(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    ms = timed(graph.replay, args.steps)
+    hit = float(sum((it["colour"][:, 3] > 0).sum().item() for it in items))
+    grids = [it["grid"] for it in items]
+    desc = {"workload": "sbs_batch_64x4x128", "scenes": len(items), "views_per_scene": 4, "width": 128, "height": 128,
+            "rays": int(n), "grids": {str(r): grids.count(r) for r in (32, 64, 128, 256)}, "brick": 8,
+            "hit_fraction": round(hit / n, 4), "scene_build_s": round(build_s, 1),
+            "ms_batched_eager": round(ms_eager, 4), "ms_per_scene_launches": round(ms_plain, 4),
+            "batched_vs_per_scene": round(ms_plain / ms, 2),
+            "recompute_misses": int(sum(it["scene"].sbs_recompute_misses() for it in items))}
+    return {
+        "metric": METRIC, "value": round(n / (ms * 1e-3) / 1e6, 2), "unit": "Mrays/s (fwd+bwd, batched)", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": desc,
+        "roofline": None, "cpu_baseline": None,
+    }
 
 
 def run_diff(args, torch, g):
@@ -227,7 +398,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="sdf_octree_d9_1024",
-                    choices=["sdf_octree_d9_1024", "icosphere7_512", "sbs_diff_256_8x256"])
+                    choices=["sdf_octree_d9_1024", "icosphere7_512", "tlas_4096x32_4m", "sbs_diff_256_8x256",
+                             "sbs_batch_64x4x128"])
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--lds-stack", type=int, default=16)
     ap.add_argument("--xcd-remap", type=int, default=1)
@@ -251,18 +423,25 @@ def main():
     if args.workload == "sbs_diff_256_8x256":
         print(json.dumps(run_diff(args, torch, g)))
         return
+    if args.workload == "sbs_batch_64x4x128":
+        print(json.dumps(run_batch(args, torch, g)))
+        return
     scene, w, h, jitter, desc, kind, build_s = build_workload(args.workload, threads)
     scene.set_launch_config(args.block, args.lds_stack, bool(args.xcd_remap))
     scene.set_tail_config(args.budget, args.tail_waves)
-    n = w * h
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
-    pos = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
-    dirs = torch.zeros_like(pos)
+    if isinstance(w, tuple):   # explicit ray buffers
+        pos, dirs = (torch.from_numpy(a).cuda() for a in w)
+        n = pos.shape[0]
+    else:
+        n = w * h
+        pos = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+        dirs = torch.zeros_like(pos)
+        jt = None if jitter is None else torch.from_numpy(jitter).cuda()
+        pi, vi = camera()
+        g.eye_rays(pi, vi, w, h, pos, dirs, jitter=jt, order=1, stream=sh)
     hits = torch.zeros((n, 8), dtype=torch.int32, device="cuda")
-    jt = None if jitter is None else torch.from_numpy(jitter).cuda()
-    pi, vi = camera()
-    g.eye_rays(pi, vi, w, h, pos, dirs, jitter=jt, order=1, stream=sh)
 
     # instrumented launch (untimed): traversal counters for the algorithmic-bytes model
     counts = torch.zeros((n, 8), dtype=torch.int32, device="cuda")

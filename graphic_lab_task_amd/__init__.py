@@ -135,6 +135,9 @@ def lib() -> ctypes.CDLL:
             "lrt_sbs_backward_dev": ([vp, u32, vp, vp, vp, u64, vp, vp, vp], i32),
             "lrt_sbs_l2_backward_dev": ([vp, u32, vp, vp, vp, u64, vp, vp, vp, vp], i32),
             "lrt_sbs_recompute_misses": ([vp, ctypes.POINTER(u64), i32], i32),
+            "lrt_sbs_batch_create": ([vp, u32, ctypes.POINTER(vp)], i32),
+            "lrt_sbs_batch_run": ([vp, vp], i32),
+            "lrt_sbs_batch_destroy": ([vp], i32),
             "lrt_stack_overflow_count": ([vp, ctypes.POINTER(u64), i32], i32),
             "lrt_get_array": ([vp, cp, vp, ctypes.POINTER(sz)], i32),
             "lrt_buffer_free": ([ctypes.POINTER(_Buffer)], None),
@@ -161,6 +164,7 @@ def exported_symbols():
             "lrt_trace_closest_host", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
             "lrt_set_launch_config", "lrt_set_tail_config", "lrt_autotune", "lrt_sbs_param_layout",
             "lrt_sbs_forward_dev", "lrt_sbs_backward_dev", "lrt_sbs_l2_backward_dev", "lrt_sbs_recompute_misses",
+            "lrt_sbs_batch_create", "lrt_sbs_batch_run", "lrt_sbs_batch_destroy",
             "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
             "lrt_camera_inverse", "lrt_gen_icosphere", "lrt_gen_geodesic", "lrt_sdf_spheres_frame_octree",
             "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed"]
@@ -371,6 +375,39 @@ class SceneRT:
         if n.value:
             _check(lib().lrt_get_array(self._h, name.encode(), _ptr(out), ctypes.byref(n)))
         return out
+
+
+class _BatchItem(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("geom_id", ctypes.c_uint32), ("params", ctypes.c_void_p),
+                ("grad", ctypes.c_void_p), ("loss", ctypes.c_void_p), ("pos_near", ctypes.c_void_p),
+                ("dir_far", ctypes.c_void_p), ("target", ctypes.c_void_p), ("colour", ctypes.c_void_p),
+                ("n_rays", ctypes.c_uint64)]
+
+
+class SbsBatch:
+    """Batched differentiable launch over several brick-set scenes (lrt_sbs_batch_*). `items` are
+    dicts with scene, geom_id, params, grad, loss, pos, dir, target, colour (torch tensors; the
+    plan keeps references to them). run() zeroes grads and losses and runs forward + L2 adjoint of
+    every ray in one persistent kernel; it only launches kernels, so it can be graph-captured."""
+
+    def __init__(self, items):
+        arr = (_BatchItem * max(1, len(items)))()
+        self._keep = items
+        for k, it in enumerate(items):
+            arr[k] = _BatchItem(it["scene"]._h, it["geom_id"], _ptr(it["params"]), _ptr(it["grad"]),
+                                _ptr(it.get("loss")), _ptr(it["pos"]), _ptr(it["dir"]), _ptr(it["target"]),
+                                _ptr(it.get("colour")), it["pos"].numel() // 4)
+        h = ctypes.c_void_p()
+        _check(lib().lrt_sbs_batch_create(arr, len(items), ctypes.byref(h)))
+        self._h = h
+
+    def run(self, stream: int = 0):
+        _check(lib().lrt_sbs_batch_run(self._h, stream))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().lrt_sbs_batch_destroy(self._h)
+            self._h = None
 
 
 def CreateSceneRT(impl_name="BVH2Common", build_name="cbvh_embree2", layout_name="SuperTreeletAlignedMerged4"):

@@ -1,7 +1,9 @@
 // extern "C" entry points declared in include/lrt.h.
 #include "../../include/lrt.h"
 
+#include <algorithm>
 #include <cstring>
+#include <memory>
 #include <exception>
 #include <stdexcept>
 #include <string>
@@ -13,6 +15,13 @@
 
 struct lrt_scene {
   lrt::Scene s;
+};
+
+struct lrt_sbs_batch {
+  int device = -1;
+  std::vector<lrt::SbsBatchItem> items;
+  lrt::DevBuf d_items, d_begins, d_masks, d_spill, d_counter;
+  lrt::SbsBatchLaunch L{};
 };
 
 namespace {
@@ -147,7 +156,18 @@ lrt_status sbs_diff_common(lrt_scene* s, uint32_t geom_id, const float* params, 
   lrt::TailQueue* q = nullptr;
   hipError_t e = s->s.work_for(stream, size_t(n), 16, &a.spill, &q);
   if (e != hipSuccess) return hip_fail(e, "work buffer");
-  e = lrt::launch_sbs_diff(s->s.make_view(), a, reinterpret_cast<hipStream_t>(stream));
+  // occupancy masks from the caller's parameters (they differ from the committed values)
+  lrt::SceneView v = s->s.make_view();
+  v.sbs_data_f = params;
+  v.sbs_mask = nullptr;
+  uint32_t* mask = nullptr;
+  e = s->s.mask_for(stream, &mask);
+  if (e != hipSuccess) return hip_fail(e, "mask buffer");
+  e = lrt::launch_sbs_masks(v, s->s.sbsMaskGeoms.data(), uint32_t(s->s.sbsMaskGeoms.size()), mask,
+                            reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "occupancy mask launch");
+  v.sbs_mask = mask;
+  e = lrt::launch_sbs_diff(v, a, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "differentiable brick-set launch");
   return LRT_OK;
 }
@@ -451,6 +471,117 @@ lrt_status lrt_sbs_l2_backward_dev(lrt_scene* s, uint32_t geom_id, const float* 
     if (!target || !dL_dparams) return fail(LRT_ERR_INVALID_ARG, "null target or gradient buffer");
     return sbs_diff_common(s, geom_id, params, pos_near, dir_far, n, nullptr, nullptr, target, dL_dparams, loss,
                            stream);
+  });
+}
+
+lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_items, lrt_sbs_batch** out) {
+  return guarded([&]() -> lrt_status {
+    if (!out || (!items && n_items)) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    auto b = std::make_unique<lrt_sbs_batch>();
+    hipError_t e = hipGetDevice(&b->device);
+    if (e != hipSuccess) return hip_fail(e, "no device");
+    std::vector<uint64_t> zero_begin{0};
+    std::vector<uint32_t> mask_begin{0}, brick_begin{0}, chunk_begin{0};
+    std::vector<size_t> mask_offset;
+    size_t mask_words = 0;
+    for (uint32_t k = 0; k < n_items; ++k) {
+      const lrt_sbs_batch_item& in = items[k];
+      if (!in.scene) return fail(LRT_ERR_INVALID_ARG, "null scene in batch item");
+      lrt::Scene& sc = in.scene->s;
+      uint32_t f_offset = 0, n_dist = 0;
+      lrt_status st = diff_geometry(in.scene, in.geom_id, &f_offset, &n_dist);
+      if (st != LRT_OK) return st;
+      if (!sc.committed) return fail(LRT_ERR_NOT_COMMITTED, "batch item scene is not committed");
+      if (sc.sbsRoots.size() != 1) return fail(LRT_ERR_UNSUPPORTED, "batch items need scenes with exactly one brick set");
+      if (!in.params || !in.grad || !in.pos_near || !in.dir_far || !in.target)
+        return fail(LRT_ERR_INVALID_ARG, "null buffer in batch item");
+      if (in.n_rays > 0xFFFFFFFFull) return fail(LRT_ERR_INVALID_ARG, "too many rays in one item");
+      lrt::SbsBatchItem it{};
+      it.view = sc.make_view();
+      it.view.sbs_data_f = in.params;
+      it.view.cache_pairs = 0;
+      it.view.cache_geom = 0xFFFFFFFFu;
+      it.args.pos = reinterpret_cast<const lrt::float4*>(in.pos_near);
+      it.args.dir = reinterpret_cast<const lrt::float4*>(in.dir_far);
+      it.args.n = uint32_t(in.n_rays);
+      it.args.params = in.params;
+      it.args.geom_id = in.geom_id;
+      it.args.f_offset = f_offset;
+      it.args.n_dist = n_dist;
+      it.args.colour = reinterpret_cast<lrt::float4*>(in.colour);
+      it.args.target = reinterpret_cast<const lrt::float4*>(in.target);
+      it.args.grad = in.grad;
+      it.args.loss = in.loss;
+      it.args.counters = sc.d_overflow ? sc.d_overflow + 1 : nullptr;
+      it.n_bricks = sc.sbsMaskGeoms[0].n_bricks;
+      it.brick_size = sc.sbsMaskGeoms[0].brick_size;
+      it.n_params = sc.sbsDataF.size();
+      const uint32_t wpb = (it.brick_size * it.brick_size * it.brick_size + 31u) / 32u;
+      mask_offset.push_back(mask_words);
+      mask_words += size_t(it.n_bricks) * (1 + wpb);
+      zero_begin.push_back(zero_begin.back() + it.n_params);
+      mask_begin.push_back(mask_begin.back() + it.n_bricks * wpb);
+      brick_begin.push_back(brick_begin.back() + it.n_bricks);
+      chunk_begin.push_back(chunk_begin.back() + (it.args.n + lrt::SBS_BATCH_BLOCK - 1) / lrt::SBS_BATCH_BLOCK);
+      b->items.push_back(it);
+    }
+    e = b->d_masks.reserve(std::max<size_t>(mask_words, 1) * 4);
+    if (e != hipSuccess) return hip_fail(e, "batch masks");
+    for (uint32_t k = 0; k < n_items; ++k) {
+      b->items[k].mask = static_cast<uint32_t*>(b->d_masks.ptr()) + mask_offset[k];
+      b->items[k].view.sbs_mask = b->items[k].mask;
+    }
+    int cus = 0, per_cu = 0;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device)) != hipSuccess)
+      return hip_fail(e, "device attribute");
+    if ((e = lrt::sbs_batch_blocks_per_cu(&per_cu)) != hipSuccess) return hip_fail(e, "occupancy");
+    const uint32_t grid = uint32_t(std::max(1, cus * std::max(1, per_cu)));
+    const size_t n1 = size_t(n_items) + 1;
+    if ((e = b->d_items.upload(b->items.data(), b->items.size() * sizeof(lrt::SbsBatchItem))) != hipSuccess)
+      return hip_fail(e, "batch table");
+    std::vector<uint8_t> begins(n1 * (8 + 4 + 4 + 4));
+    std::memcpy(begins.data(), zero_begin.data(), n1 * 8);
+    std::memcpy(begins.data() + n1 * 8, mask_begin.data(), n1 * 4);
+    std::memcpy(begins.data() + n1 * 12, brick_begin.data(), n1 * 4);
+    std::memcpy(begins.data() + n1 * 16, chunk_begin.data(), n1 * 4);
+    if ((e = b->d_begins.upload(begins.data(), begins.size())) != hipSuccess) return hip_fail(e, "batch table");
+    if ((e = b->d_spill.reserve(size_t(grid) * lrt::SBS_BATCH_BLOCK * (lrt::STACK_SIZE - 16) * 4)) != hipSuccess)
+      return hip_fail(e, "batch spill");
+    if ((e = b->d_counter.reserve(256)) != hipSuccess) return hip_fail(e, "batch counter");
+    const uint8_t* base = static_cast<const uint8_t*>(b->d_begins.ptr());
+    lrt::SbsBatchLaunch& L = b->L;
+    L.items = static_cast<const lrt::SbsBatchItem*>(b->d_items.ptr());
+    L.n_items = n_items;
+    L.zero_begin = reinterpret_cast<const uint64_t*>(base);
+    L.mask_begin = reinterpret_cast<const uint32_t*>(base + n1 * 8);
+    L.brick_begin = reinterpret_cast<const uint32_t*>(base + n1 * 12);
+    L.chunk_begin = reinterpret_cast<const uint32_t*>(base + n1 * 16);
+    L.total_zero = zero_begin.back();
+    L.total_mask_words = mask_begin.back();
+    L.total_bricks = brick_begin.back();
+    L.total_chunks = chunk_begin.back();
+    L.counter = static_cast<uint32_t*>(b->d_counter.ptr());
+    L.spill = static_cast<uint32_t*>(b->d_spill.ptr());
+    L.grid = grid;
+    *out = b.release();
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_sbs_batch_run(lrt_sbs_batch* b, void* stream) {
+  return guarded([&]() -> lrt_status {
+    if (!b) return fail(LRT_ERR_INVALID_ARG, "null batch");
+    const hipError_t e = lrt::launch_sbs_batch(b->L, reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "batch launch");
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_sbs_batch_destroy(lrt_sbs_batch* b) {
+  return guarded([&]() -> lrt_status {
+    delete b;
+    return LRT_OK;
   });
 }
 

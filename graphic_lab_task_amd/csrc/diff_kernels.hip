@@ -45,8 +45,10 @@ __device__ bool recompute_hit_voxel(const SceneView& s, uint32_t diff_geom, floa
   return __float_as_uint(hh.t) == __float_as_uint(h.t);
 }
 
-template <int MODE>   // 0 forward, 1 backward with dL/dC, 2 backward of the L2 loss against a target
-__device__ __forceinline__ void diff_ray(const SceneView& s, const SbsDiffArgs& a, uint32_t i, const CRT_Hit& h) {
+// MODE 0 forward, 1 backward with dL/dC, 2 backward of the L2 loss against a target, 3 both the
+// forward colour and the L2 backward. Returns the ray's loss term |C - T|^2 (modes 2, 3).
+template <int MODE>
+__device__ __forceinline__ float diff_ray(const SceneView& s, const SbsDiffArgs& a, uint32_t i, const CRT_Hit& h) {
   const float4 p = a.pos[i];
   const float4 d = a.dir[i];
   SbsVoxelRec rec;
@@ -67,20 +69,21 @@ __device__ __forceinline__ void diff_ray(const SceneView& s, const SbsDiffArgs& 
       C = C + w[k] * make3(a.params[ab], a.params[ab + 1], a.params[ab + 2]);
     }
   }
-  if (MODE == 0) {
-    a.colour[i] = make4(C.x, C.y, C.z, hit ? 1.0f : 0.0f);
-    return;
+  if (MODE == 0 || MODE == 3) {
+    if (a.colour) a.colour[i] = make4(C.x, C.y, C.z, hit ? 1.0f : 0.0f);
+    if (MODE == 0) return 0.0f;
   }
   float3 G;
+  float loss = 0.0f;
   if (MODE == 1) {
     G = xyz(a.dL_dcolour[i]);
   } else {
     const float3 T = xyz(a.target[i]);
     const float3 e = C - T;
     G = 2.0f * e;
-    if (a.loss) atomicAdd(a.loss, dot3(e, e));
+    loss = dot3(e, e);
   }
-  if (!hit) return;
+  if (!hit) return loss;
   float3 gf = make3(0, 0, 0);
   float gCd = 0.0f;   // sum_c G_c * dot(grad C_c, dir)
 #pragma unroll
@@ -99,6 +102,14 @@ __device__ __forceinline__ void diff_ray(const SceneView& s, const SbsDiffArgs& 
 #pragma unroll
     for (int k = 0; k < 8; ++k) unsafeAtomicAdd(a.grad + rec.idx[k], k0 * w[k]);
   }
+  return loss;
+}
+
+// Sum over the 64 lanes of a wave (every lane active).
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
 template <int MODE, int S, int BLOCK>
@@ -113,16 +124,111 @@ __global__ __launch_bounds__(BLOCK) void k_sbs_diff(SceneView s, SbsDiffArgs a) 
   __syncthreads();
   const uint32_t b = a.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint32_t i = b * BLOCK + threadIdx.x;
-  if (i >= a.n) return;
-  LaneStack<S, BLOCK> st;
-  st.col = (lds_u32*)(lds) + threadIdx.x;
-  st.spill = a.spill + size_t(i) * (STACK_SIZE - S);
-  uint32_t overflow = 0;
-  RayCounts cnt{};
-  bool aborted = false;
-  const CRT_Hit h = ray_query_nearest<LK_SBS, S, BLOCK, false, false>(s, a.pos[i], a.dir[i], st, overflow, cnt,
-                                                                      (const lds_pair*)top_pairs, 0u, aborted, nullptr);
-  diff_ray<MODE>(s, a, i, h);
+  float loss = 0.0f;
+  if (i < a.n) {
+    LaneStack<S, BLOCK> st;
+    st.col = (lds_u32*)(lds) + threadIdx.x;
+    st.spill = a.spill + size_t(i) * (STACK_SIZE - S);
+    uint32_t overflow = 0;
+    RayCounts cnt{};
+    bool aborted = false;
+    const CRT_Hit h = ray_query_nearest<LK_SBS, S, BLOCK, false, false>(s, a.pos[i], a.dir[i], st, overflow, cnt,
+                                                                        (const lds_pair*)top_pairs, 0u, aborted,
+                                                                        nullptr);
+    loss = diff_ray<MODE>(s, a, i, h);
+  }
+  if (MODE >= 2 && a.loss) {   // one atomic per wave instead of one per ray
+    loss = wave_sum(loss);
+    if (__lane_id() == 0) atomicAdd(a.loss, loss);
+  }
+}
+
+// ---- batched launch ----
+template <typename T>
+__device__ __forceinline__ uint32_t find_item(const T* begin, uint32_t n_items, T x) {
+  uint32_t lo = 0, hi = n_items;   // largest k with begin[k] <= x
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (begin[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_batch_zero(SbsBatchLaunch L) {
+  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid == 0) *L.counter = 0;
+  if (tid < L.n_items && L.items[tid].args.loss) *L.items[tid].args.loss = 0.0f;
+  for (uint64_t k = tid; k < L.total_zero; k += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t it = find_item(L.zero_begin, L.n_items, k);
+    L.items[it].args.grad[k - L.zero_begin[it]] = 0.0f;
+  }
+}
+
+__global__ void k_batch_mask_words(SbsBatchLaunch L) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= L.total_mask_words) return;
+  const uint32_t it = find_item(L.mask_begin, L.n_items, tid);
+  const SbsBatchItem& item = L.items[it];
+  const SceneView& s = item.view;
+  const SdfSBSHeader h = s.sbs_headers[0];
+  const uint32_t bs = h.brick_size, v3 = bs * bs * bs, wpb = (v3 + 31u) / 32u;
+  const uint32_t local = tid - L.mask_begin[it];
+  const uint32_t brick = local / wpb, w = local % wpb;
+  const uint32_t node_id = brick + s.sbs_roots[0];
+  const uint32_t v_size = bs + 2 * h.brick_pad + 1;
+  const float sz_inv = 2.0f / float(s.sbs_nodes[node_id].pos_z_lod_size & 0x0000FFFF);
+  uint32_t word = 0;
+  for (uint32_t b = 0; b < 32u; ++b) {
+    const uint32_t v = w * 32u + b;
+    if (v >= v3) break;
+    const float3 vp = make3(float(v / (bs * bs)), float((v / bs) % bs), float(v % bs));
+    float values[8];
+    if (sbs_load_values(s, h, node_id, vp, v_size, sz_inv, values) <= 0.0f) word |= 1u << b;
+  }
+  item.mask[size_t(brick) * (1u + wpb) + 1u + w] = word;
+}
+
+__global__ void k_batch_mask_summary(SbsBatchLaunch L) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= L.total_bricks) return;
+  const uint32_t it = find_item(L.brick_begin, L.n_items, tid);
+  const SbsBatchItem& item = L.items[it];
+  const uint32_t bs = item.brick_size, wpb = (bs * bs * bs + 31u) / 32u;
+  uint32_t* m = item.mask + size_t(tid - L.brick_begin[it]) * (1u + wpb);
+  uint32_t any = 0;
+  for (uint32_t w = 0; w < wpb; ++w) any |= m[1 + w];
+  m[0] = any != 0u ? 1u : 0u;
+}
+
+template <int S, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sbs_batch(SbsBatchLaunch L) {
+  __shared__ uint32_t lds[S * BLOCK];
+  __shared__ uint32_t chunk_sh;
+  for (;;) {
+    if (threadIdx.x == 0) chunk_sh = atomicAdd(L.counter, 1u);
+    __syncthreads();
+    const uint32_t chunk = chunk_sh;
+    __syncthreads();
+    if (chunk >= L.total_chunks) break;   // the same for the whole block: every block ends
+    const uint32_t it = __builtin_amdgcn_readfirstlane(find_item(L.chunk_begin, L.n_items, chunk));
+    const SbsBatchItem& item = L.items[it];
+    const uint32_t i = (chunk - L.chunk_begin[it]) * BLOCK + threadIdx.x;
+    float loss = 0.0f;
+    if (i < item.args.n) {
+      LaneStack<S, BLOCK> st;
+      st.col = (lds_u32*)(lds) + threadIdx.x;
+      st.spill = L.spill + size_t(blockIdx.x * BLOCK + threadIdx.x) * (STACK_SIZE - S);
+      uint32_t overflow = 0;
+      RayCounts cnt{};
+      bool aborted = false;
+      const CRT_Hit h = ray_query_nearest<LK_SBS, S, BLOCK, false, false>(item.view, item.args.pos[i], item.args.dir[i],
+                                                                          st, overflow, cnt, nullptr, 0u, aborted,
+                                                                          nullptr);
+      loss = diff_ray<3>(item.view, item.args, i, h);
+    }
+    loss = wave_sum(loss);
+    if (__lane_id() == 0 && item.args.loss) atomicAdd(item.args.loss, loss);
+  }
 }
 
 template <int MODE>
@@ -134,6 +240,21 @@ hipError_t launch_mode(const SceneView& s, const SbsDiffArgs& a, hipStream_t str
 }
 
 }  // namespace
+
+hipError_t sbs_batch_blocks_per_cu(int* blocks) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_sbs_batch<16, SBS_BATCH_BLOCK>, SBS_BATCH_BLOCK, 0);
+}
+
+hipError_t launch_sbs_batch(const SbsBatchLaunch& L, hipStream_t stream) {
+  if (L.n_items == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_zero, dim3(2048), dim3(256), 0, stream, L);
+  if (L.total_mask_words)
+    hipLaunchKernelGGL(k_batch_mask_words, dim3((L.total_mask_words + 255) / 256), dim3(256), 0, stream, L);
+  if (L.total_bricks)
+    hipLaunchKernelGGL(k_batch_mask_summary, dim3((L.total_bricks + 255) / 256), dim3(256), 0, stream, L);
+  hipLaunchKernelGGL((k_sbs_batch<16, SBS_BATCH_BLOCK>), dim3(L.grid), dim3(SBS_BATCH_BLOCK), 0, stream, L);
+  return hipGetLastError();
+}
 
 hipError_t launch_sbs_diff(const SceneView& s, const SbsDiffArgs& a, hipStream_t stream) {
   if (a.n == 0) return hipSuccess;

@@ -60,6 +60,46 @@ struct SbsDiffArgs {
   uint32_t      xcd_remap;
 };
 
+// One brick set's part of the occupancy mask: brick count, brick size, first mask word.
+struct SbsMaskGeom {
+  uint32_t sdf_id, n_bricks, brick_size, base;
+};
+// Builds the occupancy masks of the listed brick sets from the view's current values.
+hipError_t launch_sbs_masks(const SceneView& s, const SbsMaskGeom* geoms, uint32_t n_geoms, uint32_t* mask,
+                            hipStream_t stream);
+
+// One scene of a batched differentiable launch. Its view reads distances and albedo from the
+// item's parameters and uses the item's own occupancy mask; the scene holds one brick set.
+struct SbsBatchItem {
+  SceneView   view;
+  SbsDiffArgs args;      // pos, dir, n, params, geom_id, f_offset, n_dist, colour, target, grad, loss, counters
+  uint32_t*   mask;      // the item's occupancy mask (1 + ceil(bs^3/32) words per brick)
+  uint32_t    n_bricks, brick_size;
+  uint64_t    n_params;  // gradient entries zeroed by every run
+};
+
+// A batched run: zero gradients and losses, build every item's occupancy mask from its
+// parameters, then one persistent kernel whose blocks take BLOCK-ray chunks from a global
+// counter (items in order) and run the forward colour and the L2 adjoint of each ray. Only
+// kernel launches (no allocation, copy or memset), so a run can be captured in a graph.
+struct SbsBatchLaunch {
+  const SbsBatchItem* items;
+  uint32_t n_items;
+  const uint64_t* zero_begin;    // [n_items + 1] prefix sums of n_params
+  const uint32_t* mask_begin;    // [n_items + 1] prefix sums of mask words (bit words only)
+  const uint32_t* brick_begin;   // [n_items + 1] prefix sums of n_bricks
+  const uint32_t* chunk_begin;   // [n_items + 1] prefix sums of ray chunks
+  uint64_t total_zero;
+  uint32_t total_mask_words, total_bricks, total_chunks;
+  uint32_t* counter;             // work counter (reset by the zero kernel)
+  uint32_t* spill;               // grid * BLOCK * (STACK_SIZE - 16) words
+  uint32_t grid;                 // persistent blocks
+};
+constexpr int SBS_BATCH_BLOCK = 128;
+hipError_t launch_sbs_batch(const SbsBatchLaunch& L, hipStream_t stream);
+// Resident blocks of the persistent batch kernel per CU (occupancy query).
+hipError_t sbs_batch_blocks_per_cu(int* blocks);
+
 hipError_t launch_trace(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_sbs_diff(const SceneView& s, const SbsDiffArgs& a, hipStream_t stream);
 hipError_t launch_eye_rays(const EyeRayParams& prm, const float2* jitter, float4* pos, float4* dir,

@@ -452,8 +452,33 @@ void Scene::build_device_layout() {
   }
 }
 
+hipError_t Scene::mask_for(void* stream, uint32_t** mask) {
+  std::lock_guard<std::mutex> lock(spill_mu);
+  DevBuf& b = mask_bufs[stream];
+  const size_t need = std::max<size_t>(sbsMaskWords, 1) * sizeof(uint32_t);
+  if (b.bytes() < need) {
+    hipError_t e = b.reserve(need);
+    if (e != hipSuccess) return e;
+  }
+  *mask = static_cast<uint32_t*>(b.ptr());
+  return hipSuccess;
+}
+
 hipError_t Scene::commit() {
   build_device_layout();
+  // occupancy-mask layout: per brick 1 summary word + ceil(bs^3 / 32) bit words
+  sbsMaskGeoms.clear();
+  sbsMaskBase.assign(sbsRoots.size(), 0u);
+  sbsMaskWords = 0;
+  for (size_t g = 0; g < sbsRoots.size(); ++g) {
+    const uint32_t first = sbsRoots[g];
+    const uint32_t last = g + 1 < sbsRoots.size() ? sbsRoots[g + 1] : uint32_t(sbsNodes.size());
+    const uint32_t bs = sbsHeaders[g].brick_size;
+    sbsMaskBase[g] = uint32_t(sbsMaskWords);
+    sbsMaskGeoms.push_back({uint32_t(g), last - first, bs, uint32_t(sbsMaskWords)});
+    sbsMaskWords += size_t(last - first) * (1 + (size_t(bs) * bs * bs + 31) / 32);
+  }
+  if (sbsMaskWords > 0xFFFFFFFFull) return hipErrorInvalidValue;
   hipError_t e = hipGetDevice(&device);
   if (e != hipSuccess) return e;
 #define UP(name, vec)                                                        \
@@ -464,7 +489,17 @@ hipError_t Scene::commit() {
   UP("indices", indices) UP("prim_indices", primIndices) UP("orig_nodes", origNodes) UP("frame_nodes", frameNodes)
   UP("frame_roots", frameRoots) UP("sbs_nodes", sbsNodes) UP("sbs_data", sbsData) UP("sbs_data_f", sbsDataF)
   UP("sbs_roots", sbsRoots) UP("sbs_headers", sbsHeaders) UP("multi_nodes", multiNodes) UP("multi_values", multiValues)
+  UP("sbs_mask_base", sbsMaskBase)
 #undef UP
+  e = dev["sbs_mask"].reserve(sbsMaskWords * sizeof(uint32_t));
+  if (e != hipSuccess) return e;
+  if (sbsMaskWords) {
+    SceneView v = make_view();
+    v.sbs_mask = nullptr;   // built from the committed values, without a mask
+    e = launch_sbs_masks(v, sbsMaskGeoms.data(), uint32_t(sbsMaskGeoms.size()), static_cast<uint32_t*>(dev["sbs_mask"].ptr()),
+                         nullptr);
+    if (e != hipSuccess) return e;
+  }
   if (!d_overflow) {
     e = hipMalloc(&d_overflow, 2 * sizeof(uint32_t));   // [0] stack overflows, [1] adjoint recompute misses
     if (e != hipSuccess) return e;
@@ -504,6 +539,8 @@ SceneView Scene::make_view() const {
   v.sbs_data_f = (const float*)P("sbs_data_f");
   v.sbs_roots = (const uint32_t*)P("sbs_roots");
   v.sbs_headers = (const SdfSBSHeader*)P("sbs_headers");
+  v.sbs_mask = sbsMaskWords ? (const uint32_t*)P("sbs_mask") : nullptr;
+  v.sbs_mask_base = (const uint32_t*)P("sbs_mask_base");
   v.multi_nodes = (const SdfMultiOctreeNode*)P("multi_nodes");
   v.multi_values = (const float*)P("multi_values");
   v.preset = preset;

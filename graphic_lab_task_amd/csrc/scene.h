@@ -51,7 +51,10 @@ struct Scene {
   std::vector<uint32_t> sbsData;
   std::vector<float> sbsDataF;
   std::vector<uint2> sbsFSlice;     // per brick-set geometry: (first values_f entry, count)
-  std::vector<uint8_t> sbsDiffChecked;  // per brick-set geometry: 0 unchecked, 1 usable by the adjoint, 2 not
+  std::vector<uint8_t> sbsDiffChecked;
+  std::vector<SbsMaskGeom> sbsMaskGeoms;  // occupancy-mask layout of every brick set (commit)
+  std::vector<uint32_t> sbsMaskBase;      // per brick set: first mask word
+  size_t sbsMaskWords = 0;  // per brick-set geometry: 0 unchecked, 1 usable by the adjoint, 2 not
   std::vector<uint32_t> sbsRoots;
   std::vector<SdfSBSHeader> sbsHeaders;
   std::vector<SdfMultiOctreeNode> multiNodes;
@@ -79,6 +82,9 @@ struct Scene {
   uint32_t budget = 64, tail_waves = 1024;
   // Per-stream work memory for n rays: tail queue (header + list) and deep-stack spill.
   hipError_t work_for(void* stream, size_t n, int lds_stack, uint32_t** spill, TailQueue** queue);
+  // per-stream occupancy-mask buffer for calls whose values differ from the committed ones
+  hipError_t mask_for(void* stream, uint32_t** mask);
+  std::map<void*, DevBuf> mask_bufs;
 
   ~Scene();
 

@@ -54,6 +54,11 @@ struct SceneView {
   const float*        sbs_data_f;   // m_SdfSBSDataF
   const uint32_t*     sbs_roots;    // m_SdfSBSRoots
   const SdfSBSHeader* sbs_headers;  // m_SdfSBSHeaders
+  // voxel occupancy (ours): per brick one summary word (non-zero if any voxel is occupied) and
+  // ceil(bs^3/32) words with bit (x*bs+y)*bs+z set when min of the voxel's 8 corner values
+  // <= 0, i.e. exactly the voxels the brick march intersects; null = march without it
+  const uint32_t*     sbs_mask;
+  const uint32_t*     sbs_mask_base;  // first mask word of each brick set (indexed like sbs_roots)
   // multi octree
   const SdfMultiOctreeNode* multi_nodes;  // m_SdfMultiOctreeNodes
   const float*              multi_values; // m_SdfMultiOctreeValues

@@ -445,11 +445,12 @@ struct SbsVoxelRec {
 };
 
 // Sparse-brick-set leaf: BVHRT::OctreeBrickIntersect (BVH2Common.cpp:884-1112), trilinear.
-// Returns the number of voxels stepped. `rec` (indexed layouts only) records the hit voxel.
+// Returns the number of voxels whose values were loaded (all voxels stepped without an
+// occupancy mask). `rec` (indexed layouts only) records the hit voxel.
 __device__ uint32_t sbs_brick_intersect(const SceneView& s, uint32_t type, float3 ray_pos, float3 ray_dir, float tNear,
                                         uint32_t instId, uint32_t geomId, uint32_t bvhNodeId, CRT_Hit& hit,
                                         SbsVoxelRec* rec = nullptr) {
-  uint32_t steps = 0;
+  uint32_t steps = 0, loaded = 0;
   const uint32_t sdfId = s.geoms[geomId].offset.x;
   const uint32_t primId = bvhNodeId;
   const uint32_t nodeId = primId + s.sbs_roots[sdfId];
@@ -469,6 +470,13 @@ __device__ uint32_t sbs_brick_intersect(const SceneView& s, uint32_t type, float
   float2 brick_fNearFar = ray_box2(ray_pos, inv, brick_min_pos, brick_max_pos);
   const float old_t = hit.t;
   const float scale = 0.5f * sz * bs;
+  const uint32_t* bmask = nullptr;
+  if (s.sbs_mask) {
+    const uint32_t bs_u = header.brick_size;
+    bmask = s.sbs_mask + s.sbs_mask_base[sdfId] + primId * (1u + (bs_u * bs_u * bs_u + 31u) / 32u);
+    if (bmask[0] == 0u) return 0;   // no occupied voxel: the march could not change the hit
+    bmask += 1;
+  }
   const float hi = bs - 1e-6f;
   // A ray crosses at most 3*brick_size-2 voxels of a brick; the cap only ends marches whose
   // 1e-6 advance is lost to rounding (the reference would loop forever on those).
@@ -481,9 +489,19 @@ __device__ uint32_t sbs_brick_intersect(const SceneView& s, uint32_t type, float
                                   floorf(clamp_ref(local_pos.z, 1e-6f, hi)));
     const float3 min_pos = brick_min_pos + d * voxelPos;
     const float3 max_pos = min_pos + d * make3(1, 1, 1);
-    float values[8];
-    const float vmin = sbs_load_values(s, header, nodeId, voxelPos, v_size, sz_inv, values);
     const float2 fNearFar = ray_box2(ray_pos, inv, min_pos, max_pos);
+    bool occupied = true;
+    if (bmask) {
+      const uint32_t vlin = (uint32_t(voxelPos.x) * header.brick_size + uint32_t(voxelPos.y)) * header.brick_size +
+                            uint32_t(voxelPos.z);
+      occupied = ((bmask[vlin >> 5] >> (vlin & 31u)) & 1u) != 0u;
+    }
+    float values[8];
+    float vmin = 1.0f;
+    if (occupied) {
+      loaded++;
+      vmin = sbs_load_values(s, header, nodeId, voxelPos, v_size, sz_inv, values);
+    }
     if (tNear < fNearFar.x && vmin <= 0.0f) {
       const float3 start_pos = ray_pos + fNearFar.x * ray_dir;
       const float3 start_q = (start_pos - min_pos) * scale;
@@ -565,7 +583,7 @@ __device__ uint32_t sbs_brick_intersect(const SceneView& s, uint32_t type, float
       hit.coords[1] = v;
     }
   }
-  return steps;
+  return loaded;
 }
 
 }  // namespace lrt

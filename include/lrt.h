@@ -164,6 +164,31 @@ lrt_status lrt_sbs_backward_dev(lrt_scene* s, uint32_t geom_id, const float* par
 lrt_status lrt_sbs_l2_backward_dev(lrt_scene* s, uint32_t geom_id, const float* params, const float* pos_near,
                                    const float* dir_far, uint64_t n, const float* target, float* dL_dparams,
                                    float* loss, void* stream);
+/* Batched differentiable launch (no counterpart in the reference): many brick-set scenes, each
+ * with its own parameters, rays and target, run as one plan. Every run zeroes each item's grad
+ * (scene values_f-sized) and loss, builds the occupancy masks from the items' parameters, and
+ * runs ONE persistent kernel that renders the colour (if colour != NULL) and accumulates the L2
+ * adjoint of every ray of every item. A run only launches kernels (no allocation, copy or
+ * memset), so it may be captured in a HIP graph; buffers are read through the pointers given
+ * at creation. Each item's scene must be committed and hold exactly one brick set, in the
+ * differentiable layout of lrt_sbs_forward_dev. Runs of one plan must not overlap. */
+typedef struct lrt_sbs_batch lrt_sbs_batch;
+typedef struct lrt_sbs_batch_item {
+  lrt_scene* scene;
+  uint32_t geom_id;
+  const float* params;     /* device, the scene's values_f size */
+  float* grad;             /* device, same size; zeroed and accumulated by each run */
+  float* loss;             /* device, 1 float; zeroed and accumulated by each run (may be NULL) */
+  const float* pos_near;   /* device float4 per ray */
+  const float* dir_far;    /* device float4 per ray */
+  const float* target;     /* device float4 per ray (rgb used) */
+  float* colour;           /* device float4 per ray (may be NULL) */
+  uint64_t n_rays;
+} lrt_sbs_batch_item;
+lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_items, lrt_sbs_batch** out);
+lrt_status lrt_sbs_batch_run(lrt_sbs_batch* b, void* stream);
+lrt_status lrt_sbs_batch_destroy(lrt_sbs_batch* b);
+
 /* Rays whose recomputed hit voxel did not reproduce the walk's hit (expected 0). */
 lrt_status lrt_sbs_recompute_misses(lrt_scene* s, uint64_t* out, int reset);
 

@@ -143,3 +143,87 @@ def test_c3_gradients_vs_finite_differences(lrt, orc, torch_dev):
     rel = np.abs(g32[ids] - fd) / np.maximum(scale, 1e-9)
     _say(f"GPU vs central FD: p50 {np.median(rel):.3g} p90 {np.quantile(rel, 0.9):.3g} max {rel.max():.3g}")
     assert np.quantile(rel, 0.9) < 1e-3 and np.quantile(rel, 0.99) < 3e-2
+
+
+def _rel(a, b):
+    a, b = a.double(), b.double()
+    return float((a - b).norm() / b.norm().clamp_min(1e-30))
+
+
+def test_batch_matches_per_scene_calls(lrt, torch_dev):
+    """One persistent launch over several scenes equals the per-scene forward + L2 adjoint."""
+    import bench
+    torch = torch_dev
+    _say("batch vs per-scene")
+    items = bench.sbs_batch_scenes(torch, lrt, n_scenes=6, res=64, views=4, seed=3)
+    batch = lrt.SbsBatch(items)
+    batch.run()
+    torch.cuda.synchronize()
+    got = [(it["grad"].clone(), it["loss"].clone(), it["colour"].clone()) for it in items]
+    for it, (g, l, c) in zip(items, got):
+        grad = torch.zeros_like(it["params"])
+        loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+        col = torch.zeros_like(it["colour"])
+        it["scene"].sbs_forward(it["geom_id"], it["params"], it["pos"], it["dir"], col)
+        it["scene"].sbs_l2_backward(it["geom_id"], it["params"], it["pos"], it["dir"], it["target"], grad, loss)
+        torch.cuda.synchronize()
+        assert torch.equal(col, c)
+        assert _rel(g, grad) < 1e-5 and abs(l.item() - loss.item()) <= 1e-5 * max(loss.item(), 1e-30)
+        assert it["scene"].sbs_recompute_misses() == 0
+
+
+def test_batch_graph_replay(lrt, torch_dev):
+    """A captured batch run replays correctly several times, with eager work in between."""
+    import bench
+    torch = torch_dev
+    _say("batch graph replay")
+    items = bench.sbs_batch_scenes(torch, lrt, n_scenes=4, res=64, views=4, seed=4)
+    batch = lrt.SbsBatch(items)
+    batch.run()
+    torch.cuda.synchronize()
+    want = [it["grad"].clone() for it in items]
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        batch.run(side.cuda_stream)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph, stream=side):
+            batch.run(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    for r in range(3):
+        for it in items:
+            it["grad"].fill_(7.0)                 # eager work between replays
+        graph.replay()
+        torch.cuda.synchronize()
+        for it, w in zip(items, want):
+            assert _rel(it["grad"], w) < 1e-5, f"replay {r}"
+
+
+def test_batch_gradients_vs_float64(lrt, orc, torch_dev):
+    """C5-style batch (grids from {32,64,128,256}^3, 4 views): each scene's fp32 gradient against
+    the float64 twin, rel. err <= 1e-3."""
+    import bench
+    import oracle.diff as od
+    torch = torch_dev
+    _say("batch vs float64")
+    items = bench.sbs_batch_scenes(torch, lrt, n_scenes=8, res=128, views=4, seed=10, threads=16)
+    lrt.SbsBatch(items).run()
+    torch.cuda.synchronize()
+    pos, dirs = items[0]["pos"].cpu().numpy(), items[0]["dir"].cpu().numpy()
+    for it in items:
+        vf = it["params"].cpu().numpy()
+        hits, recs = od.hit_voxels(orc.Scene.from_scene(it["scene"]), pos, dirs, threads=16)
+        lay = od.Layout(0, it["n_dist"])
+        p64 = vf.astype(np.float64)
+        m = recs["hit"] == 1
+        tl = recs["t_local"].astype(np.float64)
+        C = np.zeros((recs.size, 3))
+        C[m] = od.colour(recs[m], p64, lay, tl[m])
+        g64 = od.adjoint(recs, p64, lay, 2 * (C - it["target"].cpu().numpy()[:, :3]), tl)
+        g32 = it["grad"].cpu().numpy().astype(np.float64)
+        nd = it["n_dist"]
+        r_d = np.linalg.norm(g32[:nd] - g64[:nd]) / max(np.linalg.norm(g64[:nd]), 1e-30)
+        r_a = np.linalg.norm(g32[nd:] - g64[nd:]) / max(np.linalg.norm(g64[nd:]), 1e-30)
+        _say(f"grid {it['grid']}: rel err distances {r_d:.3g} albedo {r_a:.3g}")
+        assert r_d <= 1e-3 and r_a <= 1e-3

@@ -353,3 +353,19 @@ def test_autotune_keeps_results(lrt, orc, torch_dev):
     assert r["configs_tried"] >= 10 and r["best_ms"] > 0 and r["block"] in (64, 128, 256)
     assert raw_equal(hits.cpu().numpy().view(lrt.HIT_DTYPE).reshape(-1), base)
     assert raw_equal(gpu_trace(torch_dev, s, pos, dirs), base)
+
+
+def test_tlas_batch_scene_bit_exact(lrt, orc, torch_dev):
+    """The batched two-level workload (4,096 instances of 32 geodesic meshes) with cosine-
+    hemisphere secondary rays: a quarter-million-ray sample, single pass and two-pass."""
+    import bench
+    _say("tlas 4096x32")
+    scene, meshes, mats = bench.tlas_scene()
+    pos, dirs = bench.hemisphere_rays(meshes, mats, 1 << 18, seed=19)
+    p, d = torch_dev.from_numpy(pos).cuda(), torch_dev.from_numpy(dirs).cuda()
+    want, ov = orc.trace_closest(orc.Scene.from_scene(scene), pos, dirs)
+    assert ov == 0 and (want["primId"] != MISS).mean() > 0.3
+    for budget in (0, 64):
+        scene.set_tail_config(budget, 1024)
+        got = gpu_trace(torch_dev, scene, p, d)
+        assert raw_equal(got, want), f"budget {budget}: {np.sum(got['primId'] != want['primId'])} id mismatches"
