@@ -34,6 +34,8 @@ INSTANCE_DTYPE = np.dtype([("boxMin", "<f4", (4,)), ("boxMax", "<f4", (4,)), ("g
                            ("transformInv", "<f4", (16,)),
                            ("transformInvTransposed", "<f4", (16,))])              # InstanceData, 256 B
 FRAME_NODE_DTYPE = np.dtype([("values", "<f4", (8,)), ("offset", "<u4")])         # SdfFrameOctreeNode, 36 B
+MULTI_NODE_DTYPE = np.dtype([("data_offset", "<u4"), ("children_offset", "<u4"), ("voxel_count", "<u4"),
+                             ("flags", "<u4")])                                       # SdfMultiOctreeNode, 16 B
 SBS_NODE_DTYPE = np.dtype([("pos_xy", "<u4"), ("pos_z_lod_size", "<u4"), ("data_offset", "<u4"),
                            ("_pad", "<u4")])                                       # SdfSBSNode, 16 B
 SBS_HEADER_DTYPE = np.dtype([("brick_size", "<u4"), ("brick_pad", "<u4"), ("bytes_per_value", "<u4"),
@@ -51,7 +53,8 @@ ARRAY_DTYPES = {
     "vert_pos": np.float32, "vert_norm": np.float32, "indices": np.uint32, "prim_indices": np.uint32,
     "orig_nodes": NODE_DTYPE, "frame_nodes": FRAME_NODE_DTYPE, "frame_roots": np.uint32,
     "sbs_nodes": SBS_NODE_DTYPE, "sbs_data": np.uint32, "sbs_data_f": np.float32, "sbs_roots": np.uint32,
-    "sbs_headers": SBS_HEADER_DTYPE, "preset": PRESET_DTYPE,
+    "sbs_headers": SBS_HEADER_DTYPE, "multi_nodes": MULTI_NODE_DTYPE, "multi_values": np.float32,
+    "preset": PRESET_DTYPE,
 }
 
 # enums (core/render_settings.h, BVH2Common.h:30-43, sdf_simple_shared.h:13-21)
@@ -145,6 +148,8 @@ def lib() -> ctypes.CDLL:
             "lrt_gen_icosphere": ([u32, f32, f32, u64, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
             "lrt_gen_geodesic": ([u32, f32, f32, u64, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
             "lrt_sdf_spheres_frame_octree": ([vp, sz, u32, i32, ctypes.POINTER(_Buffer)], i32),
+            "lrt_add_sdf_multi_octree": ([vp, vp, sz, vp, sz, ctypes.POINTER(u32)], i32),
+            "lrt_sdf_spheres_multi_octree": ([vp, sz, u32, i32, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
             "lrt_sdf_spheres_sbs": ([vp, sz, u32, vp, u64, i32, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
             "lrt_sdf_spheres_sbs_indexed": ([vp, sz, u32, u32, u64, i32, ctypes.POINTER(_Buffer),
                                              ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer), pu32], i32),
@@ -159,7 +164,7 @@ def lib() -> ctypes.CDLL:
 
 def exported_symbols():
     return ["lrt_last_error", "lrt_version", "lrt_create", "lrt_destroy", "lrt_clear_geom", "lrt_clear_scene",
-            "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_sbs", "lrt_add_instance",
+            "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
             "lrt_trace_closest_host", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
             "lrt_set_launch_config", "lrt_set_tail_config", "lrt_autotune", "lrt_sbs_param_layout",
@@ -255,6 +260,13 @@ class SceneRT:
         out = ctypes.c_uint32()
         _check(lib().lrt_add_sdf_frame_octree(self._h, _ptr(nodes), nodes.size, ctypes.byref(out)))
         return out.value
+
+    def AddGeom_SdfMultiOctree(self, nodes: np.ndarray, values: np.ndarray) -> int:
+        n = np.ascontiguousarray(nodes, dtype=MULTI_NODE_DTYPE).reshape(-1)
+        v = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+        gid = ctypes.c_uint32()
+        _check(lib().lrt_add_sdf_multi_octree(self._h, _ptr(n), n.size, _ptr(v), v.size, ctypes.byref(gid)))
+        return gid.value
 
     def AddGeom_SdfSBS(self, header: np.ndarray, nodes: np.ndarray, values: np.ndarray,
                        values_f: Optional[np.ndarray] = None) -> int:
@@ -455,6 +467,14 @@ def sdf_spheres_frame_octree(spheres: np.ndarray, depth: int, threads: int = 0) 
     nb = _Buffer()
     _check(lib().lrt_sdf_spheres_frame_octree(_ptr(s), s.shape[0], depth, threads, ctypes.byref(nb)))
     return _take(nb, FRAME_NODE_DTYPE)
+
+
+def sdf_spheres_multi_octree(spheres: np.ndarray, depth: int, threads: int = 0):
+    """(nodes, values) of the multi-octree of a sphere union (brick size 1)."""
+    s = np.ascontiguousarray(spheres, dtype=np.float32).reshape(-1, 4)
+    nb, vb = _Buffer(), _Buffer()
+    _check(lib().lrt_sdf_spheres_multi_octree(_ptr(s), s.shape[0], depth, threads, ctypes.byref(nb), ctypes.byref(vb)))
+    return _take(nb, MULTI_NODE_DTYPE), _take(vb, np.float32)
 
 
 def sdf_spheres_sbs(spheres: np.ndarray, depth: int, header: np.ndarray, threads: int = 0, colour_seed: int = 0):

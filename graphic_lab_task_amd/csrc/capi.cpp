@@ -249,6 +249,17 @@ lrt_status lrt_add_sdf_sbs(lrt_scene* s, const void* header, const void* nodes, 
   });
 }
 
+lrt_status lrt_add_sdf_multi_octree(lrt_scene* s, const void* nodes, size_t n_nodes, const float* values,
+                                    size_t n_values, uint32_t* out_geom_id) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !nodes || !values) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    const uint32_t id = s->s.add_sdf_multi_octree(static_cast<const lrt::SdfMultiOctreeNode*>(nodes), n_nodes, values,
+                                                  n_values);
+    if (out_geom_id) *out_geom_id = id;
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_clear_scene(lrt_scene* s) {
   return guarded([&]() -> lrt_status {
     if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
@@ -634,13 +645,14 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
     ARR("prim_indices", c.primIndices) ARR("orig_nodes", c.origNodes) ARR("frame_nodes", c.frameNodes)
     ARR("frame_roots", c.frameRoots) ARR("sbs_nodes", c.sbsNodes) ARR("sbs_data", c.sbsData)
     ARR("sbs_data_f", c.sbsDataF) ARR("sbs_roots", c.sbsRoots) ARR("sbs_headers", c.sbsHeaders)
+    ARR("multi_nodes", c.multiNodes) ARR("multi_values", c.multiValues)
 #undef ARR
     if (n == "preset") { src = &c.preset; sz = sizeof(c.preset); }
     if (!src && sz == 0) {
       static const char* known[] = {"nodes", "tlas", "instances", "geoms", "geom_tags", "start_end",
                                     "prim_id_count", "vert_pos", "vert_norm", "indices", "prim_indices",
                                     "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes", "sbs_data",
-                                    "sbs_data_f", "sbs_roots", "sbs_headers", "preset"};
+                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset"};
       bool ok = false;
       for (const char* k : known) ok = ok || n == k;
       if (!ok) return fail(LRT_ERR_INVALID_ARG, "unknown array '" + n + "'");

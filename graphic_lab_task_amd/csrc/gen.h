@@ -36,6 +36,10 @@ void sdf_to_global_octree(const SphereUnion& sdf, uint32_t depth, uint32_t brick
                           int threads, GlobalOctree& out);
 // global_octree_to_frame_octree (sdf/simple/sdf_converter.cpp:53-65)
 void global_octree_to_frame_octree(const GlobalOctree& g, std::vector<SdfFrameOctreeNode>& out);
+// global_octree_to_multi_octree (sdf/multi/sdf_multi.cpp:124-180): one node per octree node,
+// voxel_count 1 for active nodes (the builder's bricks_count), 0 for empty ones; values as is.
+void global_octree_to_multi_octree(const GlobalOctree& g, std::vector<SdfMultiOctreeNode>& nodes,
+                                   std::vector<float>& values);
 // global_octree_to_SBS (sdf/simple/sdf_converter.cpp:156-225), quantised DX layout
 // colour_seed != 0 fills the per-corner words of the DX_UV16 / DX_RGB8 layouts from a seeded field.
 void global_octree_to_sbs_dx(const GlobalOctree& g, const SdfSBSHeader& header, uint64_t colour_seed,

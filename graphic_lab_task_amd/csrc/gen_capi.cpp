@@ -99,6 +99,21 @@ lrt_status lrt_sdf_spheres_frame_octree(const float* spheres, size_t n_spheres, 
   });
 }
 
+lrt_status lrt_sdf_spheres_multi_octree(const float* spheres, size_t n_spheres, uint32_t depth, int threads,
+                                        lrt_buffer* nodes, lrt_buffer* values) {
+  if (!spheres || n_spheres == 0 || !nodes || !values) return LRT_ERR_INVALID_ARG;
+  return guarded([&]() -> lrt_status {
+    lrt::GlobalOctree g;
+    lrt::sdf_to_global_octree(lrt::SphereUnion{spheres, n_spheres}, depth, 1, 0, threads, g);
+    std::vector<lrt::SdfMultiOctreeNode> n;
+    std::vector<float> v;
+    lrt::global_octree_to_multi_octree(g, n, v);
+    lrt_status st = to_buffer(n, nodes);
+    if (st != LRT_OK) return st;
+    return to_buffer(v, values);
+  });
+}
+
 lrt_status lrt_sdf_spheres_sbs(const float* spheres, size_t n_spheres, uint32_t depth, const void* header,
                                uint64_t colour_seed, int threads, lrt_buffer* nodes, lrt_buffer* values) {
   if (!spheres || n_spheres == 0 || !header || !nodes || !values) return LRT_ERR_INVALID_ARG;

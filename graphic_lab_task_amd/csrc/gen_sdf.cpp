@@ -147,6 +147,21 @@ void sdf_to_global_octree(const SphereUnion& sdf, uint32_t depth, uint32_t brick
   }
 }
 
+// global_octree_to_multi_octree (sdf/multi/sdf_multi.cpp:124-180)
+void global_octree_to_multi_octree(const GlobalOctree& g, std::vector<SdfMultiOctreeNode>& nodes,
+                                   std::vector<float>& values) {
+  if (g.brick_size != 1 || g.brick_pad != 0) throw std::invalid_argument("multi octree needs brick size 1, pad 0");
+  nodes.resize(g.nodes.size());
+  for (size_t i = 0; i < g.nodes.size(); ++i) {
+    nodes[i].children_offset = g.nodes[i].offset;
+    nodes[i].data_offset = g.nodes[i].val_off;
+    // add_global_node_rec: bricks_count = 1, reset to 0 when the node is not active
+    nodes[i].voxel_count = (g.nodes[i].type == 1 || g.nodes[i].type == 3) ? 1u : 0u;
+    nodes[i].flags = 0;   // is_surfaced is never set by the builder
+  }
+  values = g.values;
+}
+
 void global_octree_to_frame_octree(const GlobalOctree& g, std::vector<SdfFrameOctreeNode>& out) {
   if (g.brick_size != 1 || g.brick_pad != 0) throw std::invalid_argument("frame octree needs brick size 1, pad 0");
   out.resize(g.nodes.size());

@@ -144,6 +144,9 @@ constexpr uint32_t TYPE_MESH_TRIANGLE    = 0;
 constexpr uint32_t TYPE_SDF_FRAME_OCTREE = 3;
 constexpr uint32_t TYPE_SDF_SBS          = 6;
 constexpr uint32_t TYPE_SDF_MULTI_OCTREE = 9;
+constexpr uint32_t MULTI_OCTREE_FLAG_SURFACE = 1;          // sdf_multi_shared.h:5
+constexpr uint32_t MULTI_RENDER_MODE_MO_VOXEL_COUNT = 24;  // render_settings.h:85-86
+constexpr uint32_t MULTI_RENDER_MODE_MO_FLAGS = 25;
 constexpr uint32_t TYPE_SDF_SBS_TEX      = 10;
 constexpr uint32_t TYPE_SDF_SBS_COL      = 11;
 constexpr uint32_t SH_TYPE      = 24;

@@ -210,6 +210,51 @@ uint32_t Scene::add_sdf_frame_octree(const SdfFrameOctreeNode* nodes, size_t n) 
   return add_aabb(TAG_SDF_NODE, boxes.data(), boxes.size());
 }
 
+// BVHRT::AddGeom_SdfMultiOctree (BVH2Common_host.cpp:970-1018): the octree is walked by its
+// own leaf routine; the BLAS is the [-1,1]^3 box plus a tiny dummy box. Nodes must be stored
+// after their parent (children_offset > index) and at most 15 levels deep (16-bit positions in
+// the walk), and every node's voxel values must lie inside `values`.
+uint32_t Scene::add_sdf_multi_octree(const SdfMultiOctreeNode* nodes, size_t n_nodes, const float* values,
+                                     size_t n_values) {
+  if (n_nodes == 0 || n_values == 0) throw std::invalid_argument("empty multi octree");
+  if (n_nodes >= (1u << 28)) throw std::invalid_argument("multi octree too large");
+  std::vector<uint8_t> depth(n_nodes, 0);
+  for (size_t i = 0; i < n_nodes; ++i) {
+    const SdfMultiOctreeNode& nd = nodes[i];
+    if (uint64_t(nd.data_offset) + 8ull * nd.voxel_count > n_values)
+      throw std::invalid_argument("multi octree node values out of range");
+    if (nd.children_offset > 0) {
+      if (nd.children_offset <= i || uint64_t(nd.children_offset) + 8 > n_nodes)
+        throw std::invalid_argument("multi octree children must follow their parent and lie inside the node array");
+      for (uint32_t c = 0; c < 8; ++c) {
+        if (depth[i] + 1 > 15) throw std::invalid_argument("multi octree deeper than 15 levels");
+        depth[nd.children_offset + c] = uint8_t(depth[i] + 1);
+      }
+    }
+  }
+  const uint32_t n_offset = uint32_t(multiNodes.size());
+  const uint32_t v_offset = uint32_t(multiValues.size());
+  GeomData g{};
+  g.boxMin = make4(-1, -1, -1, 1);
+  g.boxMax = make4(1, 1, 1, 1);
+  g.offset = uint2{n_offset, 0};
+  g.bvhOffset = uint32_t(nodePairs.size());
+  g.type = TYPE_SDF_MULTI_OCTREE;
+  geomData.push_back(g);
+  multiNodes.insert(multiNodes.end(), nodes, nodes + n_nodes);
+  multiValues.insert(multiValues.end(), values, values + n_values);
+  for (size_t i = n_offset; i < multiNodes.size(); i++) {
+    if (multiNodes[i].children_offset > 0) multiNodes[i].children_offset += n_offset;
+    multiNodes[i].data_offset += v_offset;
+  }
+  BVHNode boxes[2]{};
+  boxes[0].boxMin = make3(-1, -1, -1);
+  boxes[0].boxMax = make3(1, 1, 1);
+  boxes[1].boxMin = make3(-1.0001f, -1.0001f, -1.0001f);
+  boxes[1].boxMax = make3(-1, -1, -1);
+  return add_aabb(TAG_SDF_OCTREE, boxes, 2);
+}
+
 // BVHRT::AddGeom_SdfSBS (BVH2Common_host.cpp:601-695)
 uint32_t Scene::add_sdf_sbs(const SdfSBSHeader& header, const SdfSBSNode* nodes, size_t n_nodes,
                             const uint32_t* values, size_t n_values, const float* values_f, size_t n_values_f) {

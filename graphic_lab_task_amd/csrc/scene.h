@@ -96,6 +96,8 @@ struct Scene {
   uint32_t add_sdf_frame_octree(const SdfFrameOctreeNode* nodes, size_t n);
   uint32_t add_sdf_sbs(const SdfSBSHeader& header, const SdfSBSNode* nodes, size_t n_nodes, const uint32_t* values,
                        size_t n_values, const float* values_f, size_t n_values_f);
+  uint32_t add_sdf_multi_octree(const SdfMultiOctreeNode* nodes, size_t n_nodes, const float* values,
+                                size_t n_values);
   uint32_t add_instance(uint32_t geom_id, const float4x4& m);
   void update_instance(uint32_t inst_id, const float4x4& m);
   void build_host();   // TLAS over the instance boxes (host only)

@@ -181,11 +181,12 @@ __device__ __forceinline__ float eval_dist_trilinear(const float v[8], float3 dp
 // BVHRT::LocalSurfaceIntersection (BVH2Common.cpp:345-725) for the non-compact SDF types
 // used here (frame octree, SBS, multi octree). `is_surface_node` is always false for them.
 // Returns true when the hit record was updated; then *t_local (if given) receives the hit
-// parameter in voxel units along ray_dir from start_q.
+// parameter in voxel units along ray_dir from start_q. `surface_node` (multi-octree nodes
+// flagged as surface) makes the start sign follow the start distance, as SURFACE mode does.
 __device__ bool local_surface_intersection(const SceneView& s, uint32_t type, float3 ray_dir, uint32_t instId,
                                            uint32_t geomId, const float values[8], uint32_t primId, float d,
                                            float qNear, float qFar, float2 fNearFar, float3 start_q,
-                                           CRT_Hit& hit, float* t_local = nullptr) {
+                                           CRT_Hit& hit, float* t_local = nullptr, bool surface_node = false) {
   const float EPS = 1e-6f;
   const float d_inv = 1.0f / d;
   float t = qNear;
@@ -194,7 +195,7 @@ __device__ bool local_surface_intersection(const SceneView& s, uint32_t type, fl
   unsigned iter = 0;
   float start_sign = 1;
   float start_dist = eval_dist_trilinear(values, start_q + t * ray_dir);
-  if (s.preset.representation_mode == REPRESENTATION_MODE_SURFACE) start_sign = sign_ref(start_dist);
+  if (surface_node || s.preset.representation_mode == REPRESENTATION_MODE_SURFACE) start_sign = sign_ref(start_dist);
   start_dist *= start_sign;
   const uint32_t mode = s.preset.sdf_node_intersect;
 
@@ -584,6 +585,157 @@ __device__ uint32_t sbs_brick_intersect(const SceneView& s, uint32_t type, float
     }
   }
   return loaded;
+}
+
+}  // namespace lrt
+
+namespace lrt {
+
+// ---- multi-octree (BVHRT::OctreeIntersect, MultiRenderer/BVH2Common.cpp:2176-2363) ----
+// first_node / new_node of the parametric octree walk (utils/raytracing/intersection_common.h:58-93)
+__device__ __forceinline__ int ot_first_node(float3 t0, float3 tm) {
+  uint32_t answer = 0;
+  if (t0.x > t0.y) {
+    if (t0.x > t0.z) {
+      if (t0.x > tm.y) answer |= 2;
+      if (t0.x > tm.z) answer |= 1;
+      return int(answer);
+    }
+  } else {
+    if (t0.y > t0.z) {
+      if (tm.x < t0.y) answer |= 4;
+      if (t0.y > tm.z) answer |= 1;
+      return int(answer);
+    }
+  }
+  if (tm.x < t0.z) answer |= 4;
+  if (tm.y < t0.z) answer |= 2;
+  return int(answer);
+}
+
+__device__ __forceinline__ int ot_new_node(float txm, int x, float tym, int y, float tzm, int z) {
+  return (txm < tym) ? (txm < tzm ? x : z) : (tym < tzm ? y : z);
+}
+
+struct OTEntry {
+  uint32_t node_id;
+  uint32_t info;    // > 0: leaf (its data offset)
+  uint32_t p_xy;    // (x << 16) | y
+  uint32_t p_zs;    // (z << 16) | level size
+};
+constexpr int OT_STACK = 32;
+
+// Walks the octree in front-to-back child order with mirrored coordinates (octant mask `a`)
+// and stops at the first leaf whose surface is hit. The stack lives in private memory.
+__device__ void multi_octree_intersect(const SceneView& s, uint32_t type, float3 ray_pos, float3 ray_dir, float tNear,
+                                       uint32_t instId, uint32_t geomId, CRT_Hit& hit) {
+  float3 pos_ray_pos = ray_pos;
+  float3 pos_ray_dir = ray_dir;
+  uint32_t a = 0;
+  if (ray_dir.x < 0) { pos_ray_pos.x *= -1; pos_ray_dir.x *= -1; a |= 4; }
+  if (ray_dir.y < 0) { pos_ray_pos.y *= -1; pos_ray_dir.y *= -1; a |= 2; }
+  if (ray_dir.z < 0) { pos_ray_pos.z *= -1; pos_ray_dir.z *= -1; a |= 1; }
+  pos_ray_pos = -1.0f * pos_ray_pos;
+  const float3 inv = safe_inverse(pos_ray_dir);
+  const float3 t0_ = pos_ray_pos * inv - inv;
+  const float3 t1_ = pos_ray_pos * inv + inv;
+  const float3 l_ = t1_ - t0_;
+  // nn_indices of the reference, packed as x | y << 4 | z << 8
+  const uint32_t nn[8] = {0x124u, 0x835u, 0x386u, 0x887u, 0x568u, 0x878u, 0x788u, 0x888u};
+
+  OTEntry stack[OT_STACK];
+  int top = 0;
+  const float old_t = hit.t;
+  const uint32_t root = s.geoms[geomId].offset.x;
+  stack[0] = OTEntry{root, 0u, 0u, 1u};
+  while (top >= 0) {
+    const uint32_t level_sz = stack[top].p_zs & 0xFFFF;
+    const uint32_t px = stack[top].p_xy >> 16, py = stack[top].p_xy & 0xFFFF, pz = stack[top].p_zs >> 16;
+    float d = 1.0f / float(level_sz);
+    const float3 p_f = make3(float(px), float(py), float(pz));
+    const float3 t0 = t0_ + d * p_f * l_;
+    const float3 t1 = t0_ + d * (p_f + 1.0f) * l_;
+    const float t_out = fmin_ref(fmin_ref(t1.x, t1.y), t1.z);
+    if (t_out < tNear) {
+      top--;
+      continue;
+    }
+    bool is_leaf = stack[top].info > 0;
+    if (!is_leaf) {
+      const float t = s.preset.fixed_lod > 0 ? 1.0f : fmax_ref(t0.x, fmax_ref(t0.y, t0.z));
+      const float target_lod_size = (powf(2.0f, s.preset.level_of_detail) - 0.01f) / t;
+      if (float(level_sz) > target_lod_size) {
+        const SdfMultiOctreeNode n = s.multi_nodes[stack[top].node_id];
+        stack[top].info = n.data_offset;
+        is_leaf = true;
+        if (n.voxel_count == 0) {
+          top--;
+          continue;
+        }
+      }
+    }
+    if (is_leaf) {
+      const float tmin = fmax_ref(t0.x, fmax_ref(t0.y, t0.z));
+      const float tmax = fmin_ref(t1.x, fmin_ref(t1.y, t1.z));
+      const uint32_t p_mask = level_sz - 1;
+      const uint32_t rx = (a & 4) ? (~px & p_mask) : px;
+      const uint32_t ry = (a & 2) ? (~py & p_mask) : py;
+      const uint32_t rz = (a & 1) ? (~pz & p_mask) : pz;
+      const uint32_t node_id = stack[top].node_id;
+      const float sz = 0.5f * float(level_sz);
+      d = 1.0f / sz;
+      const float3 min_pos = make3(-1, -1, -1) + d * make3(float(rx), float(ry), float(rz));
+      float2 fNearFar;
+      fNearFar.x = tmin;
+      fNearFar.y = tmax;
+      const float3 start_pos = ray_pos + fNearFar.x * ray_dir;
+      const float3 start_q = sz * (start_pos - min_pos);
+      const float qNear = fmax_ref(0.0f, sz * (tNear - fNearFar.x));
+      const float qFar = sz * (fNearFar.y - fNearFar.x);
+      const SdfMultiOctreeNode n = s.multi_nodes[node_id];
+      const bool surface = (n.flags & MULTI_OCTREE_FLAG_SURFACE) != 0;
+      const uint32_t primId = node_id - root;
+      for (uint32_t v = 0; v < n.voxel_count; ++v) {
+        float values[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) values[i] = s.multi_values[stack[top].info + 8 * v + i];
+        local_surface_intersection(s, TYPE_SDF_MULTI_OCTREE, ray_dir, instId, geomId, values, primId, d, qNear, qFar,
+                                   fNearFar, start_q, hit, nullptr, surface);
+      }
+      if (s.preset.render_mode == MULTI_RENDER_MODE_MO_VOXEL_COUNT) {
+        if (hit.t < old_t) hit.primId = n.voxel_count;
+      } else if (s.preset.render_mode == MULTI_RENDER_MODE_MO_FLAGS) {
+        if (hit.t < old_t) hit.primId = n.flags;
+      }
+      top = (hit.t < old_t) ? -1 : top - 1;
+    } else {
+      OTEntry buf[4];
+      int buf_top = 0;
+      const float3 tm = 0.5f * (t0 + t1);
+      int curr = ot_first_node(t0, tm);
+      const uint32_t children = s.multi_nodes[stack[top].node_id].children_offset;
+      do {
+        const uint32_t child = uint32_t(curr) ^ a;
+        const SdfMultiOctreeNode cn = s.multi_nodes[children + child];
+        const bool child_is_node = cn.children_offset > 0;
+        if ((cn.voxel_count > 0 || child_is_node) && buf_top < 4) {
+          buf[buf_top].node_id = children + child;
+          buf[buf_top].info = child_is_node ? 0u : cn.data_offset;
+          buf[buf_top].p_xy = (stack[top].p_xy << 1) | (((uint32_t(curr) & 4u) << (16 - 2)) | ((uint32_t(curr) & 2u) >> 1));
+          buf[buf_top].p_zs = (stack[top].p_zs << 1) | ((uint32_t(curr) & 1u) << 16);
+          buf_top++;
+        }
+        const uint32_t ni = nn[curr];
+        curr = ot_new_node((curr & 4) ? t1.x : tm.x, int(ni & 0xF), (curr & 2) ? t1.y : tm.y, int((ni >> 4) & 0xF),
+                           (curr & 1) ? t1.z : tm.z, int(ni >> 8));
+      } while (curr < 8);
+      // children replace the current entry, the first one on top; entries past the stack are dropped
+      for (int i = 0; i < buf_top; i++)
+        if (top + i < OT_STACK) stack[top + i] = buf[buf_top - i - 1];
+      top += buf_top - 1;
+      if (top >= OT_STACK) top = OT_STACK - 1;
+    }
+  }
 }
 
 }  // namespace lrt

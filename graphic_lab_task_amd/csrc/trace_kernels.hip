@@ -184,7 +184,8 @@ hipError_t launch_trace(const SceneView& s, const TraceArgs& a, hipStream_t stre
   if (k == LK_TRI) return launch_kinds<LK_TRI>(s, a, stream);
   if (k == LK_FRAME) return launch_kinds<LK_FRAME>(s, a, stream);
   if (k == LK_SBS) return launch_kinds<LK_SBS>(s, a, stream);
-  return launch_kinds<LK_TRI | LK_FRAME | LK_SBS>(s, a, stream);
+  if (k == LK_MULTI) return launch_kinds<LK_MULTI>(s, a, stream);
+  return launch_kinds<LK_TRI | LK_FRAME | LK_SBS | LK_MULTI>(s, a, stream);
 }
 
 hipError_t launch_eye_rays(const EyeRayParams& prm, const float2* jitter, float4* pos, float4* dir,

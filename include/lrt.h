@@ -78,6 +78,13 @@ lrt_status lrt_add_sdf_sbs(lrt_scene* s, const void* header, const void* nodes, 
                            const uint32_t* values, size_t n_values, const float* values_f, size_t n_values_f,
                            uint32_t* out_geom_id);
 
+/* Replaces BVHRT::AddGeom_SdfMultiOctree (BVH2Common_host.cpp:970-1018). nodes:
+ * SdfMultiOctreeNode[n_nodes] (data_offset, children_offset, voxel_count, flags,
+ * sdf/multi/sdf_multi_shared.h:8-14; children stored after their parent, at most 15 levels),
+ * values: 8 floats per voxel. */
+lrt_status lrt_add_sdf_multi_octree(lrt_scene* s, const void* nodes, size_t n_nodes, const float* values,
+                                    size_t n_values, uint32_t* out_geom_id);
+
 /* Replaces BVHRT::ClearScene (clears instances, keeps geometry). */
 lrt_status lrt_clear_scene(lrt_scene* s);
 

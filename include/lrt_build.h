@@ -47,6 +47,11 @@ lrt_status lrt_gen_geodesic(uint32_t n, float amplitude, float frequency, uint64
 lrt_status lrt_sdf_spheres_frame_octree(const float* spheres, size_t n_spheres, uint32_t depth, int threads,
                                         lrt_buffer* nodes);
 
+/* sdf_to_global_octree (brick size 1) + global_octree_to_multi_octree (sdf/multi/sdf_multi.cpp:
+ * 124-180) for the sphere union: nodes = SdfMultiOctreeNode array, values = 8 floats per node. */
+lrt_status lrt_sdf_spheres_multi_octree(const float* spheres, size_t n_spheres, uint32_t depth, int threads,
+                                        lrt_buffer* nodes, lrt_buffer* values);
+
 /* sdf_converter::create_sdf_SBS (sdf/simple/sdf_converter.cpp:311-322) for the sphere union:
  * quantised DX / DX_UV16 / DX_RGB8 layouts. header: SdfSBSHeader (brick_size, brick_pad,
  * bytes_per_value, aux_data). colour_seed != 0 fills the 8 per-corner texture (u16|v16) or

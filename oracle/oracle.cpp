@@ -97,6 +97,7 @@ struct Inst { f4 boxMin, boxMax; uint32_t geomId, pad[7]; float transform[16], t
 struct FrameNode { float values[8]; uint32_t offset; };
 struct SBSNode { uint32_t pos_xy, pos_z_lod_size, data_offset, pad; };
 struct SBSHeader { uint32_t brick_size, brick_pad, bytes_per_value, aux_data; };
+struct MultiNode { uint32_t data_offset, children_offset, voxel_count, flags; };   // sdf_multi_shared.h:8-14
 struct Preset {
   uint32_t render_mode, sdf_node_intersect, normal_mode, ray_gen_mode, interpolation_mode, representation_mode, spp,
       fixed_lod;
@@ -108,7 +109,7 @@ static_assert(sizeof(Geom) == 64 && sizeof(Inst) == 256 && sizeof(FrameNode) == 
 
 constexpr uint32_t LEAF_BIT = 0x80000000u, START_MASK = 0x0FFFFFFFu, SIZE_MASK = 0x70000000u;
 constexpr int STACK_SIZE = 80;
-constexpr uint32_t TAG_NONE = 0, TAG_TRIANGLE = 1, TAG_SDF_BRICK = 4, TAG_SDF_NODE = 8;
+constexpr uint32_t TAG_NONE = 0, TAG_TRIANGLE = 1, TAG_SDF_BRICK = 4, TAG_SDF_NODE = 8, TAG_SDF_OCTREE = 15;
 constexpr uint32_t TYPE_MESH_TRIANGLE = 0, TYPE_SDF_FRAME_OCTREE = 3, SH_TYPE = 24, GEOM_ID_MASK = 0x00FFFFFFu;
 constexpr uint32_t LAYOUT_DX_UV16 = 2u << 24, LAYOUT_DX_RGB8 = 3u << 24, LAYOUT_ID32F = 4u << 24, LAYOUT_ID32F_IN = 5u << 24;
 
@@ -137,6 +138,8 @@ struct OracleScene {
   const float* sbs_data_f;
   const uint32_t* sbs_roots;
   const SBSHeader* sbs_headers;
+  const MultiNode* multi_nodes;
+  const float* multi_values;
   Preset preset;
 };
 
@@ -242,7 +245,8 @@ struct Tracer {
 
   // BVH2Common.cpp:345-725 (non-compact SDF types, no cutting plane, no tex types)
   void LocalSurfaceIntersection(uint32_t type, f3 ray_dir, uint32_t instId, uint32_t geomId, const float values[8],
-                                uint32_t primId, float d, float qNear, float qFar, f2 fNearFar, f3 start_q, Hit* pHit) {
+                                uint32_t primId, float d, float qNear, float qFar, f2 fNearFar, f3 start_q, Hit* pHit,
+                                bool is_surface_node = false) {
     const float EPS = 1e-6f;
     const float d_inv = 1.0f / d;
     float t = qNear;
@@ -250,7 +254,7 @@ struct Tracer {
     unsigned iter = 0;
     float start_sign = 1;
     float start_dist = eval_dist_trilinear(values, start_q + t * ray_dir);
-    if (s.preset.representation_mode == 0) start_sign = lm_sign(start_dist);
+    if (is_surface_node || s.preset.representation_mode == 0) start_sign = lm_sign(start_dist);
     start_dist *= start_sign;
     const uint32_t mode = s.preset.sdf_node_intersect;
     if (start_dist <= EPS || mode == 3) {
@@ -548,6 +552,107 @@ struct Tracer {
     }
   }
 
+  // BVH2Common.cpp:2176-2363 with first_node/new_node of intersection_common.h:58-93: the
+  // octree as [-1,1]^3, coordinates mirrored so the ray direction is positive (octant mask a),
+  // children visited in parametric front-to-back order, the walk ending at the first leaf hit.
+  static int FirstNode(f3 t0, f3 tm) {
+    int answer = 0;
+    if (t0.x > t0.y) {
+      if (t0.x > t0.z) {
+        if (t0.x > tm.y) answer |= 2;
+        if (t0.x > tm.z) answer |= 1;
+        return answer;
+      }
+    } else if (t0.y > t0.z) {
+      if (tm.x < t0.y) answer |= 4;
+      if (t0.y > tm.z) answer |= 1;
+      return answer;
+    }
+    if (tm.x < t0.z) answer |= 4;
+    if (tm.y < t0.z) answer |= 2;
+    return answer;
+  }
+  static int NewNode(float txm, int x, float tym, int y, float tzm, int z) {
+    return (txm < tym) ? (txm < tzm ? x : z) : (tym < tzm ? y : z);
+  }
+
+  void OctreeIntersect(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId, Hit* pHit) {
+    struct Elem { uint32_t nodeId, info, px, py, pz, size; };
+    f3 rp = ray_pos, rd = ray_dir;
+    uint32_t a = 0;
+    if (ray_dir.x < 0) { rp.x *= -1; rd.x *= -1; a |= 4; }
+    if (ray_dir.y < 0) { rp.y *= -1; rd.y *= -1; a |= 2; }
+    if (ray_dir.z < 0) { rp.z *= -1; rd.z *= -1; a |= 1; }
+    rp = rp * -1.0f;
+    const f3 inv = SafeInverse(rd);
+    const f3 T0 = rp * inv - inv, T1 = rp * inv + inv, L = T1 - T0;
+    static const int nn[8][3] = {{4, 2, 1}, {5, 3, 8}, {6, 8, 3}, {7, 8, 8}, {8, 6, 5}, {8, 7, 8}, {8, 8, 7}, {8, 8, 8}};
+    Elem stack[32];
+    int top = 0;
+    const float old_t = pHit->t;
+    const uint32_t root = s.geoms[geomId].offset[0];
+    stack[0] = Elem{root, 0, 0, 0, 0, 1};
+    while (top >= 0) {
+      Elem& e = stack[top];
+      float d = 1.0f / float(e.size);
+      const f3 pf(float(e.px), float(e.py), float(e.pz));
+      const f3 t0 = T0 + d * pf * L;
+      const f3 t1 = T0 + d * (pf + 1.0f) * L;
+      if (std::min(std::min(t1.x, t1.y), t1.z) < tNear) { top--; continue; }
+      bool leaf = e.info > 0;
+      if (!leaf) {
+        const float t = s.preset.fixed_lod > 0 ? 1.0f : std::max(t0.x, std::max(t0.y, t0.z));
+        const float target = (std::pow(2.0f, s.preset.level_of_detail) - 0.01f) / t;
+        if (float(e.size) > target) {
+          e.info = s.multi_nodes[e.nodeId].data_offset;
+          leaf = true;
+          if (s.multi_nodes[e.nodeId].voxel_count == 0) { top--; continue; }
+        }
+      }
+      if (leaf) {
+        const uint32_t mask = e.size - 1;
+        const uint32_t rx = (a & 4) ? (~e.px & mask) : e.px, ry = (a & 2) ? (~e.py & mask) : e.py,
+                       rz = (a & 1) ? (~e.pz & mask) : e.pz;
+        const float sz = 0.5f * float(e.size);
+        d = 1.0f / sz;
+        const f3 mn = f3(-1, -1, -1) + d * f3(float(rx), float(ry), float(rz));
+        const f2 fnf{std::max(t0.x, std::max(t0.y, t0.z)), std::min(t1.x, std::min(t1.y, t1.z))};
+        const f3 start_q = sz * ((ray_pos + fnf.x * ray_dir) - mn);
+        const float qNear = std::max(0.0f, sz * (tNear - fnf.x));
+        const float qFar = sz * (fnf.y - fnf.x);
+        const MultiNode& n = s.multi_nodes[e.nodeId];
+        for (uint32_t v = 0; v < n.voxel_count; v++) {
+          float values[8];
+          for (int i = 0; i < 8; i++) values[i] = s.multi_values[e.info + 8 * v + i];
+          LocalSurfaceIntersection(9, ray_dir, instId, geomId, values, e.nodeId - root, d, qNear, qFar, fnf, start_q,
+                                   pHit, (n.flags & 1u) != 0);
+        }
+        if (s.preset.render_mode == 24 && pHit->t < old_t) pHit->primId = n.voxel_count;
+        else if (s.preset.render_mode == 25 && pHit->t < old_t) pHit->primId = n.flags;
+        top = pHit->t < old_t ? -1 : top - 1;
+      } else {
+        Elem buf[4];
+        int nb = 0;
+        const f3 tm = 0.5f * (t0 + t1);
+        int cur = FirstNode(t0, tm);
+        const uint32_t children = s.multi_nodes[e.nodeId].children_offset;
+        do {
+          const uint32_t child = uint32_t(cur) ^ a;
+          const MultiNode& c = s.multi_nodes[children + child];
+          const bool is_node = c.children_offset > 0;
+          if ((c.voxel_count > 0 || is_node) && nb < 4)
+            buf[nb++] = Elem{children + child, is_node ? 0u : c.data_offset, 2 * e.px + ((cur & 4) ? 1u : 0u),
+                             2 * e.py + ((cur & 2) ? 1u : 0u), 2 * e.pz + ((cur & 1) ? 1u : 0u), 2 * e.size};
+          cur = NewNode((cur & 4) ? t1.x : tm.x, nn[cur][0], (cur & 2) ? t1.y : tm.y, nn[cur][1], (cur & 1) ? t1.z : tm.z,
+                        nn[cur][2]);
+        } while (cur < 8);
+        for (int i = 0; i < nb; i++)
+          if (top + i < 32) stack[top + i] = buf[nb - i - 1];
+        top = std::min(top + nb - 1, 31);
+      }
+    }
+  }
+
   // BVH2Common.cpp:47-124
   uint32_t IntersectAllPrimitivesInLeaf(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
                                         uint32_t aabbId, Hit* pHit) {
@@ -561,6 +666,7 @@ struct Tracer {
       case TAG_TRIANGLE: IntersectAllTrianglesInLeaf(ray_pos, ray_dir, tNear, instId, geomId, a_start, a_count, pHit); break;
       case TAG_SDF_NODE: OctreeNodeIntersect(s.geoms[geomId].type, ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
       case TAG_SDF_BRICK: OctreeBrickIntersect(s.geoms[geomId].type, ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
+      case TAG_SDF_OCTREE: OctreeIntersect(ray_pos, ray_dir, tNear, instId, geomId, pHit); break;
       default: break;
     }
     if (s.preset.render_mode == 26) pHit->instId = (pHit->instId == 0xFFFFFFFF) ? 1 : pHit->instId + 1;

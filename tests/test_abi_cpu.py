@@ -90,3 +90,20 @@ def test_icosphere_and_geodesic_counts(lrt):
         assert f.shape[0] == 20 * n * n and v.shape[0] == 10 * n * n + 2
     v, _ = lrt.gen_icosphere(3)
     assert np.allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-6)
+
+
+def test_multi_octree_validation(lrt):
+    nodes = np.zeros(9, lrt.MULTI_NODE_DTYPE)
+    nodes[0]["children_offset"] = 1
+    values = np.zeros(8 * 9, np.float32)
+    s = lrt.SceneRT()
+    assert s.AddGeom_SdfMultiOctree(nodes, values) == 0
+    bad = nodes.copy()
+    bad[3]["children_offset"] = 2          # child stored before itself: a cycle is possible
+    with pytest.raises(lrt.LrtError) as e:
+        s.AddGeom_SdfMultiOctree(bad, values)
+    assert e.value.code == 1
+    bad = nodes.copy()
+    bad[5]["data_offset"], bad[5]["voxel_count"] = 70, 1   # values past the end
+    with pytest.raises(lrt.LrtError):
+        s.AddGeom_SdfMultiOctree(bad, values)

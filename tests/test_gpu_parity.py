@@ -369,3 +369,32 @@ def test_tlas_batch_scene_bit_exact(lrt, orc, torch_dev):
         scene.set_tail_config(budget, 1024)
         got = gpu_trace(torch_dev, scene, p, d)
         assert raw_equal(got, want), f"budget {budget}: {np.sum(got['primId'] != want['primId'])} id mismatches"
+
+
+@pytest.mark.parametrize("depth,res,lod", [(7, 256, None), (9, 1024, None), (8, 256, 5.0)])
+def test_sdf_multi_octree_parity(lrt, orc, torch_dev, depth, res, lod):
+    """Multi-octree walk (BVH2Common.cpp:2176-2363) over the sphere-union octree: the C2 scene
+    at depth 9 as a true octree walk, and a level-of-detail cut."""
+    _say(f"multi octree depth={depth} res={res} lod={lod}")
+    mn, mv = lrt.sdf_spheres_multi_octree(sphere_set(), depth)
+    s = lrt.SceneRT()
+    s.AddInstance(s.AddGeom_SdfMultiOctree(mn, mv))
+    s.CommitScene()
+    if lod is not None:
+        p = lrt.default_preset()
+        p["fixed_lod"], p["level_of_detail"] = 1, lod
+        s.SetPreset(p)
+    jit = np.random.default_rng(3).random((res * res, 2), dtype=np.float32)
+    pi, vi, pos, dirs = dev_rays(torch_dev, lrt, res, res, jitter=jit)
+    got = gpu_trace(torch_dev, s, pos, dirs)
+    p, d = pos.cpu().numpy(), dirs.cpu().numpy()
+    if res > 512:
+        idx = np.random.default_rng(4).choice(res * res, res * res // 4, replace=False)
+        got, p, d = got[idx], p[idx], d[idx]
+    want, ov = orc.trace_closest(orc.Scene.from_scene(s), p, d)
+    r = sdf_compare(got, want, p, d)
+    _say(str(r))
+    assert r["hits"] > 0.1 * r["n"]
+    assert r["hit_flip"] <= 2e-5 * r["n"] + 2
+    assert r["id_diff_beyond_tie"] <= 2e-5 * r["n"] + 2
+    assert r["t_rel_gt_1e-5"] <= 2e-5 * r["n"] + 2

@@ -342,3 +342,48 @@ def test_quantised_brick_colour_and_texcoords(lrt, orc, camera_batch, layout):
         uv = np.stack([(words >> 16) & 0xFFFF, words & 0xFFFF], 2).astype(np.float64) / 65535.0
         want = np.einsum("hc,hck->hk", w, uv)
         assert np.abs(hits["coords"][hit, :2] - want).max() < 2e-3
+
+
+@pytest.mark.parametrize("depth", [6, 7])
+def test_multi_octree_matches_frame_octree(lrt, orc, camera_batch, depth):
+    """The multi-octree walk (BVH2Common.cpp:2176-2363) over the octree of one sphere union hits
+    the same surface as the frame-octree BVH of the same octree, and that surface is the
+    analytic one."""
+    sp = sphere_set()
+    vox = 2.0 / 2 ** depth
+    mn, mv = lrt.sdf_spheres_multi_octree(sp, depth)
+    fr = lrt.sdf_spheres_frame_octree(sp, depth)
+    assert mn.size == fr.size and mv.size == 8 * mn.size
+    sm = host_scene(lrt, lambda s: s.AddInstance(s.AddGeom_SdfMultiOctree(mn, mv)))
+    sf = host_scene(lrt, lambda s: s.AddInstance(s.AddGeom_SdfFrameOctree(fr)))
+    pos, dir = camera_batch
+    hm = orc.trace_closest(orc.Scene.from_scene(sm), pos, dir)[0]
+    hf = orc.trace_closest(orc.Scene.from_scene(sf), pos, dir)[0]
+    a, b = hm["primId"] != MISS, hf["primId"] != MISS
+    assert a.sum() > 0.15 * a.size and np.array_equal(a, b)
+    dt = np.abs(hm["t"][a] - hf["t"][a]) / vox
+    assert dt.max() < 5e-3 and np.mean(hm["t"][a] == hf["t"][a]) > 0.8
+    p = pos[a, :3].astype(np.float64) + hm["t"][a, None].astype(np.float64) * dir[a, :3]
+    f, _ = union_distance(sp, p)
+    assert np.abs(f).max() <= 0.35 * vox
+
+
+def test_multi_octree_level_of_detail(lrt, orc, camera_batch):
+    """With a level-of-detail cut (fixed_lod = 1, level_of_detail = 5: nodes of 32 cells per axis
+    act as leaves) the walk stops at depth 5 and matches the frame octree built at depth 5."""
+    sp = sphere_set()
+    mn, mv = lrt.sdf_spheres_multi_octree(sp, 7)
+    sm = host_scene(lrt, lambda s: s.AddInstance(s.AddGeom_SdfMultiOctree(mn, mv)))
+    p = lrt.default_preset()
+    p["fixed_lod"], p["level_of_detail"] = 1, 5.0
+    sm.SetPreset(p)
+    f5 = lrt.sdf_spheres_frame_octree(sp, 5)
+    sf = host_scene(lrt, lambda s: s.AddInstance(s.AddGeom_SdfFrameOctree(f5)))
+    pos, dir = camera_batch
+    hm = orc.trace_closest(orc.Scene.from_scene(sm), pos, dir)[0]
+    hf = orc.trace_closest(orc.Scene.from_scene(sf), pos, dir)[0]
+    a, b = hm["primId"] != MISS, hf["primId"] != MISS
+    assert a.sum() > 0.15 * a.size
+    assert np.sum(a != b) <= 0.002 * a.size
+    both = a & b
+    assert np.quantile(np.abs(hm["t"][both] - hf["t"][both]), 0.99) < 1e-3 * (2.0 / 32)
