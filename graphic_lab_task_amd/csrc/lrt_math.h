@@ -49,6 +49,32 @@ LRT_HD float3 safe_inverse(float3 d) {
   return r;
 }
 
+#ifdef __HIPCC__
+// ray_box2 with the hardware min/max (v_min3/v_max3). It differs from ray_box2 only where a
+// product is NaN (inf * 0: a zero direction component, or a NaN input) and in the sign of a zero
+// result, which only ever meets comparisons. The walks use it for rays whose direction
+// components are all finite and non-zero and whose origin is finite (see box_test_fast).
+__device__ __forceinline__ float2 ray_box2_fast(float3 o, float3 inv, float3 bmin, float3 bmax) {
+  const float lo  = inv.x * (bmin.x - o.x);
+  const float hi  = inv.x * (bmax.x - o.x);
+  const float lo1 = inv.y * (bmin.y - o.y);
+  const float hi1 = inv.y * (bmax.y - o.y);
+  const float lo2 = inv.z * (bmin.z - o.z);
+  const float hi2 = inv.z * (bmax.z - o.z);
+  float2 r;
+  r.x = fmaxf(fmaxf(fminf(lo, hi), fminf(lo1, hi1)), fminf(lo2, hi2));
+  r.y = fminf(fminf(fmaxf(lo, hi), fmaxf(lo1, hi1)), fmaxf(lo2, hi2));
+  return r;
+}
+
+// True when ray_box2_fast gives the decisions of ray_box2 for this ray (see above).
+__device__ __forceinline__ bool box_test_fast(float3 o, float3 d) {
+  const float inf = __int_as_float(0x7f800000);
+  return fabsf(d.x) > 0.0f && fabsf(d.y) > 0.0f && fabsf(d.z) > 0.0f && fabsf(d.x) < inf && fabsf(d.y) < inf &&
+         fabsf(d.z) < inf && fabsf(o.x) < inf && fabsf(o.y) < inf && fabsf(o.z) < inf;
+}
+#endif
+
 // raytrace_common.h:69-82; returns (tmin, tmax)
 LRT_HD float2 ray_box2(float3 o, float3 inv, float3 bmin, float3 bmax) {
   const float lo  = inv.x * (bmin.x - o.x);

@@ -42,9 +42,14 @@ __global__ __launch_bounds__(BLOCK) void k_trace_closest(SceneView s, const floa
   RayCounts cnt{};
   bool aborted = false;
   const uint64_t t_start = COUNT ? wall_clock64() : 0;
-  const CRT_Hit h = ray_query_nearest<KINDS, S, BLOCK, COUNT, false>(s, p, d, st, overflow, cnt,
-                                                                     (const lds_pair*)top_pairs,
-                                                                     COUNT ? 0u : budget, aborted, nullptr);
+  CRT_Hit h;
+  if (!COUNT && box_test_fast(xyz(p), xyz(d)))
+    h = ray_query_nearest<KINDS, S, BLOCK, COUNT, false, true>(s, p, d, st, overflow, cnt, (const lds_pair*)top_pairs,
+                                                               budget, aborted, nullptr);
+  else
+    h = ray_query_nearest<KINDS, S, BLOCK, COUNT, false, false>(s, p, d, st, overflow, cnt,
+                                                                (const lds_pair*)top_pairs, COUNT ? 0u : budget,
+                                                                aborted, nullptr);
   if (aborted) {
     queue->list[atomicAdd(&queue->count, 1u)] = i;
     return;

@@ -11,7 +11,7 @@ namespace lrt {
 // fat nodes, nearer child first, far child pushed; leaves dispatched on the geometry tag.
 // `steps` counts fat-node fetches and leaf visits; past `budget` the walk stops and sets
 // `aborted` (budget 0 = unlimited).
-template <uint32_t KINDS, int S, int BLOCK, bool COUNT>
+template <uint32_t KINDS, int S, int BLOCK, bool COUNT, bool FAST = false>
 __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos, float3 ray_dir, float tNear,
                                               uint32_t instId, uint32_t geomId, bool stopOnFirstHit, CRT_Hit& hit,
                                               PendingTri& pend, LaneStack<S, BLOCK>& st, uint32_t& overflow,
@@ -32,8 +32,10 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
                                                       : load_pair(nodes + leftNodeOffset);
       const uint32_t n0 = fat.left.leftOffset;
       const uint32_t n1 = fat.right.leftOffset;
-      const float2 tm0 = ray_box2(ray_pos, rayDirInv, fat.left.boxMin, fat.left.boxMax);
-      const float2 tm1 = ray_box2(ray_pos, rayDirInv, fat.right.boxMin, fat.right.boxMax);
+      const float2 tm0 = FAST ? ray_box2_fast(ray_pos, rayDirInv, fat.left.boxMin, fat.left.boxMax)
+                              : ray_box2(ray_pos, rayDirInv, fat.left.boxMin, fat.left.boxMax);
+      const float2 tm1 = FAST ? ray_box2_fast(ray_pos, rayDirInv, fat.right.boxMin, fat.right.boxMax)
+                              : ray_box2(ray_pos, rayDirInv, fat.right.boxMin, fat.right.boxMax);
       const bool hit0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= hit.t);
       const bool hit1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= hit.t);
       leftNodeOffset = hit0 ? n0 : n1;
@@ -114,8 +116,10 @@ __device__ __forceinline__ void finish_hit(const SceneView& s, const PendingTri&
 
 // BVHRT::RayQuery_NearestHit (MultiRenderer/BVH2Common.cpp:3322-3392): stackless TLAS walk
 // over escape indices, ray moved to instance space (direction not renormalised).
-// COOP = true: the wave holds one ray and walks each BLAS cooperatively.
-template <uint32_t KINDS, int S, int BLOCK, bool COUNT, bool COOP>
+// COOP = true: the wave holds one ray and walks each BLAS cooperatively. FAST = true: box tests
+// with hardware min/max (the caller checked box_test_fast for the world ray; each instance-space
+// ray is checked again before its BLAS walk).
+template <uint32_t KINDS, int S, int BLOCK, bool COUNT, bool COOP, bool FAST = false>
 __device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 posAndNear, float4 dirAndFar,
                                                      LaneStack<S, BLOCK>& st, uint32_t& overflow, RayCounts& cnt,
                                                      const lds_pair* top_pairs, uint32_t budget, bool& aborted,
@@ -136,7 +140,8 @@ __device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 
     do {
       if (COUNT) cnt.tlas_nodes++;
       const BVHNode cur = load_node(s.tlas + nodeIdx);
-      const float2 bh = ray_box2(pos, rayDirInv, cur.boxMin, cur.boxMax);
+      const float2 bh = FAST ? ray_box2_fast(pos, rayDirInv, cur.boxMin, cur.boxMax)
+                             : ray_box2(pos, rayDirInv, cur.boxMin, cur.boxMax);
       const bool intersects = (bh.x <= bh.y) && (bh.y > posAndNear.w) && (bh.x < hit.t);
       travFlags = (cur.leftOffset & LEAF_BIT) | uint32_t(intersects);
       leftOffset = cur.leftOffset;
@@ -168,8 +173,13 @@ __device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 
           __syncthreads();
         }
       } else {
-        blas_traverse<KINDS, S, BLOCK, COUNT>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId, stopOnFirstHit, hit,
-                                              pend, st, overflow, cnt, top_pairs, budget, steps, aborted);
+        if (FAST && box_test_fast(ray_pos, ray_dir))
+          blas_traverse<KINDS, S, BLOCK, COUNT, true>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId, stopOnFirstHit,
+                                                      hit, pend, st, overflow, cnt, top_pairs, budget, steps, aborted);
+        else
+          blas_traverse<KINDS, S, BLOCK, COUNT, false>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId,
+                                                       stopOnFirstHit, hit, pend, st, overflow, cnt, top_pairs, budget,
+                                                       steps, aborted);
         if (aborted) return hit;
       }
     }

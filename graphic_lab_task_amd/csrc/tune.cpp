@@ -57,7 +57,7 @@ hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, 
   }
   if (e == hipSuccess) {
     const TuneConfig shape = r.best;
-    for (uint32_t budget : {48u, 64u, 96u, 128u, 192u}) {
+    for (uint32_t budget : {32u, 48u, 64u, 96u, 128u, 192u}) {
       for (uint32_t waves : {1024u, 2048u}) {
         TuneConfig c = shape;
         c.budget = budget;
