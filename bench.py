@@ -6,6 +6,8 @@ Prints one JSON line (see DESIGN.md for the metric, the roofline model and the C
 
 Workloads (synthetic, seeded):
   sdf_octree_d9_1024  64-sphere union, frame octree depth 9, 1024x1024 jittered primary rays
+  sdf_multi_d9_1024   the same depth-9 octree and rays as sdf_octree_d9_1024, walked as a multi-octree
+                      (front-to-back octree walk instead of a BVH over the leaf cells)
   icosphere7_512      icosphere subdivided 7x (327,680 tris), radial value noise 0.05, 512x512 rays
   tlas_4096x32_4m     two-level BVH: 4,096 instances (random rotation, scale U[0.5,2], translation
                       U[0,100]^3, seed 8) of 32 noise-displaced geodesic spheres with 20 n^2
@@ -151,6 +153,15 @@ def build_workload(name, threads):
                 "surface_leaves": int(scene.get_array("orig_nodes").size), "rays": 1024 * 1024,
                 "width": 1024, "height": 1024}
         return scene, 1024, 1024, jitter, desc, "frame", time.time() - t0
+    if name == "sdf_multi_d9_1024":
+        mn, mv = g.sdf_spheres_multi_octree(spheres(), 9, threads)
+        scene = g.SceneRT()
+        scene.AddInstance(scene.AddGeom_SdfMultiOctree(mn, mv))
+        scene.CommitScene()
+        jitter = np.random.default_rng(3).random((1024 * 1024, 2), dtype=np.float32)
+        desc = {"workload": name, "octree_nodes": int(mn.size), "depth": 9, "spheres": 64, "rays": 1024 * 1024,
+                "width": 1024, "height": 1024}
+        return scene, 1024, 1024, jitter, desc, "multi", time.time() - t0
     if name == "tlas_4096x32_4m":
         scene, meshes, mats = tlas_scene()
         pos, dirs = hemisphere_rays(meshes, mats, 4 * 1024 * 1024)
@@ -326,6 +337,8 @@ def algorithmic_bytes(cnt, hits, kind):
     b += B_TRI * c[:, 2].sum() + B_INSTANCE * c[:, 4].sum()
     if kind == "frame":
         b += B_FRAME_LEAF * c[:, 1].sum()
+    if kind == "multi":   # the walk reports its octree reads in 16-byte units (node records, voxel values)
+        b += 16 * c[:, 5].sum()
     if kind == "tri":
         b += B_REMAP * float((hits["primId"] != g.MISS).sum())
     return b
@@ -398,7 +411,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="sdf_octree_d9_1024",
-                    choices=["sdf_octree_d9_1024", "icosphere7_512", "tlas_4096x32_4m", "sbs_diff_256_8x256",
+                    choices=["sdf_octree_d9_1024", "sdf_multi_d9_1024", "icosphere7_512", "tlas_4096x32_4m",
+                             "sbs_diff_256_8x256",
                              "sbs_batch_64x4x128"])
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--lds-stack", type=int, default=16)

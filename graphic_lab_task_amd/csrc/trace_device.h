@@ -662,7 +662,8 @@ constexpr int OT_STACK = 32;
 
 // Walks the octree in front-to-back child order with mirrored coordinates (octant mask `a`)
 // and stops at the first leaf whose surface is hit. The stack lives in private memory.
-__device__ void multi_octree_intersect(const SceneView& s, uint32_t type, float3 ray_pos, float3 ray_dir, float tNear,
+// Returns the bytes read from the octree in 16-byte units (node records and voxel values).
+__device__ uint32_t multi_octree_intersect(const SceneView& s, uint32_t type, float3 ray_pos, float3 ray_dir, float tNear,
                                        uint32_t instId, uint32_t geomId, CRT_Hit& hit) {
   float3 pos_ray_pos = ray_pos;
   float3 pos_ray_dir = ray_dir;
@@ -680,6 +681,7 @@ __device__ void multi_octree_intersect(const SceneView& s, uint32_t type, float3
 
   OTEntry stack[OT_STACK];
   int top = 0;
+  uint32_t units = 0;
   const float old_t = hit.t;
   const uint32_t root = s.geoms[geomId].offset.x;
   stack[0] = OTEntry{root, 0u, 0u, 1u};
@@ -701,6 +703,7 @@ __device__ void multi_octree_intersect(const SceneView& s, uint32_t type, float3
       const float target_lod_size = (powf(2.0f, s.preset.level_of_detail) - 0.01f) / t;
       if (float(level_sz) > target_lod_size) {
         const SdfMultiOctreeNode n = s.multi_nodes[stack[top].node_id];
+        units++;
         stack[top].info = n.data_offset;
         is_leaf = true;
         if (n.voxel_count == 0) {
@@ -728,6 +731,7 @@ __device__ void multi_octree_intersect(const SceneView& s, uint32_t type, float3
       const float qNear = fmax_ref(0.0f, sz * (tNear - fNearFar.x));
       const float qFar = sz * (fNearFar.y - fNearFar.x);
       const SdfMultiOctreeNode n = s.multi_nodes[node_id];
+      units += 1 + 2 * n.voxel_count;
       const bool surface = (n.flags & MULTI_OCTREE_FLAG_SURFACE) != 0;
       const uint32_t primId = node_id - root;
       for (uint32_t v = 0; v < n.voxel_count; ++v) {
@@ -752,6 +756,7 @@ __device__ void multi_octree_intersect(const SceneView& s, uint32_t type, float3
       do {
         const uint32_t child = uint32_t(curr) ^ a;
         const SdfMultiOctreeNode cn = s.multi_nodes[children + child];
+        units++;
         const bool child_is_node = cn.children_offset > 0;
         if ((cn.voxel_count > 0 || child_is_node) && buf_top < 4) {
           buf[buf_top].node_id = children + child;
@@ -771,6 +776,7 @@ __device__ void multi_octree_intersect(const SceneView& s, uint32_t type, float3
       if (top >= OT_STACK) top = OT_STACK - 1;
     }
   }
+  return units;
 }
 
 }  // namespace lrt

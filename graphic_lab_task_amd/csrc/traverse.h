@@ -77,7 +77,8 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
                                               a_start, hit);
         if (hit.t < tPrev) pend.valid = 0;
       } else if ((KINDS & LK_MULTI) && tag == TAG_SDF_OCTREE) {
-        multi_octree_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId, hit);
+        cnt.sdf_voxels += multi_octree_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId,
+                                                 hit);
         if (hit.t < tPrev) pend.valid = 0;
       }
       if (s.preset.render_mode == MULTI_RENDER_MODE_INTERSECTION_COUNT)

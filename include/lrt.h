@@ -115,7 +115,7 @@ lrt_status lrt_trace_closest_host(lrt_scene* s, const float* pos_near, const flo
 /* Instrumented variant of lrt_trace_closest_dev, replacing RayQuery_NearestHitWithRayInfo
  * (BVH2Common.cpp:3403-3571): also writes 8 uint32 words per ray to `counters` (device, 8*n):
  * fat BLAS nodes fetched, leaves entered, triangles tested, TLAS nodes fetched, instances
- * entered, SBS voxels stepped, then the device wall clock (100 MHz ticks, low 32 bits) when
+ * entered, SBS voxels whose values were loaded (multi-octrees: bytes read / 16), then the device wall clock (100 MHz ticks, low 32 bits) when
  * the ray started and the ray's elapsed ticks. Same hits as the plain call. */
 lrt_status lrt_trace_closest_counted_dev(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,
                                          uint32_t* counters, uint64_t n, void* stream);
