@@ -398,3 +398,29 @@ def test_sdf_multi_octree_parity(lrt, orc, torch_dev, depth, res, lod):
     assert r["hit_flip"] <= 2e-5 * r["n"] + 2
     assert r["id_diff_beyond_tie"] <= 2e-5 * r["n"] + 2
     assert r["t_rel_gt_1e-5"] <= 2e-5 * r["n"] + 2
+
+
+def test_trace_graph_replay(lrt, orc, torch_dev):
+    """A closest-hit call (two-pass: queue reset kernel, pass 1, tail) captured in a HIP graph
+    replays with identical records, with eager work between replays."""
+    torch = torch_dev
+    _say("trace graph replay")
+    v, f, s = mesh_scene(lrt, 6, 0.05)
+    pi, vi, pos, dirs = dev_rays(torch, lrt, 256, 256)
+    s.set_tail_config(16, 256)   # many rays go through the tail pass
+    base = gpu_trace(torch, s, pos, dirs)
+    hits = torch.zeros((pos.shape[0], 8), dtype=torch.int32, device="cuda")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        s.trace_closest(pos, dirs, hits, stream=side.cuda_stream)   # work buffers for this stream
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph, stream=side):
+            s.trace_closest(pos, dirs, hits, stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    for r in range(3):
+        hits.fill_(-1)
+        graph.replay()
+        torch.cuda.synchronize()
+        assert raw_equal(hits.cpu().numpy().view(lrt.HIT_DTYPE).reshape(-1), base), f"replay {r}"
