@@ -190,27 +190,6 @@ __device__ __forceinline__ float3 eval_grad_trilinear(const float v[8], float3 q
   return make3(gx, gy, gz);
 }
 
-// Early-out of the surface solve (ours, not in the reference): along the ray the trilinear is
-// a cubic in t, and on t in [0, qFar] it lies inside the hull of its Bernstein coefficients
-// f(0), f(0) + T/3 f'(0), f(T) - T/3 f'(T), f(T). When all four exceed EPS by a margin of 1%
-// of the largest corner magnitude (far above the rounding of any solver), no intersection
-// mode can report a hit in this cell and the solve is skipped. f0 is the sign-adjusted start
-// distance at t = 0.
-__device__ __forceinline__ bool segment_clear(const float v[8], float3 start_q, float3 dir, float qFar,
-                                              float start_sign, float f0, float EPS) {
-  float vmax = 0.0f;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) vmax = fmax_ref(vmax, fabs_ref(v[i]));
-  const float margin = EPS + 0.01f * vmax;
-  if (!(f0 > margin) || !(qFar >= 0.0f) || !(qFar < 1e4f)) return false;
-  const float3 q1 = start_q + qFar * dir;
-  const float f1 = start_sign * eval_dist_trilinear(v, q1);
-  const float g0 = start_sign * dot3(eval_grad_trilinear(v, start_q), dir);
-  const float g1 = start_sign * dot3(eval_grad_trilinear(v, q1), dir);
-  const float k = qFar * (1.0f / 3.0f);
-  return f1 > margin && (f0 + k * g0) > margin && (f1 - k * g1) > margin;
-}
-
 // BVHRT::LocalSurfaceIntersection (BVH2Common.cpp:345-725) for the non-compact SDF types
 // used here (frame octree, SBS, multi octree). `is_surface_node` is always false for them.
 // Returns true when the hit record was updated; then *t_local (if given) receives the hit
@@ -235,8 +214,6 @@ __device__ bool local_surface_intersection(const SceneView& s, uint32_t type, fl
   if (start_dist <= EPS || mode == SDF_OCTREE_NODE_INTERSECT_BBOX) {
     early_hit = true;
     hitp = true;
-  } else if (qNear == 0.0f && segment_clear(values, start_q, ray_dir, qFar, start_sign, start_dist, EPS)) {
-    hitp = false;
   } else if (mode == SDF_OCTREE_NODE_INTERSECT_ST) {
     const unsigned ST_max_iters = 256;
     float dist = start_dist;

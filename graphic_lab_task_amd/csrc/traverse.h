@@ -24,7 +24,53 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
   uint32_t leftNodeOffset = 0;
   bool hitFound = false;
   const float3 rayDirInv = safe_inverse(ray_dir);
-  while (top >= 0 && !hitFound) {
+  const uint32_t tag = s.geom_tags[geomId];
+  // BVHRT::IntersectAllPrimitivesInLeaf (BVH2Common.cpp:47-124); device-layout leaves carry
+  // PackOffsetAndSize(start, count) themselves. Returns false when the step budget ran out.
+  auto leaf = [&](uint32_t word) -> bool {
+    if (budget && ++steps > budget) return false;
+    const uint32_t a_start = extract_start(word);
+    const uint32_t a_count = extract_count(word);
+    const float tNearSdf = fmax_ref(tNear, 0.01f);
+    const float tPrev = hit.t;
+    if (COUNT) {
+      cnt.leaves++;
+      if (tag == TAG_TRIANGLE) cnt.prims += a_count;
+    }
+    if ((KINDS & LK_TRI) && tag == TAG_TRIANGLE) {
+      intersect_tris(s, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, a_count, hit, pend);
+    } else if ((KINDS & LK_FRAME) && tag == TAG_SDF_NODE) {
+      frame_node_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
+      if (hit.t < tPrev) pend.valid = 0;
+    } else if ((KINDS & LK_SBS) && tag == TAG_SDF_BRICK) {
+      cnt.sdf_voxels += sbs_brick_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId,
+                                            a_start, hit);
+      if (hit.t < tPrev) pend.valid = 0;
+    } else if ((KINDS & LK_MULTI) && tag == TAG_SDF_OCTREE) {
+      cnt.sdf_voxels += multi_octree_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId,
+                                               hit);
+      if (hit.t < tPrev) pend.valid = 0;
+    }
+    if (s.preset.render_mode == MULTI_RENDER_MODE_INTERSECTION_COUNT)
+      hit.instId = (hit.instId == 0xFFFFFFFFu) ? 1 : hit.instId + 1;
+    const uint32_t hitTag = (hit.t >= tPrev) ? TAG_NONE : tag;
+    hitFound = (hitTag != TAG_NONE) && (hitTag != TAG_TRIANGLE || stopOnFirstHit);
+    return true;
+  };
+  // SDF walks end at the first leaf that hits, and hit.t does not change before that. So a lane
+  // that reaches a leaf notes it and keeps walking until every lane of the wave holds a noted
+  // leaf; the noted leaves are then intersected together, each lane's in its walk order: the
+  // same leaves in the same order as the sequential walk, with fewer wave-wide waits
+  // (postponed leaves, Aila & Laine 2009). Triangle walks keep the plain order.
+  const bool postpone = !COUNT && tag != TAG_TRIANGLE;
+  uint32_t postponed = 0;   // the noted leaf word (leaf words always have LEAF_BIT set)
+  auto note_leaf = [&]() {
+    if (leftNodeOffset != 0xFFFFFFFFu) postponed = leftNodeOffset;
+    top--;
+    leftNodeOffset = st.get(top > 0 ? top : 0);
+  };
+  while (!hitFound && (top >= 0 || postponed != 0)) {
+    if (postpone && postponed == 0 && top >= 0 && (leftNodeOffset & LEAF_BIT)) note_leaf();
     while (top >= 0 && ((leftNodeOffset & LEAF_BIT) == 0)) {
       if (budget && ++steps > budget) { aborted = true; return; }
       if (COUNT) cnt.blas_nodes++;
@@ -53,41 +99,26 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
         top--;
         leftNodeOffset = st.get(top > 0 ? top : 0);
       }
-    }
-    if (top >= 0 && leftNodeOffset != 0xFFFFFFFFu) {
-      if (budget && ++steps > budget) { aborted = true; return; }
-      // BVHRT::IntersectAllPrimitivesInLeaf (BVH2Common.cpp:47-124); device-layout leaves carry
-      // PackOffsetAndSize(start, count) themselves
-      const uint32_t a_start = extract_start(leftNodeOffset);
-      const uint32_t a_count = extract_count(leftNodeOffset);
-      const float tNearSdf = fmax_ref(tNear, 0.01f);
-      const float tPrev = hit.t;
-      const uint32_t tag = s.geom_tags[geomId];
-      if (COUNT) {
-        cnt.leaves++;
-        if (tag == TAG_TRIANGLE) cnt.prims += a_count;
+      if (postpone) {
+        if (postponed == 0 && top >= 0 && (leftNodeOffset & LEAF_BIT)) note_leaf();
+        if (!__any(postponed == 0)) break;
       }
-      if ((KINDS & LK_TRI) && tag == TAG_TRIANGLE) {
-        intersect_tris(s, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, a_count, hit, pend);
-      } else if ((KINDS & LK_FRAME) && tag == TAG_SDF_NODE) {
-        frame_node_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
-        if (hit.t < tPrev) pend.valid = 0;
-      } else if ((KINDS & LK_SBS) && tag == TAG_SDF_BRICK) {
-        cnt.sdf_voxels += sbs_brick_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId,
-                                              a_start, hit);
-        if (hit.t < tPrev) pend.valid = 0;
-      } else if ((KINDS & LK_MULTI) && tag == TAG_SDF_OCTREE) {
-        cnt.sdf_voxels += multi_octree_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId,
-                                                 hit);
-        if (hit.t < tPrev) pend.valid = 0;
-      }
-      if (s.preset.render_mode == MULTI_RENDER_MODE_INTERSECTION_COUNT)
-        hit.instId = (hit.instId == 0xFFFFFFFFu) ? 1 : hit.instId + 1;
-      const uint32_t hitTag = (hit.t >= tPrev) ? TAG_NONE : tag;
-      hitFound = (hitTag != TAG_NONE) && (hitTag != TAG_TRIANGLE || stopOnFirstHit);
     }
-    top--;
-    leftNodeOffset = st.get(top > 0 ? top : 0);
+    if (postpone) {
+      if (postponed != 0) {
+        const uint32_t w = postponed;
+        postponed = 0;
+        if (!leaf(w)) { aborted = true; return; }
+      } else if (top >= 0 && (leftNodeOffset & LEAF_BIT)) {
+        if (leftNodeOffset != 0xFFFFFFFFu && !leaf(leftNodeOffset)) { aborted = true; return; }
+        top--;
+        leftNodeOffset = st.get(top > 0 ? top : 0);
+      }
+    } else {
+      if (top >= 0 && leftNodeOffset != 0xFFFFFFFFu && !leaf(leftNodeOffset)) { aborted = true; return; }
+      top--;
+      leftNodeOffset = st.get(top > 0 ? top : 0);
+    }
   }
 }
 
