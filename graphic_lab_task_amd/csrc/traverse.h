@@ -62,7 +62,7 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
   // leaf; the noted leaves are then intersected together, each lane's in its walk order: the
   // same leaves in the same order as the sequential walk, with fewer wave-wide waits
   // (postponed leaves, Aila & Laine 2009). Triangle walks keep the plain order.
-  const bool postpone = !COUNT && tag != TAG_TRIANGLE;
+  const bool postpone = !COUNT && (KINDS & (LK_FRAME | LK_SBS | LK_MULTI)) != 0 && tag != TAG_TRIANGLE;
   uint32_t postponed = 0;   // the noted leaf word (leaf words always have LEAF_BIT set)
   auto note_leaf = [&]() {
     if (leftNodeOffset != 0xFFFFFFFFu) postponed = leftNodeOffset;
