@@ -583,6 +583,10 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
 lrt_status lrt_sbs_batch_run(lrt_sbs_batch* b, void* stream) {
   return guarded([&]() -> lrt_status {
     if (!b) return fail(LRT_ERR_INVALID_ARG, "null batch");
+    int dev = -1;
+    hipError_t ge = hipGetDevice(&dev);
+    if (ge != hipSuccess) return hip_fail(ge, "no device");
+    if (dev != b->device) return fail(LRT_ERR_INVALID_ARG, "batch plan was created on another device");
     const hipError_t e = lrt::launch_sbs_batch(b->L, reinterpret_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "batch launch");
     return LRT_OK;

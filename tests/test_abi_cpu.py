@@ -3,6 +3,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -107,3 +108,10 @@ def test_multi_octree_validation(lrt):
     bad[5]["data_offset"], bad[5]["voxel_count"] = 70, 1   # values past the end
     with pytest.raises(lrt.LrtError):
         s.AddGeom_SdfMultiOctree(bad, values)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message and a non-zero status."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode != 0 and "one GPU" in r.stderr and r.stdout.strip() == ""
