@@ -83,7 +83,7 @@ class LrtError(RuntimeError):
 class _TuneResult(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("lds_stack", ctypes.c_int32), ("xcd_remap", ctypes.c_int32),
                 ("budget", ctypes.c_uint32), ("tail_waves", ctypes.c_uint32), ("best_ms", ctypes.c_float),
-                ("configs_tried", ctypes.c_uint32)]
+                ("configs_tried", ctypes.c_uint32), ("leaf_postpone", ctypes.c_int32)]
 
 
 class _Buffer(ctypes.Structure):
@@ -132,6 +132,7 @@ def lib() -> ctypes.CDLL:
             "lrt_eye_rays_dev": ([vp, vp, u32, u32, vp, u32, vp, vp, vp], i32),
             "lrt_set_launch_config": ([vp, i32, i32, i32], i32),
             "lrt_set_tail_config": ([vp, u32, u32], i32),
+            "lrt_set_leaf_postponement": ([vp, i32], i32),
             "lrt_autotune": ([vp, vp, vp, vp, u64, vp, vp], i32),
             "lrt_sbs_param_layout": ([vp, u32, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u64)], i32),
             "lrt_sbs_forward_dev": ([vp, u32, vp, vp, vp, u64, vp, vp], i32),
@@ -167,7 +168,7 @@ def exported_symbols():
             "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
             "lrt_trace_closest_host", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
-            "lrt_set_launch_config", "lrt_set_tail_config", "lrt_autotune", "lrt_sbs_param_layout",
+            "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_autotune", "lrt_sbs_param_layout",
             "lrt_sbs_forward_dev", "lrt_sbs_backward_dev", "lrt_sbs_l2_backward_dev", "lrt_sbs_recompute_misses",
             "lrt_sbs_batch_create", "lrt_sbs_batch_run", "lrt_sbs_batch_destroy",
             "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
@@ -373,6 +374,11 @@ class SceneRT:
 
     def set_tail_config(self, budget: int = 64, tail_waves: int = 1024):
         _check(lib().lrt_set_tail_config(self._h, budget, tail_waves))
+
+    def set_leaf_postponement(self, mode: int = -1):
+        """Triangle leaf postponement of the one-ray-per-lane pass: -1 automatic (on for one
+        instance), 0 off, 1 on."""
+        _check(lib().lrt_set_leaf_postponement(self._h, mode))
 
     def stack_overflow_count(self, reset: bool = False) -> int:
         v = ctypes.c_uint64()

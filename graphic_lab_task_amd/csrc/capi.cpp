@@ -87,6 +87,8 @@ lrt_status trace_common(lrt_scene* s, const float* pos, const float* dir, lrt_hi
   // INTERSECTION_COUNT debug mode accumulates in instId; that mode runs single-pass.
   a.budget = s->s.preset.render_mode == lrt::MULTI_RENDER_MODE_INTERSECTION_COUNT ? 0u : s->s.budget;
   a.tail_waves = s->s.tail_waves;
+  // extra leaf visits would change the counts of that mode too
+  a.postpone_tri = s->s.preset.render_mode != lrt::MULTI_RENDER_MODE_INTERSECTION_COUNT && s->s.postpone_triangles();
   e = lrt::launch_trace(s->s.make_view(), a, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "trace launch");
   return LRT_OK;
@@ -402,6 +404,15 @@ lrt_status lrt_set_tail_config(lrt_scene* s, uint32_t budget, uint32_t tail_wave
   });
 }
 
+lrt_status lrt_set_leaf_postponement(lrt_scene* s, int mode) {
+  return guarded([&]() -> lrt_status {
+    if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+    if (mode < -1 || mode > 1) return fail(LRT_ERR_INVALID_ARG, "mode must be -1, 0 or 1");
+    s->s.leaf_postpone = mode;
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits, uint64_t n,
                         void* stream, lrt_tune_result* out) {
   return guarded([&]() -> lrt_status {
@@ -409,13 +420,17 @@ lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_fa
     if (!s->s.committed) return fail(LRT_ERR_NOT_COMMITTED, "scene is not committed");
     if (!pos_near || !dir_far || !hits || n == 0) return fail(LRT_ERR_INVALID_ARG, "autotune needs a non-empty ray batch");
     lrt::Scene& sc = s->s;
-    const lrt::TuneConfig start{sc.block, sc.lds_stack, sc.xcd_remap, sc.budget, sc.tail_waves};
+    const lrt::TuneConfig start{sc.block, sc.lds_stack, sc.xcd_remap, sc.budget, sc.tail_waves,
+                                sc.postpone_triangles() ? 1 : 0};
+    // the option only exists for triangle-only scenes
+    const bool tune_postpone = sc.leaf_kinds() == lrt::LK_TRI;
     auto apply = [&](const lrt::TuneConfig& c) {
       sc.block = c.block;
       sc.lds_stack = c.lds_stack;
       sc.xcd_remap = c.xcd_remap;
       sc.budget = c.budget;
       sc.tail_waves = c.tail_waves;
+      if (tune_postpone) sc.leaf_postpone = c.postpone;
     };
     lrt_status st = LRT_OK;
     auto launch = [&](const lrt::TuneConfig& c) -> hipError_t {
@@ -424,9 +439,11 @@ lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_fa
       return st == LRT_OK ? hipSuccess : hipErrorLaunchFailure;
     };
     lrt::TuneResult r{};
-    const hipError_t e = lrt::autotune(launch, reinterpret_cast<hipStream_t>(stream), start, 3, &r);
+    const int start_mode = sc.leaf_postpone;
+    const hipError_t e = lrt::autotune(launch, reinterpret_cast<hipStream_t>(stream), start, tune_postpone, 3, &r);
     if (e != hipSuccess) {
       apply(start);
+      sc.leaf_postpone = start_mode;
       return st != LRT_OK ? st : hip_fail(e, "autotune");
     }
     apply(r.best);
@@ -438,6 +455,7 @@ lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_fa
       out->tail_waves = r.best.tail_waves;
       out->best_ms = r.best_ms;
       out->configs_tried = r.tried;
+      out->leaf_postpone = r.best.postpone;
     }
     return LRT_OK;
   });

@@ -27,6 +27,7 @@ struct TraceArgs {
   uint32_t      budget;     // pass-1 steps per ray before it is queued for pass 2 (0: single pass)
   TailQueue*    queue;      // device queue (nullable when budget is 0)
   uint32_t      tail_waves; // persistent waves of pass 2 (0: default)
+  uint32_t      postpone_tri; // 1: postpone triangle leaves (triangle-only scenes; pass 1)
 };
 
 struct EyeRayParams {

@@ -394,6 +394,13 @@ uint32_t Scene::leaf_kinds() const {
   return k;
 }
 
+// Postponed triangle leaves help coherent rays into one mesh (fewer wave-wide waits) and hurt
+// incoherent rays crossing many instances (more boxes pass the stale hit.t), hence the default.
+bool Scene::postpone_triangles() const {
+  if (leaf_postpone >= 0) return leaf_postpone != 0;
+  return instances.size() == 1 && leaf_kinds() == LK_TRI;
+}
+
 // BVHRT::CommitScene (BVH2Common_host.cpp:1230-1262), then device upload.
 void Scene::build_host() {
   if (instances.empty()) throw std::invalid_argument("commit with no instances");

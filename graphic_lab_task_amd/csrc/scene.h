@@ -80,6 +80,10 @@ struct Scene {
   std::map<void*, DevBuf> spill_bufs;
   std::map<void*, size_t> spill_rays;
   uint32_t budget = 64, tail_waves = 1024;
+  // Triangle leaf postponement in pass 1: -1 automatic (on for a triangle-only scene with one
+  // instance), 0 off, 1 on.
+  int leaf_postpone = -1;
+  bool postpone_triangles() const;
   // Per-stream work memory for n rays: tail queue (header + list) and deep-stack spill.
   hipError_t work_for(void* stream, size_t n, int lds_stack, uint32_t** spill, TailQueue** queue);
   // per-stream occupancy-mask buffer for calls whose values differ from the committed ones

@@ -11,6 +11,9 @@ enum LeafKind : uint32_t {
   LK_FRAME = 1u << 1,   // TAG_SDF_NODE over a frame octree
   LK_SBS   = 1u << 2,   // TAG_SDF_BRICK
   LK_MULTI = 1u << 3,   // TAG_SDF_OCTREE (multi-octree walk)
+  // Walk option, not a leaf kind (never in SceneView::leaf_kinds): triangle leaves are
+  // postponed like SDF leaves (traverse.h).
+  LK_TRI_POSTPONE = 1u << 4,
 };
 
 // Frame-octree leaf as the kernels read it: the leaf box, its 8 corner distances and the

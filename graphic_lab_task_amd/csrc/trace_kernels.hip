@@ -149,9 +149,13 @@ static hipError_t launch_trace_t(const SceneView& s, const TraceArgs& a, hipStre
   const uint32_t grid = (a.n + BLOCK - 1) / BLOCK;
   if (grid == 0) return hipSuccess;
   if (a.counts) {
-    hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, true>), dim3(grid), dim3(BLOCK), 0, stream, s, a.pos, a.dir,
-                       a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts, a.spill, 0u, a.queue);
-    return hipGetLastError();
+    if constexpr ((KINDS & LK_TRI_POSTPONE) != 0) {
+      return hipErrorInvalidValue;   // counted runs take the plain walk (launch_trace routes them)
+    } else {
+      hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, true>), dim3(grid), dim3(BLOCK), 0, stream, s, a.pos,
+                         a.dir, a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts, a.spill, 0u, a.queue);
+      return hipGetLastError();
+    }
   }
   const bool tail = a.budget != 0 && a.queue != nullptr;
   if (tail) hipLaunchKernelGGL(k_tail_reset, dim3(1), dim3(64), 0, stream, a.queue);
@@ -160,8 +164,9 @@ static hipError_t launch_trace_t(const SceneView& s, const TraceArgs& a, hipStre
                      a.queue);
   if (tail) {
     const uint32_t waves = a.tail_waves ? a.tail_waves : 1024u;
-    hipLaunchKernelGGL((k_trace_tail<KINDS>), dim3(waves), dim3(64), 0, stream, s, a.pos, a.dir, a.out, a.any_out,
-                       a.overflow_count, a.queue);
+    // pass 2 walks cooperatively; the postponement option only shapes pass 1
+    hipLaunchKernelGGL((k_trace_tail<KINDS & ~LK_TRI_POSTPONE>), dim3(waves), dim3(64), 0, stream, s, a.pos, a.dir,
+                       a.out, a.any_out, a.overflow_count, a.queue);
   }
   return hipGetLastError();
 }
@@ -186,7 +191,9 @@ static hipError_t launch_kinds(const SceneView& s, const TraceArgs& a, hipStream
 
 hipError_t launch_trace(const SceneView& s, const TraceArgs& a, hipStream_t stream) {
   const uint32_t k = s.leaf_kinds;
-  if (k == LK_TRI) return launch_kinds<LK_TRI>(s, a, stream);
+  if (k == LK_TRI)
+    return a.postpone_tri && !a.counts ? launch_kinds<LK_TRI | LK_TRI_POSTPONE>(s, a, stream)
+                                       : launch_kinds<LK_TRI>(s, a, stream);
   if (k == LK_FRAME) return launch_kinds<LK_FRAME>(s, a, stream);
   if (k == LK_SBS) return launch_kinds<LK_SBS>(s, a, stream);
   if (k == LK_MULTI) return launch_kinds<LK_MULTI>(s, a, stream);

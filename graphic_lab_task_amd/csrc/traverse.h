@@ -61,8 +61,14 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
   // that reaches a leaf notes it and keeps walking until every lane of the wave holds a noted
   // leaf; the noted leaves are then intersected together, each lane's in its walk order: the
   // same leaves in the same order as the sequential walk, with fewer wave-wide waits
-  // (postponed leaves, Aila & Laine 2009). Triangle walks keep the plain order.
-  const bool postpone = !COUNT && (KINDS & (LK_FRAME | LK_SBS | LK_MULTI)) != 0 && tag != TAG_TRIANGLE;
+  // (postponed leaves, Aila & Laine 2009). Triangle walks keep the plain order unless the
+  // kernel is built with LK_TRI_POSTPONE: a closest-hit triangle walk then culls the boxes met
+  // while a leaf waits against the older, larger hit.t, so it may visit extra leaves but never
+  // skips one; leaves keep their walk order, and an extra leaf's triangles lie at or beyond its
+  // box entry, past the final hit, so the result only differs where a triangle's t rounds below
+  // its own box entry by a few ulp and ties the final hit that closely.
+  const bool postpone = !COUNT && (((KINDS & LK_TRI_POSTPONE) != 0) ||
+                                   ((KINDS & (LK_FRAME | LK_SBS | LK_MULTI)) != 0 && tag != TAG_TRIANGLE));
   uint32_t postponed = 0;   // the noted leaf word (leaf words always have LEAF_BIT set)
   auto note_leaf = [&]() {
     if (leftNodeOffset != 0xFFFFFFFFu) postponed = leftNodeOffset;

@@ -27,7 +27,7 @@ hipError_t time_one(const std::function<hipError_t(const TuneConfig&)>& launch, 
 }  // namespace
 
 hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, hipStream_t stream,
-                    const TuneConfig& start, int reps, TuneResult* out) {
+                    const TuneConfig& start, bool tune_postpone, int reps, TuneResult* out) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -43,25 +43,30 @@ hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, 
     }
     return hipSuccess;
   };
-  if (e == hipSuccess) {
+  const int n_postpone = tune_postpone ? 2 : 1;
+  for (int pp = 0; pp < n_postpone && e == hipSuccess; ++pp) {
     for (int block : {64, 128, 256}) {
       for (int lds : {8, 16, 32}) {
         TuneConfig c = start;
         c.block = block;
         c.lds_stack = lds;
         c.budget = 0;
+        if (tune_postpone) c.postpone = pp;
         if ((e = consider(c)) != hipSuccess) break;
       }
       if (e != hipSuccess) break;
     }
   }
-  if (e == hipSuccess) {
-    const TuneConfig shape = r.best;
+  // the two-pass scheme changes which rays the postponed walk holds up, so stage 2 tries both
+  // leaf orders again
+  const TuneConfig shape = r.best;
+  for (int pp = 0; pp < n_postpone && e == hipSuccess; ++pp) {
     for (uint32_t budget : {32u, 48u, 64u, 96u, 128u, 192u}) {
       for (uint32_t waves : {1024u, 2048u}) {
         TuneConfig c = shape;
         c.budget = budget;
         c.tail_waves = waves;
+        if (tune_postpone) c.postpone = pp;
         if ((e = consider(c)) != hipSuccess) break;
       }
       if (e != hipSuccess) break;

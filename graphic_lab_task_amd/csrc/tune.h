@@ -11,6 +11,7 @@ namespace lrt {
 struct TuneConfig {
   int block, lds_stack, xcd_remap;
   uint32_t budget, tail_waves;
+  int postpone;   // triangle leaf postponement (0/1)
 };
 
 struct TuneResult {
@@ -21,8 +22,9 @@ struct TuneResult {
 
 // Two stages, each candidate timed with HIP events on `stream` (one warm-up launch, then the
 // minimum of `reps`): (1) block size x LDS stack entries with the single-pass walk, (2) step
-// budget x tail waves of the two-pass scheme on the best shape from (1).
+// budget x tail waves of the two-pass scheme on the best shape from (1); both stages also try
+// triangle leaf postponement off and on when `tune_postpone`.
 hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, hipStream_t stream,
-                    const TuneConfig& start, int reps, TuneResult* out);
+                    const TuneConfig& start, bool tune_postpone, int reps, TuneResult* out);
 
 }  // namespace lrt

@@ -148,6 +148,16 @@ lrt_status lrt_set_launch_config(lrt_scene* s, int block, int lds_stack, int xcd
  * No counterpart in the reference (its CPU loop has no wave tail). */
 lrt_status lrt_set_tail_config(lrt_scene* s, uint32_t budget, uint32_t tail_waves);
 
+/* Leaf order of the one-ray-per-lane pass on triangle-only scenes: 1 postpones triangle
+ * leaves until every lane of the wave has reached one (fewer wave-wide waits for coherent
+ * rays; boxes met meanwhile are culled against the older hit distance, so extra leaves may be
+ * visited), 0 intersects each leaf when reached, as BVH2Common.cpp:3239-3320 does, -1
+ * (default) chooses 1 for a scene with one instance. Hits are the sequential walk's except
+ * where a triangle's distance rounds below its own box entry and ties the final hit within a
+ * few ulp. SDF leaves are always postponed (same leaves in the same order). The
+ * INTERSECTION_COUNT render mode ignores the setting (its counts are per leaf visit). */
+lrt_status lrt_set_leaf_postponement(lrt_scene* s, int mode);
+
 /* ---- Differentiable sparse brick set (no counterpart in the reference) ----
  * Geometry: an indexed fp32 brick set (layout ID32F_IRGB32F) whose values_f holds n_dist
  * distance values followed by n_dist RGB albedos, one per lattice point (4*n_dist floats);
@@ -205,12 +215,15 @@ typedef struct lrt_tune_result {
   uint32_t budget, tail_waves;
   float best_ms;           /* fastest time of one closest-hit call on the tuning batch */
   uint32_t configs_tried;
+  int32_t leaf_postpone;   /* triangle leaf postponement chosen (0/1; tuned on triangle-only scenes) */
 } lrt_tune_result;
 
-/* Chooses the launch configuration (block size, LDS stack entries, step budget, tail waves) by
- * timing closest-hit calls on the caller's device ray batch (`hits` is overwritten with the
- * batch's hit records), and keeps the fastest on the scene. Every configuration returns
- * identical records. No counterpart in the reference (its CPU loop has nothing to tune). */
+/* Chooses the launch configuration (block size, LDS stack entries, step budget, tail waves,
+ * and on triangle-only scenes the leaf postponement) by timing closest-hit calls on the
+ * caller's device ray batch (`hits` is overwritten with the batch's hit records), and keeps
+ * the fastest on the scene. Every configuration returns identical records, up to the ulp-level
+ * ties noted at lrt_set_leaf_postponement. No counterpart in the reference (its CPU loop has
+ * nothing to tune). */
 lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits, uint64_t n,
                         void* stream, lrt_tune_result* out);
 

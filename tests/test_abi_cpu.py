@@ -59,6 +59,9 @@ def test_errors_are_reported_not_thrown(lrt):
     with pytest.raises(lrt.LrtError):
         s.CommitScene()                                             # no instances
     with pytest.raises(lrt.LrtError):
+        s.set_leaf_postponement(2)                                  # modes are -1, 0, 1
+    s.set_leaf_postponement(0)
+    with pytest.raises(lrt.LrtError):
         s.set_launch_config(100, 16, True)
 
 

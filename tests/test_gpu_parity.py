@@ -371,6 +371,41 @@ def test_tlas_batch_scene_bit_exact(lrt, orc, torch_dev):
         assert raw_equal(got, want), f"budget {budget}: {np.sum(got['primId'] != want['primId'])} id mismatches"
 
 
+def test_triangle_leaf_postponement(lrt, orc, torch_dev):
+    """Both leaf orders of the triangle walk against the oracle's sequential walk: records equal,
+    except rows that tie the oracle's hit within 1e-5*t (the parity bar's tie exemption), on a
+    single mesh (closest and stop-on-first-hit), an instanced scene and a sample of the
+    4,096-instance scene, single pass and two-pass."""
+    import bench
+    v, f, mesh = mesh_scene(lrt, 6, 0.05)
+    pi, vi, pos, dirs = dev_rays(torch_dev, lrt, 256, 256)
+    d_first = dirs.clone()
+    d_first[:, 3] *= -1
+    inst, ipos, idirs = _instanced(lrt)
+    tl, meshes, mats = bench.tlas_scene()
+    tpos, tdirs = bench.hemisphere_rays(meshes, mats, 1 << 16, seed=23)
+    cases = [("mesh", mesh, pos, dirs), ("mesh first-hit", mesh, pos, d_first),
+             ("instanced", inst, torch_dev.from_numpy(ipos).cuda(), torch_dev.from_numpy(idirs).cuda()),
+             ("tlas", tl, torch_dev.from_numpy(tpos).cuda(), torch_dev.from_numpy(tdirs).cuda())]
+    for name, s, p, d in cases:
+        want, ov = orc.trace_closest(orc.Scene.from_scene(s), p.cpu().numpy(), d.cpu().numpy())
+        for mode in (0, 1):
+            for budget in (0, 64):
+                _say(f"postponement {name} mode={mode} budget={budget}")
+                s.set_leaf_postponement(mode)
+                s.set_tail_config(budget, 1024)
+                got = gpu_trace(torch_dev, s, p, d)
+                diff = np.ascontiguousarray(got).view(np.uint8).reshape(got.size, -1) != \
+                    np.ascontiguousarray(want).view(np.uint8).reshape(want.size, -1)
+                bad = np.flatnonzero(diff.any(axis=1))
+                tie = (got["primId"][bad] != MISS) & (want["primId"][bad] != MISS) & \
+                    (np.abs(got["t"][bad] - want["t"][bad]) <= 1e-5 * np.abs(want["t"][bad]))
+                assert tie.all(), f"{name} mode={mode} budget={budget}: {int((~tie).sum())} rows beyond a tie"
+                assert bad.size <= 2e-5 * got.size + 2, f"{name} mode={mode}: {bad.size} tied rows"
+        s.set_leaf_postponement(-1)
+        s.set_tail_config(64, 1024)
+
+
 @pytest.mark.parametrize("depth,res,lod", [(7, 256, None), (9, 1024, None), (8, 256, 5.0)])
 def test_sdf_multi_octree_parity(lrt, orc, torch_dev, depth, res, lod):
     """Multi-octree walk (BVH2Common.cpp:2176-2363) over the sphere-union octree: the C2 scene
