@@ -39,7 +39,7 @@ __device__ bool recompute_hit_voxel(const SceneView& s, uint32_t diff_geom, floa
   const float3 ray_pos = matmul4x3(m, xyz(p));
   const float3 ray_dir = matmul3x3(m, xyz(d));
   CRT_Hit hh = miss_hit(fabs_ref(d.w));
-  sbs_brick_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, fmax_ref(p.w, 0.01f), h.instId, geomId, h.primId, hh,
+  sbs_brick_intersect(s, geom_ref(s, geomId), ray_pos, ray_dir, fmax_ref(p.w, 0.01f), h.instId, geomId, h.primId, hh,
                       &rec);
   ray_dir_local = ray_dir;
   return __float_as_uint(hh.t) == __float_as_uint(h.t);

@@ -93,32 +93,30 @@ struct CoopCand {
   float t;
   uint64_t key;
   CRT_Hit hit;
-  PendingTri pend;
 };
 
 // Processes one leaf for this lane against the incoming hit h_in; keeps the lane's best
 // candidate by (t, key) for triangles or by key for SDF/stop-on-first-hit.
 template <uint32_t KINDS>
 __device__ __forceinline__ void coop_leaf(const SceneView& s, float3 ray_pos, float3 ray_dir, float tNear,
-                                          uint32_t instId, uint32_t geomId, uint32_t tag, uint32_t leafWord,
+                                          uint32_t instId, uint32_t geomId, const GeomRef& g, uint32_t tag,
+                                          uint32_t leafWord,
                                           uint64_t key, bool by_key, const CRT_Hit& h_in, CoopCand& c,
                                           uint32_t& leaves, uint32_t& prims) {
   const float tNearSdf = fmax_ref(tNear, 0.01f);
   const uint32_t a_start = extract_start(leafWord);
   const uint32_t a_count = extract_count(leafWord);
   CRT_Hit h = h_in;
-  PendingTri pend;
-  pend.valid = 0;
   leaves++;
   if ((KINDS & LK_TRI) && tag == TAG_TRIANGLE) {
     prims += a_count;
-    intersect_tris(s, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, a_count, h, pend);
+    intersect_tris(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, a_count, h);
   } else if ((KINDS & LK_FRAME) && tag == TAG_SDF_NODE) {
-    frame_node_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, h);
+    frame_node_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, h);
   } else if ((KINDS & LK_SBS) && tag == TAG_SDF_BRICK) {
-    sbs_brick_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, h);
+    sbs_brick_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, h);
   } else if ((KINDS & LK_MULTI) && tag == TAG_SDF_OCTREE) {
-    multi_octree_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId, h);
+    multi_octree_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, h);
   }
   if (s.preset.render_mode == MULTI_RENDER_MODE_INTERSECTION_COUNT)
     h.instId = (h.instId == 0xFFFFFFFFu) ? 1 : h.instId + 1;
@@ -129,19 +127,18 @@ __device__ __forceinline__ void coop_leaf(const SceneView& s, float3 ray_pos, fl
     c.t = h.t;
     c.key = key;
     c.hit = h;
-    c.pend = pend;
   }
 }
 
 // Cooperative BVH2TraverseF32 for one BLAS. Every lane holds the same ray; on return every
-// lane holds the same hit and pending-normal state. Returns false if the frontier outgrew
+// lane holds the same hit. Returns false if the frontier outgrew
 // COOP_CAP or the tree is deeper than 63 levels (the caller then walks sequentially).
 template <uint32_t KINDS>
 __device__ bool coop_blas(const SceneView& s, lds_entry* st, float3 ray_pos, float3 ray_dir, float tNear,
-                          uint32_t instId, uint32_t geomId, bool stopOnFirstHit, CRT_Hit& hit, PendingTri& pend,
-                          RayCounts& cnt) {
+                          uint32_t instId, uint32_t geomId, bool stopOnFirstHit, CRT_Hit& hit, RayCounts& cnt) {
   const uint32_t lane = __lane_id();
   const uint32_t bvhOffset = s.geoms[geomId].bvhOffset;
+  const GeomRef g = geom_ref(s, geomId);
   const uint32_t tag = s.geom_tags[geomId];
   const bool first_hit = stopOnFirstHit || tag != TAG_TRIANGLE;   // first_hit_is_closest / stop-on-first
   const float3 inv = safe_inverse(ray_dir);
@@ -150,7 +147,6 @@ __device__ bool coop_blas(const SceneView& s, lds_entry* st, float3 ray_pos, flo
   float best_t = h_in.t;
   uint64_t best_key = ~0ull;
   CRT_Hit best_hit = h_in;
-  PendingTri best_pend = pend;
   bool ok = true;
   uint32_t leaves = 0, prims = 0, nodes = 0;
 
@@ -208,8 +204,8 @@ __device__ bool coop_blas(const SceneView& s, lds_entry* st, float3 ray_pos, flo
     c.valid = false;
     c.t = 0.0f;
     c.key = ~0ull;
-    if (leaf0) coop_leaf<KINDS>(s, ray_pos, ray_dir, tNear, instId, geomId, tag, c0, k0, first_hit, h_in, c, leaves, prims);
-    if (leaf1) coop_leaf<KINDS>(s, ray_pos, ray_dir, tNear, instId, geomId, tag, c1, k1, first_hit, h_in, c, leaves, prims);
+    if (leaf0) coop_leaf<KINDS>(s, ray_pos, ray_dir, tNear, instId, geomId, g, tag, c0, k0, first_hit, h_in, c, leaves, prims);
+    if (leaf1) coop_leaf<KINDS>(s, ray_pos, ray_dir, tNear, instId, geomId, g, tag, c1, k1, first_hit, h_in, c, leaves, prims);
     if (__ballot(c.valid)) {
       int winner;
       if (first_hit) {
@@ -226,10 +222,6 @@ __device__ bool coop_blas(const SceneView& s, lds_entry* st, float3 ray_pos, flo
       const bool improves = !have || (first_hit ? wkey < best_key : (wt < best_t || (wt == best_t && wkey < best_key)));
       if (improves) {
         best_hit = shfl_hit(c.hit, winner);
-        best_pend.valid = __shfl(c.pend.valid, winner, 64);
-        best_pend.geomId = __shfl(c.pend.geomId, winner, 64);
-        best_pend.instId = __shfl(c.pend.instId, winner, 64);
-        best_pend.triId = __shfl(c.pend.triId, winner, 64);
         best_t = wt;
         best_key = wkey;
         have = true;
@@ -245,11 +237,7 @@ __device__ bool coop_blas(const SceneView& s, lds_entry* st, float3 ray_pos, flo
   cnt.leaves += leaves;
   cnt.prims += prims;
   if (!ok) return false;
-  if (have) {
-    hit = best_hit;
-    if (tag == TAG_TRIANGLE) pend = best_pend;
-    else pend.valid = 0;
-  }
+  if (have) hit = best_hit;
   return true;
 }
 

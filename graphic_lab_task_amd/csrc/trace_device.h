@@ -40,11 +40,17 @@ struct RayCounts {
 // clock (100 MHz ticks) at ray start (low 32 bits) and the ray's elapsed ticks.
 constexpr int RAY_COUNTERS = 8;
 
-// Triangle hit whose normal is resolved once the closest hit is final (the normal depends
-// only on the winning triangle and its barycentrics, BVH2Common.cpp:1609-1626).
-struct PendingTri {
-  uint32_t valid, geomId, instId, triId;
+// The fields of a geometry record that the leaf intersectors read, loaded once per BLAS walk
+// rather than once per leaf.
+struct GeomRef {
+  uint32_t type;
+  uint2    offset;
 };
+
+__device__ __forceinline__ GeomRef geom_ref(const SceneView& s, uint32_t geomId) {
+  const GeomData* g = s.geoms + geomId;
+  return GeomRef{g->type, g->offset};
+}
 
 __device__ __forceinline__ BVHNodePair load_pair(const BVHNodePair* p) {
   const uint4* q = reinterpret_cast<const uint4*>(p);
@@ -104,10 +110,10 @@ __device__ __forceinline__ void load_mat(const float4x4& m, float* out) {
 // ---------------------------------------------------------------------------------------
 // Triangles: BVHRT::IntersectAllTrianglesInLeaf (BVH2Common.cpp:1571-1635), reading the
 // leaf-ordered vertex soup: triangle triId of geometry geomId is record off.x/3 + triId.
-__device__ __forceinline__ void intersect_tris(const SceneView& s, float3 ray_pos, float3 ray_dir, float tNear,
-                                               uint32_t instId, uint32_t geomId, uint32_t a_start, uint32_t a_count,
-                                               CRT_Hit& hit, PendingTri& pend) {
-  const uint32_t tri0 = s.geoms[geomId].offset.x / 3;
+__device__ __forceinline__ void intersect_tris(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir,
+                                               float tNear, uint32_t instId, uint32_t geomId, uint32_t a_start,
+                                               uint32_t a_count, CRT_Hit& hit) {
+  const uint32_t tri0 = g.offset.x / 3;
   for (uint32_t triId = a_start; triId < a_start + a_count; triId++) {
     const float4* q = s.tri_soup + size_t(tri0 + triId) * 3;
     const float4 a = q[0], b = q[1], c = q[2];
@@ -132,16 +138,18 @@ __device__ __forceinline__ void intersect_tris(const SceneView& s, float3 ray_po
       hit.coords[1] = v;
       hit.coords[2] = 0.0f;
       hit.coords[3] = 0.0f;
-      pend.valid = 1; pend.geomId = geomId; pend.instId = instId; pend.triId = triId;
     }
   }
 }
 
-// Normal of the final triangle hit (BVH2Common.cpp:1609-1631).
-__device__ __forceinline__ void resolve_tri_normal(const SceneView& s, const PendingTri& pend, CRT_Hit& hit) {
-  if (!pend.valid || !preset_needs_normal(s.preset.render_mode)) return;
-  const uint2 off = s.geoms[pend.geomId].offset;
-  const float4* q = s.tri_soup + size_t(off.x / 3 + pend.triId) * 3;
+// Normal of the final triangle hit (BVH2Common.cpp:1609-1631), computed once the closest hit
+// is known: it depends only on the winning triangle and its barycentrics, which the hit record
+// names (geomId, instId and primId before the m_primIndices remap; the render mode that
+// rewrites instId, INTERSECTION_COUNT, takes no normal).
+__device__ __forceinline__ void resolve_tri_normal(const SceneView& s, CRT_Hit& hit) {
+  if (!preset_needs_normal(s.preset.render_mode)) return;
+  const uint2 off = s.geoms[hit.geomId & GEOM_ID_MASK].offset;
+  const float4* q = s.tri_soup + size_t(off.x / 3 + hit.primId) * 3;
   const float4 qa = q[0], qb = q[1], qc = q[2];
   const uint32_t A = __float_as_uint(qa.w), B = __float_as_uint(qb.w), C = __float_as_uint(qc.w);
   const float u = hit.coords[0], v = hit.coords[1];
@@ -157,7 +165,7 @@ __device__ __forceinline__ void resolve_tri_normal(const SceneView& s, const Pen
     n = make3(nA.x * wA + nB.x * v + u * nC.x, nA.y * wA + nB.y * v + u * nC.y, nA.z * wA + nB.z * v + u * nC.z);
   }
   float m[16];
-  load_mat(s.instances[pend.instId].transformInvTransposed, m);
+  load_mat(s.instances[hit.instId].transformInvTransposed, m);
   n = normalize3(matmul4x3(m, n));
   const float2 e = encode_normal(n);
   hit.coords[2] = e.x;
@@ -393,11 +401,12 @@ __device__ bool local_surface_intersection(const SceneView& s, uint32_t type, fl
 
 // Frame-octree leaf: BVHRT::OctreeNodeIntersect + RayNodeIntersection (BVH2Common.cpp:727-749,
 // 141-167), reading the leaf's box, corner values and node index from one FrameLeaf record.
-__device__ __forceinline__ void frame_node_intersect(const SceneView& s, uint32_t type, float3 ray_pos, float3 ray_dir,
+__device__ __forceinline__ void frame_node_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir,
                                                      float tNear, uint32_t instId, uint32_t geomId,
                                                      uint32_t bvhNodeId, CRT_Hit& hit) {
+  const uint32_t type = g.type;
   if (type != TYPE_SDF_FRAME_OCTREE) return;
-  const uint2 go = s.geoms[geomId].offset;   // y: first leaf box of this octree
+  const uint2 go = g.offset;   // y: first leaf box of this octree
   const float4* q = reinterpret_cast<const float4*>(s.frame_leaves + go.y + bvhNodeId);
   const float4 r0 = q[0], r1 = q[1], r2 = q[2], r3 = q[3];
   const float3 min_pos = make3(r0.x, r0.y, r0.z);
@@ -460,11 +469,12 @@ struct SbsVoxelRec {
 // Sparse-brick-set leaf: BVHRT::OctreeBrickIntersect (BVH2Common.cpp:884-1112), trilinear.
 // Returns the number of voxels whose values were loaded (all voxels stepped without an
 // occupancy mask). `rec` (indexed layouts only) records the hit voxel.
-__device__ uint32_t sbs_brick_intersect(const SceneView& s, uint32_t type, float3 ray_pos, float3 ray_dir, float tNear,
+__device__ uint32_t sbs_brick_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir, float tNear,
                                         uint32_t instId, uint32_t geomId, uint32_t bvhNodeId, CRT_Hit& hit,
                                         SbsVoxelRec* rec = nullptr) {
   uint32_t steps = 0, loaded = 0;
-  const uint32_t sdfId = s.geoms[geomId].offset.x;
+  const uint32_t type = g.type;
+  const uint32_t sdfId = g.offset.x;
   const uint32_t primId = bvhNodeId;
   const uint32_t nodeId = primId + s.sbs_roots[sdfId];
   const SdfSBSHeader header = s.sbs_headers[sdfId];
@@ -640,7 +650,7 @@ constexpr int OT_STACK = 32;
 // Walks the octree in front-to-back child order with mirrored coordinates (octant mask `a`)
 // and stops at the first leaf whose surface is hit. The stack lives in private memory.
 // Returns the bytes read from the octree in 16-byte units (node records and voxel values).
-__device__ uint32_t multi_octree_intersect(const SceneView& s, uint32_t type, float3 ray_pos, float3 ray_dir, float tNear,
+__device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir, float tNear,
                                        uint32_t instId, uint32_t geomId, CRT_Hit& hit) {
   float3 pos_ray_pos = ray_pos;
   float3 pos_ray_dir = ray_dir;
@@ -660,7 +670,8 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, uint32_t type, fl
   int top = 0;
   uint32_t units = 0;
   const float old_t = hit.t;
-  const uint32_t root = s.geoms[geomId].offset.x;
+  const uint32_t type = g.type;
+  const uint32_t root = g.offset.x;
   stack[0] = OTEntry{root, 0u, 0u, 1u};
   while (top >= 0) {
     const uint32_t level_sz = stack[top].p_zs & 0xFFFF;

@@ -21,7 +21,10 @@ __global__ __launch_bounds__(BLOCK) void k_trace_closest(SceneView s, const floa
                                                          uint32_t* __restrict__ counts, uint32_t* __restrict__ spill,
                                                          uint32_t budget, TailQueue* queue) {
   __shared__ uint32_t lds[S * BLOCK];
-  __shared__ BVHNodePair top_pairs[LDS_TOP_PAIRS];
+  __shared__ uint32_t keep[6 * BLOCK];   // world ray of each lane while its BLAS is walked
+  // top BFS levels of the cached geometry's BLAS: dynamic, s.cache_pairs entries (none for a
+  // scene without a cached geometry, which then keeps its LDS for more resident waves)
+  extern __shared__ BVHNodePair top_pairs[];
   // stage the top BFS levels of the cached geometry's BLAS (every thread reaches the barrier)
   if (s.cache_pairs) {
     const uint4* src = reinterpret_cast<const uint4*>(s.nodes + s.geoms[s.cache_geom].bvhOffset);
@@ -45,11 +48,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace_closest(SceneView s, const floa
   CRT_Hit h;
   if (!COUNT && box_test_fast(xyz(p), xyz(d)))
     h = ray_query_nearest<KINDS, S, BLOCK, COUNT, false, true>(s, p, d, st, overflow, cnt, (const lds_pair*)top_pairs,
-                                                               budget, aborted, nullptr);
+                                                               budget, aborted, nullptr,
+                                                               (lds_u32*)(keep) + threadIdx.x);
   else
     h = ray_query_nearest<KINDS, S, BLOCK, COUNT, false, false>(s, p, d, st, overflow, cnt,
                                                                 (const lds_pair*)top_pairs, COUNT ? 0u : budget,
-                                                                aborted, nullptr);
+                                                                aborted, nullptr, (lds_u32*)(keep) + threadIdx.x);
   if (aborted) {
     queue->list[atomicAdd(&queue->count, 1u)] = i;
     return;
@@ -148,20 +152,22 @@ template <uint32_t KINDS, int S, int BLOCK>
 static hipError_t launch_trace_t(const SceneView& s, const TraceArgs& a, hipStream_t stream) {
   const uint32_t grid = (a.n + BLOCK - 1) / BLOCK;
   if (grid == 0) return hipSuccess;
+  const size_t lds_cache = size_t(s.cache_pairs) * sizeof(BVHNodePair);
   if (a.counts) {
     if constexpr ((KINDS & LK_TRI_POSTPONE) != 0) {
       return hipErrorInvalidValue;   // counted runs take the plain walk (launch_trace routes them)
     } else {
-      hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, true>), dim3(grid), dim3(BLOCK), 0, stream, s, a.pos,
-                         a.dir, a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts, a.spill, 0u, a.queue);
+      hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, true>), dim3(grid), dim3(BLOCK), lds_cache, stream, s,
+                         a.pos, a.dir, a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts, a.spill, 0u,
+                         a.queue);
       return hipGetLastError();
     }
   }
   const bool tail = a.budget != 0 && a.queue != nullptr;
   if (tail) hipLaunchKernelGGL(k_tail_reset, dim3(1), dim3(64), 0, stream, a.queue);
-  hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, false>), dim3(grid), dim3(BLOCK), 0, stream, s, a.pos, a.dir,
-                     a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts, a.spill, tail ? a.budget : 0u,
-                     a.queue);
+  hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, false>), dim3(grid), dim3(BLOCK), lds_cache, stream, s,
+                     a.pos, a.dir, a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts, a.spill,
+                     tail ? a.budget : 0u, a.queue);
   if (tail) {
     const uint32_t waves = a.tail_waves ? a.tail_waves : 1024u;
     // pass 2 walks cooperatively; the postponement option only shapes pass 1

@@ -14,10 +14,11 @@ namespace lrt {
 template <uint32_t KINDS, int S, int BLOCK, bool COUNT, bool FAST = false>
 __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos, float3 ray_dir, float tNear,
                                               uint32_t instId, uint32_t geomId, bool stopOnFirstHit, CRT_Hit& hit,
-                                              PendingTri& pend, LaneStack<S, BLOCK>& st, uint32_t& overflow,
+                                              LaneStack<S, BLOCK>& st, uint32_t& overflow,
                                               RayCounts& cnt, const lds_pair* top_pairs, uint32_t budget,
                                               uint32_t& steps, bool& aborted) {
   const uint32_t bvhOffset = s.geoms[geomId].bvhOffset;
+  const GeomRef g = geom_ref(s, geomId);
   const BVHNodePair* nodes = s.nodes + bvhOffset;
   const uint32_t cached = (top_pairs != nullptr && geomId == s.cache_geom) ? s.cache_pairs : 0u;
   int top = 0;
@@ -38,18 +39,13 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
       if (tag == TAG_TRIANGLE) cnt.prims += a_count;
     }
     if ((KINDS & LK_TRI) && tag == TAG_TRIANGLE) {
-      intersect_tris(s, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, a_count, hit, pend);
+      intersect_tris(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, a_count, hit);
     } else if ((KINDS & LK_FRAME) && tag == TAG_SDF_NODE) {
-      frame_node_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
-      if (hit.t < tPrev) pend.valid = 0;
+      frame_node_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
     } else if ((KINDS & LK_SBS) && tag == TAG_SDF_BRICK) {
-      cnt.sdf_voxels += sbs_brick_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId,
-                                            a_start, hit);
-      if (hit.t < tPrev) pend.valid = 0;
+      cnt.sdf_voxels += sbs_brick_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
     } else if ((KINDS & LK_MULTI) && tag == TAG_SDF_OCTREE) {
-      cnt.sdf_voxels += multi_octree_intersect(s, s.geoms[geomId].type, ray_pos, ray_dir, tNearSdf, instId, geomId,
-                                               hit);
-      if (hit.t < tPrev) pend.valid = 0;
+      cnt.sdf_voxels += multi_octree_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, hit);
     }
     if (s.preset.render_mode == MULTI_RENDER_MODE_INTERSECTION_COUNT)
       hit.instId = (hit.instId == 0xFFFFFFFFu) ? 1 : hit.instId + 1;
@@ -143,9 +139,9 @@ __device__ __forceinline__ CRT_Hit miss_hit(float far) {
 
 // Normal of a final triangle hit and the m_primIndices remap (BVH2Common.cpp:3382-3388).
 template <uint32_t KINDS>
-__device__ __forceinline__ void finish_hit(const SceneView& s, const PendingTri& pend, CRT_Hit& hit) {
-  if ((KINDS & LK_TRI) && pend.valid && hit.geomId != 0xFFFFFFFFu && (hit.geomId >> SH_TYPE) == TYPE_MESH_TRIANGLE)
-    resolve_tri_normal(s, pend, hit);
+__device__ __forceinline__ void finish_hit(const SceneView& s, CRT_Hit& hit) {
+  if ((KINDS & LK_TRI) && hit.geomId != 0xFFFFFFFFu && (hit.geomId >> SH_TYPE) == TYPE_MESH_TRIANGLE)
+    resolve_tri_normal(s, hit);
   if (hit.geomId < 0xFFFFFFFFu && ((hit.geomId >> SH_TYPE) == TYPE_MESH_TRIANGLE)) {
     const uint2 go = s.geoms[hit.geomId & GEOM_ID_MASK].offset;
     hit.primId = s.prim_indices[go.x / 3 + hit.primId];
@@ -156,23 +152,38 @@ __device__ __forceinline__ void finish_hit(const SceneView& s, const PendingTri&
 // over escape indices, ray moved to instance space (direction not renormalised).
 // COOP = true: the wave holds one ray and walks each BLAS cooperatively. FAST = true: box tests
 // with hardware min/max (the caller checked box_test_fast for the world ray; each instance-space
-// ray is checked again before its BLAS walk).
+// ray is checked again before its BLAS walk). `keep` (LDS, 6 words per lane at keep[k * BLOCK],
+// may be null): the world ray is parked there and read back after each BLAS walk, so that its
+// registers are free while the BLAS is walked (the inverse direction is recomputed, bit for bit).
 template <uint32_t KINDS, int S, int BLOCK, bool COUNT, bool COOP, bool FAST = false>
 __device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 posAndNear, float4 dirAndFar,
                                                      LaneStack<S, BLOCK>& st, uint32_t& overflow, RayCounts& cnt,
                                                      const lds_pair* top_pairs, uint32_t budget, bool& aborted,
-                                                     lds_entry* coop_stack) {
+                                                     lds_entry* coop_stack, lds_u32* keep = nullptr) {
   bool stopOnFirstHit = (dirAndFar.w <= 0.0f);
   if (stopOnFirstHit) dirAndFar.w *= -1.0f;
-  const float3 pos = xyz(posAndNear);
-  const float3 dir = xyz(dirAndFar);
-  const float3 rayDirInv = safe_inverse(dir);
+  if (keep) {
+    keep[0 * BLOCK] = __float_as_uint(posAndNear.x);
+    keep[1 * BLOCK] = __float_as_uint(posAndNear.y);
+    keep[2 * BLOCK] = __float_as_uint(posAndNear.z);
+    keep[3 * BLOCK] = __float_as_uint(dirAndFar.x);
+    keep[4 * BLOCK] = __float_as_uint(dirAndFar.y);
+    keep[5 * BLOCK] = __float_as_uint(dirAndFar.z);
+  }
   CRT_Hit hit = miss_hit(dirAndFar.w);
-  PendingTri pend;
-  pend.valid = 0;
   uint32_t steps = 0;
   uint32_t nodeIdx = 0;
   do {
+    // compiler-only fence (emits nothing): the world ray is re-read below instead of being held
+    // in registers across the BLAS walk
+    if (keep) asm volatile("" ::: "memory");
+    const float3 pos = keep ? make3(__uint_as_float(keep[0 * BLOCK]), __uint_as_float(keep[1 * BLOCK]),
+                                    __uint_as_float(keep[2 * BLOCK]))
+                            : xyz(posAndNear);
+    const float3 dir = keep ? make3(__uint_as_float(keep[3 * BLOCK]), __uint_as_float(keep[4 * BLOCK]),
+                                    __uint_as_float(keep[5 * BLOCK]))
+                            : xyz(dirAndFar);
+    const float3 rayDirInv = safe_inverse(dir);
     uint32_t travFlags = 0;
     uint32_t leftOffset = 0;
     do {
@@ -197,7 +208,7 @@ __device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 
       const float3 ray_dir = matmul3x3(m, dir);
       if (COOP) {
         if (!coop_blas<KINDS>(s, coop_stack, ray_pos, ray_dir, posAndNear.w, instId, geomId, stopOnFirstHit, hit,
-                              pend, cnt)) {
+                              cnt)) {
           // frontier too wide or tree deeper than the key: every lane walks the same ray
           // sequentially on one shared LDS stack
           LaneStack<STACK_SIZE, 1> seq;
@@ -206,23 +217,23 @@ __device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 
           uint32_t seq_steps = 0;
           bool seq_abort = false;
           blas_traverse<KINDS, STACK_SIZE, 1, false>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId,
-                                                     stopOnFirstHit, hit, pend, seq, overflow, cnt, nullptr, 0u,
+                                                     stopOnFirstHit, hit, seq, overflow, cnt, nullptr, 0u,
                                                      seq_steps, seq_abort);
           __syncthreads();
         }
       } else {
         if (FAST && box_test_fast(ray_pos, ray_dir))
           blas_traverse<KINDS, S, BLOCK, COUNT, true>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId, stopOnFirstHit,
-                                                      hit, pend, st, overflow, cnt, top_pairs, budget, steps, aborted);
+                                                      hit, st, overflow, cnt, top_pairs, budget, steps, aborted);
         else
           blas_traverse<KINDS, S, BLOCK, COUNT, false>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId,
-                                                       stopOnFirstHit, hit, pend, st, overflow, cnt, top_pairs, budget,
+                                                       stopOnFirstHit, hit, st, overflow, cnt, top_pairs, budget,
                                                        steps, aborted);
         if (aborted) return hit;
       }
     }
   } while (nodeIdx < 0xFFFFFFFEu && !(stopOnFirstHit && hit.primId != 0xFFFFFFFFu));
-  finish_hit<KINDS>(s, pend, hit);
+  finish_hit<KINDS>(s, hit);
   return hit;
 }
 
