@@ -1,5 +1,7 @@
 #include "tune.h"
 
+#include <algorithm>
+#include <utility>
 #include <vector>
 
 namespace lrt {
@@ -32,11 +34,13 @@ hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, 
   hipError_t e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   TuneResult r{start, 1e30f, 0};
-  auto consider = [&](const TuneConfig& c) -> hipError_t {
+  std::vector<std::pair<float, TuneConfig>> shapes;   // stage-1 results
+  auto consider = [&](const TuneConfig& c, bool stage1) -> hipError_t {
     float ms = 0.0f;
     hipError_t err = time_one(launch, stream, c, reps, e0, e1, &ms);
     if (err != hipSuccess) return err;
     r.tried++;
+    if (stage1) shapes.emplace_back(ms, c);
     if (ms < r.best_ms) {
       r.best_ms = ms;
       r.best = c;
@@ -52,25 +56,37 @@ hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, 
         c.lds_stack = lds;
         c.budget = 0;
         if (tune_postpone) c.postpone = pp;
-        if ((e = consider(c)) != hipSuccess) break;
+        if ((e = consider(c, true)) != hipSuccess) break;
       }
       if (e != hipSuccess) break;
     }
   }
-  // the two-pass scheme changes which rays the postponed walk holds up, so stage 2 tries both
-  // leaf orders again
-  const TuneConfig shape = r.best;
-  for (int pp = 0; pp < n_postpone && e == hipSuccess; ++pp) {
-    for (uint32_t budget : {32u, 48u, 64u, 96u, 128u, 192u}) {
-      for (uint32_t waves : {1024u, 2048u}) {
-        TuneConfig c = shape;
-        c.budget = budget;
-        c.tail_waves = waves;
-        if (tune_postpone) c.postpone = pp;
-        if ((e = consider(c)) != hipSuccess) break;
+  // Stage 2 on the three fastest stage-1 shapes: the single-pass ranking of shapes is only a
+  // guide once the slowest rays move to the second pass. Both leaf orders are tried again, as
+  // the two-pass scheme changes which rays a postponed walk holds up.
+  using Timed = std::pair<float, TuneConfig>;
+  std::sort(shapes.begin(), shapes.end(), [](const Timed& a, const Timed& b) { return a.first < b.first; });
+  std::vector<TuneConfig> tops;
+  for (const auto& sh : shapes) {
+    bool seen = false;
+    for (const TuneConfig& t : tops) seen = seen || (t.block == sh.second.block && t.lds_stack == sh.second.lds_stack);
+    if (!seen) tops.push_back(sh.second);
+    if (tops.size() == 3) break;
+  }
+  for (const TuneConfig& shape : tops) {
+    for (int pp = 0; pp < n_postpone && e == hipSuccess; ++pp) {
+      for (uint32_t budget : {32u, 48u, 64u, 96u, 128u, 192u}) {
+        for (uint32_t waves : {1024u, 2048u}) {
+          TuneConfig c = shape;
+          c.budget = budget;
+          c.tail_waves = waves;
+          if (tune_postpone) c.postpone = pp;
+          if ((e = consider(c, false)) != hipSuccess) break;
+        }
+        if (e != hipSuccess) break;
       }
-      if (e != hipSuccess) break;
     }
+    if (e != hipSuccess) break;
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
