@@ -421,6 +421,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the given launch configuration")
     ap.add_argument("--budget", type=int, default=64, help="tail step budget when not autotuning")
     ap.add_argument("--tail-waves", type=int, default=1024, help="tail waves when not autotuning")
+    ap.add_argument("--ray-interleave", type=int, default=8,
+                    help="pass-1 lanes per interleaved group when not autotuning (0: thread order)")
     ap.add_argument("--leaf-postpone", type=int, default=-1, choices=[-1, 0, 1],
                     help="triangle leaf postponement when not autotuning (-1: on for one instance)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -446,6 +448,7 @@ def main():
     scene.set_launch_config(args.block, args.lds_stack, bool(args.xcd_remap))
     scene.set_tail_config(args.budget, args.tail_waves)
     scene.set_leaf_postponement(args.leaf_postpone)
+    scene.set_ray_interleave(args.ray_interleave)
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
     if isinstance(w, tuple):   # explicit ray buffers
@@ -497,10 +500,12 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, pos.cpu().numpy(), dirs.cpu().numpy(), threads)
 
-    launch = ({k: tune[k] for k in ("block", "lds_stack", "xcd_remap", "budget", "tail_waves", "leaf_postpone")}
+    launch = ({k: tune[k] for k in ("block", "lds_stack", "xcd_remap", "ray_interleave", "budget", "tail_waves",
+                                    "leaf_postpone")}
               if tune else
-              {"block": args.block, "lds_stack": args.lds_stack, "xcd_remap": args.xcd_remap, "budget": args.budget,
-               "tail_waves": args.tail_waves, "leaf_postpone": args.leaf_postpone})
+              {"block": args.block, "lds_stack": args.lds_stack, "xcd_remap": args.xcd_remap,
+               "ray_interleave": args.ray_interleave, "budget": args.budget, "tail_waves": args.tail_waves,
+               "leaf_postpone": args.leaf_postpone})
     desc.update({"hit_fraction": round(float((hits_np["primId"] != g.MISS).mean()), 4),
                  "scene_build_s": round(build_s, 2), "launch": launch, "autotuned": tune is not None,
                  "stack_overflow_rays": overflow,

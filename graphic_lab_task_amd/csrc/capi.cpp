@@ -87,6 +87,7 @@ lrt_status trace_common(lrt_scene* s, const float* pos, const float* dir, lrt_hi
   // INTERSECTION_COUNT debug mode accumulates in instId; that mode runs single-pass.
   a.budget = s->s.preset.render_mode == lrt::MULTI_RENDER_MODE_INTERSECTION_COUNT ? 0u : s->s.budget;
   a.tail_waves = s->s.tail_waves;
+  a.ray_groups = s->s.ray_groups;
   // extra leaf visits would change the counts of that mode too
   a.postpone_tri = s->s.preset.render_mode != lrt::MULTI_RENDER_MODE_INTERSECTION_COUNT && s->s.postpone_triangles();
   e = lrt::launch_trace(s->s.make_view(), a, reinterpret_cast<hipStream_t>(stream));
@@ -404,6 +405,16 @@ lrt_status lrt_set_tail_config(lrt_scene* s, uint32_t budget, uint32_t tail_wave
   });
 }
 
+lrt_status lrt_set_ray_interleave(lrt_scene* s, uint32_t lanes_per_group) {
+  return guarded([&]() -> lrt_status {
+    if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+    if (lanes_per_group > 64 || (lanes_per_group & (lanes_per_group - 1)) != 0)
+      return fail(LRT_ERR_INVALID_ARG, "lanes_per_group must be 0 or a power of two up to 64");
+    s->s.ray_groups = lanes_per_group;
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_set_leaf_postponement(lrt_scene* s, int mode) {
   return guarded([&]() -> lrt_status {
     if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
@@ -421,7 +432,7 @@ lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_fa
     if (!pos_near || !dir_far || !hits || n == 0) return fail(LRT_ERR_INVALID_ARG, "autotune needs a non-empty ray batch");
     lrt::Scene& sc = s->s;
     const lrt::TuneConfig start{sc.block, sc.lds_stack, sc.xcd_remap, sc.budget, sc.tail_waves,
-                                sc.postpone_triangles() ? 1 : 0};
+                                sc.postpone_triangles() ? 1 : 0, sc.ray_groups};
     // the option only exists for triangle-only scenes
     const bool tune_postpone = sc.leaf_kinds() == lrt::LK_TRI;
     auto apply = [&](const lrt::TuneConfig& c) {
@@ -430,6 +441,7 @@ lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_fa
       sc.xcd_remap = c.xcd_remap;
       sc.budget = c.budget;
       sc.tail_waves = c.tail_waves;
+      sc.ray_groups = c.ray_groups;
       if (tune_postpone) sc.leaf_postpone = c.postpone;
     };
     lrt_status st = LRT_OK;
@@ -456,6 +468,7 @@ lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_fa
       out->best_ms = r.best_ms;
       out->configs_tried = r.tried;
       out->leaf_postpone = r.best.postpone;
+      out->ray_interleave = r.best.ray_groups;
     }
     return LRT_OK;
   });

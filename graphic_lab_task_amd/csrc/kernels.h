@@ -28,6 +28,8 @@ struct TraceArgs {
   TailQueue*    queue;      // device queue (nullable when budget is 0)
   uint32_t      tail_waves; // persistent waves of pass 2 (0: default)
   uint32_t      postpone_tri; // 1: postpone triangle leaves (triangle-only scenes; pass 1)
+  uint32_t      ray_groups;   // pass 1: lanes per group dealt to distant parts of the batch
+                              // (power of two; 0 or 64: rays in thread order)
 };
 
 struct EyeRayParams {

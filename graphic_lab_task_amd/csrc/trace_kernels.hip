@@ -12,6 +12,21 @@
 
 namespace lrt {
 
+// Ray of thread i when the lanes of each wave are dealt out in groups of 2^group_log2: group g
+// of wave w takes the same lanes of 64-ray chunk (w + g * n64 / groups) mod n64, so a wave holds
+// rays from 64 / 2^group_log2 distant parts of the batch. Neighbouring primary rays graze the
+// same silhouettes and a wave steps at the pace of its slowest lane; spreading them shortens the
+// slowest waves. A bijection on [0, 64 * n64); the last n mod 64 rays keep their thread.
+__device__ __forceinline__ uint32_t interleave_ray(uint32_t i, uint32_t n, uint32_t group_log2) {
+  const uint32_t n64 = n >> 6;
+  const uint32_t groups = 64u >> group_log2;
+  if (n64 < groups || i >= n64 * 64u) return i;
+  const uint32_t w = i >> 6, l = i & 63u, g = l >> group_log2;
+  uint32_t c = w + g * (n64 / groups);   // < 2 * n64
+  if (c >= n64) c -= n64;
+  return c * 64u + l;
+}
+
 // Pass 1. `budget` 0: unlimited (no queue).
 template <uint32_t KINDS, int S, int BLOCK, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_trace_closest(SceneView s, const float4* __restrict__ pos,
@@ -19,7 +34,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_closest(SceneView s, const floa
                                                          uint32_t* __restrict__ any_out, uint32_t n,
                                                          uint32_t* overflow_count, uint32_t xcd_remap,
                                                          uint32_t* __restrict__ counts, uint32_t* __restrict__ spill,
-                                                         uint32_t budget, TailQueue* queue) {
+                                                         uint32_t budget, TailQueue* queue, uint32_t group_log2) {
   __shared__ uint32_t lds[S * BLOCK];
   __shared__ uint32_t keep[6 * BLOCK];   // world ray of each lane while its BLAS is walked
   // top BFS levels of the cached geometry's BLAS: dynamic, s.cache_pairs entries (none for a
@@ -33,8 +48,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace_closest(SceneView s, const floa
   }
   __syncthreads();
   const uint32_t b = xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-  const uint32_t i = b * BLOCK + threadIdx.x;
+  uint32_t i = b * BLOCK + threadIdx.x;
   if (i >= n) return;
+  if (!COUNT && group_log2 < 6) i = interleave_ray(i, n, group_log2);
   LaneStack<S, BLOCK> st;
   st.col = (lds_u32*)(lds) + threadIdx.x;
   st.spill = spill + size_t(i) * (STACK_SIZE - S);
@@ -148,6 +164,13 @@ __global__ void k_eye_rays(EyeRayParams prm, const float2* __restrict__ jitter, 
 }
 
 // ---------------------------------------------------------------------------------------
+// log2 of the lanes per group of interleave_ray; 6 (whole waves) means no interleaving
+static uint32_t lanes_log2(uint32_t lanes) {
+  uint32_t l = 0;
+  while (l < 6 && (1u << l) < lanes) ++l;
+  return lanes == 0 ? 6u : l;
+}
+
 template <uint32_t KINDS, int S, int BLOCK>
 static hipError_t launch_trace_t(const SceneView& s, const TraceArgs& a, hipStream_t stream) {
   const uint32_t grid = (a.n + BLOCK - 1) / BLOCK;
@@ -159,7 +182,7 @@ static hipError_t launch_trace_t(const SceneView& s, const TraceArgs& a, hipStre
     } else {
       hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, true>), dim3(grid), dim3(BLOCK), lds_cache, stream, s,
                          a.pos, a.dir, a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts, a.spill, 0u,
-                         a.queue);
+                         a.queue, 6u);
       return hipGetLastError();
     }
   }
@@ -167,7 +190,7 @@ static hipError_t launch_trace_t(const SceneView& s, const TraceArgs& a, hipStre
   if (tail) hipLaunchKernelGGL(k_tail_reset, dim3(1), dim3(64), 0, stream, a.queue);
   hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, false>), dim3(grid), dim3(BLOCK), lds_cache, stream, s,
                      a.pos, a.dir, a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts, a.spill,
-                     tail ? a.budget : 0u, a.queue);
+                     tail ? a.budget : 0u, a.queue, lanes_log2(a.ray_groups));
   if (tail) {
     const uint32_t waves = a.tail_waves ? a.tail_waves : 1024u;
     // pass 2 walks cooperatively; the postponement option only shapes pass 1

@@ -49,14 +49,18 @@ hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, 
   };
   const int n_postpone = tune_postpone ? 2 : 1;
   for (int pp = 0; pp < n_postpone && e == hipSuccess; ++pp) {
-    for (int block : {64, 128, 256}) {
-      for (int lds : {8, 16, 32}) {
-        TuneConfig c = start;
-        c.block = block;
-        c.lds_stack = lds;
-        c.budget = 0;
-        if (tune_postpone) c.postpone = pp;
-        if ((e = consider(c, true)) != hipSuccess) break;
+    for (uint32_t groups : {0u, 8u}) {
+      for (int block : {64, 128, 256}) {
+        for (int lds : {8, 16, 32}) {
+          TuneConfig c = start;
+          c.block = block;
+          c.lds_stack = lds;
+          c.budget = 0;
+          c.ray_groups = groups;
+          if (tune_postpone) c.postpone = pp;
+          if ((e = consider(c, true)) != hipSuccess) break;
+        }
+        if (e != hipSuccess) break;
       }
       if (e != hipSuccess) break;
     }
@@ -69,7 +73,9 @@ hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, 
   std::vector<TuneConfig> tops;
   for (const auto& sh : shapes) {
     bool seen = false;
-    for (const TuneConfig& t : tops) seen = seen || (t.block == sh.second.block && t.lds_stack == sh.second.lds_stack);
+    for (const TuneConfig& t : tops)
+      seen = seen || (t.block == sh.second.block && t.lds_stack == sh.second.lds_stack &&
+                      t.ray_groups == sh.second.ray_groups);
     if (!seen) tops.push_back(sh.second);
     if (tops.size() == 3) break;
   }

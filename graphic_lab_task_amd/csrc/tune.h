@@ -11,7 +11,8 @@ namespace lrt {
 struct TuneConfig {
   int block, lds_stack, xcd_remap;
   uint32_t budget, tail_waves;
-  int postpone;   // triangle leaf postponement (0/1)
+  int postpone;          // triangle leaf postponement (0/1)
+  uint32_t ray_groups;   // pass-1 ray interleaving: lanes per group (0: off)
 };
 
 struct TuneResult {
@@ -21,9 +22,10 @@ struct TuneResult {
 };
 
 // Two stages, each candidate timed with HIP events on `stream` (one warm-up launch, then the
-// minimum of `reps`): (1) block size x LDS stack entries with the single-pass walk, (2) step
-// budget x tail waves of the two-pass scheme on the three best shapes from (1); both stages
-// also try triangle leaf postponement off and on when `tune_postpone`.
+// minimum of `reps`): (1) block size x LDS stack entries x ray interleaving (off, groups of 8
+// lanes) with the single-pass walk, (2) step budget x tail waves of the two-pass scheme on the
+// three best shapes from (1); both stages also try triangle leaf postponement off and on when
+// `tune_postpone`.
 hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, hipStream_t stream,
                     const TuneConfig& start, bool tune_postpone, int reps, TuneResult* out);
 

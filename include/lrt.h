@@ -158,6 +158,14 @@ lrt_status lrt_set_tail_config(lrt_scene* s, uint32_t budget, uint32_t tail_wave
  * INTERSECTION_COUNT render mode ignores the setting (its counts are per leaf visit). */
 lrt_status lrt_set_leaf_postponement(lrt_scene* s, int mode);
 
+/* Ray-to-lane assignment of the one-ray-per-lane pass: the lanes of each wave are dealt out in
+ * groups of `lanes_per_group` (a power of two), each group taking rays from a different, distant
+ * part of the batch (a fixed permutation; results are unchanged, each hit is stored at its
+ * ray's index). Neighbouring primary rays graze the same silhouettes and a wave steps at the pace
+ * of its slowest lane, so spreading them shortens the slowest waves. 0 or 64 = rays in thread
+ * order. Default 8. No counterpart in the reference. */
+lrt_status lrt_set_ray_interleave(lrt_scene* s, uint32_t lanes_per_group);
+
 /* ---- Differentiable sparse brick set (no counterpart in the reference) ----
  * Geometry: an indexed fp32 brick set (layout ID32F_IRGB32F) whose values_f holds n_dist
  * distance values followed by n_dist RGB albedos, one per lattice point (4*n_dist floats);
@@ -216,10 +224,11 @@ typedef struct lrt_tune_result {
   float best_ms;           /* fastest time of one closest-hit call on the tuning batch */
   uint32_t configs_tried;
   int32_t leaf_postpone;   /* triangle leaf postponement chosen (0/1; tuned on triangle-only scenes) */
+  uint32_t ray_interleave; /* lanes per group chosen (lrt_set_ray_interleave; 0 = off) */
 } lrt_tune_result;
 
-/* Chooses the launch configuration (block size, LDS stack entries, step budget, tail waves,
- * and on triangle-only scenes the leaf postponement) by timing closest-hit calls on the
+/* Chooses the launch configuration (block size, LDS stack entries, ray interleaving, step
+ * budget, tail waves, and on triangle-only scenes the leaf postponement) by timing closest-hit calls on the
  * caller's device ray batch (`hits` is overwritten with the batch's hit records), and keeps
  * the fastest on the scene. Every configuration returns identical records, up to the ulp-level
  * ties noted at lrt_set_leaf_postponement. No counterpart in the reference (its CPU loop has

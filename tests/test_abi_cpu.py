@@ -61,6 +61,10 @@ def test_errors_are_reported_not_thrown(lrt):
     with pytest.raises(lrt.LrtError):
         s.set_leaf_postponement(2)                                  # modes are -1, 0, 1
     s.set_leaf_postponement(0)
+    for bad in (3, 65, 128):
+        with pytest.raises(lrt.LrtError):
+            s.set_ray_interleave(bad)                               # powers of two up to 64
+    s.set_ray_interleave(0)
     with pytest.raises(lrt.LrtError):
         s.set_launch_config(100, 16, True)
 

@@ -93,6 +93,33 @@ def test_launch_configs_identical(lrt, torch_dev, block, lds, xcd):
     assert raw_equal(gpu_trace(torch_dev, s, pos, dirs), base)
 
 
+def test_ray_interleave_identical(lrt, orc, torch_dev):
+    """Interleaved ray-to-lane assignment only permutes which lane walks which ray: records equal
+    the thread-order ones for every group size, for ragged batch sizes (not a multiple of 64,
+    fewer rays than groups), on a mesh (two-pass) and an SDF octree (single pass)."""
+    v, f, mesh = mesh_scene(lrt, 5, 0.05)
+    nodes = lrt.sdf_spheres_frame_octree(sphere_set(), 7)
+    sdf = lrt.SceneRT()
+    sdf.AddInstance(sdf.AddGeom_SdfFrameOctree(nodes))
+    sdf.CommitScene()
+    pi, vi, pos, dirs = dev_rays(torch_dev, lrt, 256, 256)
+    for name, s, budget in (("mesh", mesh, 16), ("sdf", sdf, 0)):
+        s.set_tail_config(budget, 1024)
+        for n in (pos.shape[0], pos.shape[0] - 37, 64 * 40 + 5, 300):
+            p, d = pos[:n].contiguous(), dirs[:n].contiguous()
+            s.set_ray_interleave(0)
+            base = gpu_trace(torch_dev, s, p, d)
+            if n == pos.shape[0]:
+                want, _ = orc.trace_closest(orc.Scene.from_scene(s), p.cpu().numpy(), d.cpu().numpy())
+                if name == "mesh":
+                    assert raw_equal(base, want)
+            for lanes in (1, 2, 8, 32, 64):
+                _say(f"ray interleave {name} n={n} lanes={lanes}")
+                s.set_ray_interleave(lanes)
+                assert raw_equal(gpu_trace(torch_dev, s, p, d), base), f"{name} n={n} lanes={lanes}"
+        s.set_ray_interleave(8)
+
+
 def test_counters_match_oracle(lrt, orc, torch_dev):
     _say("counters")
     v, f, s = mesh_scene(lrt, 5, 0.05)
