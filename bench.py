@@ -18,10 +18,12 @@ Workloads (synthetic, seeded):
                       of 128x128 each; target = each scene rendered with inverted albedo; a step =
                       one batched run (forward + L2 adjoint of all 4.2 M rays, one persistent
                       kernel, HIP-graph replay); also timed: the same work as per-scene launches
+                      (forward then L2 backward per scene, and the fused per-scene call)
   sbs_diff_256_8x256  differentiable brick set: 64-sphere union (seed 4) on a 256^3 grid of 8^3
                       fp32 bricks with per-lattice-point RGB albedo; 8 ring views of 256x256; a
                       step = forward render + adjoint of the L2 loss against the image of a second
-                      scene (seed 5); rays/s counts each ray once per step
+                      scene (seed 5), one fused traversal per ray; rays/s counts each ray once per
+                      step
 """
 import argparse
 import json
@@ -228,7 +230,15 @@ def run_batch(args, torch, g):
         torch.cuda.synchronize()
         return a.elapsed_time(b) / steps
 
+    def plain_fused():
+        for it in items:
+            it["grad"].zero_()
+            it["loss"].zero_()
+            it["scene"].sbs_render_l2_backward(it["geom_id"], it["params"], it["pos"], it["dir"], it["target"],
+                                               it["colour"], it["grad"], it["loss"], stream=sh)
+
     ms_plain = timed(plain, max(3, args.steps // 5))
+    ms_plain_fused = timed(plain_fused, max(3, args.steps // 5))
     ms_eager = timed(lambda: batch.run(sh), args.steps)
     graph = torch.cuda.CUDAGraph()
     side = torch.cuda.Stream()
@@ -247,6 +257,8 @@ def run_batch(args, torch, g):
             "hit_fraction": round(hit / n, 4), "scene_build_s": round(build_s, 1),
             "ms_batched_eager": round(ms_eager, 4), "ms_per_scene_launches": round(ms_plain, 4),
             "batched_vs_per_scene": round(ms_plain / ms, 2),
+            "ms_per_scene_fused_launches": round(ms_plain_fused, 4),
+            "batched_vs_per_scene_fused": round(ms_plain_fused / ms, 2),
             "recompute_misses": int(sum(it["scene"].sbs_recompute_misses() for it in items))}
     return {
         "metric": METRIC, "value": round(n / (ms * 1e-3) / 1e6, 2), "unit": "Mrays/s (fwd+bwd, batched)", "n_gpus": 1,
@@ -293,15 +305,15 @@ def run_diff(args, torch, g):
     hit_n = float((hits.cpu().numpy().view(g.HIT_DTYPE).reshape(-1)["primId"] != g.MISS).sum())
     trav = B_FAT_NODE * c[:, 0].sum() + B_TLAS_NODE * c[:, 3].sum() + B_LEAF * c[:, 1].sum() + \
         B_BRICK * c[:, 1].sum() + B_VOXEL * c[:, 5].sum() + B_INSTANCE * c[:, 4].sum()
-    # forward: rays in, colour out, traversal; backward: rays + target in, traversal again, the hit
+    # one fused pass: ray (32 B) and target (16 B) in, colour (16 B) out, one traversal, the hit
     # brick re-marched (bounded by one brick's voxels), 32 parameters read and 32 atomics per hit
-    alg_bytes = n * (32 + 16) + trav + n * (32 + 16) + trav + hit_n * (B_VOXEL * 8 + 32 * 4 + 32 * 8)
+    alg_bytes = n * (32 + 16 + 16) + trav + hit_n * (B_VOXEL * 8 + 32 * 4 + 32 * 8)
 
     def step():
         grad.zero_()
         loss.zero_()
-        scene.sbs_forward(gid, params, pos, dirs, colour, stream=sh)
-        scene.sbs_l2_backward(gid, params, pos, dirs, target, grad, loss, stream=sh)
+        # forward colour and L2 adjoint in one traversal per ray
+        scene.sbs_render_l2_backward(gid, params, pos, dirs, target, colour, grad, loss, stream=sh)
 
     for _ in range(args.warmup):
         step()

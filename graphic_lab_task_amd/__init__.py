@@ -140,6 +140,7 @@ def lib() -> ctypes.CDLL:
             "lrt_sbs_forward_dev": ([vp, u32, vp, vp, vp, u64, vp, vp], i32),
             "lrt_sbs_backward_dev": ([vp, u32, vp, vp, vp, u64, vp, vp, vp], i32),
             "lrt_sbs_l2_backward_dev": ([vp, u32, vp, vp, vp, u64, vp, vp, vp, vp], i32),
+            "lrt_sbs_render_l2_backward_dev": ([vp, u32, vp, vp, vp, u64, vp, vp, vp, vp, vp], i32),
             "lrt_sbs_recompute_misses": ([vp, ctypes.POINTER(u64), i32], i32),
             "lrt_sbs_batch_create": ([vp, u32, ctypes.POINTER(vp)], i32),
             "lrt_sbs_batch_run": ([vp, vp], i32),
@@ -172,7 +173,7 @@ def exported_symbols():
             "lrt_trace_closest_host", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
             "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_set_ray_interleave",
             "lrt_autotune", "lrt_sbs_param_layout",
-            "lrt_sbs_forward_dev", "lrt_sbs_backward_dev", "lrt_sbs_l2_backward_dev", "lrt_sbs_recompute_misses",
+            "lrt_sbs_forward_dev", "lrt_sbs_backward_dev", "lrt_sbs_l2_backward_dev", "lrt_sbs_render_l2_backward_dev", "lrt_sbs_recompute_misses",
             "lrt_sbs_batch_create", "lrt_sbs_batch_run", "lrt_sbs_batch_destroy",
             "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
             "lrt_camera_inverse", "lrt_gen_icosphere", "lrt_gen_geodesic", "lrt_sdf_spheres_frame_octree",
@@ -355,6 +356,12 @@ class SceneRT:
     def sbs_l2_backward(self, geom_id, params, pos, dir, target, grad, loss=None, stream: int = 0):
         _check(lib().lrt_sbs_l2_backward_dev(self._h, geom_id, _ptr(params), _ptr(pos), _ptr(dir), pos.numel() // 4,
                                              _ptr(target), _ptr(grad), _ptr(loss), stream))
+
+    def sbs_render_l2_backward(self, geom_id, params, pos, dir, target, colour, grad, loss=None, stream: int = 0):
+        """Forward colour and the L2 adjoint (grad and loss accumulated) in one traversal per ray."""
+        _check(lib().lrt_sbs_render_l2_backward_dev(self._h, geom_id, _ptr(params), _ptr(pos), _ptr(dir),
+                                                    pos.numel() // 4, _ptr(target), _ptr(colour), _ptr(grad),
+                                                    _ptr(loss), stream))
 
     def sbs_recompute_misses(self, reset: bool = False) -> int:
         v = ctypes.c_uint64()

@@ -516,6 +516,16 @@ lrt_status lrt_sbs_l2_backward_dev(lrt_scene* s, uint32_t geom_id, const float* 
   });
 }
 
+lrt_status lrt_sbs_render_l2_backward_dev(lrt_scene* s, uint32_t geom_id, const float* params, const float* pos_near,
+                                          const float* dir_far, uint64_t n, const float* target, float* colour,
+                                          float* dL_dparams, float* loss, void* stream) {
+  return guarded([&]() -> lrt_status {
+    if (!target || !colour || !dL_dparams) return fail(LRT_ERR_INVALID_ARG, "null target, colour or gradient buffer");
+    return sbs_diff_common(s, geom_id, params, pos_near, dir_far, n, colour, nullptr, target, dL_dparams, loss,
+                           stream);
+  });
+}
+
 lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_items, lrt_sbs_batch** out) {
   return guarded([&]() -> lrt_status {
     if (!out || (!items && n_items)) return fail(LRT_ERR_INVALID_ARG, "null argument");
@@ -565,7 +575,7 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
       zero_begin.push_back(zero_begin.back() + it.n_params);
       mask_begin.push_back(mask_begin.back() + it.n_bricks * wpb);
       brick_begin.push_back(brick_begin.back() + it.n_bricks);
-      chunk_begin.push_back(chunk_begin.back() + (it.args.n + lrt::SBS_BATCH_BLOCK - 1) / lrt::SBS_BATCH_BLOCK);
+      chunk_begin.push_back(chunk_begin.back() + (it.args.n + lrt::SBS_BATCH_CHUNK - 1) / lrt::SBS_BATCH_CHUNK);
       b->items.push_back(it);
     }
     e = b->d_masks.reserve(std::max<size_t>(mask_words, 1) * 4);

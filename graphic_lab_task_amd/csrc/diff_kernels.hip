@@ -123,9 +123,10 @@ __global__ __launch_bounds__(BLOCK) void k_sbs_diff(SceneView s, SbsDiffArgs a) 
   }
   __syncthreads();
   const uint32_t b = a.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-  const uint32_t i = b * BLOCK + threadIdx.x;
+  uint32_t i = b * BLOCK + threadIdx.x;
   float loss = 0.0f;
   if (i < a.n) {
+    i = interleave_ray(i, a.n, 3);   // groups of 8 lanes from distant parts of the batch
     LaneStack<S, BLOCK> st;
     st.col = (lds_u32*)(lds) + threadIdx.x;
     st.spill = a.spill + size_t(i) * (STACK_SIZE - S);
@@ -200,8 +201,14 @@ __global__ void k_batch_mask_summary(SbsBatchLaunch L) {
   m[0] = any != 0u ? 1u : 0u;
 }
 
+// Persistent one-wave blocks: each takes the next 64-ray chunk of the whole batch from the
+// global counter (one wave per block, so the block barrier that publishes the chunk never makes
+// a wave wait for another's slow rays; the barrier also keeps the wave converged around the
+// broadcast, which a lane-0 atomic followed by a lane read does not guarantee). Rays stay in
+// batch order: here throughput counts, and neighbouring rays of similar cost share a chunk.
 template <int S, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sbs_batch(SbsBatchLaunch L) {
+  static_assert(BLOCK == 64 && SBS_BATCH_CHUNK == 64, "one wave per block, one chunk per wave");
   __shared__ uint32_t lds[S * BLOCK];
   __shared__ uint32_t chunk_sh;
   for (;;) {
@@ -212,7 +219,7 @@ __global__ __launch_bounds__(BLOCK) void k_sbs_batch(SbsBatchLaunch L) {
     if (chunk >= L.total_chunks) break;   // the same for the whole block: every block ends
     const uint32_t it = __builtin_amdgcn_readfirstlane(find_item(L.chunk_begin, L.n_items, chunk));
     const SbsBatchItem& item = L.items[it];
-    const uint32_t i = (chunk - L.chunk_begin[it]) * BLOCK + threadIdx.x;
+    const uint32_t i = (chunk - L.chunk_begin[it]) * SBS_BATCH_CHUNK + threadIdx.x;
     float loss = 0.0f;
     if (i < item.args.n) {
       LaneStack<S, BLOCK> st;
@@ -260,6 +267,7 @@ hipError_t launch_sbs_diff(const SceneView& s, const SbsDiffArgs& a, hipStream_t
   if (a.n == 0) return hipSuccess;
   SceneView v = s;
   v.sbs_data_f = a.params;   // distances come from the caller's parameter array
+  if (a.colour && a.target) return launch_mode<3>(v, a, stream);
   if (a.colour) return launch_mode<0>(v, a, stream);
   if (a.dL_dcolour) return launch_mode<1>(v, a, stream);
   return launch_mode<2>(v, a, stream);

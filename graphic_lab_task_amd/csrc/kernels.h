@@ -91,14 +91,15 @@ struct SbsBatchLaunch {
   const uint64_t* zero_begin;    // [n_items + 1] prefix sums of n_params
   const uint32_t* mask_begin;    // [n_items + 1] prefix sums of mask words (bit words only)
   const uint32_t* brick_begin;   // [n_items + 1] prefix sums of n_bricks
-  const uint32_t* chunk_begin;   // [n_items + 1] prefix sums of ray chunks
+  const uint32_t* chunk_begin;   // [n_items + 1] prefix sums of ray chunks (SBS_BATCH_CHUNK rays)
   uint64_t total_zero;
   uint32_t total_mask_words, total_bricks, total_chunks;
   uint32_t* counter;             // work counter (reset by the zero kernel)
   uint32_t* spill;               // grid * BLOCK * (STACK_SIZE - 16) words
   uint32_t grid;                 // persistent blocks
 };
-constexpr int SBS_BATCH_BLOCK = 128;
+constexpr int SBS_BATCH_BLOCK = 64;
+constexpr int SBS_BATCH_CHUNK = 64;   // rays per work item of the persistent batch kernel (one wave)
 hipError_t launch_sbs_batch(const SbsBatchLaunch& L, hipStream_t stream);
 // Resident blocks of the persistent batch kernel per CU (occupancy query).
 hipError_t sbs_batch_blocks_per_cu(int* blocks);

@@ -237,6 +237,21 @@ __device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 
   return hit;
 }
 
+// Ray of thread i when the lanes of each wave are dealt out in groups of 2^group_log2: group g
+// of wave w takes the same lanes of 64-ray chunk (w + g * n64 / groups) mod n64, so a wave holds
+// rays from 64 / 2^group_log2 distant parts of the batch. Neighbouring primary rays graze the
+// same silhouettes and a wave steps at the pace of its slowest lane; spreading them shortens the
+// slowest waves. A bijection on [0, 64 * n64); the last n mod 64 rays keep their thread.
+__device__ __forceinline__ uint32_t interleave_ray(uint32_t i, uint32_t n, uint32_t group_log2) {
+  const uint32_t n64 = n >> 6;
+  const uint32_t groups = 64u >> group_log2;
+  if (n64 < groups || i >= n64 * 64u) return i;
+  const uint32_t w = i >> 6, l = i & 63u, g = l >> group_log2;
+  uint32_t c = w + g * (n64 / groups);   // < 2 * n64
+  if (c >= n64) c -= n64;
+  return c * 64u + l;
+}
+
 // Logical block index so that each XCD (blocks are dealt round-robin to the 8 XCDs) works
 // on one contiguous range of rays and keeps its own L2 warm with the nodes those rays touch.
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {

@@ -189,6 +189,11 @@ lrt_status lrt_sbs_backward_dev(lrt_scene* s, uint32_t geom_id, const float* par
 lrt_status lrt_sbs_l2_backward_dev(lrt_scene* s, uint32_t geom_id, const float* params, const float* pos_near,
                                    const float* dir_far, uint64_t n, const float* target, float* dL_dparams,
                                    float* loss, void* stream);
+/* Forward render and L2 adjoint in one traversal per ray: colour as lrt_sbs_forward_dev, and
+ * dL_dparams / *loss as lrt_sbs_l2_backward_dev (loss may be NULL). */
+lrt_status lrt_sbs_render_l2_backward_dev(lrt_scene* s, uint32_t geom_id, const float* params, const float* pos_near,
+                                          const float* dir_far, uint64_t n, const float* target, float* colour,
+                                          float* dL_dparams, float* loss, void* stream);
 /* Batched differentiable launch (no counterpart in the reference): many brick-set scenes, each
  * with its own parameters, rays and target, run as one plan. Every run zeroes each item's grad
  * (scene values_f-sized) and loss, builds the occupancy masks from the items' parameters, and

@@ -106,6 +106,18 @@ def test_fused_l2_and_autograd_agree(lrt, torch_dev):
     torch.cuda.synchronize()
     assert abs(L.item() - loss.item()) <= 1e-4 * L.item()
     assert rel_norm(p.grad.cpu().numpy(), g_fused.cpu().numpy()) < 1e-5
+    # forward render and L2 adjoint in one traversal: the same colour bits, the same gradient
+    # up to the order of the atomic sums
+    col_f = torch.zeros_like(tgt)
+    g_rf = torch.zeros_like(params)
+    loss_rf = torch.zeros(1, dtype=torch.float32, device="cuda")
+    s.sbs_render_l2_backward(gid, params, pos, dirs, tgt, col_f, g_rf, loss_rf)
+    col0 = torch.zeros_like(tgt)
+    s.sbs_forward(gid, params, pos, dirs, col0)
+    torch.cuda.synchronize()
+    assert torch.equal(col_f, col0)
+    assert abs(loss_rf.item() - loss.item()) <= 1e-5 * loss.item()
+    assert rel_norm(g_rf.cpu().numpy(), g_fused.cpu().numpy()) < 1e-5
 
 
 def test_c3_gradients_vs_finite_differences(lrt, orc, torch_dev):
