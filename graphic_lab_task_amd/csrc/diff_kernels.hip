@@ -165,28 +165,28 @@ __global__ void k_batch_zero(SbsBatchLaunch L) {
   }
 }
 
+// One thread per mask bit, 32 consecutive lanes per word (as k_sbs_mask_words).
 __global__ void k_batch_mask_words(SbsBatchLaunch L) {
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= L.total_mask_words) return;
-  const uint32_t it = find_item(L.mask_begin, L.n_items, tid);
-  const SbsBatchItem& item = L.items[it];
-  const SceneView& s = item.view;
-  const SdfSBSHeader h = s.sbs_headers[0];
-  const uint32_t bs = h.brick_size, v3 = bs * bs * bs, wpb = (v3 + 31u) / 32u;
-  const uint32_t local = tid - L.mask_begin[it];
-  const uint32_t brick = local / wpb, w = local % wpb;
-  const uint32_t node_id = brick + s.sbs_roots[0];
-  const uint32_t v_size = bs + 2 * h.brick_pad + 1;
-  const float sz_inv = 2.0f / float(s.sbs_nodes[node_id].pos_z_lod_size & 0x0000FFFF);
-  uint32_t word = 0;
-  for (uint32_t b = 0; b < 32u; ++b) {
-    const uint32_t v = w * 32u + b;
-    if (v >= v3) break;
-    const float3 vp = make3(float(v / (bs * bs)), float((v / bs) % bs), float(v % bs));
-    float values[8];
-    if (sbs_load_values(s, h, node_id, vp, v_size, sz_inv, values) <= 0.0f) word |= 1u << b;
+  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t word_id = tid >> 5;
+  const bool in = word_id < L.total_mask_words;
+  bool bit = false;
+  uint32_t it = 0, brick = 0, w = 0, wpb = 1;
+  if (in) {
+    it = find_item(L.mask_begin, L.n_items, uint32_t(word_id));
+    const SceneView& s = L.items[it].view;
+    const SdfSBSHeader h = s.sbs_headers[0];
+    const uint32_t bs = h.brick_size, v3 = bs * bs * bs;
+    wpb = (v3 + 31u) / 32u;
+    const uint32_t local = uint32_t(word_id) - L.mask_begin[it];
+    brick = local / wpb;
+    w = local % wpb;
+    const uint32_t v = w * 32u + uint32_t(tid & 31u);
+    bit = v < v3 && sbs_voxel_occupied(s, h, s.sbs_roots[0], brick, v);
   }
-  item.mask[size_t(brick) * (1u + wpb) + 1u + w] = word;
+  const uint64_t m = __ballot(bit);   // every lane of the wave reaches this
+  if (in && (threadIdx.x & 31u) == 0u)
+    L.items[it].mask[size_t(brick) * (1u + wpb) + 1u + w] = uint32_t(m >> (threadIdx.x & 32u));
 }
 
 __global__ void k_batch_mask_summary(SbsBatchLaunch L) {
@@ -256,7 +256,8 @@ hipError_t launch_sbs_batch(const SbsBatchLaunch& L, hipStream_t stream) {
   if (L.n_items == 0) return hipSuccess;
   hipLaunchKernelGGL(k_batch_zero, dim3(2048), dim3(256), 0, stream, L);
   if (L.total_mask_words)
-    hipLaunchKernelGGL(k_batch_mask_words, dim3((L.total_mask_words + 255) / 256), dim3(256), 0, stream, L);
+    hipLaunchKernelGGL(k_batch_mask_words, dim3(uint32_t((uint64_t(L.total_mask_words) * 32u + 255) / 256)), dim3(256),
+                       0, stream, L);
   if (L.total_bricks)
     hipLaunchKernelGGL(k_batch_mask_summary, dim3((L.total_bricks + 255) / 256), dim3(256), 0, stream, L);
   hipLaunchKernelGGL((k_sbs_batch<16, SBS_BATCH_BLOCK>), dim3(L.grid), dim3(SBS_BATCH_BLOCK), 0, stream, L);

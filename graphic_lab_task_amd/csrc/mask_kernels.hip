@@ -9,26 +9,19 @@ namespace lrt {
 
 namespace {
 
+// One thread per mask bit, 32 consecutive lanes per word: every voxel's 8 corner loads are in
+// flight at once, and one wave ballot yields two words.
 __global__ void k_sbs_mask_words(SceneView s, uint32_t sdf_id, uint32_t n_bricks, uint32_t* __restrict__ mask) {
   const SdfSBSHeader h = s.sbs_headers[sdf_id];
   const uint32_t bs = h.brick_size, v3 = bs * bs * bs, wpb = (v3 + 31u) / 32u;
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= n_bricks * wpb) return;
-  const uint32_t brick = tid / wpb, w = tid % wpb;
-  const uint32_t node_id = brick + s.sbs_roots[sdf_id];
-  const uint32_t v_size = bs + 2 * h.brick_pad + 1;
-  const float sz = float(s.sbs_nodes[node_id].pos_z_lod_size & 0x0000FFFF);
-  const float sz_inv = 2.0f / sz;
-  uint32_t word = 0;
-  for (uint32_t b = 0; b < 32u; ++b) {
-    const uint32_t v = w * 32u + b;
-    if (v >= v3) break;
-    const float3 vp = make3(float(v / (bs * bs)), float((v / bs) % bs), float(v % bs));
-    float values[8];
-    const float vmin = sbs_load_values(s, h, node_id, vp, v_size, sz_inv, values);
-    if (vmin <= 0.0f) word |= 1u << b;
-  }
-  mask[size_t(brick) * (1u + wpb) + 1u + w] = word;
+  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t word_id = tid >> 5;
+  const bool in = word_id < uint64_t(n_bricks) * wpb;
+  const uint32_t brick = in ? uint32_t(word_id / wpb) : 0u, w = in ? uint32_t(word_id % wpb) : 0u;
+  const uint32_t v = w * 32u + uint32_t(tid & 31u);
+  const bool bit = in && v < v3 && sbs_voxel_occupied(s, h, s.sbs_roots[sdf_id], brick, v);
+  const uint64_t m = __ballot(bit);   // every lane of the wave reaches this
+  if (in && (threadIdx.x & 31u) == 0u) mask[size_t(brick) * (1u + wpb) + 1u + w] = uint32_t(m >> (threadIdx.x & 32u));
 }
 
 __global__ void k_sbs_mask_summary(uint32_t n_bricks, uint32_t wpb, uint32_t* __restrict__ mask) {
@@ -48,9 +41,9 @@ hipError_t launch_sbs_masks(const SceneView& s, const SbsMaskGeom* geoms, uint32
     const SbsMaskGeom& m = geoms[g];
     if (m.n_bricks == 0) continue;
     const uint32_t wpb = (m.brick_size * m.brick_size * m.brick_size + 31u) / 32u;
-    const uint32_t words = m.n_bricks * wpb;
-    hipLaunchKernelGGL(k_sbs_mask_words, dim3((words + 255) / 256), dim3(256), 0, stream, s, m.sdf_id, m.n_bricks,
-                       mask + m.base);
+    const uint64_t bits = uint64_t(m.n_bricks) * wpb * 32u;
+    hipLaunchKernelGGL(k_sbs_mask_words, dim3(uint32_t((bits + 255) / 256)), dim3(256), 0, stream, s, m.sdf_id,
+                       m.n_bricks, mask + m.base);
     hipLaunchKernelGGL(k_sbs_mask_summary, dim3((m.n_bricks + 255) / 256), dim3(256), 0, stream, m.n_bricks, wpb,
                        mask + m.base);
   }

@@ -457,6 +457,19 @@ __device__ __forceinline__ float sbs_load_values(const SceneView& s, const SdfSB
   return vmin;
 }
 
+// Occupancy bit of voxel v (x-major, v < bs^3) of brick `brick`: the minimum of its 8 corner
+// values, loaded as the march loads them, is <= 0.
+__device__ __forceinline__ bool sbs_voxel_occupied(const SceneView& s, const SdfSBSHeader& h, uint32_t root,
+                                                   uint32_t brick, uint32_t v) {
+  const uint32_t bs = h.brick_size;
+  const uint32_t node_id = brick + root;
+  const uint32_t v_size = bs + 2 * h.brick_pad + 1;
+  const float sz_inv = 2.0f / float(s.sbs_nodes[node_id].pos_z_lod_size & 0x0000FFFF);
+  const float3 vp = make3(float(v / (bs * bs)), float((v / bs) % bs), float(v % bs));
+  float values[8];
+  return sbs_load_values(s, h, node_id, vp, v_size, sz_inv, values) <= 0.0f;
+}
+
 // The voxel whose surface produced a brick-set hit: the value indices of its 8 corners (into
 // the scene's values_f, indexed layouts), the entry point in voxel units, and the hit
 // parameter along the ray direction from there.

@@ -9,6 +9,7 @@
       FETCH_SIZE of the copy, 256 MiB written / WRITE_SIZE of the copy.
 """
 import argparse
+import re
 import csv
 import glob
 import os
@@ -37,9 +38,11 @@ def _col(row, *names):
 
 
 def short_name(name):
-    for k in TRACE_KERNELS:
-        if k in name:
-            return name[name.index(k):].split("(")[0]
+    """Kernel name without return type, namespaces and parameter list (library kernels)."""
+    name = name.replace("(anonymous namespace)::", "")
+    m = re.search(r"\blrt::(k_\w+)", name)
+    if m:
+        return name[m.start(1):].split("(")[0]
     return name.split("(")[0][:80]
 
 
@@ -87,11 +90,11 @@ def cmd_traffic(a):
     ck = copy_names[0]
     kib_f, kib_w = statistics.median(fetch[ck]), statistics.median(write[ck])
     scale_f, scale_w = COPY_BYTES / (kib_f * 1024.0), COPY_BYTES / (kib_w * 1024.0)
-    lines = [f"# HBM traffic per trace call from rocprofv3 counters", f"command: {a.command}",
+    lines = [f"# HBM traffic per step from rocprofv3 counters", f"command: {a.command}",
              f"copy reference kernel: {ck}: reads and writes {COPY_BYTES} bytes; FETCH_SIZE {kib_f:.1f} KiB -> "
              f"scale {scale_f:.4f}, WRITE_SIZE {kib_w:.1f} KiB -> scale {scale_w:.4f}", ""]
     total = 0.0
-    for k in TRACE_KERNELS:
+    for k in (a.kernels.split(",") if a.kernels else TRACE_KERNELS):
         f = [v for name, vs in fetch.items() if name.startswith(k) and "true" not in name for v in vs]
         w = [v for name, vs in write.items() if name.startswith(k) and "true" not in name for v in vs]
         if not f:
@@ -115,6 +118,7 @@ def main(argv=None):
     t.add_argument("write_dir")
     t.add_argument("--workload", required=True)
     t.add_argument("--command", required=True)
+    t.add_argument("--kernels", default="", help="comma-separated kernel name prefixes (default: the trace kernels)")
     a = ap.parse_args(argv)
     {"stats": cmd_stats, "traffic": cmd_traffic}[a.cmd](a)
 
