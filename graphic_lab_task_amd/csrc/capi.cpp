@@ -530,6 +530,7 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
   return guarded([&]() -> lrt_status {
     if (!out || (!items && n_items)) return fail(LRT_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
+    if (n_items > 65535) return fail(LRT_ERR_INVALID_ARG, "at most 65535 items per batch");
     auto b = std::make_unique<lrt_sbs_batch>();
     hipError_t e = hipGetDevice(&b->device);
     if (e != hipSuccess) return hip_fail(e, "no device");
@@ -537,6 +538,7 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
     std::vector<uint32_t> mask_begin{0}, brick_begin{0}, chunk_begin{0};
     std::vector<size_t> mask_offset;
     size_t mask_words = 0;
+    uint64_t max_zero4 = 1, max_bits = 1, max_bricks = 1;   // per-item extents of the set-up grids
     for (uint32_t k = 0; k < n_items; ++k) {
       const lrt_sbs_batch_item& in = items[k];
       if (!in.scene) return fail(LRT_ERR_INVALID_ARG, "null scene in batch item");
@@ -572,6 +574,9 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
       const uint32_t wpb = (it.brick_size * it.brick_size * it.brick_size + 31u) / 32u;
       mask_offset.push_back(mask_words);
       mask_words += size_t(it.n_bricks) * (1 + wpb);
+      max_zero4 = std::max<uint64_t>(max_zero4, (it.n_params + 3) / 4);
+      max_bits = std::max<uint64_t>(max_bits, uint64_t(it.n_bricks) * wpb * 32u);
+      max_bricks = std::max<uint64_t>(max_bricks, it.n_bricks);
       zero_begin.push_back(zero_begin.back() + it.n_params);
       mask_begin.push_back(mask_begin.back() + it.n_bricks * wpb);
       brick_begin.push_back(brick_begin.back() + it.n_bricks);
@@ -616,6 +621,10 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
     L.counter = static_cast<uint32_t*>(b->d_counter.ptr());
     L.spill = static_cast<uint32_t*>(b->d_spill.ptr());
     L.grid = grid;
+    auto blocks = [](uint64_t work, uint64_t cap) { return uint32_t(std::min<uint64_t>(cap, (work + 255) / 256)); };
+    L.zero_blocks = blocks(max_zero4, 1024);
+    L.mask_blocks = blocks(max_bits, 4096);
+    L.summary_blocks = blocks(max_bricks, 256);
     *out = b.release();
     return LRT_OK;
   });

@@ -155,50 +155,58 @@ __device__ __forceinline__ uint32_t find_item(const T* begin, uint32_t n_items, 
   return lo;
 }
 
+// The per-item set-up kernels run on a 2D grid, blockIdx.y = item, each item's blocks striding
+// over its own range: no thread searches for its item.
 __global__ void k_batch_zero(SbsBatchLaunch L) {
-  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tid == 0) *L.counter = 0;
-  if (tid < L.n_items && L.items[tid].args.loss) *L.items[tid].args.loss = 0.0f;
-  for (uint64_t k = tid; k < L.total_zero; k += uint64_t(gridDim.x) * blockDim.x) {
-    const uint32_t it = find_item(L.zero_begin, L.n_items, k);
-    L.items[it].args.grad[k - L.zero_begin[it]] = 0.0f;
+  const SbsBatchItem& item = L.items[blockIdx.y];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blockIdx.y == 0) *L.counter = 0;
+    if (item.args.loss) *item.args.loss = 0.0f;
   }
+  float* g = item.args.grad;
+  const uint64_t n = item.n_params;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint64_t head = 0;   // 16-byte stores from the first 16-byte aligned element on
+  if ((reinterpret_cast<uintptr_t>(g) & 15u) == 0u) {
+    uint4* g4 = reinterpret_cast<uint4*>(g);
+    for (uint64_t k = t; k < n / 4; k += stride) g4[k] = make_uint4(0u, 0u, 0u, 0u);
+    head = n / 4 * 4;
+  }
+  for (uint64_t k = head + t; k < n; k += stride) g[k] = 0.0f;
 }
 
-// One thread per mask bit, 32 consecutive lanes per word (as k_sbs_mask_words).
+// One thread per mask bit, 32 consecutive lanes per word (as k_sbs_mask_words); the loop bound
+// is the same for a whole wave, so every lane reaches the ballot.
 __global__ void k_batch_mask_words(SbsBatchLaunch L) {
-  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const uint64_t word_id = tid >> 5;
-  const bool in = word_id < L.total_mask_words;
-  bool bit = false;
-  uint32_t it = 0, brick = 0, w = 0, wpb = 1;
-  if (in) {
-    it = find_item(L.mask_begin, L.n_items, uint32_t(word_id));
-    const SceneView& s = L.items[it].view;
-    const SdfSBSHeader h = s.sbs_headers[0];
-    const uint32_t bs = h.brick_size, v3 = bs * bs * bs;
-    wpb = (v3 + 31u) / 32u;
-    const uint32_t local = uint32_t(word_id) - L.mask_begin[it];
-    brick = local / wpb;
-    w = local % wpb;
+  const SbsBatchItem& item = L.items[blockIdx.y];
+  const SceneView& s = item.view;
+  const SdfSBSHeader h = s.sbs_headers[0];
+  const uint32_t bs = h.brick_size, v3 = bs * bs * bs, wpb = (v3 + 31u) / 32u;
+  const uint64_t bits = uint64_t(item.n_bricks) * wpb * 32u;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint64_t wave = uint64_t(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u); wave < bits; wave += stride) {
+    const uint64_t tid = wave + lane;
+    const uint64_t word_id = tid >> 5;
+    const bool in = tid < bits;
+    const uint32_t brick = in ? uint32_t(word_id / wpb) : 0u, w = in ? uint32_t(word_id % wpb) : 0u;
     const uint32_t v = w * 32u + uint32_t(tid & 31u);
-    bit = v < v3 && sbs_voxel_occupied(s, h, s.sbs_roots[0], brick, v);
+    const bool bit = in && v < v3 && sbs_voxel_occupied(s, h, s.sbs_roots[0], brick, v);
+    const uint64_t m = __ballot(bit);
+    if (in && (lane & 31u) == 0u) item.mask[size_t(brick) * (1u + wpb) + 1u + w] = uint32_t(m >> (lane & 32u));
   }
-  const uint64_t m = __ballot(bit);   // every lane of the wave reaches this
-  if (in && (threadIdx.x & 31u) == 0u)
-    L.items[it].mask[size_t(brick) * (1u + wpb) + 1u + w] = uint32_t(m >> (threadIdx.x & 32u));
 }
 
 __global__ void k_batch_mask_summary(SbsBatchLaunch L) {
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= L.total_bricks) return;
-  const uint32_t it = find_item(L.brick_begin, L.n_items, tid);
-  const SbsBatchItem& item = L.items[it];
+  const SbsBatchItem& item = L.items[blockIdx.y];
   const uint32_t bs = item.brick_size, wpb = (bs * bs * bs + 31u) / 32u;
-  uint32_t* m = item.mask + size_t(tid - L.brick_begin[it]) * (1u + wpb);
-  uint32_t any = 0;
-  for (uint32_t w = 0; w < wpb; ++w) any |= m[1 + w];
-  m[0] = any != 0u ? 1u : 0u;
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < item.n_bricks; b += gridDim.x * blockDim.x) {
+    uint32_t* m = item.mask + size_t(b) * (1u + wpb);
+    uint32_t any = 0;
+    for (uint32_t w = 0; w < wpb; ++w) any |= m[1 + w];
+    m[0] = any != 0u ? 1u : 0u;
+  }
 }
 
 // Persistent one-wave blocks: each takes the next 64-ray chunk of the whole batch from the
@@ -254,12 +262,12 @@ hipError_t sbs_batch_blocks_per_cu(int* blocks) {
 
 hipError_t launch_sbs_batch(const SbsBatchLaunch& L, hipStream_t stream) {
   if (L.n_items == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_batch_zero, dim3(2048), dim3(256), 0, stream, L);
+  // x extents sized for the largest item (smaller items leave blocks idle)
+  hipLaunchKernelGGL(k_batch_zero, dim3(L.zero_blocks, L.n_items), dim3(256), 0, stream, L);
   if (L.total_mask_words)
-    hipLaunchKernelGGL(k_batch_mask_words, dim3(uint32_t((uint64_t(L.total_mask_words) * 32u + 255) / 256)), dim3(256),
-                       0, stream, L);
+    hipLaunchKernelGGL(k_batch_mask_words, dim3(L.mask_blocks, L.n_items), dim3(256), 0, stream, L);
   if (L.total_bricks)
-    hipLaunchKernelGGL(k_batch_mask_summary, dim3((L.total_bricks + 255) / 256), dim3(256), 0, stream, L);
+    hipLaunchKernelGGL(k_batch_mask_summary, dim3(L.summary_blocks, L.n_items), dim3(256), 0, stream, L);
   hipLaunchKernelGGL((k_sbs_batch<16, SBS_BATCH_BLOCK>), dim3(L.grid), dim3(SBS_BATCH_BLOCK), 0, stream, L);
   return hipGetLastError();
 }

@@ -94,6 +94,7 @@ struct SbsBatchLaunch {
   const uint32_t* chunk_begin;   // [n_items + 1] prefix sums of ray chunks (SBS_BATCH_CHUNK rays)
   uint64_t total_zero;
   uint32_t total_mask_words, total_bricks, total_chunks;
+  uint32_t zero_blocks, mask_blocks, summary_blocks;   // x extents of the per-item set-up grids
   uint32_t* counter;             // work counter (reset by the zero kernel)
   uint32_t* spill;               // grid * BLOCK * (STACK_SIZE - 16) words
   uint32_t grid;                 // persistent blocks

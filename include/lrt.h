@@ -215,6 +215,7 @@ typedef struct lrt_sbs_batch_item {
   float* colour;           /* device float4 per ray (may be NULL) */
   uint64_t n_rays;
 } lrt_sbs_batch_item;
+/* n_items <= 65535. */
 lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_items, lrt_sbs_batch** out);
 lrt_status lrt_sbs_batch_run(lrt_sbs_batch* b, void* stream);
 lrt_status lrt_sbs_batch_destroy(lrt_sbs_batch* b);
