@@ -659,12 +659,44 @@ struct OTEntry {
   uint32_t p_zs;    // (z << 16) | level size
 };
 constexpr int OT_STACK = 32;
+constexpr int OT_LDS = 8;    // entries kept in LDS when the caller provides room
+
+// The walk's stack: entries below OT_LDS in the caller's LDS columns (entry k, word j at
+// col[(4k + j) * BLOCK]) when `col` is set, the others in private memory, which a walk rarely
+// reaches (private memory is scratch, written back through the caches to HBM).
+template <int BLOCK>
+struct OTStack {
+  lds_u32* col;
+  OTEntry priv[OT_STACK];
+  __device__ __forceinline__ OTEntry get(int k) const {
+    if (col && k < OT_LDS)
+      return OTEntry{col[(4 * k) * BLOCK], col[(4 * k + 1) * BLOCK], col[(4 * k + 2) * BLOCK], col[(4 * k + 3) * BLOCK]};
+    return priv[k];
+  }
+  __device__ __forceinline__ void set(int k, const OTEntry& e) {
+    if (col && k < OT_LDS) {
+      col[(4 * k) * BLOCK] = e.node_id;
+      col[(4 * k + 1) * BLOCK] = e.info;
+      col[(4 * k + 2) * BLOCK] = e.p_xy;
+      col[(4 * k + 3) * BLOCK] = e.p_zs;
+    } else {
+      priv[k] = e;
+    }
+  }
+  __device__ __forceinline__ void set_info(int k, uint32_t info) {
+    if (col && k < OT_LDS) col[(4 * k + 1) * BLOCK] = info;
+    else priv[k].info = info;
+  }
+};
 
 // Walks the octree in front-to-back child order with mirrored coordinates (octant mask `a`)
-// and stops at the first leaf whose surface is hit. The stack lives in private memory.
+// and stops at the first leaf whose surface is hit. `ot_col`: LDS columns for the top of the
+// stack (OT_LDS x 4 words per lane, stride BLOCK; may be null: all in private memory).
 // Returns the bytes read from the octree in 16-byte units (node records and voxel values).
-__device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir, float tNear,
-                                       uint32_t instId, uint32_t geomId, CRT_Hit& hit) {
+template <int BLOCK = 64>
+__device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir,
+                                           float tNear, uint32_t instId, uint32_t geomId, CRT_Hit& hit,
+                                           lds_u32* ot_col = nullptr) {
   float3 pos_ray_pos = ray_pos;
   float3 pos_ray_dir = ray_dir;
   uint32_t a = 0;
@@ -679,16 +711,18 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g,
   // nn_indices of the reference, packed as x | y << 4 | z << 8
   const uint32_t nn[8] = {0x124u, 0x835u, 0x386u, 0x887u, 0x568u, 0x878u, 0x788u, 0x888u};
 
-  OTEntry stack[OT_STACK];
+  OTStack<BLOCK> stack;
+  stack.col = ot_col;
   int top = 0;
   uint32_t units = 0;
   const float old_t = hit.t;
   const uint32_t type = g.type;
   const uint32_t root = g.offset.x;
-  stack[0] = OTEntry{root, 0u, 0u, 1u};
+  stack.set(0, OTEntry{root, 0u, 0u, 1u});
   while (top >= 0) {
-    const uint32_t level_sz = stack[top].p_zs & 0xFFFF;
-    const uint32_t px = stack[top].p_xy >> 16, py = stack[top].p_xy & 0xFFFF, pz = stack[top].p_zs >> 16;
+    OTEntry e = stack.get(top);
+    const uint32_t level_sz = e.p_zs & 0xFFFF;
+    const uint32_t px = e.p_xy >> 16, py = e.p_xy & 0xFFFF, pz = e.p_zs >> 16;
     float d = 1.0f / float(level_sz);
     const float3 p_f = make3(float(px), float(py), float(pz));
     const float3 t0 = t0_ + d * p_f * l_;
@@ -698,14 +732,15 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g,
       top--;
       continue;
     }
-    bool is_leaf = stack[top].info > 0;
+    bool is_leaf = e.info > 0;
     if (!is_leaf) {
       const float t = s.preset.fixed_lod > 0 ? 1.0f : fmax_ref(t0.x, fmax_ref(t0.y, t0.z));
       const float target_lod_size = (powf(2.0f, s.preset.level_of_detail) - 0.01f) / t;
       if (float(level_sz) > target_lod_size) {
-        const SdfMultiOctreeNode n = s.multi_nodes[stack[top].node_id];
+        const SdfMultiOctreeNode n = s.multi_nodes[e.node_id];
         units++;
-        stack[top].info = n.data_offset;
+        e.info = n.data_offset;
+        stack.set_info(top, e.info);
         is_leaf = true;
         if (n.voxel_count == 0) {
           top--;
@@ -720,7 +755,7 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g,
       const uint32_t rx = (a & 4) ? (~px & p_mask) : px;
       const uint32_t ry = (a & 2) ? (~py & p_mask) : py;
       const uint32_t rz = (a & 1) ? (~pz & p_mask) : pz;
-      const uint32_t node_id = stack[top].node_id;
+      const uint32_t node_id = e.node_id;
       const float sz = 0.5f * float(level_sz);
       d = 1.0f / sz;
       const float3 min_pos = make3(-1, -1, -1) + d * make3(float(rx), float(ry), float(rz));
@@ -738,7 +773,7 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g,
       for (uint32_t v = 0; v < n.voxel_count; ++v) {
         float values[8];
 #pragma unroll
-        for (int i = 0; i < 8; i++) values[i] = s.multi_values[stack[top].info + 8 * v + i];
+        for (int i = 0; i < 8; i++) values[i] = s.multi_values[e.info + 8 * v + i];
         local_surface_intersection(s, TYPE_SDF_MULTI_OCTREE, ray_dir, instId, geomId, values, primId, d, qNear, qFar,
                                    fNearFar, start_q, hit, nullptr, surface);
       }
@@ -753,7 +788,7 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g,
       int buf_top = 0;
       const float3 tm = 0.5f * (t0 + t1);
       int curr = ot_first_node(t0, tm);
-      const uint32_t children = s.multi_nodes[stack[top].node_id].children_offset;
+      const uint32_t children = s.multi_nodes[e.node_id].children_offset;
       do {
         const uint32_t child = uint32_t(curr) ^ a;
         const SdfMultiOctreeNode cn = s.multi_nodes[children + child];
@@ -762,8 +797,8 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g,
         if ((cn.voxel_count > 0 || child_is_node) && buf_top < 4) {
           buf[buf_top].node_id = children + child;
           buf[buf_top].info = child_is_node ? 0u : cn.data_offset;
-          buf[buf_top].p_xy = (stack[top].p_xy << 1) | (((uint32_t(curr) & 4u) << (16 - 2)) | ((uint32_t(curr) & 2u) >> 1));
-          buf[buf_top].p_zs = (stack[top].p_zs << 1) | ((uint32_t(curr) & 1u) << 16);
+          buf[buf_top].p_xy = (e.p_xy << 1) | (((uint32_t(curr) & 4u) << (16 - 2)) | ((uint32_t(curr) & 2u) >> 1));
+          buf[buf_top].p_zs = (e.p_zs << 1) | ((uint32_t(curr) & 1u) << 16);
           buf_top++;
         }
         const uint32_t ni = nn[curr];
@@ -772,7 +807,7 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g,
       } while (curr < 8);
       // children replace the current entry, the first one on top; entries past the stack are dropped
       for (int i = 0; i < buf_top; i++)
-        if (top + i < OT_STACK) stack[top + i] = buf[buf_top - i - 1];
+        if (top + i < OT_STACK) stack.set(top + i, buf[buf_top - i - 1]);
       top += buf_top - 1;
       if (top >= OT_STACK) top = OT_STACK - 1;
     }

@@ -16,7 +16,7 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
                                               uint32_t instId, uint32_t geomId, bool stopOnFirstHit, CRT_Hit& hit,
                                               LaneStack<S, BLOCK>& st, uint32_t& overflow,
                                               RayCounts& cnt, const lds_pair* top_pairs, uint32_t budget,
-                                              uint32_t& steps, bool& aborted) {
+                                              uint32_t& steps, bool& aborted, lds_u32* ot_col = nullptr) {
   const uint32_t bvhOffset = s.geoms[geomId].bvhOffset;
   const GeomRef g = geom_ref(s, geomId);
   const BVHNodePair* nodes = s.nodes + bvhOffset;
@@ -45,7 +45,7 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
     } else if ((KINDS & LK_SBS) && tag == TAG_SDF_BRICK) {
       cnt.sdf_voxels += sbs_brick_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
     } else if ((KINDS & LK_MULTI) && tag == TAG_SDF_OCTREE) {
-      cnt.sdf_voxels += multi_octree_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, hit);
+      cnt.sdf_voxels += multi_octree_intersect<BLOCK>(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, hit, ot_col);
     }
     if (s.preset.render_mode == MULTI_RENDER_MODE_INTERSECTION_COUNT)
       hit.instId = (hit.instId == 0xFFFFFFFFu) ? 1 : hit.instId + 1;
@@ -154,7 +154,8 @@ __device__ __forceinline__ void finish_hit(const SceneView& s, CRT_Hit& hit) {
 // with hardware min/max (the caller checked box_test_fast for the world ray; each instance-space
 // ray is checked again before its BLAS walk). `keep` (LDS, 6 words per lane at keep[k * BLOCK],
 // may be null): the world ray is parked there and read back after each BLAS walk, so that its
-// registers are free while the BLAS is walked (the inverse direction is recomputed, bit for bit).
+// registers are free while the BLAS is walked (the inverse direction is recomputed, bit for bit);
+// kernels that walk multi-octrees extend it by 4 * OT_LDS words per lane for the walk's stack.
 template <uint32_t KINDS, int S, int BLOCK, bool COUNT, bool COOP, bool FAST = false>
 __device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 posAndNear, float4 dirAndFar,
                                                      LaneStack<S, BLOCK>& st, uint32_t& overflow, RayCounts& cnt,
@@ -222,13 +223,15 @@ __device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 
           __syncthreads();
         }
       } else {
+        lds_u32* ot_col = ((KINDS & LK_MULTI) && keep) ? keep + 6 * BLOCK : nullptr;
         if (FAST && box_test_fast(ray_pos, ray_dir))
           blas_traverse<KINDS, S, BLOCK, COUNT, true>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId, stopOnFirstHit,
-                                                      hit, st, overflow, cnt, top_pairs, budget, steps, aborted);
+                                                      hit, st, overflow, cnt, top_pairs, budget, steps, aborted,
+                                                      ot_col);
         else
           blas_traverse<KINDS, S, BLOCK, COUNT, false>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId,
                                                        stopOnFirstHit, hit, st, overflow, cnt, top_pairs, budget,
-                                                       steps, aborted);
+                                                       steps, aborted, ot_col);
         if (aborted) return hit;
       }
     }
