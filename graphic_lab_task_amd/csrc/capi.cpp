@@ -490,7 +490,7 @@ lrt_status lrt_sbs_param_layout(lrt_scene* s, uint32_t geom_id, uint32_t* f_offs
 lrt_status lrt_sbs_forward_dev(lrt_scene* s, uint32_t geom_id, const float* params, const float* pos_near,
                                const float* dir_far, uint64_t n, float* colour, void* stream) {
   return guarded([&]() -> lrt_status {
-    if (!colour) return fail(LRT_ERR_INVALID_ARG, "null colour buffer");
+    if (n && !colour) return fail(LRT_ERR_INVALID_ARG, "null colour buffer");
     return sbs_diff_common(s, geom_id, params, pos_near, dir_far, n, colour, nullptr, nullptr, nullptr, nullptr,
                            stream);
   });
@@ -500,7 +500,7 @@ lrt_status lrt_sbs_backward_dev(lrt_scene* s, uint32_t geom_id, const float* par
                                 const float* dir_far, uint64_t n, const float* dL_dcolour, float* dL_dparams,
                                 void* stream) {
   return guarded([&]() -> lrt_status {
-    if (!dL_dcolour || !dL_dparams) return fail(LRT_ERR_INVALID_ARG, "null gradient buffer");
+    if (n && (!dL_dcolour || !dL_dparams)) return fail(LRT_ERR_INVALID_ARG, "null gradient buffer");
     return sbs_diff_common(s, geom_id, params, pos_near, dir_far, n, nullptr, dL_dcolour, nullptr, dL_dparams,
                            nullptr, stream);
   });
@@ -510,7 +510,7 @@ lrt_status lrt_sbs_l2_backward_dev(lrt_scene* s, uint32_t geom_id, const float* 
                                    const float* dir_far, uint64_t n, const float* target, float* dL_dparams,
                                    float* loss, void* stream) {
   return guarded([&]() -> lrt_status {
-    if (!target || !dL_dparams) return fail(LRT_ERR_INVALID_ARG, "null target or gradient buffer");
+    if (n && (!target || !dL_dparams)) return fail(LRT_ERR_INVALID_ARG, "null target or gradient buffer");
     return sbs_diff_common(s, geom_id, params, pos_near, dir_far, n, nullptr, nullptr, target, dL_dparams, loss,
                            stream);
   });
@@ -520,7 +520,8 @@ lrt_status lrt_sbs_render_l2_backward_dev(lrt_scene* s, uint32_t geom_id, const 
                                           const float* dir_far, uint64_t n, const float* target, float* colour,
                                           float* dL_dparams, float* loss, void* stream) {
   return guarded([&]() -> lrt_status {
-    if (!target || !colour || !dL_dparams) return fail(LRT_ERR_INVALID_ARG, "null target, colour or gradient buffer");
+    if (n && (!target || !colour || !dL_dparams))
+      return fail(LRT_ERR_INVALID_ARG, "null target, colour or gradient buffer");
     return sbs_diff_common(s, geom_id, params, pos_near, dir_far, n, colour, nullptr, target, dL_dparams, loss,
                            stream);
   });
@@ -548,7 +549,7 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
       if (st != LRT_OK) return st;
       if (!sc.committed) return fail(LRT_ERR_NOT_COMMITTED, "batch item scene is not committed");
       if (sc.sbsRoots.size() != 1) return fail(LRT_ERR_UNSUPPORTED, "batch items need scenes with exactly one brick set");
-      if (!in.params || !in.grad || !in.pos_near || !in.dir_far || !in.target)
+      if (!in.params || !in.grad || (in.n_rays && (!in.pos_near || !in.dir_far || !in.target)))
         return fail(LRT_ERR_INVALID_ARG, "null buffer in batch item");
       if (in.n_rays > 0xFFFFFFFFull) return fail(LRT_ERR_INVALID_ARG, "too many rays in one item");
       lrt::SbsBatchItem it{};

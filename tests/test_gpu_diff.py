@@ -184,6 +184,34 @@ def test_batch_matches_per_scene_calls(lrt, torch_dev):
         assert it["scene"].sbs_recompute_misses() == 0
 
 
+def test_batch_ragged_items_and_fused_entry(lrt, torch_dev):
+    """Items whose ray counts are not multiples of the 64-ray chunk (and one empty item), against
+    per-scene calls of the fused render + L2 adjoint entry on the same rays."""
+    import bench
+    torch = torch_dev
+    _say("batch ragged items / fused entry")
+    items = bench.sbs_batch_scenes(torch, lrt, n_scenes=4, res=48, views=2, seed=6)
+    for it, n in zip(items, (4607, 1, 0, 3000)):
+        it["pos"], it["dir"] = it["pos"][:n].contiguous(), it["dir"][:n].contiguous()
+        it["target"], it["colour"] = it["target"][:n].contiguous(), it["colour"][:n].contiguous()
+    batch = lrt.SbsBatch(items)
+    batch.run()
+    torch.cuda.synchronize()
+    for it in items:
+        grad = torch.zeros_like(it["params"])
+        loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+        col = torch.zeros_like(it["colour"])
+        it["scene"].sbs_render_l2_backward(it["geom_id"], it["params"], it["pos"], it["dir"], it["target"], col, grad,
+                                           loss)
+        torch.cuda.synchronize()
+        assert torch.equal(col, it["colour"])
+        if it["pos"].shape[0] == 0:
+            assert float(grad.abs().sum()) == 0.0 and float(it["grad"].abs().sum()) == 0.0 and loss.item() == 0.0
+            continue
+        assert _rel(it["grad"], grad) < 1e-5 or float(grad.abs().sum()) == 0.0
+        assert abs(it["loss"].item() - loss.item()) <= 1e-5 * max(loss.item(), 1e-30)
+
+
 def test_batch_graph_replay(lrt, torch_dev):
     """A captured batch run replays correctly several times, with eager work in between."""
     import bench
