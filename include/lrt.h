@@ -173,8 +173,9 @@ lrt_status lrt_set_ray_interleave(lrt_scene* s, uint32_t lanes_per_group);
  * hit voxel at the hit point of the reference brick march (BVH2Common.cpp:884-1112); 0 on a
  * miss. `params` is a device array of the scene's whole values_f (lrt_get_array "sbs_data_f"
  * size) and replaces the committed copy for the call; gradients have the same size and
- * layout and are ACCUMULATED (zero them first). Ray buffers as in lrt_trace_closest_dev. The
- * backward passes recompute every hit (no state is kept between calls). */
+ * layout and are ACCUMULATED (zero them first). Ray buffers as in lrt_trace_closest_dev (n may be
+ * 0, and then the ray and output buffers may be NULL). The backward passes recompute every hit
+ * (no state is kept between calls). */
 lrt_status lrt_sbs_param_layout(lrt_scene* s, uint32_t geom_id, uint32_t* f_offset, uint32_t* n_dist,
                                 uint64_t* n_params);
 /* colour: float4 (r, g, b, hit 0/1) per ray. */
