@@ -1,6 +1,6 @@
 // Closest-hit / any-hit ray query kernels for gfx950 and the primary-ray generator.
 //
-// A trace call runs in two passes (plus a one-block reset):
+// A trace call runs in two passes:
 //  1. one ray per lane, the reference's sequential walk, with a step budget: the few rays
 //     that need more steps than the budget (grazing rays crossing many boxes) leave their
 //     index in a queue instead of holding their whole wave hostage;
@@ -85,7 +85,16 @@ __global__ __launch_bounds__(64) void k_trace_tail(SceneView s, const float4* __
     uint32_t idx = 0;
     if (__lane_id() == 0) idx = atomicAdd(&queue->next, 1u);
     idx = __shfl(idx, 0, 64);
-    if (idx >= total) break;
+    if (idx >= total) {
+      // Every wave ends on exactly one fetch past the queue, after its last ray; the fetch that
+      // returns total + waves - 1 is the last of all, so that wave clears the queue for the next
+      // call on this stream (no reset launch precedes pass 1).
+      if (__lane_id() == 0 && idx == total + gridDim.x - 1) {
+        queue->count = 0;
+        queue->next = 0;
+      }
+      break;
+    }
     const uint32_t i = queue->list[idx];
     float4 p = pos[i];
     float4 d = dir[i];
@@ -104,13 +113,6 @@ __global__ __launch_bounds__(64) void k_trace_tail(SceneView s, const float4* __
       if (overflow && overflow_count) atomicAdd(overflow_count, 1u);
     }
     __syncthreads();
-  }
-}
-
-__global__ void k_tail_reset(TailQueue* queue) {
-  if (threadIdx.x == 0) {
-    queue->count = 0;
-    queue->next = 0;
   }
 }
 
@@ -172,8 +174,8 @@ static hipError_t launch_trace_t(const SceneView& s, const TraceArgs& a, hipStre
       return hipGetLastError();
     }
   }
+  // the queue starts cleared (allocation) and every second pass leaves it cleared
   const bool tail = a.budget != 0 && a.queue != nullptr;
-  if (tail) hipLaunchKernelGGL(k_tail_reset, dim3(1), dim3(64), 0, stream, a.queue);
   hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, false>), dim3(grid), dim3(BLOCK), lds_cache, stream, s,
                      a.pos, a.dir, a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts, a.spill,
                      tail ? a.budget : 0u, a.queue, lanes_log2(a.ray_groups));
@@ -183,7 +185,10 @@ static hipError_t launch_trace_t(const SceneView& s, const TraceArgs& a, hipStre
     hipLaunchKernelGGL((k_trace_tail<KINDS & ~LK_TRI_POSTPONE>), dim3(waves), dim3(64), 0, stream, s, a.pos, a.dir,
                        a.out, a.any_out, a.overflow_count, a.queue);
   }
-  return hipGetLastError();
+  const hipError_t e = hipGetLastError();
+  // a call whose second pass did not launch must not leave queued rays to the next call
+  if (e != hipSuccess && tail) (void)hipMemsetAsync(a.queue, 0, 2 * sizeof(uint32_t), stream);
+  return e;
 }
 
 template <uint32_t KINDS>

@@ -17,7 +17,7 @@ import statistics
 import sys
 from collections import defaultdict
 
-TRACE_KERNELS = ("k_trace_closest", "k_trace_tail", "k_tail_reset")
+TRACE_KERNELS = ("k_trace_closest", "k_trace_tail")
 COPY_BYTES = (64 << 20) * 4              # bytes read (and written) by the 256 MiB float copy
 
 
