@@ -54,7 +54,7 @@ ARRAY_DTYPES = {
     "orig_nodes": NODE_DTYPE, "frame_nodes": FRAME_NODE_DTYPE, "frame_roots": np.uint32,
     "sbs_nodes": SBS_NODE_DTYPE, "sbs_data": np.uint32, "sbs_data_f": np.float32, "sbs_roots": np.uint32,
     "sbs_headers": SBS_HEADER_DTYPE, "multi_nodes": MULTI_NODE_DTYPE, "multi_values": np.float32,
-    "preset": PRESET_DTYPE,
+    "preset": PRESET_DTYPE, "sbs_mask": np.uint32,
 }
 
 # enums (core/render_settings.h, BVH2Common.h:30-43, sdf_simple_shared.h:13-21)
@@ -65,6 +65,7 @@ SDF_SBS_NODE_LAYOUT_DX = 1 << 24
 SDF_SBS_NODE_LAYOUT_DX_UV16 = 2 << 24
 SDF_SBS_NODE_LAYOUT_DX_RGB8 = 3 << 24
 SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F = 4 << 24
+SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN = 5 << 24
 SDF_OCTREE_NODE_INTERSECT_ST, SDF_OCTREE_NODE_INTERSECT_ANALYTIC = 0, 1
 SDF_OCTREE_NODE_INTERSECT_NEWTON, SDF_OCTREE_NODE_INTERSECT_BBOX, SDF_OCTREE_NODE_INTERSECT_IT = 2, 3, 4
 MULTI_RENDER_MODE_MASK, MULTI_RENDER_MODE_LAMBERT_NO_TEX = 0, 1

@@ -540,6 +540,7 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
     std::vector<size_t> mask_offset;
     size_t mask_words = 0;
     uint64_t max_zero4 = 1, max_bits = 1, max_bricks = 1;   // per-item extents of the set-up grids
+    size_t max_flag_bytes = 4;
     for (uint32_t k = 0; k < n_items; ++k) {
       const lrt_sbs_batch_item& in = items[k];
       if (!in.scene) return fail(LRT_ERR_INVALID_ARG, "null scene in batch item");
@@ -578,6 +579,7 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
       max_zero4 = std::max<uint64_t>(max_zero4, (it.n_params + 3) / 4);
       max_bits = std::max<uint64_t>(max_bits, uint64_t(it.n_bricks) * wpb * 32u);
       max_bricks = std::max<uint64_t>(max_bricks, it.n_bricks);
+      max_flag_bytes = std::max<size_t>(max_flag_bytes, size_t(lrt::sbs_mask_flag_words(it.brick_size)) * 4u);
       zero_begin.push_back(zero_begin.back() + it.n_params);
       mask_begin.push_back(mask_begin.back() + it.n_bricks * wpb);
       brick_begin.push_back(brick_begin.back() + it.n_bricks);
@@ -626,6 +628,8 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
     L.zero_blocks = blocks(max_zero4, 1024);
     L.mask_blocks = blocks(max_bits, 4096);
     L.summary_blocks = blocks(max_bricks, 256);
+    L.brick_blocks = uint32_t(std::min<uint64_t>(max_bricks, 4096));
+    L.mask_flag_bytes = max_flag_bytes <= lrt::SBS_MASK_LDS_MAX ? uint32_t(max_flag_bytes) : 0u;
     *out = b.release();
     return LRT_OK;
   });
@@ -703,11 +707,25 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
     ARR("multi_nodes", c.multiNodes) ARR("multi_values", c.multiValues)
 #undef ARR
     if (n == "preset") { src = &c.preset; sz = sizeof(c.preset); }
+    std::vector<uint32_t> mask_copy;   // the committed occupancy masks live on the device only
+    if (n == "sbs_mask") {
+      if (!c.committed) return fail(LRT_ERR_NOT_COMMITTED, "scene is not committed");
+      mask_copy.resize(c.sbsMaskWords);
+      if (c.sbsMaskWords) {
+        auto it = c.dev.find("sbs_mask");
+        if (it == c.dev.end()) return fail(LRT_ERR_HIP, "no device mask");
+        const hipError_t e = hipMemcpy(mask_copy.data(), it->second.ptr(), mask_copy.size() * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "mask copy");
+      }
+      src = mask_copy.data();
+      sz = mask_copy.size() * 4;
+    }
     if (!src && sz == 0) {
       static const char* known[] = {"nodes", "tlas", "instances", "geoms", "geom_tags", "start_end",
                                     "prim_id_count", "vert_pos", "vert_norm", "indices", "prim_indices",
                                     "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes", "sbs_data",
-                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset"};
+                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset",
+                                    "sbs_mask"};
       bool ok = false;
       for (const char* k : known) ok = ok || n == k;
       if (!ok) return fail(LRT_ERR_INVALID_ARG, "unknown array '" + n + "'");

@@ -198,6 +198,18 @@ __global__ void k_batch_mask_words(SbsBatchLaunch L) {
   }
 }
 
+// One wave per brick of each item (blockIdx.y), as k_sbs_mask_bricks.
+__global__ __launch_bounds__(64) void k_batch_mask_bricks(SbsBatchLaunch L) {
+  extern __shared__ uint32_t flags[];
+  const SbsBatchItem& item = L.items[blockIdx.y];
+  const SceneView& s = item.view;
+  const SdfSBSHeader h = s.sbs_headers[0];
+  const uint32_t bs = h.brick_size, wpb = (bs * bs * bs + 31u) / 32u;
+  const uint32_t root = s.sbs_roots[0];
+  for (uint32_t b = blockIdx.x; b < item.n_bricks; b += gridDim.x)
+    sbs_mask_brick(s, h, root, b, item.mask + size_t(b) * (1u + wpb), flags);
+}
+
 __global__ void k_batch_mask_summary(SbsBatchLaunch L) {
   const SbsBatchItem& item = L.items[blockIdx.y];
   const uint32_t bs = item.brick_size, wpb = (bs * bs * bs + 31u) / 32u;
@@ -264,10 +276,12 @@ hipError_t launch_sbs_batch(const SbsBatchLaunch& L, hipStream_t stream) {
   if (L.n_items == 0) return hipSuccess;
   // x extents sized for the largest item (smaller items leave blocks idle)
   hipLaunchKernelGGL(k_batch_zero, dim3(L.zero_blocks, L.n_items), dim3(256), 0, stream, L);
-  if (L.total_mask_words)
+  if (L.total_bricks && L.mask_flag_bytes) {
+    hipLaunchKernelGGL(k_batch_mask_bricks, dim3(L.brick_blocks, L.n_items), dim3(64), L.mask_flag_bytes, stream, L);
+  } else if (L.total_bricks) {   // very large bricks: one thread per voxel bit, then the summary words
     hipLaunchKernelGGL(k_batch_mask_words, dim3(L.mask_blocks, L.n_items), dim3(256), 0, stream, L);
-  if (L.total_bricks)
     hipLaunchKernelGGL(k_batch_mask_summary, dim3(L.summary_blocks, L.n_items), dim3(256), 0, stream, L);
+  }
   hipLaunchKernelGGL((k_sbs_batch<16, SBS_BATCH_BLOCK>), dim3(L.grid), dim3(SBS_BATCH_BLOCK), 0, stream, L);
   return hipGetLastError();
 }

@@ -67,6 +67,14 @@ struct SbsDiffArgs {
 struct SbsMaskGeom {
   uint32_t sdf_id, n_bricks, brick_size, base;
 };
+// LDS bound of the one-wave-per-brick mask build (lattice flag bits; bricks up to 24^3 voxels)
+constexpr size_t SBS_MASK_LDS_MAX = 2048;
+// LDS words that build needs for bricks of size bs: one flag bit per lattice point, rounded up
+// to whole waves.
+inline uint32_t sbs_mask_flag_words(uint32_t bs) {
+  const uint32_t np = (bs + 1) * (bs + 1) * (bs + 1);
+  return (np + 63u) / 64u * 2u;
+}
 // Builds the occupancy masks of the listed brick sets from the view's current values.
 hipError_t launch_sbs_masks(const SceneView& s, const SbsMaskGeom* geoms, uint32_t n_geoms, uint32_t* mask,
                             hipStream_t stream);
@@ -94,7 +102,8 @@ struct SbsBatchLaunch {
   const uint32_t* chunk_begin;   // [n_items + 1] prefix sums of ray chunks (SBS_BATCH_CHUNK rays)
   uint64_t total_zero;
   uint32_t total_mask_words, total_bricks, total_chunks;
-  uint32_t zero_blocks, mask_blocks, summary_blocks;   // x extents of the per-item set-up grids
+  uint32_t zero_blocks, mask_blocks, summary_blocks, brick_blocks;   // x extents of the set-up grids
+  uint32_t mask_flag_bytes;      // LDS of the one-wave-per-brick mask build (0: bricks too large)
   uint32_t* counter;             // work counter (reset by the zero kernel)
   uint32_t* spill;               // grid * BLOCK * (STACK_SIZE - 16) words
   uint32_t grid;                 // persistent blocks

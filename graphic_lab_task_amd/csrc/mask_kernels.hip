@@ -2,6 +2,8 @@
 // voxel, set when the minimum of the voxel's 8 corner values is <= 0 — exactly the voxels where
 // the brick march (BVH2Common.cpp:884-1112) calls LocalSurfaceIntersection. The march then skips
 // the value loads of empty voxels and whole empty bricks; its results do not change.
+#include <algorithm>
+
 #include "kernels.h"
 #include "trace_device.h"
 
@@ -24,6 +26,17 @@ __global__ void k_sbs_mask_words(SceneView s, uint32_t sdf_id, uint32_t n_bricks
   if (in && (threadIdx.x & 31u) == 0u) mask[size_t(brick) * (1u + wpb) + 1u + w] = uint32_t(m >> (threadIdx.x & 32u));
 }
 
+// One wave per brick (grid-strided): sbs_mask_brick writes the brick's bits and summary word.
+__global__ __launch_bounds__(64) void k_sbs_mask_bricks(SceneView s, uint32_t sdf_id, uint32_t n_bricks,
+                                                        uint32_t* __restrict__ mask) {
+  extern __shared__ uint32_t flags[];
+  const SdfSBSHeader h = s.sbs_headers[sdf_id];
+  const uint32_t bs = h.brick_size, wpb = (bs * bs * bs + 31u) / 32u;
+  const uint32_t root = s.sbs_roots[sdf_id];
+  for (uint32_t b = blockIdx.x; b < n_bricks; b += gridDim.x)
+    sbs_mask_brick(s, h, root, b, mask + size_t(b) * (1u + wpb), flags);
+}
+
 __global__ void k_sbs_mask_summary(uint32_t n_bricks, uint32_t wpb, uint32_t* __restrict__ mask) {
   const uint32_t brick = blockIdx.x * blockDim.x + threadIdx.x;
   if (brick >= n_bricks) return;
@@ -41,6 +54,13 @@ hipError_t launch_sbs_masks(const SceneView& s, const SbsMaskGeom* geoms, uint32
     const SbsMaskGeom& m = geoms[g];
     if (m.n_bricks == 0) continue;
     const uint32_t wpb = (m.brick_size * m.brick_size * m.brick_size + 31u) / 32u;
+    const size_t flag_bytes = size_t(sbs_mask_flag_words(m.brick_size)) * 4u;
+    if (flag_bytes <= SBS_MASK_LDS_MAX) {
+      hipLaunchKernelGGL(k_sbs_mask_bricks, dim3(std::min<uint32_t>(m.n_bricks, 16384u)), dim3(64), flag_bytes, stream,
+                         s, m.sdf_id, m.n_bricks, mask + m.base);
+      continue;
+    }
+    // very large bricks: one thread per voxel bit, then the summary words
     const uint64_t bits = uint64_t(m.n_bricks) * wpb * 32u;
     hipLaunchKernelGGL(k_sbs_mask_words, dim3(uint32_t((bits + 255) / 256)), dim3(256), 0, stream, s, m.sdf_id,
                        m.n_bricks, mask + m.base);

@@ -457,6 +457,65 @@ __device__ __forceinline__ float sbs_load_values(const SceneView& s, const SdfSB
   return vmin;
 }
 
+// Value of lattice point (x, y, z) (corner coordinates 0..bs) of a brick, decoded exactly as
+// sbs_load_values decodes it.
+__device__ __forceinline__ float sbs_lattice_value(const SceneView& s, const SdfSBSHeader& h, uint32_t v_off,
+                                                   uint32_t v_size, float sz_inv, uint32_t x, uint32_t y, uint32_t z) {
+  const uint32_t vId = (x + h.brick_pad) * v_size * v_size + (y + h.brick_pad) * v_size + (z + h.brick_pad);
+  if (h.aux_data == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F || h.aux_data == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN)
+    return s.sbs_data_f[s.sbs_data[v_off + vId]];
+  const uint32_t vals_per_int = 4 / h.bytes_per_value;
+  const uint32_t bits = 8 * h.bytes_per_value;
+  const uint32_t max_val = h.bytes_per_value == 4 ? 0xFFFFFFFFu : ((1u << bits) - 1);
+  const float d_max = 1.73205081f * sz_inv;
+  const float mult = 2 * d_max / float(max_val);
+  const uint32_t word = s.sbs_data[v_off + vId / vals_per_int];
+  const uint32_t sh = bits * (vId % vals_per_int);
+  return -d_max + mult * float((sh < 32 ? (word >> sh) : 0u) & max_val);
+}
+
+// Occupancy mask of one brick, built by one wave (the block is one wave): each of the brick's
+// (bs+1)^3 lattice values is decoded once (not once per voxel corner) and its "<= 0" flag kept as
+// a bit in LDS; a voxel's bit is the OR of its 8 corner flags, which is the predicate "minimum of
+// the 8 corner values <= 0" (that minimum skips NaN values, as the comparison does). Writes the
+// summary word and the bs^3 bits to out[0 .. wpb].
+__device__ __forceinline__ void sbs_mask_brick(const SceneView& s, const SdfSBSHeader& h, uint32_t root, uint32_t brick,
+                                               uint32_t* __restrict__ out, uint32_t* flags) {
+  const uint32_t bs = h.brick_size, n1 = bs + 1, np = n1 * n1 * n1, v3 = bs * bs * bs, wpb = (v3 + 31u) / 32u;
+  const uint32_t node_id = brick + root;
+  const uint32_t v_size = bs + 2 * h.brick_pad + 1;
+  const float sz_inv = 2.0f / float(s.sbs_nodes[node_id].pos_z_lod_size & 0x0000FFFF);
+  const uint32_t v_off = s.sbs_nodes[node_id].data_offset;
+  const uint32_t lane = threadIdx.x;
+  for (uint32_t base = 0; base < np; base += 64u) {
+    const uint32_t q = base + lane;
+    bool f = false;
+    if (q < np) f = sbs_lattice_value(s, h, v_off, v_size, sz_inv, q / (n1 * n1), (q / n1) % n1, q % n1) <= 0.0f;
+    const uint64_t m = __ballot(f);
+    if ((lane & 31u) == 0u) flags[base / 32u + (lane >> 5)] = uint32_t(m >> (lane & 32u));
+  }
+  __syncthreads();
+  bool any = false;
+  for (uint32_t base = 0; base < v3; base += 64u) {
+    const uint32_t v = base + lane;
+    bool bit = false;
+    if (v < v3) {
+      const uint32_t vx = v / (bs * bs), vy = (v / bs) % bs, vz = v % bs;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t q = (vx + ((i >> 2) & 1)) * n1 * n1 + (vy + ((i >> 1) & 1)) * n1 + (vz + (i & 1));
+        bit = bit || ((flags[q >> 5] >> (q & 31u)) & 1u) != 0u;
+      }
+    }
+    const uint64_t m = __ballot(bit);
+    any = any || m != 0;
+    const uint32_t w = base / 32u + (lane >> 5);
+    if ((lane & 31u) == 0u && w < wpb) out[1 + w] = uint32_t(m >> (lane & 32u));
+  }
+  if (lane == 0) out[0] = any ? 1u : 0u;
+  __syncthreads();   // the flags are reused for the next brick
+}
+
 // Occupancy bit of voxel v (x-major, v < bs^3) of brick `brick`: the minimum of its 8 corner
 // values, loaded as the march loads them, is <= 0.
 __device__ __forceinline__ bool sbs_voxel_occupied(const SceneView& s, const SdfSBSHeader& h, uint32_t root,

@@ -251,7 +251,10 @@ lrt_status lrt_stack_overflow_count(lrt_scene* s, uint64_t* out, int reset);
  * "nodes" (BVHNodePair), "tlas" (BVHNode), "instances" (InstanceData), "geoms" (GeomData),
  * "geom_tags", "start_end" (uint2), "prim_id_count", "vert_pos" (float4), "vert_norm" (float4),
  * "indices", "prim_indices", "orig_nodes" (BVHNode), "frame_nodes" (SdfFrameOctreeNode),
- * "frame_roots", "sbs_nodes", "sbs_data", "sbs_data_f", "sbs_roots", "sbs_headers".
+ * "frame_roots", "sbs_nodes", "sbs_data", "sbs_data_f", "sbs_roots", "sbs_headers",
+ * "multi_nodes", "multi_values", "preset", and "sbs_mask" (uint32: the committed voxel occupancy
+ * masks, per brick set in order and per brick one summary word then ceil(bs^3/32) bit words,
+ * bit v set when the minimum of voxel v's 8 corner values is <= 0; copied from the device).
  * With dst == NULL only *bytes is written. */
 lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* bytes);
 

@@ -369,6 +369,61 @@ def test_sdf_brick_set_parity(lrt, orc, torch_dev, kind, depth, bs, bpv):
         assert np.array_equal(got["coords"][same_t, :2], want["coords"][same_t, :2])
 
 
+def _mask_from_definition(lrt, s):
+    """Occupancy masks of a committed brick-set scene from their definition, in float32 numpy:
+    each lattice value decoded as the march decodes it (indexed: values_f[data[...]]; quantised:
+    -d_max + (2 d_max / max_val) q), voxel bit = any of its 8 corner values <= 0."""
+    nodes, data = s.get_array("sbs_nodes"), s.get_array("sbs_data")
+    vf, roots, heads = s.get_array("sbs_data_f"), s.get_array("sbs_roots"), s.get_array("sbs_headers")
+    out = []
+    for k in range(roots.size):
+        h = heads[k]
+        bs, pad, bpv, aux = int(h["brick_size"]), int(h["brick_pad"]), int(h["bytes_per_value"]), int(h["aux_data"])
+        n_bricks = (roots[k + 1] if k + 1 < roots.size else nodes.size) - roots[k]
+        vs, n1 = bs + 2 * pad + 1, bs + 1
+        g = np.arange(n1)
+        x, y, z = np.meshgrid(g, g, g, indexing="ij")
+        vid = ((x + pad) * vs * vs + (y + pad) * vs + (z + pad)).reshape(-1).astype(np.int64)
+        wpb = (bs ** 3 + 31) // 32
+        for b in range(int(n_bricks)):
+            nd = nodes[roots[k] + b]
+            off = int(nd["data_offset"])
+            if aux in (lrt.SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F, lrt.SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN):
+                val = vf[data[off + vid]]
+            else:
+                vpi, bits = 4 // bpv, 8 * bpv
+                max_val = 0xFFFFFFFF if bpv == 4 else (1 << bits) - 1
+                sz_inv = np.float32(2.0) / np.float32(int(nd["pos_z_lod_size"]) & 0xFFFF)
+                d_max = np.float32(1.73205081) * sz_inv
+                mult = np.float32(2.0) * d_max / np.float32(max_val)
+                word = data[off + vid // vpi].astype(np.uint64)
+                q = ((word >> (bits * (vid % vpi)).astype(np.uint64)) & max_val).astype(np.float32)
+                val = -d_max + mult * q
+            f = (val <= 0).reshape(n1, n1, n1)
+            occ = np.zeros((bs, bs, bs), bool)
+            for dx in (0, 1):
+                for dy in (0, 1):
+                    for dz in (0, 1):
+                        occ |= f[dx:dx + bs, dy:dy + bs, dz:dz + bs]
+            bits_ = np.zeros(wpb * 32, np.uint64)
+            bits_[:bs ** 3] = occ.reshape(-1)
+            words = (bits_.reshape(wpb, 32) << np.arange(32, dtype=np.uint64)).sum(1).astype(np.uint32)
+            out.append(np.concatenate([[np.uint32(words.any())], words]))
+    return np.concatenate(out) if out else np.zeros(0, np.uint32)
+
+
+@pytest.mark.parametrize("kind,depth,bs,bpv", [("indexed", 5, 8, 4), ("dx", 4, 8, 1), ("dx", 4, 4, 2),
+                                               ("rgb8", 4, 4, 1), ("indexed", 7, 1, 4)])
+def test_sbs_occupancy_masks_match_definition(lrt, torch_dev, kind, depth, bs, bpv):
+    """The device-built occupancy masks, word for word, against their definition."""
+    _say(f"sbs masks {kind} depth={depth} brick={bs} bytes={bpv}")
+    s = _sbs_scene(lrt, kind, depth, bs, bpv)
+    got = s.get_array("sbs_mask")
+    want = _mask_from_definition(lrt, s)
+    assert got.size == want.size and np.array_equal(got, want), f"{np.sum(got != want)} of {got.size} words differ"
+    assert want[0::(bs ** 3 + 31) // 32 + 1].any()
+
+
 def test_autotune_keeps_results(lrt, orc, torch_dev):
     _say("autotune")
     v, f, s = mesh_scene(lrt, 5, 0.05)
