@@ -47,15 +47,26 @@ __device__ bool recompute_hit_voxel(const SceneView& s, uint32_t diff_geom, floa
 
 // MODE 0 forward, 1 backward with dL/dC, 2 backward of the L2 loss against a target, 3 both the
 // forward colour and the L2 backward. Returns the ray's loss term |C - T|^2 (modes 2, 3).
+// `rec`: the voxel record the traversal's own brick march left (t_local NaN if it recorded none);
+// without one the hit brick is marched once more.
 template <int MODE>
-__device__ __forceinline__ float diff_ray(const SceneView& s, const SbsDiffArgs& a, uint32_t i, const CRT_Hit& h) {
+__device__ __forceinline__ float diff_ray(const SceneView& s, const SbsDiffArgs& a, uint32_t i, const CRT_Hit& h,
+                                          SbsVoxelRec& rec) {
   const float4 p = a.pos[i];
   const float4 d = a.dir[i];
-  SbsVoxelRec rec;
-  float3 dir_l;
-  const bool hit = recompute_hit_voxel(s, a.geom_id, p, d, h, rec, dir_l);
-  if (h.geomId != 0xFFFFFFFFu && (h.geomId & GEOM_ID_MASK) == a.geom_id && !hit && a.counters)
-    atomicAdd(a.counters, 1u);
+  float3 dir_l = make3(0, 0, 0);
+  bool hit = false;
+  if (h.geomId != 0xFFFFFFFFu && (h.geomId & GEOM_ID_MASK) == a.geom_id) {
+    if (rec.t_local == rec.t_local) {   // recorded by the traversal
+      float m[16];
+      load_mat(s.instances[h.instId].transformInv, m);
+      dir_l = matmul3x3(m, xyz(d));
+      hit = true;
+    } else {
+      hit = recompute_hit_voxel(s, a.geom_id, p, d, h, rec, dir_l);
+      if (!hit && a.counters) atomicAdd(a.counters, 1u);
+    }
+  }
   float3 C = make3(0, 0, 0);
   float w[8];
   float3 gw[8];
@@ -133,10 +144,12 @@ __global__ __launch_bounds__(BLOCK) void k_sbs_diff(SceneView s, SbsDiffArgs a) 
     uint32_t overflow = 0;
     RayCounts cnt{};
     bool aborted = false;
+    SbsVoxelRec rec;
+    rec.t_local = __int_as_float(0x7fc00000);
     const CRT_Hit h = ray_query_nearest<LK_SBS, S, BLOCK, false, false>(s, a.pos[i], a.dir[i], st, overflow, cnt,
                                                                         (const lds_pair*)top_pairs, 0u, aborted,
-                                                                        nullptr);
-    loss = diff_ray<MODE>(s, a, i, h);
+                                                                        nullptr, nullptr, &rec);
+    loss = diff_ray<MODE>(s, a, i, h, rec);
   }
   if (MODE >= 2 && a.loss) {   // one atomic per wave instead of one per ray
     loss = wave_sum(loss);
@@ -248,10 +261,14 @@ __global__ __launch_bounds__(BLOCK) void k_sbs_batch(SbsBatchLaunch L) {
       uint32_t overflow = 0;
       RayCounts cnt{};
       bool aborted = false;
+      // the hit voxel is found by a second march of the hit brick: recording it during the walk
+      // would hold 12 more registers through the walk, costing this kernel a wave per SIMD
+      SbsVoxelRec rec;
+      rec.t_local = __int_as_float(0x7fc00000);
       const CRT_Hit h = ray_query_nearest<LK_SBS, S, BLOCK, false, false>(item.view, item.args.pos[i], item.args.dir[i],
                                                                           st, overflow, cnt, nullptr, 0u, aborted,
                                                                           nullptr);
-      loss = diff_ray<3>(item.view, item.args, i, h);
+      loss = diff_ray<3>(item.view, item.args, i, h, rec);
     }
     loss = wave_sum(loss);
     if (__lane_id() == 0 && item.args.loss) atomicAdd(item.args.loss, loss);

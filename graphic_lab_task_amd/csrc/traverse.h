@@ -16,7 +16,8 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
                                               uint32_t instId, uint32_t geomId, bool stopOnFirstHit, CRT_Hit& hit,
                                               LaneStack<S, BLOCK>& st, uint32_t& overflow,
                                               RayCounts& cnt, const lds_pair* top_pairs, uint32_t budget,
-                                              uint32_t& steps, bool& aborted, lds_u32* ot_col = nullptr) {
+                                              uint32_t& steps, bool& aborted, lds_u32* ot_col = nullptr,
+                                              SbsVoxelRec* rec = nullptr) {
   const uint32_t bvhOffset = s.geoms[geomId].bvhOffset;
   const GeomRef g = geom_ref(s, geomId);
   const BVHNodePair* nodes = s.nodes + bvhOffset;
@@ -43,7 +44,7 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
     } else if ((KINDS & LK_FRAME) && tag == TAG_SDF_NODE) {
       frame_node_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
     } else if ((KINDS & LK_SBS) && tag == TAG_SDF_BRICK) {
-      cnt.sdf_voxels += sbs_brick_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
+      cnt.sdf_voxels += sbs_brick_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit, rec);
     } else if ((KINDS & LK_MULTI) && tag == TAG_SDF_OCTREE) {
       cnt.sdf_voxels += multi_octree_intersect<BLOCK>(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, hit, ot_col);
     }
@@ -156,11 +157,15 @@ __device__ __forceinline__ void finish_hit(const SceneView& s, CRT_Hit& hit) {
 // may be null): the world ray is parked there and read back after each BLAS walk, so that its
 // registers are free while the BLAS is walked (the inverse direction is recomputed, bit for bit);
 // kernels that walk multi-octrees extend it by 4 * OT_LDS words per lane for the walk's stack.
+// `rec` (brick-set kernels, may be null): the brick march records the voxel of every surface hit
+// it makes; an SDF walk ends at its first hitting leaf and a later hit is always closer, so after
+// the walk it holds the voxel of the final hit when that hit is a brick-set hit.
 template <uint32_t KINDS, int S, int BLOCK, bool COUNT, bool COOP, bool FAST = false>
 __device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 posAndNear, float4 dirAndFar,
                                                      LaneStack<S, BLOCK>& st, uint32_t& overflow, RayCounts& cnt,
                                                      const lds_pair* top_pairs, uint32_t budget, bool& aborted,
-                                                     lds_entry* coop_stack, lds_u32* keep = nullptr) {
+                                                     lds_entry* coop_stack, lds_u32* keep = nullptr,
+                                                     SbsVoxelRec* rec = nullptr) {
   bool stopOnFirstHit = (dirAndFar.w <= 0.0f);
   if (stopOnFirstHit) dirAndFar.w *= -1.0f;
   if (keep) {
@@ -227,11 +232,11 @@ __device__ __forceinline__ CRT_Hit ray_query_nearest(const SceneView& s, float4 
         if (FAST && box_test_fast(ray_pos, ray_dir))
           blas_traverse<KINDS, S, BLOCK, COUNT, true>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId, stopOnFirstHit,
                                                       hit, st, overflow, cnt, top_pairs, budget, steps, aborted,
-                                                      ot_col);
+                                                      ot_col, rec);
         else
           blas_traverse<KINDS, S, BLOCK, COUNT, false>(s, ray_pos, ray_dir, posAndNear.w, instId, geomId,
                                                        stopOnFirstHit, hit, st, overflow, cnt, top_pairs, budget,
-                                                       steps, aborted, ot_col);
+                                                       steps, aborted, ot_col, rec);
         if (aborted) return hit;
       }
     }
