@@ -535,8 +535,8 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
     auto b = std::make_unique<lrt_sbs_batch>();
     hipError_t e = hipGetDevice(&b->device);
     if (e != hipSuccess) return hip_fail(e, "no device");
-    std::vector<uint64_t> zero_begin{0};
-    std::vector<uint32_t> mask_begin{0}, brick_begin{0}, chunk_begin{0};
+    std::vector<uint32_t> chunk_begin{0};
+    uint64_t total_bricks = 0;
     std::vector<size_t> mask_offset;
     size_t mask_words = 0;
     uint64_t max_zero4 = 1, max_bits = 1, max_bricks = 1;   // per-item extents of the set-up grids
@@ -580,9 +580,7 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
       max_bits = std::max<uint64_t>(max_bits, uint64_t(it.n_bricks) * wpb * 32u);
       max_bricks = std::max<uint64_t>(max_bricks, it.n_bricks);
       max_flag_bytes = std::max<size_t>(max_flag_bytes, size_t(lrt::sbs_mask_flag_words(it.brick_size)) * 4u);
-      zero_begin.push_back(zero_begin.back() + it.n_params);
-      mask_begin.push_back(mask_begin.back() + it.n_bricks * wpb);
-      brick_begin.push_back(brick_begin.back() + it.n_bricks);
+      total_bricks += it.n_bricks;
       chunk_begin.push_back(chunk_begin.back() + (it.args.n + lrt::SBS_BATCH_CHUNK - 1) / lrt::SBS_BATCH_CHUNK);
       b->items.push_back(it);
     }
@@ -600,26 +598,17 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
     const size_t n1 = size_t(n_items) + 1;
     if ((e = b->d_items.upload(b->items.data(), b->items.size() * sizeof(lrt::SbsBatchItem))) != hipSuccess)
       return hip_fail(e, "batch table");
-    std::vector<uint8_t> begins(n1 * (8 + 4 + 4 + 4));
-    std::memcpy(begins.data(), zero_begin.data(), n1 * 8);
-    std::memcpy(begins.data() + n1 * 8, mask_begin.data(), n1 * 4);
-    std::memcpy(begins.data() + n1 * 12, brick_begin.data(), n1 * 4);
-    std::memcpy(begins.data() + n1 * 16, chunk_begin.data(), n1 * 4);
-    if ((e = b->d_begins.upload(begins.data(), begins.size())) != hipSuccess) return hip_fail(e, "batch table");
+    if ((e = b->d_begins.upload(chunk_begin.data(), n1 * sizeof(uint32_t))) != hipSuccess)
+      return hip_fail(e, "batch table");
     if ((e = b->d_spill.reserve(size_t(grid) * lrt::SBS_BATCH_BLOCK * (lrt::STACK_SIZE - 16) * 4)) != hipSuccess)
       return hip_fail(e, "batch spill");
     if ((e = b->d_counter.reserve(256)) != hipSuccess) return hip_fail(e, "batch counter");
-    const uint8_t* base = static_cast<const uint8_t*>(b->d_begins.ptr());
     lrt::SbsBatchLaunch& L = b->L;
     L.items = static_cast<const lrt::SbsBatchItem*>(b->d_items.ptr());
     L.n_items = n_items;
-    L.zero_begin = reinterpret_cast<const uint64_t*>(base);
-    L.mask_begin = reinterpret_cast<const uint32_t*>(base + n1 * 8);
-    L.brick_begin = reinterpret_cast<const uint32_t*>(base + n1 * 12);
-    L.chunk_begin = reinterpret_cast<const uint32_t*>(base + n1 * 16);
-    L.total_zero = zero_begin.back();
-    L.total_mask_words = mask_begin.back();
-    L.total_bricks = brick_begin.back();
+    L.chunk_begin = static_cast<const uint32_t*>(b->d_begins.ptr());
+    if (total_bricks > 0xFFFFFFFFull) return fail(LRT_ERR_INVALID_ARG, "too many bricks in one batch");
+    L.total_bricks = uint32_t(total_bricks);
     L.total_chunks = chunk_begin.back();
     L.counter = static_cast<uint32_t*>(b->d_counter.ptr());
     L.spill = static_cast<uint32_t*>(b->d_spill.ptr());

@@ -90,18 +90,15 @@ struct SbsBatchItem {
 };
 
 // A batched run: zero gradients and losses, build every item's occupancy mask from its
-// parameters, then one persistent kernel whose blocks take BLOCK-ray chunks from a global
-// counter (items in order) and run the forward colour and the L2 adjoint of each ray. Only
-// kernel launches (no allocation, copy or memset), so a run can be captured in a graph.
+// parameters (set-up kernels on 2D grids, blockIdx.y = item), then one persistent kernel whose
+// one-wave blocks take SBS_BATCH_CHUNK-ray chunks from a global counter (items in order) and run
+// the forward colour and the L2 adjoint of each ray. Only kernel launches (no allocation, copy
+// or memset), so a run can be captured in a graph.
 struct SbsBatchLaunch {
   const SbsBatchItem* items;
   uint32_t n_items;
-  const uint64_t* zero_begin;    // [n_items + 1] prefix sums of n_params
-  const uint32_t* mask_begin;    // [n_items + 1] prefix sums of mask words (bit words only)
-  const uint32_t* brick_begin;   // [n_items + 1] prefix sums of n_bricks
   const uint32_t* chunk_begin;   // [n_items + 1] prefix sums of ray chunks (SBS_BATCH_CHUNK rays)
-  uint64_t total_zero;
-  uint32_t total_mask_words, total_bricks, total_chunks;
+  uint32_t total_bricks, total_chunks;
   uint32_t zero_blocks, mask_blocks, summary_blocks, brick_blocks;   // x extents of the set-up grids
   uint32_t mask_flag_bytes;      // LDS of the one-wave-per-brick mask build (0: bricks too large)
   uint32_t* counter;             // work counter (reset by the zero kernel)
