@@ -107,6 +107,35 @@ def cmd_traffic(a):
     print("\n".join(lines))
 
 
+def cmd_counters(a):
+    """Median per-dispatch value of every counter found in the given runs, for one kernel, plus
+    the ratios that say what bounds it."""
+    vals = {}
+    for d in a.dirs:
+        names = sorted({_col(r, "Counter_Name") for r in _rows(d, "counter_collection.csv")})
+        for c in names:
+            per = _counter_per_dispatch(d, c)
+            v = [x for k, xs in per.items() if k.startswith(a.kernel) and "true" not in k for x in xs]
+            if v:
+                vals[c] = statistics.median(v)
+    lines = [f"# rocprofv3 counters, {a.kernel} (median per dispatch)", f"command: {a.command}", ""]
+    lines += [f"{c:32s} {v:18.1f}" for c, v in sorted(vals.items())]
+    derived = []
+    if "TCC_HIT_sum" in vals and "TCC_MISS_sum" in vals:
+        derived.append(("L2 hit rate", vals["TCC_HIT_sum"] / max(1.0, vals["TCC_HIT_sum"] + vals["TCC_MISS_sum"])))
+    if "SQ_ACTIVE_INST_VALU" in vals and "SQ_BUSY_CYCLES" in vals:
+        derived.append(("VALU issue cycles / busy cycles (per SQ)",
+                        vals["SQ_ACTIVE_INST_VALU"] / max(1.0, vals["SQ_BUSY_CYCLES"])))
+    if "SQ_WAIT_INST_ANY" in vals and "SQ_WAVE_CYCLES" in vals:
+        derived.append(("wave cycles waiting on any instruction",
+                        vals["SQ_WAIT_INST_ANY"] / max(1.0, vals["SQ_WAVE_CYCLES"])))
+    if "TA_BUSY_avr" in vals and "GRBM_GUI_ACTIVE" in vals:
+        derived.append(("TA busy / GPU active", vals["TA_BUSY_avr"] / max(1.0, vals["GRBM_GUI_ACTIVE"])))
+    if derived:
+        lines += ["", "derived:"] + [f"{k:44s} {v:8.3f}" for k, v in derived]
+    print("\n".join(lines))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -119,8 +148,12 @@ def main(argv=None):
     t.add_argument("--workload", required=True)
     t.add_argument("--command", required=True)
     t.add_argument("--kernels", default="", help="comma-separated kernel name prefixes (default: the trace kernels)")
+    c = sub.add_parser("counters")
+    c.add_argument("dirs", nargs="+")
+    c.add_argument("--kernel", required=True, help="kernel name prefix")
+    c.add_argument("--command", required=True)
     a = ap.parse_args(argv)
-    {"stats": cmd_stats, "traffic": cmd_traffic}[a.cmd](a)
+    {"stats": cmd_stats, "traffic": cmd_traffic, "counters": cmd_counters}[a.cmd](a)
 
 
 if __name__ == "__main__":
