@@ -19,6 +19,7 @@ from collections import defaultdict
 
 TRACE_KERNELS = ("k_trace_closest", "k_trace_tail")
 COPY_BYTES = (64 << 20) * 4              # bytes read (and written) by the 256 MiB float copy
+XCDS = 8                                  # MI355X: accelerator complex dies per GPU
 
 
 def _rows(directory, suffix):
@@ -123,14 +124,13 @@ def cmd_counters(a):
     derived = []
     if "TCC_HIT_sum" in vals and "TCC_MISS_sum" in vals:
         derived.append(("L2 hit rate", vals["TCC_HIT_sum"] / max(1.0, vals["TCC_HIT_sum"] + vals["TCC_MISS_sum"])))
-    if "SQ_ACTIVE_INST_VALU" in vals and "SQ_BUSY_CYCLES" in vals:
-        derived.append(("VALU issue cycles / busy cycles (per SQ)",
-                        vals["SQ_ACTIVE_INST_VALU"] / max(1.0, vals["SQ_BUSY_CYCLES"])))
-    if "SQ_WAIT_INST_ANY" in vals and "SQ_WAVE_CYCLES" in vals:
-        derived.append(("wave cycles waiting on any instruction",
-                        vals["SQ_WAIT_INST_ANY"] / max(1.0, vals["SQ_WAVE_CYCLES"])))
     if "TA_BUSY_avr" in vals and "GRBM_GUI_ACTIVE" in vals:
-        derived.append(("TA busy / GPU active", vals["TA_BUSY_avr"] / max(1.0, vals["GRBM_GUI_ACTIVE"])))
+        # GRBM_GUI_ACTIVE reads as the sum over the part's XCDs
+        derived.append((f"TA busy / active cycles ({XCDS} XCDs)",
+                        vals["TA_BUSY_avr"] * XCDS / max(1.0, vals["GRBM_GUI_ACTIVE"])))
+    if "SQ_WAIT_INST_ANY" in vals and "SQ_WAVE_CYCLES" in vals:
+        derived.append(("wave cycles waiting on an instruction",
+                        vals["SQ_WAIT_INST_ANY"] / max(1.0, vals["SQ_WAVE_CYCLES"])))
     if derived:
         lines += ["", "derived:"] + [f"{k:44s} {v:8.3f}" for k, v in derived]
     print("\n".join(lines))
