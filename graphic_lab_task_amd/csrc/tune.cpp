@@ -81,7 +81,7 @@ hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, 
   }
   for (const TuneConfig& shape : tops) {
     for (int pp = 0; pp < n_postpone && e == hipSuccess; ++pp) {
-      for (uint32_t budget : {32u, 48u, 64u, 96u, 128u, 192u}) {
+      for (uint32_t budget : {32u, 48u, 64u, 80u, 96u, 112u, 128u, 192u}) {
         for (uint32_t waves : {1024u, 2048u}) {
           TuneConfig c = shape;
           c.budget = budget;
