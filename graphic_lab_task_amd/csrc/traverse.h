@@ -67,6 +67,7 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
   const bool postpone = !COUNT && (((KINDS & LK_TRI_POSTPONE) != 0) ||
                                    ((KINDS & (LK_FRAME | LK_SBS | LK_MULTI)) != 0 && tag != TAG_TRIANGLE));
   uint32_t postponed = 0;   // the noted leaf word (leaf words always have LEAF_BIT set)
+  constexpr bool LEAF_HALF = (KINDS & (LK_SBS | LK_MULTI)) == 0;
   auto note_leaf = [&]() {
     if (leftNodeOffset != 0xFFFFFFFFu) postponed = leftNodeOffset;
     top--;
@@ -104,7 +105,11 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
       }
       if (postpone) {
         if (postponed == 0 && top >= 0 && (leftNodeOffset & LEAF_BIT)) note_leaf();
-        if (!__any(postponed == 0)) break;
+        // Frame-octree and triangle leaves are cheap: the noted leaves are taken once half of
+        // the wave's walking lanes hold one (C2 -3%). Brick-set and multi-octree leaves are whole
+        // marches and wait for every lane (taking them at half costs C3 +6%).
+        if (LEAF_HALF ? __popcll(__ballot(postponed == 0)) * 2 <= __popcll(__ballot(true)) : !__any(postponed == 0))
+          break;
       }
     }
     if (postpone) {
