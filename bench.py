@@ -2,7 +2,9 @@
 """Ray-query benchmark on one MI355X.
 
 A step is one closest-hit launch over the workload's full ray batch, inputs resident in HBM.
-Prints one JSON line (see DESIGN.md for the metric, the roofline model and the CPU baseline).
+Prints one JSON line: the metric, the roofline of the dominant kernel (algorithmic bytes per step
+from traversal counters / step time, against 8 TB/s; HBM traffic from the committed counter
+summaries) and, on the trace workloads, the CPU oracle timed on the host cores.
 
 Workloads (synthetic, seeded):
   sdf_octree_d9_1024  64-sphere union, frame octree depth 9, 1024x1024 jittered primary rays
@@ -454,8 +456,8 @@ def main():
                     help="also run a 256 MiB device copy (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the sharded multi-GPU path is "
-              f"designed in DESIGN.md but not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); sharding the rays or scenes over "
+              f"several GPUs is designed but not built", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -822,8 +822,8 @@ int oracle_flavor(void) {
 #endif
 }
 
-// Batched BVHRT::RayQuery_NearestHit over host arrays; OpenMP over rays
-// (schedule(dynamic,256), as the baseline plan asks). Returns the number of rays whose stack
+// Batched BVHRT::RayQuery_NearestHit over host arrays; OpenMP over rays with
+// schedule(dynamic,256). Returns the number of rays whose stack
 // reached the 80-entry limit. counts may be NULL.
 uint64_t oracle_trace_closest(const OracleScene* sc, const float* pos, const float* dir, void* hits, uint64_t n,
                               int threads, OracleCounts* counts) {
