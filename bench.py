@@ -448,7 +448,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the given launch configuration")
     ap.add_argument("--budget", type=int, default=64, help="tail step budget when not autotuning")
     ap.add_argument("--tail-waves", type=int, default=1024, help="tail waves when not autotuning")
-    ap.add_argument("--ray-interleave", type=int, default=8,
+    ap.add_argument("--ray-interleave", type=int, default=32,
                     help="pass-1 lanes per interleaved group when not autotuning (0: thread order)")
     ap.add_argument("--leaf-postpone", type=int, default=-1, choices=[-1, 0, 1],
                     help="triangle leaf postponement when not autotuning (-1: on for one instance)")

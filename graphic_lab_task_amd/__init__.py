@@ -386,7 +386,7 @@ class SceneRT:
     def set_tail_config(self, budget: int = 64, tail_waves: int = 1024):
         _check(lib().lrt_set_tail_config(self._h, budget, tail_waves))
 
-    def set_ray_interleave(self, lanes_per_group: int = 8):
+    def set_ray_interleave(self, lanes_per_group: int = 32):
         """Pass-1 ray-to-lane assignment: groups of `lanes_per_group` lanes take rays from distant
         parts of the batch (0 or 64: thread order)."""
         _check(lib().lrt_set_ray_interleave(self._h, lanes_per_group))

@@ -74,7 +74,7 @@ struct Scene {
   SceneView view{};
   uint32_t* d_overflow = nullptr;
   int block = 256, lds_stack = 16, xcd_remap = 1;
-  uint32_t ray_groups = 8;   // pass-1 lanes per group dealt to distant parts of the batch (0: off)
+  uint32_t ray_groups = 32;  // pass-1 lanes per group dealt to distant parts of the batch (0: off)
   // Deep-stack spill buffers, one per stream: launches on one stream run in order, so rays of
   // two launches never share a slot at the same time.
   std::mutex spill_mu;

@@ -49,7 +49,7 @@ hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, 
   };
   const int n_postpone = tune_postpone ? 2 : 1;
   for (int pp = 0; pp < n_postpone && e == hipSuccess; ++pp) {
-    for (uint32_t groups : {0u, 8u}) {
+    for (uint32_t groups : {0u, 8u, 32u}) {
       for (int block : {64, 128, 256}) {
         for (int lds : {8, 16, 32}) {
           TuneConfig c = start;

@@ -23,7 +23,7 @@ struct TuneResult {
 
 // Two stages, each candidate timed with HIP events on `stream` (one warm-up launch, then the
 // minimum of `reps`): (1) block size x LDS stack entries x ray interleaving (off, groups of 8
-// lanes) with the single-pass walk, (2) step budget x tail waves of the two-pass scheme on the
+// or 32 lanes) with the single-pass walk, (2) step budget x tail waves of the two-pass scheme on the
 // three best shapes from (1); both stages also try triangle leaf postponement off and on when
 // `tune_postpone`.
 hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, hipStream_t stream,

@@ -163,7 +163,7 @@ lrt_status lrt_set_leaf_postponement(lrt_scene* s, int mode);
  * part of the batch (a fixed permutation; results are unchanged, each hit is stored at its
  * ray's index). Neighbouring primary rays graze the same silhouettes and a wave steps at the pace
  * of its slowest lane, so spreading them shortens the slowest waves. 0 or 64 = rays in thread
- * order. Default 8. No counterpart in the reference. */
+ * order. Default 32. No counterpart in the reference. */
 lrt_status lrt_set_ray_interleave(lrt_scene* s, uint32_t lanes_per_group);
 
 /* ---- Differentiable sparse brick set (no counterpart in the reference) ----

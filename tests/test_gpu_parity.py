@@ -117,7 +117,7 @@ def test_ray_interleave_identical(lrt, orc, torch_dev):
                 _say(f"ray interleave {name} n={n} lanes={lanes}")
                 s.set_ray_interleave(lanes)
                 assert raw_equal(gpu_trace(torch_dev, s, p, d), base), f"{name} n={n} lanes={lanes}"
-        s.set_ray_interleave(8)
+        s.set_ray_interleave(32)
 
 
 def test_counters_match_oracle(lrt, orc, torch_dev):
