@@ -20,6 +20,10 @@ struct lrt_scene {
 struct lrt_sbs_batch {
   int device = -1;
   std::vector<lrt::SbsBatchItem> items;
+  // per item: the scene's commit generation when the plan was made (a re-commit, clear or
+  // destruction of the scene invalidates the device pointers the plan holds)
+  std::vector<std::weak_ptr<uint64_t>> scene_gen;
+  std::vector<uint64_t> gen_at_create;
   lrt::DevBuf d_items, d_begins, d_masks, d_spill, d_counter;
   lrt::SbsBatchLaunch L{};
 };
@@ -81,7 +85,11 @@ lrt_status trace_common(lrt_scene* s, const float* pos, const float* dir, lrt_hi
   a.lds_stack = s->s.lds_stack;
   a.xcd_remap = uint32_t(s->s.xcd_remap);
   a.counts = counts;
-  hipError_t e = s->s.work_for(stream, size_t(n), a.lds_stack, &a.spill, &a.queue);
+  // held until both passes are enqueued: another thread on this stream must not slip its passes
+  // between ours (they share the stream's tail queue) or grow the buffer under us
+  lrt::Scene::StreamWork& w = s->s.stream_work(stream);
+  std::lock_guard<std::mutex> lock(w.mu);
+  hipError_t e = s->s.work_for(w, stream, size_t(n), &a.spill, &a.queue);
   if (e != hipSuccess) return hip_fail(e, "work buffer");
   // The cooperative pass reproduces every hit field except the per-leaf call count that the
   // INTERSECTION_COUNT debug mode accumulates in instId; that mode runs single-pass.
@@ -102,6 +110,10 @@ lrt_status diff_geometry(lrt_scene* s, uint32_t geom_id, uint32_t* f_offset, uin
   lrt::Scene& sc = s->s;
   if (geom_id >= sc.geomData.size() || sc.geomTags[geom_id] != lrt::TAG_SDF_BRICK)
     return fail(LRT_ERR_INVALID_ARG, "geometry is not a sparse brick set");
+  // the differentiable kernels walk brick-set leaves only: another leaf kind in the scene would be
+  // skipped silently (an occluding mesh rendered invisible, its shadowed gradients wrong)
+  if (sc.leaf_kinds() != lrt::LK_SBS)
+    return fail(LRT_ERR_UNSUPPORTED, "differentiable calls need a scene whose geometries are all sparse brick sets");
   const uint32_t sdf = sc.geomData[geom_id].offset.x;
   const lrt::uint2 sl = sc.sbsFSlice[sdf];
   if (sl.y == 0 || sl.y % 4 != 0)
@@ -157,14 +169,16 @@ lrt_status sbs_diff_common(lrt_scene* s, uint32_t geom_id, const float* params, 
   a.counters = s->s.d_overflow ? s->s.d_overflow + 1 : nullptr;
   a.xcd_remap = uint32_t(s->s.xcd_remap);
   lrt::TailQueue* q = nullptr;
-  hipError_t e = s->s.work_for(stream, size_t(n), 16, &a.spill, &q);
+  lrt::Scene::StreamWork& w = s->s.stream_work(stream);
+  std::lock_guard<std::mutex> lock(w.mu);
+  hipError_t e = s->s.work_for(w, stream, size_t(n), &a.spill, &q);
   if (e != hipSuccess) return hip_fail(e, "work buffer");
   // occupancy masks from the caller's parameters (they differ from the committed values)
   lrt::SceneView v = s->s.make_view();
   v.sbs_data_f = params;
   v.sbs_mask = nullptr;
   uint32_t* mask = nullptr;
-  e = s->s.mask_for(stream, &mask);
+  e = s->s.mask_for(w, stream, &mask);
   if (e != hipSuccess) return hip_fail(e, "mask buffer");
   e = lrt::launch_sbs_masks(v, s->s.sbsMaskGeoms.data(), uint32_t(s->s.sbsMaskGeoms.size()), mask,
                             reinterpret_cast<hipStream_t>(stream));
@@ -337,15 +351,23 @@ lrt_status lrt_trace_closest_host(lrt_scene* s, const float* pos_near, const flo
     if (!s->s.committed) return fail(LRT_ERR_NOT_COMMITTED, "scene is not committed");
     if (n == 0) return LRT_OK;
     if (!pos_near || !dir_far || !hits) return fail(LRT_ERR_INVALID_ARG, "null buffer");
-    lrt::DevBuf dp, dd, dh;
+    if (n > 0xFFFFFFFFull) return fail(LRT_ERR_INVALID_ARG, "too many rays in one call");
+    // one grow-only staging buffer per scene: rays (2 x 16 B) then hits (32 B) per ray
+    std::lock_guard<std::mutex> lock(s->s.host_mu);
+    lrt::DevBuf& io = s->s.host_io;
     hipError_t e;
-    if ((e = dp.upload(pos_near, n * 16)) != hipSuccess) return hip_fail(e, "copy rays");
-    if ((e = dd.upload(dir_far, n * 16)) != hipSuccess) return hip_fail(e, "copy rays");
-    if ((e = dh.reserve(n * sizeof(lrt_hit))) != hipSuccess) return hip_fail(e, "alloc hits");
-    lrt_status st = trace_common(s, (const float*)dp.ptr(), (const float*)dd.ptr(), (lrt_hit*)dh.ptr(), nullptr, n,
-                                 nullptr);
+    if (io.bytes() < n * 64) {
+      if ((e = io.reserve(n * 64)) != hipSuccess) return hip_fail(e, "host staging buffer");
+    }
+    char* base = static_cast<char*>(io.ptr());
+    float* dp = reinterpret_cast<float*>(base);
+    float* dd = reinterpret_cast<float*>(base + n * 16);
+    lrt_hit* dh = reinterpret_cast<lrt_hit*>(base + n * 32);
+    if ((e = hipMemcpy(dp, pos_near, n * 16, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "copy rays");
+    if ((e = hipMemcpy(dd, dir_far, n * 16, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "copy rays");
+    lrt_status st = trace_common(s, dp, dd, dh, nullptr, n, nullptr);
     if (st != LRT_OK) return st;
-    if ((e = hipMemcpy(hits, dh.ptr(), n * sizeof(lrt_hit), hipMemcpyDeviceToHost)) != hipSuccess)
+    if ((e = hipMemcpy(hits, dh, n * sizeof(lrt_hit), hipMemcpyDeviceToHost)) != hipSuccess)
       return hip_fail(e, "copy hits");
     return LRT_OK;
   });
@@ -583,6 +605,8 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
       total_bricks += it.n_bricks;
       chunk_begin.push_back(chunk_begin.back() + (it.args.n + lrt::SBS_BATCH_CHUNK - 1) / lrt::SBS_BATCH_CHUNK);
       b->items.push_back(it);
+      b->scene_gen.push_back(sc.generation);
+      b->gen_at_create.push_back(*sc.generation);
     }
     e = b->d_masks.reserve(std::max<size_t>(mask_words, 1) * 4);
     if (e != hipSuccess) return hip_fail(e, "batch masks");
@@ -631,6 +655,12 @@ lrt_status lrt_sbs_batch_run(lrt_sbs_batch* b, void* stream) {
     hipError_t ge = hipGetDevice(&dev);
     if (ge != hipSuccess) return hip_fail(ge, "no device");
     if (dev != b->device) return fail(LRT_ERR_INVALID_ARG, "batch plan was created on another device");
+    for (size_t k = 0; k < b->scene_gen.size(); ++k) {
+      const std::shared_ptr<uint64_t> g = b->scene_gen[k].lock();
+      if (!g || *g != b->gen_at_create[k])
+        return fail(LRT_ERR_INVALID_ARG, "batch plan is stale: item " + std::to_string(k) +
+                                             "'s scene was changed, re-committed or destroyed since the plan was made");
+    }
     const hipError_t e = lrt::launch_sbs_batch(b->L, reinterpret_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "batch launch");
     return LRT_OK;

@@ -35,30 +35,42 @@ void DevBuf::release() {
   cap_ = used_ = 0;
 }
 
-hipError_t Scene::work_for(void* stream, size_t n, int lds_stack, uint32_t** spill, TailQueue** queue) {
-  std::lock_guard<std::mutex> lock(spill_mu);
-  DevBuf& b = spill_bufs[stream];
-  size_t& cap = spill_rays[stream];
+Scene::StreamWork& Scene::stream_work(void* stream) {
+  std::lock_guard<std::mutex> lock(work_mu);
+  std::unique_ptr<StreamWork>& w = work[stream];
+  if (!w) w = std::make_unique<StreamWork>();
+  return *w;
+}
+
+hipError_t Scene::work_for(StreamWork& w, void* stream, size_t n, uint32_t** spill, TailQueue** queue) {
   const size_t head = 256;
   const size_t list_bytes = (n * 4 + 255) / 256 * 256;
   // spill sized for the smallest LDS part so a later launch config change never overruns it
   const size_t need = head + list_bytes + n * size_t(STACK_SIZE - 8) * 4;
-  (void)lds_stack;
-  if (cap < n || b.bytes() < need) {
-    hipError_t e = b.reserve(need);
+  if (w.cap_rays < n || w.buf.bytes() < need) {
+    // earlier calls on this stream may still be reading the old buffer
+    hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return e;
-    TailQueue q{0u, 0u, reinterpret_cast<uint32_t*>(static_cast<char*>(b.ptr()) + head)};
-    e = hipMemcpy(b.ptr(), &q, sizeof(q), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return e;
-    cap = n;
+    e = w.buf.reserve(need);
+    if (e != hipSuccess) {
+      w.cap_rays = 0;
+      return e;
+    }
+    TailQueue q{0u, 0u, reinterpret_cast<uint32_t*>(static_cast<char*>(w.buf.ptr()) + head)};
+    e = hipMemcpy(w.buf.ptr(), &q, sizeof(q), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      w.cap_rays = 0;
+      return e;
+    }
+    w.cap_rays = n;
   }
-  *queue = static_cast<TailQueue*>(b.ptr());
-  *spill = reinterpret_cast<uint32_t*>(static_cast<char*>(b.ptr()) + head + (cap * 4 + 255) / 256 * 256);
+  *queue = static_cast<TailQueue*>(w.buf.ptr());
+  *spill = reinterpret_cast<uint32_t*>(static_cast<char*>(w.buf.ptr()) + head + (w.cap_rays * 4 + 255) / 256 * 256);
   return hipSuccess;
 }
 
 Scene::~Scene() {
-  spill_bufs.clear();
+  work.clear();
   dev.clear();
   if (d_overflow) (void)hipFree(d_overflow);
 }
@@ -114,7 +126,7 @@ void add_border_nodes_rec(const SdfFrameOctreeNode* nodes, size_t n_nodes, std::
 void Scene::clear_scene() {
   instances.clear();
   tlas.clear();
-  committed = false;
+  invalidate();
 }
 
 void Scene::clear_geom() {
@@ -169,7 +181,7 @@ uint32_t Scene::add_triangles(const float* vpos, size_t n_verts, size_t v_stride
   const size_t oldSize = primIdCount.size();
   primIdCount.insert(primIdCount.end(), bvh.startCount.begin(), bvh.startCount.end());
   startEnd.push_back(uint2{uint32_t(oldSize), uint32_t(primIdCount.size())});
-  committed = false;
+  invalidate();
   return uint32_t(startEnd.size() - 1);
 }
 
@@ -182,7 +194,7 @@ uint32_t Scene::add_aabb(uint32_t tag, const BVHNode* boxes, size_t n) {
   for (size_t i = 0; i < n; i++) primIdCount[oldSize + i] = uint32_t(i);
   startEnd.push_back(uint2{uint32_t(oldSize), uint32_t(primIdCount.size())});
   geomTags.push_back(tag);
-  committed = false;
+  invalidate();
   return uint32_t(startEnd.size() - 1);
 }
 
@@ -354,7 +366,7 @@ uint32_t Scene::add_instance(uint32_t geom_id, const float4x4& m) {
   inst.transformInv = inverse4x4(m);
   inst.transformInvTransposed = transpose4x4(inverse4x4(m));
   instances.push_back(inst);
-  committed = false;
+  invalidate();
   return uint32_t(instances.size() - 1);
 }
 
@@ -374,7 +386,7 @@ void Scene::update_instance(uint32_t inst_id, const float4x4& m) {
   instances[inst_id].boxMax = box.mx;
   instances[inst_id].transform = m;
   instances[inst_id].transformInv = inverse4x4(m);
-  committed = false;
+  invalidate();
 }
 
 // BVHRT::SetPreset (BVH2Common_host.cpp:1437-1440)
@@ -504,19 +516,20 @@ void Scene::build_device_layout() {
   }
 }
 
-hipError_t Scene::mask_for(void* stream, uint32_t** mask) {
-  std::lock_guard<std::mutex> lock(spill_mu);
-  DevBuf& b = mask_bufs[stream];
+hipError_t Scene::mask_for(StreamWork& w, void* stream, uint32_t** mask) {
   const size_t need = std::max<size_t>(sbsMaskWords, 1) * sizeof(uint32_t);
-  if (b.bytes() < need) {
-    hipError_t e = b.reserve(need);
+  if (w.mask.bytes() < need) {
+    hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return e;
+    e = w.mask.reserve(need);
     if (e != hipSuccess) return e;
   }
-  *mask = static_cast<uint32_t*>(b.ptr());
+  *mask = static_cast<uint32_t*>(w.mask.ptr());
   return hipSuccess;
 }
 
 hipError_t Scene::commit() {
+  invalidate();   // buffers below may move
   build_device_layout();
   // occupancy-mask layout: per brick 1 summary word + ceil(bs^3 / 32) bit words
   sbsMaskGeoms.clear();

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -75,21 +76,44 @@ struct Scene {
   uint32_t* d_overflow = nullptr;
   int block = 256, lds_stack = 16, xcd_remap = 1;
   uint32_t ray_groups = 32;  // pass-1 lanes per group dealt to distant parts of the batch (0: off)
-  // Deep-stack spill buffers, one per stream: launches on one stream run in order, so rays of
-  // two launches never share a slot at the same time.
-  std::mutex spill_mu;
-  std::map<void*, DevBuf> spill_bufs;
-  std::map<void*, size_t> spill_rays;
   uint32_t budget = 64, tail_waves = 1024;
   // Triangle leaf postponement in pass 1: -1 automatic (on for a triangle-only scene with one
   // instance), 0 off, 1 on.
   int leaf_postpone = -1;
   bool postpone_triangles() const;
-  // Per-stream work memory for n rays: tail queue (header + list) and deep-stack spill.
-  hipError_t work_for(void* stream, size_t n, int lds_stack, uint32_t** spill, TailQueue** queue);
-  // per-stream occupancy-mask buffer for calls whose values differ from the committed ones
-  hipError_t mask_for(void* stream, uint32_t** mask);
-  std::map<void*, DevBuf> mask_bufs;
+
+  // Work memory of one stream: the tail queue (256-B header + list), the deep-stack spill and
+  // the occupancy masks of calls whose values differ from the committed ones. A call holds
+  // `mu` from sizing the buffers until its last kernel is enqueued, so that two threads tracing
+  // on one stream never interleave their passes over the same queue; a buffer only grows, and
+  // only after the stream has drained the work that may still read it.
+  struct StreamWork {
+    std::mutex mu;
+    DevBuf buf;
+    size_t cap_rays = 0;
+    DevBuf mask;
+  };
+  std::mutex work_mu;   // guards the map only
+  std::map<void*, std::unique_ptr<StreamWork>> work;
+  StreamWork& stream_work(void* stream);
+  // With w.mu held: queue and spill for n rays on `stream`.
+  hipError_t work_for(StreamWork& w, void* stream, size_t n, uint32_t** spill, TailQueue** queue);
+  // With w.mu held: the stream's occupancy-mask buffer.
+  hipError_t mask_for(StreamWork& w, void* stream, uint32_t** mask);
+
+  // Staging for the host-buffer trace calls (lrt_trace_closest_host, lrt_query_one): grows
+  // only, so a picking loop allocates on its first call and never again.
+  std::mutex host_mu;
+  DevBuf host_io;
+
+  // Commit generation: bumped by every change that invalidates device pointers handed out
+  // earlier (commit, clear, add, instance update). Batch plans record it and refuse to run once
+  // it moved; the weak_ptr of a plan also sees the scene's destruction.
+  std::shared_ptr<uint64_t> generation = std::make_shared<uint64_t>(0);
+  void invalidate() {
+    committed = false;
+    ++*generation;
+  }
 
   ~Scene();
 

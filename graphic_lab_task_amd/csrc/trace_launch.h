@@ -1,0 +1,184 @@
+// Closest-hit / any-hit ray query kernels for gfx950 and their launch templates. Each leaf-kind
+// instantiation is compiled in a translation unit of its own (trace_<kind>.hip).
+//
+// A trace call runs in two passes:
+//  1. one ray per lane, the reference's sequential walk, with a step budget: the few rays
+//     that need more steps than the budget (grazing rays crossing many boxes) leave their
+//     index in a queue instead of holding their whole wave hostage;
+//  2. a persistent kernel where each wave takes one queued ray and walks its BLAS
+//     cooperatively (trace_coop.h), which returns the same hit in far fewer dependent steps.
+// The instrumented (counted) variant runs pass 1 without a budget.
+#pragma once
+#include "kernels.h"
+#include "traverse.h"
+
+namespace lrt {
+
+// Pass 1. `budget` 0: unlimited (no queue).
+template <uint32_t KINDS, int S, int BLOCK, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_trace_closest(SceneView s, const float4* __restrict__ pos,
+                                                         const float4* __restrict__ dir, CRT_Hit* __restrict__ out,
+                                                         uint32_t* __restrict__ any_out, uint32_t n,
+                                                         uint32_t* overflow_count, uint32_t xcd_remap,
+                                                         uint32_t* __restrict__ counts, uint32_t* __restrict__ spill,
+                                                         uint32_t budget, TailQueue* queue, uint32_t group_log2) {
+  __shared__ uint32_t lds[S * BLOCK];
+  // world ray of each lane while its BLAS is walked (+ the top of the multi-octree walk's stack)
+  __shared__ uint32_t keep[(6 + ((KINDS & LK_MULTI) ? 4 * OT_LDS : 0)) * BLOCK];
+  // top BFS levels of the cached geometry's BLAS: dynamic, s.cache_pairs entries (none for a
+  // scene without a cached geometry, which then keeps its LDS for more resident waves)
+  extern __shared__ BVHNodePair top_pairs[];
+  // stage the top BFS levels of the cached geometry's BLAS (every thread reaches the barrier)
+  if (s.cache_pairs) {
+    const uint4* src = reinterpret_cast<const uint4*>(s.nodes + s.geoms[s.cache_geom].bvhOffset);
+    uint4* dst = reinterpret_cast<uint4*>(top_pairs);
+    for (uint32_t k = threadIdx.x; k < s.cache_pairs * 4; k += BLOCK) dst[k] = src[k];
+  }
+  __syncthreads();
+  const uint32_t b = xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  uint32_t i = b * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  if (!COUNT && group_log2 < 6) i = interleave_ray(i, n, group_log2);
+  LaneStack<S, BLOCK> st;
+  st.col = (lds_u32*)(lds) + threadIdx.x;
+  st.spill = spill + size_t(i) * (STACK_SIZE - S);
+  uint32_t overflow = 0;
+  float4 p = pos[i];
+  float4 d = dir[i];
+  if (any_out) d.w *= -1.0f;   // RayQuery_AnyHit flips the sign of w (BVH2Common.cpp:3394-3399)
+  RayCounts cnt{};
+  bool aborted = false;
+  const uint64_t t_start = COUNT ? wall_clock64() : 0;
+  CRT_Hit h;
+  if (!COUNT && box_test_fast(xyz(p), xyz(d)))
+    h = ray_query_nearest<KINDS, S, BLOCK, COUNT, false, true>(s, p, d, st, overflow, cnt, (const lds_pair*)top_pairs,
+                                                               budget, aborted, nullptr,
+                                                               (lds_u32*)(keep) + threadIdx.x);
+  else
+    h = ray_query_nearest<KINDS, S, BLOCK, COUNT, false, false>(s, p, d, st, overflow, cnt,
+                                                                (const lds_pair*)top_pairs, COUNT ? 0u : budget,
+                                                                aborted, nullptr, (lds_u32*)(keep) + threadIdx.x);
+  if (aborted) {
+    queue->list[atomicAdd(&queue->count, 1u)] = i;
+    return;
+  }
+  if (out) store_hit(out, i, h);
+  if (COUNT) {
+    const uint64_t t_end = wall_clock64();
+    uint32_t* c = counts + size_t(i) * RAY_COUNTERS;
+    c[0] = cnt.blas_nodes; c[1] = cnt.leaves; c[2] = cnt.prims;
+    c[3] = cnt.tlas_nodes; c[4] = cnt.instances; c[5] = cnt.sdf_voxels;
+    c[6] = uint32_t(t_start);   // device wall clock (100 MHz) at ray start, low 32 bits
+    c[7] = uint32_t(t_end - t_start);
+  }
+  if (any_out) any_out[i] = (h.geomId != 0xFFFFFFFFu) ? 1u : 0u;
+  if (overflow && overflow_count) atomicAdd(overflow_count, 1u);
+}
+
+// Pass 2: persistent waves; each takes the next queued ray and traces it cooperatively.
+template <uint32_t KINDS>
+__global__ __launch_bounds__(64) void k_trace_tail(SceneView s, const float4* __restrict__ pos,
+                                                   const float4* __restrict__ dir, CRT_Hit* __restrict__ out,
+                                                   uint32_t* __restrict__ any_out, uint32_t* overflow_count,
+                                                   TailQueue* queue) {
+  __shared__ uint32_t coop[COOP_CAP * 4];
+  const uint32_t total = __atomic_load_n(&queue->count, __ATOMIC_RELAXED);
+  for (;;) {
+    uint32_t idx = 0;
+    if (__lane_id() == 0) idx = atomicAdd(&queue->next, 1u);
+    idx = __shfl(idx, 0, 64);
+    if (idx >= total) {
+      // Every wave ends on exactly one fetch past the queue, after its last ray; the fetch that
+      // returns total + waves - 1 is the last of all, so that wave clears the queue for the next
+      // call on this stream (no reset launch precedes pass 1).
+      if (__lane_id() == 0 && idx == total + gridDim.x - 1) {
+        queue->count = 0;
+        queue->next = 0;
+      }
+      break;
+    }
+    const uint32_t i = queue->list[idx];
+    float4 p = pos[i];
+    float4 d = dir[i];
+    if (any_out) d.w *= -1.0f;
+    LaneStack<1, 64> unused;
+    unused.col = nullptr;
+    unused.spill = nullptr;
+    RayCounts cnt{};
+    uint32_t overflow = 0;
+    bool aborted = false;
+    const CRT_Hit h = ray_query_nearest<KINDS, 1, 64, false, true>(s, p, d, unused, overflow, cnt, nullptr, 0u,
+                                                                   aborted, (lds_entry*)coop);
+    if (__lane_id() == 0) {
+      if (out) store_hit(out, i, h);
+      if (any_out) any_out[i] = (h.geomId != 0xFFFFFFFFu) ? 1u : 0u;
+      if (overflow && overflow_count) atomicAdd(overflow_count, 1u);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// log2 of the lanes per group of interleave_ray; 6 (whole waves) means no interleaving
+static inline uint32_t lanes_log2(uint32_t lanes) {
+  uint32_t l = 0;
+  while (l < 6 && (1u << l) < lanes) ++l;
+  return lanes == 0 ? 6u : l;
+}
+
+template <uint32_t KINDS, int S, int BLOCK>
+static hipError_t launch_trace_t(const SceneView& s, const TraceArgs& a, hipStream_t stream) {
+  const uint32_t grid = (a.n + BLOCK - 1) / BLOCK;
+  if (grid == 0) return hipSuccess;
+  const size_t lds_cache = size_t(s.cache_pairs) * sizeof(BVHNodePair);
+  if (a.counts) {
+    if constexpr ((KINDS & LK_TRI_POSTPONE) != 0) {
+      return hipErrorInvalidValue;   // counted runs take the plain walk (launch_trace routes them)
+    } else {
+      hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, true>), dim3(grid), dim3(BLOCK), lds_cache, stream, s,
+                         a.pos, a.dir, a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts, a.spill, 0u,
+                         a.queue, 6u);
+      return hipGetLastError();
+    }
+  }
+  // the queue starts cleared (allocation) and every second pass leaves it cleared
+  const bool tail = a.budget != 0 && a.queue != nullptr;
+  hipLaunchKernelGGL((k_trace_closest<KINDS, S, BLOCK, false>), dim3(grid), dim3(BLOCK), lds_cache, stream, s,
+                     a.pos, a.dir, a.out, a.any_out, a.n, a.overflow_count, a.xcd_remap, a.counts, a.spill,
+                     tail ? a.budget : 0u, a.queue, lanes_log2(a.ray_groups));
+  if (tail) {
+    const uint32_t waves = a.tail_waves ? a.tail_waves : 1024u;
+    // pass 2 walks cooperatively; the postponement option only shapes pass 1
+    hipLaunchKernelGGL((k_trace_tail<KINDS & ~LK_TRI_POSTPONE>), dim3(waves), dim3(64), 0, stream, s, a.pos, a.dir,
+                       a.out, a.any_out, a.overflow_count, a.queue);
+  }
+  return hipGetLastError();
+}
+
+template <uint32_t KINDS>
+hipError_t launch_kinds(const SceneView& s, const TraceArgs& a, hipStream_t stream) {
+  if (a.counts) return launch_trace_t<KINDS, 16, 256>(s, a, stream);   // one shape for the counted variant
+  const int cfg = a.block * 1000 + a.lds_stack;
+  switch (cfg) {
+    case 64 * 1000 + 16:  return launch_trace_t<KINDS, 16, 64>(s, a, stream);
+    case 128 * 1000 + 16: return launch_trace_t<KINDS, 16, 128>(s, a, stream);
+    case 256 * 1000 + 16: return launch_trace_t<KINDS, 16, 256>(s, a, stream);
+    case 64 * 1000 + 32:  return launch_trace_t<KINDS, 32, 64>(s, a, stream);
+    case 128 * 1000 + 32: return launch_trace_t<KINDS, 32, 128>(s, a, stream);
+    case 256 * 1000 + 32: return launch_trace_t<KINDS, 32, 256>(s, a, stream);
+    case 64 * 1000 + 8:   return launch_trace_t<KINDS, 8, 64>(s, a, stream);
+    case 128 * 1000 + 8:  return launch_trace_t<KINDS, 8, 128>(s, a, stream);
+    case 256 * 1000 + 8:  return launch_trace_t<KINDS, 8, 256>(s, a, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// One instantiation per translation unit (trace_tri.hip, trace_frame.hip, ...).
+hipError_t launch_trace_tri(const SceneView& s, const TraceArgs& a, hipStream_t stream);
+hipError_t launch_trace_tri_postponed(const SceneView& s, const TraceArgs& a, hipStream_t stream);
+hipError_t launch_trace_frame(const SceneView& s, const TraceArgs& a, hipStream_t stream);
+hipError_t launch_trace_sbs(const SceneView& s, const TraceArgs& a, hipStream_t stream);
+hipError_t launch_trace_multi(const SceneView& s, const TraceArgs& a, hipStream_t stream);
+hipError_t launch_trace_mixed(const SceneView& s, const TraceArgs& a, hipStream_t stream);
+
+}  // namespace lrt

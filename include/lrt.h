@@ -10,8 +10,11 @@
  * Conventions
  *  - Every function returns lrt_status (0 = LRT_OK). On failure lrt_last_error()
  *    returns a message for the calling thread. Nothing throws across the ABI.
- *  - Build calls (add/commit/instance) need external synchronisation per scene;
- *    trace calls are thread-safe per committed scene.
+ *  - Build calls (add/commit/instance/preset/launch settings) need external synchronisation
+ *    per scene. Trace and differentiable calls on a committed scene may come from several
+ *    threads at once, on one stream or on several: a call holds its stream's work buffers
+ *    from sizing them until its last kernel is enqueued, so the calls of two threads on one
+ *    stream run one after the other. They must not overlap a build call on the same scene.
  *  - The library copies every array passed to lrt_add_* (the reference copies
  *    caller arrays into its own vectors, BVH2Common_host.cpp:317-327, 651-666).
  *  - Ray and hit buffers of the *_dev calls are device pointers owned by the caller;
@@ -108,7 +111,8 @@ lrt_status lrt_set_preset(lrt_scene* s, const void* preset);
 lrt_status lrt_trace_closest_dev(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,
                                  uint64_t n, void* stream);
 
-/* Same over host buffers (copies in, traces, copies out, synchronises). */
+/* Same over host buffers (copies in, traces, copies out, synchronises). The device staging
+ * buffer belongs to the scene and only grows: repeated calls of one size allocate nothing. */
 lrt_status lrt_trace_closest_host(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,
                                   uint64_t n);
 
@@ -125,7 +129,8 @@ lrt_status lrt_trace_any_dev(lrt_scene* s, const float* pos_near, const float* d
                              uint64_t n, void* stream);
 
 /* Single-ray RayQuery_NearestHit for picking (demo_app/plugins/MRFull/execute_commands.cpp:189);
- * runs the same gfx950 kernel on one ray and synchronises. */
+ * runs the same gfx950 kernel on one ray and synchronises. Uses the scene's staging buffer of
+ * lrt_trace_closest_host: after the first call, a picking loop allocates nothing. */
 lrt_status lrt_query_one(lrt_scene* s, const float* pos_near4, const float* dir_far4, lrt_hit* out);
 
 /* Primary rays of MultiRenderer::initEyeRay (MultiRenderer/MultiRenderer.cpp:142-183) for a
@@ -175,7 +180,8 @@ lrt_status lrt_set_ray_interleave(lrt_scene* s, uint32_t lanes_per_group);
  * size) and replaces the committed copy for the call; gradients have the same size and
  * layout and are ACCUMULATED (zero them first). Ray buffers as in lrt_trace_closest_dev (n may be
  * 0, and then the ray and output buffers may be NULL). The backward passes recompute every hit
- * (no state is kept between calls). */
+ * (no state is kept between calls). Every geometry of the scene must be a brick set
+ * (LRT_ERR_UNSUPPORTED otherwise: the differentiable walk has no other leaf intersectors). */
 lrt_status lrt_sbs_param_layout(lrt_scene* s, uint32_t geom_id, uint32_t* f_offset, uint32_t* n_dist,
                                 uint64_t* n_params);
 /* colour: float4 (r, g, b, hit 0/1) per ray. */
@@ -202,7 +208,10 @@ lrt_status lrt_sbs_render_l2_backward_dev(lrt_scene* s, uint32_t geom_id, const 
  * adjoint of every ray of every item. A run only launches kernels (no allocation, copy or
  * memset), so it may be captured in a HIP graph; buffers are read through the pointers given
  * at creation. Each item's scene must be committed and hold exactly one brick set, in the
- * differentiable layout of lrt_sbs_forward_dev. Runs of one plan must not overlap. */
+ * differentiable layout of lrt_sbs_forward_dev. Runs of one plan must not overlap. A plan is
+ * invalidated by any later change to one of its scenes (lrt_commit, lrt_clear_*, lrt_add_*,
+ * lrt_update_instance, lrt_destroy), which may move the device buffers it reads: lrt_sbs_batch_run
+ * then fails with LRT_ERR_INVALID_ARG; create a new plan. */
 typedef struct lrt_sbs_batch lrt_sbs_batch;
 typedef struct lrt_sbs_batch_item {
   lrt_scene* scene;

@@ -122,3 +122,25 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], stdout=subprocess.PIPE,
                        stderr=subprocess.PIPE, text=True, timeout=120)
     assert r.returncode != 0 and "one GPU" in r.stderr and r.stdout.strip() == ""
+
+
+def test_differentiable_calls_refuse_mixed_scenes(lrt):
+    """The adjoint walk intersects brick sets only: a scene that also holds a mesh is refused
+    (its occluders would otherwise be skipped without a word)."""
+    rng = np.random.default_rng(2)
+    sph = np.concatenate([rng.uniform(-0.5, 0.5, (4, 3)), rng.uniform(0.1, 0.3, (4, 1))], 1).astype(np.float32)
+    h, nodes, vals, vf, nd = lrt.sdf_spheres_sbs_indexed(sph, 2, 8, 1)
+    v, f = lrt.gen_icosphere(1)
+
+    def build(s, mesh):
+        gid = s.AddGeom_SdfSBS(h, nodes, vals, vf)
+        s.AddInstance(gid)
+        if mesh:
+            s.AddInstance(s.AddGeom_Triangles3f(v, f))
+
+    ok = host_scene(lrt, lambda s: build(s, False))
+    assert ok.sbs_param_layout(0)[1] == nd
+    mixed = host_scene(lrt, lambda s: build(s, True))
+    with pytest.raises(lrt.LrtError) as e:
+        mixed.sbs_param_layout(0)
+    assert e.value.code == 4 and "brick sets" in str(e.value)
