@@ -61,6 +61,7 @@ ARRAY_DTYPES = {
 TYPE_MESH_TRIANGLE, TYPE_SDF_FRAME_OCTREE, TYPE_SDF_SBS, TYPE_SDF_SBS_COL = 0, 3, 6, 11
 SH_TYPE, GEOM_ID_MASK = 24, 0x00FFFFFF
 MISS = 0xFFFFFFFF
+NOALLOC_MAX_RAYS = 65536   # LRT_NOALLOC_MAX_RAYS (include/lrt.h)
 SDF_SBS_NODE_LAYOUT_DX = 1 << 24
 SDF_SBS_NODE_LAYOUT_DX_UV16 = 2 << 24
 SDF_SBS_NODE_LAYOUT_DX_RGB8 = 3 << 24
@@ -128,6 +129,7 @@ def lib() -> ctypes.CDLL:
             "lrt_commit": ([vp, u32], i32), "lrt_set_preset": ([vp, vp], i32),
             "lrt_trace_closest_dev": ([vp, vp, vp, vp, u64, vp], i32),
             "lrt_trace_closest_host": ([vp, vp, vp, vp, u64], i32),
+            "lrt_trace_closest_noalloc_dev": ([vp, vp, vp, vp, u64, vp], i32),
             "lrt_trace_any_dev": ([vp, vp, vp, vp, u64, vp], i32),
             "lrt_trace_closest_counted_dev": ([vp, vp, vp, vp, vp, u64, vp], i32),
             "lrt_query_one": ([vp, vp, vp, vp], i32),
@@ -171,7 +173,7 @@ def exported_symbols():
     return ["lrt_last_error", "lrt_version", "lrt_create", "lrt_destroy", "lrt_clear_geom", "lrt_clear_scene",
             "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
-            "lrt_trace_closest_host", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
+            "lrt_trace_closest_host", "lrt_trace_closest_noalloc_dev", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
             "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_set_ray_interleave",
             "lrt_autotune", "lrt_sbs_param_layout",
             "lrt_sbs_forward_dev", "lrt_sbs_backward_dev", "lrt_sbs_l2_backward_dev", "lrt_sbs_render_l2_backward_dev", "lrt_sbs_recompute_misses",
@@ -324,6 +326,12 @@ class SceneRT:
         """Device buffers (torch tensors): pos/dir float32 (n,4), hits uint8/int32 storage of n*32 bytes."""
         n = pos.numel() // 4
         _check(lib().lrt_trace_closest_dev(self._h, _ptr(pos), _ptr(dir), _ptr(hits), n, stream))
+
+    def trace_closest_noalloc(self, pos, dir, hits, stream: int = 0):
+        """trace_closest through the allocation-free entry point (at most NOALLOC_MAX_RAYS rays;
+        overlapping calls on one GPU share its work array and must be serialised)."""
+        n = pos.numel() // 4
+        _check(lib().lrt_trace_closest_noalloc_dev(self._h, _ptr(pos), _ptr(dir), _ptr(hits), n, stream))
 
     def trace_closest_counted(self, pos, dir, hits, counters, stream: int = 0):
         """As trace_closest, plus 8 uint32 words per ray: 6 traversal counters, start tick, elapsed ticks."""

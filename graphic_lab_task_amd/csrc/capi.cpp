@@ -67,13 +67,9 @@ lrt_status check_device() {
   return LRT_OK;
 }
 
-lrt_status trace_common(lrt_scene* s, const float* pos, const float* dir, lrt_hit* hits, uint32_t* any, uint64_t n,
-                        void* stream, uint32_t* counts = nullptr) {
-  if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
-  if (!s->s.committed) return fail(LRT_ERR_NOT_COMMITTED, "scene is not committed");
-  if (n == 0) return LRT_OK;
-  if (!pos || !dir || (!hits && !any)) return fail(LRT_ERR_INVALID_ARG, "null ray or hit buffer");
-  if (n > 0xFFFFFFFFull) return fail(LRT_ERR_INVALID_ARG, "too many rays in one call");
+// Launch arguments of a trace call from the scene's tuned configuration (queue and spill unset).
+lrt::TraceArgs trace_args(lrt_scene* s, const float* pos, const float* dir, lrt_hit* hits, uint32_t* any, uint64_t n,
+                          uint32_t* counts) {
   lrt::TraceArgs a{};
   a.pos = reinterpret_cast<const lrt::float4*>(pos);
   a.dir = reinterpret_cast<const lrt::float4*>(dir);
@@ -85,12 +81,6 @@ lrt_status trace_common(lrt_scene* s, const float* pos, const float* dir, lrt_hi
   a.lds_stack = s->s.lds_stack;
   a.xcd_remap = uint32_t(s->s.xcd_remap);
   a.counts = counts;
-  // held until both passes are enqueued: another thread on this stream must not slip its passes
-  // between ours (they share the stream's tail queue) or grow the buffer under us
-  lrt::Scene::StreamWork& w = s->s.stream_work(stream);
-  std::lock_guard<std::mutex> lock(w.mu);
-  hipError_t e = s->s.work_for(w, stream, size_t(n), &a.spill, &a.queue);
-  if (e != hipSuccess) return hip_fail(e, "work buffer");
   // The cooperative pass reproduces every hit field except the per-leaf call count that the
   // INTERSECTION_COUNT debug mode accumulates in instId; that mode runs single-pass.
   a.budget = s->s.preset.render_mode == lrt::MULTI_RENDER_MODE_INTERSECTION_COUNT ? 0u : s->s.budget;
@@ -98,6 +88,23 @@ lrt_status trace_common(lrt_scene* s, const float* pos, const float* dir, lrt_hi
   a.ray_groups = s->s.ray_groups;
   // extra leaf visits would change the counts of that mode too
   a.postpone_tri = s->s.preset.render_mode != lrt::MULTI_RENDER_MODE_INTERSECTION_COUNT && s->s.postpone_triangles();
+  return a;
+}
+
+lrt_status trace_common(lrt_scene* s, const float* pos, const float* dir, lrt_hit* hits, uint32_t* any, uint64_t n,
+                        void* stream, uint32_t* counts = nullptr) {
+  if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+  if (!s->s.committed) return fail(LRT_ERR_NOT_COMMITTED, "scene is not committed");
+  if (n == 0) return LRT_OK;
+  if (!pos || !dir || (!hits && !any)) return fail(LRT_ERR_INVALID_ARG, "null ray or hit buffer");
+  if (n > 0xFFFFFFFFull) return fail(LRT_ERR_INVALID_ARG, "too many rays in one call");
+  lrt::TraceArgs a = trace_args(s, pos, dir, hits, any, n, counts);
+  // held until both passes are enqueued: another thread on this stream must not slip its passes
+  // between ours (they share the stream's tail queue) or grow the buffer under us
+  lrt::Scene::StreamWork& w = s->s.stream_work(stream);
+  std::lock_guard<std::mutex> lock(w.mu);
+  hipError_t e = s->s.work_for(w, stream, size_t(n), &a.spill, &a.queue);
+  if (e != hipSuccess) return hip_fail(e, "work buffer");
   e = lrt::launch_trace(s->s.make_view(), a, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "trace launch");
   return LRT_OK;
@@ -326,6 +333,28 @@ lrt_status lrt_set_preset(lrt_scene* s, const void* preset) {
 lrt_status lrt_trace_closest_dev(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits, uint64_t n,
                                  void* stream) {
   return guarded([&]() -> lrt_status { return trace_common(s, pos_near, dir_far, hits, nullptr, n, stream); });
+}
+
+lrt_status lrt_trace_closest_noalloc_dev(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,
+                                         uint64_t n, void* stream) {
+  return guarded([&]() -> lrt_status {
+    if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+    if (!s->s.committed) return fail(LRT_ERR_NOT_COMMITTED, "scene is not committed");
+    if (n == 0) return LRT_OK;
+    if (!pos_near || !dir_far || !hits) return fail(LRT_ERR_INVALID_ARG, "null ray or hit buffer");
+    if (n > LRT_NOALLOC_MAX_RAYS)
+      return fail(LRT_ERR_INVALID_ARG, "lrt_trace_closest_noalloc_dev takes at most " +
+                                           std::to_string(LRT_NOALLOC_MAX_RAYS) + " rays per call (got " +
+                                           std::to_string(n) + ")");
+    int device = -1;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return hip_fail(e, "current device");
+    if (device != s->s.device) return fail(LRT_ERR_INVALID_ARG, "the current device is not the scene's device");
+    const lrt::TraceArgs a = trace_args(s, pos_near, dir_far, hits, nullptr, n, nullptr);
+    e = lrt::launch_trace_noalloc(s->s.make_view(), a, device, reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "allocation-free trace launch");
+    return LRT_OK;
+  });
 }
 
 lrt_status lrt_trace_closest_counted_dev(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,

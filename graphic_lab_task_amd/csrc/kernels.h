@@ -112,6 +112,9 @@ hipError_t launch_sbs_batch(const SbsBatchLaunch& L, hipStream_t stream);
 hipError_t sbs_batch_blocks_per_cu(int* blocks);
 
 hipError_t launch_trace(const SceneView& s, const TraceArgs& a, hipStream_t stream);
+// launch_trace with the library's fixed per-device work array as queue and spill (a.queue and
+// a.spill are ignored): at most LRT_NOALLOC_MAX_RAYS rays, no counters; `device` is current.
+hipError_t launch_trace_noalloc(const SceneView& s, TraceArgs a, int device, hipStream_t stream);
 hipError_t launch_sbs_diff(const SceneView& s, const SbsDiffArgs& a, hipStream_t stream);
 hipError_t launch_eye_rays(const EyeRayParams& prm, const float2* jitter, float4* pos, float4* dir,
                            hipStream_t stream);

@@ -1,7 +1,30 @@
-// Trace-call dispatch on the scene's leaf kinds, the queue reset and the primary-ray generator.
+// Trace-call dispatch on the scene's leaf kinds, the queue reset, the allocation-free trace and
+// the primary-ray generator.
+#include <mutex>
+
+#include "../../include/lrt.h"
 #include "trace_launch.h"
 
 namespace lrt {
+
+// Work memory of the allocation-free trace: a 256-B tail-queue header, the queue (one word per
+// ray) and the deep-stack spill (STACK_SIZE - 8 words per ray, enough for every LDS stack depth),
+// for at most LRT_NOALLOC_MAX_RAYS rays. Host code never uses this name as an address: each
+// device's copy is resolved once with hipGetSymbolAddress and that pointer is passed on.
+constexpr size_t NOALLOC_HEAD_WORDS = 64;
+constexpr size_t NOALLOC_WORDS =
+    NOALLOC_HEAD_WORDS + LRT_NOALLOC_MAX_RAYS + size_t(LRT_NOALLOC_MAX_RAYS) * (STACK_SIZE - 8);
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
+// Sets the queue header at the start of every allocation-free call (a kernel, not a memset, so a
+// captured call records kernels only).
+__global__ void k_queue_init(TailQueue* queue, uint32_t* list) {
+  if (threadIdx.x == 0) {
+    queue->count = 0;
+    queue->next = 0;
+    queue->list = list;
+  }
+}
 
 // Clears the tail queue when a call's second pass could not be launched (a kernel rather than a
 // memset, so that nothing but kernels enters a graph captured from the stream).
@@ -66,6 +89,41 @@ hipError_t launch_trace(const SceneView& s, const TraceArgs& a, hipStream_t stre
     (void)hipGetLastError();
   }
   return e;
+}
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(int device, uint32_t** out) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    // the caller made `device` current; its copy of the array has its own address
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
+hipError_t launch_trace_noalloc(const SceneView& s, TraceArgs a, int device, hipStream_t stream) {
+  if (a.n > LRT_NOALLOC_MAX_RAYS || a.counts != nullptr) return hipErrorInvalidValue;
+  uint32_t* base = nullptr;
+  hipError_t e = noalloc_scratch(device, &base);
+  if (e != hipSuccess) return e;
+  a.queue = reinterpret_cast<TailQueue*>(base);
+  uint32_t* list = base + NOALLOC_HEAD_WORDS;
+  a.spill = list + LRT_NOALLOC_MAX_RAYS;
+  if (a.budget != 0) {
+    hipLaunchKernelGGL(k_queue_init, dim3(1), dim3(64), 0, stream, a.queue, list);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return launch_trace(s, a, stream);
 }
 
 hipError_t launch_eye_rays(const EyeRayParams& prm, const float2* jitter, float4* pos, float4* dir,

@@ -111,6 +111,18 @@ lrt_status lrt_set_preset(lrt_scene* s, const void* preset);
 lrt_status lrt_trace_closest_dev(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,
                                  uint64_t n, void* stream);
 
+/* Allocation-free variant of lrt_trace_closest_dev. It has no counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments and the same hit records as
+ * lrt_trace_closest_dev; its tail queue and stack spill live in a fixed device array of the
+ * library (about 19 MB, one per GPU, held until the process ends) instead of the per-stream work
+ * buffer, so no call allocates. n may be at most LRT_NOALLOC_MAX_RAYS (LRT_ERR_INVALID_ARG
+ * otherwise); the current device must be the scene's. Calls that overlap on one GPU (on any
+ * streams or threads) share that array and race: serialise them. */
+#define LRT_NOALLOC_MAX_RAYS 65536u
+lrt_status lrt_trace_closest_noalloc_dev(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,
+                                         uint64_t n, void* stream);
+
 /* Same over host buffers (copies in, traces, copies out, synchronises). The device staging
  * buffer belongs to the scene and only grows: repeated calls of one size allocate nothing. */
 lrt_status lrt_trace_closest_host(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits,
