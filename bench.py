@@ -1,10 +1,14 @@
 #!/usr/bin/env python3
 """Ray-query benchmark on one MI355X.
 
-A step is one closest-hit launch over the workload's full ray batch, inputs resident in HBM.
-Prints one JSON line: the metric, the roofline of the dominant kernel (algorithmic bytes per step
-from traversal counters / step time, against 8 TB/s; HBM traffic from the committed counter
-summaries) and, on the trace workloads, the CPU oracle timed on the host cores.
+A step is one pass of a workload's hot path over its full ray batch, inputs resident in HBM: a
+closest-hit trace call (both passes), or one fused forward + adjoint of the differentiable brick
+set. The default run (--workload all) prints ONE JSON line whose headline is the closest-hit
+icosphere7_512 workload (512x512 primary rays on a 327,680-triangle mesh, the reference's
+closest-hit configuration) and which carries every other workload under "sub_results", each with
+its ms/step, Mrays/s, roofline (algorithmic bytes per step from traversal counters / step time,
+against 8 TB/s; measured HBM bytes per step from the committed counter summaries; the limiting
+stage named by the committed counter ratios) and the CPU oracle timed on the host cores.
 
 Workloads (synthetic, seeded):
   sdf_octree_d9_1024  64-sphere union, frame octree depth 9, 1024x1024 jittered primary rays
@@ -197,11 +201,48 @@ def sbs_batch_scenes(torch, g, n_scenes=64, res=128, views=4, threads=0, seed=10
         s.sbs_forward(gid, tparams, pos, dirs, target, stream=stream)
         items.append({"scene": s, "geom_id": gid, "params": params, "grad": torch.zeros_like(params),
                       "loss": torch.zeros(1, dtype=torch.float32, device="cuda"), "pos": pos, "dir": dirs,
-                      "target": target, "colour": torch.zeros_like(target), "grid": int(gr), "n_dist": int(nd)})
+                      "target": target, "colour": torch.zeros_like(target), "grid": int(gr), "n_dist": int(nd),
+                      "bricks": int(nodes.size)})
     return items
 
 
-def run_batch(args, torch, g):
+def timed(torch, stream, fn, steps, warmup):
+    """Mean ms per call of fn over `steps` calls after `warmup`, HIP events on `stream`."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(steps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def sbs_traversal_bytes(torch, g, scene, pos, dirs, sh):
+    """Traversal bytes (brick headers, voxels stepped, nodes) and hit count of one brick-set trace
+    of the given rays, from the instrumented launch's per-ray counters (untimed)."""
+    n = pos.shape[0]
+    hits = torch.zeros((n, 8), dtype=torch.int32, device="cuda")
+    counts = torch.zeros((n, 8), dtype=torch.int32, device="cuda")
+    scene.trace_closest_counted(pos, dirs, hits, counts, stream=sh)
+    torch.cuda.synchronize()
+    c = counts.cpu().numpy().view(np.uint32).astype(np.float64)
+    hit_n = float((hits.cpu().numpy().view(g.HIT_DTYPE).reshape(-1)["primId"] != g.MISS).sum())
+    trav = B_FAT_NODE * c[:, 0].sum() + B_TLAS_NODE * c[:, 3].sum() + B_LEAF * c[:, 1].sum() + \
+        B_BRICK * c[:, 1].sum() + B_VOXEL * c[:, 5].sum() + B_INSTANCE * c[:, 4].sum()
+    return trav, hit_n
+
+
+# Per hit of the fused forward + L2 adjoint: 32 parameters read (8 distances + 24 albedo floats)
+# and 32 float atomics (read-modify-write, 8 B each).
+B_DIFF_HIT = 32 * 4 + 32 * 8
+# Per ray of the fused pass: ray in (32 B), target in (16 B), colour out (16 B).
+B_DIFF_RAY = 32 + 16 + 16
+
+
+def run_batch(args, torch, g, cpu_target_s):
     """Batched differentiable workload: one persistent launch per step vs. per-scene launches."""
     t0 = time.time()
     threads = min(16, len(os.sched_getaffinity(0)))
@@ -212,6 +253,17 @@ def run_batch(args, torch, g):
     n = sum(it["pos"].shape[0] for it in items)
     batch = g.SbsBatch(items)
 
+    # algorithmic bytes of one batched step: per item, the traversal of its rays (instrumented
+    # launch), ray I/O and the per-hit gradient scatter, plus the set-up kernels: gradient zeroing
+    # (4 B per parameter) and the occupancy masks (one pass over each brick's (bs+1)^3 value
+    # indices and values, 8 B each). The hit brick's second march is not counted.
+    alg_bytes, hit_n = 0.0, 0.0
+    for it in items:
+        trav, hn = sbs_traversal_bytes(torch, g, it["scene"], it["pos"], it["dir"], sh)
+        alg_bytes += it["pos"].shape[0] * B_DIFF_RAY + trav + hn * B_DIFF_HIT
+        alg_bytes += it["params"].numel() * 4 + it["bricks"] * 729 * 8
+        hit_n += hn
+
     def plain():
         for it in items:
             it["grad"].zero_()
@@ -220,18 +272,6 @@ def run_batch(args, torch, g):
             it["scene"].sbs_l2_backward(it["geom_id"], it["params"], it["pos"], it["dir"], it["target"], it["grad"],
                                         it["loss"], stream=sh)
 
-    def timed(fn, steps):
-        for _ in range(args.warmup):
-            fn()
-        torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        for _ in range(steps):
-            fn()
-        b.record(stream)
-        torch.cuda.synchronize()
-        return a.elapsed_time(b) / steps
-
     def plain_fused():
         for it in items:
             it["grad"].zero_()
@@ -239,9 +279,9 @@ def run_batch(args, torch, g):
             it["scene"].sbs_render_l2_backward(it["geom_id"], it["params"], it["pos"], it["dir"], it["target"],
                                                it["colour"], it["grad"], it["loss"], stream=sh)
 
-    ms_plain = timed(plain, max(3, args.steps // 5))
-    ms_plain_fused = timed(plain_fused, max(3, args.steps // 5))
-    ms_eager = timed(lambda: batch.run(sh), args.steps)
+    ms_plain = timed(torch, stream, plain, max(3, args.steps // 5), args.warmup)
+    ms_plain_fused = timed(torch, stream, plain_fused, max(3, args.steps // 5), args.warmup)
+    ms_eager = timed(torch, stream, lambda: batch.run(sh), args.steps, args.warmup)
     graph = torch.cuda.CUDAGraph()
     side = torch.cuda.Stream()
     side.wait_stream(stream)
@@ -251,23 +291,25 @@ def run_batch(args, torch, g):
         with torch.cuda.graph(graph, stream=side):
             batch.run(torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
-    ms = timed(graph.replay, args.steps)
-    hit = float(sum((it["colour"][:, 3] > 0).sum().item() for it in items))
+    ms = timed(torch, stream, graph.replay, args.steps, args.warmup)
     grids = [it["grid"] for it in items]
     desc = {"workload": "sbs_batch_64x4x128", "scenes": len(items), "views_per_scene": 4, "width": 128, "height": 128,
             "rays": int(n), "grids": {str(r): grids.count(r) for r in (32, 64, 128, 256)}, "brick": 8,
-            "hit_fraction": round(hit / n, 4), "scene_build_s": round(build_s, 1),
+            "hit_fraction": round(hit_n / n, 4), "scene_build_s": round(build_s, 1),
             "ms_batched_eager": round(ms_eager, 4), "ms_per_scene_launches": round(ms_plain, 4),
             "batched_vs_per_scene": round(ms_plain / ms, 2),
             "ms_per_scene_fused_launches": round(ms_plain_fused, 4),
             "batched_vs_per_scene_fused": round(ms_plain_fused / ms, 2),
-            "recompute_misses": int(sum(it["scene"].sbs_recompute_misses() for it in items))}
-    return {
-        "metric": METRIC, "value": round(n / (ms * 1e-3) / 1e6, 2), "unit": "Mrays/s (fwd+bwd, batched)", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": desc,
-        "roofline": None, "cpu_baseline": None,
-    }
+            "recompute_misses": int(sum(it["scene"].sbs_recompute_misses() for it in items)),
+            "algorithmic_bytes_per_ray": round(alg_bytes / n, 1)}
+    cpu = None
+    if cpu_target_s > 0:
+        # the oracle's forward walk (closest hit of every ray) over an evenly strided sample of
+        # each scene's rays: the CPU side of the forward half of the step
+        pos_np, dir_np = items[0]["pos"].cpu().numpy(), items[0]["dir"].cpu().numpy()
+        cpu = cpu_baseline([it["scene"] for it in items], pos_np, dir_np, threads, cpu_target_s,
+                           what="forward closest hit of the same rays (the oracle has no adjoint)")
+    return result("sbs_batch_64x4x128", n, ms, args, desc, alg_bytes, cpu, unit="Mrays/s (fwd+bwd, batched)")
 
 
 def copy_calibration(torch):
@@ -280,8 +322,8 @@ def copy_calibration(torch):
     del src, dst
 
 
-def run_diff(args, torch, g):
-    """Forward + adjoint steps of the differentiable brick-set workload (one JSON line)."""
+def run_diff(args, torch, g, cpu_target_s):
+    """Forward + adjoint steps of the differentiable brick-set workload."""
     import graphic_lab_task_amd.diff as gd
     t0 = time.time()
     threads = min(16, len(os.sched_getaffinity(0)))
@@ -308,18 +350,10 @@ def run_diff(args, torch, g):
     grad = torch.zeros_like(params)
     loss = torch.zeros(1, dtype=torch.float32, device="cuda")
 
-    # traversal counters of the same rays (untimed) for the algorithmic-bytes model
-    hits = torch.zeros((n, 8), dtype=torch.int32, device="cuda")
-    counts = torch.zeros((n, 8), dtype=torch.int32, device="cuda")
-    scene.trace_closest_counted(pos, dirs, hits, counts, stream=sh)
-    torch.cuda.synchronize()
-    c = counts.cpu().numpy().view(np.uint32).astype(np.float64)
-    hit_n = float((hits.cpu().numpy().view(g.HIT_DTYPE).reshape(-1)["primId"] != g.MISS).sum())
-    trav = B_FAT_NODE * c[:, 0].sum() + B_TLAS_NODE * c[:, 3].sum() + B_LEAF * c[:, 1].sum() + \
-        B_BRICK * c[:, 1].sum() + B_VOXEL * c[:, 5].sum() + B_INSTANCE * c[:, 4].sum()
-    # one fused pass: ray (32 B) and target (16 B) in, colour (16 B) out, one traversal, the hit
-    # brick re-marched (bounded by one brick's voxels), 32 parameters read and 32 atomics per hit
-    alg_bytes = n * (32 + 16 + 16) + trav + hit_n * (B_VOXEL * 8 + 32 * 4 + 32 * 8)
+    # one fused pass: ray I/O, one traversal (the hit voxel is recorded by the walk's own brick
+    # march, not marched again), 32 parameters read and 32 atomics per hit
+    trav, hit_n = sbs_traversal_bytes(torch, g, scene, pos, dirs, sh)
+    alg_bytes = n * B_DIFF_RAY + trav + hit_n * B_DIFF_HIT
 
     if args.copy_calibration:
         copy_calibration(torch)
@@ -330,31 +364,17 @@ def run_diff(args, torch, g):
         # forward colour and L2 adjoint in one traversal per ray
         scene.sbs_render_l2_backward(gid, params, pos, dirs, target, colour, grad, loss, stream=sh)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(args.steps):
-        step()
-    end.record(stream)
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(end) / args.steps
-    mrays = n / (ms * 1e-3) / 1e6
-    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    ms = timed(torch, stream, step, args.steps, args.warmup)
     desc = {"workload": "sbs_diff_256_8x256", "virtual_grid": 256, "brick": 8, "bricks": int(n_bricks),
             "distance_values": int(nd), "views": 8, "width": 256, "height": 256, "rays": int(n),
             "hit_fraction": round(hit_n / n, 4), "scene_build_s": round(build_s, 2),
             "recompute_misses": scene.sbs_recompute_misses(), "loss": float(loss.item()),
             "algorithmic_bytes_per_ray": round(alg_bytes / n, 1)}
-    return {
-        "metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s (fwd+bwd)", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": desc,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("sbs_diff_256_8x256")},
-        "cpu_baseline": None,
-    }
+    cpu = None
+    if cpu_target_s > 0:
+        cpu = cpu_baseline([scene], pos.cpu().numpy(), dirs.cpu().numpy(), threads, cpu_target_s,
+                           what="forward closest hit of the same rays (the oracle has no adjoint)")
+    return result("sbs_diff_256_8x256", n, ms, args, desc, alg_bytes, cpu, unit="Mrays/s (fwd+bwd)")
 
 
 def algorithmic_bytes(cnt, hits, kind):
@@ -377,30 +397,33 @@ def strided(a, k):
     return np.ascontiguousarray(a[::step][:k])
 
 
-def cpu_baseline(scene, pos, dir, threads):
-    """Oracle (reference flags) on the host cores over an evenly strided sample of the same rays;
-    median of 5 after 1 warm-up; plus a 1-thread run (the reference loop is serial)."""
+def cpu_baseline(scenes, pos, dir, threads, target_s=20.0, what="closest hit of the same rays"):
+    """Oracle (reference flags) on the host cores over an evenly strided sample of the same rays
+    (of every scene, when several share one ray set); median of 5 after 1 warm-up; plus a 1-thread
+    run (the reference loop is serial). The sample is sized for about target_s seconds in total."""
     import oracle
     if not oracle.available("fma"):
         return None
-    sc = oracle.Scene.from_scene(scene)
-    n = pos.shape[0]
+    scs = [oracle.Scene.from_scene(s) for s in scenes]
+    n = pos.shape[0] * len(scs)
 
     def run(p, d, th):
         t = time.perf_counter()
-        oracle.trace_closest(sc, p, d, threads=th, flavor="fma")
+        for sc in scs:
+            oracle.trace_closest(sc, p, d, threads=th, flavor="fma")
         return time.perf_counter() - t
 
-    probe = min(n, 65536)
+    probe = min(pos.shape[0], max(1024, 65536 // len(scs)))
     first = run(strided(pos, probe), strided(dir, probe), threads)
-    est_full = first * n / probe
-    sample = n if est_full * 6 < 20.0 else max(probe, int(n * 20.0 / (6 * est_full)))
-    sample = min(sample, n)
-    p, d = strided(pos, sample), strided(dir, sample)
+    est_full = first * pos.shape[0] / probe
+    per_scene = pos.shape[0] if est_full * 6 < target_s else max(probe, int(pos.shape[0] * target_s / (6 * est_full)))
+    per_scene = min(per_scene, pos.shape[0])
+    p, d = strided(pos, per_scene), strided(dir, per_scene)
     run(p, d, threads)
     times = sorted(run(p, d, threads) for _ in range(5))
-    mrays = p.shape[0] / times[2] / 1e6
-    s1 = min(p.shape[0], 32768)
+    rays = p.shape[0] * len(scs)
+    mrays = rays / times[2] / 1e6
+    s1 = min(p.shape[0], max(256, 32768 // len(scs)))
     p1, d1 = strided(p, s1), strided(d, s1)
     run(p1, d1, 1)
     t1 = sorted(run(p1, d1, 1) for _ in range(3))[1]
@@ -413,11 +436,12 @@ def cpu_baseline(scene, pos, dir, threads):
                     break
     except OSError:
         pass
+    scenes_txt = f" in each of {len(scs)} scenes" if len(scs) > 1 else ""
     return {"value": round(mrays, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{p.shape[0]} of {n} rays of the same workload (every {max(1, n // p.shape[0])}th), median "
-                      f"of 5 after 1 warm-up, OpenMP dynamic(256); oracle built with the reference flags (-O3, FMA "
-                      f"contraction); 1-thread value on {p1.shape[0]} of them",
-            "value_1thread": round(p1.shape[0] / t1 / 1e6, 3), "cpu_model": cpu_model}
+            "sample": f"{p.shape[0]} of {pos.shape[0]} rays{scenes_txt} (every {max(1, pos.shape[0] // p.shape[0])}th), "
+                      f"{what}, median of 5 after 1 warm-up, OpenMP dynamic(256); oracle built with the reference "
+                      f"flags (-O3, FMA contraction); 1-thread value on {p1.shape[0] * len(scs)} rays",
+            "value_1thread": round(p1.shape[0] * len(scs) / t1 / 1e6, 3), "cpu_model": cpu_model}
 
 
 def load_traffic(workload):
@@ -432,46 +456,67 @@ def load_traffic(workload):
     return None
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="sdf_octree_d9_1024",
-                    choices=["sdf_octree_d9_1024", "sdf_multi_d9_1024", "icosphere7_512", "tlas_4096x32_4m",
-                             "sbs_diff_256_8x256",
-                             "sbs_batch_64x4x128"])
-    ap.add_argument("--block", type=int, default=256)
-    ap.add_argument("--lds-stack", type=int, default=16)
-    ap.add_argument("--xcd-remap", type=int, default=1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the given launch configuration")
-    ap.add_argument("--budget", type=int, default=64, help="tail step budget when not autotuning")
-    ap.add_argument("--tail-waves", type=int, default=1024, help="tail waves when not autotuning")
-    ap.add_argument("--ray-interleave", type=int, default=32,
-                    help="pass-1 lanes per interleaved group when not autotuning (0: thread order)")
-    ap.add_argument("--leaf-postpone", type=int, default=-1, choices=[-1, 0, 1],
-                    help="triangle leaf postponement when not autotuning (-1: on for one instance)")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a 256 MiB device copy (reference kernel for the HBM byte counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); sharding the rays or scenes over "
-              f"several GPUs is designed but not built", file=sys.stderr)
-        sys.exit(2)
+def load_counters(workload):
+    """Derived ratios of the committed counter summary (profiles/counters_<workload>.txt)."""
+    path = os.path.join(ROOT, "profiles", f"counters_{workload}.txt")
+    out = {}
+    if not os.path.exists(path):
+        return out
+    with open(path) as fh:
+        derived = False
+        for line in fh:
+            if line.startswith("derived:"):
+                derived = True
+                continue
+            if derived and line.strip():
+                key, _, val = line.rstrip().rpartition(" ")
+                try:
+                    out[key.strip()] = float(val)
+                except ValueError:
+                    pass
+    return out
 
-    import torch
-    import graphic_lab_task_amd as g
 
+def limiter(workload, ms, traffic):
+    """What the counters say bounds the kernel. The only roofline that applies is HBM (no MFMA:
+    traversal is not a contraction); `limiter` names the stage closest to saturation."""
+    hbm = None if traffic is None else traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    cnt = load_counters(workload)
+    ta = cnt.get("TA busy / active cycles (8 XCDs)")
+    if hbm is not None and hbm > 0.6:
+        name = "hbm bandwidth"
+    elif ta is not None and ta > 0.5:
+        name = "texture addresser (address expansion of divergent gathers)"
+    else:
+        name = "latency (dependent node loads on the critical path of the slowest waves)"
+    out = {"limiter": name}
+    if hbm is not None:
+        out["hbm_measured_frac"] = round(hbm, 4)
+    if cnt.get("L2 hit rate") is not None:
+        out["l2_hit_rate"] = cnt["L2 hit rate"]
+    if ta is not None:
+        out["ta_busy"] = ta
+    return out
+
+
+def result(workload, n, ms, args, desc, alg_bytes, cpu, unit="Mrays/s"):
+    mrays = n / (ms * 1e-3) / 1e6
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    traffic = load_traffic(workload)
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
+    roof.update(limiter(workload, ms, traffic))
+    out = {"value": round(mrays, 2), "unit": unit, "ms_per_step": round(ms, 4), "steps": args.steps,
+           "warmup": args.warmup, "dtype": "f32", "config": desc, "roofline": roof, "cpu_baseline": cpu}
+    if cpu:
+        out["speedup_vs_cpu"] = round(mrays / cpu["value"], 1)
+    return out
+
+
+def run_trace(name, args, torch, g, cpu_target_s):
+    """A closest-hit workload: scene, rays, counted launch, autotune, timed steps, CPU oracle."""
     threads = min(16, len(os.sched_getaffinity(0)))
-    torch.cuda.set_device(0)
-    if args.workload == "sbs_diff_256_8x256":
-        print(json.dumps(run_diff(args, torch, g)))
-        return
-    if args.workload == "sbs_batch_64x4x128":
-        print(json.dumps(run_batch(args, torch, g)))
-        return
-    scene, w, h, jitter, desc, kind, build_s = build_workload(args.workload, threads)
+    scene, w, h, jitter, desc, kind, build_s = build_workload(name, threads)
     scene.set_launch_config(args.block, args.lds_stack, bool(args.xcd_remap))
     scene.set_tail_config(args.budget, args.tail_waves)
     scene.set_leaf_postponement(args.leaf_postpone)
@@ -495,6 +540,7 @@ def main():
     scene.trace_closest_counted(pos, dirs, hits, counts, stream=sh)
     torch.cuda.synchronize()
     cnt = counts.cpu().numpy().view(np.uint32)
+    del counts
     hits_np = hits.cpu().numpy().view(g.HIT_DTYPE).reshape(-1)
     alg_bytes = algorithmic_bytes(cnt, hits_np, kind)
 
@@ -504,24 +550,26 @@ def main():
     if args.copy_calibration:
         copy_calibration(torch)
 
-    for _ in range(args.warmup):
-        scene.trace_closest(pos, dirs, hits, stream=sh)
+    ms_eager = timed(torch, stream, lambda: scene.trace_closest(pos, dirs, hits, stream=sh), args.steps, args.warmup)
+    # the step as a captured HIP graph (both passes; the library's work buffer for the capture
+    # stream is made by an eager call first, so the captured call allocates nothing)
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(stream)
+    with torch.cuda.stream(side):
+        scene.trace_closest(pos, dirs, hits, stream=side.cuda_stream)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph, stream=side):
+            scene.trace_closest(pos, dirs, hits, stream=torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(args.steps):
-        scene.trace_closest(pos, dirs, hits, stream=sh)
-    end.record(stream)
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(end) / args.steps
-    mrays = n / (ms * 1e-3) / 1e6
-    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    ms = timed(torch, stream, graph.replay, args.steps, args.warmup)
+    # records of the timed (tuned, graph-replayed) launch against the instrumented single-pass one
+    replay = hits.cpu().numpy().view(np.uint8).reshape(n, 32)
+    differ = int((replay != hits_np.view(np.uint8).reshape(n, 32)).any(axis=1).sum())
     overflow = scene.stack_overflow_count()
-
     cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(scene, pos.cpu().numpy(), dirs.cpu().numpy(), threads)
-
+    if cpu_target_s > 0:
+        cpu = cpu_baseline([scene], pos.cpu().numpy(), dirs.cpu().numpy(), threads, cpu_target_s)
     launch = ({k: tune[k] for k in ("block", "lds_stack", "xcd_remap", "ray_interleave", "budget", "tail_waves",
                                     "leaf_postpone")}
               if tune else
@@ -530,19 +578,81 @@ def main():
                "leaf_postpone": args.leaf_postpone})
     desc.update({"hit_fraction": round(float((hits_np["primId"] != g.MISS).mean()), 4),
                  "scene_build_s": round(build_s, 2), "launch": launch, "autotuned": tune is not None,
-                 "stack_overflow_rays": overflow,
+                 "stack_overflow_rays": overflow, "ms_eager": round(ms_eager, 4), "timed": "hip graph replay",
+                 "records_differing_from_single_pass": differ,
                  "mean_fat_nodes_per_ray": round(float(cnt[:, 0].mean()), 2),
                  "algorithmic_bytes_per_ray": round(alg_bytes / n, 1)})
+    return result(name, n, ms, args, desc, alg_bytes, cpu)
+
+
+TRACE_WORKLOADS = ["icosphere7_512", "sdf_octree_d9_1024", "sdf_multi_d9_1024", "tlas_4096x32_4m"]
+ALL_WORKLOADS = TRACE_WORKLOADS + ["sbs_diff_256_8x256", "sbs_batch_64x4x128"]
+
+
+def run_one(name, args, torch, g, cpu_target_s):
+    if name == "sbs_diff_256_8x256":
+        return run_diff(args, torch, g, cpu_target_s)
+    if name == "sbs_batch_64x4x128":
+        return run_batch(args, torch, g, cpu_target_s)
+    return run_trace(name, args, torch, g, cpu_target_s)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="all", choices=["all"] + ALL_WORKLOADS,
+                    help="all: the closest-hit headline (icosphere7_512) with every other workload as a "
+                         "sub-result of the same line")
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--lds-stack", type=int, default=16)
+    ap.add_argument("--xcd-remap", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="approximate CPU time of each workload's oracle baseline (all cores)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the given launch configuration")
+    ap.add_argument("--budget", type=int, default=64, help="tail step budget when not autotuning")
+    ap.add_argument("--tail-waves", type=int, default=1024, help="tail waves when not autotuning")
+    ap.add_argument("--ray-interleave", type=int, default=32,
+                    help="pass-1 lanes per interleaved group when not autotuning (0: thread order)")
+    ap.add_argument("--leaf-postpone", type=int, default=-1, choices=[-1, 0, 1],
+                    help="triangle leaf postponement when not autotuning (-1: on for one instance)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device copy (reference kernel for the HBM byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); sharding the rays or scenes over "
+              f"several GPUs is designed but not built", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import graphic_lab_task_amd as g
+
+    torch.cuda.set_device(0)
+    cpu_s = 0.0 if args.no_cpu_baseline else args.cpu_seconds
+    names = ALL_WORKLOADS if args.workload == "all" else [args.workload]
+    head_name = names[0]
+    t0 = time.time()
+    head = run_one(head_name, args, torch, g, cpu_s)
+    print(f"# {head_name}: {head['value']} {head['unit']} ({time.time() - t0:.0f} s)", file=sys.stderr, flush=True)
+    subs = {}
+    for name in names[1:]:
+        torch.cuda.empty_cache()
+        t1 = time.time()
+        subs[name] = run_one(name, args, torch, g, cpu_s)
+        print(f"# {name}: {subs[name]['value']} {subs[name]['unit']} ({time.time() - t1:.0f} s)", file=sys.stderr,
+              flush=True)
     out = {
-        "metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": desc,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(args.workload)},
-        "cpu_baseline": cpu,
+        "metric": METRIC, "value": head["value"], "unit": head["unit"], "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": head["config"],
+        "roofline": head["roofline"], "cpu_baseline": head["cpu_baseline"],
     }
-    if cpu:
-        out["speedup_vs_cpu"] = round(mrays / cpu["value"], 1)
+    if "speedup_vs_cpu" in head:
+        out["speedup_vs_cpu"] = head["speedup_vs_cpu"]
+    if subs:
+        out["sub_results"] = subs
     print(json.dumps(out))
 
 
