@@ -42,3 +42,81 @@ def host_scene(g, build):
         if e.code != 5:
             raise
     return s
+
+
+MISS = 0xFFFFFFFF
+
+
+def decode_normal(e):
+    """Inverse of the octahedral encoding of encode_normal (utils/raytracing/intersection_common.h:96-101;
+    Cigolle et al., JCGT 2014): (n, 2) float32 codes -> (n, 3) float64 unit vectors."""
+    e = np.asarray(e, np.float64)
+    x, y = e[:, 0].copy(), e[:, 1].copy()
+    z = 1.0 - np.abs(x) - np.abs(y)
+    neg = z < 0
+    sx, sy = np.where(x >= 0, 1.0, -1.0), np.where(y >= 0, 1.0, -1.0)
+    x2, y2 = (1.0 - np.abs(e[:, 1])) * sx, (1.0 - np.abs(e[:, 0])) * sy
+    x, y = np.where(neg, x2, x), np.where(neg, y2, y)
+    v = np.stack([x, y, z], 1)
+    return v / np.maximum(np.linalg.norm(v, axis=1, keepdims=True), 1e-300)
+
+
+def sdf_check(got, want, pos, dirs, extent=2.0, angle_bar=1e-3, colour=False, exact_normals=False):
+    """Row-by-row check of SDF hit records against the oracle at the path's bar, returning the
+    count of rows that break each rule (all must be zero) plus summary figures:
+      hit_flip           hit on one side, miss on the other
+      id_beyond_tie      primId/instId/geomId differ although the two t are not within 1e-5*t
+      t_rel_gt_1e-5      both hit, t differs by more than 1e-5 relative
+      point_over_bar     hit points farther apart than 1e-4 * scene extent
+      normal_bits        (exact_normals=True) t and ids bit-identical but the encoded normal
+                         (coords[2..3]) is not
+      normal_angle       same ids: decoded normals more than angle_bar radians apart
+      colour_bits        (colour=True) t bit-identical but coords[0..1] (colour / texcoords) differ
+    The device's acosf/cosf/powf differ from glibc's in the last ulps, and only the analytic cubic
+    calls them: its voxel-local root, and with it the hit point where the normal's central
+    differences are taken, can move by an ulp even where the world-space t rounds to the same
+    float, so under that mode normals are held to the angle bar; the other modes are exact.""" 
+    gh, wh = got["primId"] != MISS, want["primId"] != MISS
+    both = gh & wh
+    tg, tw = got["t"].astype(np.float64), want["t"].astype(np.float64)
+    rel = np.zeros(got.size)
+    rel[both] = np.abs(tg[both] - tw[both]) / np.maximum(np.abs(tw[both]), 1e-30)
+    ids_differ = (got["primId"] != want["primId"]) | (got["instId"] != want["instId"]) | \
+        (got["geomId"] != want["geomId"])
+    pg = pos[:, :3].astype(np.float64) + tg[:, None] * dirs[:, :3]
+    pw = pos[:, :3].astype(np.float64) + tw[:, None] * dirs[:, :3]
+    dp = np.where(both, np.linalg.norm(pg - pw, axis=1), 0.0)
+    same = both & (got["t"].view(np.uint32) == want["t"].view(np.uint32)) & ~ids_differ
+    nb = same & np.any(got["coords"][:, 2:].view(np.uint32) != want["coords"][:, 2:].view(np.uint32), axis=1)
+    if not exact_normals:
+        nb[:] = False
+    other = both & ~ids_differ
+    ang = np.zeros(got.size)
+    if other.any():
+        ng, nw = decode_normal(got["coords"][other, 2:]), decode_normal(want["coords"][other, 2:])
+        ang[other] = np.arccos(np.clip(np.sum(ng * nw, axis=1), -1.0, 1.0))
+    r = {"n": int(got.size), "hits": int(both.sum()), "hit_flip": int(np.sum(gh != wh)),
+         "id_beyond_tie": int(np.sum(both & ids_differ & (rel > 1e-5))),
+         "t_rel_gt_1e-5": int(np.sum(rel > 1e-5)), "point_over_bar": int(np.sum(dp > 1e-4 * extent)),
+         "normal_bits": int(nb.sum()), "normal_angle": int(np.sum(ang > angle_bar)),
+         "t_bit_identical": round(float(same.sum()) / max(int(both.sum()), 1), 4),
+         "t_rel_max": float(rel.max()) if rel.size else 0.0, "point_err_max": float(dp.max()) if dp.size else 0.0,
+         "normal_angle_max": float(ang.max()) if ang.size else 0.0}
+    if colour:
+        r["colour_bits"] = int(np.sum(same & np.any(got["coords"][:, :2].view(np.uint32) !=
+                                                      want["coords"][:, :2].view(np.uint32), axis=1)))
+    bad = (gh != wh) | (both & ids_differ & (rel > 1e-5)) | (rel > 1e-5) | (dp > 1e-4 * extent) | nb | (ang > angle_bar)
+    r["bad_rows"] = [int(i) for i in np.flatnonzero(bad)[:8]]
+    return r
+
+
+SDF_RULES = ("hit_flip", "id_beyond_tie", "t_rel_gt_1e-5", "point_over_bar", "normal_bits", "normal_angle",
+             "colour_bits")
+
+
+def assert_sdf_parity(r, got=None, want=None):
+    broken = {k: r[k] for k in SDF_RULES if r.get(k, 0) != 0}
+    detail = ""
+    if broken and got is not None:
+        detail = "; rows " + "; ".join(f"{i}: gpu {got[i]} oracle {want[i]}" for i in r["bad_rows"][:4])
+    assert not broken, f"SDF parity rules broken: {broken} ({r}){detail}"
