@@ -160,6 +160,13 @@ def lib() -> ctypes.CDLL:
             "lrt_sdf_spheres_sbs": ([vp, sz, u32, vp, u64, i32, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
             "lrt_sdf_spheres_sbs_indexed": ([vp, sz, u32, u32, u64, i32, ctypes.POINTER(_Buffer),
                                              ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer), pu32], i32),
+            "lrt_load_sdf_frame_octree": ([cp, ctypes.POINTER(_Buffer)], i32),
+            "lrt_save_sdf_frame_octree": ([cp, vp, sz], i32),
+            "lrt_load_sdf_sbs": ([cp, vp, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)],
+                                 i32),
+            "lrt_save_sdf_sbs": ([cp, vp, vp, sz, vp, sz, vp, sz], i32),
+            "lrt_load_sdf_multi_octree": ([cp, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
+            "lrt_save_sdf_multi_octree": ([cp, vp, sz, vp, sz], i32),
         }
         for name, (args, res) in sigs.items():
             fn = getattr(L, name)
@@ -180,7 +187,9 @@ def exported_symbols():
             "lrt_sbs_batch_create", "lrt_sbs_batch_run", "lrt_sbs_batch_destroy",
             "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
             "lrt_camera_inverse", "lrt_gen_icosphere", "lrt_gen_geodesic", "lrt_sdf_spheres_frame_octree",
-            "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed"]
+            "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed", "lrt_load_sdf_frame_octree",
+            "lrt_save_sdf_frame_octree", "lrt_load_sdf_sbs", "lrt_save_sdf_sbs", "lrt_load_sdf_multi_octree",
+            "lrt_save_sdf_multi_octree"]
 
 
 def _check(code: int):
@@ -528,3 +537,45 @@ def sdf_spheres_sbs_indexed(spheres: np.ndarray, depth: int, brick_size: int, al
     h["brick_size"], h["brick_pad"], h["bytes_per_value"] = brick_size, 0, 4
     h["aux_data"] = SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F
     return h, _take(nb, SBS_NODE_DTYPE), _take(vb, np.uint32), _take(fb, np.float32), nd.value
+
+
+# ---- reference-format SDF scene files (sdf/simple/sdf_simple.cpp, sdf/multi/sdf_multi.cpp) ----
+def load_sdf_frame_octree(path: str) -> np.ndarray:
+    """load_sdf_frame_octree (sdf/simple/sdf_simple.cpp:58-66): SdfFrameOctreeNode array."""
+    b = _Buffer()
+    _check(lib().lrt_load_sdf_frame_octree(os.fsencode(path), ctypes.byref(b)))
+    return _take(b, FRAME_NODE_DTYPE)
+
+
+def save_sdf_frame_octree(path: str, nodes: np.ndarray):
+    n = np.ascontiguousarray(nodes, dtype=FRAME_NODE_DTYPE)
+    _check(lib().lrt_save_sdf_frame_octree(os.fsencode(path), _ptr(n), n.size))
+
+
+def load_sdf_sbs(path: str):
+    """load_sdf_SBS (sdf/simple/sdf_simple.cpp:124-142): (header, nodes, values uint32, values_f float32)."""
+    h = np.zeros((), SBS_HEADER_DTYPE)
+    bn, bv, bf = _Buffer(), _Buffer(), _Buffer()
+    _check(lib().lrt_load_sdf_sbs(os.fsencode(path), _ptr(h), ctypes.byref(bn), ctypes.byref(bv), ctypes.byref(bf)))
+    return h, _take(bn, SBS_NODE_DTYPE), _take(bv, np.uint32), _take(bf, np.float32)
+
+
+def save_sdf_sbs(path: str, header, nodes, values, values_f=None):
+    h = np.ascontiguousarray(header, dtype=SBS_HEADER_DTYPE).reshape(())
+    n = np.ascontiguousarray(nodes, dtype=SBS_NODE_DTYPE)
+    v = np.ascontiguousarray(values, dtype=np.uint32)
+    f = np.ascontiguousarray(values_f if values_f is not None else np.zeros(0), dtype=np.float32)
+    _check(lib().lrt_save_sdf_sbs(os.fsencode(path), _ptr(h), _ptr(n), n.size, _ptr(v), v.size, _ptr(f), f.size))
+
+
+def load_sdf_multi_octree(path: str):
+    """load_sdf_multi_octree (sdf/multi/sdf_multi.cpp:21-35): (nodes, values float32)."""
+    bn, bv = _Buffer(), _Buffer()
+    _check(lib().lrt_load_sdf_multi_octree(os.fsencode(path), ctypes.byref(bn), ctypes.byref(bv)))
+    return _take(bn, MULTI_NODE_DTYPE), _take(bv, np.float32)
+
+
+def save_sdf_multi_octree(path: str, nodes, values):
+    n = np.ascontiguousarray(nodes, dtype=MULTI_NODE_DTYPE)
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    _check(lib().lrt_save_sdf_multi_octree(os.fsencode(path), _ptr(n), n.size, _ptr(v), v.size))

@@ -198,6 +198,11 @@ lrt_status sbs_diff_common(lrt_scene* s, uint32_t geom_id, const float* params, 
 
 }  // namespace
 
+namespace lrt {
+// Error message of the calling thread for the entry points outside this file (lrt_build.h).
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace lrt
+
 extern "C" {
 
 const char* lrt_last_error(void) { return g_err.c_str(); }

@@ -9,6 +9,10 @@
 #include "gen.h"
 #include "host_math.h"
 
+namespace lrt {
+void set_last_error(const std::string& msg);
+}
+
 namespace {
 
 thread_local std::string g_gen_err;
@@ -28,6 +32,7 @@ lrt_status guarded(F&& f) {
     return f();
   } catch (const std::exception& e) {
     g_gen_err = e.what();
+    lrt::set_last_error(g_gen_err);
     return LRT_ERR_INVALID_ARG;
   }
 }

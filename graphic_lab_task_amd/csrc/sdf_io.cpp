@@ -1,0 +1,185 @@
+// Reference-format SDF scene files (the binary files the reference's save_sdf_* functions write):
+// extern "C" loaders and savers declared in include/lrt_build.h. The loaders check every count
+// against the file size and fail with a message instead of reading past the end.
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lrt_build.h"
+#include "lrt_types.h"
+
+namespace lrt {
+void set_last_error(const std::string& msg);
+}
+
+namespace {
+
+struct File {
+  std::FILE* f = nullptr;
+  File(const char* path, const char* mode) : f(path ? std::fopen(path, mode) : nullptr) {
+    if (!f) throw std::runtime_error(std::string("cannot open '") + (path ? path : "(null)") + "'");
+  }
+  ~File() {
+    if (f) std::fclose(f);
+  }
+  long size() {
+    std::fseek(f, 0, SEEK_END);
+    const long n = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    return n;
+  }
+};
+
+// Reads `bytes` bytes; `left` is what the file still holds from the current position.
+void take(File& fl, void* dst, size_t bytes, size_t& left, const char* what) {
+  if (bytes > left) throw std::runtime_error(std::string("file ends inside ") + what);
+  if (bytes && std::fread(dst, 1, bytes, fl.f) != bytes) throw std::runtime_error(std::string("read error in ") + what);
+  left -= bytes;
+}
+
+void put(File& fl, const void* src, size_t bytes) {
+  if (bytes && std::fwrite(src, 1, bytes, fl.f) != bytes) throw std::runtime_error("write error");
+}
+
+lrt_status to_buf(const std::vector<char>& v, lrt_buffer* out) {
+  out->bytes = v.size();
+  out->data = std::malloc(v.size() ? v.size() : 1);
+  if (!out->data) throw std::runtime_error("out of host memory");
+  if (!v.empty()) std::memcpy(out->data, v.data(), v.size());
+  return LRT_OK;
+}
+
+// u32 count then count records of `rec` bytes
+std::vector<char> counted(File& fl, size_t rec, size_t& left, const char* what) {
+  uint32_t n = 0;
+  take(fl, &n, 4, left, what);
+  if (size_t(n) * rec > left) throw std::runtime_error(std::string(what) + ": count " + std::to_string(n) +
+                                                       " exceeds the file");
+  std::vector<char> v(size_t(n) * rec);
+  take(fl, v.data(), v.size(), left, what);
+  return v;
+}
+
+template <class F>
+lrt_status io_guarded(F&& f) {
+  try {
+    lrt::set_last_error("");
+    return f();
+  } catch (const std::exception& e) {
+    lrt::set_last_error(e.what());
+    return LRT_ERR_INVALID_ARG;
+  }
+}
+
+void check_u32(size_t n, const char* what) {
+  if (n > 0xFFFFFFFFull) throw std::runtime_error(std::string(what) + ": too many records for the file format");
+}
+
+}  // namespace
+
+extern "C" {
+
+lrt_status lrt_load_sdf_frame_octree(const char* path, lrt_buffer* nodes) {
+  return io_guarded([&]() -> lrt_status {
+    if (!nodes) throw std::runtime_error("null output buffer");
+    File fl(path, "rb");
+    size_t left = size_t(fl.size());
+    std::vector<char> v = counted(fl, sizeof(lrt::SdfFrameOctreeNode), left, "frame-octree nodes");
+    return to_buf(v, nodes);
+  });
+}
+
+lrt_status lrt_save_sdf_frame_octree(const char* path, const void* nodes, size_t n_nodes) {
+  return io_guarded([&]() -> lrt_status {
+    if (!nodes && n_nodes) throw std::runtime_error("null nodes");
+    check_u32(n_nodes, "frame-octree nodes");
+    File fl(path, "wb");
+    const uint32_t n = uint32_t(n_nodes);
+    put(fl, &n, 4);
+    put(fl, nodes, n_nodes * sizeof(lrt::SdfFrameOctreeNode));
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_load_sdf_sbs(const char* path, void* header, lrt_buffer* nodes, lrt_buffer* values,
+                            lrt_buffer* values_f) {
+  return io_guarded([&]() -> lrt_status {
+    if (!header || !nodes || !values || !values_f) throw std::runtime_error("null output");
+    File fl(path, "rb");
+    size_t left = size_t(fl.size());
+    take(fl, header, sizeof(lrt::SdfSBSHeader), left, "brick-set header");
+    std::vector<char> nd = counted(fl, sizeof(lrt::SdfSBSNode), left, "brick-set nodes");
+    std::vector<char> vals = counted(fl, 4, left, "brick-set values");
+    // files written before the fp32 block existed end here (the reference reads no count then)
+    std::vector<char> vf;
+    if (left > 0) vf = counted(fl, 4, left, "brick-set fp32 values");
+    if (left != 0) throw std::runtime_error("brick-set file: " + std::to_string(left) + " bytes after the last block");
+    to_buf(nd, nodes);
+    to_buf(vals, values);
+    return to_buf(vf, values_f);
+  });
+}
+
+lrt_status lrt_save_sdf_sbs(const char* path, const void* header, const void* nodes, size_t n_nodes,
+                            const uint32_t* values, size_t n_values, const float* values_f, size_t n_values_f) {
+  return io_guarded([&]() -> lrt_status {
+    if (!header || (!nodes && n_nodes) || (!values && n_values) || (!values_f && n_values_f))
+      throw std::runtime_error("null input");
+    check_u32(n_nodes, "brick-set nodes");
+    check_u32(n_values, "brick-set values");
+    check_u32(n_values_f, "brick-set fp32 values");
+    File fl(path, "wb");
+    put(fl, header, sizeof(lrt::SdfSBSHeader));
+    uint32_t n = uint32_t(n_nodes);
+    put(fl, &n, 4);
+    put(fl, nodes, n_nodes * sizeof(lrt::SdfSBSNode));
+    n = uint32_t(n_values);
+    put(fl, &n, 4);
+    put(fl, values, n_values * 4);
+    n = uint32_t(n_values_f);
+    put(fl, &n, 4);
+    put(fl, values_f, n_values_f * 4);
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_load_sdf_multi_octree(const char* path, lrt_buffer* nodes, lrt_buffer* values) {
+  return io_guarded([&]() -> lrt_status {
+    if (!nodes || !values) throw std::runtime_error("null output buffer");
+    File fl(path, "rb");
+    size_t left = size_t(fl.size());
+    uint32_t nn = 0, nv = 0;
+    take(fl, &nn, 4, left, "multi-octree node count");
+    take(fl, &nv, 4, left, "multi-octree value count");
+    if (size_t(nn) * sizeof(lrt::SdfMultiOctreeNode) + size_t(nv) * 4 != left)
+      throw std::runtime_error("multi-octree file: counts " + std::to_string(nn) + " / " + std::to_string(nv) +
+                               " do not match the file size");
+    std::vector<char> nd(size_t(nn) * sizeof(lrt::SdfMultiOctreeNode)), vals(size_t(nv) * 4);
+    take(fl, nd.data(), nd.size(), left, "multi-octree nodes");
+    take(fl, vals.data(), vals.size(), left, "multi-octree values");
+    to_buf(nd, nodes);
+    return to_buf(vals, values);
+  });
+}
+
+lrt_status lrt_save_sdf_multi_octree(const char* path, const void* nodes, size_t n_nodes, const float* values,
+                                     size_t n_values) {
+  return io_guarded([&]() -> lrt_status {
+    if ((!nodes && n_nodes) || (!values && n_values)) throw std::runtime_error("null input");
+    check_u32(n_nodes, "multi-octree nodes");
+    check_u32(n_values, "multi-octree values");
+    File fl(path, "wb");
+    uint32_t n = uint32_t(n_nodes), m = uint32_t(n_values);
+    put(fl, &n, 4);
+    put(fl, &m, 4);
+    put(fl, nodes, n_nodes * sizeof(lrt::SdfMultiOctreeNode));
+    put(fl, values, n_values * 4);
+    return LRT_OK;
+  });
+}
+
+}  // extern "C"

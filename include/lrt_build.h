@@ -67,6 +67,31 @@ lrt_status lrt_sdf_spheres_sbs_indexed(const float* spheres, size_t n_spheres, u
                                        uint64_t albedo_seed, int threads, lrt_buffer* nodes, lrt_buffer* values,
                                        lrt_buffer* values_f, uint32_t* n_distance_values);
 
+/* Reference-format SDF scene files. Each loader checks the counts against the file size and fails
+ * with LRT_ERR_INVALID_ARG and a message (the reference reads unchecked). Output arrays are
+ * lrt_buffers; the scene structs (36-B SdfFrameOctreeNode, 16-B SdfSBSNode / SdfMultiOctreeNode,
+ * 16-B SdfSBSHeader) are byte-identical to the reference's. */
+
+/* load_sdf_frame_octree / save_sdf_frame_octree (sdf/simple/sdf_simple.cpp:49-66):
+ * u32 count, then count SdfFrameOctreeNode. */
+lrt_status lrt_load_sdf_frame_octree(const char* path, lrt_buffer* nodes);
+lrt_status lrt_save_sdf_frame_octree(const char* path, const void* nodes, size_t n_nodes);
+
+/* load_sdf_SBS / save_sdf_SBS (sdf/simple/sdf_simple.cpp:96-142): SdfSBSHeader, u32 count +
+ * SdfSBSNode[count], u32 count + uint32 values, u32 count + float values_f. A file that ends
+ * after the uint32 values (written before the fp32 block existed, e.g. saves/test_10_sbs.bin)
+ * loads with an empty values_f. header: 16 bytes written by the loader. */
+lrt_status lrt_load_sdf_sbs(const char* path, void* header, lrt_buffer* nodes, lrt_buffer* values,
+                            lrt_buffer* values_f);
+lrt_status lrt_save_sdf_sbs(const char* path, const void* header, const void* nodes, size_t n_nodes,
+                            const uint32_t* values, size_t n_values, const float* values_f, size_t n_values_f);
+
+/* load_sdf_multi_octree / save_sdf_multi_octree (sdf/multi/sdf_multi.cpp:8-35): u32 node count,
+ * u32 value count, SdfMultiOctreeNode[nodes], float[values]. */
+lrt_status lrt_load_sdf_multi_octree(const char* path, lrt_buffer* nodes, lrt_buffer* values);
+lrt_status lrt_save_sdf_multi_octree(const char* path, const void* nodes, size_t n_nodes, const float* values,
+                                     size_t n_values);
+
 #ifdef __cplusplus
 }
 #endif
