@@ -54,8 +54,22 @@ ARRAY_DTYPES = {
     "orig_nodes": NODE_DTYPE, "frame_nodes": FRAME_NODE_DTYPE, "frame_roots": np.uint32,
     "sbs_nodes": SBS_NODE_DTYPE, "sbs_data": np.uint32, "sbs_data_f": np.float32, "sbs_roots": np.uint32,
     "sbs_headers": SBS_HEADER_DTYPE, "multi_nodes": MULTI_NODE_DTYPE, "multi_values": np.float32,
-    "preset": PRESET_DTYPE, "sbs_mask": np.uint32,
+    "preset": PRESET_DTYPE, "sbs_mask": np.uint32, "orig_indices": np.uint32,
 }
+LIGHT_DTYPE = np.dtype([("space", "<f4", (3,)), ("type", "<u4"), ("color", "<f4", (3,)), ("_pad", "<u4")])  # Light, 32 B
+LIGHT_TYPE_DIRECT, LIGHT_TYPE_POINT, LIGHT_TYPE_AMBIENT = 0, 1, 2
+
+
+def default_lights() -> np.ndarray:
+    """MultiRenderer's lights (MultiRenderer_host.cpp:55-56): a direct light along normalize(1,1,1)
+    of colour 2/3, an ambient light of 0 and one of 0.25 (create_*_light, core/IRenderer.h:80-93)."""
+    L = np.zeros(3, LIGHT_DTYPE)
+    inv = np.float32(1.0) / np.sqrt(np.float32(3.0))   # LiteMath normalize: v * (1 / length(v))
+    L[0]["space"] = np.float32(1.0) * inv
+    L[0]["type"], L[0]["color"] = LIGHT_TYPE_DIRECT, np.float32(2.0) / np.float32(3.0)
+    L[1]["type"], L[1]["color"] = LIGHT_TYPE_AMBIENT, 0.0
+    L[2]["type"], L[2]["color"] = LIGHT_TYPE_AMBIENT, 0.25
+    return L
 
 # enums (core/render_settings.h, BVH2Common.h:30-43, sdf_simple_shared.h:13-21)
 TYPE_MESH_TRIANGLE, TYPE_SDF_FRAME_OCTREE, TYPE_SDF_SBS, TYPE_SDF_SBS_COL = 0, 3, 6, 11
@@ -91,6 +105,16 @@ class _TuneResult(ctypes.Structure):
 
 class _Buffer(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("bytes", ctypes.c_size_t)]
+
+
+class _ResolveDesc(ctypes.Structure):
+    """lrt_resolve_desc (include/lrt.h)."""
+    _fields_ = [("proj_inv16", ctypes.c_void_p), ("world_view_inv16", ctypes.c_void_p), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32), ("order", ctypes.c_uint32), ("sample_id", ctypes.c_uint32),
+                ("aa_frame", ctypes.c_uint32), ("lights", ctypes.c_void_p), ("n_lights", ctypes.c_uint32),
+                ("background", ctypes.c_float * 4), ("highlight_mode", ctypes.c_uint32),
+                ("highlight_prim", ctypes.c_uint32), ("highlight_geom", ctypes.c_uint32),
+                ("highlight_inst", ctypes.c_uint32)]
 
 
 _lib_handle = None
@@ -160,6 +184,8 @@ def lib() -> ctypes.CDLL:
             "lrt_sdf_spheres_sbs": ([vp, sz, u32, vp, u64, i32, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
             "lrt_sdf_spheres_sbs_indexed": ([vp, sz, u32, u32, u64, i32, ctypes.POINTER(_Buffer),
                                              ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer), pu32], i32),
+            "lrt_resolve_dev": ([vp, vp, ctypes.POINTER(_ResolveDesc), vp, vp], i32),
+            "lrt_tonemap_dev": ([vp, u64, u32, vp, vp], i32),
             "lrt_load_sdf_frame_octree": ([cp, ctypes.POINTER(_Buffer)], i32),
             "lrt_save_sdf_frame_octree": ([cp, vp, sz], i32),
             "lrt_load_sdf_sbs": ([cp, vp, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)],
@@ -187,7 +213,8 @@ def exported_symbols():
             "lrt_sbs_batch_create", "lrt_sbs_batch_run", "lrt_sbs_batch_destroy",
             "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
             "lrt_camera_inverse", "lrt_gen_icosphere", "lrt_gen_geodesic", "lrt_sdf_spheres_frame_octree",
-            "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed", "lrt_load_sdf_frame_octree",
+            "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed", "lrt_resolve_dev", "lrt_tonemap_dev",
+            "lrt_load_sdf_frame_octree",
             "lrt_save_sdf_frame_octree", "lrt_load_sdf_sbs", "lrt_save_sdf_sbs", "lrt_load_sdf_multi_octree",
             "lrt_save_sdf_multi_octree"]
 
@@ -335,6 +362,25 @@ class SceneRT:
         """Device buffers (torch tensors): pos/dir float32 (n,4), hits uint8/int32 storage of n*32 bytes."""
         n = pos.numel() // 4
         _check(lib().lrt_trace_closest_dev(self._h, _ptr(pos), _ptr(dir), _ptr(hits), n, stream))
+
+    def resolve(self, gbuffer, proj_inv, world_view_inv, width: int, height: int, colour, order: int = 1,
+                sample_id: int = 0, lights: Optional[np.ndarray] = None, background=(0.0, 0.0, 0.0, 1.0),
+                aa_frame: int = 0, highlight=None, stream: int = 0):
+        """MultiRenderer's resolve of a traced g-buffer (device, width*height hits in the eye-ray
+        `order`) into `colour` (device float32 (width*height, 4)); mode, normal mode and spp from
+        the scene's preset."""
+        L = np.ascontiguousarray(default_lights() if lights is None else lights, dtype=LIGHT_DTYPE)
+        pi = np.ascontiguousarray(proj_inv, dtype=np.float32).reshape(16)
+        vi = np.ascontiguousarray(world_view_inv, dtype=np.float32).reshape(16)
+        d = _ResolveDesc()
+        d.proj_inv16, d.world_view_inv16 = _ptr(pi), _ptr(vi)
+        d.width, d.height, d.order, d.sample_id, d.aa_frame = width, height, order, sample_id, aa_frame
+        d.lights, d.n_lights = (_ptr(L) if L.size else None), L.size
+        d.background[:] = [float(x) for x in background]
+        if highlight is not None:
+            d.highlight_mode = 1
+            d.highlight_prim, d.highlight_geom, d.highlight_inst = (int(x) for x in highlight)
+        _check(lib().lrt_resolve_dev(self._h, _ptr(gbuffer), ctypes.byref(d), _ptr(colour), stream))
 
     def trace_closest_noalloc(self, pos, dir, hits, stream: int = 0):
         """trace_closest through the allocation-free entry point (at most NOALLOC_MAX_RAYS rays;
@@ -579,3 +625,9 @@ def save_sdf_multi_octree(path: str, nodes, values):
     n = np.ascontiguousarray(nodes, dtype=MULTI_NODE_DTYPE)
     v = np.ascontiguousarray(values, dtype=np.float32)
     _check(lib().lrt_save_sdf_multi_octree(os.fsencode(path), _ptr(n), n.size, _ptr(v), v.size))
+
+
+def tonemap(colour, rgba8, sample_count: int = 1, stream: int = 0):
+    """MultiRenderer::kernel1D_Tonemap on device buffers: colour float32 (n, 4) -> rgba8 uint32/int32 (n)."""
+    n = colour.numel() // 4
+    _check(lib().lrt_tonemap_dev(_ptr(colour), n, sample_count, _ptr(rgba8), stream))

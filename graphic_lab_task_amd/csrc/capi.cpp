@@ -411,6 +411,91 @@ lrt_status lrt_query_one(lrt_scene* s, const float* pos_near4, const float* dir_
   return lrt_trace_closest_host(s, pos_near4, dir_far4, out, 1);
 }
 
+lrt_status lrt_resolve_dev(lrt_scene* s, const lrt_hit* gbuffer, const lrt_resolve_desc* desc, float* colour,
+                           void* stream) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !gbuffer || !desc || !colour || !desc->proj_inv16 || !desc->world_view_inv16)
+      return fail(LRT_ERR_INVALID_ARG, "null argument");
+    lrt::Scene& sc = s->s;
+    if (!sc.committed) return fail(LRT_ERR_NOT_COMMITTED, "scene is not committed");
+    const uint32_t w = desc->width, h = desc->height;
+    if (w == 0 || h == 0 || w > 65535 || h > 65535) return fail(LRT_ERR_INVALID_ARG, "bad viewport");
+    if (desc->order > 1) return fail(LRT_ERR_INVALID_ARG, "order must be 0 or 1");
+    if (desc->order == 1 && (w % 8 != 0 || h % 8 != 0))
+      return fail(LRT_ERR_INVALID_ARG, "packed order needs width and height multiples of 8");
+    if (desc->n_lights > lrt::RESOLVE_MAX_LIGHTS || (desc->n_lights && !desc->lights))
+      return fail(LRT_ERR_INVALID_ARG, "at most 8 lights");
+    const uint32_t m = sc.preset.render_mode;
+    using namespace lrt;
+    const bool debug = m == MULTI_RENDER_MODE_MASK || m == MULTI_RENDER_MODE_DEPTH ||
+                       m == MULTI_RENDER_MODE_LINEAR_DEPTH || m == MULTI_RENDER_MODE_INVERSE_LINEAR_DEPTH ||
+                       m == MULTI_RENDER_MODE_PRIMITIVE || m == MULTI_RENDER_MODE_TYPE ||
+                       m == MULTI_RENDER_MODE_NORMAL || m == MULTI_RENDER_MODE_BARYCENTRIC ||
+                       m == MULTI_RENDER_MODE_ST_ITERATIONS || m == MULTI_RENDER_MODE_TEX_COORDS ||
+                       m == MULTI_RENDER_MODE_HSV_DEPTH || m == MULTI_RENDER_MODE_MO_VOXEL_COUNT ||
+                       m == MULTI_RENDER_MODE_MO_FLAGS || m == MULTI_RENDER_MODE_INTERSECTION_COUNT;
+    bool common = m == MULTI_RENDER_MODE_LAMBERT_NO_TEX || m == MULTI_RENDER_MODE_GEOM || m == MULTI_RENDER_MODE_LOD;
+    if (m == MULTI_RENDER_MODE_LAMBERT) {
+      common = !sc.geomData.empty();
+      for (const GeomData& gd : sc.geomData) common = common && gd.type == TYPE_SDF_SBS_COL;
+      if (!common)
+        return fail(LRT_ERR_UNSUPPORTED, "LAMBERT resolve needs material tables unless every geometry carries "
+                                         "per-voxel colour");
+    }
+    if (!debug && !common)
+      return fail(LRT_ERR_UNSUPPORTED, "render mode " + std::to_string(m) + " needs the renderer's material, "
+                                       "texture or channel tables");
+    ResolveParams P{};
+    std::memcpy(P.proj_inv, desc->proj_inv16, sizeof(P.proj_inv));
+    std::memcpy(P.world_view_inv, desc->world_view_inv16, sizeof(P.world_view_inv));
+    P.width = w;
+    P.height = h;
+    P.packed_w = 8 * ((w + 7) / 8);   // MultiRenderer_host.cpp:80-81
+    P.packed_h = 8 * ((h + 7) / 8);
+    P.count = w * h;
+    P.order = desc->order;
+    P.render_mode = m;
+    P.normal_mode = sc.preset.normal_mode;
+    P.spp = sc.preset.spp ? sc.preset.spp : 1;
+    P.aa_frame = desc->aa_frame;
+    P.sample_id = desc->sample_id;
+    P.n_lights = desc->n_lights;
+    for (uint32_t i = 0; i < desc->n_lights; ++i) std::memcpy(&P.lights[i], &desc->lights[i], sizeof(ResolveLight));
+    std::memcpy(P.background, desc->background, sizeof(P.background));
+    P.highlight_mode = desc->highlight_mode;
+    P.highlight_prim = desc->highlight_prim;
+    P.highlight_geom = desc->highlight_geom;
+    P.highlight_inst = desc->highlight_inst;
+    const SceneView v = sc.make_view();
+    P.geoms = v.geoms;
+    P.vert_pos = v.vert_pos;
+    P.vert_norm = v.vert_norm;
+    auto dptr = [&](const char* n) -> const void* {
+      auto it = sc.dev.find(n);
+      return (it == sc.dev.end() || it->second.bytes() == 0) ? nullptr : it->second.ptr();
+    };
+    P.orig_indices = static_cast<const uint32_t*>(dptr("orig_indices"));
+    P.inst_inv_t = static_cast<const float*>(dptr("inst_inv_t"));
+    hipError_t e = launch_resolve(P, reinterpret_cast<const CRT_Hit*>(gbuffer), reinterpret_cast<lrt::float4*>(colour),
+                                  reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "resolve launch");
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_tonemap_dev(const float* colour, uint64_t n, uint32_t sample_count, uint32_t* rgba8, void* stream) {
+  return guarded([&]() -> lrt_status {
+    if (n == 0) return LRT_OK;
+    if (!colour || !rgba8) return fail(LRT_ERR_INVALID_ARG, "null buffer");
+    if (n > 0xFFFFFFFFull) return fail(LRT_ERR_INVALID_ARG, "too many pixels");
+    if (sample_count == 0) return fail(LRT_ERR_INVALID_ARG, "sample_count must be positive");
+    hipError_t e = lrt::launch_tonemap(reinterpret_cast<const lrt::float4*>(colour), uint32_t(n), sample_count, rgba8,
+                                       reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "tonemap launch");
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_eye_rays_dev(const float* proj_inv16, const float* world_view_inv16, uint32_t width, uint32_t height,
                             const float* jitter, uint32_t order, float* pos_near, float* dir_far, void* stream) {
   return guarded([&]() -> lrt_status {
@@ -757,7 +842,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
     ARR("prim_indices", c.primIndices) ARR("orig_nodes", c.origNodes) ARR("frame_nodes", c.frameNodes)
     ARR("frame_roots", c.frameRoots) ARR("sbs_nodes", c.sbsNodes) ARR("sbs_data", c.sbsData)
     ARR("sbs_data_f", c.sbsDataF) ARR("sbs_roots", c.sbsRoots) ARR("sbs_headers", c.sbsHeaders)
-    ARR("multi_nodes", c.multiNodes) ARR("multi_values", c.multiValues)
+    ARR("multi_nodes", c.multiNodes) ARR("multi_values", c.multiValues) ARR("orig_indices", c.origIndices)
 #undef ARR
     if (n == "preset") { src = &c.preset; sz = sizeof(c.preset); }
     std::vector<uint32_t> mask_copy;   // the committed occupancy masks live on the device only
@@ -777,7 +862,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
       static const char* known[] = {"nodes", "tlas", "instances", "geoms", "geom_tags", "start_end",
                                     "prim_id_count", "vert_pos", "vert_norm", "indices", "prim_indices",
                                     "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes", "sbs_data",
-                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset",
+                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices",
                                     "sbs_mask"};
       bool ok = false;
       for (const char* k : known) ok = ok || n == k;

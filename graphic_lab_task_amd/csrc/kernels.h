@@ -111,6 +111,36 @@ hipError_t launch_sbs_batch(const SbsBatchLaunch& L, hipStream_t stream);
 // Resident blocks of the persistent batch kernel per CU (occupancy query).
 hipError_t sbs_batch_blocks_per_cu(int* blocks);
 
+// Light (core/IRenderer.h:72-78), 32 B
+struct ResolveLight {
+  float    space[3];   // direction (direct light, normalised) or position (point light)
+  uint32_t type;       // LIGHT_TYPE_*
+  float    color[3];
+  uint32_t pad;
+};
+constexpr uint32_t RESOLVE_MAX_LIGHTS = 8;
+
+// One g-buffer resolve launch (MultiRenderer::kernel1D_ResolveCommon / kernel1D_ResolveDebug).
+struct ResolveParams {
+  float    proj_inv[16], world_view_inv[16];   // camera of the eye rays (column-major)
+  uint32_t width, height, packed_w, packed_h;
+  uint32_t count;          // width * height pixels
+  uint32_t order;          // 0: hit of pixel i at i; 1: at SuperBlockIndex2DOpt(x, y)
+  uint32_t render_mode, normal_mode, spp, aa_frame, sample_id;
+  uint32_t n_lights;
+  ResolveLight lights[RESOLVE_MAX_LIGHTS];
+  float    background[4];
+  uint32_t highlight_mode, highlight_prim, highlight_geom, highlight_inst;
+  const GeomData* geoms;
+  const float4*   vert_pos;
+  const float4*   vert_norm;
+  const uint32_t* orig_indices;   // triangle indices in the caller's order (local vertex ids)
+  const float*    inst_inv_t;     // per instance transpose(inverse(transform)), float4x4 layout
+};
+hipError_t launch_resolve(const ResolveParams& p, const CRT_Hit* gbuf, float4* out, hipStream_t stream);
+// Tonemap: rgba8[i] = pack(255 * clamp(colour[i] / samples, 0, 1))
+hipError_t launch_tonemap(const float4* colour, uint32_t n, uint32_t samples, uint32_t* out, hipStream_t stream);
+
 hipError_t launch_trace(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 // launch_trace with the library's fixed per-device work array as queue and spill (a.queue and
 // a.spill are ignored): at most LRT_NOALLOC_MAX_RAYS rays, no counters; `device` is current.

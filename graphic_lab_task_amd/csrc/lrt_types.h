@@ -189,6 +189,18 @@ constexpr uint32_t MULTI_RENDER_MODE_PBR              = 21;
 constexpr uint32_t MULTI_RENDER_MODE_CHANNEL          = 23;
 constexpr uint32_t MULTI_RENDER_MODE_INTERSECTION_COUNT = 26;
 constexpr uint32_t MULTI_RENDER_MODE_CHANNEL_LAMBERT  = 27;
+constexpr uint32_t MULTI_RENDER_MODE_DEPTH            = 2;
+constexpr uint32_t MULTI_RENDER_MODE_LINEAR_DEPTH     = 3;
+constexpr uint32_t MULTI_RENDER_MODE_INVERSE_LINEAR_DEPTH = 4;
+constexpr uint32_t MULTI_RENDER_MODE_PRIMITIVE        = 5;
+constexpr uint32_t MULTI_RENDER_MODE_TYPE             = 6;
+constexpr uint32_t MULTI_RENDER_MODE_BARYCENTRIC      = 9;
+constexpr uint32_t MULTI_RENDER_MODE_TEX_COORDS       = 15;
+constexpr uint32_t MULTI_RENDER_MODE_HSV_DEPTH        = 19;
+// Light types (core/IRenderer.h:16-18)
+constexpr uint32_t LIGHT_TYPE_DIRECT  = 0;
+constexpr uint32_t LIGHT_TYPE_POINT   = 1;
+constexpr uint32_t LIGHT_TYPE_AMBIENT = 2;
 constexpr uint32_t NORMAL_MODE_GEOMETRY     = 0;
 constexpr uint32_t NORMAL_MODE_VERTEX       = 1;
 constexpr uint32_t NORMAL_MODE_SDF_SMOOTHED = 2;

@@ -131,7 +131,7 @@ void Scene::clear_scene() {
 
 void Scene::clear_geom() {
   geomData.clear(); geomTags.clear(); startEnd.clear();
-  vertPos.clear(); vertNorm.clear(); indices.clear(); primIndices.clear(); primIdCount.clear();
+  vertPos.clear(); vertNorm.clear(); indices.clear(); primIndices.clear(); primIdCount.clear(); origIndices.clear();
   nodePairs.clear(); origNodes.clear();
   frameNodes.clear(); frameRoots.clear();
   sbsNodes.clear(); sbsData.clear(); sbsDataF.clear(); sbsRoots.clear(); sbsHeaders.clear(); sbsFSlice.clear(); sbsDiffChecked.clear();
@@ -178,6 +178,7 @@ uint32_t Scene::add_triangles(const float* vpos, size_t n_verts, size_t v_stride
     indices[oldSizeInd + 3 * i + 1] = idx[triId * 3 + 1];
     indices[oldSizeInd + 3 * i + 2] = idx[triId * 3 + 2];
   }
+  origIndices.insert(origIndices.end(), idx, idx + n_idx);
   const size_t oldSize = primIdCount.size();
   primIdCount.insert(primIdCount.end(), bvh.startCount.begin(), bvh.startCount.end());
   startEnd.push_back(uint2{uint32_t(oldSize), uint32_t(primIdCount.size())});
@@ -554,7 +555,12 @@ hipError_t Scene::commit() {
   UP("indices", indices) UP("prim_indices", primIndices) UP("orig_nodes", origNodes) UP("frame_nodes", frameNodes)
   UP("frame_roots", frameRoots) UP("sbs_nodes", sbsNodes) UP("sbs_data", sbsData) UP("sbs_data_f", sbsDataF)
   UP("sbs_roots", sbsRoots) UP("sbs_headers", sbsHeaders) UP("multi_nodes", multiNodes) UP("multi_values", multiValues)
-  UP("sbs_mask_base", sbsMaskBase)
+  UP("sbs_mask_base", sbsMaskBase) UP("orig_indices", origIndices)
+  // MultiRenderer's own m_instanceTransformInvTransposed (MultiRenderer_host.cpp:870-878):
+  // transpose(inverse4x4(matrix)), refreshed by UpdateInstance (unlike the BVHRT copy)
+  std::vector<float4x4> inst_inv_t(instances.size());
+  for (size_t i = 0; i < instances.size(); ++i) inst_inv_t[i] = transpose4x4(instances[i].transformInv);
+  UP("inst_inv_t", inst_inv_t)
 #undef UP
   e = dev["sbs_mask"].reserve(sbsMaskWords * sizeof(uint32_t));
   if (e != hipSuccess) return e;

@@ -44,6 +44,7 @@ struct Scene {
   std::vector<uint2> startEnd;
   std::vector<float4> vertPos, vertNorm;
   std::vector<uint32_t> indices, primIndices, primIdCount;
+  std::vector<uint32_t> origIndices;   // triangle indices in the caller's order (g-buffer resolve)
   std::vector<BVHNodePair> nodePairs;
   std::vector<BVHNode> origNodes;
   std::vector<SdfFrameOctreeNode> frameNodes;

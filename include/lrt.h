@@ -140,6 +140,45 @@ lrt_status lrt_trace_closest_counted_dev(lrt_scene* s, const float* pos_near, co
 lrt_status lrt_trace_any_dev(lrt_scene* s, const float* pos_near, const float* dir_far, uint32_t* flags,
                              uint64_t n, void* stream);
 
+/* ---- g-buffer resolve and tonemap (the consumer of the hit records) ---- */
+
+/* Light (core/IRenderer.h:72-78, 32 B): type 0 direct (space = unit direction), 1 point
+ * (space = position), 2 ambient. */
+typedef struct lrt_light { float space[3]; uint32_t type; float color[3]; uint32_t pad; } lrt_light;
+
+/* Frame description of a resolve launch: the camera the eye rays came from, the g-buffer order
+ * and the renderer state MultiRenderer keeps beside its preset. */
+typedef struct lrt_resolve_desc {
+  const float* proj_inv16;        /* inverse4x4(proj), column-major (lrt_camera_inverse) */
+  const float* world_view_inv16;  /* inverse4x4(worldView), column-major */
+  uint32_t width, height;
+  uint32_t order;                 /* order the g-buffer was traced in (lrt_eye_rays_dev order) */
+  uint32_t sample_id;             /* 0 writes the colour buffer, > 0 accumulates into it */
+  uint32_t aa_frame;              /* m_AAFrameNum: 0 keeps alpha 1 / the background's alpha */
+  const lrt_light* lights;        /* host array of n_lights (at most 8) */
+  uint32_t n_lights;
+  float background[4];            /* m_backgroundColor */
+  uint32_t highlight_mode;        /* 1: add (0.5,0,0,0) where prim, geom and inst ids match */
+  uint32_t highlight_prim, highlight_geom, highlight_inst;
+} lrt_resolve_desc;
+
+/* MultiRenderer::kernel1D_ResolveCommon / kernel1D_ResolveDebug (MultiRenderer/
+ * MultiRenderer_resolve.cpp:529-556, 567-806, 808-1005) over `gbuffer` (lrt_hit[width*height],
+ * device, as a trace of the eye rays wrote it), into `colour` (float4[width*height], device,
+ * row-major). Render mode, normal mode and spp come from the scene's preset, as the reference
+ * takes them from m_preset. Supported modes: MASK, LAMBERT_NO_TEX, DEPTH, LINEAR_DEPTH,
+ * INVERSE_LINEAR_DEPTH, PRIMITIVE, TYPE, GEOM, NORMAL, BARYCENTRIC, ST_ITERATIONS, TEX_COORDS,
+ * HSV_DEPTH, LOD, MO_VOXEL_COUNT, MO_FLAGS, INTERSECTION_COUNT, and LAMBERT on scenes whose
+ * geometries all carry per-voxel colour; the material, texture and channel modes need the
+ * renderer's material tables and fail with LRT_ERR_UNSUPPORTED. Meshes carry no texture
+ * coordinates through this ABI (tc = 0). Asynchronous on `stream`. */
+lrt_status lrt_resolve_dev(lrt_scene* s, const lrt_hit* gbuffer, const lrt_resolve_desc* desc, float* colour,
+                           void* stream);
+
+/* MultiRenderer::kernel1D_Tonemap (MultiRenderer_resolve.cpp:529-544): rgba8[i] = packed
+ * uint4(255 * clamp(colour[i] / sample_count, 0, 1)), red in the low byte. Device buffers. */
+lrt_status lrt_tonemap_dev(const float* colour, uint64_t n, uint32_t sample_count, uint32_t* rgba8, void* stream);
+
 /* Single-ray RayQuery_NearestHit for picking (demo_app/plugins/MRFull/execute_commands.cpp:189);
  * runs the same gfx950 kernel on one ray and synchronises. Uses the scene's staging buffer of
  * lrt_trace_closest_host: after the first call, a picking loop allocates nothing. */
@@ -273,7 +312,8 @@ lrt_status lrt_stack_overflow_count(lrt_scene* s, uint64_t* out, int reset);
  * "geom_tags", "start_end" (uint2), "prim_id_count", "vert_pos" (float4), "vert_norm" (float4),
  * "indices", "prim_indices", "orig_nodes" (BVHNode), "frame_nodes" (SdfFrameOctreeNode),
  * "frame_roots", "sbs_nodes", "sbs_data", "sbs_data_f", "sbs_roots", "sbs_headers",
- * "multi_nodes", "multi_values", "preset", and "sbs_mask" (uint32: the committed voxel occupancy
+ * "multi_nodes", "multi_values", "orig_indices" (uint32: triangle indices in the caller's order),
+ * "preset", and "sbs_mask" (uint32: the committed voxel occupancy
  * masks, per brick set in order and per brick one summary word then ceil(bs^3/32) bit words,
  * bit v set when the minimum of voxel v's 8 corner values is <= 0; copied from the device).
  * With dst == NULL only *bytes is written. */

@@ -75,6 +75,11 @@ def load(flavor: str = "exact") -> ctypes.CDLL:
         L.oracle_trace_any.restype = None
         L.oracle_brute_force_tris.argtypes = [vp, vp, u64, vp, vp, u64, vp, vp, vp, i32]
         L.oracle_brute_force_tris.restype = None
+        L.oracle_resolve.argtypes = [ctypes.POINTER(OracleScene), vp, vp, u32, vp, vp, u32, u32, u32, u32, vp, u32, vp,
+                                     vp, vp]
+        L.oracle_resolve.restype = None
+        L.oracle_tonemap.argtypes = [vp, u64, u32, vp]
+        L.oracle_tonemap.restype = None
         L.oracle_eye_rays.argtypes = [vp, vp, u32, u32, vp, u32, vp, vp]
         L.oracle_eye_rays.restype = None
         L.oracle_camera_inverse.argtypes = [vp, vp, vp, f32, f32, f32, f32, vp, vp]
@@ -185,3 +190,32 @@ def local_surface(preset: np.ndarray, values, d: float, q_near: float, q_far: fl
     h = np.zeros((), HIT_DTYPE)
     load(flavor).oracle_local_surface(_p(raw), _p(v), d, q_near, q_far, f_near, _p(sq), _p(rd), _p(h))
     return h
+
+
+def resolve(scene: Scene, orig_indices, gbuf, proj_inv, wv_inv, width, height, order=1, sample_id=0, aa_frame=0,
+            lights=None, background=(0.0, 0.0, 0.0, 1.0), highlight=None, colour=None, flavor: str = "exact"):
+    """MultiRenderer's g-buffer resolve (MultiRenderer_resolve.cpp:529-1005) on the host; lights:
+    32-B Light records (uint8/structured array); returns float32 (width*height, 4)."""
+    L = load(flavor)
+    out = np.zeros((width * height, 4), np.float32) if colour is None else colour
+    lt = np.ascontiguousarray(lights if lights is not None else np.zeros(0, np.uint8))
+    hl = np.zeros(4, np.uint32)
+    if highlight is not None:
+        hl[0] = 1
+        hl[1:] = highlight
+    bg = np.ascontiguousarray(background, dtype=np.float32)
+    oi = np.ascontiguousarray(orig_indices, dtype=np.uint32)
+    g = np.ascontiguousarray(gbuf)
+    pi = np.ascontiguousarray(proj_inv, dtype=np.float32)
+    vi = np.ascontiguousarray(wv_inv, dtype=np.float32)
+    L.oracle_resolve(ctypes.byref(scene.view), _p(oi), _p(g), order, _p(pi), _p(vi), width, height, sample_id, aa_frame,
+                     _p(lt) if lt.size else None, int(lt.size if lt.dtype.itemsize == 32 else lt.nbytes // 32),
+                     _p(bg), _p(hl), _p(out))
+    return out
+
+
+def tonemap(colour: np.ndarray, samples: int = 1, flavor: str = "exact") -> np.ndarray:
+    c = np.ascontiguousarray(colour, dtype=np.float32)
+    out = np.zeros(c.shape[0], np.uint32)
+    load(flavor).oracle_tonemap(_p(c), c.shape[0], samples, _p(out))
+    return out

@@ -1001,3 +1001,263 @@ void oracle_sbs_hit_voxels(const OracleScene* sc, const float* pos, const float*
 }
 
 }  // extern "C"
+
+// ---- g-buffer resolve and tonemap: MultiRenderer::kernel1D_ResolveCommon / ResolveDebug /
+//      Tonemap (MultiRenderer/MultiRenderer_resolve.cpp:10-75, 529-1005), MultiRenderer.cpp:142-183 ----
+namespace {
+
+struct Col { float x, y, z, w; };
+inline Col operator*(float a, Col b) { return Col{a * b.x, a * b.y, a * b.z, a * b.w}; }
+inline Col operator+(Col a, Col b) { return Col{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+inline Col operator/(Col a, float b) { return Col{a.x / b, a.y / b, a.z / b, a.w / b}; }
+
+// MultiRenderer.h:270-284
+const uint32_t kPalette[20] = {0xffe6194b, 0xff3cb44b, 0xffffe119, 0xff0082c8, 0xfff58231, 0xff911eb4, 0xff46f0f0,
+                               0xfff032e6, 0xffd2f53c, 0xfffabebe, 0xff008080, 0xffe6beff, 0xffaa6e28, 0xfffffac8,
+                               0xff800000, 0xffaaffc3, 0xff808000, 0xffffd8b1, 0xff000080, 0xff808080};
+const uint32_t kLodPalette[20] = {0xff000000, 0xff000040, 0xff000080, 0xff0000c0, 0xff0040ff, 0xff0080ff, 0xff00c0ff,
+                                  0xff00ffff, 0xff00ffc0, 0xff00ff80, 0xff00ff40, 0xff00ff00, 0xff00c000, 0xff008000,
+                                  0xff004000, 0xff000000, 0xffff0000, 0xffff0000, 0xffff0000, 0xffff0000};
+const f2 kHamilton[16] = {{0.500000f, 0.333333f}, {0.250000f, 0.666667f}, {0.750000f, 0.111111f},
+                          {0.125000f, 0.444444f}, {0.625000f, 0.777778f}, {0.375000f, 0.222222f},
+                          {0.875000f, 0.555556f}, {0.062500f, 0.888889f}, {0.562500f, 0.037037f},
+                          {0.312500f, 0.370370f}, {0.812500f, 0.703704f}, {0.187500f, 0.148148f},
+                          {0.687500f, 0.481481f}, {0.437500f, 0.814815f}, {0.937500f, 0.259259f},
+                          {0.031250f, 0.592593f}};
+
+// render_common.h:250-254
+inline Col decode_RGBA8(uint32_t c) {
+  const uint32_t r = c & 0xFF, g = (c >> 8) & 0xFF, b = (c >> 16) & 0xFF, a = (c >> 24) & 0xFF;
+  return Col{r * (1.0f / 255.0f), g * (1.0f / 255.0f), b * (1.0f / 255.0f), a * (1.0f / 255.0f)};
+}
+// render_common.h:256-267
+inline f3 decode_normal(f2 e) {
+  f3 v(e.x, e.y, 1.0f - std::abs(e.x) - std::abs(e.y));
+  if (v.z < 0) {
+    const float vx = v.x;
+    v.x = (1.0f - std::abs(v.y)) * ((v.x >= 0.0f) ? +1.0f : -1.0f);
+    v.y = (1.0f - std::abs(vx)) * ((v.y >= 0.0f) ? +1.0f : -1.0f);
+  }
+  return normalize(v);
+}
+inline float lm_clamp(float u, float a, float b) { return std::min(std::max(a, u), b); }
+
+struct ResolveLight { float space[3]; uint32_t type; float color[3]; uint32_t pad; };
+
+struct Resolver {
+  const OracleScene& s;
+  const uint32_t* orig_indices;
+  const float* proj_inv;
+  const float* wv_inv;
+  uint32_t width, height, pw, ph, aa_frame;
+  const ResolveLight* lights;
+  uint32_t n_lights;
+  const float* bg;
+  const uint32_t* hl;   // mode, prim, geom, inst
+  Preset p;
+
+  // MultiRenderer::initEyeRay for sample 0, cutting plane disabled
+  void init_eye_ray(uint32_t x, uint32_t y, f3* pos, f3* dir) const {
+    const uint32_t spp_sqrt = uint32_t(std::sqrt(float(p.spp)));
+    const float i_spp_sqrt = 1.0f / spp_sqrt;
+    const f2 d = (aa_frame >= 2) ? kHamilton[aa_frame % 16] : f2{i_spp_sqrt * float(0 / spp_sqrt + 0.5),
+                                                                  i_spp_sqrt * float(0 % spp_sqrt + 0.5)};
+    const float fx = (float(x) + d.x) / float(pw), fy = (float(y) + d.y) / float(ph);
+    f4 q{2.0f * fx - 1.0f, -2.0f * fy + 1.0f, 1.0f, 1.0f};
+    q = mul(proj_inv, q);
+    const float w = q.w;
+    q = f4{q.x / w, q.y / w, q.z / w, q.w / w};
+    f3 rd = normalize(to3(q));
+    const f3 zero(0, 0, 0);
+    const f3 o = matmul4x3(wv_inv, zero);
+    const f3 o2 = matmul4x3(wv_inv, zero + 100.0f * rd);
+    *pos = o;
+    *dir = normalize(o2 - o);
+  }
+
+  // MultiRenderer::loadNormalTcFromGbuffer (no texture coordinates on meshes here)
+  void load_normal_tc(const Hit& hit, f3 ray_dir, f3* norm, f2* tc) const {
+    const uint32_t type = hit.geomId >> SH_TYPE, geomId = hit.geomId & GEOM_ID_MASK;
+    if (type == TYPE_MESH_TRIANGLE) {
+      const Geom& g = s.geoms[geomId];
+      const uint32_t A = orig_indices[g.offset[0] + hit.primId * 3 + 0];
+      const uint32_t B = orig_indices[g.offset[0] + hit.primId * 3 + 1];
+      const uint32_t C = orig_indices[g.offset[0] + hit.primId * 3 + 2];
+      f3 raw(1, 0, 0);
+      if (p.normal_mode == 0) {
+        raw = cross(to3(s.vert_pos[g.offset[1] + B]) - to3(s.vert_pos[g.offset[1] + A]),
+                    to3(s.vert_pos[g.offset[1] + C]) - to3(s.vert_pos[g.offset[1] + A]));
+      } else if (p.normal_mode == 1) {
+        const f4 nA = s.vert_norm[g.offset[1] + A], nB = s.vert_norm[g.offset[1] + B], nC = s.vert_norm[g.offset[1] + C];
+        const float u = hit.coords[0], v = hit.coords[1], k = 1.0f - u - v;
+        raw = f3(k * nA.x + v * nB.x + u * nC.x, k * nA.y + v * nB.y + u * nC.y, k * nA.z + v * nB.z + u * nC.z);
+      }
+      *tc = f2{0.0f, 0.0f};
+      // transpose(inverse4x4(M)) of the renderer's own copy: the transpose of transformInv
+      const float* inv = s.instances[hit.instId].transformInv;
+      float it[16];
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) it[4 * r + c] = inv[4 * c + r];
+      *norm = normalize(matmul4x3(it, raw));
+      *norm = *norm * lm_sign(dot(-1.0f * ray_dir, *norm));
+    } else {
+      *tc = f2{hit.coords[0], hit.coords[1]};
+      *norm = decode_normal(f2{hit.coords[2], hit.coords[3]});
+    }
+  }
+
+  Col resolve(uint32_t tid, const Hit& hit) const {
+    if (hit.primId == 0xFFFFFFFFu) return Col{bg[0], bg[1], bg[2], aa_frame == 0 ? bg[3] : -1.0f};
+    const uint32_t type = hit.geomId >> SH_TYPE, geomId = hit.geomId & GEOM_ID_MASK;
+    f3 rpos, rdir;
+    init_eye_ray(tid % pw, tid / pw, &rpos, &rdir);
+    f2 tc{0, 0};
+    f3 norm(1, 0, 0);
+    load_normal_tc(hit, rdir, &norm, &tc);
+    const uint32_t m = p.render_mode;
+    Col res{1, 0, 1, 1};
+    if (m == 1 || m == 7 || m == 20 || m == 17) {
+      f3 color(0, 0, 1);
+      if (m == 1) color = f3(1, 1, 1);
+      else if (m == 20) { const Col c = decode_RGBA8(kLodPalette[hit.primId % 20]); color = f3(c.x, c.y, c.z); }
+      else if (m == 7) { const Col c = decode_RGBA8(kPalette[geomId % 20]); color = f3(c.x, c.y, c.z); }
+      else {
+        color.x = std::round(hit.coords[0]) / 255.0f;
+        color.y = hit.coords[0] - std::floor(hit.coords[0]);
+        color.z = std::round(hit.coords[1]) / 255.0f;
+      }
+      f3 fin(0, 0, 0);
+      for (uint32_t i = 0; i < n_lights; i++) {
+        const ResolveLight& L = lights[i];
+        const f3 lc(L.color[0], L.color[1], L.color[2]), ls(L.space[0], L.space[1], L.space[2]);
+        if (L.type == 0) {
+          const float q = std::max(0.0f, dot(norm, ls));
+          fin = fin + lc * color * q;
+        } else if (L.type == 1) {
+          const f3 surf = rpos + hit.t * rdir;
+          f3 dir = ls - surf;
+          const float l = length(dir);
+          dir = dir / l;
+          const float q = std::max(0.0f, dot(norm, dir));
+          fin = fin + lc * color * q / (l * l);
+        } else {
+          fin = fin + lc * color;
+        }
+      }
+      res = Col{fin.x, fin.y, fin.z, 1};
+    } else {
+      const float z = hit.t, z_near = 0.1f, z_far = 10.0f;
+      switch (m) {
+        case 0: res = Col{1.0f, 1.0f, 1.0f, 1.0f}; break;
+        case 2: { const float d = (1.0f / z - 1.0f / z_near) / (1.0f / z_far - 1.0f / z_near); res = Col{d, d, d, 1}; } break;
+        case 3: { const float d = ((z - z_near) / (z_far - z_near)); res = Col{d, d, d, 1.0f}; } break;
+        case 19: {
+          const f3 ip = rpos + z * rdir;
+          float zt = (ip.z + 1.f) * 2.f;
+          zt = zt - int(zt);
+          if (zt < 0.f) zt = 1.f + zt;
+          const f3 hsv(zt, 1.f, 1.f);
+          float rc[4] = {hsv.z, hsv.z, hsv.z, 1.f};
+          const float Vmin = (1.f - hsv.y) * hsv.z;
+          float H_i = 0.f;
+          const float a = (hsv.z - Vmin) * std::modf(hsv.x * 6.f, &H_i);
+          uint32_t hi = uint32_t(H_i);
+          hi = hi <= 5u ? hi : 5u;
+          rc[(2u + (hi >> 1)) % 3u] = Vmin;
+          if ((hi & 1u) != 0u) rc[hi >> 1] = hsv.z - a;
+          else rc[(1u + (hi >> 1)) % 3u] = Vmin + a;
+          res = Col{rc[0], rc[1], rc[2], 1.f};
+        } break;
+        case 4: { const float d = 1 - ((z - z_near) / (z_far - z_near)); res = Col{d, d, d, 1}; } break;
+        case 5: res = decode_RGBA8(kPalette[hit.primId % 20]); break;
+        case 6: res = decode_RGBA8(kPalette[type % 20]); break;
+        case 8: {
+          const f3 v(std::abs(norm.x), std::abs(norm.y), std::abs(norm.z));
+          res = Col{0.25f, 0.25f, 0.25f, 0.5f} +
+                0.5f * Col{std::pow(v.x, 1.0f / 1.5f), std::pow(v.y, 1.0f / 1.5f), std::pow(v.z, 1.0f / 1.5f), 1.0f};
+        } break;
+        case 9: res = Col{hit.coords[0], hit.coords[1], 1 - hit.coords[0] - hit.coords[1], 1}; break;
+        case 10:
+          if (hit.primId == 0) res = Col{0.5f, 0.5f, 0.5f, 1.0f};
+          else if (hit.primId <= 64) {
+            const float q = lm_clamp(float(hit.primId) / 64, 0.0f, 1.0f);
+            res = q * Col{0, 0, 1, 1} + (1.0f - q) * Col{0, 1, 0, 1};
+          } else {
+            const float q = lm_clamp(float(hit.primId - 64) / (256 - 64), 0.0f, 1.0f);
+            res = q * Col{1, 0, 0, 1} + (1.0f - q) * Col{0, 0, 1, 1};
+          }
+          break;
+        case 15: res = Col{tc.x, tc.y, 0, 1}; break;
+        case 24:
+          if (hit.primId == 0) res = Col{0.5, 0.5, 0.5, 1};
+          else if (hit.primId == 1) res = Col{0.5, 1, 0.5, 1};
+          else if (hit.primId == 2) res = Col{0.5, 0.5, 1, 1};
+          else if (hit.primId <= 64) {
+            const float q = lm_clamp(float(hit.primId) / 64, 0.0f, 1.0f);
+            res = q * Col{1, 0, 0, 1} + (1 - q) * Col{0.5, 0.5, 1, 1};
+          } else res = Col{1, 0, 0, 1};
+          break;
+        case 25: res = decode_RGBA8(kPalette[hit.primId % 20]); break;
+        case 26:
+          if (hit.instId == 0) res = Col{0.3f, 0.3f, 0.3f, 1};
+          else if (hit.instId == 1) res = Col{0, 100, 0, 255} / 255.0f;
+          else if (hit.instId == 2) res = Col{0, 200, 0, 255} / 255.0f;
+          else if (hit.instId == 3) res = Col{50, 250, 50, 255} / 255.0f;
+          else if (hit.instId == 4) res = Col{150, 250, 150, 255} / 255.0f;
+          else if (hit.instId == 5) res = Col{250, 250, 250, 255} / 255.0f;
+          else if (hit.instId <= 20)
+            res = Col{250, (1000 - 50 * hit.instId) * 0.333f, (1000 - 50 * hit.instId) * 0.333f, 255} / 255.0f;
+          else res = Col{1, 0, std::min(1.0f, 0.025f * (hit.instId - 20)), 1};
+          break;
+        default: break;
+      }
+    }
+    if (hl[0] == 1 && hl[1] == hit.primId && hl[2] == hit.geomId && hl[3] == hit.instId) res = res + Col{0.5, 0, 0, 0};
+    res.w = aa_frame == 0 ? 1.0f : hit.t;
+    return res;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+// colour: float4[width*height] (row-major); sample_id > 0 accumulates. Render mode, normal mode
+// and spp from sc->preset; lights: 32-B Light records.
+void oracle_resolve(const OracleScene* sc, const uint32_t* orig_indices, const void* gbuf, uint32_t order,
+                    const float* proj_inv, const float* wv_inv, uint32_t width, uint32_t height, uint32_t sample_id,
+                    uint32_t aa_frame, const void* lights, uint32_t n_lights, const float* bg, const uint32_t* hl,
+                    float* colour) {
+  Resolver r{*sc, orig_indices, proj_inv, wv_inv, width, height, 8 * ((width + 7) / 8), 8 * ((height + 7) / 8),
+             aa_frame, static_cast<const ResolveLight*>(lights), n_lights, bg, hl, Preset{}};
+  r.p = sc->preset;
+  if (r.p.spp == 0) r.p.spp = 1;
+  const Hit* g = static_cast<const Hit*>(gbuf);
+  for (uint32_t tid = 0; tid < width * height; ++tid) {
+    uint32_t slot = tid;
+    if (order == 1) {
+      const uint32_t x = tid % width, y = tid / width;
+      const uint32_t local = (y & 3u) * 4 + (x & 3u), bx = x >> 2, by = y >> 2;
+      const uint32_t localH = (by & 1u) * 2 + (bx & 1u), wbH = (r.pw >> 2) >> 1;
+      slot = ((bx >> 1) + (by >> 1) * wbH) * 64 + localH * 16 + local;
+    }
+    const Col c = r.resolve(tid, g[slot]);
+    float* o = colour + 4 * size_t(tid);
+    if (sample_id == 0) { o[0] = c.x; o[1] = c.y; o[2] = c.z; o[3] = c.w; }
+    else { o[0] += c.x; o[1] += c.y; o[2] += c.z; o[3] += c.w; }
+  }
+}
+
+// MultiRenderer::Tonemap (MultiRenderer_resolve.cpp:538-544)
+void oracle_tonemap(const float* colour, uint64_t n, uint32_t samples, uint32_t* out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    const float* c = colour + 4 * i;
+    const float s = float(samples);
+    const float f[4] = {c[0] / s, c[1] / s, c[2] / s, c[3] / s};
+    uint32_t col[4];
+    for (int k = 0; k < 4; ++k) col[k] = uint32_t(255 * lm_clamp(f[k], 0.0f, 1.0f));
+    out[i] = (col[3] << 24) | (col[2] << 16) | (col[1] << 8) | col[0];
+  }
+}
+
+}  // extern "C"
