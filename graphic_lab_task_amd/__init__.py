@@ -18,7 +18,8 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liblrt_hip.so")
+# LRT_LIBRARY overrides the library file (the host-sanitizer build: `make -C csrc asan`)
+LIB_PATH = os.environ.get("LRT_LIBRARY") or os.path.join(_HERE, "liblrt_hip.so")
 
 # ---- reference structs as numpy dtypes (byte-compatible, see csrc/lrt_types.h) ----
 HIT_DTYPE = np.dtype([("t", "<f4"), ("primId", "<u4"), ("instId", "<u4"), ("geomId", "<u4"),

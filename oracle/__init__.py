@@ -37,7 +37,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_DIR = os.path.join(_HERE, "lib")
+# ORACLE_LIB_DIR overrides the build folder (the host-sanitizer build: `make -C oracle asan`)
+LIB_DIR = os.environ.get("ORACLE_LIB_DIR") or os.path.join(_HERE, "lib")
 _FILES = {"exact": "liboracle.so", "fma": "liboracle_fma.so"}
 
 HIT_DTYPE = np.dtype([("t", "<f4"), ("primId", "<u4"), ("instId", "<u4"), ("geomId", "<u4"),
