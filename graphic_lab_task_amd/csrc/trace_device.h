@@ -711,46 +711,42 @@ __device__ __forceinline__ int ot_new_node(float txm, int x, float tym, int y, f
   return (txm < tym) ? (txm < tzm ? x : z) : (tym < tzm ? y : z);
 }
 
+// A stack entry of the octree walk, 8 bytes: the node and where it sits under its parent. The
+// node's position is not stored: every entry's parent is an ancestor (or itself) of the node the
+// walk expanded last (a depth-first stack holds deeper levels above shallower ones, and a level's
+// entries all come from its most recent expansion), so it is that node's position shifted down
+// to the parent's level, times two, plus the entry's octant bits.
 struct OTEntry {
   uint32_t node_id;
-  uint32_t info;    // > 0: leaf (its data offset)
-  uint32_t p_xy;    // (x << 16) | y
-  uint32_t p_zs;    // (z << 16) | level size
+  uint32_t meta;   // bits 0-2 octant (x<<2 | y<<1 | z), bits 3-6 level, bit 7 leaf with data (info > 0)
 };
 constexpr int OT_STACK = 32;
-constexpr int OT_LDS = 8;    // entries kept in LDS when the caller provides room
+constexpr int OT_LDS = 16;   // entries kept in LDS when the caller provides room (2 words each)
 
 // The walk's stack: entries below OT_LDS in the caller's LDS columns (entry k, word j at
-// col[(4k + j) * BLOCK]) when `col` is set, the others in private memory, which a walk rarely
-// reaches (private memory is scratch, written back through the caches to HBM).
+// col[(2k + j) * BLOCK]) when `col` is set, the others in private memory, which only walks
+// deeper than OT_LDS entries reach.
 template <int BLOCK>
 struct OTStack {
   lds_u32* col;
   OTEntry priv[OT_STACK];
   __device__ __forceinline__ OTEntry get(int k) const {
-    if (col && k < OT_LDS)
-      return OTEntry{col[(4 * k) * BLOCK], col[(4 * k + 1) * BLOCK], col[(4 * k + 2) * BLOCK], col[(4 * k + 3) * BLOCK]};
+    if (col && k < OT_LDS) return OTEntry{col[(2 * k) * BLOCK], col[(2 * k + 1) * BLOCK]};
     return priv[k];
   }
   __device__ __forceinline__ void set(int k, const OTEntry& e) {
     if (col && k < OT_LDS) {
-      col[(4 * k) * BLOCK] = e.node_id;
-      col[(4 * k + 1) * BLOCK] = e.info;
-      col[(4 * k + 2) * BLOCK] = e.p_xy;
-      col[(4 * k + 3) * BLOCK] = e.p_zs;
+      col[(2 * k) * BLOCK] = e.node_id;
+      col[(2 * k + 1) * BLOCK] = e.meta;
     } else {
       priv[k] = e;
     }
-  }
-  __device__ __forceinline__ void set_info(int k, uint32_t info) {
-    if (col && k < OT_LDS) col[(4 * k + 1) * BLOCK] = info;
-    else priv[k].info = info;
   }
 };
 
 // Walks the octree in front-to-back child order with mirrored coordinates (octant mask `a`)
 // and stops at the first leaf whose surface is hit. `ot_col`: LDS columns for the top of the
-// stack (OT_LDS x 4 words per lane, stride BLOCK; may be null: all in private memory).
+// stack (OT_LDS x 2 words per lane, stride BLOCK; may be null: all in private memory).
 // Returns the bytes read from the octree in 16-byte units (node records and voxel values).
 template <int BLOCK = 64>
 __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir,
@@ -777,11 +773,20 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g,
   const float old_t = hit.t;
   const uint32_t type = g.type;
   const uint32_t root = g.offset.x;
-  stack.set(0, OTEntry{root, 0u, 0u, 1u});
+  // position and level of the node expanded last (the root's until the first expansion)
+  uint32_t cx = 0, cy = 0, cz = 0, cl = 0;
+  stack.set(0, OTEntry{root, 0u});
   while (top >= 0) {
-    OTEntry e = stack.get(top);
-    const uint32_t level_sz = e.p_zs & 0xFFFF;
-    const uint32_t px = e.p_xy >> 16, py = e.p_xy & 0xFFFF, pz = e.p_zs >> 16;
+    const OTEntry e = stack.get(top);
+    const uint32_t level = (e.meta >> 3) & 0xFu;
+    uint32_t px = 0, py = 0, pz = 0;
+    if (level > 0) {
+      const uint32_t sh = cl - (level - 1);
+      px = ((cx >> sh) << 1) | ((e.meta >> 2) & 1u);
+      py = ((cy >> sh) << 1) | ((e.meta >> 1) & 1u);
+      pz = ((cz >> sh) << 1) | (e.meta & 1u);
+    }
+    const uint32_t level_sz = 1u << level;
     float d = 1.0f / float(level_sz);
     const float3 p_f = make3(float(px), float(py), float(pz));
     const float3 t0 = t0_ + d * p_f * l_;
@@ -791,15 +796,16 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g,
       top--;
       continue;
     }
-    bool is_leaf = e.info > 0;
+    bool is_leaf = (e.meta & 0x80u) != 0;
+    SdfMultiOctreeNode n;
+    bool have_node = false;
     if (!is_leaf) {
       const float t = s.preset.fixed_lod > 0 ? 1.0f : fmax_ref(t0.x, fmax_ref(t0.y, t0.z));
       const float target_lod_size = (powf(2.0f, s.preset.level_of_detail) - 0.01f) / t;
       if (float(level_sz) > target_lod_size) {
-        const SdfMultiOctreeNode n = s.multi_nodes[e.node_id];
+        n = s.multi_nodes[e.node_id];
+        have_node = true;
         units++;
-        e.info = n.data_offset;
-        stack.set_info(top, e.info);
         is_leaf = true;
         if (n.voxel_count == 0) {
           top--;
@@ -825,14 +831,15 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g,
       const float3 start_q = sz * (start_pos - min_pos);
       const float qNear = fmax_ref(0.0f, sz * (tNear - fNearFar.x));
       const float qFar = sz * (fNearFar.y - fNearFar.x);
-      const SdfMultiOctreeNode n = s.multi_nodes[node_id];
+      if (!have_node) n = s.multi_nodes[node_id];
+      const uint32_t info = n.data_offset;
       units += 1 + 2 * n.voxel_count;
       const bool surface = (n.flags & MULTI_OCTREE_FLAG_SURFACE) != 0;
       const uint32_t primId = node_id - root;
       for (uint32_t v = 0; v < n.voxel_count; ++v) {
         float values[8];
 #pragma unroll
-        for (int i = 0; i < 8; i++) values[i] = s.multi_values[e.info + 8 * v + i];
+        for (int i = 0; i < 8; i++) values[i] = s.multi_values[info + 8 * v + i];
         local_surface_intersection(s, TYPE_SDF_MULTI_OCTREE, ray_dir, instId, geomId, values, primId, d, qNear, qFar,
                                    fNearFar, start_q, hit, nullptr, surface);
       }
@@ -848,22 +855,28 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g,
       const float3 tm = 0.5f * (t0 + t1);
       int curr = ot_first_node(t0, tm);
       const uint32_t children = s.multi_nodes[e.node_id].children_offset;
+      const uint32_t child_level = (level + 1) << 3;
       do {
         const uint32_t child = uint32_t(curr) ^ a;
         const SdfMultiOctreeNode cn = s.multi_nodes[children + child];
         units++;
         const bool child_is_node = cn.children_offset > 0;
         if ((cn.voxel_count > 0 || child_is_node) && buf_top < 4) {
+          // a leaf entry of the reference carries info = its data offset and counts as a leaf
+          // only when that is non-zero (info > 0); a node's info is 0
+          const bool leaf = !child_is_node && cn.data_offset > 0;
           buf[buf_top].node_id = children + child;
-          buf[buf_top].info = child_is_node ? 0u : cn.data_offset;
-          buf[buf_top].p_xy = (e.p_xy << 1) | (((uint32_t(curr) & 4u) << (16 - 2)) | ((uint32_t(curr) & 2u) >> 1));
-          buf[buf_top].p_zs = (e.p_zs << 1) | ((uint32_t(curr) & 1u) << 16);
+          buf[buf_top].meta = uint32_t(curr) | child_level | (leaf ? 0x80u : 0u);
           buf_top++;
         }
         const uint32_t ni = nn[curr];
         curr = ot_new_node((curr & 4) ? t1.x : tm.x, int(ni & 0xF), (curr & 2) ? t1.y : tm.y, int((ni >> 4) & 0xF),
                            (curr & 1) ? t1.z : tm.z, int(ni >> 8));
       } while (curr < 8);
+      cx = px;
+      cy = py;
+      cz = pz;
+      cl = level;
       // children replace the current entry, the first one on top; entries past the stack are dropped
       for (int i = 0; i < buf_top; i++)
         if (top + i < OT_STACK) stack.set(top + i, buf[buf_top - i - 1]);

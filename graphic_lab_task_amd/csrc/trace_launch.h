@@ -24,7 +24,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_closest(SceneView s, const floa
                                                          uint32_t budget, TailQueue* queue, uint32_t group_log2) {
   __shared__ uint32_t lds[S * BLOCK];
   // world ray of each lane while its BLAS is walked (+ the top of the multi-octree walk's stack)
-  __shared__ uint32_t keep[(6 + ((KINDS & LK_MULTI) ? 4 * OT_LDS : 0)) * BLOCK];
+  __shared__ uint32_t keep[(6 + ((KINDS & LK_MULTI) ? 2 * OT_LDS : 0)) * BLOCK];
   // top BFS levels of the cached geometry's BLAS: dynamic, s.cache_pairs entries (none for a
   // scene without a cached geometry, which then keeps its LDS for more resident waves)
   extern __shared__ BVHNodePair top_pairs[];

@@ -161,7 +161,7 @@ __device__ __forceinline__ void finish_hit(const SceneView& s, CRT_Hit& hit) {
 // ray is checked again before its BLAS walk). `keep` (LDS, 6 words per lane at keep[k * BLOCK],
 // may be null): the world ray is parked there and read back after each BLAS walk, so that its
 // registers are free while the BLAS is walked (the inverse direction is recomputed, bit for bit);
-// kernels that walk multi-octrees extend it by 4 * OT_LDS words per lane for the walk's stack.
+// kernels that walk multi-octrees extend it by 2 * OT_LDS words per lane for the walk's stack.
 // `rec` (brick-set kernels, may be null): the brick march records the voxel of every surface hit
 // it makes; an SDF walk ends at its first hitting leaf and a later hit is always closer, so after
 // the walk it holds the voxel of the final hit when that hit is a brick-set hit.
