@@ -521,6 +521,7 @@ def run_trace(name, args, torch, g, cpu_target_s):
     scene.set_tail_config(args.budget, args.tail_waves)
     scene.set_leaf_postponement(args.leaf_postpone)
     scene.set_ray_interleave(args.ray_interleave)
+    scene.set_persistent(bool(args.persistent))
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
     if isinstance(w, tuple):   # explicit ray buffers
@@ -571,11 +572,11 @@ def run_trace(name, args, torch, g, cpu_target_s):
     if cpu_target_s > 0:
         cpu = cpu_baseline([scene], pos.cpu().numpy(), dirs.cpu().numpy(), threads, cpu_target_s)
     launch = ({k: tune[k] for k in ("block", "lds_stack", "xcd_remap", "ray_interleave", "budget", "tail_waves",
-                                    "leaf_postpone")}
+                                    "leaf_postpone", "persistent")}
               if tune else
               {"block": args.block, "lds_stack": args.lds_stack, "xcd_remap": args.xcd_remap,
                "ray_interleave": args.ray_interleave, "budget": args.budget, "tail_waves": args.tail_waves,
-               "leaf_postpone": args.leaf_postpone})
+               "leaf_postpone": args.leaf_postpone, "persistent": args.persistent})
     desc.update({"hit_fraction": round(float((hits_np["primId"] != g.MISS).mean()), 4),
                  "scene_build_s": round(build_s, 2), "launch": launch, "autotuned": tune is not None,
                  "stack_overflow_rays": overflow, "ms_eager": round(ms_eager, 4), "timed": "hip graph replay",
@@ -618,6 +619,8 @@ def main():
                     help="pass-1 lanes per interleaved group when not autotuning (0: thread order)")
     ap.add_argument("--leaf-postpone", type=int, default=-1, choices=[-1, 0, 1],
                     help="triangle leaf postponement when not autotuning (-1: on for one instance)")
+    ap.add_argument("--persistent", type=int, default=0, choices=[0, 1],
+                    help="persistent single-pass kernel when not autotuning")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for the HBM byte counters)")
     args = ap.parse_args()

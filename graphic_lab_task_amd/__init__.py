@@ -101,7 +101,7 @@ class _TuneResult(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("lds_stack", ctypes.c_int32), ("xcd_remap", ctypes.c_int32),
                 ("budget", ctypes.c_uint32), ("tail_waves", ctypes.c_uint32), ("best_ms", ctypes.c_float),
                 ("configs_tried", ctypes.c_uint32), ("leaf_postpone", ctypes.c_int32),
-                ("ray_interleave", ctypes.c_uint32)]
+                ("ray_interleave", ctypes.c_uint32), ("persistent", ctypes.c_int32)]
 
 
 class _Buffer(ctypes.Structure):
@@ -163,6 +163,7 @@ def lib() -> ctypes.CDLL:
             "lrt_set_tail_config": ([vp, u32, u32], i32),
             "lrt_set_leaf_postponement": ([vp, i32], i32),
             "lrt_set_ray_interleave": ([vp, u32], i32),
+            "lrt_set_persistent": ([vp, i32], i32),
             "lrt_autotune": ([vp, vp, vp, vp, u64, vp, vp], i32),
             "lrt_sbs_param_layout": ([vp, u32, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u64)], i32),
             "lrt_sbs_forward_dev": ([vp, u32, vp, vp, vp, u64, vp, vp], i32),
@@ -208,7 +209,7 @@ def exported_symbols():
             "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
             "lrt_trace_closest_host", "lrt_trace_closest_noalloc_dev", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
-            "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_set_ray_interleave",
+            "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_set_ray_interleave", "lrt_set_persistent",
             "lrt_autotune", "lrt_sbs_param_layout",
             "lrt_sbs_forward_dev", "lrt_sbs_backward_dev", "lrt_sbs_l2_backward_dev", "lrt_sbs_render_l2_backward_dev", "lrt_sbs_recompute_misses",
             "lrt_sbs_batch_create", "lrt_sbs_batch_run", "lrt_sbs_batch_destroy",
@@ -454,6 +455,10 @@ class SceneRT:
         """Pass-1 ray-to-lane assignment: groups of `lanes_per_group` lanes take rays from distant
         parts of the batch (0 or 64: thread order)."""
         _check(lib().lrt_set_ray_interleave(self._h, lanes_per_group))
+
+    def set_persistent(self, on: bool = True):
+        """One persistent single-pass kernel with per-lane ray refill (True) or the two-pass grid launch."""
+        _check(lib().lrt_set_persistent(self._h, 1 if on else 0))
 
     def set_leaf_postponement(self, mode: int = -1):
         """Triangle leaf postponement of the one-ray-per-lane pass: -1 automatic (on for one

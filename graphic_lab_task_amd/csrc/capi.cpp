@@ -88,6 +88,7 @@ lrt::TraceArgs trace_args(lrt_scene* s, const float* pos, const float* dir, lrt_
   a.ray_groups = s->s.ray_groups;
   // extra leaf visits would change the counts of that mode too
   a.postpone_tri = s->s.preset.render_mode != lrt::MULTI_RENDER_MODE_INTERSECTION_COUNT && s->s.postpone_triangles();
+  a.persistent = s->s.persistent ? 1u : 0u;
   return a;
 }
 
@@ -103,7 +104,7 @@ lrt_status trace_common(lrt_scene* s, const float* pos, const float* dir, lrt_hi
   // between ours (they share the stream's tail queue) or grow the buffer under us
   lrt::Scene::StreamWork& w = s->s.stream_work(stream);
   std::lock_guard<std::mutex> lock(w.mu);
-  hipError_t e = s->s.work_for(w, stream, size_t(n), &a.spill, &a.queue);
+  hipError_t e = s->s.work_for(w, stream, size_t(n), &a.spill, &a.queue, &a.pcount);
   if (e != hipSuccess) return hip_fail(e, "work buffer");
   e = lrt::launch_trace(s->s.make_view(), a, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "trace launch");
@@ -178,7 +179,8 @@ lrt_status sbs_diff_common(lrt_scene* s, uint32_t geom_id, const float* params, 
   lrt::TailQueue* q = nullptr;
   lrt::Scene::StreamWork& w = s->s.stream_work(stream);
   std::lock_guard<std::mutex> lock(w.mu);
-  hipError_t e = s->s.work_for(w, stream, size_t(n), &a.spill, &q);
+  uint32_t* pcount = nullptr;
+  hipError_t e = s->s.work_for(w, stream, size_t(n), &a.spill, &q, &pcount);
   if (e != hipSuccess) return hip_fail(e, "work buffer");
   // occupancy masks from the caller's parameters (they differ from the committed values)
   lrt::SceneView v = s->s.make_view();
@@ -556,6 +558,15 @@ lrt_status lrt_set_ray_interleave(lrt_scene* s, uint32_t lanes_per_group) {
   });
 }
 
+lrt_status lrt_set_persistent(lrt_scene* s, int on) {
+  return guarded([&]() -> lrt_status {
+    if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+    if (on != 0 && on != 1) return fail(LRT_ERR_INVALID_ARG, "on must be 0 or 1");
+    s->s.persistent = on;
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_set_leaf_postponement(lrt_scene* s, int mode) {
   return guarded([&]() -> lrt_status {
     if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
@@ -573,7 +584,7 @@ lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_fa
     if (!pos_near || !dir_far || !hits || n == 0) return fail(LRT_ERR_INVALID_ARG, "autotune needs a non-empty ray batch");
     lrt::Scene& sc = s->s;
     const lrt::TuneConfig start{sc.block, sc.lds_stack, sc.xcd_remap, sc.budget, sc.tail_waves,
-                                sc.postpone_triangles() ? 1 : 0, sc.ray_groups};
+                                sc.postpone_triangles() ? 1 : 0, sc.ray_groups, sc.persistent};
     // the option only exists for triangle-only scenes
     const bool tune_postpone = sc.leaf_kinds() == lrt::LK_TRI;
     auto apply = [&](const lrt::TuneConfig& c) {
@@ -583,6 +594,7 @@ lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_fa
       sc.budget = c.budget;
       sc.tail_waves = c.tail_waves;
       sc.ray_groups = c.ray_groups;
+      sc.persistent = c.persistent;
       if (tune_postpone) sc.leaf_postpone = c.postpone;
     };
     lrt_status st = LRT_OK;
@@ -610,6 +622,7 @@ lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_fa
       out->configs_tried = r.tried;
       out->leaf_postpone = r.best.postpone;
       out->ray_interleave = r.best.ray_groups;
+      out->persistent = r.best.persistent;
     }
     return LRT_OK;
   });

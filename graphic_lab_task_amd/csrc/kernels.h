@@ -30,7 +30,12 @@ struct TraceArgs {
   uint32_t      postpone_tri; // 1: postpone triangle leaves (triangle-only scenes; pass 1)
   uint32_t      ray_groups;   // pass 1: lanes per group dealt to distant parts of the batch
                               // (power of two; 0 or 64: rays in thread order)
+  uint32_t      persistent;   // 1: one persistent single-pass kernel with per-lane ray refill
+  uint32_t*     pcount;       // its two counters (device, zero between calls)
 };
+// Rays beyond n that the work buffer's spill covers, so that a persistent grid (whole blocks of
+// resident lanes, at most ceil(n / block) blocks) can index it by lane slot.
+constexpr uint32_t SPILL_SLACK_RAYS = 256;
 
 struct EyeRayParams {
   float    proj_inv[16];        // inverse4x4(proj), column-major

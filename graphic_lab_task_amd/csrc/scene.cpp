@@ -42,11 +42,15 @@ Scene::StreamWork& Scene::stream_work(void* stream) {
   return *w;
 }
 
-hipError_t Scene::work_for(StreamWork& w, void* stream, size_t n, uint32_t** spill, TailQueue** queue) {
+hipError_t Scene::work_for(StreamWork& w, void* stream, size_t n, uint32_t** spill, TailQueue** queue,
+                           uint32_t** pcount) {
+  // 256-B header: the tail queue's counters and list pointer at 0, the persistent kernel's two
+  // counters at byte 64; then the queue list (4 B per ray) and the spill, sized for the smallest
+  // LDS part so that a later launch-config change never overruns it, and for a persistent grid's
+  // lane slots (up to SPILL_SLACK_RAYS past n)
   const size_t head = 256;
   const size_t list_bytes = (n * 4 + 255) / 256 * 256;
-  // spill sized for the smallest LDS part so a later launch config change never overruns it
-  const size_t need = head + list_bytes + n * size_t(STACK_SIZE - 8) * 4;
+  const size_t need = head + list_bytes + (n + SPILL_SLACK_RAYS) * size_t(STACK_SIZE - 8) * 4;
   if (w.cap_rays < n || w.buf.bytes() < need) {
     // earlier calls on this stream may still be reading the old buffer
     hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
@@ -56,8 +60,10 @@ hipError_t Scene::work_for(StreamWork& w, void* stream, size_t n, uint32_t** spi
       w.cap_rays = 0;
       return e;
     }
+    uint8_t header[256] = {};
     TailQueue q{0u, 0u, reinterpret_cast<uint32_t*>(static_cast<char*>(w.buf.ptr()) + head)};
-    e = hipMemcpy(w.buf.ptr(), &q, sizeof(q), hipMemcpyHostToDevice);
+    std::memcpy(header, &q, sizeof(q));
+    e = hipMemcpy(w.buf.ptr(), header, sizeof(header), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       w.cap_rays = 0;
       return e;
@@ -65,6 +71,7 @@ hipError_t Scene::work_for(StreamWork& w, void* stream, size_t n, uint32_t** spi
     w.cap_rays = n;
   }
   *queue = static_cast<TailQueue*>(w.buf.ptr());
+  *pcount = reinterpret_cast<uint32_t*>(static_cast<char*>(w.buf.ptr()) + 64);
   *spill = reinterpret_cast<uint32_t*>(static_cast<char*>(w.buf.ptr()) + head + (w.cap_rays * 4 + 255) / 256 * 256);
   return hipSuccess;
 }

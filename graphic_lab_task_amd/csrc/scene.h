@@ -81,6 +81,7 @@ struct Scene {
   // Triangle leaf postponement in pass 1: -1 automatic (on for a triangle-only scene with one
   // instance), 0 off, 1 on.
   int leaf_postpone = -1;
+  int persistent = 0;   // 1: persistent single-pass trace kernel with per-lane ray refill
   bool postpone_triangles() const;
 
   // Work memory of one stream: the tail queue (256-B header + list), the deep-stack spill and
@@ -97,8 +98,8 @@ struct Scene {
   std::mutex work_mu;   // guards the map only
   std::map<void*, std::unique_ptr<StreamWork>> work;
   StreamWork& stream_work(void* stream);
-  // With w.mu held: queue and spill for n rays on `stream`.
-  hipError_t work_for(StreamWork& w, void* stream, size_t n, uint32_t** spill, TailQueue** queue);
+  // With w.mu held: queue, spill and persistent-kernel counters for n rays on `stream`.
+  hipError_t work_for(StreamWork& w, void* stream, size_t n, uint32_t** spill, TailQueue** queue, uint32_t** pcount);
   // With w.mu held: the stream's occupancy-mask buffer.
   hipError_t mask_for(StreamWork& w, void* stream, uint32_t** mask);
 

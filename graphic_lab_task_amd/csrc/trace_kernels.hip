@@ -7,22 +7,25 @@
 
 namespace lrt {
 
-// Work memory of the allocation-free trace: a 256-B tail-queue header, the queue (one word per
-// ray) and the deep-stack spill (STACK_SIZE - 8 words per ray, enough for every LDS stack depth),
-// for at most LRT_NOALLOC_MAX_RAYS rays. Host code never uses this name as an address: each
+// Work memory of the allocation-free trace: a 256-B header (tail queue at word 0, the persistent
+// kernel's counters at word 16), the queue (one word per ray) and the deep-stack spill
+// (STACK_SIZE - 8 words per ray, enough for every LDS stack depth, plus the persistent grid's
+// slack), for at most LRT_NOALLOC_MAX_RAYS rays. Host code never uses this name as an address: each
 // device's copy is resolved once with hipGetSymbolAddress and that pointer is passed on.
 constexpr size_t NOALLOC_HEAD_WORDS = 64;
 constexpr size_t NOALLOC_WORDS =
-    NOALLOC_HEAD_WORDS + LRT_NOALLOC_MAX_RAYS + size_t(LRT_NOALLOC_MAX_RAYS) * (STACK_SIZE - 8);
+    NOALLOC_HEAD_WORDS + LRT_NOALLOC_MAX_RAYS + size_t(LRT_NOALLOC_MAX_RAYS + SPILL_SLACK_RAYS) * (STACK_SIZE - 8);
 static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
 
 // Sets the queue header at the start of every allocation-free call (a kernel, not a memset, so a
 // captured call records kernels only).
-__global__ void k_queue_init(TailQueue* queue, uint32_t* list) {
+__global__ void k_queue_init(TailQueue* queue, uint32_t* list, uint32_t* pcount) {
   if (threadIdx.x == 0) {
     queue->count = 0;
     queue->next = 0;
     queue->list = list;
+    pcount[0] = 0;
+    pcount[1] = 0;
   }
 }
 
@@ -117,12 +120,11 @@ hipError_t launch_trace_noalloc(const SceneView& s, TraceArgs a, int device, hip
   hipError_t e = noalloc_scratch(device, &base);
   if (e != hipSuccess) return e;
   a.queue = reinterpret_cast<TailQueue*>(base);
+  a.pcount = base + 16;
   uint32_t* list = base + NOALLOC_HEAD_WORDS;
   a.spill = list + LRT_NOALLOC_MAX_RAYS;
-  if (a.budget != 0) {
-    hipLaunchKernelGGL(k_queue_init, dim3(1), dim3(64), 0, stream, a.queue, list);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
+  hipLaunchKernelGGL(k_queue_init, dim3(1), dim3(64), 0, stream, a.queue, list, a.pcount);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   return launch_trace(s, a, stream);
 }
 

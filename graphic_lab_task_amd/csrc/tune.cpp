@@ -57,8 +57,13 @@ hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, 
           c.lds_stack = lds;
           c.budget = 0;
           c.ray_groups = groups;
+          c.persistent = 0;
           if (tune_postpone) c.postpone = pp;
           if ((e = consider(c, true)) != hipSuccess) break;
+          if (groups == 0) {   // the persistent kernel takes rays in order: no interleaving
+            c.persistent = 1;
+            if ((e = consider(c, false)) != hipSuccess) break;
+          }
         }
         if (e != hipSuccess) break;
       }

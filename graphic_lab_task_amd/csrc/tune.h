@@ -13,6 +13,7 @@ struct TuneConfig {
   uint32_t budget, tail_waves;
   int postpone;          // triangle leaf postponement (0/1)
   uint32_t ray_groups;   // pass-1 ray interleaving: lanes per group (0: off)
+  int persistent;        // 1: persistent single-pass kernel with per-lane ray refill
 };
 
 struct TuneResult {
@@ -23,8 +24,8 @@ struct TuneResult {
 
 // Two stages, each candidate timed with HIP events on `stream` (one warm-up launch, then the
 // minimum of `reps`): (1) block size x LDS stack entries x ray interleaving (off, groups of 8
-// or 32 lanes) with the single-pass walk, (2) step budget x tail waves of the two-pass scheme on the
-// three best shapes from (1); both stages also try triangle leaf postponement off and on when
+// or 32 lanes) with the single-pass walk, and block size x LDS stack entries as the persistent
+// kernel, (2) step budget x tail waves of the two-pass scheme on the three best grid shapes from (1); both stages also try triangle leaf postponement off and on when
 // `tune_postpone`.
 hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, hipStream_t stream,
                     const TuneConfig& start, bool tune_postpone, int reps, TuneResult* out);

@@ -115,7 +115,7 @@ lrt_status lrt_trace_closest_dev(lrt_scene* s, const float* pos_near, const floa
  * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc). Same arguments and the same hit records as
  * lrt_trace_closest_dev; its tail queue and stack spill live in a fixed device array of the
- * library (about 19 MB, one per GPU, held until the process ends) instead of the per-stream work
+ * library (about 19.2 MB, one per GPU, held until the process ends) instead of the per-stream work
  * buffer, so no call allocates. n may be at most LRT_NOALLOC_MAX_RAYS (LRT_ERR_INVALID_ARG
  * otherwise); the current device must be the scene's. Calls that overlap on one GPU (on any
  * streams or threads) share that array and race: serialise them. */
@@ -292,10 +292,17 @@ typedef struct lrt_tune_result {
   uint32_t configs_tried;
   int32_t leaf_postpone;   /* triangle leaf postponement chosen (0/1; tuned on triangle-only scenes) */
   uint32_t ray_interleave; /* lanes per group chosen (lrt_set_ray_interleave; 0 = off) */
+  int32_t persistent;      /* persistent kernel chosen (lrt_set_persistent; 0/1) */
 } lrt_tune_result;
 
+/* Trace launch shape: 1 runs every closest-hit / any-hit call as ONE persistent kernel of
+ * resident blocks whose lanes each take the next ray as soon as their current ray is finished
+ * (single pass; the step budget and ray interleaving do not apply), 0 (default) as the two-pass
+ * grid launch. Same hit records either way. No counterpart in the reference. */
+lrt_status lrt_set_persistent(lrt_scene* s, int on);
+
 /* Chooses the launch configuration (block size, LDS stack entries, ray interleaving, step
- * budget, tail waves, and on triangle-only scenes the leaf postponement) by timing closest-hit calls on the
+ * budget, tail waves, persistent or grid launch, and on triangle-only scenes the leaf postponement) by timing closest-hit calls on the
  * caller's device ray batch (`hits` is overwritten with the batch's hit records), and keeps
  * the fastest on the scene. Every configuration returns identical records, up to the ulp-level
  * ties noted at lrt_set_leaf_postponement. No counterpart in the reference (its CPU loop has
