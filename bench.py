@@ -252,6 +252,7 @@ def run_batch(args, torch, g, cpu_target_s):
     build_s = time.time() - t0
     n = sum(it["pos"].shape[0] for it in items)
     batch = g.SbsBatch(items)
+    tune = None if args.no_autotune else batch.autotune(sh)
 
     # algorithmic bytes of one batched step: per item, the traversal of its rays (instrumented
     # launch), ray I/O and the per-hit gradient scatter, plus the set-up kernels: gradient zeroing
@@ -301,6 +302,7 @@ def run_batch(args, torch, g, cpu_target_s):
             "ms_per_scene_fused_launches": round(ms_plain_fused, 4),
             "batched_vs_per_scene_fused": round(ms_plain_fused / ms, 2),
             "recompute_misses": int(sum(it["scene"].sbs_recompute_misses() for it in items)),
+            "launch": tune, "autotuned": tune is not None,
             "algorithmic_bytes_per_ray": round(alg_bytes / n, 1)}
     cpu = None
     if cpu_target_s > 0:

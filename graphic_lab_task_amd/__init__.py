@@ -104,6 +104,11 @@ class _TuneResult(ctypes.Structure):
                 ("ray_interleave", ctypes.c_uint32), ("persistent", ctypes.c_int32)]
 
 
+class _BatchTuneResult(ctypes.Structure):
+    _fields_ = [("lds_stack", ctypes.c_int32), ("record", ctypes.c_int32), ("grid", ctypes.c_uint32),
+                ("best_ms", ctypes.c_float), ("configs_tried", ctypes.c_uint32)]
+
+
 class _Buffer(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("bytes", ctypes.c_size_t)]
 
@@ -173,6 +178,7 @@ def lib() -> ctypes.CDLL:
             "lrt_sbs_recompute_misses": ([vp, ctypes.POINTER(u64), i32], i32),
             "lrt_sbs_batch_create": ([vp, u32, ctypes.POINTER(vp)], i32),
             "lrt_sbs_batch_run": ([vp, vp], i32),
+            "lrt_sbs_batch_autotune": ([vp, vp, ctypes.POINTER(_BatchTuneResult)], i32),
             "lrt_sbs_batch_destroy": ([vp], i32),
             "lrt_stack_overflow_count": ([vp, ctypes.POINTER(u64), i32], i32),
             "lrt_get_array": ([vp, cp, vp, ctypes.POINTER(sz)], i32),
@@ -212,7 +218,7 @@ def exported_symbols():
             "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_set_ray_interleave", "lrt_set_persistent",
             "lrt_autotune", "lrt_sbs_param_layout",
             "lrt_sbs_forward_dev", "lrt_sbs_backward_dev", "lrt_sbs_l2_backward_dev", "lrt_sbs_render_l2_backward_dev", "lrt_sbs_recompute_misses",
-            "lrt_sbs_batch_create", "lrt_sbs_batch_run", "lrt_sbs_batch_destroy",
+            "lrt_sbs_batch_create", "lrt_sbs_batch_run", "lrt_sbs_batch_autotune", "lrt_sbs_batch_destroy",
             "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
             "lrt_camera_inverse", "lrt_gen_icosphere", "lrt_gen_geodesic", "lrt_sdf_spheres_frame_octree",
             "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed", "lrt_resolve_dev", "lrt_tonemap_dev",
@@ -506,6 +512,12 @@ class SbsBatch:
 
     def run(self, stream: int = 0):
         _check(lib().lrt_sbs_batch_run(self._h, stream))
+
+    def autotune(self, stream: int = 0) -> dict:
+        """Times the batch kernel's shapes on this plan (overwriting its outputs) and keeps the fastest."""
+        r = _BatchTuneResult()
+        _check(lib().lrt_sbs_batch_autotune(self._h, stream, ctypes.byref(r)))
+        return {f: getattr(r, f) for f, _ in _BatchTuneResult._fields_}
 
     def __del__(self):
         if getattr(self, "_h", None):

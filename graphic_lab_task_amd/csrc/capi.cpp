@@ -26,6 +26,7 @@ struct lrt_sbs_batch {
   std::vector<uint64_t> gen_at_create;
   lrt::DevBuf d_items, d_begins, d_masks, d_spill, d_counter;
   lrt::SbsBatchLaunch L{};
+  uint32_t resident[6] = {};   // resident blocks per variant: (LDS stack 8 << v/2, record v%2)
 };
 
 namespace {
@@ -746,17 +747,26 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
       b->items[k].mask = static_cast<uint32_t*>(b->d_masks.ptr()) + mask_offset[k];
       b->items[k].view.sbs_mask = b->items[k].mask;
     }
-    int cus = 0, per_cu = 0;
+    int cus = 0;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device)) != hipSuccess)
       return hip_fail(e, "device attribute");
-    if ((e = lrt::sbs_batch_blocks_per_cu(&per_cu)) != hipSuccess) return hip_fail(e, "occupancy");
-    const uint32_t grid = uint32_t(std::max(1, cus * std::max(1, per_cu)));
+    // resident blocks of every kernel variant (LDS stack 8/16/32 x voxel record off/on); the spill
+    // covers the largest grid
+    uint32_t max_grid = 1;
+    for (int v = 0; v < 6; ++v) {
+      int per_cu = 0;
+      if ((e = lrt::sbs_batch_blocks_per_cu(8 << (v / 2), v % 2, &per_cu)) != hipSuccess)
+        return hip_fail(e, "occupancy");
+      b->resident[v] = uint32_t(std::max(1, cus * std::max(1, per_cu)));
+      max_grid = std::max(max_grid, b->resident[v]);
+    }
+    const uint32_t grid = b->resident[2];   // LDS stack 16, hit brick marched again
     const size_t n1 = size_t(n_items) + 1;
     if ((e = b->d_items.upload(b->items.data(), b->items.size() * sizeof(lrt::SbsBatchItem))) != hipSuccess)
       return hip_fail(e, "batch table");
     if ((e = b->d_begins.upload(chunk_begin.data(), n1 * sizeof(uint32_t))) != hipSuccess)
       return hip_fail(e, "batch table");
-    if ((e = b->d_spill.reserve(size_t(grid) * lrt::SBS_BATCH_BLOCK * (lrt::STACK_SIZE - 16) * 4)) != hipSuccess)
+    if ((e = b->d_spill.reserve(size_t(max_grid) * lrt::SBS_BATCH_BLOCK * (lrt::STACK_SIZE - 8) * 4)) != hipSuccess)
       return hip_fail(e, "batch spill");
     if ((e = b->d_counter.reserve(256)) != hipSuccess) return hip_fail(e, "batch counter");
     lrt::SbsBatchLaunch& L = b->L;
@@ -769,6 +779,8 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
     L.counter = static_cast<uint32_t*>(b->d_counter.ptr());
     L.spill = static_cast<uint32_t*>(b->d_spill.ptr());
     L.grid = grid;
+    L.lds_stack = 16;
+    L.record = 0;
     auto blocks = [](uint64_t work, uint64_t cap) { return uint32_t(std::min<uint64_t>(cap, (work + 255) / 256)); };
     L.zero_blocks = blocks(max_zero4, 1024);
     L.mask_blocks = blocks(max_bits, 4096);
@@ -780,19 +792,85 @@ lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_item
   });
 }
 
+namespace {
+// The plan may run on the current device and its scenes are unchanged since it was made.
+lrt_status batch_usable(const lrt_sbs_batch* b) {
+  int dev = -1;
+  hipError_t ge = hipGetDevice(&dev);
+  if (ge != hipSuccess) return hip_fail(ge, "no device");
+  if (dev != b->device) return fail(LRT_ERR_INVALID_ARG, "batch plan was created on another device");
+  for (size_t k = 0; k < b->scene_gen.size(); ++k) {
+    const std::shared_ptr<uint64_t> g = b->scene_gen[k].lock();
+    if (!g || *g != b->gen_at_create[k])
+      return fail(LRT_ERR_INVALID_ARG, "batch plan is stale: item " + std::to_string(k) +
+                                           "'s scene was changed, re-committed or destroyed since the plan was made");
+  }
+  return LRT_OK;
+}
+}  // namespace
+
+lrt_status lrt_sbs_batch_autotune(lrt_sbs_batch* b, void* stream, lrt_sbs_batch_tune_result* out) {
+  return guarded([&]() -> lrt_status {
+    if (!b) return fail(LRT_ERR_INVALID_ARG, "null batch");
+    lrt_status st = batch_usable(b);
+    if (st != LRT_OK) return st;
+    const hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const lrt::SbsBatchLaunch start = b->L;
+    lrt::SbsBatchLaunch best = start;
+    float best_ms = 1e30f;
+    uint32_t tried = 0;
+    // LDS stack 8 / 16 / 32 entries x voxel record off / on x 1/2, 3/4 or all of the resident blocks
+    for (int v = 0; v < 6 && e == hipSuccess; ++v) {
+      for (int quarters : {2, 3, 4}) {
+        lrt::SbsBatchLaunch c = start;
+        c.lds_stack = 8 << (v / 2);
+        c.record = v % 2;
+        c.grid = std::max<uint32_t>(1u, b->resident[v] * uint32_t(quarters) / 4u);
+        float ms = 1e30f;
+        if ((e = lrt::launch_sbs_batch(c, hs)) != hipSuccess) break;   // warm-up
+        for (int r = 0; r < 3 && e == hipSuccess; ++r) {
+          float t = 0.0f;
+          if ((e = hipEventRecord(e0, hs)) != hipSuccess) break;
+          if ((e = lrt::launch_sbs_batch(c, hs)) != hipSuccess) break;
+          if ((e = hipEventRecord(e1, hs)) != hipSuccess) break;
+          if ((e = hipEventSynchronize(e1)) != hipSuccess) break;
+          if ((e = hipEventElapsedTime(&t, e0, e1)) != hipSuccess) break;
+          ms = std::min(ms, t);
+        }
+        if (e != hipSuccess) break;
+        ++tried;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = c;
+        }
+      }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) {
+      b->L = start;
+      return hip_fail(e, "batch autotune");
+    }
+    b->L = best;
+    if (out) {
+      out->lds_stack = best.lds_stack;
+      out->record = best.record;
+      out->grid = best.grid;
+      out->best_ms = best_ms;
+      out->configs_tried = tried;
+    }
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_sbs_batch_run(lrt_sbs_batch* b, void* stream) {
   return guarded([&]() -> lrt_status {
     if (!b) return fail(LRT_ERR_INVALID_ARG, "null batch");
-    int dev = -1;
-    hipError_t ge = hipGetDevice(&dev);
-    if (ge != hipSuccess) return hip_fail(ge, "no device");
-    if (dev != b->device) return fail(LRT_ERR_INVALID_ARG, "batch plan was created on another device");
-    for (size_t k = 0; k < b->scene_gen.size(); ++k) {
-      const std::shared_ptr<uint64_t> g = b->scene_gen[k].lock();
-      if (!g || *g != b->gen_at_create[k])
-        return fail(LRT_ERR_INVALID_ARG, "batch plan is stale: item " + std::to_string(k) +
-                                             "'s scene was changed, re-committed or destroyed since the plan was made");
-    }
+    lrt_status st = batch_usable(b);
+    if (st != LRT_OK) return st;
     const hipError_t e = lrt::launch_sbs_batch(b->L, reinterpret_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "batch launch");
     return LRT_OK;

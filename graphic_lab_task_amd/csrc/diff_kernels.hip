@@ -239,7 +239,9 @@ __global__ void k_batch_mask_summary(SbsBatchLaunch L) {
 // a wave wait for another's slow rays; the barrier also keeps the wave converged around the
 // broadcast, which a lane-0 atomic followed by a lane read does not guarantee). Rays stay in
 // batch order: here throughput counts, and neighbouring rays of similar cost share a chunk.
-template <int S, int BLOCK>
+// S: stack entries in LDS. REC: the walk's own brick march records the hit voxel (12 more
+// registers held through the walk) instead of a second march over the hit brick.
+template <int S, int BLOCK, bool REC>
 __global__ __launch_bounds__(BLOCK) void k_sbs_batch(SbsBatchLaunch L) {
   static_assert(BLOCK == 64 && SBS_BATCH_CHUNK == 64, "one wave per block, one chunk per wave");
   __shared__ uint32_t lds[S * BLOCK];
@@ -261,18 +263,21 @@ __global__ __launch_bounds__(BLOCK) void k_sbs_batch(SbsBatchLaunch L) {
       uint32_t overflow = 0;
       RayCounts cnt{};
       bool aborted = false;
-      // the hit voxel is found by a second march of the hit brick: recording it during the walk
-      // would hold 12 more registers through the walk, costing this kernel a wave per SIMD
       SbsVoxelRec rec;
       rec.t_local = __int_as_float(0x7fc00000);
       const CRT_Hit h = ray_query_nearest<LK_SBS, S, BLOCK, false, false>(item.view, item.args.pos[i], item.args.dir[i],
                                                                           st, overflow, cnt, nullptr, 0u, aborted,
-                                                                          nullptr);
+                                                                          nullptr, nullptr, REC ? &rec : nullptr);
       loss = diff_ray<3>(item.view, item.args, i, h, rec);
     }
     loss = wave_sum(loss);
     if (__lane_id() == 0 && item.args.loss) atomicAdd(item.args.loss, loss);
   }
+}
+
+template <int S, bool REC>
+const void* batch_kernel() {
+  return reinterpret_cast<const void*>(k_sbs_batch<S, SBS_BATCH_BLOCK, REC>);
 }
 
 template <int MODE>
@@ -285,8 +290,18 @@ hipError_t launch_mode(const SceneView& s, const SbsDiffArgs& a, hipStream_t str
 
 }  // namespace
 
-hipError_t sbs_batch_blocks_per_cu(int* blocks) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_sbs_batch<16, SBS_BATCH_BLOCK>, SBS_BATCH_BLOCK, 0);
+hipError_t sbs_batch_blocks_per_cu(int lds_stack, int record, int* blocks) {
+  const void* k = nullptr;
+  switch (lds_stack * 2 + (record ? 1 : 0)) {
+    case 16: k = batch_kernel<8, false>(); break;
+    case 17: k = batch_kernel<8, true>(); break;
+    case 32: k = batch_kernel<16, false>(); break;
+    case 33: k = batch_kernel<16, true>(); break;
+    case 64: k = batch_kernel<32, false>(); break;
+    case 65: k = batch_kernel<32, true>(); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k, SBS_BATCH_BLOCK, 0);
 }
 
 hipError_t launch_sbs_batch(const SbsBatchLaunch& L, hipStream_t stream) {
@@ -299,7 +314,16 @@ hipError_t launch_sbs_batch(const SbsBatchLaunch& L, hipStream_t stream) {
     hipLaunchKernelGGL(k_batch_mask_words, dim3(L.mask_blocks, L.n_items), dim3(256), 0, stream, L);
     hipLaunchKernelGGL(k_batch_mask_summary, dim3(L.summary_blocks, L.n_items), dim3(256), 0, stream, L);
   }
-  hipLaunchKernelGGL((k_sbs_batch<16, SBS_BATCH_BLOCK>), dim3(L.grid), dim3(SBS_BATCH_BLOCK), 0, stream, L);
+  const dim3 g(L.grid), b(SBS_BATCH_BLOCK);
+  switch (L.lds_stack * 2 + (L.record ? 1 : 0)) {
+    case 16: hipLaunchKernelGGL((k_sbs_batch<8, SBS_BATCH_BLOCK, false>), g, b, 0, stream, L); break;
+    case 17: hipLaunchKernelGGL((k_sbs_batch<8, SBS_BATCH_BLOCK, true>), g, b, 0, stream, L); break;
+    case 32: hipLaunchKernelGGL((k_sbs_batch<16, SBS_BATCH_BLOCK, false>), g, b, 0, stream, L); break;
+    case 33: hipLaunchKernelGGL((k_sbs_batch<16, SBS_BATCH_BLOCK, true>), g, b, 0, stream, L); break;
+    case 64: hipLaunchKernelGGL((k_sbs_batch<32, SBS_BATCH_BLOCK, false>), g, b, 0, stream, L); break;
+    case 65: hipLaunchKernelGGL((k_sbs_batch<32, SBS_BATCH_BLOCK, true>), g, b, 0, stream, L); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -107,14 +107,16 @@ struct SbsBatchLaunch {
   uint32_t zero_blocks, mask_blocks, summary_blocks, brick_blocks;   // x extents of the set-up grids
   uint32_t mask_flag_bytes;      // LDS of the one-wave-per-brick mask build (0: bricks too large)
   uint32_t* counter;             // work counter (reset by the zero kernel)
-  uint32_t* spill;               // grid * BLOCK * (STACK_SIZE - 16) words
-  uint32_t grid;                 // persistent blocks
+  uint32_t* spill;               // grid * BLOCK * (STACK_SIZE - 8) words
+  uint32_t grid;                 // persistent blocks (at most the resident blocks of the variant)
+  int lds_stack;                 // stack entries in LDS: 8, 16 or 32
+  int record;                    // 1: the walk records the hit voxel; 0: the hit brick is marched again
 };
 constexpr int SBS_BATCH_BLOCK = 64;
 constexpr int SBS_BATCH_CHUNK = 64;   // rays per work item of the persistent batch kernel (one wave)
 hipError_t launch_sbs_batch(const SbsBatchLaunch& L, hipStream_t stream);
-// Resident blocks of the persistent batch kernel per CU (occupancy query).
-hipError_t sbs_batch_blocks_per_cu(int* blocks);
+// Resident blocks per CU of the persistent batch kernel variant (occupancy query).
+hipError_t sbs_batch_blocks_per_cu(int lds_stack, int record, int* blocks);
 
 // Light (core/IRenderer.h:72-78), 32 B
 struct ResolveLight {

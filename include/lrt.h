@@ -279,6 +279,19 @@ typedef struct lrt_sbs_batch_item {
 /* n_items <= 65535. */
 lrt_status lrt_sbs_batch_create(const lrt_sbs_batch_item* items, uint32_t n_items, lrt_sbs_batch** out);
 lrt_status lrt_sbs_batch_run(lrt_sbs_batch* b, void* stream);
+
+/* Chooses the batch kernel's shape by timing runs of the plan on `stream` (which overwrite its
+ * outputs): stack entries in LDS (8/16/32), whether the walk records the hit voxel or the hit
+ * brick is marched again, and the persistent grid (1/2, 3/4 or all of the resident blocks). The
+ * fastest shape is kept in the plan. Gradients agree to float rounding across shapes (the atomic
+ * accumulation order differs). No counterpart in the reference. */
+typedef struct lrt_sbs_batch_tune_result {
+  int32_t lds_stack, record;
+  uint32_t grid;
+  float best_ms;
+  uint32_t configs_tried;
+} lrt_sbs_batch_tune_result;
+lrt_status lrt_sbs_batch_autotune(lrt_sbs_batch* b, void* stream, lrt_sbs_batch_tune_result* out);
 lrt_status lrt_sbs_batch_destroy(lrt_sbs_batch* b);
 
 /* Rays whose recomputed hit voxel did not reproduce the walk's hit (expected 0). */

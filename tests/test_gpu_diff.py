@@ -267,3 +267,51 @@ def test_batch_gradients_vs_float64(lrt, orc, torch_dev):
         r_a = np.linalg.norm(g32[nd:] - g64[nd:]) / max(np.linalg.norm(g64[nd:]), 1e-30)
         _say(f"grid {it['grid']}: rel err distances {r_d:.3g} albedo {r_a:.3g}")
         assert r_d <= 1e-3 and r_a <= 1e-3
+
+
+@pytest.mark.timeout(600)
+def test_batch_c5_full_64_scenes(lrt, orc, torch_dev):
+    """The whole batched workload (64 scenes, grids from {32,64,128,256}^3, 4 views of 128^2):
+    the autotuned persistent run against per-scene calls of the fused entry for every item
+    (colour equal, gradients and loss within float-rounding of the accumulation order), and the
+    float64 twin on 8 scenes spread over the batch (rel. err <= 1e-3)."""
+    import bench
+    import oracle.diff as od
+    torch = torch_dev
+    _say("batch C5: 64 scenes")
+    items = bench.sbs_batch_scenes(torch, lrt, n_scenes=64, res=128, views=4, seed=10, threads=16)
+    batch = lrt.SbsBatch(items)
+    t = batch.autotune()
+    _say(f"batch autotune: {t}")
+    assert t["configs_tried"] == 18 and t["lds_stack"] in (8, 16, 32) and t["grid"] >= 1
+    batch.run()
+    torch.cuda.synchronize()
+    for k, it in enumerate(items):
+        grad = torch.zeros_like(it["params"])
+        loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+        col = torch.zeros_like(it["colour"])
+        it["scene"].sbs_render_l2_backward(it["geom_id"], it["params"], it["pos"], it["dir"], it["target"], col, grad,
+                                           loss)
+        torch.cuda.synchronize()
+        assert torch.equal(col, it["colour"]), f"item {k} colour"
+        assert _rel(it["grad"], grad) < 1e-5, f"item {k} gradient"
+        assert abs(it["loss"].item() - loss.item()) <= 1e-5 * max(loss.item(), 1e-30), f"item {k} loss"
+        assert it["scene"].sbs_recompute_misses() == 0
+    pos, dirs = items[0]["pos"].cpu().numpy(), items[0]["dir"].cpu().numpy()
+    for k in range(0, 64, 9):
+        it = items[k]
+        vf = it["params"].cpu().numpy()
+        hits, recs = od.hit_voxels(orc.Scene.from_scene(it["scene"]), pos, dirs, threads=16)
+        lay = od.Layout(0, it["n_dist"])
+        p64 = vf.astype(np.float64)
+        m = recs["hit"] == 1
+        tl = recs["t_local"].astype(np.float64)
+        C = np.zeros((recs.size, 3))
+        C[m] = od.colour(recs[m], p64, lay, tl[m])
+        g64 = od.adjoint(recs, p64, lay, 2 * (C - it["target"].cpu().numpy()[:, :3]), tl)
+        g32 = it["grad"].cpu().numpy().astype(np.float64)
+        nd = it["n_dist"]
+        r_d = np.linalg.norm(g32[:nd] - g64[:nd]) / max(np.linalg.norm(g64[:nd]), 1e-30)
+        r_a = np.linalg.norm(g32[nd:] - g64[nd:]) / max(np.linalg.norm(g64[nd:]), 1e-30)
+        _say(f"item {k} grid {it['grid']}: rel err distances {r_d:.3g} albedo {r_a:.3g}")
+        assert r_d <= 1e-3 and r_a <= 1e-3
