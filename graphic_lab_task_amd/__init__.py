@@ -37,6 +37,7 @@ INSTANCE_DTYPE = np.dtype([("boxMin", "<f4", (4,)), ("boxMax", "<f4", (4,)), ("g
 FRAME_NODE_DTYPE = np.dtype([("values", "<f4", (8,)), ("offset", "<u4")])         # SdfFrameOctreeNode, 36 B
 MULTI_NODE_DTYPE = np.dtype([("data_offset", "<u4"), ("children_offset", "<u4"), ("voxel_count", "<u4"),
                              ("flags", "<u4")])                                       # SdfMultiOctreeNode, 16 B
+SVS_NODE_DTYPE = np.dtype([("pos_xy", "<u4"), ("pos_z_lod_size", "<u4"), ("values", "<u4", (2,))])  # SdfSVSNode, 16 B
 SBS_NODE_DTYPE = np.dtype([("pos_xy", "<u4"), ("pos_z_lod_size", "<u4"), ("data_offset", "<u4"),
                            ("_pad", "<u4")])                                       # SdfSBSNode, 16 B
 SBS_HEADER_DTYPE = np.dtype([("brick_size", "<u4"), ("brick_pad", "<u4"), ("bytes_per_value", "<u4"),
@@ -55,7 +56,8 @@ ARRAY_DTYPES = {
     "orig_nodes": NODE_DTYPE, "frame_nodes": FRAME_NODE_DTYPE, "frame_roots": np.uint32,
     "sbs_nodes": SBS_NODE_DTYPE, "sbs_data": np.uint32, "sbs_data_f": np.float32, "sbs_roots": np.uint32,
     "sbs_headers": SBS_HEADER_DTYPE, "multi_nodes": MULTI_NODE_DTYPE, "multi_values": np.float32,
-    "preset": PRESET_DTYPE, "sbs_mask": np.uint32, "orig_indices": np.uint32,
+    "preset": PRESET_DTYPE, "sbs_mask": np.uint32, "orig_indices": np.uint32, "svs_nodes": SVS_NODE_DTYPE,
+    "svs_roots": np.uint32,
 }
 LIGHT_DTYPE = np.dtype([("space", "<f4", (3,)), ("type", "<u4"), ("color", "<f4", (3,)), ("_pad", "<u4")])  # Light, 32 B
 LIGHT_TYPE_DIRECT, LIGHT_TYPE_POINT, LIGHT_TYPE_AMBIENT = 0, 1, 2
@@ -153,6 +155,9 @@ def lib() -> ctypes.CDLL:
             "lrt_destroy": ([vp], i32), "lrt_clear_geom": ([vp], i32), "lrt_clear_scene": ([vp], i32),
             "lrt_add_triangles": ([vp, vp, sz, sz, vp, vp, sz, u32, pu32], i32),
             "lrt_add_sdf_frame_octree": ([vp, vp, sz, pu32], i32),
+            "lrt_add_sdf_svs": ([vp, vp, sz, pu32], i32),
+            "lrt_load_sdf_svs": ([cp, ctypes.POINTER(_Buffer)], i32),
+            "lrt_save_sdf_svs": ([cp, vp, sz], i32),
             "lrt_add_sdf_sbs": ([vp, vp, vp, sz, vp, sz, vp, sz, pu32], i32),
             "lrt_add_instance": ([vp, u32, vp, pu32], i32),
             "lrt_update_instance": ([vp, u32, vp], i32),
@@ -212,7 +217,7 @@ def lib() -> ctypes.CDLL:
 
 def exported_symbols():
     return ["lrt_last_error", "lrt_version", "lrt_create", "lrt_destroy", "lrt_clear_geom", "lrt_clear_scene",
-            "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
+            "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_svs", "lrt_load_sdf_svs", "lrt_save_sdf_svs", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
             "lrt_trace_closest_host", "lrt_trace_closest_noalloc_dev", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
             "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_set_ray_interleave", "lrt_set_persistent",
@@ -311,6 +316,13 @@ class SceneRT:
         nodes = np.ascontiguousarray(nodes, dtype=FRAME_NODE_DTYPE)
         out = ctypes.c_uint32()
         _check(lib().lrt_add_sdf_frame_octree(self._h, _ptr(nodes), nodes.size, ctypes.byref(out)))
+        return out.value
+
+    def AddGeom_SdfSVS(self, nodes: np.ndarray) -> int:
+        """BVHRT::AddGeom_SdfSVS (BVH2Common_host.cpp:536-597): SdfSVSNode array."""
+        nodes = np.ascontiguousarray(nodes, dtype=SVS_NODE_DTYPE)
+        out = ctypes.c_uint32()
+        _check(lib().lrt_add_sdf_svs(self._h, _ptr(nodes), nodes.size, ctypes.byref(out)))
         return out.value
 
     def AddGeom_SdfMultiOctree(self, nodes: np.ndarray, values: np.ndarray) -> int:
@@ -614,6 +626,18 @@ def load_sdf_frame_octree(path: str) -> np.ndarray:
 def save_sdf_frame_octree(path: str, nodes: np.ndarray):
     n = np.ascontiguousarray(nodes, dtype=FRAME_NODE_DTYPE)
     _check(lib().lrt_save_sdf_frame_octree(os.fsencode(path), _ptr(n), n.size))
+
+
+def load_sdf_svs(path: str) -> np.ndarray:
+    """load_sdf_SVS (sdf/simple/sdf_simple.cpp:85-94): SdfSVSNode array."""
+    b = _Buffer()
+    _check(lib().lrt_load_sdf_svs(os.fsencode(path), ctypes.byref(b)))
+    return _take(b, SVS_NODE_DTYPE)
+
+
+def save_sdf_svs(path: str, nodes: np.ndarray):
+    n = np.ascontiguousarray(nodes, dtype=SVS_NODE_DTYPE)
+    _check(lib().lrt_save_sdf_svs(os.fsencode(path), _ptr(n), n.size))
 
 
 def load_sdf_sbs(path: str):

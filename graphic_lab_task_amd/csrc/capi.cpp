@@ -269,6 +269,15 @@ lrt_status lrt_add_sdf_frame_octree(lrt_scene* s, const void* nodes, size_t n_no
   });
 }
 
+lrt_status lrt_add_sdf_svs(lrt_scene* s, const void* nodes, size_t n_nodes, uint32_t* out_geom_id) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !nodes) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    const uint32_t id = s->s.add_sdf_svs(static_cast<const lrt::SdfSVSNode*>(nodes), n_nodes);
+    if (out_geom_id) *out_geom_id = id;
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_add_sdf_sbs(lrt_scene* s, const void* header, const void* nodes, size_t n_nodes, const uint32_t* values,
                            size_t n_values, const float* values_f, size_t n_values_f, uint32_t* out_geom_id) {
   return guarded([&]() -> lrt_status {
@@ -934,6 +943,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
     ARR("frame_roots", c.frameRoots) ARR("sbs_nodes", c.sbsNodes) ARR("sbs_data", c.sbsData)
     ARR("sbs_data_f", c.sbsDataF) ARR("sbs_roots", c.sbsRoots) ARR("sbs_headers", c.sbsHeaders)
     ARR("multi_nodes", c.multiNodes) ARR("multi_values", c.multiValues) ARR("orig_indices", c.origIndices)
+    ARR("svs_nodes", c.svsNodes) ARR("svs_roots", c.svsRoots)
 #undef ARR
     if (n == "preset") { src = &c.preset; sz = sizeof(c.preset); }
     std::vector<uint32_t> mask_copy;   // the committed occupancy masks live on the device only
@@ -953,7 +963,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
       static const char* known[] = {"nodes", "tlas", "instances", "geoms", "geom_tags", "start_end",
                                     "prim_id_count", "vert_pos", "vert_norm", "indices", "prim_indices",
                                     "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes", "sbs_data",
-                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices",
+                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices", "svs_nodes", "svs_roots",
                                     "sbs_mask"};
       bool ok = false;
       for (const char* k : known) ok = ok || n == k;

@@ -72,6 +72,13 @@ struct SdfFrameOctreeNode {
   float    values[8];
   uint32_t offset;     // children block offset, 0 = leaf
 };
+// SdfSVSNode (sdf/simple/sdf_simple_shared.h:30-35): one voxel of a sparse voxel set, 16 B
+struct SdfSVSNode {
+  uint32_t pos_xy;           // x << 16 | y, position of the voxel in its level
+  uint32_t pos_z_lod_size;   // z << 16 | level size (2^level)
+  uint32_t values[2];        // 8 distance values, 1 byte each
+};
+static_assert(sizeof(SdfSVSNode) == 16, "SdfSVSNode layout");
 
 // sdf/simple/sdf_simple_shared.h:39-45
 struct SdfSBSNode {
@@ -142,6 +149,7 @@ constexpr uint32_t ESCAPE_ROOT = 0xFFFFFFFEu;
 // ---- geometry types and tags (core/render_settings.h:3-42, BVH2Common.h:30-43) ----
 constexpr uint32_t TYPE_MESH_TRIANGLE    = 0;
 constexpr uint32_t TYPE_SDF_FRAME_OCTREE = 3;
+constexpr uint32_t TYPE_SDF_SVS          = 5;
 constexpr uint32_t TYPE_SDF_SBS          = 6;
 constexpr uint32_t TYPE_SDF_MULTI_OCTREE = 9;
 constexpr uint32_t MULTI_OCTREE_FLAG_SURFACE = 1;          // sdf_multi_shared.h:5

@@ -140,7 +140,7 @@ void Scene::clear_geom() {
   geomData.clear(); geomTags.clear(); startEnd.clear();
   vertPos.clear(); vertNorm.clear(); indices.clear(); primIndices.clear(); primIdCount.clear(); origIndices.clear();
   nodePairs.clear(); origNodes.clear();
-  frameNodes.clear(); frameRoots.clear();
+  frameNodes.clear(); frameRoots.clear(); svsNodes.clear(); svsRoots.clear();
   sbsNodes.clear(); sbsData.clear(); sbsDataF.clear(); sbsRoots.clear(); sbsHeaders.clear(); sbsFSlice.clear(); sbsDiffChecked.clear();
   multiNodes.clear(); multiValues.clear();
   clear_scene();
@@ -226,6 +226,33 @@ uint32_t Scene::add_sdf_frame_octree(const SdfFrameOctreeNode* nodes, size_t n) 
   frameRoots.push_back(n_offset);
   frameNodes.insert(frameNodes.end(), nodes, nodes + n);
   for (size_t i = n_offset; i < frameNodes.size(); i++) frameNodes[i].offset += n_offset;
+  origNodes.insert(origNodes.end(), boxes.begin(), boxes.end());
+  return add_aabb(TAG_SDF_NODE, boxes.data(), boxes.size());
+}
+
+// BVHRT::AddGeom_SdfSVS (BVH2Common_host.cpp:536-597): every voxel is a BLAS leaf whose box
+// follows from its position in its level. The boxes are also appended to the leaf-box array (the
+// reference keeps them only for the BLAS build), where the device leaf records take them from;
+// GeomData.offset.y holds the first of them, as for frame octrees.
+uint32_t Scene::add_sdf_svs(const SdfSVSNode* nodes, size_t n) {
+  if (n == 0 || n >= (1u << 28)) throw std::invalid_argument("sparse voxel set size out of range");
+  std::vector<BVHNode> boxes(n);
+  for (size_t i = 0; i < n; i++) {
+    const float px = float(nodes[i].pos_xy >> 16), py = float(nodes[i].pos_xy & 0x0000FFFF);
+    const float pz = float(nodes[i].pos_z_lod_size >> 16), sz = float(nodes[i].pos_z_lod_size & 0x0000FFFF);
+    if (sz == 0.0f) throw std::invalid_argument("sparse voxel set node with level size 0");
+    boxes[i].boxMin = make3(-1, -1, -1) + (2.0f * make3(px, py, pz)) / sz;
+    boxes[i].boxMax = boxes[i].boxMin + (2.0f * make3(1, 1, 1)) / sz;
+  }
+  GeomData g{};
+  g.boxMin = make4(-1, -1, -1, 1);
+  g.boxMax = make4(1, 1, 1, 1);
+  g.offset = uint2{uint32_t(svsRoots.size()), uint32_t(origNodes.size())};
+  g.bvhOffset = uint32_t(nodePairs.size());
+  g.type = TYPE_SDF_SVS;
+  geomData.push_back(g);
+  svsRoots.push_back(uint32_t(svsNodes.size()));
+  svsNodes.insert(svsNodes.end(), nodes, nodes + n);
   origNodes.insert(origNodes.end(), boxes.begin(), boxes.end());
   return add_aabb(TAG_SDF_NODE, boxes.data(), boxes.size());
 }
@@ -514,6 +541,24 @@ void Scene::build_device_layout() {
       fl.boxMax = on.boxMax;
       for (int j = 0; j < 8; ++j) fl.values[j] = frameNodes[on.leftOffset + root].values[j];
       fl.primId = on.leftOffset;
+    }
+  }
+  // sparse-voxel-set leaf records: the voxel's box and its 8 byte-quantised values decoded as
+  // BVHRT::RayNodeIntersection does (BVH2Common.cpp:190-215); primId = the voxel's index
+  for (size_t g = 0; g < geomData.size(); ++g) {
+    if (geomTags[g] != TAG_SDF_NODE || geomData[g].type != TYPE_SDF_SVS) continue;
+    const uint32_t root = svsRoots[geomData[g].offset.x], ob = geomData[g].offset.y;
+    const uint32_t nb = startEnd[g].y - startEnd[g].x;
+    for (uint32_t k = 0; k < nb; ++k) {
+      const SdfSVSNode& nd = svsNodes[root + k];
+      FrameLeaf& fl = frameLeaves[ob + k];
+      fl.boxMin = origNodes[ob + k].boxMin;
+      fl.boxMax = origNodes[ob + k].boxMax;
+      const float sz = float(nd.pos_z_lod_size & 0x0000FFFF);
+      const float d_max = 2 * 1.73205081f / sz;
+      for (int j = 0; j < 8; ++j)
+        fl.values[j] = -d_max + 2.0f * d_max * (1.0f / 255.0f) * float((nd.values[j / 4] >> (8 * (j % 4))) & 0xFF);
+      fl.primId = k;
     }
   }
   cacheGeom = 0xFFFFFFFFu;

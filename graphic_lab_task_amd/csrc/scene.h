@@ -49,6 +49,8 @@ struct Scene {
   std::vector<BVHNode> origNodes;
   std::vector<SdfFrameOctreeNode> frameNodes;
   std::vector<uint32_t> frameRoots;
+  std::vector<SdfSVSNode> svsNodes;
+  std::vector<uint32_t> svsRoots;
   std::vector<SdfSBSNode> sbsNodes;
   std::vector<uint32_t> sbsData;
   std::vector<float> sbsDataF;
@@ -125,6 +127,7 @@ struct Scene {
                          const uint32_t* idx, size_t n_idx);
   uint32_t add_aabb(uint32_t tag, const BVHNode* boxes, size_t n);
   uint32_t add_sdf_frame_octree(const SdfFrameOctreeNode* nodes, size_t n);
+  uint32_t add_sdf_svs(const SdfSVSNode* nodes, size_t n);
   uint32_t add_sdf_sbs(const SdfSBSHeader& header, const SdfSBSNode* nodes, size_t n_nodes, const uint32_t* values,
                        size_t n_values, const float* values_f, size_t n_values_f);
   uint32_t add_sdf_multi_octree(const SdfMultiOctreeNode* nodes, size_t n_nodes, const float* values,

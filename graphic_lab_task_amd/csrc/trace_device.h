@@ -405,8 +405,9 @@ __device__ __forceinline__ void frame_node_intersect(const SceneView& s, const G
                                                      float tNear, uint32_t instId, uint32_t geomId,
                                                      uint32_t bvhNodeId, CRT_Hit& hit) {
   const uint32_t type = g.type;
-  if (type != TYPE_SDF_FRAME_OCTREE) return;
-  const uint2 go = g.offset;   // y: first leaf box of this octree
+  // frame octrees and sparse voxel sets (BVH2Common.cpp:141-218): one leaf record per BLAS leaf
+  if (type != TYPE_SDF_FRAME_OCTREE && type != TYPE_SDF_SVS) return;
+  const uint2 go = g.offset;   // y: first leaf record of this geometry
   const float4* q = reinterpret_cast<const float4*>(s.frame_leaves + go.y + bvhNodeId);
   const float4 r0 = q[0], r1 = q[1], r2 = q[2], r3 = q[3];
   const float3 min_pos = make3(r0.x, r0.y, r0.z);

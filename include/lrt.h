@@ -74,6 +74,11 @@ lrt_status lrt_add_triangles(lrt_scene* s, const float* vpos, size_t n_verts, si
  * nodes: SdfFrameOctreeNode[n_nodes] (36 B each, sdf/simple/sdf_simple_shared.h:23-27). */
 lrt_status lrt_add_sdf_frame_octree(lrt_scene* s, const void* nodes, size_t n_nodes, uint32_t* out_geom_id);
 
+/* Replaces BVHRT::AddGeom_SdfSVS (BVH2Common_host.cpp:536-597): a sparse voxel set, every voxel
+ * a BLAS leaf. nodes: SdfSVSNode[n_nodes] (16 B each, sdf/simple/sdf_simple_shared.h:30-35:
+ * x << 16 | y, z << 16 | level size, 8 one-byte distance values). */
+lrt_status lrt_add_sdf_svs(lrt_scene* s, const void* nodes, size_t n_nodes, uint32_t* out_geom_id);
+
 /* Replaces BVHRT::AddGeom_SdfSBS (BVH2Common_host.cpp:601-695).
  * header: SdfSBSHeader, nodes: SdfSBSNode[n_nodes], values: uint32[n_values],
  * values_f: float[n_values_f] (indexed layouts only, may be NULL otherwise). */
@@ -333,6 +338,7 @@ lrt_status lrt_stack_overflow_count(lrt_scene* s, uint64_t* out, int reset);
  * "indices", "prim_indices", "orig_nodes" (BVHNode), "frame_nodes" (SdfFrameOctreeNode),
  * "frame_roots", "sbs_nodes", "sbs_data", "sbs_data_f", "sbs_roots", "sbs_headers",
  * "multi_nodes", "multi_values", "orig_indices" (uint32: triangle indices in the caller's order),
+ * "svs_nodes" (SdfSVSNode), "svs_roots",
  * "preset", and "sbs_mask" (uint32: the committed voxel occupancy
  * masks, per brick set in order and per brick one summary word then ceil(bs^3/32) bit words,
  * bit v set when the minimum of voxel v's 8 corner values is <= 0; copied from the device).

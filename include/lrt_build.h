@@ -77,6 +77,10 @@ lrt_status lrt_sdf_spheres_sbs_indexed(const float* spheres, size_t n_spheres, u
 lrt_status lrt_load_sdf_frame_octree(const char* path, lrt_buffer* nodes);
 lrt_status lrt_save_sdf_frame_octree(const char* path, const void* nodes, size_t n_nodes);
 
+/* load_sdf_SVS / save_sdf_SVS (sdf/simple/sdf_simple.cpp:69-94): u32 count, then count SdfSVSNode. */
+lrt_status lrt_load_sdf_svs(const char* path, lrt_buffer* nodes);
+lrt_status lrt_save_sdf_svs(const char* path, const void* nodes, size_t n_nodes);
+
 /* load_sdf_SBS / save_sdf_SBS (sdf/simple/sdf_simple.cpp:96-142): SdfSBSHeader, u32 count +
  * SdfSBSNode[count], u32 count + uint32 values, u32 count + float values_f. A file that ends
  * after the uint32 values (written before the fp32 block existed, e.g. saves/test_10_sbs.bin)

@@ -98,6 +98,7 @@ struct FrameNode { float values[8]; uint32_t offset; };
 struct SBSNode { uint32_t pos_xy, pos_z_lod_size, data_offset, pad; };
 struct SBSHeader { uint32_t brick_size, brick_pad, bytes_per_value, aux_data; };
 struct MultiNode { uint32_t data_offset, children_offset, voxel_count, flags; };   // sdf_multi_shared.h:8-14
+struct SVSNode { uint32_t pos_xy, pos_z_lod_size, values[2]; };                    // sdf_simple_shared.h:30-35
 struct Preset {
   uint32_t render_mode, sdf_node_intersect, normal_mode, ray_gen_mode, interpolation_mode, representation_mode, spp,
       fixed_lod;
@@ -110,6 +111,7 @@ static_assert(sizeof(Geom) == 64 && sizeof(Inst) == 256 && sizeof(FrameNode) == 
 constexpr uint32_t LEAF_BIT = 0x80000000u, START_MASK = 0x0FFFFFFFu, SIZE_MASK = 0x70000000u;
 constexpr int STACK_SIZE = 80;
 constexpr uint32_t TAG_NONE = 0, TAG_TRIANGLE = 1, TAG_SDF_BRICK = 4, TAG_SDF_NODE = 8, TAG_SDF_OCTREE = 15;
+constexpr uint32_t TYPE_SDF_SVS = 5;
 constexpr uint32_t TYPE_MESH_TRIANGLE = 0, TYPE_SDF_FRAME_OCTREE = 3, SH_TYPE = 24, GEOM_ID_MASK = 0x00FFFFFFu;
 constexpr uint32_t LAYOUT_DX_UV16 = 2u << 24, LAYOUT_DX_RGB8 = 3u << 24, LAYOUT_ID32F = 4u << 24, LAYOUT_ID32F_IN = 5u << 24;
 
@@ -140,6 +142,8 @@ struct OracleScene {
   const SBSHeader* sbs_headers;
   const MultiNode* multi_nodes;
   const float* multi_values;
+  const SVSNode* svs_nodes;
+  const uint32_t* svs_roots;
   Preset preset;
 };
 
@@ -411,16 +415,33 @@ struct Tracer {
   }
 
   // BVH2Common.cpp:727-749 + 141-167. Leaf boxes of geometry g start at geoms[g].offset[1].
+  // SVS: BVH2Common.cpp:190-215 (box from the voxel's position, byte values decoded)
   void OctreeNodeIntersect(uint32_t type, f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
                            uint32_t a_start, Hit* pHit) {
-    if (type != TYPE_SDF_FRAME_OCTREE) return;
+    if (type != TYPE_SDF_FRAME_OCTREE && type != TYPE_SDF_SVS) return;
     const uint32_t sdfId = s.geoms[geomId].offset[0];
-    const Node& on = s.orig_nodes[s.geoms[geomId].offset[1] + a_start];
-    const uint32_t primId = on.leftOffset;
-    const uint32_t nodeId = primId + s.frame_roots[sdfId];
-    const f3 min_pos = on.boxMin, max_pos = on.boxMax, size = max_pos - min_pos;
+    uint32_t primId = 0;
+    f3 min_pos, max_pos;
     float values[8];
-    for (int i = 0; i < 8; i++) values[i] = s.frame_nodes[nodeId].values[i];
+    if (type == TYPE_SDF_FRAME_OCTREE) {
+      const Node& on = s.orig_nodes[s.geoms[geomId].offset[1] + a_start];
+      primId = on.leftOffset;
+      const uint32_t nodeId = primId + s.frame_roots[sdfId];
+      min_pos = on.boxMin;
+      max_pos = on.boxMax;
+      for (int i = 0; i < 8; i++) values[i] = s.frame_nodes[nodeId].values[i];
+    } else {
+      primId = a_start;
+      const SVSNode& nd = s.svs_nodes[primId + s.svs_roots[sdfId]];
+      const float px = nd.pos_xy >> 16, py = nd.pos_xy & 0x0000FFFF;
+      const float pz = nd.pos_z_lod_size >> 16, sz = nd.pos_z_lod_size & 0x0000FFFF;
+      const float d_max = 2 * 1.73205081f / sz;
+      min_pos = f3(-1, -1, -1) + 2.0f * f3(px, py, pz) / sz;
+      max_pos = min_pos + 2.0f * f3(1, 1, 1) / sz;
+      for (int i = 0; i < 8; i++)
+        values[i] = -d_max + 2.0f * d_max * (1.0f / 255.0f) * ((nd.values[i / 4] >> (8 * (i % 4))) & 0xFF);
+    }
+    const f3 size = max_pos - min_pos;
     const f2 fNearFar = RayBoxIntersection2(ray_pos, SafeInverse(ray_dir), min_pos, max_pos);
     const f3 start_pos = ray_pos + fNearFar.x * ray_dir;
     const float d = std::max(size.x, std::max(size.y, size.z));

@@ -1,5 +1,6 @@
 """GPU parity on the reference's own SDF data files (tests/golden/, see test_sdf_io_cpu.py): the
-8,489-node engine octree as a frame octree and as a multi-octree, and the 49,487-brick DX set,
+8,489-node engine octree as a frame octree and as a multi-octree, the 49,487-brick DX set and the
+49,487-voxel sparse voxel set,
 each traced by the gfx950 kernels and by the oracle on the same arrays at the path's bar; plus the
 frame-octree vs multi-octree cross-check on the GPU's own records.
 """
@@ -52,6 +53,8 @@ def _scene(g, kind):
     elif kind == "multi":
         mn, mv = g.load_sdf_multi_octree(os.path.join(GOLD, "engine_sdf_multi_octree_0.bin"))
         s.AddInstance(s.AddGeom_SdfMultiOctree(mn, mv))
+    elif kind == "svs":
+        s.AddInstance(s.AddGeom_SdfSVS(g.load_sdf_svs(os.path.join(GOLD, "test_10_svs.bin"))))
     else:
         h, nodes, vals, vf = g.load_sdf_sbs(os.path.join(GOLD, "test_10_sbs.bin"))
         s.AddInstance(s.AddGeom_SdfSBS(h, nodes, vals, vf if vf.size else None))
@@ -60,7 +63,7 @@ def _scene(g, kind):
 
 
 @pytest.mark.parametrize("eye", [(0.4, 0.6, 3.0), (-2.5, 1.0, -1.5)])
-@pytest.mark.parametrize("kind", ["frame", "multi", "sbs"])
+@pytest.mark.parametrize("kind", ["frame", "multi", "sbs", "svs"])
 def test_reference_file_parity(lrt, orc, torch_dev, kind, eye):
     g = lrt
     _say(f"reference data {kind} eye={eye}")

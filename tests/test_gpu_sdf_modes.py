@@ -6,7 +6,7 @@ that calls acos/cos/pow) bit-exact in every field.
 
 Reference: LocalSurfaceIntersection (MultiRenderer/BVH2Common.cpp:345-725) for the five
 intersection modes, OctreeNodeIntersect (:727-749), OctreeBrickIntersect (:884-1112),
-OctreeIntersect (:2176-2363); render modes core/render_settings.h (ST_ITERATIONS 10,
+OctreeIntersect (:2176-2363), the sparse voxel set's leaf decode (:190-215); render modes core/render_settings.h (ST_ITERATIONS 10,
 MO_VOXEL_COUNT 24, MO_FLAGS 25, INTERSECTION_COUNT 26).
 """
 import sys
@@ -14,7 +14,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import MISS, assert_sdf_parity, sdf_check
+from conftest import MISS, assert_sdf_parity, sdf_check, svs_spheres
 
 pytestmark = pytest.mark.gpu
 
@@ -65,6 +65,10 @@ def scenes(lrt, torch_dev):
     s.AddInstance(s.AddGeom_SdfMultiOctree(mn, mv))
     s.CommitScene()
     out["multi"] = s
+    s = g.SceneRT()
+    s.AddInstance(s.AddGeom_SdfSVS(svs_spheres(sp, 48)))
+    s.CommitScene()
+    out["svs"] = s
     return out
 
 
@@ -97,7 +101,7 @@ def _run(lrt, orc, torch, s, pos, dirs, preset):
 
 @pytest.mark.parametrize("representation", [0, 1])
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
-@pytest.mark.parametrize("kind", ["frame", "sbs_indexed", "sbs_dx1", "multi"])
+@pytest.mark.parametrize("kind", ["frame", "sbs_indexed", "sbs_dx1", "multi", "svs"])
 def test_sdf_intersection_modes(lrt, orc, torch_dev, scenes, rays, kind, mode, representation):
     _say(f"sdf modes kind={kind} mode={MODES[mode]} representation={representation}")
     pos, dirs = rays
@@ -117,6 +121,7 @@ def test_sdf_intersection_modes(lrt, orc, torch_dev, scenes, rays, kind, mode, r
 
 @pytest.mark.parametrize("kind,render_mode,mode", [
     ("frame", 10, 0), ("sbs_indexed", 10, 0), ("sbs_dx1", 10, 0), ("multi", 10, 0),
+    ("svs", 10, 0), ("svs", 26, 1),
     ("frame", 26, 1), ("sbs_dx1", 26, 1), ("multi", 26, 1), ("frame", 26, 0),
     ("multi", 24, 1), ("multi", 25, 1), ("multi", 24, 0), ("multi", 25, 3)])
 def test_sdf_debug_render_modes(lrt, orc, torch_dev, scenes, rays, kind, render_mode, mode):

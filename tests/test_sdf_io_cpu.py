@@ -3,9 +3,12 @@ the reference itself ships (copied under tests/golden/ as fixtures):
   engine_sdf_frame_octree_0.bin  scenes/06_engine_scenes/framed_octree/sdf_frame_octree_0.bin
   engine_sdf_multi_octree_0.bin  scenes/06_engine_scenes/multi_octree/sdf_multi_octree_0.bin
   test_10_sbs.bin                saves/test_10_sbs.bin
+  test_10_svs.bin                saves/test_10_svs.bin
 The two engine files hold one 8,489-node octree (scene.xml beside each names num_primitives=8489)
 as a frame octree and as a multi-octree, so per ray their hits must agree: a cross-check of the
-oracle's two walks on real data. Reference: sdf/simple/sdf_simple.cpp:49-142,
+oracle's two walks on real data. The two test_10 files hold one adaptive voxel set as a DX brick
+set (brick 1) and as a sparse voxel set, quantised apart by at most one byte step per value.
+Reference: sdf/simple/sdf_simple.cpp:49-142,
 sdf/multi/sdf_multi.cpp:8-35.
 """
 import os
@@ -19,6 +22,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FRAME = os.path.join(GOLD, "engine_sdf_frame_octree_0.bin")
 MULTI = os.path.join(GOLD, "engine_sdf_multi_octree_0.bin")
 SBS = os.path.join(GOLD, "test_10_sbs.bin")
+SVS = os.path.join(GOLD, "test_10_svs.bin")
 
 
 def test_fixture_counts(lrt):
@@ -44,6 +48,16 @@ def test_fixture_counts(lrt):
     assert set(np.unique(lod).tolist()) == {16, 32, 64, 128}
     px, py, pz = nodes["pos_xy"] >> 16, nodes["pos_xy"] & 0xFFFF, nodes["pos_z_lod_size"] >> 16
     assert np.all(px < lod) and np.all(py < lod) and np.all(pz < lod)
+    sv = g.load_sdf_svs(SVS)
+    assert sv.size == 49487
+    # the same voxels as the brick set (other order), values within one quantisation step
+    k_svs = (sv["pos_xy"].astype(np.uint64) << np.uint64(32)) | sv["pos_z_lod_size"]
+    k_sbs = (nodes["pos_xy"].astype(np.uint64) << np.uint64(32)) | nodes["pos_z_lod_size"]
+    i1, i2 = np.argsort(k_svs), np.argsort(k_sbs)
+    assert np.array_equal(k_svs[i1], k_sbs[i2])
+    b1 = sv["values"][i1].view(np.uint8).reshape(-1, 8).astype(int)
+    b2 = vals.reshape(-1, 2)[i2].view(np.uint8).reshape(-1, 8).astype(int)
+    assert np.abs(b1 - b2).max() <= 1 and (b1 != b2).any()
 
 
 def test_round_trip_is_byte_identical(lrt, tmp_path):
@@ -60,6 +74,9 @@ def test_round_trip_is_byte_identical(lrt, tmp_path):
     assert (tmp_path / "s.bin").read_bytes() == open(SBS, "rb").read() + b"\0\0\0\0"
     h2, n2, v2, f2 = g.load_sdf_sbs(str(tmp_path / "s.bin"))
     assert h2.tobytes() == h.tobytes() and np.array_equal(n2, nodes) and np.array_equal(v2, vals) and f2.size == 0
+    sv = g.load_sdf_svs(SVS)
+    g.save_sdf_svs(str(tmp_path / "v.bin"), sv)
+    assert (tmp_path / "v.bin").read_bytes() == open(SVS, "rb").read()
 
 
 def test_truncated_and_missing_files_are_errors(lrt, tmp_path):
@@ -77,6 +94,9 @@ def test_truncated_and_missing_files_are_errors(lrt, tmp_path):
     (tmp_path / "cut3.bin").write_bytes(raw[:30])
     with pytest.raises(g.LrtError):
         g.load_sdf_sbs(str(tmp_path / "cut3.bin"))
+    (tmp_path / "cut4.bin").write_bytes(open(SVS, "rb").read()[:-16])
+    with pytest.raises(g.LrtError):
+        g.load_sdf_svs(str(tmp_path / "cut4.bin"))
     with pytest.raises(g.LrtError) as e:
         g.load_sdf_frame_octree(str(tmp_path / "none.bin"))
     assert "cannot open" in str(e.value)
@@ -106,28 +126,22 @@ def test_engine_frame_and_multi_octree_agree(lrt, orc):
     assert r["hits"] > 0.1 * pos.shape[0]
 
 
-def test_sbs_fixture_traces(lrt, orc):
-    """The 49,487-brick DX set (brick 1, one byte per value): the oracle's march finds hits, and
-    every hit point lies on the surface its brick's 8 decoded values describe: their trilinear,
-    evaluated in float64 at the hit point, is zero to within 1e-4 of the brick's distance range
-    (d_max = sqrt(3) * 2 / lod_size, sdf_converter.cpp:182-187; BVH2Common.cpp:846-882), or, for
-    a hit on the brick's entry face, not above it."""
-    g = lrt
-    h, nodes, vals, vf = g.load_sdf_sbs(SBS)
-    s = host_scene(g, lambda s: s.AddInstance(s.AddGeom_SdfSBS(h, nodes, vals)))
-    pos, dirs = _rays(orc, g, 96)
-    hits, _ = orc.trace_closest(orc.Scene.from_scene(s), pos, dirs)
+def _on_voxel_surface(hits, pos, dirs, cells, words):
+    """Every hit lies on the surface its voxel's 8 decoded byte values describe: their trilinear,
+    evaluated in float64 at the hit point, is zero to within 1e-4 of the voxel's distance range
+    (d_max = sqrt(3) * 2 / lod_size, sdf_converter.cpp:182-187; BVH2Common.cpp:846-882 and
+    :190-215), or, for a hit on the voxel's entry face, not above it. cells: (pos_xy,
+    pos_z_lod_size) of each hit's voxel; words: its two value words."""
     hit = hits["primId"] != 0xFFFFFFFF
-    assert hit.sum() > 300
     p = pos[hit, :3].astype(np.float64) + hits["t"][hit, None].astype(np.float64) * dirs[hit, :3]
-    nd = nodes[hits["primId"][hit]]
-    lod = (nd["pos_z_lod_size"] & 0xFFFF).astype(np.float64)
-    cell = np.stack([nd["pos_xy"] >> 16, nd["pos_xy"] & 0xFFFF, nd["pos_z_lod_size"] >> 16], 1).astype(np.float64)
+    pxy, pzl = cells
+    lod = (pzl & 0xFFFF).astype(np.float64)
+    cell = np.stack([pxy >> 16, pxy & 0xFFFF, pzl >> 16], 1).astype(np.float64)
     size = 2.0 / lod
     q = (p - (-1.0 + size[:, None] * cell)) / size[:, None]
-    assert np.all(q > -1e-3) and np.all(q < 1 + 1e-3), "hit point outside its brick"
+    assert np.all(q > -1e-3) and np.all(q < 1 + 1e-3), "hit point outside its voxel"
     q = np.clip(q, 0.0, 1.0)
-    words = vals[nd["data_offset"][:, None] + np.arange(2)[None, :]].astype(np.uint64)
+    words = words.astype(np.uint64)
     raw = np.stack([(words[:, i // 4] >> np.uint64(8 * (i % 4))) & np.uint64(255) for i in range(8)], 1)
     d_max = np.sqrt(3.0) * size
     v = -d_max[:, None] + (2 * d_max / 255.0)[:, None] * raw.astype(np.float64)
@@ -138,9 +152,63 @@ def test_sbs_fixture_traces(lrt, orc):
         wz = q[:, 2] if i & 1 else 1 - q[:, 2]
         f += wx * wy * wz * v[:, i]
     rel = f / d_max
-    # volume representation (the default preset): a ray that enters a brick already inside the
-    # surface hits at the brick face, where the value is <= 0; elsewhere the hit is a root
+    # volume representation (the default preset): a ray that enters a voxel already inside the
+    # surface hits at the voxel face, where the value is <= 0; elsewhere the hit is a root
     face = np.any((q < 1e-4) | (q > 1 - 1e-4), axis=1)
     assert np.all(rel[face] <= 1e-4)
     assert (~face).sum() > 100
     assert np.abs(rel[~face]).max() < 1e-4, np.abs(rel[~face]).max()
+
+
+def test_sbs_fixture_traces(lrt, orc):
+    """The 49,487-brick DX set (brick 1, one byte per value): the oracle's march finds hits on the
+    surface of the brick they name."""
+    g = lrt
+    h, nodes, vals, vf = g.load_sdf_sbs(SBS)
+    s = host_scene(g, lambda s: s.AddInstance(s.AddGeom_SdfSBS(h, nodes, vals)))
+    pos, dirs = _rays(orc, g, 96)
+    hits, _ = orc.trace_closest(orc.Scene.from_scene(s), pos, dirs)
+    hit = hits["primId"] != 0xFFFFFFFF
+    assert hit.sum() > 300
+    nd = nodes[hits["primId"][hit]]
+    words = vals[nd["data_offset"][:, None] + np.arange(2)[None, :]]
+    _on_voxel_surface(hits, pos, dirs, (nd["pos_xy"], nd["pos_z_lod_size"]), words)
+
+
+def test_svs_fixture_traces(lrt, orc):
+    """The 49,487-voxel sparse voxel set (BVHRT::AddGeom_SdfSVS, BVH2Common_host.cpp:536-597):
+    one BLAS leaf per voxel, primId = the voxel's index; hits on the surface of the voxel they name."""
+    g = lrt
+    sv = g.load_sdf_svs(SVS)
+    s = host_scene(g, lambda s: s.AddInstance(s.AddGeom_SdfSVS(sv)))
+    pos, dirs = _rays(orc, g, 96)
+    hits, _ = orc.trace_closest(orc.Scene.from_scene(s), pos, dirs)
+    hit = hits["primId"] != 0xFFFFFFFF
+    assert hit.sum() > 300
+    nd = sv[hits["primId"][hit]]
+    _on_voxel_surface(hits, pos, dirs, (nd["pos_xy"], nd["pos_z_lod_size"]), nd["values"])
+
+
+def test_svs_and_sbs_fixtures_agree(lrt, orc):
+    """The oracle's per-voxel leaf (sparse voxel set) and its brick march (brick set of 1-voxel
+    bricks) over the test_10 pair: the two files quantise the same distances apart by one byte
+    step, so surfaces move by up to 2 d_max / 255 and silhouettes may shift: hit/miss agree on
+    all but 0.05% of rays, the same voxel is hit on at least 97% of common hits, and 99% of
+    their t agree within a quarter of the finest voxel."""
+    g = lrt
+    sv = g.load_sdf_svs(SVS)
+    h, nodes, vals, vf = g.load_sdf_sbs(SBS)
+    s1 = host_scene(g, lambda s: s.AddInstance(s.AddGeom_SdfSVS(sv)))
+    s2 = host_scene(g, lambda s: s.AddInstance(s.AddGeom_SdfSBS(h, nodes, vals)))
+    pos, dirs = _rays(orc, g, 256)
+    a, _ = orc.trace_closest(orc.Scene.from_scene(s1), pos, dirs)
+    b, _ = orc.trace_closest(orc.Scene.from_scene(s2), pos, dirs)
+    ha, hb = a["primId"] != 0xFFFFFFFF, b["primId"] != 0xFFFFFFFF
+    assert ha.sum() > 2000
+    assert (ha != hb).sum() <= 5e-4 * pos.shape[0]
+    both = ha & hb
+    k_svs = (sv["pos_xy"].astype(np.uint64) << np.uint64(32)) | sv["pos_z_lod_size"]
+    k_sbs = (nodes["pos_xy"].astype(np.uint64) << np.uint64(32)) | nodes["pos_z_lod_size"]
+    assert (k_svs[a["primId"][both]] == k_sbs[b["primId"][both]]).mean() >= 0.97
+    dt = np.abs(a["t"][both].astype(np.float64) - b["t"][both])
+    assert np.quantile(dt, 0.99) <= 0.25 * 2.0 / 128
