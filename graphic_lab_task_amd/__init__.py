@@ -57,7 +57,7 @@ ARRAY_DTYPES = {
     "sbs_nodes": SBS_NODE_DTYPE, "sbs_data": np.uint32, "sbs_data_f": np.float32, "sbs_roots": np.uint32,
     "sbs_headers": SBS_HEADER_DTYPE, "multi_nodes": MULTI_NODE_DTYPE, "multi_values": np.float32,
     "preset": PRESET_DTYPE, "sbs_mask": np.uint32, "orig_indices": np.uint32, "svs_nodes": SVS_NODE_DTYPE,
-    "svs_roots": np.uint32,
+    "svs_roots": np.uint32, "tlas_parent": np.uint32,
 }
 LIGHT_DTYPE = np.dtype([("space", "<f4", (3,)), ("type", "<u4"), ("color", "<f4", (3,)), ("_pad", "<u4")])  # Light, 32 B
 LIGHT_TYPE_DIRECT, LIGHT_TYPE_POINT, LIGHT_TYPE_AMBIENT = 0, 1, 2
@@ -109,6 +109,16 @@ class _TuneResult(ctypes.Structure):
 class _BatchTuneResult(ctypes.Structure):
     _fields_ = [("lds_stack", ctypes.c_int32), ("record", ctypes.c_int32), ("grid", ctypes.c_uint32),
                 ("best_ms", ctypes.c_float), ("configs_tried", ctypes.c_uint32)]
+
+
+class _CommitInfo(ctypes.Structure):
+    """lrt_commit_info (include/lrt.h)."""
+    _fields_ = [("kind", ctypes.c_uint32), ("gpu_builder", ctypes.c_uint32), ("blas_max_depth", ctypes.c_uint32),
+                ("blas_leaves", ctypes.c_uint32), ("blas_merged_leaves", ctypes.c_uint32),
+                ("tlas_nodes", ctypes.c_uint32), ("tlas_area_ratio", ctypes.c_float)]
+
+
+COMMIT_FULL, COMMIT_REFIT, COMMIT_TLAS_REBUILD = 0, 1, 2
 
 
 class _Buffer(ctypes.Structure):
@@ -174,6 +184,8 @@ def lib() -> ctypes.CDLL:
             "lrt_set_leaf_postponement": ([vp, i32], i32),
             "lrt_set_ray_interleave": ([vp, u32], i32),
             "lrt_set_persistent": ([vp, i32], i32),
+            "lrt_set_tlas_update": ([vp, i32], i32),
+            "lrt_get_commit_info": ([vp, ctypes.POINTER(_CommitInfo)], i32),
             "lrt_autotune": ([vp, vp, vp, vp, u64, vp, vp], i32),
             "lrt_sbs_param_layout": ([vp, u32, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u64)], i32),
             "lrt_sbs_forward_dev": ([vp, u32, vp, vp, vp, u64, vp, vp], i32),
@@ -221,7 +233,7 @@ def exported_symbols():
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
             "lrt_trace_closest_host", "lrt_trace_closest_noalloc_dev", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
             "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_set_ray_interleave", "lrt_set_persistent",
-            "lrt_autotune", "lrt_sbs_param_layout",
+            "lrt_set_tlas_update", "lrt_get_commit_info", "lrt_autotune", "lrt_sbs_param_layout",
             "lrt_sbs_forward_dev", "lrt_sbs_backward_dev", "lrt_sbs_l2_backward_dev", "lrt_sbs_render_l2_backward_dev", "lrt_sbs_recompute_misses",
             "lrt_sbs_batch_create", "lrt_sbs_batch_run", "lrt_sbs_batch_autotune", "lrt_sbs_batch_destroy",
             "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
@@ -477,6 +489,17 @@ class SceneRT:
     def set_persistent(self, on: bool = True):
         """One persistent single-pass kernel with per-lane ray refill (True) or the two-pass grid launch."""
         _check(lib().lrt_set_persistent(self._h, 1 if on else 0))
+
+    def set_tlas_update(self, mode: int = 1):
+        """Commits after UpdateInstance only: 1 refit the TLAS on the GPU, 0 rebuild it on the host."""
+        _check(lib().lrt_set_tlas_update(self._h, mode))
+
+    def commit_info(self) -> dict:
+        """What the last commit did (COMMIT_FULL / COMMIT_REFIT / COMMIT_TLAS_REBUILD) and the last
+        BLAS build's depth and leaf counts."""
+        ci = _CommitInfo()
+        _check(lib().lrt_get_commit_info(self._h, ctypes.byref(ci)))
+        return {k: getattr(ci, k) for k, _ in _CommitInfo._fields_}
 
     def set_leaf_postponement(self, mode: int = -1):
         """Triangle leaf postponement of the one-ray-per-lane pass: -1 automatic (on for one

@@ -13,6 +13,7 @@
 //   * triangle leaves hold PackOffsetAndSize(leafId,1) and a per-leaf start/count word
 //     (cbvh.cpp:502-511); box leaves hold PackOffsetAndSize(boxId,1) (cbvh.cpp:645-655).
 #include "bvh_build.h"
+#include "bvh_rules.h"
 #include "lrt_math.h"
 
 #include <algorithm>
@@ -39,36 +40,8 @@ inline void grow(Box& b, float3 p) {
   b.mx = max3(b.mx, p);
 }
 inline float comp(float3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
-inline void set_comp(float3& v, int a, float x) {
-  if (a == 0) v.x = x; else if (a == 1) v.y = x; else v.z = x;
-}
-// Box4f::surfaceArea (LiteMath.h:3440-3444)
-inline float surface_area(const Box& b) {
-  const float3 d = b.mx - b.mn;
-  return 2.0f * (d.x * d.y + d.x * d.z + d.y * d.z);
-}
-
-// BoxBoxOverlap (LiteMath.h:3535-3562): zero-size box when disjoint.
-Box box_overlap(const Box& b1, const Box& b2) {
-  Box t1 = b1, t2 = b2, r;
-  for (int a = 0; a < 3; ++a) {
-    if (comp(t2.mn, a) < comp(t1.mn, a)) {
-      float s = comp(t1.mn, a); set_comp(t1.mn, a, comp(t2.mn, a)); set_comp(t2.mn, a, s);
-      s = comp(t1.mx, a); set_comp(t1.mx, a, comp(t2.mx, a)); set_comp(t2.mx, a, s);
-    }
-    if (comp(t1.mx, a) < comp(t2.mn, a)) return Box{b1.mx, b1.mx};
-    set_comp(r.mn, a, comp(t2.mn, a));
-    set_comp(r.mx, a, fmin_ref(comp(t1.mx, a), comp(t2.mx, a)));
-  }
-  return r;
-}
-
-// RelativeBoxOverlap (cbvh_embree2.cpp:39-48)
-float relative_box_overlap(const Box& b1, const Box& b2) {
-  const Box o = box_overlap(b1, b2);
-  const float sa1 = surface_area(b1), sa2 = surface_area(b2), sa3 = surface_area(o);
-  return fmin_ref(sa3 / fmin_ref(sa1, sa2), 1.0f);
-}
+inline float surface_area(const Box& b) { return box_surface_area(b.mn, b.mx); }
+float relative_box_overlap(const Box& b1, const Box& b2) { return lrt::relative_box_overlap(b1.mn, b1.mx, b2.mn, b2.mx); }
 
 struct SahNode {
   Box box[2];
@@ -288,7 +261,7 @@ FatBlas build_blas_triangles(const float4* verts, size_t n_verts, const uint32_t
     boxes[i].mx = max3(a, max3(b, c));
   }
   // BuildBVHEmbree (cbvh_embree2.cpp:344-347): over-split threshold 0.5 below 4 prims per leaf.
-  const float over_split = prims_in_leaf >= 4 ? 0.25f : 0.5f;
+  const float over_split = over_split_threshold(prims_in_leaf);
   Emitter em = build_aligned(boxes, uint32_t(prims_in_leaf), over_split, stats);
   uint32_t leaf_id = 0;
   for (size_t i = 0; i < em.nodes.size(); ++i) {

@@ -32,4 +32,11 @@ FatBlas build_blas_boxes(const BVHNode* boxes, size_t n, BuildStats* stats = nul
 // stackless BVH2_LEFT_OFFSET nodes with escape indices; a single instance gives one leaf node.
 std::vector<BVHNode> build_tlas(const float4* box_min, const float4* box_max, size_t n);
 
+// The same two builds on the current HIP device (bvh_kernels.hip): a linear BVH over 63-bit
+// Morton codes of the primitive centroids with the leaf-merge rule and layout contract above.
+// Small inputs (one leaf) take the host path. Throws std::runtime_error on a HIP failure.
+FatBlas gpu_build_blas_triangles(const float4* verts, size_t n_verts, const uint32_t* tri_idx, size_t n_idx,
+                                 int prims_in_leaf, BuildStats* stats = nullptr);
+FatBlas gpu_build_blas_boxes(const BVHNode* boxes, size_t n, BuildStats* stats = nullptr);
+
 }  // namespace lrt

@@ -227,6 +227,7 @@ lrt_status lrt_create(const char* impl_name, const char* build_name, const char*
     if (!b.empty() && b.back() >= '0' && b.back() <= '9') sc->s.prims_in_leaf = b.back() - '0';
     if (sc->s.prims_in_leaf < 1) sc->s.prims_in_leaf = 1;
     if (sc->s.prims_in_leaf > 7) sc->s.prims_in_leaf = 7;   // 3-bit leaf count field
+    sc->s.gpu_builder = b.compare(0, 3, "gpu") == 0;
     sc->s.set_preset(lrt::default_preset());
     *out = sc;
     return LRT_OK;
@@ -330,11 +331,37 @@ lrt_status lrt_commit(lrt_scene* s, uint32_t build_flags) {
   (void)build_flags;
   return guarded([&]() -> lrt_status {
     if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
-    s->s.build_host();
     lrt_status st = check_device();
-    if (st != LRT_OK) return fail(st, "host arrays built; no HIP device to upload them to");
+    if (st != LRT_OK) {
+      s->s.build_host();
+      return fail(st, "host arrays built; no HIP device to upload them to");
+    }
     hipError_t e = s->s.commit();
     if (e != hipSuccess) return hip_fail(e, "commit");
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_set_tlas_update(lrt_scene* s, int mode) {
+  return guarded([&]() -> lrt_status {
+    if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
+    if (mode != 0 && mode != 1) return fail(LRT_ERR_INVALID_ARG, "mode must be 0 (rebuild) or 1 (refit)");
+    s->s.tlas_update = mode;
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_get_commit_info(lrt_scene* s, lrt_commit_info* out) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !out) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    const lrt::Scene& sc = s->s;
+    out->kind = sc.lastCommitKind;
+    out->gpu_builder = sc.gpu_builder ? 1u : 0u;
+    out->blas_max_depth = sc.lastBuild.max_depth;
+    out->blas_leaves = sc.lastBuild.leaves;
+    out->blas_merged_leaves = sc.lastBuild.merged_leaves;
+    out->tlas_nodes = uint32_t(sc.tlas.size());
+    out->tlas_area_ratio = sc.tlasAreaBuilt > 0.0 ? float(sc.tlasAreaNow / sc.tlasAreaBuilt) : 1.0f;
     return LRT_OK;
   });
 }
@@ -943,7 +970,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
     ARR("frame_roots", c.frameRoots) ARR("sbs_nodes", c.sbsNodes) ARR("sbs_data", c.sbsData)
     ARR("sbs_data_f", c.sbsDataF) ARR("sbs_roots", c.sbsRoots) ARR("sbs_headers", c.sbsHeaders)
     ARR("multi_nodes", c.multiNodes) ARR("multi_values", c.multiValues) ARR("orig_indices", c.origIndices)
-    ARR("svs_nodes", c.svsNodes) ARR("svs_roots", c.svsRoots)
+    ARR("svs_nodes", c.svsNodes) ARR("svs_roots", c.svsRoots) ARR("tlas_parent", c.tlasParent)
 #undef ARR
     if (n == "preset") { src = &c.preset; sz = sizeof(c.preset); }
     std::vector<uint32_t> mask_copy;   // the committed occupancy masks live on the device only
@@ -963,7 +990,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
       static const char* known[] = {"nodes", "tlas", "instances", "geoms", "geom_tags", "start_end",
                                     "prim_id_count", "vert_pos", "vert_norm", "indices", "prim_indices",
                                     "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes", "sbs_data",
-                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices", "svs_nodes", "svs_roots",
+                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices", "svs_nodes", "svs_roots", "tlas_parent",
                                     "sbs_mask"};
       bool ok = false;
       for (const char* k : known) ok = ok || n == k;

@@ -148,6 +148,12 @@ hipError_t launch_resolve(const ResolveParams& p, const CRT_Hit* gbuf, float4* o
 // Tonemap: rgba8[i] = pack(255 * clamp(colour[i] / samples, 0, 1))
 hipError_t launch_tonemap(const float4* colour, uint32_t n, uint32_t samples, uint32_t* out, hipStream_t stream);
 
+// Refit of a TLAS in place: leaf boxes from the instances' boxes, every parent the union of its
+// two children; parent[i] = node index of node i's parent (0xFFFFFFFF for the root's children);
+// counters: n_nodes words, cleared by the launch.
+hipError_t launch_tlas_refit(BVHNode* tlas, uint32_t n_nodes, const uint32_t* parent, const InstanceData* inst,
+                             uint32_t* counters, hipStream_t stream);
+
 hipError_t launch_trace(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 // launch_trace with the library's fixed per-device work array as queue and spill (a.queue and
 // a.spill are ignored): at most LRT_NOALLOC_MAX_RAYS rays, no counters; `device` is current.

@@ -133,6 +133,8 @@ void add_border_nodes_rec(const SdfFrameOctreeNode* nodes, size_t n_nodes, std::
 void Scene::clear_scene() {
   instances.clear();
   tlas.clear();
+  tlasParent.clear();
+  inst_set_dirty = true;
   invalidate();
 }
 
@@ -143,6 +145,7 @@ void Scene::clear_geom() {
   frameNodes.clear(); frameRoots.clear(); svsNodes.clear(); svsRoots.clear();
   sbsNodes.clear(); sbsData.clear(); sbsDataF.clear(); sbsRoots.clear(); sbsHeaders.clear(); sbsFSlice.clear(); sbsDiffChecked.clear();
   multiNodes.clear(); multiValues.clear();
+  geom_dirty = true;
   clear_scene();
 }
 
@@ -175,7 +178,8 @@ uint32_t Scene::add_triangles(const float* vpos, size_t n_verts, size_t v_stride
   geomData.push_back(g);
   geomTags.push_back(TAG_TRIANGLE);
 
-  FatBlas bvh = build_blas_triangles(vertPos.data() + oldSizeVert, n_verts, idx, n_idx, prims_in_leaf);
+  FatBlas bvh = gpu_builder ? gpu_build_blas_triangles(vertPos.data() + oldSizeVert, n_verts, idx, n_idx, prims_in_leaf, &lastBuild)
+                            : build_blas_triangles(vertPos.data() + oldSizeVert, n_verts, idx, n_idx, prims_in_leaf, &lastBuild);
   nodePairs.insert(nodePairs.end(), bvh.nodes.begin(), bvh.nodes.end());
   primIndices.insert(primIndices.end(), bvh.indices.begin(), bvh.indices.end());
   indices.resize(oldSizeInd + bvh.indices.size() * 3);
@@ -189,19 +193,21 @@ uint32_t Scene::add_triangles(const float* vpos, size_t n_verts, size_t v_stride
   const size_t oldSize = primIdCount.size();
   primIdCount.insert(primIdCount.end(), bvh.startCount.begin(), bvh.startCount.end());
   startEnd.push_back(uint2{uint32_t(oldSize), uint32_t(primIdCount.size())});
+  geom_dirty = true;
   invalidate();
   return uint32_t(startEnd.size() - 1);
 }
 
 // BVHRT::AddGeom_AABB (BVH2Common_host.cpp:441-459)
 uint32_t Scene::add_aabb(uint32_t tag, const BVHNode* boxes, size_t n) {
-  FatBlas bvh = build_blas_boxes(boxes, n);
+  FatBlas bvh = gpu_builder ? gpu_build_blas_boxes(boxes, n, &lastBuild) : build_blas_boxes(boxes, n, &lastBuild);
   nodePairs.insert(nodePairs.end(), bvh.nodes.begin(), bvh.nodes.end());
   const size_t oldSize = primIdCount.size();
   primIdCount.resize(oldSize + n);
   for (size_t i = 0; i < n; i++) primIdCount[oldSize + i] = uint32_t(i);
   startEnd.push_back(uint2{uint32_t(oldSize), uint32_t(primIdCount.size())});
   geomTags.push_back(tag);
+  geom_dirty = true;
   invalidate();
   return uint32_t(startEnd.size() - 1);
 }
@@ -401,6 +407,7 @@ uint32_t Scene::add_instance(uint32_t geom_id, const float4x4& m) {
   inst.transformInv = inverse4x4(m);
   inst.transformInvTransposed = transpose4x4(inverse4x4(m));
   instances.push_back(inst);
+  inst_set_dirty = true;
   invalidate();
   return uint32_t(instances.size() - 1);
 }
@@ -449,11 +456,85 @@ bool Scene::postpone_triangles() const {
 }
 
 // BVHRT::CommitScene (BVH2Common_host.cpp:1230-1262), then device upload.
+namespace {
+double tlas_area(const std::vector<BVHNode>& t) {
+  double a = 0.0;
+  for (const BVHNode& n : t) {
+    const float3 d = n.boxMax - n.boxMin;
+    a += 2.0 * (double(d.x) * d.y + double(d.x) * d.z + double(d.y) * d.z);
+  }
+  return a;
+}
+}  // namespace
+
 void Scene::build_host() {
   if (instances.empty()) throw std::invalid_argument("commit with no instances");
   std::vector<float4> bmin(instances.size()), bmax(instances.size());
   for (size_t i = 0; i < instances.size(); ++i) { bmin[i] = instances[i].boxMin; bmax[i] = instances[i].boxMax; }
   tlas = build_tlas(bmin.data(), bmax.data(), instances.size());
+  tlasParent.assign(tlas.size(), 0xFFFFFFFFu);
+  for (uint32_t i = 0; i < uint32_t(tlas.size()); ++i) {
+    const uint32_t lo = tlas[i].leftOffset;
+    if (lo & LEAF_BIT) continue;
+    tlasParent[lo] = i;
+    tlasParent[lo + 1] = i;
+  }
+  tlasAreaBuilt = tlasAreaNow = tlas_area(tlas);
+}
+
+void Scene::update_cache_geom() {
+  cacheGeom = 0xFFFFFFFFu;
+  cachePairs = 0;
+  if (instances.size() == 1) {
+    cacheGeom = instances[0].geomId;
+    const uint32_t b = geomData[cacheGeom].bvhOffset;
+    const uint32_t e = cacheGeom + 1 < geomData.size() ? geomData[cacheGeom + 1].bvhOffset : uint32_t(nodePairs.size());
+    cachePairs = std::min<uint32_t>(LDS_TOP_PAIRS, e - b);
+  }
+}
+
+// Commit after UpdateInstance calls only: the instance array (and the resolve's inverse
+// transposes) are uploaded, the BLAS arrays stay; the TLAS keeps its topology and its boxes are
+// refitted on the GPU, then copied back so the host arrays describe what the kernels walk. A
+// refit whose summed node area grows past twice that of the last full build, or tlas_update 0,
+// rebuilds the TLAS on the host instead (CommitScene, BVH2Common_host.cpp:1230-1251).
+hipError_t Scene::commit_instances() {
+  invalidate();
+  hipError_t e = dev["instances"].upload(instances.data(), instances.size() * sizeof(InstanceData));
+  if (e != hipSuccess) return e;
+  std::vector<float4x4> inst_inv_t(instances.size());
+  for (size_t i = 0; i < instances.size(); ++i) inst_inv_t[i] = transpose4x4(instances[i].transformInv);
+  e = dev["inst_inv_t"].upload(inst_inv_t.data(), inst_inv_t.size() * sizeof(float4x4));
+  if (e != hipSuccess) return e;
+  bool rebuild = tlas_update == 0;
+  if (!rebuild) {
+    DevBuf& cnt = dev["tlas_refit_cnt"];
+    e = cnt.reserve(tlas.size() * sizeof(uint32_t));
+    if (e != hipSuccess) return e;
+    e = launch_tlas_refit(static_cast<BVHNode*>(dev["tlas"].ptr()), uint32_t(tlas.size()),
+                          static_cast<const uint32_t*>(dev["tlas_parent"].ptr()),
+                          static_cast<const InstanceData*>(dev["instances"].ptr()), static_cast<uint32_t*>(cnt.ptr()),
+                          nullptr);
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(tlas.data(), dev["tlas"].ptr(), tlas.size() * sizeof(BVHNode), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    tlasAreaNow = tlas_area(tlas);
+    rebuild = tlasAreaNow > 2.0 * tlasAreaBuilt;
+    lastCommitKind = 1;
+  }
+  if (rebuild) {
+    build_host();
+    e = dev["tlas"].upload(tlas.data(), tlas.size() * sizeof(BVHNode));
+    if (e != hipSuccess) return e;
+    e = dev["tlas_parent"].upload(tlasParent.data(), tlasParent.size() * sizeof(uint32_t));
+    if (e != hipSuccess) return e;
+    lastCommitKind = 2;
+  }
+  update_cache_geom();
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) return e;
+  committed = true;
+  return hipSuccess;
 }
 
 // Derives the arrays the kernels read from the reference-layout host arrays. The traversal
@@ -561,12 +642,7 @@ void Scene::build_device_layout() {
       fl.primId = k;
     }
   }
-  cacheGeom = 0xFFFFFFFFu;
-  cachePairs = 0;
-  if (instances.size() == 1) {
-    cacheGeom = instances[0].geomId;
-    cachePairs = std::min<uint32_t>(LDS_TOP_PAIRS, ranges[cacheGeom].second - ranges[cacheGeom].first);
-  }
+  update_cache_geom();
 }
 
 hipError_t Scene::mask_for(StreamWork& w, void* stream, uint32_t** mask) {
@@ -582,7 +658,10 @@ hipError_t Scene::mask_for(StreamWork& w, void* stream, uint32_t** mask) {
 }
 
 hipError_t Scene::commit() {
+  if (!geom_dirty && !inst_set_dirty && !tlas.empty() && dev["tlas"].bytes() == tlas.size() * sizeof(BVHNode))
+    return commit_instances();
   invalidate();   // buffers below may move
+  build_host();
   build_device_layout();
   // occupancy-mask layout: per brick 1 summary word + ceil(bs^3 / 32) bit words
   sbsMaskGeoms.clear();
@@ -607,7 +686,7 @@ hipError_t Scene::commit() {
   UP("indices", indices) UP("prim_indices", primIndices) UP("orig_nodes", origNodes) UP("frame_nodes", frameNodes)
   UP("frame_roots", frameRoots) UP("sbs_nodes", sbsNodes) UP("sbs_data", sbsData) UP("sbs_data_f", sbsDataF)
   UP("sbs_roots", sbsRoots) UP("sbs_headers", sbsHeaders) UP("multi_nodes", multiNodes) UP("multi_values", multiValues)
-  UP("sbs_mask_base", sbsMaskBase) UP("orig_indices", origIndices)
+  UP("sbs_mask_base", sbsMaskBase) UP("orig_indices", origIndices) UP("tlas_parent", tlasParent)
   // MultiRenderer's own m_instanceTransformInvTransposed (MultiRenderer_host.cpp:870-878):
   // transpose(inverse4x4(matrix)), refreshed by UpdateInstance (unlike the BVHRT copy)
   std::vector<float4x4> inst_inv_t(instances.size());
@@ -631,6 +710,8 @@ hipError_t Scene::commit() {
   }
   e = hipDeviceSynchronize();
   if (e != hipSuccess) return e;
+  geom_dirty = inst_set_dirty = false;
+  lastCommitKind = 0;
   committed = true;
   return hipSuccess;
 }

@@ -84,6 +84,19 @@ struct Scene {
   // instance), 0 off, 1 on.
   int leaf_postpone = -1;
   int persistent = 0;   // 1: persistent single-pass trace kernel with per-lane ray refill
+
+  // BLAS builder: false the host binned SAH, true the GPU linear BVH (build names "gpu...").
+  bool gpu_builder = false;
+  BuildStats lastBuild;            // stats of the most recent BLAS build
+  // What the next commit must redo: geometry (every array) or the instance set (TLAS rebuild).
+  // Instance updates alone leave both false: the commit then uploads the instances and refits
+  // the TLAS on the GPU (tlas_update 1) or rebuilds it on the host (tlas_update 0).
+  bool geom_dirty = true, inst_set_dirty = true;
+  int tlas_update = 1;
+  std::vector<uint32_t> tlasParent;   // parent node of every TLAS node (0xFFFFFFFF: root's children)
+  double tlasAreaBuilt = 0.0;         // summed node surface area of the last full TLAS build
+  double tlasAreaNow = 0.0;           // the same after the last refit
+  uint32_t lastCommitKind = 0;        // LRT_COMMIT_* of the last commit
   bool postpone_triangles() const;
 
   // Work memory of one stream: the tail queue (256-B header + list), the deep-stack spill and
@@ -134,7 +147,9 @@ struct Scene {
                                 size_t n_values);
   uint32_t add_instance(uint32_t geom_id, const float4x4& m);
   void update_instance(uint32_t inst_id, const float4x4& m);
-  void build_host();   // TLAS over the instance boxes (host only)
+  void build_host();   // TLAS over the instance boxes (host only) and its parent array
+  hipError_t commit_instances();   // commit after instance updates only (geometry unchanged)
+  void update_cache_geom();
   void build_device_layout();
   hipError_t commit(); // device upload of every array
   void set_preset(const MultiRenderPreset& p);

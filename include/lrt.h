@@ -53,8 +53,11 @@ const char* lrt_version(void);
 /* Replaces CreateSceneRT(impl, build, layout) (MultiRenderer/BVH2Common_host.cpp:1594-1600).
  * impl must be "BVH2Common"; the last digit of build_name sets primitives per leaf
  * (BuilderPresetsFromString, utils/raytracing/cbvh.cpp:57-69); layout_name is accepted and
- * ignored as in BuildBVHFat (cbvh_fat.cpp:150-166). Fails with LRT_ERR_UNSUPPORTED on an
- * unknown impl (the reference returns an empty shared_ptr). */
+ * ignored as in BuildBVHFat (cbvh_fat.cpp:150-166). A build_name starting with "gpu" (for
+ * example "gpu_lbvh2") builds every BLAS on the current HIP device (a linear BVH over Morton
+ * order with the same layout contract and leaf-merge rule); any other name uses the host's
+ * binned-SAH builder. Fails with LRT_ERR_UNSUPPORTED on an unknown impl (the reference returns
+ * an empty shared_ptr). */
 lrt_status lrt_create(const char* impl_name, const char* build_name, const char* layout_name, lrt_scene** out);
 
 /* Destroys the scene and frees its device buffers. */
@@ -103,8 +106,28 @@ lrt_status lrt_add_instance(lrt_scene* s, uint32_t geom_id, const float* matrix1
 lrt_status lrt_update_instance(lrt_scene* s, uint32_t inst_id, const float* matrix16);
 
 /* Replaces BVHRT::CommitScene (BVH2Common_host.cpp:1230-1262): builds the TLAS and uploads
- * every array to the current HIP device. */
+ * every array to the current HIP device. When only lrt_update_instance calls happened since the
+ * last commit, only the instances are uploaded and the TLAS is refitted on the GPU (see
+ * lrt_set_tlas_update). */
 lrt_status lrt_commit(lrt_scene* s, uint32_t build_flags);
+
+/* TLAS policy of commits that follow instance updates only: 1 (default) refit the existing
+ * TLAS's boxes on the GPU, rebuilding on the host once the summed node area exceeds twice that
+ * of the last full build; 0 rebuild on the host every time, as CommitScene does. Hits are the
+ * same either way (closest hits do not depend on the tree). No counterpart in the reference. */
+lrt_status lrt_set_tlas_update(lrt_scene* s, int mode);
+
+#define LRT_COMMIT_FULL 0u           /* every array built and uploaded */
+#define LRT_COMMIT_REFIT 1u          /* instances uploaded, TLAS refitted on the GPU */
+#define LRT_COMMIT_TLAS_REBUILD 2u   /* instances uploaded, TLAS rebuilt on the host */
+typedef struct lrt_commit_info {
+  uint32_t kind;                /* LRT_COMMIT_* of the last commit */
+  uint32_t gpu_builder;         /* 1: BLASes are built on the GPU */
+  uint32_t blas_max_depth, blas_leaves, blas_merged_leaves;   /* the last BLAS build */
+  uint32_t tlas_nodes;
+  float tlas_area_ratio;        /* summed TLAS node area now / after the last full build */
+} lrt_commit_info;
+lrt_status lrt_get_commit_info(lrt_scene* s, lrt_commit_info* out);
 
 /* Replaces BVHRT::SetPreset (BVH2Common_host.cpp:1437-1454). preset: MultiRenderPreset
  * (core/render_settings.h:116-136, 60 B). Takes effect on the next trace call. */
@@ -338,7 +361,7 @@ lrt_status lrt_stack_overflow_count(lrt_scene* s, uint64_t* out, int reset);
  * "indices", "prim_indices", "orig_nodes" (BVHNode), "frame_nodes" (SdfFrameOctreeNode),
  * "frame_roots", "sbs_nodes", "sbs_data", "sbs_data_f", "sbs_roots", "sbs_headers",
  * "multi_nodes", "multi_values", "orig_indices" (uint32: triangle indices in the caller's order),
- * "svs_nodes" (SdfSVSNode), "svs_roots",
+ * "svs_nodes" (SdfSVSNode), "svs_roots", "tlas_parent" (uint32: parent node of each TLAS node),
  * "preset", and "sbs_mask" (uint32: the committed voxel occupancy
  * masks, per brick set in order and per brick one summary word then ceil(bs^3/32) bit words,
  * bit v set when the minimum of voxel v's 8 corner values is <= 0; copied from the device).

@@ -156,6 +156,35 @@ def brute_force_tris(verts: np.ndarray, tris: np.ndarray, pos, dir, threads: int
     return t, ids, t2
 
 
+def tlas_refit(tlas: np.ndarray, instances: np.ndarray) -> np.ndarray:
+    """The TLAS with its topology kept and its boxes recomputed from the instances' boxes: a leaf
+    takes its instance's (BVHRT::UpdateInstance, BVH2Common_host.cpp:1303-1335, computes them),
+    an inner node the componentwise min/max of its two children at leftOffset and leftOffset + 1
+    (the node layout of BuildBVH over boxes, cbvh.cpp:597-664). Pure numpy, children before
+    parents by recursion from the two root children."""
+    out = tlas.copy()
+    leaf = (out["leftOffset"] & np.uint32(0x80000000)) != 0
+
+    def rec(i):
+        if leaf[i]:
+            k = int(out["leftOffset"][i]) & 0x0FFFFFFF
+            out["boxMin"][i] = instances["boxMin"][k][:3]
+            out["boxMax"][i] = instances["boxMax"][k][:3]
+        else:
+            c = int(out["leftOffset"][i])
+            rec(c)
+            rec(c + 1)
+            out["boxMin"][i] = np.minimum(out["boxMin"][c], out["boxMin"][c + 1])
+            out["boxMax"][i] = np.maximum(out["boxMax"][c], out["boxMax"][c + 1])
+
+    import sys
+    sys.setrecursionlimit(max(sys.getrecursionlimit(), 10000))
+    rec(0)
+    if out.size > 1:
+        rec(1)
+    return out
+
+
 def eye_rays(proj_inv, wv_inv, width, height, jitter=None, order=1, flavor: str = "exact"):
     pi = np.ascontiguousarray(proj_inv, dtype=np.float32)
     vi = np.ascontiguousarray(wv_inv, dtype=np.float32)

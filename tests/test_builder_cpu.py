@@ -149,3 +149,54 @@ def test_frame_octree_leaf_boxes(lrt):
         b = orig[lo & 0x0FFFFFFF]
         return b["boxMin"], b["boxMax"]
     subtree_boxes_contain(fat, leaf_box)
+
+
+def test_oracle_tlas_refit_pinned(lrt, orc):
+    """The oracle's TLAS refit (checker of the GPU refit): on a freshly built TLAS every stored box
+    already is the union of its subtree's instance boxes, so a refit changes nothing; after one
+    instance box grows, exactly that leaf and its ancestors change, each to a box containing the
+    new one."""
+    rng = np.random.default_rng(12)
+    v, f = lrt.gen_icosphere(1)
+    mats = []
+    for _ in range(53):
+        m = np.eye(4, dtype=np.float32)
+        m[:3, 3] = rng.uniform(-30, 30, 3)
+        mats.append(m)
+
+    def build(s):
+        gid = s.AddGeom_Triangles3f(v, f)
+        for m in mats:
+            s.AddInstance(gid, m)
+    s = host_scene(lrt, build)
+    tlas, inst = s.get_array("tlas"), s.get_array("instances")
+    assert orc.tlas_refit(tlas, inst).tobytes() == tlas.tobytes()
+    parent = s.get_array("tlas_parent")
+    inst2 = inst.copy()
+    k = 17
+    inst2["boxMax"][k][:3] += 5.0
+    r = orc.tlas_refit(tlas, inst2)
+    changed = set(np.nonzero(np.any(r["boxMax"] != tlas["boxMax"], axis=1))[0].tolist())
+    leaf = [i for i in range(tlas.size) if tlas["leftOffset"][i] & LEAF_BIT and (tlas["leftOffset"][i] & 0x0FFFFFFF) == k][0]
+    path, i = [], leaf
+    while i != 0xFFFFFFFF:
+        path.append(i)
+        i = int(parent[i])
+    assert changed <= set(path) and leaf in changed
+    assert np.all(r["boxMax"][path] >= inst2["boxMax"][k][:3])
+    assert np.array_equal(r["boxMin"], tlas["boxMin"])
+    # the parent array names every inner node's two children
+    for i in range(tlas.size):
+        lo = int(tlas["leftOffset"][i])
+        if not lo & LEAF_BIT:
+            assert parent[lo] == i and parent[lo + 1] == i
+
+
+def test_gpu_builder_without_device_fails_loudly(lrt):
+    """build names starting with "gpu" build on the HIP device; without one the add call fails
+    with a HIP error instead of falling back to the host builder."""
+    v, f = lrt.gen_icosphere(2)
+    s = lrt.SceneRT(build_name="gpu_lbvh2")
+    with pytest.raises(lrt.LrtError) as e:
+        s.AddGeom_Triangles3f(v, f)
+    assert "GPU BVH build" in str(e.value)
