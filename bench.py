@@ -588,8 +588,74 @@ def run_trace(name, args, torch, g, cpu_target_s):
     return result(name, n, ms, args, desc, alg_bytes, cpu)
 
 
+def run_build(args, torch, g):
+    """Acceleration-structure construction: the 327,680-triangle BLAS built on the GPU (linear
+    BVH, build name gpu_lbvh2) against the host's binned SAH, the primary-ray trace on each tree,
+    and the 4096-instance TLAS after moving every instance: GPU refit against host rebuild.
+    Wall-clock times of the ABI calls (uploads and downloads included); step = one GPU build."""
+    v, f = g.gen_icosphere(7, 0.05, 4.0, 1)
+    n_tris = int(f.shape[0])
+    t = time.perf_counter()
+    sh = g.SceneRT()
+    sh.AddInstance(sh.AddGeom_Triangles3f(v, f))
+    host_s = time.perf_counter() - t
+    host_depth = sh.commit_info()["blas_max_depth"]
+    gpu_times = []
+    for _ in range(max(args.warmup, 1) + 3):
+        t = time.perf_counter()
+        sg = g.SceneRT(build_name="gpu_lbvh2")
+        gid = sg.AddGeom_Triangles3f(v, f)
+        gpu_times.append(time.perf_counter() - t)
+    gpu_s = float(np.median(gpu_times[-3:]))
+    info = sg.commit_info()
+    sg.AddInstance(gid)
+    sg.CommitScene()
+    sh.CommitScene()
+    stream = torch.cuda.current_stream()
+    h = stream.cuda_stream
+    pos = torch.zeros((512 * 512, 4), dtype=torch.float32, device="cuda")
+    dirs = torch.zeros_like(pos)
+    pi, vi = camera()
+    g.eye_rays(pi, vi, 512, 512, pos, dirs, order=1, stream=h)
+    hits = torch.zeros((512 * 512, 8), dtype=torch.int32, device="cuda")
+    trace_ms = {}
+    for name, sc in (("sah_host", sh), ("lbvh_gpu", sg)):
+        sc.autotune(pos, dirs, hits, stream=h)
+        trace_ms[name] = round(timed(torch, stream, lambda: sc.trace_closest(pos, dirs, hits, stream=h),
+                                     args.steps, args.warmup), 4)
+    # TLAS: every instance moved by up to 0.5 units, then one commit
+    scene, meshes, mats = tlas_scene()
+    rng = np.random.default_rng(44)
+    commit_ms = {}
+    for mode, label in ((1, "refit_gpu"), (0, "rebuild_host")):
+        scene.set_tlas_update(mode)
+        ts = []
+        for _ in range(3):
+            for i in range(mats.shape[0]):
+                m = mats[i].copy()
+                m[:3, 3] += rng.uniform(-0.5, 0.5, 3).astype(np.float32)
+                scene.UpdateInstance(i, m)
+            t = time.perf_counter()
+            scene.CommitScene()
+            ts.append(time.perf_counter() - t)
+        commit_ms[label] = round(1e3 * float(np.median(ts)), 3)
+        commit_ms[label + "_kind"] = scene.commit_info()["kind"]
+    desc = {"workload": "bvh_build_icosphere7", "triangles": n_tris, "builder": "linear BVH, 63-bit Morton",
+            "gpu_build_ms": round(1e3 * gpu_s, 2), "host_sah_build_ms": round(1e3 * host_s, 1),
+            "gpu_vs_host_build": round(host_s / gpu_s, 1), "blas_leaves": info["blas_leaves"],
+            "blas_max_depth": {"lbvh_gpu": info["blas_max_depth"], "sah_host": host_depth},
+            "trace_ms_512x512": trace_ms, "tlas_instances": int(mats.shape[0]), "tlas_commit_ms": commit_ms,
+            "timed": "wall clock of the ABI call, median of 3 after warm-up"}
+    mtris = n_tris / gpu_s / 1e6
+    return {"value": round(mtris, 2), "unit": "Mtris/s", "ms_per_step": round(1e3 * gpu_s, 3), "steps": 3,
+            "warmup": max(args.warmup, 1), "dtype": "f32", "config": desc, "roofline": None,
+            "cpu_baseline": {"value": round(n_tris / host_s / 1e6, 3), "unit": "Mtris/s", "cores": 1,
+                             "kind": "port", "sample": "the host binned-SAH build of the same mesh"},
+            "speedup_vs_cpu": round(host_s / gpu_s, 1)}
+
+
 TRACE_WORKLOADS = ["icosphere7_512", "sdf_octree_d9_1024", "sdf_multi_d9_1024", "tlas_4096x32_4m"]
-ALL_WORKLOADS = TRACE_WORKLOADS + ["sbs_diff_256_8x256", "sbs_batch_64x4x128"]
+ALL_WORKLOADS = TRACE_WORKLOADS + ["sbs_diff_256_8x256", "sbs_batch_64x4x128", "bvh_build_icosphere7"]
 
 
 def run_one(name, args, torch, g, cpu_target_s):
@@ -597,6 +663,8 @@ def run_one(name, args, torch, g, cpu_target_s):
         return run_diff(args, torch, g, cpu_target_s)
     if name == "sbs_batch_64x4x128":
         return run_batch(args, torch, g, cpu_target_s)
+    if name == "bvh_build_icosphere7":
+        return run_build(args, torch, g)
     return run_trace(name, args, torch, g, cpu_target_s)
 
 
