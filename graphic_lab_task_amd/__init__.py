@@ -638,6 +638,52 @@ def sdf_spheres_sbs_indexed(spheres: np.ndarray, depth: int, brick_size: int, al
     return h, _take(nb, SBS_NODE_DTYPE), _take(vb, np.uint32), _take(fb, np.float32), nd.value
 
 
+def sbs_with_neighbors(header, nodes: np.ndarray, values: np.ndarray, values_f: np.ndarray):
+    """SBS_ind_to_SBS_ind_with_neighbors (sdf/simple/sdf_converter_exotic.cpp:529-587): an indexed
+    brick set (ID32F_IRGB32F) converted to the layout with neighbour indices (ID32F_IRGB32F_IN):
+    every brick keeps its v_size^3 distance indices and 8 colour indices and gains 27 words, word
+    9 (dx + 1) + 3 (dy + 1) + (dz + 1) holding the index of the brick of the same size at offset
+    (dx, dy, dz) in the brick grid, INVALID_IDX (1 << 31) where there is none and for (0, 0, 0).
+    The reference pairs bricks through shared corner value indices; bricks of one size on the grid
+    share a corner exactly when their positions differ by at most one step on every axis, which is
+    the rule applied here. Returns (header, nodes, values, values_f)."""
+    h = np.array(header, dtype=SBS_HEADER_DTYPE).copy()
+    if int(h["aux_data"]) & 0xFF000000 != SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F:
+        raise ValueError("needs the ID32F_IRGB32F layout")
+    v = int(h["brick_size"]) + 2 * int(h["brick_pad"]) + 1
+    old = v * v * v + 8
+    nodes = np.ascontiguousarray(nodes, dtype=SBS_NODE_DTYPE)
+    values = np.ascontiguousarray(values, dtype=np.uint32)
+    n = nodes.size
+    x = (nodes["pos_xy"] >> 16).astype(np.int64)
+    y = (nodes["pos_xy"] & 0xFFFF).astype(np.int64)
+    z = (nodes["pos_z_lod_size"] >> 16).astype(np.int64)
+    lod = (nodes["pos_z_lod_size"] & 0xFFFF).astype(np.int64)
+    key = ((lod * 65536 + x) * 65536 + y) * 65536 + z
+    order = np.argsort(key)
+    skey = key[order]
+    nbr = np.full((n, 27), 1 << 31, np.uint32)
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                if dx == dy == dz == 0:
+                    continue
+                q = ((lod * 65536 + x + dx) * 65536 + y + dy) * 65536 + z + dz
+                ok = (x + dx >= 0) & (y + dy >= 0) & (z + dz >= 0) & (x + dx < lod) & (y + dy < lod) & (z + dz < lod)
+                pos = np.clip(np.searchsorted(skey, q), 0, n - 1)
+                hit = ok & (skey[pos] == q)
+                nbr[hit, 9 * (dx + 1) + 3 * (dy + 1) + (dz + 1)] = order[pos[hit]].astype(np.uint32)
+    out = np.zeros(n * (old + 27), np.uint32)
+    base = nodes["data_offset"].astype(np.int64)
+    blocks = values[base[:, None] + np.arange(old)[None, :]]
+    out.reshape(n, old + 27)[:, :old] = blocks
+    out.reshape(n, old + 27)[:, old:] = nbr
+    nn = nodes.copy()
+    nn["data_offset"] = (np.arange(n) * (old + 27)).astype(np.uint32)
+    h["aux_data"] = SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN
+    return h, nn, out, np.ascontiguousarray(values_f, dtype=np.float32)
+
+
 # ---- reference-format SDF scene files (sdf/simple/sdf_simple.cpp, sdf/multi/sdf_multi.cpp) ----
 def load_sdf_frame_octree(path: str) -> np.ndarray:
     """load_sdf_frame_octree (sdf/simple/sdf_simple.cpp:58-66): SdfFrameOctreeNode array."""

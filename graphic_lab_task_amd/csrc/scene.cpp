@@ -329,7 +329,8 @@ uint32_t Scene::add_sdf_sbs(const SdfSBSHeader& header, const SdfSBSNode* nodes,
   // Bounds of every data access the intersector can make (checked on the host once).
   const uint32_t v_size = header.brick_size + 2 * header.brick_pad + 1;
   const uint32_t vpi = 4 / header.bytes_per_value;
-  const size_t per_brick = indexed ? size_t(v_size) * v_size * v_size + 8
+  const bool with_neighbors = layout == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN;
+  const size_t per_brick = indexed ? size_t(v_size) * v_size * v_size + 8 + (with_neighbors ? 27 : 0)
                                    : (size_t(v_size) * v_size * v_size + vpi - 1) / vpi +
                                          ((layout == SDF_SBS_NODE_LAYOUT_DX_RGB8 || layout == SDF_SBS_NODE_LAYOUT_DX_UV16) ? 8 : 0);
   for (size_t i = 0; i < n_nodes; ++i) {
@@ -345,6 +346,15 @@ uint32_t Scene::add_sdf_sbs(const SdfSBSHeader& header, const SdfSBSNode* nodes,
         if (w[k] >= n_values_f) throw std::invalid_argument("distance index out of range");
       for (size_t k = v3; k < v3 + 8; ++k)
         if (size_t(w[k]) + 3 > n_values_f) throw std::invalid_argument("colour index out of range");
+      // neighbour brick ids (ID32F_IRGB32F_IN, read by the smoothed normals as scene node ids):
+      // every word of an indexed brick set is shifted by the set's first values_f entry below,
+      // as in the reference, so they stay valid for the scene's first indexed set only
+      const uint64_t f_shift = sbsDataF.size(), total_nodes = sbsNodes.size() + n_nodes;
+      for (size_t k = v3 + 8; with_neighbors && k < v3 + 8 + 27; ++k) {
+        const uint64_t nid = uint64_t(w[k]) + f_shift;
+        if (nid != INVALID_IDX && nid >= total_nodes)
+          throw std::invalid_argument("neighbour brick index out of range (or a later indexed brick set with neighbours)");
+      }
     }
   }
   GeomData g{};

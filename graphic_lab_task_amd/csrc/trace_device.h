@@ -533,6 +533,83 @@ __device__ __forceinline__ bool sbs_voxel_occupied(const SceneView& s, const Sdf
 // The voxel whose surface produced a brick-set hit: the value indices of its 8 corners (into
 // the scene's values_f, indexed layouts), the entry point in voxel units, and the hit
 // parameter along the ray direction from there.
+// eval_dist_trilinear_diff (BVH2Common.cpp:751-781): gradient of the trilinear of 8 values.
+__device__ __forceinline__ float3 eval_dist_trilinear_diff(const float v[8], float3 dp) {
+  const float ddx = -(1 - dp.y) * (1 - dp.z) * v[0] + -(1 - dp.y) * (dp.z) * v[1] + -(dp.y) * (1 - dp.z) * v[2] +
+                    -(dp.y) * (dp.z) * v[3] + (1 - dp.y) * (1 - dp.z) * v[4] + (1 - dp.y) * (dp.z) * v[5] +
+                    (dp.y) * (1 - dp.z) * v[6] + (dp.y) * (dp.z) * v[7];
+  const float ddy = -(1 - dp.x) * (1 - dp.z) * v[0] + -(1 - dp.x) * (dp.z) * v[1] + (1 - dp.x) * (1 - dp.z) * v[2] +
+                    (1 - dp.x) * (dp.z) * v[3] + -(dp.x) * (1 - dp.z) * v[4] + -(dp.x) * (dp.z) * v[5] +
+                    (dp.x) * (1 - dp.z) * v[6] + (dp.x) * (dp.z) * v[7];
+  const float ddz = -(1 - dp.x) * (1 - dp.y) * v[0] + (1 - dp.x) * (1 - dp.y) * v[1] + -(1 - dp.x) * (dp.y) * v[2] +
+                    (1 - dp.x) * (dp.y) * v[3] + -(dp.x) * (1 - dp.y) * v[4] + (dp.x) * (1 - dp.y) * v[5] +
+                    -(dp.x) * (dp.y) * v[6] + (dp.x) * (dp.y) * v[7];
+  return make3(ddx, ddy, ddz);
+}
+
+// step (BVH2Common.cpp:784-788): 0 at edge0 rising linearly to 1 at edge1.
+__device__ __forceinline__ float step_ref(float edge0, float edge1, float x) {
+  return clamp_ref((x - edge0) / (edge1 - edge0), 0.0f, 1.0f);
+}
+
+// Smoothed normal of the brick-set layout with neighbour indices (ID32F_IRGB32F_IN;
+// BVH2Common.cpp:958-1028): the gradients of the 8 voxels around the hit point (taken from the
+// neighbouring bricks listed after the brick's colour indices where they leave the brick; a
+// missing neighbour contributes zero) blended trilinearly by the hit point's offset from the
+// voxel centres, normalised and encoded into coords[2..3]. As in the reference, the instance
+// transform is not applied to it. Out of line: it runs once per hit and keeps its registers off
+// the march.
+__device__ __noinline__ void sbs_smoothed_normal(const SceneView& s, const SdfSBSHeader& header, uint32_t nodeId,
+                                                 uint32_t data_offset, float3 voxelPos, uint32_t v_size, float sz_inv,
+                                                 float d, float3 start_q, float f_near, float3 ray_dir, CRT_Hit& hit) {
+  const float beta = 0.5f;
+  const float3 dp = start_q + ((hit.t - f_near) / d) * ray_dir;
+  float3 normals[8];
+  float values_n[8];
+  for (int i = 0; i < 8; i++) normals[i] = make3(0, 0, 0);
+  float3 nmq;
+  nmq.x = dp.x < 0.5f ? step_ref(-beta, beta, dp.x) : step_ref(-beta, beta, dp.x - 1.0f);
+  nmq.y = dp.y < 0.5f ? step_ref(-beta, beta, dp.y) : step_ref(-beta, beta, dp.y - 1.0f);
+  nmq.z = dp.z < 0.5f ? step_ref(-beta, beta, dp.z) : step_ref(-beta, beta, dp.z - 1.0f);
+  const int bs = int(header.brick_size);
+  const int vx0 = dp.x < 0.5f ? int(voxelPos.x) - 1 : int(voxelPos.x);
+  const int vy0 = dp.y < 0.5f ? int(voxelPos.y) - 1 : int(voxelPos.y);
+  const int vz0 = dp.z < 0.5f ? int(voxelPos.z) - 1 : int(voxelPos.z);
+  for (int i = 0; i < 8; i++) {
+    const int vxi = vx0 + ((i & 4) >> 2), vyi = vy0 + ((i & 2) >> 1), vzi = vz0 + (i & 1);
+    const int bx = vxi >= 0 ? (vxi < bs ? 0 : 1) : -1;
+    const int by = vyi >= 0 ? (vyi < bs ? 0 : 1) : -1;
+    const int bz = vzi >= 0 ? (vzi < bs ? 0 : 1) : -1;
+    const uint32_t neighborId = uint32_t(9 * (bx + 1) + 3 * (by + 1) + (bz + 1));
+    const float3 vpi = make3(float(vxi), float(vyi), float(vzi));
+    const float3 dVoxelPos = vpi - voxelPos;
+    uint32_t nnode;
+    float3 nvp;
+    if (neighborId == 9 + 3 + 1) {
+      nnode = nodeId;
+      nvp = vpi;
+    } else {
+      nnode = s.sbs_data[data_offset + v_size * v_size * v_size + 8 + neighborId];
+      nvp = vpi - float(header.brick_size) * make3(float(bx), float(by), float(bz));
+    }
+    if (nnode != INVALID_IDX) {
+      sbs_load_values(s, header, nnode, nvp, v_size, sz_inv, values_n);
+      normals[i] = normalize3(eval_dist_trilinear_diff(values_n, dp - dVoxelPos));
+    }
+  }
+  float3 sm = ((1 - nmq.x) * (1 - nmq.y) * (1 - nmq.z)) * normals[0];
+  sm = sm + ((1 - nmq.x) * (1 - nmq.y) * (nmq.z)) * normals[1];
+  sm = sm + ((1 - nmq.x) * (nmq.y) * (1 - nmq.z)) * normals[2];
+  sm = sm + ((1 - nmq.x) * (nmq.y) * (nmq.z)) * normals[3];
+  sm = sm + ((nmq.x) * (1 - nmq.y) * (1 - nmq.z)) * normals[4];
+  sm = sm + ((nmq.x) * (1 - nmq.y) * (nmq.z)) * normals[5];
+  sm = sm + ((nmq.x) * (nmq.y) * (1 - nmq.z)) * normals[6];
+  sm = sm + ((nmq.x) * (nmq.y) * (nmq.z)) * normals[7];
+  const float2 e = encode_normal(normalize3(sm));
+  hit.coords[2] = e.x;
+  hit.coords[3] = e.y;
+}
+
 struct SbsVoxelRec {
   uint32_t idx[8];
   float3 start_q;
@@ -605,6 +682,11 @@ __device__ uint32_t sbs_brick_intersect(const SceneView& s, const GeomRef& g, fl
       float t_local = 0.0f;
       const bool got = local_surface_intersection(s, type, ray_dir, instId, geomId, values, primId, d, 0.0f, qFar,
                                                   fNearFar, start_q, hit, rec ? &t_local : nullptr);
+      if (got && header.aux_data == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN &&
+          s.preset.interpolation_mode == INTERPOLATION_MODE_TRILINEAR &&
+          s.preset.normal_mode == NORMAL_MODE_SDF_SMOOTHED && preset_needs_normal(s.preset.render_mode))
+        sbs_smoothed_normal(s, header, nodeId, node.data_offset, voxelPos, v_size, sz_inv, d, start_q, fNearFar.x,
+                            ray_dir, hit);
       if (rec && got) {
         const uint32_t vx = uint32_t(voxelPos.x), vy = uint32_t(voxelPos.y), vz = uint32_t(voxelPos.z);
 #pragma unroll

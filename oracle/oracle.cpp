@@ -477,6 +477,77 @@ struct Tracer {
     return vmin;
   }
 
+  // eval_dist_trilinear_diff (BVH2Common.cpp:751-781)
+  static f3 eval_dist_trilinear_diff(const float v[8], f3 dp) {
+    float ddist_dx = -(1 - dp.y) * (1 - dp.z) * v[0] + -(1 - dp.y) * (dp.z) * v[1] + -(dp.y) * (1 - dp.z) * v[2] +
+                     -(dp.y) * (dp.z) * v[3] + (1 - dp.y) * (1 - dp.z) * v[4] + (1 - dp.y) * (dp.z) * v[5] +
+                     (dp.y) * (1 - dp.z) * v[6] + (dp.y) * (dp.z) * v[7];
+    float ddist_dy = -(1 - dp.x) * (1 - dp.z) * v[0] + -(1 - dp.x) * (dp.z) * v[1] + (1 - dp.x) * (1 - dp.z) * v[2] +
+                     (1 - dp.x) * (dp.z) * v[3] + -(dp.x) * (1 - dp.z) * v[4] + -(dp.x) * (dp.z) * v[5] +
+                     (dp.x) * (1 - dp.z) * v[6] + (dp.x) * (dp.z) * v[7];
+    float ddist_dz = -(1 - dp.x) * (1 - dp.y) * v[0] + (1 - dp.x) * (1 - dp.y) * v[1] + -(1 - dp.x) * (dp.y) * v[2] +
+                     (1 - dp.x) * (dp.y) * v[3] + -(dp.x) * (1 - dp.y) * v[4] + (dp.x) * (1 - dp.y) * v[5] +
+                     -(dp.x) * (dp.y) * v[6] + (dp.x) * (dp.y) * v[7];
+    return f3(ddist_dx, ddist_dy, ddist_dz);
+  }
+  // step (BVH2Common.cpp:784-788)
+  static float step(float edge0, float edge1, float x) {
+    return std::min(std::max(0.f, (x - edge0) / (edge1 - edge0)), 1.f);
+  }
+
+  // BVH2Common.cpp:958-1028: smoothed normal of the ID32F_IRGB32F_IN layout (not transformed by
+  // the instance, as in the reference)
+  void SmoothedNormal(const SBSHeader& header, uint32_t nodeId, uint32_t data_offset, f3 voxelPos, uint32_t v_size,
+                      float sz_inv, float d, f3 start_q, float f_near, f3 ray_dir, Hit* pHit) {
+    const float beta = 0.5f;
+    f3 dp = start_q + ((pHit->t - f_near) / d) * ray_dir;
+    f3 normals[8];
+    float values_n[8];
+    for (int i = 0; i < 8; i++) normals[i] = f3(0, 0, 0);
+    f3 nmq(0.5f, 0.5f, 0.5f);
+    nmq.x = dp.x < 0.5f ? step(-beta, beta, dp.x) : step(-beta, beta, dp.x - 1.0f);
+    nmq.y = dp.y < 0.5f ? step(-beta, beta, dp.y) : step(-beta, beta, dp.y - 1.0f);
+    nmq.z = dp.z < 0.5f ? step(-beta, beta, dp.z) : step(-beta, beta, dp.z - 1.0f);
+    const int bs = int(header.brick_size);
+    const int vp0[3] = {dp.x < 0.5f ? int(voxelPos.x) - 1 : int(voxelPos.x), dp.y < 0.5f ? int(voxelPos.y) - 1 : int(voxelPos.y),
+                        dp.z < 0.5f ? int(voxelPos.z) - 1 : int(voxelPos.z)};
+    for (int i = 0; i < 8; i++) {
+      const int VI[3] = {vp0[0] + ((i & 4) >> 2), vp0[1] + ((i & 2) >> 1), vp0[2] + (i & 1)};
+      int BI[3];
+      for (int a = 0; a < 3; a++) BI[a] = VI[a] >= 0 ? (VI[a] < bs ? 0 : 1) : -1;
+      uint32_t neighborId = 0;
+      neighborId += 3 * 3 * (BI[0] + 1);
+      neighborId += 3 * (BI[1] + 1);
+      neighborId += (BI[2] + 1);
+      const f3 vpi = f3(float(VI[0]), float(VI[1]), float(VI[2]));
+      f3 dVoxelPos = vpi - voxelPos;
+      uint32_t neighbor_nodeId;
+      f3 neighbor_voxelPos;
+      if (neighborId == 9 + 3 + 1) {
+        neighbor_nodeId = nodeId;
+        neighbor_voxelPos = vpi;
+      } else {
+        neighbor_nodeId = s.sbs_data[data_offset + v_size * v_size * v_size + 8 + neighborId];
+        neighbor_voxelPos = vpi - float(header.brick_size) * f3(float(BI[0]), float(BI[1]), float(BI[2]));
+      }
+      if (neighbor_nodeId != 0x80000000u) {   // INVALID_IDX, sdf_common_shared.h:30
+        load_distance_values(neighbor_nodeId, neighbor_voxelPos, v_size, sz_inv, header, values_n);
+        normals[i] = normalize(eval_dist_trilinear_diff(values_n, dp - dVoxelPos));
+      }
+    }
+    f3 sm = ((1 - nmq.x) * (1 - nmq.y) * (1 - nmq.z)) * normals[0];
+    sm = sm + ((1 - nmq.x) * (1 - nmq.y) * (nmq.z)) * normals[1];
+    sm = sm + ((1 - nmq.x) * (nmq.y) * (1 - nmq.z)) * normals[2];
+    sm = sm + ((1 - nmq.x) * (nmq.y) * (nmq.z)) * normals[3];
+    sm = sm + ((nmq.x) * (1 - nmq.y) * (1 - nmq.z)) * normals[4];
+    sm = sm + ((nmq.x) * (1 - nmq.y) * (nmq.z)) * normals[5];
+    sm = sm + ((nmq.x) * (nmq.y) * (1 - nmq.z)) * normals[6];
+    sm = sm + ((nmq.x) * (nmq.y) * (nmq.z)) * normals[7];
+    f2 e = encode_normal(normalize(sm));
+    pHit->coords[2] = e.x;
+    pHit->coords[3] = e.y;
+  }
+
   // BVH2Common.cpp:884-1112 (trilinear)
   void OctreeBrickIntersect(uint32_t type, f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
                             uint32_t bvhNodeId, Hit* pHit) {
@@ -516,6 +587,9 @@ struct Tracer {
         float qFar = (fNearFar.y - fNearFar.x) * (0.5f * sz * header.brick_size);
         const float t_before = pHit->t;
         LocalSurfaceIntersection(type, ray_dir, instId, geomId, values, primId, d, 0.0f, qFar, fNearFar, start_q, pHit);
+        if (s.preset.interpolation_mode == 0 && header.aux_data == LAYOUT_ID32F_IN && s.preset.normal_mode == 2 &&
+            need_normal() && pHit->t != t_before)
+          SmoothedNormal(header, nodeId, nd.data_offset, voxelPos, v_size, sz_inv, d, start_q, fNearFar.x, ray_dir, pHit);
         if (rec && pHit->t < t_before && (header.aux_data == LAYOUT_ID32F || header.aux_data == LAYOUT_ID32F_IN)) {
           rec->hit = 1;
           for (int i = 0; i < 8; i++) {

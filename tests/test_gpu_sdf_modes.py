@@ -149,3 +149,39 @@ def test_sdf_debug_render_modes(lrt, orc, torch_dev, scenes, rays, kind, render_
         assert want["instId"][~hit].max() >= 2 and np.all(want["instId"][hit] == 1)
     if render_mode in (24, 25):
         assert np.all(want["primId"][hit] != MISS)
+
+
+@pytest.fixture(scope="module")
+def in_scene(lrt, torch_dev):
+    g = lrt
+    h, n, v, vf, nd = g.sdf_spheres_sbs_indexed(sphere_set(), 4, 8, 1)
+    H, N, V, F = g.sbs_with_neighbors(h, n, v, vf)
+    s = g.SceneRT()
+    gid = s.AddGeom_SdfSBS(H, N, V, F)
+    s.AddInstance(gid)
+    m = np.eye(4, dtype=np.float32)
+    m[:3, :3] = np.array([[0, -1, 0], [1, 0, 0], [0, 0, 1]], np.float32) * 0.5
+    m[:3, 3] = [1.2, 0.3, -0.5]
+    s.AddInstance(gid, m)
+    s.CommitScene()
+    return s
+
+
+@pytest.mark.parametrize("mode", [0, 1, 3, 4])
+def test_sbs_smoothed_normals(lrt, orc, torch_dev, in_scene, rays, mode):
+    """Brick set with neighbour indices (ID32F_IRGB32F_IN), normal_mode SDF_SMOOTHED, Lambert:
+    the smoothed normals (BVH2Common.cpp:958-1028, read across bricks) against the oracle, two
+    instances (one rotated: the smoothed normal stays in the brick set's frame, as in the
+    reference); bit-exact except in the analytic mode, which keeps the SDF bar."""
+    _say(f"sbs smoothed normals intersect={MODES[mode]}")
+    pos, dirs = rays
+    p = lrt.default_preset()
+    p["render_mode"], p["normal_mode"], p["sdf_node_intersect"] = 1, 2, mode
+    got, want = _run(lrt, orc, torch_dev, in_scene, pos, dirs, p)
+    in_scene.SetPreset(lrt.default_preset())
+    r = sdf_check(got, want, pos.cpu().numpy(), dirs.cpu().numpy(), colour=True)
+    _say(str(r))
+    assert r["hits"] > 0.1 * r["n"]
+    assert_sdf_parity(r, got, want)
+    if mode != 1:
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
