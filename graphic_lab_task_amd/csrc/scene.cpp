@@ -760,6 +760,13 @@ SceneView Scene::make_view() const {
   v.preset = preset;
   v.lod_thr = lod_thr;
   v.leaf_kinds = leaf_kinds();
+  // smoothed brick-set normals (BVH2Common.cpp:958-961): neighbour-index layout, trilinear,
+  // normal mode SDF_SMOOTHED and a render mode that needs normals
+  if ((v.leaf_kinds & LK_SBS) && preset.normal_mode == NORMAL_MODE_SDF_SMOOTHED &&
+      preset.interpolation_mode == INTERPOLATION_MODE_TRILINEAR && preset_needs_normal(preset.render_mode)) {
+    for (const SdfSBSHeader& h : sbsHeaders)
+      if (h.aux_data == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN) v.leaf_kinds |= LK_SBS_SMOOTH;
+  }
   v.n_instances = uint32_t(instances.size());
   v.cache_geom = cacheGeom;
   v.cache_pairs = cachePairs;

@@ -14,6 +14,10 @@ enum LeafKind : uint32_t {
   // Walk option, not a leaf kind (never in SceneView::leaf_kinds): triangle leaves are
   // postponed like SDF leaves (traverse.h).
   LK_TRI_POSTPONE = 1u << 4,
+  // Walk option set in SceneView::leaf_kinds when the preset asks for smoothed normals and a
+  // brick set has neighbour indices (ID32F_IRGB32F_IN): selects the kernels that compile the
+  // smoothing in (the others stay free of its registers).
+  LK_SBS_SMOOTH = 1u << 5,
 };
 
 // Frame-octree leaf as the kernels read it: the leaf box, its 8 corner distances and the

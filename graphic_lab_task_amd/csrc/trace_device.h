@@ -557,9 +557,8 @@ __device__ __forceinline__ float step_ref(float edge0, float edge1, float x) {
 // neighbouring bricks listed after the brick's colour indices where they leave the brick; a
 // missing neighbour contributes zero) blended trilinearly by the hit point's offset from the
 // voxel centres, normalised and encoded into coords[2..3]. As in the reference, the instance
-// transform is not applied to it. Out of line: it runs once per hit and keeps its registers off
-// the march.
-__device__ __noinline__ void sbs_smoothed_normal(const SceneView& s, const SdfSBSHeader& header, uint32_t nodeId,
+// transform is not applied to it. Compiled only into the kernels selected by LK_SBS_SMOOTH.
+__device__ __forceinline__ void sbs_smoothed_normal(const SceneView& s, const SdfSBSHeader& header, uint32_t nodeId,
                                                  uint32_t data_offset, float3 voxelPos, uint32_t v_size, float sz_inv,
                                                  float d, float3 start_q, float f_near, float3 ray_dir, CRT_Hit& hit) {
   const float beta = 0.5f;
@@ -618,7 +617,9 @@ struct SbsVoxelRec {
 
 // Sparse-brick-set leaf: BVHRT::OctreeBrickIntersect (BVH2Common.cpp:884-1112), trilinear.
 // Returns the number of voxels whose values were loaded (all voxels stepped without an
-// occupancy mask). `rec` (indexed layouts only) records the hit voxel.
+// occupancy mask). `rec` (indexed layouts only) records the hit voxel. SMOOTH compiles in the
+// smoothed normals of the neighbour-index layout (applied when the preset asks for them).
+template <bool SMOOTH = false>
 __device__ uint32_t sbs_brick_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir, float tNear,
                                         uint32_t instId, uint32_t geomId, uint32_t bvhNodeId, CRT_Hit& hit,
                                         SbsVoxelRec* rec = nullptr) {
@@ -682,7 +683,7 @@ __device__ uint32_t sbs_brick_intersect(const SceneView& s, const GeomRef& g, fl
       float t_local = 0.0f;
       const bool got = local_surface_intersection(s, type, ray_dir, instId, geomId, values, primId, d, 0.0f, qFar,
                                                   fNearFar, start_q, hit, rec ? &t_local : nullptr);
-      if (got && header.aux_data == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN &&
+      if (SMOOTH && got && header.aux_data == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN &&
           s.preset.interpolation_mode == INTERPOLATION_MODE_TRILINEAR &&
           s.preset.normal_mode == NORMAL_MODE_SDF_SMOOTHED && preset_needs_normal(s.preset.render_mode))
         sbs_smoothed_normal(s, header, nodeId, node.data_offset, voxelPos, v_size, sz_inv, d, start_q, fNearFar.x,
