@@ -252,7 +252,15 @@ def run_batch(args, torch, g, cpu_target_s):
     build_s = time.time() - t0
     n = sum(it["pos"].shape[0] for it in items)
     batch = g.SbsBatch(items)
-    tune = None if args.no_autotune else batch.autotune(sh)
+    tune = None
+    if args.batch_config:
+        lds, rec, grid = (int(x) for x in args.batch_config.split(","))
+        batch.configure(lds, rec, grid)
+        tune = {"lds_stack": lds, "record": rec, "grid": grid, "given": True}
+    elif not args.no_autotune:
+        tune = batch.autotune(sh)
+    if args.copy_calibration:
+        copy_calibration(torch)
 
     # algorithmic bytes of one batched step: per item, the traversal of its rays (instrumented
     # launch), ray I/O and the per-hit gradient scatter, plus the set-up kernels: gradient zeroing
@@ -620,7 +628,8 @@ def run_build(args, torch, g):
     hits = torch.zeros((512 * 512, 8), dtype=torch.int32, device="cuda")
     trace_ms = {}
     for name, sc in (("sah_host", sh), ("lbvh_gpu", sg)):
-        sc.autotune(pos, dirs, hits, stream=h)
+        if not args.no_autotune:
+            sc.autotune(pos, dirs, hits, stream=h)
         trace_ms[name] = round(timed(torch, stream, lambda: sc.trace_closest(pos, dirs, hits, stream=h),
                                      args.steps, args.warmup), 4)
     # TLAS: every instance moved by up to 0.5 units, then one commit
@@ -691,6 +700,8 @@ def main():
                     help="triangle leaf postponement when not autotuning (-1: on for one instance)")
     ap.add_argument("--persistent", type=int, default=0, choices=[0, 1],
                     help="persistent single-pass kernel when not autotuning")
+    ap.add_argument("--batch-config", default="",
+                    help="lds_stack,record,grid of the batched adjoint kernel instead of its autotuner")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for the HBM byte counters)")
     args = ap.parse_args()

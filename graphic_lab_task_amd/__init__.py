@@ -196,6 +196,7 @@ def lib() -> ctypes.CDLL:
             "lrt_sbs_batch_create": ([vp, u32, ctypes.POINTER(vp)], i32),
             "lrt_sbs_batch_run": ([vp, vp], i32),
             "lrt_sbs_batch_autotune": ([vp, vp, ctypes.POINTER(_BatchTuneResult)], i32),
+            "lrt_sbs_batch_configure": ([vp, i32, i32, u32], i32),
             "lrt_sbs_batch_destroy": ([vp], i32),
             "lrt_stack_overflow_count": ([vp, ctypes.POINTER(u64), i32], i32),
             "lrt_get_array": ([vp, cp, vp, ctypes.POINTER(sz)], i32),
@@ -235,7 +236,7 @@ def exported_symbols():
             "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_set_ray_interleave", "lrt_set_persistent",
             "lrt_set_tlas_update", "lrt_get_commit_info", "lrt_autotune", "lrt_sbs_param_layout",
             "lrt_sbs_forward_dev", "lrt_sbs_backward_dev", "lrt_sbs_l2_backward_dev", "lrt_sbs_render_l2_backward_dev", "lrt_sbs_recompute_misses",
-            "lrt_sbs_batch_create", "lrt_sbs_batch_run", "lrt_sbs_batch_autotune", "lrt_sbs_batch_destroy",
+            "lrt_sbs_batch_create", "lrt_sbs_batch_run", "lrt_sbs_batch_autotune", "lrt_sbs_batch_configure", "lrt_sbs_batch_destroy",
             "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
             "lrt_camera_inverse", "lrt_gen_icosphere", "lrt_gen_geodesic", "lrt_sdf_spheres_frame_octree",
             "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed", "lrt_resolve_dev", "lrt_tonemap_dev",
@@ -553,6 +554,10 @@ class SbsBatch:
         r = _BatchTuneResult()
         _check(lib().lrt_sbs_batch_autotune(self._h, stream, ctypes.byref(r)))
         return {f: getattr(r, f) for f, _ in _BatchTuneResult._fields_}
+
+    def configure(self, lds_stack: int, record: int, grid: int):
+        """Sets the kernel variant (LDS stack entries, hit-voxel record) and persistent grid."""
+        _check(lib().lrt_sbs_batch_configure(self._h, lds_stack, record, grid))
 
     def __del__(self):
         if getattr(self, "_h", None):

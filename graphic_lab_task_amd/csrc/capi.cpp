@@ -902,6 +902,21 @@ lrt_status lrt_sbs_batch_autotune(lrt_sbs_batch* b, void* stream, lrt_sbs_batch_
   });
 }
 
+lrt_status lrt_sbs_batch_configure(lrt_sbs_batch* b, int lds_stack, int record, uint32_t grid) {
+  return guarded([&]() -> lrt_status {
+    if (!b) return fail(LRT_ERR_INVALID_ARG, "null batch");
+    if (lds_stack != 8 && lds_stack != 16 && lds_stack != 32) return fail(LRT_ERR_INVALID_ARG, "lds_stack must be 8, 16 or 32");
+    if (record != 0 && record != 1) return fail(LRT_ERR_INVALID_ARG, "record must be 0 or 1");
+    const int v = (lds_stack == 8 ? 0 : lds_stack == 16 ? 2 : 4) + record;
+    if (grid == 0 || grid > b->resident[v])
+      return fail(LRT_ERR_INVALID_ARG, "grid must be 1.." + std::to_string(b->resident[v]) + " for this variant");
+    b->L.lds_stack = lds_stack;
+    b->L.record = record;
+    b->L.grid = grid;
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_sbs_batch_run(lrt_sbs_batch* b, void* stream) {
   return guarded([&]() -> lrt_status {
     if (!b) return fail(LRT_ERR_INVALID_ARG, "null batch");

@@ -320,6 +320,10 @@ typedef struct lrt_sbs_batch_tune_result {
   uint32_t configs_tried;
 } lrt_sbs_batch_tune_result;
 lrt_status lrt_sbs_batch_autotune(lrt_sbs_batch* b, void* stream, lrt_sbs_batch_tune_result* out);
+/* Sets the batch kernel variant and persistent grid directly (a configuration lrt_sbs_batch_autotune
+ * reported earlier): lds_stack 8, 16 or 32, record 0 or 1, grid 1 .. the variant's resident
+ * blocks. No counterpart in the reference. */
+lrt_status lrt_sbs_batch_configure(lrt_sbs_batch* b, int lds_stack, int record, uint32_t grid);
 lrt_status lrt_sbs_batch_destroy(lrt_sbs_batch* b);
 
 /* Rays whose recomputed hit voxel did not reproduce the walk's hit (expected 0). */

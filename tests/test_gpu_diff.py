@@ -184,6 +184,39 @@ def test_batch_matches_per_scene_calls(lrt, torch_dev):
         assert it["scene"].sbs_recompute_misses() == 0
 
 
+def test_batch_configure_variants_agree(lrt, torch_dev):
+    """lrt_sbs_batch_configure: every kernel variant (LDS stack 8/16/32, hit-voxel record off/on)
+    at a small and the full grid gives the same colours and losses and gradients within float
+    atomics' reordering; out-of-range settings are refused."""
+    import bench
+    torch = torch_dev
+    _say("batch configure")
+    items = bench.sbs_batch_scenes(torch, lrt, n_scenes=3, res=48, views=2, seed=9)
+    batch = lrt.SbsBatch(items)
+    ref = None
+    for lds in (8, 16, 32):
+        for rec in (0, 1):
+            for grid in (7, 1 << 30):
+                try:
+                    batch.configure(lds, rec, grid)
+                except lrt.LrtError:
+                    assert grid == 1 << 30   # above the resident blocks
+                    continue
+                batch.run()
+                torch.cuda.synchronize()
+                out = [(it["colour"].clone(), it["loss"].clone(), it["grad"].clone()) for it in items]
+                if ref is None:
+                    ref = out
+                    continue
+                for (c, l, g), (c0, l0, g0) in zip(out, ref):
+                    assert torch.equal(c, c0)
+                    assert abs(l.item() - l0.item()) <= 1e-5 * max(l0.item(), 1e-30)
+                    assert _rel(g, g0) < 1e-5
+    for bad in ((12, 0, 1), (16, 2, 1), (16, 0, 0)):
+        with pytest.raises(lrt.LrtError):
+            batch.configure(*bad)
+
+
 def test_batch_ragged_items_and_fused_entry(lrt, torch_dev):
     """Items whose ray counts are not multiples of the 64-ray chunk (and one empty item), against
     per-scene calls of the fused render + L2 adjoint entry on the same rays."""
