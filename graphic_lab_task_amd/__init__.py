@@ -57,7 +57,8 @@ ARRAY_DTYPES = {
     "sbs_nodes": SBS_NODE_DTYPE, "sbs_data": np.uint32, "sbs_data_f": np.float32, "sbs_roots": np.uint32,
     "sbs_headers": SBS_HEADER_DTYPE, "multi_nodes": MULTI_NODE_DTYPE, "multi_values": np.float32,
     "preset": PRESET_DTYPE, "sbs_mask": np.uint32, "orig_indices": np.uint32, "svs_nodes": SVS_NODE_DTYPE,
-    "svs_roots": np.uint32, "tlas_parent": np.uint32,
+    "svs_roots": np.uint32, "tlas_parent": np.uint32, "grid_data": np.float32, "grid_offsets": np.uint32,
+    "grid_sizes": np.uint32,
 }
 LIGHT_DTYPE = np.dtype([("space", "<f4", (3,)), ("type", "<u4"), ("color", "<f4", (3,)), ("_pad", "<u4")])  # Light, 32 B
 LIGHT_TYPE_DIRECT, LIGHT_TYPE_POINT, LIGHT_TYPE_AMBIENT = 0, 1, 2
@@ -166,6 +167,9 @@ def lib() -> ctypes.CDLL:
             "lrt_add_triangles": ([vp, vp, sz, sz, vp, vp, sz, u32, pu32], i32),
             "lrt_add_sdf_frame_octree": ([vp, vp, sz, pu32], i32),
             "lrt_add_sdf_svs": ([vp, vp, sz, pu32], i32),
+            "lrt_add_sdf_grid": ([vp, pu32, vp, pu32], i32),
+            "lrt_load_sdf_grid": ([cp, pu32, ctypes.POINTER(_Buffer)], i32),
+            "lrt_save_sdf_grid": ([cp, pu32, vp], i32),
             "lrt_load_sdf_svs": ([cp, ctypes.POINTER(_Buffer)], i32),
             "lrt_save_sdf_svs": ([cp, vp, sz], i32),
             "lrt_add_sdf_sbs": ([vp, vp, vp, sz, vp, sz, vp, sz, pu32], i32),
@@ -230,7 +234,8 @@ def lib() -> ctypes.CDLL:
 
 def exported_symbols():
     return ["lrt_last_error", "lrt_version", "lrt_create", "lrt_destroy", "lrt_clear_geom", "lrt_clear_scene",
-            "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_svs", "lrt_load_sdf_svs", "lrt_save_sdf_svs", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
+            "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_svs", "lrt_load_sdf_svs", "lrt_save_sdf_svs",
+            "lrt_add_sdf_grid", "lrt_load_sdf_grid", "lrt_save_sdf_grid", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
             "lrt_trace_closest_host", "lrt_trace_closest_noalloc_dev", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
             "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_set_ray_interleave", "lrt_set_persistent",
@@ -329,6 +334,18 @@ class SceneRT:
         nodes = np.ascontiguousarray(nodes, dtype=FRAME_NODE_DTYPE)
         out = ctypes.c_uint32()
         _check(lib().lrt_add_sdf_frame_octree(self._h, _ptr(nodes), nodes.size, ctypes.byref(out)))
+        return out.value
+
+    def AddGeom_SdfGrid(self, data: np.ndarray) -> int:
+        """BVHRT::AddGeom_SdfGrid (BVH2Common_host.cpp:466-499): data[z, y, x] distances over the
+        unit cube (a 3-D array; x fastest in memory)."""
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        if data.ndim != 3:
+            raise ValueError("grid data must be a 3-D array indexed [z, y, x]")
+        size = np.array([data.shape[2], data.shape[1], data.shape[0]], np.uint32)
+        out = ctypes.c_uint32()
+        _check(lib().lrt_add_sdf_grid(self._h, size.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _ptr(data),
+                                      ctypes.byref(out)))
         return out.value
 
     def AddGeom_SdfSVS(self, nodes: np.ndarray) -> int:
@@ -700,6 +717,21 @@ def load_sdf_frame_octree(path: str) -> np.ndarray:
 def save_sdf_frame_octree(path: str, nodes: np.ndarray):
     n = np.ascontiguousarray(nodes, dtype=FRAME_NODE_DTYPE)
     _check(lib().lrt_save_sdf_frame_octree(os.fsencode(path), _ptr(n), n.size))
+
+
+def load_sdf_grid(path: str) -> np.ndarray:
+    """load_sdf_grid (sdf/simple/sdf_simple.cpp:29-36): distances as a [z, y, x] array."""
+    size = np.zeros(3, np.uint32)
+    b = _Buffer()
+    _check(lib().lrt_load_sdf_grid(os.fsencode(path), size.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                   ctypes.byref(b)))
+    return _take(b, np.float32).reshape(int(size[2]), int(size[1]), int(size[0]))
+
+
+def save_sdf_grid(path: str, data: np.ndarray):
+    d = np.ascontiguousarray(data, dtype=np.float32)
+    size = np.array([d.shape[2], d.shape[1], d.shape[0]], np.uint32)
+    _check(lib().lrt_save_sdf_grid(os.fsencode(path), size.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _ptr(d)))
 
 
 def load_sdf_svs(path: str) -> np.ndarray:

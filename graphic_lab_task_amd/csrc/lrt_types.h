@@ -148,6 +148,7 @@ constexpr uint32_t ESCAPE_ROOT = 0xFFFFFFFEu;
 
 // ---- geometry types and tags (core/render_settings.h:3-42, BVH2Common.h:30-43) ----
 constexpr uint32_t TYPE_MESH_TRIANGLE    = 0;
+constexpr uint32_t TYPE_SDF_GRID         = 1;
 constexpr uint32_t TYPE_SDF_FRAME_OCTREE = 3;
 constexpr uint32_t TYPE_SDF_SVS          = 5;
 constexpr uint32_t TYPE_SDF_SBS          = 6;
@@ -164,6 +165,7 @@ constexpr uint32_t TAG_NONE       = 0;
 constexpr uint32_t TAG_TRIANGLE   = 1;
 constexpr uint32_t TAG_SDF_BRICK  = 4;
 constexpr uint32_t TAG_SDF_NODE   = 8;
+constexpr uint32_t TAG_SDF_GRID   = 11;
 constexpr uint32_t TAG_SDF_OCTREE = 15;
 
 // SBS node layouts (sdf/simple/sdf_simple_shared.h:13-21)

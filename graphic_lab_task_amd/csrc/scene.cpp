@@ -145,6 +145,7 @@ void Scene::clear_geom() {
   frameNodes.clear(); frameRoots.clear(); svsNodes.clear(); svsRoots.clear();
   sbsNodes.clear(); sbsData.clear(); sbsDataF.clear(); sbsRoots.clear(); sbsHeaders.clear(); sbsFSlice.clear(); sbsDiffChecked.clear();
   multiNodes.clear(); multiValues.clear();
+  gridData.clear(); gridOffsets.clear(); gridSizes.clear();
   geom_dirty = true;
   clear_scene();
 }
@@ -261,6 +262,30 @@ uint32_t Scene::add_sdf_svs(const SdfSVSNode* nodes, size_t n) {
   svsNodes.insert(svsNodes.end(), nodes, nodes + n);
   origNodes.insert(origNodes.end(), boxes.begin(), boxes.end());
   return add_aabb(TAG_SDF_NODE, boxes.data(), boxes.size());
+}
+
+// BVHRT::AddGeom_SdfGrid (BVH2Common_host.cpp:466-499): a dense grid of distances over the unit
+// cube, sphere-traced as a whole; the BLAS holds the cube's two halves (GetBoxes_SdfGrid,
+// :1337-1346).
+uint32_t Scene::add_sdf_grid(const uint32_t size[3], const float* data) {
+  const uint64_t n = uint64_t(size[0]) * size[1] * size[2];
+  if (n == 0 || n >= (1ull << 28)) throw std::invalid_argument("SDF grid size out of range");
+  GeomData g{};
+  g.boxMin = make4(-1, -1, -1, 1);
+  g.boxMax = make4(1, 1, 1, 1);
+  g.offset = uint2{uint32_t(gridOffsets.size()), 0};
+  g.bvhOffset = uint32_t(nodePairs.size());
+  g.type = TYPE_SDF_GRID;
+  geomData.push_back(g);
+  gridOffsets.push_back(uint32_t(gridData.size()));
+  gridSizes.insert(gridSizes.end(), size, size + 3);
+  gridData.insert(gridData.end(), data, data + n);
+  BVHNode boxes[2]{};
+  boxes[0].boxMin = make3(-1, -1, -1);
+  boxes[0].boxMax = make3(1, 1, 0);
+  boxes[1].boxMin = make3(-1, -1, 0);
+  boxes[1].boxMax = make3(1, 1, 1);
+  return add_aabb(TAG_SDF_GRID, boxes, 2);
 }
 
 // BVHRT::AddGeom_SdfMultiOctree (BVH2Common_host.cpp:970-1018): the octree is walked by its
@@ -454,6 +479,7 @@ uint32_t Scene::leaf_kinds() const {
     else if (t == TAG_SDF_NODE) k |= LK_FRAME;
     else if (t == TAG_SDF_BRICK) k |= LK_SBS;
     else if (t == TAG_SDF_OCTREE) k |= LK_MULTI;
+    else if (t == TAG_SDF_GRID) k |= LK_GRID;
   }
   return k;
 }
@@ -697,6 +723,7 @@ hipError_t Scene::commit() {
   UP("frame_roots", frameRoots) UP("sbs_nodes", sbsNodes) UP("sbs_data", sbsData) UP("sbs_data_f", sbsDataF)
   UP("sbs_roots", sbsRoots) UP("sbs_headers", sbsHeaders) UP("multi_nodes", multiNodes) UP("multi_values", multiValues)
   UP("sbs_mask_base", sbsMaskBase) UP("orig_indices", origIndices) UP("tlas_parent", tlasParent)
+  UP("grid_data", gridData) UP("grid_offsets", gridOffsets) UP("grid_sizes", gridSizes)
   // MultiRenderer's own m_instanceTransformInvTransposed (MultiRenderer_host.cpp:870-878):
   // transpose(inverse4x4(matrix)), refreshed by UpdateInstance (unlike the BVHRT copy)
   std::vector<float4x4> inst_inv_t(instances.size());
@@ -757,6 +784,9 @@ SceneView Scene::make_view() const {
   v.sbs_mask_base = (const uint32_t*)P("sbs_mask_base");
   v.multi_nodes = (const SdfMultiOctreeNode*)P("multi_nodes");
   v.multi_values = (const float*)P("multi_values");
+  v.grid_data = (const float*)P("grid_data");
+  v.grid_offsets = (const uint32_t*)P("grid_offsets");
+  v.grid_sizes = (const uint32_t*)P("grid_sizes");
   v.preset = preset;
   v.lod_thr = lod_thr;
   v.leaf_kinds = leaf_kinds();

@@ -18,6 +18,7 @@ enum LeafKind : uint32_t {
   // brick set has neighbour indices (ID32F_IRGB32F_IN): selects the kernels that compile the
   // smoothing in (the others stay free of its registers).
   LK_SBS_SMOOTH = 1u << 5,
+  LK_GRID  = 1u << 6,   // TAG_SDF_GRID (dense SDF grid, sphere tracing)
 };
 
 // Frame-octree leaf as the kernels read it: the leaf box, its 8 corner distances and the
@@ -69,6 +70,10 @@ struct SceneView {
   // multi octree
   const SdfMultiOctreeNode* multi_nodes;  // m_SdfMultiOctreeNodes
   const float*              multi_values; // m_SdfMultiOctreeValues
+  // dense SDF grids
+  const float*              grid_data;    // m_SdfGridData
+  const uint32_t*           grid_offsets; // m_SdfGridOffsets
+  const uint32_t*           grid_sizes;   // m_SdfGridSizes, 3 words per grid
 
   MultiRenderPreset preset;
   float    lod_thr;                 // m_precomputedLoDThr

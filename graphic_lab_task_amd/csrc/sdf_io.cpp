@@ -127,6 +127,31 @@ lrt_status lrt_save_sdf_svs(const char* path, const void* nodes, size_t n_nodes)
   });
 }
 
+lrt_status lrt_load_sdf_grid(const char* path, uint32_t size[3], lrt_buffer* data) {
+  return io_guarded([&]() -> lrt_status {
+    if (!size || !data) throw std::runtime_error("null output");
+    File fl(path, "rb");
+    size_t left = size_t(fl.size());
+    take(fl, size, 12, left, "grid size");
+    const uint64_t n = uint64_t(size[0]) * size[1] * size[2];
+    if (n * 4 > left) throw std::runtime_error("grid of " + std::to_string(n) + " values exceeds the file");
+    std::vector<char> v(size_t(n) * 4);
+    take(fl, v.data(), v.size(), left, "grid values");
+    return to_buf(v, data);
+  });
+}
+
+lrt_status lrt_save_sdf_grid(const char* path, const uint32_t size[3], const float* data) {
+  return io_guarded([&]() -> lrt_status {
+    if (!size || !data) throw std::runtime_error("null argument");
+    const uint64_t n = uint64_t(size[0]) * size[1] * size[2];
+    File fl(path, "wb");
+    put(fl, size, 12);
+    put(fl, data, size_t(n) * 4);
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_load_sdf_sbs(const char* path, void* header, lrt_buffer* nodes, lrt_buffer* values,
                             lrt_buffer* values_f) {
   return io_guarded([&]() -> lrt_status {

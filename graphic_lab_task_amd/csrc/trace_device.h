@@ -533,6 +533,92 @@ __device__ __forceinline__ bool sbs_voxel_occupied(const SceneView& s, const Sdf
 // The voxel whose surface produced a brick-set hit: the value indices of its 8 corners (into
 // the scene's values_f, indexed layouts), the entry point in voxel units, and the hit
 // parameter along the ray direction from there.
+// BVHRT::eval_distance_sdf_grid (BVH2Common.cpp:1442-1478), trilinear: the grid spans the unit
+// cube; outside the last cell the nearest sample is returned.
+__device__ __forceinline__ float sdf_grid_eval(const SceneView& s, uint32_t grid_id, float3 pos) {
+  const uint32_t off = s.grid_offsets[grid_id];
+  const uint32_t sx = s.grid_sizes[3 * grid_id], sy = s.grid_sizes[3 * grid_id + 1], sz = s.grid_sizes[3 * grid_id + 2];
+  const float3 gsf = make3(float(sx - 1), float(sy - 1), float(sz - 1));
+  float3 vox_f = gsf * ((pos - make3(-1, -1, -1)) / make3(2, 2, 2));
+  vox_f = min3(max3(vox_f, make3(0.0f, 0.0f, 0.0f)), gsf - make3(1e-5f, 1e-5f, 1e-5f));
+  const uint32_t ux = uint32_t(vox_f.x), uy = uint32_t(vox_f.y), uz = uint32_t(vox_f.z);
+  const float3 dp = vox_f - make3(float(ux), float(uy), float(uz));
+  const float* data = s.grid_data + off;
+  float res = 0.0f;
+  if (ux < sx - 1 && uy < sy - 1 && uz < sz - 1) {
+    for (uint32_t i = 0; i < 2; i++)
+      for (uint32_t j = 0; j < 2; j++)
+        for (uint32_t k = 0; k < 2; k++) {
+          const float qx = (1 - dp.x + float(i) * (2 * dp.x - 1));
+          const float qy = (1 - dp.y + float(j) * (2 * dp.y - 1));
+          const float qz = (1 - dp.z + float(k) * (2 * dp.z - 1));
+          res += qx * qy * qz * data[(uz + k) * sx * sy + (uy + j) * sx + (ux + i)];
+        }
+  } else {
+    res += data[uz * sx * sy + uy * sx + ux];
+  }
+  return res;
+}
+
+// BVHRT::box_intersects (BVH2Common.cpp:34-45)
+__device__ __forceinline__ float2 box_intersects_ref(float3 min_pos, float3 max_pos, float3 origin, float3 dir) {
+  const float3 safe_dir = make3(sign_ref(dir.x) * fmax_ref(1e-9f, fabs_ref(dir.x)),
+                                sign_ref(dir.y) * fmax_ref(1e-9f, fabs_ref(dir.y)),
+                                sign_ref(dir.z) * fmax_ref(1e-9f, fabs_ref(dir.z)));
+  const float3 tMin = (min_pos - origin) / safe_dir;
+  const float3 tMax = (max_pos - origin) / safe_dir;
+  const float3 t1 = min3(tMin, tMax), t2 = max3(tMin, tMax);
+  float2 r;
+  r.x = fmax_ref(t1.x, fmax_ref(t1.y, t1.z));
+  r.y = fmin_ref(t2.x, fmin_ref(t2.y, t2.z));
+  return r;
+}
+
+// BVHRT::IntersectSDFGrid (BVH2Common.cpp:1114-1182): sphere tracing along the normalised
+// direction from the grid box's entry (at most 1000 steps, EPS 1e-5), central-difference normal.
+__device__ __forceinline__ void sdf_grid_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir,
+                                                   float tNear, uint32_t instId, uint32_t geomId, CRT_Hit& hit) {
+  const uint32_t sdfId = g.offset.x;
+  const float dir_len = length3(ray_dir);
+  const float3 dir = make3(ray_dir.x / dir_len, ray_dir.y / dir_len, ray_dir.z / dir_len);
+  const float EPS = 1e-5f;
+  const float2 tt = box_intersects_ref(make3(-1, -1, -1), make3(1, 1, 1), ray_pos, dir);
+  float t = fmax_ref(tNear, tt.x);
+  const float tFar = tt.y;
+  if (t > tFar) return;
+  uint32_t iter = 0;
+  float d = sdf_grid_eval(s, sdfId, ray_pos + t * dir);
+  while (iter < 1000 && d > EPS && t < tFar) {
+    t += d + EPS;
+    d = sdf_grid_eval(s, sdfId, ray_pos + t * dir);
+    iter++;
+  }
+  if (d > EPS) return;
+  const float3 p0 = ray_pos + t * dir;
+  if (t > tNear && t < hit.t) {
+    float3 norm = make3(0, 0, 1);
+    if (preset_needs_normal(s.preset.render_mode)) {
+      const float h = 0.001f;
+      const float ddx = (sdf_grid_eval(s, sdfId, p0 + make3(h, 0, 0)) - sdf_grid_eval(s, sdfId, p0 + make3(-h, 0, 0))) / (2 * h);
+      const float ddy = (sdf_grid_eval(s, sdfId, p0 + make3(0, h, 0)) - sdf_grid_eval(s, sdfId, p0 + make3(0, -h, 0))) / (2 * h);
+      const float ddz = (sdf_grid_eval(s, sdfId, p0 + make3(0, 0, h)) - sdf_grid_eval(s, sdfId, p0 + make3(0, 0, -h))) / (2 * h);
+      norm = normalize3(make3(ddx, ddy, ddz));
+    }
+    float m[16];
+    load_mat(s.instances[instId].transformInvTransposed, m);
+    const float2 e = encode_normal(normalize3(matmul4x3(m, norm)));
+    hit.t = t;
+    hit.primId = 0;
+    hit.instId = instId;
+    hit.geomId = geomId | (TYPE_SDF_GRID << SH_TYPE);
+    hit.coords[0] = 0;
+    hit.coords[1] = 0;
+    hit.coords[2] = e.x;
+    hit.coords[3] = e.y;
+    if (s.preset.render_mode == MULTI_RENDER_MODE_ST_ITERATIONS) hit.primId = iter;
+  }
+}
+
 // eval_dist_trilinear_diff (BVH2Common.cpp:751-781): gradient of the trilinear of 8 values.
 __device__ __forceinline__ float3 eval_dist_trilinear_diff(const float v[8], float3 dp) {
   const float ddx = -(1 - dp.y) * (1 - dp.z) * v[0] + -(1 - dp.y) * (dp.z) * v[1] + -(dp.y) * (1 - dp.z) * v[2] +

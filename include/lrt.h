@@ -77,6 +77,10 @@ lrt_status lrt_add_triangles(lrt_scene* s, const float* vpos, size_t n_verts, si
  * nodes: SdfFrameOctreeNode[n_nodes] (36 B each, sdf/simple/sdf_simple_shared.h:23-27). */
 lrt_status lrt_add_sdf_frame_octree(lrt_scene* s, const void* nodes, size_t n_nodes, uint32_t* out_geom_id);
 
+/* Replaces BVHRT::AddGeom_SdfGrid (BVH2Common_host.cpp:466-499): a dense grid of distances over
+ * the unit cube, size[0] * size[1] * size[2] floats with x fastest (fewer than 2^28). */
+lrt_status lrt_add_sdf_grid(lrt_scene* s, const uint32_t size[3], const float* data, uint32_t* out_geom_id);
+
 /* Replaces BVHRT::AddGeom_SdfSVS (BVH2Common_host.cpp:536-597): a sparse voxel set, every voxel
  * a BLAS leaf. nodes: SdfSVSNode[n_nodes] (16 B each, sdf/simple/sdf_simple_shared.h:30-35:
  * x << 16 | y, z << 16 | level size, 8 one-byte distance values). */
@@ -130,7 +134,8 @@ typedef struct lrt_commit_info {
 lrt_status lrt_get_commit_info(lrt_scene* s, lrt_commit_info* out);
 
 /* Replaces BVHRT::SetPreset (BVH2Common_host.cpp:1437-1454). preset: MultiRenderPreset
- * (core/render_settings.h:116-136, 60 B). Takes effect on the next trace call. */
+ * (core/render_settings.h:116-136, 60 B). Takes effect on the next trace call. Only trilinear
+ * interpolation is built: another interpolation_mode fails with LRT_ERR_UNSUPPORTED. */
 lrt_status lrt_set_preset(lrt_scene* s, const void* preset);
 
 /* Batched RayQuery_NearestHit (BVH2Common.cpp:3322-3392) over device buffers:
@@ -366,6 +371,7 @@ lrt_status lrt_stack_overflow_count(lrt_scene* s, uint64_t* out, int reset);
  * "frame_roots", "sbs_nodes", "sbs_data", "sbs_data_f", "sbs_roots", "sbs_headers",
  * "multi_nodes", "multi_values", "orig_indices" (uint32: triangle indices in the caller's order),
  * "svs_nodes" (SdfSVSNode), "svs_roots", "tlas_parent" (uint32: parent node of each TLAS node),
+ * "grid_data" (float), "grid_offsets", "grid_sizes" (uint32, 3 per grid),
  * "preset", and "sbs_mask" (uint32: the committed voxel occupancy
  * masks, per brick set in order and per brick one summary word then ceil(bs^3/32) bit words,
  * bit v set when the minimum of voxel v's 8 corner values is <= 0; copied from the device).

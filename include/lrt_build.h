@@ -77,6 +77,11 @@ lrt_status lrt_sdf_spheres_sbs_indexed(const float* spheres, size_t n_spheres, u
 lrt_status lrt_load_sdf_frame_octree(const char* path, lrt_buffer* nodes);
 lrt_status lrt_save_sdf_frame_octree(const char* path, const void* nodes, size_t n_nodes);
 
+/* load_sdf_grid / save_sdf_grid (sdf/simple/sdf_simple.cpp:20-36): 3 x u32 size, then
+ * size.x * size.y * size.z floats (x fastest). */
+lrt_status lrt_load_sdf_grid(const char* path, uint32_t size[3], lrt_buffer* data);
+lrt_status lrt_save_sdf_grid(const char* path, const uint32_t size[3], const float* data);
+
 /* load_sdf_SVS / save_sdf_SVS (sdf/simple/sdf_simple.cpp:69-94): u32 count, then count SdfSVSNode. */
 lrt_status lrt_load_sdf_svs(const char* path, lrt_buffer* nodes);
 lrt_status lrt_save_sdf_svs(const char* path, const void* nodes, size_t n_nodes);

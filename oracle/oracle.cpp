@@ -111,6 +111,7 @@ static_assert(sizeof(Geom) == 64 && sizeof(Inst) == 256 && sizeof(FrameNode) == 
 constexpr uint32_t LEAF_BIT = 0x80000000u, START_MASK = 0x0FFFFFFFu, SIZE_MASK = 0x70000000u;
 constexpr int STACK_SIZE = 80;
 constexpr uint32_t TAG_NONE = 0, TAG_TRIANGLE = 1, TAG_SDF_BRICK = 4, TAG_SDF_NODE = 8, TAG_SDF_OCTREE = 15;
+constexpr uint32_t TAG_SDF_GRID = 11, TYPE_SDF_GRID = 1;
 constexpr uint32_t TYPE_SDF_SVS = 5;
 constexpr uint32_t TYPE_MESH_TRIANGLE = 0, TYPE_SDF_FRAME_OCTREE = 3, SH_TYPE = 24, GEOM_ID_MASK = 0x00FFFFFFu;
 constexpr uint32_t LAYOUT_DX_UV16 = 2u << 24, LAYOUT_DX_RGB8 = 3u << 24, LAYOUT_ID32F = 4u << 24, LAYOUT_ID32F_IN = 5u << 24;
@@ -144,6 +145,9 @@ struct OracleScene {
   const float* multi_values;
   const SVSNode* svs_nodes;
   const uint32_t* svs_roots;
+  const float* grid_data;
+  const uint32_t* grid_offsets;
+  const uint32_t* grid_sizes;     // 3 per grid
   Preset preset;
 };
 
@@ -477,6 +481,85 @@ struct Tracer {
     return vmin;
   }
 
+  // BVHRT::eval_distance_sdf_grid (BVH2Common.cpp:1442-1478), trilinear
+  float eval_distance_sdf_grid(uint32_t grid_id, f3 pos) const {
+    const uint32_t off = s.grid_offsets[grid_id];
+    const uint32_t sx = s.grid_sizes[3 * grid_id], sy = s.grid_sizes[3 * grid_id + 1], sz = s.grid_sizes[3 * grid_id + 2];
+    const f3 grid_size_f(float(sx - 1), float(sy - 1), float(sz - 1));
+    f3 vox_f = grid_size_f * ((pos - f3(-1, -1, -1)) / f3(2, 2, 2));
+    vox_f = f3(std::min(std::max(vox_f.x, 0.0f), grid_size_f.x - 1e-5f), std::min(std::max(vox_f.y, 0.0f), grid_size_f.y - 1e-5f),
+               std::min(std::max(vox_f.z, 0.0f), grid_size_f.z - 1e-5f));
+    const uint32_t vx = uint32_t(vox_f.x), vy = uint32_t(vox_f.y), vz = uint32_t(vox_f.z);
+    const f3 dp = vox_f - f3(float(vx), float(vy), float(vz));
+    float res = 0.0f;
+    if (vx < sx - 1 && vy < sy - 1 && vz < sz - 1) {
+      for (uint32_t i = 0; i < 2; i++)
+        for (uint32_t j = 0; j < 2; j++)
+          for (uint32_t k = 0; k < 2; k++) {
+            float qx = (1 - dp.x + i * (2 * dp.x - 1));
+            float qy = (1 - dp.y + j * (2 * dp.y - 1));
+            float qz = (1 - dp.z + k * (2 * dp.z - 1));
+            res += qx * qy * qz * s.grid_data[off + (vz + k) * sx * sy + (vy + j) * sx + (vx + i)];
+          }
+    } else {
+      res += s.grid_data[off + vz * sx * sy + vy * sx + vx];
+    }
+    return res;
+  }
+
+  // BVHRT::box_intersects (BVH2Common.cpp:34-45)
+  static f2 box_intersects(f3 min_pos, f3 max_pos, f3 origin, f3 dir) {
+    f3 safe_dir(lm_sign(dir.x) * std::max(1e-9f, std::abs(dir.x)), lm_sign(dir.y) * std::max(1e-9f, std::abs(dir.y)),
+                lm_sign(dir.z) * std::max(1e-9f, std::abs(dir.z)));
+    f3 tMin = (min_pos - origin) / safe_dir;
+    f3 tMax = (max_pos - origin) / safe_dir;
+    f3 t1(std::min(tMin.x, tMax.x), std::min(tMin.y, tMax.y), std::min(tMin.z, tMax.z));
+    f3 t2(std::max(tMin.x, tMax.x), std::max(tMin.y, tMax.y), std::max(tMin.z, tMax.z));
+    return f2{std::max(t1.x, std::max(t1.y, t1.z)), std::min(t2.x, std::min(t2.y, t2.z))};
+  }
+
+  // BVHRT::IntersectSDFGrid (BVH2Common.cpp:1114-1182)
+  void IntersectSDFGrid(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId, Hit* pHit) {
+    const uint32_t sdfId = s.geoms[geomId].offset[0];
+    const float dir_len = length(ray_dir);
+    const f3 dir = ray_dir / dir_len;
+    const float EPS = 1e-5f;
+    f2 tt = box_intersects(f3(-1, -1, -1), f3(1, 1, 1), ray_pos, dir);
+    float t = std::max(tNear, tt.x);
+    float tFar = tt.y;
+    if (t > tFar) return;
+    uint32_t iter = 0;
+    float d = eval_distance_sdf_grid(sdfId, ray_pos + t * dir);
+    while (iter < 1000 && d > EPS && t < tFar) {
+      t += d + EPS;
+      d = eval_distance_sdf_grid(sdfId, ray_pos + t * dir);
+      iter++;
+    }
+    if (d > EPS) return;
+    f3 p0 = ray_pos + t * dir;
+    if (t > tNear && t < pHit->t) {
+      f3 norm(0, 0, 1);
+      if (need_normal()) {
+        const float h = 0.001f;
+        float ddx = (eval_distance_sdf_grid(sdfId, p0 + f3(h, 0, 0)) - eval_distance_sdf_grid(sdfId, p0 + f3(-h, 0, 0))) / (2 * h);
+        float ddy = (eval_distance_sdf_grid(sdfId, p0 + f3(0, h, 0)) - eval_distance_sdf_grid(sdfId, p0 + f3(0, -h, 0))) / (2 * h);
+        float ddz = (eval_distance_sdf_grid(sdfId, p0 + f3(0, 0, h)) - eval_distance_sdf_grid(sdfId, p0 + f3(0, 0, -h))) / (2 * h);
+        norm = normalize(f3(ddx, ddy, ddz));
+      }
+      f3 n = normalize(matmul4x3(s.instances[instId].transformInvTransposed, norm));
+      f2 e = encode_normal(n);
+      pHit->t = t;
+      pHit->primId = 0;
+      pHit->instId = instId;
+      pHit->geomId = geomId | (TYPE_SDF_GRID << SH_TYPE);
+      pHit->coords[0] = 0;
+      pHit->coords[1] = 0;
+      pHit->coords[2] = e.x;
+      pHit->coords[3] = e.y;
+      if (s.preset.render_mode == 10) pHit->primId = iter;   // MULTI_RENDER_MODE_ST_ITERATIONS
+    }
+  }
+
   // eval_dist_trilinear_diff (BVH2Common.cpp:751-781)
   static f3 eval_dist_trilinear_diff(const float v[8], f3 dp) {
     float ddist_dx = -(1 - dp.y) * (1 - dp.z) * v[0] + -(1 - dp.y) * (dp.z) * v[1] + -(dp.y) * (1 - dp.z) * v[2] +
@@ -762,6 +845,7 @@ struct Tracer {
       case TAG_SDF_NODE: OctreeNodeIntersect(s.geoms[geomId].type, ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
       case TAG_SDF_BRICK: OctreeBrickIntersect(s.geoms[geomId].type, ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
       case TAG_SDF_OCTREE: OctreeIntersect(ray_pos, ray_dir, tNear, instId, geomId, pHit); break;
+      case TAG_SDF_GRID: IntersectSDFGrid(ray_pos, ray_dir, tNear, instId, geomId, pHit); break;
       default: break;
     }
     if (s.preset.render_mode == 26) pHit->instId = (pHit->instId == 0xFFFFFFFF) ? 1 : pHit->instId + 1;
