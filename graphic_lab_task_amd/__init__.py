@@ -37,6 +37,10 @@ INSTANCE_DTYPE = np.dtype([("boxMin", "<f4", (4,)), ("boxMax", "<f4", (4,)), ("g
 FRAME_NODE_DTYPE = np.dtype([("values", "<f4", (8,)), ("offset", "<u4")])         # SdfFrameOctreeNode, 36 B
 MULTI_NODE_DTYPE = np.dtype([("data_offset", "<u4"), ("children_offset", "<u4"), ("voxel_count", "<u4"),
                              ("flags", "<u4")])                                       # SdfMultiOctreeNode, 16 B
+GPRIM_HEADER_DTYPE = np.dtype([("color", "<f4", (3,)), ("prim_type", "<u4")])   # GraphicsPrimHeader, 16 B
+(GRAPH_PRIM_POINT, GRAPH_PRIM_LINE, GRAPH_PRIM_LINE_SEGMENT, GRAPH_PRIM_LINE_SEGMENT_DIR, GRAPH_PRIM_BOX,
+ GRAPH_PRIM_POINT_COLOR, GRAPH_PRIM_LINE_COLOR, GRAPH_PRIM_LINE_SEGMENT_COLOR, GRAPH_PRIM_LINE_SEGMENT_DIR_COLOR,
+ GRAPH_PRIM_BOX_COLOR) = range(10)
 SVS_NODE_DTYPE = np.dtype([("pos_xy", "<u4"), ("pos_z_lod_size", "<u4"), ("values", "<u4", (2,))])  # SdfSVSNode, 16 B
 SBS_NODE_DTYPE = np.dtype([("pos_xy", "<u4"), ("pos_z_lod_size", "<u4"), ("data_offset", "<u4"),
                            ("_pad", "<u4")])                                       # SdfSBSNode, 16 B
@@ -58,7 +62,7 @@ ARRAY_DTYPES = {
     "sbs_headers": SBS_HEADER_DTYPE, "multi_nodes": MULTI_NODE_DTYPE, "multi_values": np.float32,
     "preset": PRESET_DTYPE, "sbs_mask": np.uint32, "orig_indices": np.uint32, "svs_nodes": SVS_NODE_DTYPE,
     "svs_roots": np.uint32, "tlas_parent": np.uint32, "grid_data": np.float32, "grid_offsets": np.uint32,
-    "grid_sizes": np.uint32,
+    "grid_sizes": np.uint32, "gprim_headers": GPRIM_HEADER_DTYPE, "gprim_points": np.float32,
 }
 LIGHT_DTYPE = np.dtype([("space", "<f4", (3,)), ("type", "<u4"), ("color", "<f4", (3,)), ("_pad", "<u4")])  # Light, 32 B
 LIGHT_TYPE_DIRECT, LIGHT_TYPE_POINT, LIGHT_TYPE_AMBIENT = 0, 1, 2
@@ -168,6 +172,7 @@ def lib() -> ctypes.CDLL:
             "lrt_add_sdf_frame_octree": ([vp, vp, sz, pu32], i32),
             "lrt_add_sdf_svs": ([vp, vp, sz, pu32], i32),
             "lrt_add_sdf_grid": ([vp, pu32, vp, pu32], i32),
+            "lrt_add_graphics_prims": ([vp, vp, vp, sz, pu32], i32),
             "lrt_load_sdf_grid": ([cp, pu32, ctypes.POINTER(_Buffer)], i32),
             "lrt_save_sdf_grid": ([cp, pu32, vp], i32),
             "lrt_load_sdf_svs": ([cp, ctypes.POINTER(_Buffer)], i32),
@@ -235,7 +240,7 @@ def lib() -> ctypes.CDLL:
 def exported_symbols():
     return ["lrt_last_error", "lrt_version", "lrt_create", "lrt_destroy", "lrt_clear_geom", "lrt_clear_scene",
             "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_svs", "lrt_load_sdf_svs", "lrt_save_sdf_svs",
-            "lrt_add_sdf_grid", "lrt_load_sdf_grid", "lrt_save_sdf_grid", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
+            "lrt_add_sdf_grid", "lrt_load_sdf_grid", "lrt_save_sdf_grid", "lrt_add_graphics_prims", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
             "lrt_trace_closest_host", "lrt_trace_closest_noalloc_dev", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
             "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_set_ray_interleave", "lrt_set_persistent",
@@ -334,6 +339,16 @@ class SceneRT:
         nodes = np.ascontiguousarray(nodes, dtype=FRAME_NODE_DTYPE)
         out = ctypes.c_uint32()
         _check(lib().lrt_add_sdf_frame_octree(self._h, _ptr(nodes), nodes.size, ctypes.byref(out)))
+        return out.value
+
+    def AddGeom_GraphicsPrim(self, prim_type: int, points: np.ndarray, color=(1.0, 1.0, 1.0)) -> int:
+        """BVHRT::AddGeom_GraphicsPrim (BVH2Common_host.cpp:767-903): points (n, 4) float32 as the
+        primitive type reads them; color is the header colour of the types without their own."""
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 4)
+        h = np.zeros((), GPRIM_HEADER_DTYPE)
+        h["color"], h["prim_type"] = color, prim_type
+        out = ctypes.c_uint32()
+        _check(lib().lrt_add_graphics_prims(self._h, _ptr(h), _ptr(pts), pts.shape[0], ctypes.byref(out)))
         return out.value
 
     def AddGeom_SdfGrid(self, data: np.ndarray) -> int:

@@ -270,6 +270,17 @@ lrt_status lrt_add_sdf_frame_octree(lrt_scene* s, const void* nodes, size_t n_no
   });
 }
 
+lrt_status lrt_add_graphics_prims(lrt_scene* s, const void* header, const float* points, size_t n_points,
+                                  uint32_t* out_geom_id) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !header || !points) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    const uint32_t id = s->s.add_graphics_prims(*static_cast<const lrt::GraphicsPrimHeader*>(header),
+                                                reinterpret_cast<const lrt::float4*>(points), n_points);
+    if (out_geom_id) *out_geom_id = id;
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_add_sdf_grid(lrt_scene* s, const uint32_t size[3], const float* data, uint32_t* out_geom_id) {
   return guarded([&]() -> lrt_status {
     if (!s || !size || !data) return fail(LRT_ERR_INVALID_ARG, "null argument");
@@ -999,6 +1010,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
     ARR("multi_nodes", c.multiNodes) ARR("multi_values", c.multiValues) ARR("orig_indices", c.origIndices)
     ARR("svs_nodes", c.svsNodes) ARR("svs_roots", c.svsRoots) ARR("tlas_parent", c.tlasParent)
     ARR("grid_data", c.gridData) ARR("grid_offsets", c.gridOffsets) ARR("grid_sizes", c.gridSizes)
+    ARR("gprim_headers", c.gprimHeaders) ARR("gprim_points", c.gprimPoints)
 #undef ARR
     if (n == "preset") { src = &c.preset; sz = sizeof(c.preset); }
     std::vector<uint32_t> mask_copy;   // the committed occupancy masks live on the device only
@@ -1018,7 +1030,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
       static const char* known[] = {"nodes", "tlas", "instances", "geoms", "geom_tags", "start_end",
                                     "prim_id_count", "vert_pos", "vert_norm", "indices", "prim_indices",
                                     "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes", "sbs_data",
-                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices", "svs_nodes", "svs_roots", "tlas_parent", "grid_data", "grid_offsets", "grid_sizes",
+                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices", "svs_nodes", "svs_roots", "tlas_parent", "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points",
                                     "sbs_mask"};
       bool ok = false;
       for (const char* k : known) ok = ok || n == k;

@@ -80,6 +80,25 @@ struct SdfSVSNode {
 };
 static_assert(sizeof(SdfSVSNode) == 16, "SdfSVSNode layout");
 
+// GraphicsPrimHeader (utils/misc/graphics_primitive_common.h:19-23) and GraphicsPrimType (:7-16):
+// points carry (centre, radius); lines, segments, arrows and boxes two float4 (the first with
+// the radius); the *_COLOR kinds one more float4 with an RGB colour per primitive.
+struct GraphicsPrimHeader {
+  float    color[3];
+  uint32_t prim_type;
+};
+constexpr uint32_t GRAPH_PRIM_POINT = 0, GRAPH_PRIM_LINE = 1, GRAPH_PRIM_LINE_SEGMENT = 2,
+                   GRAPH_PRIM_LINE_SEGMENT_DIR = 3, GRAPH_PRIM_BOX = 4, GRAPH_PRIM_POINT_COLOR = 5,
+                   GRAPH_PRIM_LINE_COLOR = 6, GRAPH_PRIM_LINE_SEGMENT_COLOR = 7,
+                   GRAPH_PRIM_LINE_SEGMENT_DIR_COLOR = 8, GRAPH_PRIM_BOX_COLOR = 9;
+// float4 records per primitive (IntersectGraphicPrims, BVH2Common.cpp:1236-1240)
+LRT_HD uint32_t graphics_prim_len(uint32_t prim_type) {
+  uint32_t len = 2u;
+  if (prim_type == GRAPH_PRIM_POINT) len = 1u;
+  if (prim_type >= GRAPH_PRIM_LINE_COLOR) len = 3u;
+  return len;
+}
+
 // sdf/simple/sdf_simple_shared.h:39-45
 struct SdfSBSNode {
   uint32_t pos_xy;          // (x << 16) | y, brick position in its LOD grid
@@ -149,6 +168,7 @@ constexpr uint32_t ESCAPE_ROOT = 0xFFFFFFFEu;
 // ---- geometry types and tags (core/render_settings.h:3-42, BVH2Common.h:30-43) ----
 constexpr uint32_t TYPE_MESH_TRIANGLE    = 0;
 constexpr uint32_t TYPE_SDF_GRID         = 1;
+constexpr uint32_t TYPE_GRAPHICS_PRIM    = 17;
 constexpr uint32_t TYPE_SDF_FRAME_OCTREE = 3;
 constexpr uint32_t TYPE_SDF_SVS          = 5;
 constexpr uint32_t TYPE_SDF_SBS          = 6;
@@ -166,6 +186,7 @@ constexpr uint32_t TAG_TRIANGLE   = 1;
 constexpr uint32_t TAG_SDF_BRICK  = 4;
 constexpr uint32_t TAG_SDF_NODE   = 8;
 constexpr uint32_t TAG_SDF_GRID   = 11;
+constexpr uint32_t TAG_GRAPHICS_PRIM = 9;
 constexpr uint32_t TAG_SDF_OCTREE = 15;
 
 // SBS node layouts (sdf/simple/sdf_simple_shared.h:13-21)

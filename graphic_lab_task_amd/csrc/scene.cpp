@@ -146,6 +146,7 @@ void Scene::clear_geom() {
   sbsNodes.clear(); sbsData.clear(); sbsDataF.clear(); sbsRoots.clear(); sbsHeaders.clear(); sbsFSlice.clear(); sbsDiffChecked.clear();
   multiNodes.clear(); multiValues.clear();
   gridData.clear(); gridOffsets.clear(); gridSizes.clear();
+  gprimHeaders.clear(); gprimPoints.clear();
   geom_dirty = true;
   clear_scene();
 }
@@ -286,6 +287,114 @@ uint32_t Scene::add_sdf_grid(const uint32_t size[3], const float* data) {
   boxes[1].boxMin = make3(-1, -1, 0);
   boxes[1].boxMax = make3(1, 1, 1);
   return add_aabb(TAG_SDF_GRID, boxes, 2);
+}
+
+namespace {
+// getDiskAABB (BVH2Common_host.cpp:754-765): box of a disk of radius `rad` around `pt` facing `norm`
+BVHNode disk_box(float3 pt, float3 norm, float rad) {
+  BVHNode r{};
+  norm = normalize3(norm) * rad;
+  const float xy = std::sqrt(norm.x * norm.x + norm.y * norm.y);
+  const float xz = std::sqrt(norm.x * norm.x + norm.z * norm.z);
+  const float yz = std::sqrt(norm.y * norm.y + norm.z * norm.z);
+  r.boxMin = pt - make3(yz, xz, xy);
+  r.boxMax = pt + make3(yz, xz, xy);
+  return r;
+}
+inline float comp3(const float3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+}  // namespace
+
+// BVHRT::AddGeom_GraphicsPrim (BVH2Common_host.cpp:767-903): one BLAS box per primitive, from the
+// primitive's extent (a line: its part inside the unit cube, as the reference clips it).
+uint32_t Scene::add_graphics_prims(const GraphicsPrimHeader& header, const float4* points, size_t n_points) {
+  if (header.prim_type > GRAPH_PRIM_BOX_COLOR) throw std::invalid_argument("unknown graphics primitive type");
+  const uint32_t step = graphics_prim_len(header.prim_type);
+  if (n_points == 0 || n_points % step != 0) throw std::invalid_argument("graphics primitive points: not whole primitives");
+  if (n_points / step >= (1u << 28)) throw std::invalid_argument("too many graphics primitives");
+  float3 mn = make3(1, 1, 1), mx = make3(-1, -1, -1);
+  std::vector<BVHNode> boxes;
+  const uint32_t pt = header.prim_type;
+  for (size_t i = 0; i < n_points; i += step) {
+    BVHNode nd{};
+    if (pt == GRAPH_PRIM_POINT || pt == GRAPH_PRIM_POINT_COLOR) {
+      const float4 p = points[i];
+      nd.boxMin = xyz(p) - p.w;
+      nd.boxMax = xyz(p) + p.w;
+    } else if (pt == GRAPH_PRIM_LINE || pt == GRAPH_PRIM_LINE_COLOR) {
+      const float4 p1 = points[i], p2 = points[i + 1];
+      const float3 pos = xyz(p2);
+      const float3 dir = normalize3(xyz(p1) - pos);
+      float tx = 0.f, ty = 0.f;
+      bool on_pos = false, on_neg = false;
+      for (int a = 0; a < 3; ++a) {
+        const float da = comp3(dir, a), pa = comp3(pos, a);
+        if (da > 1e-4f) {
+          float dist = 1.f - pa;
+          if (std::abs(dist) < 1e-4f) {
+            on_pos = true;
+          } else {
+            const float t1 = dist / da;
+            if (tx == 0.f || std::abs(t1) < std::abs(tx)) tx = t1;
+          }
+          dist = -1.f - pa;
+          if (std::abs(dist) < 1e-4f) {
+            on_neg = true;
+          } else {
+            const float t2 = dist / da;
+            if (ty == 0.f || std::abs(t2) < std::abs(ty)) ty = t2;
+          }
+        }
+      }
+      const float3 i1 = on_pos ? pos : pos + tx * dir;
+      const float3 i2 = on_neg ? pos : pos + ty * dir;
+      nd = disk_box(i1, dir, p1.w);
+      const BVHNode o = disk_box(i2, dir, p1.w);
+      nd.boxMin = min3(nd.boxMin, o.boxMin);
+      nd.boxMax = max3(nd.boxMax, o.boxMax);
+    } else if (pt == GRAPH_PRIM_LINE_SEGMENT || pt == GRAPH_PRIM_LINE_SEGMENT_COLOR) {
+      const float4 p1 = points[i], p2 = points[i + 1];
+      const float3 dir = normalize3(make3(p2.x - p1.x, p2.y - p1.y, p2.z - p1.z));
+      nd = disk_box(xyz(p2), dir, p1.w);
+      const BVHNode o = disk_box(xyz(p1), dir, p1.w);
+      nd.boxMin = min3(nd.boxMin, o.boxMin);
+      nd.boxMax = max3(nd.boxMax, o.boxMax);
+    } else if (pt == GRAPH_PRIM_LINE_SEGMENT_DIR || pt == GRAPH_PRIM_LINE_SEGMENT_DIR_COLOR) {
+      const float4 p1r = points[i];
+      const float3 p2 = xyz(points[i + 1]), p1 = xyz(p1r);
+      const float3 dir = p2 - p1;
+      const float cone_ra = length3(p2 - p1) * 0.15f;
+      const float3 pa = p2 * 0.7f + p1 * 0.3f;
+      nd = disk_box(p1, dir, p1r.w);
+      nd.boxMin = min3(nd.boxMin, p2);
+      nd.boxMax = max3(nd.boxMax, p2);
+      const BVHNode o = disk_box(pa, dir, cone_ra);
+      nd.boxMin = min3(nd.boxMin, o.boxMin);
+      nd.boxMax = max3(nd.boxMax, o.boxMax);
+    } else {   // boxes
+      const float4 p1r = points[i];
+      const float3 p2 = xyz(points[i + 1]);
+      nd.boxMin = min3(xyz(p1r), p2) - p1r.w;
+      nd.boxMax = max3(xyz(p1r), p2) + p1r.w;
+    }
+    mn = min3(mn, nd.boxMin);
+    mx = max3(mx, nd.boxMax);
+    boxes.push_back(nd);
+  }
+  if (boxes.size() == 1) {
+    boxes.resize(2);
+    boxes[1].boxMin = boxes[0].boxMin - 0.001f;
+    boxes[1].boxMax = boxes[0].boxMin;
+  }
+  GeomData g{};
+  g.boxMin = make4(mn.x, mn.y, mn.z, 1);
+  g.boxMax = make4(mx.x, mx.y, mx.z, 1);
+  g.offset = uint2{uint32_t(gprimHeaders.size()), uint32_t(gprimPoints.size())};
+  g.bvhOffset = uint32_t(nodePairs.size());
+  g.type = TYPE_GRAPHICS_PRIM;
+  geomData.push_back(g);
+  gprimHeaders.push_back(header);
+  gprimPoints.insert(gprimPoints.end(), points, points + n_points);
+  return add_aabb(TAG_GRAPHICS_PRIM, boxes.data(), boxes.size());
 }
 
 // BVHRT::AddGeom_SdfMultiOctree (BVH2Common_host.cpp:970-1018): the octree is walked by its
@@ -480,6 +589,7 @@ uint32_t Scene::leaf_kinds() const {
     else if (t == TAG_SDF_BRICK) k |= LK_SBS;
     else if (t == TAG_SDF_OCTREE) k |= LK_MULTI;
     else if (t == TAG_SDF_GRID) k |= LK_GRID;
+    else if (t == TAG_GRAPHICS_PRIM) k |= LK_GPRIM;
   }
   return k;
 }
@@ -724,6 +834,7 @@ hipError_t Scene::commit() {
   UP("sbs_roots", sbsRoots) UP("sbs_headers", sbsHeaders) UP("multi_nodes", multiNodes) UP("multi_values", multiValues)
   UP("sbs_mask_base", sbsMaskBase) UP("orig_indices", origIndices) UP("tlas_parent", tlasParent)
   UP("grid_data", gridData) UP("grid_offsets", gridOffsets) UP("grid_sizes", gridSizes)
+  UP("gprim_headers", gprimHeaders) UP("gprim_points", gprimPoints)
   // MultiRenderer's own m_instanceTransformInvTransposed (MultiRenderer_host.cpp:870-878):
   // transpose(inverse4x4(matrix)), refreshed by UpdateInstance (unlike the BVHRT copy)
   std::vector<float4x4> inst_inv_t(instances.size());
@@ -787,6 +898,8 @@ SceneView Scene::make_view() const {
   v.grid_data = (const float*)P("grid_data");
   v.grid_offsets = (const uint32_t*)P("grid_offsets");
   v.grid_sizes = (const uint32_t*)P("grid_sizes");
+  v.gprim_headers = (const GraphicsPrimHeader*)P("gprim_headers");
+  v.gprim_points = (const float4*)P("gprim_points");
   v.preset = preset;
   v.lod_thr = lod_thr;
   v.leaf_kinds = leaf_kinds();

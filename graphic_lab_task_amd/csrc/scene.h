@@ -65,6 +65,8 @@ struct Scene {
   std::vector<float> multiValues;
   std::vector<float> gridData;
   std::vector<uint32_t> gridOffsets, gridSizes;   // sizes: 3 words per grid
+  std::vector<GraphicsPrimHeader> gprimHeaders;
+  std::vector<float4> gprimPoints;
   std::vector<InstanceData> instances;
   std::vector<BVHNode> tlas;
 
@@ -144,6 +146,7 @@ struct Scene {
   uint32_t add_sdf_frame_octree(const SdfFrameOctreeNode* nodes, size_t n);
   uint32_t add_sdf_svs(const SdfSVSNode* nodes, size_t n);
   uint32_t add_sdf_grid(const uint32_t size[3], const float* data);
+  uint32_t add_graphics_prims(const GraphicsPrimHeader& header, const float4* points, size_t n_points);
   uint32_t add_sdf_sbs(const SdfSBSHeader& header, const SdfSBSNode* nodes, size_t n_nodes, const uint32_t* values,
                        size_t n_values, const float* values_f, size_t n_values_f);
   uint32_t add_sdf_multi_octree(const SdfMultiOctreeNode* nodes, size_t n_nodes, const float* values,

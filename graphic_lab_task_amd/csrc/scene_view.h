@@ -19,6 +19,7 @@ enum LeafKind : uint32_t {
   // smoothing in (the others stay free of its registers).
   LK_SBS_SMOOTH = 1u << 5,
   LK_GRID  = 1u << 6,   // TAG_SDF_GRID (dense SDF grid, sphere tracing)
+  LK_GPRIM = 1u << 7,   // TAG_GRAPHICS_PRIM (points, lines, segments, arrows, wire boxes)
 };
 
 // Frame-octree leaf as the kernels read it: the leaf box, its 8 corner distances and the
@@ -74,6 +75,9 @@ struct SceneView {
   const float*              grid_data;    // m_SdfGridData
   const uint32_t*           grid_offsets; // m_SdfGridOffsets
   const uint32_t*           grid_sizes;   // m_SdfGridSizes, 3 words per grid
+  // graphics primitives
+  const GraphicsPrimHeader* gprim_headers; // m_GraphicsPrimHeaders
+  const float4*             gprim_points;  // m_GraphicsPrimPoints
 
   MultiRenderPreset preset;
   float    lod_thr;                 // m_precomputedLoDThr

@@ -619,6 +619,157 @@ __device__ __forceinline__ void sdf_grid_intersect(const SceneView& s, const Geo
   }
 }
 
+// rayCapsuleIntersect (BVH2Common.cpp:1184-1223): (normal, distance), distance 2e15 on a miss.
+__device__ __forceinline__ float4 ray_capsule(float3 ray_pos, float3 ray_dir, float3 pa, float3 pb, float ra) {
+  const float3 ba = pb - pa, oa = ray_pos - pa, ob = ray_pos - pb;
+  const float baba = dot3(ba, ba), bard = dot3(ba, ray_dir), baoa = dot3(ba, oa), rdoa = dot3(ray_dir, oa),
+              oaoa = dot3(oa, oa);
+  const float a = baba - bard * bard;
+  float b = baba * rdoa - baoa * bard;
+  float c = baba * oaoa - baoa * baoa - ra * ra * baba;
+  float h = b * b - a * c;
+  if (h >= 0.f) {
+    float dist = (-b - std::sqrt(h)) / a;
+    const float y = baoa + dist * bard;
+    if (y > 0.f && y < baba) {
+      const float3 n = normalize3(oa + dist * ray_dir - ba * y / baba);
+      return make4(n.x, n.y, n.z, dist);
+    }
+    const float3 oc = (y <= 0.0f) ? oa : ob;
+    b = dot3(ray_dir, oc);
+    c = dot3(oc, oc) - ra * ra;
+    h = b * b - c;
+    if (h > 0.f) {
+      dist = -b - std::sqrt(h);
+      const float3 n = normalize3(ray_pos + dist * ray_dir + (oc - ray_pos));
+      return make4(n.x, n.y, n.z, dist);
+    }
+  }
+  return make4(2e15f, 2e15f, 2e15f, 2e15f);
+}
+
+// BVHRT::IntersectGraphicPrims (BVH2Common.cpp:1225-1440): one primitive per BLAS leaf — a
+// sphere (point), an infinite cylinder (line), a capped cylinder (segment), a cylinder with a
+// cone (arrow) or the 12 edge capsules of a box; colour from the header or the primitive.
+__device__ __forceinline__ void gprim_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir,
+                                                float tNear, uint32_t instId, uint32_t geomId, uint32_t a_start,
+                                                CRT_Hit& hit) {
+  const float EPS = 1e-5f, T_MAX = 1e15f;
+  const GraphicsPrimHeader header = s.gprim_headers[g.offset.x];
+  float3 color = make3(header.color[0], header.color[1], header.color[2]);
+  const uint32_t ptype = header.prim_type;
+  const uint32_t prim_len = graphics_prim_len(ptype);
+  const uint32_t i = g.offset.y + prim_len * a_start;
+  const bool has_custom_color = ptype >= GRAPH_PRIM_POINT_COLOR;
+  float t = T_MAX;
+  float3 norm = make3(0.f, 0.f, 0.f);
+  if (ptype == GRAPH_PRIM_POINT || ptype == GRAPH_PRIM_POINT_COLOR) {
+    const float4 point = s.gprim_points[i];
+    if (has_custom_color) color = xyz(s.gprim_points[i + 1]);
+    const float3 point3 = xyz(point);
+    const float pt_radius = point.w;
+    const float3 pmp = ray_pos - point3;
+    const float b = 2.f * dot3(ray_dir, pmp);
+    const float c = dot3(pmp, pmp) - pt_radius * pt_radius;
+    float D = b * b - 4 * c;
+    if (!(D >= EPS)) return;
+    D = std::sqrt(D);
+    t = -(b + D) * 0.5f;
+    if (t < EPS) t = (-b + D) * 0.5f;
+    norm = normalize3((ray_pos + t * ray_dir - point3) * 100.f);
+  } else {
+    const float4 point1 = s.gprim_points[i], point2 = s.gprim_points[i + 1];
+    if (has_custom_color) color = xyz(s.gprim_points[i + 2]);
+    const float3 point1_3 = xyz(point1);
+    float3 point2_3 = xyz(point2);
+    const bool arrow = ptype == GRAPH_PRIM_LINE_SEGMENT_DIR || ptype == GRAPH_PRIM_LINE_SEGMENT_DIR_COLOR;
+    if (arrow) point2_3 = point2_3 * 0.71f + point1_3 * 0.29f;
+    float ra = point1.w;
+    if (ptype == GRAPH_PRIM_LINE || ptype == GRAPH_PRIM_LINE_SEGMENT || arrow || ptype == GRAPH_PRIM_LINE_COLOR ||
+        ptype == GRAPH_PRIM_LINE_SEGMENT_COLOR) {
+      const float3 oc = ray_pos - point1_3, ba = point2_3 - point1_3;
+      const float baba = dot3(ba, ba), bard = dot3(ba, ray_dir), baoc = dot3(ba, oc);
+      const float k2 = baba - (bard * bard);
+      const float k1 = baba * dot3(oc, ray_dir) - baoc * bard;
+      const float k0 = baba * dot3(oc, oc) - baoc * baoc - ra * ra * baba;
+      const float D = k1 * k1 - k2 * k0;
+      if (D > 0.f) {
+        float dist = -(k1 + std::sqrt(D)) / k2;
+        const float y = baoc + dist * bard;
+        if (ptype == GRAPH_PRIM_LINE || (y > 0.f && y < baba)) {
+          t = dist;
+          norm = normalize3(oc + t * ray_dir - ba * y / baba);
+        } else {
+          dist = ((y < 0.f ? 0.f : baba) - baoc) / bard;
+          if (fabs_ref(k1 + k2 * dist) < std::sqrt(D)) {
+            t = dist;
+            norm = normalize3(ba * sign_ref(y));
+          }
+        }
+      }
+    }
+    if (arrow) {
+      ra = length3(point2_3 - point1_3) * 0.15f;
+      point2_3 = xyz(point2);
+      const float3 pa = point2_3 * 0.7f + point1_3 * 0.3f;
+      const float3 ba = point2_3 - pa, oa = ray_pos - pa;
+      const float m0 = dot3(ba, ba), m1 = dot3(oa, ba), m2 = dot3(ray_dir, ba), m3 = dot3(ray_dir, oa),
+                  m5 = dot3(oa, oa);
+      if (m1 < 0.f) {
+        const float3 tmp1 = oa * m2 - ray_dir * m1;
+        if (dot3(tmp1, tmp1) < (ra * ra * m2 * m2)) {
+          t = -m1 / m2;
+          norm = normalize3(-1.0f * ba);
+        }
+      }
+      const float rr = ra;
+      const float hy = m0 + rr * rr;
+      const float k2 = m0 * m0 - m2 * m2 * hy;
+      const float k1 = m0 * m0 * m3 - m1 * m2 * hy + m0 * ra * (rr * m2);
+      const float k0 = m0 * m0 * m5 - m1 * m1 * hy + m0 * ra * (rr * m1 * 2.0f - m0 * ra);
+      const float D = k1 * k1 - k2 * k0;
+      if (D > 0.f) {
+        const float dist = -(k1 + std::sqrt(D)) / k2;
+        const float y = m1 + dist * m2;
+        if (y >= 0.f && y <= m0) {
+          t = dist;
+          norm = normalize3(m0 * (m0 * (oa + t * ray_dir) + rr * ba * ra) - ba * hy * y);
+        }
+      }
+    }
+    if (ptype == GRAPH_PRIM_BOX || ptype == GRAPH_PRIM_BOX_COLOR) {
+      const float3 lo = min3(point2_3, point1_3), hi = max3(point2_3, point1_3);
+      const float mm[3][2] = {{lo.x, hi.x}, {lo.y, hi.y}, {lo.z, hi.z}};
+      const int verts[4] = {0, 3, 5, 6};
+      for (int vi = 0; vi < 4; ++vi) {
+        const int ax[3] = {int((verts[vi] & 4) != 0), int((verts[vi] & 2) != 0), int((verts[vi] & 1) != 0)};
+        const float3 pa = make3(mm[0][ax[0]], mm[1][ax[1]], mm[2][ax[2]]);
+        for (int axis = 0; axis < 3; ++axis) {
+          float3 pb = pa;
+          const float other = mm[axis][1 - ax[axis]];
+          if (axis == 0) pb.x = other; else if (axis == 1) pb.y = other; else pb.z = other;
+          const float4 nd = ray_capsule(ray_pos, ray_dir, pa, pb, ra);
+          if (nd.w < t) {
+            t = nd.w;
+            norm = make3(nd.x, nd.y, nd.z);
+          }
+        }
+      }
+    }
+  }
+  if (t > tNear && t < hit.t) {
+    const float2 e = encode_normal(norm);
+    hit.t = t;
+    hit.primId = a_start;
+    hit.instId = instId;
+    hit.geomId = geomId | (TYPE_GRAPHICS_PRIM << SH_TYPE);
+    hit.coords[0] = color.x + color.y / 256.0f;
+    hit.coords[1] = color.z;
+    hit.coords[2] = e.x;
+    hit.coords[3] = e.y;
+  }
+}
+
 // eval_dist_trilinear_diff (BVH2Common.cpp:751-781): gradient of the trilinear of 8 values.
 __device__ __forceinline__ float3 eval_dist_trilinear_diff(const float v[8], float3 dp) {
   const float ddx = -(1 - dp.y) * (1 - dp.z) * v[0] + -(1 - dp.y) * (dp.z) * v[1] + -(dp.y) * (1 - dp.z) * v[2] +

@@ -77,6 +77,12 @@ lrt_status lrt_add_triangles(lrt_scene* s, const float* vpos, size_t n_verts, si
  * nodes: SdfFrameOctreeNode[n_nodes] (36 B each, sdf/simple/sdf_simple_shared.h:23-27). */
 lrt_status lrt_add_sdf_frame_octree(lrt_scene* s, const void* nodes, size_t n_nodes, uint32_t* out_geom_id);
 
+/* Replaces BVHRT::AddGeom_GraphicsPrim (BVH2Common_host.cpp:767-903). header: GraphicsPrimHeader
+ * (float color[3], uint32 prim_type; utils/misc/graphics_primitive_common.h:19-23); points:
+ * n_points float4 (1 per point, 2 per line / segment / arrow / box, one more with *_COLOR types). */
+lrt_status lrt_add_graphics_prims(lrt_scene* s, const void* header, const float* points, size_t n_points,
+                                  uint32_t* out_geom_id);
+
 /* Replaces BVHRT::AddGeom_SdfGrid (BVH2Common_host.cpp:466-499): a dense grid of distances over
  * the unit cube, size[0] * size[1] * size[2] floats with x fastest (fewer than 2^28). */
 lrt_status lrt_add_sdf_grid(lrt_scene* s, const uint32_t size[3], const float* data, uint32_t* out_geom_id);
@@ -372,6 +378,7 @@ lrt_status lrt_stack_overflow_count(lrt_scene* s, uint64_t* out, int reset);
  * "multi_nodes", "multi_values", "orig_indices" (uint32: triangle indices in the caller's order),
  * "svs_nodes" (SdfSVSNode), "svs_roots", "tlas_parent" (uint32: parent node of each TLAS node),
  * "grid_data" (float), "grid_offsets", "grid_sizes" (uint32, 3 per grid),
+ * "gprim_headers" (GraphicsPrimHeader), "gprim_points" (float4),
  * "preset", and "sbs_mask" (uint32: the committed voxel occupancy
  * masks, per brick set in order and per brick one summary word then ceil(bs^3/32) bit words,
  * bit v set when the minimum of voxel v's 8 corner values is <= 0; copied from the device).

@@ -49,7 +49,7 @@ COUNTS_DTYPE = np.dtype([("blas_nodes", "<u4"), ("leaves", "<u4"), ("prims", "<u
 _ARRAYS = ["nodes", "tlas", "instances", "geoms", "geom_tags", "start_end", "prim_id_count", "vert_pos",
            "vert_norm", "indices", "prim_indices", "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes",
            "sbs_data", "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "svs_nodes", "svs_roots",
-           "grid_data", "grid_offsets", "grid_sizes"]
+           "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points"]
 
 
 class OracleScene(ctypes.Structure):

@@ -99,6 +99,7 @@ struct SBSNode { uint32_t pos_xy, pos_z_lod_size, data_offset, pad; };
 struct SBSHeader { uint32_t brick_size, brick_pad, bytes_per_value, aux_data; };
 struct MultiNode { uint32_t data_offset, children_offset, voxel_count, flags; };   // sdf_multi_shared.h:8-14
 struct SVSNode { uint32_t pos_xy, pos_z_lod_size, values[2]; };                    // sdf_simple_shared.h:30-35
+struct GPrimHeader { float color[3]; uint32_t prim_type; };                         // graphics_primitive_common.h:19-23
 struct Preset {
   uint32_t render_mode, sdf_node_intersect, normal_mode, ray_gen_mode, interpolation_mode, representation_mode, spp,
       fixed_lod;
@@ -111,7 +112,10 @@ static_assert(sizeof(Geom) == 64 && sizeof(Inst) == 256 && sizeof(FrameNode) == 
 constexpr uint32_t LEAF_BIT = 0x80000000u, START_MASK = 0x0FFFFFFFu, SIZE_MASK = 0x70000000u;
 constexpr int STACK_SIZE = 80;
 constexpr uint32_t TAG_NONE = 0, TAG_TRIANGLE = 1, TAG_SDF_BRICK = 4, TAG_SDF_NODE = 8, TAG_SDF_OCTREE = 15;
-constexpr uint32_t TAG_SDF_GRID = 11, TYPE_SDF_GRID = 1;
+constexpr uint32_t TAG_SDF_GRID = 11, TYPE_SDF_GRID = 1, TAG_GRAPHICS_PRIM = 9, TYPE_GRAPHICS_PRIM = 17;
+// GraphicsPrimType (utils/misc/graphics_primitive_common.h:7-16)
+constexpr uint32_t GP_POINT = 0, GP_LINE = 1, GP_SEGMENT = 2, GP_SEGMENT_DIR = 3, GP_BOX = 4, GP_POINT_COLOR = 5,
+                   GP_LINE_COLOR = 6, GP_SEGMENT_COLOR = 7, GP_SEGMENT_DIR_COLOR = 8, GP_BOX_COLOR = 9;
 constexpr uint32_t TYPE_SDF_SVS = 5;
 constexpr uint32_t TYPE_MESH_TRIANGLE = 0, TYPE_SDF_FRAME_OCTREE = 3, SH_TYPE = 24, GEOM_ID_MASK = 0x00FFFFFFu;
 constexpr uint32_t LAYOUT_DX_UV16 = 2u << 24, LAYOUT_DX_RGB8 = 3u << 24, LAYOUT_ID32F = 4u << 24, LAYOUT_ID32F_IN = 5u << 24;
@@ -148,6 +152,8 @@ struct OracleScene {
   const float* grid_data;
   const uint32_t* grid_offsets;
   const uint32_t* grid_sizes;     // 3 per grid
+  const GPrimHeader* gprim_headers;
+  const f4* gprim_points;
   Preset preset;
 };
 
@@ -479,6 +485,173 @@ struct Tracer {
       vmin = std::min(vmin, values[i]);
     }
     return vmin;
+  }
+
+  // rayCapsuleIntersect (BVH2Common.cpp:1184-1223)
+  static f4 rayCapsuleIntersect(f3 ray_pos, f3 ray_dir, f3 pa, f3 pb, float ra) {
+    f3 ba = pb - pa;
+    f3 oa = ray_pos - pa;
+    f3 ob = ray_pos - pb;
+    float baba = dot(ba, ba);
+    float bard = dot(ba, ray_dir);
+    float baoa = dot(ba, oa);
+    float rdoa = dot(ray_dir, oa);
+    float oaoa = dot(oa, oa);
+    float a = baba - bard * bard;
+    float b = baba * rdoa - baoa * bard;
+    float c = baba * oaoa - baoa * baoa - ra * ra * baba;
+    float h = b * b - a * c;
+    if (h >= 0.f) {
+      float dist = (-b - std::sqrt(h)) / a;
+      float y = baoa + dist * bard;
+      if (y > 0.f && y < baba) {
+        f3 n = normalize(oa + dist * ray_dir - ba * y / baba);
+        return f4{n.x, n.y, n.z, dist};
+      }
+      f3 oc = (y <= 0.0f) ? oa : ob;
+      b = dot(ray_dir, oc);
+      c = dot(oc, oc) - ra * ra;
+      h = b * b - c;
+      if (h > 0.f) {
+        dist = -b - std::sqrt(h);
+        f3 n = normalize(ray_pos + dist * ray_dir + (oc - ray_pos));
+        return f4{n.x, n.y, n.z, dist};
+      }
+    }
+    return f4{2e15f, 2e15f, 2e15f, 2e15f};
+  }
+
+  // BVHRT::IntersectGraphicPrims (BVH2Common.cpp:1225-1440)
+  void IntersectGraphicPrims(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId, uint32_t a_start,
+                             Hit* pHit) {
+    const float EPS = 1e-5f, T_MAX = 1e15f;
+    const GPrimHeader header = s.gprim_headers[s.geoms[geomId].offset[0]];
+    f3 color(header.color[0], header.color[1], header.color[2]);
+    const uint32_t pt = header.prim_type;
+    uint32_t prim_len = 2u;
+    if (pt == GP_POINT) prim_len = 1u;
+    if (pt >= GP_LINE_COLOR) prim_len = 3u;
+    const uint32_t start_index = s.geoms[geomId].offset[1] + prim_len * a_start;
+    const uint32_t end_index = start_index + prim_len;
+    const bool has_custom_color = pt >= GP_POINT_COLOR;
+    auto store = [&](float t, f3 norm) {
+      f2 e = encode_normal(norm);
+      pHit->t = t;
+      pHit->primId = a_start;
+      pHit->instId = instId;
+      pHit->geomId = geomId | (TYPE_GRAPHICS_PRIM << SH_TYPE);
+      pHit->coords[0] = color.x + color.y / 256.0f;
+      pHit->coords[1] = color.z;
+      pHit->coords[2] = e.x;
+      pHit->coords[3] = e.y;
+    };
+    if (pt == GP_POINT || pt == GP_POINT_COLOR) {
+      for (uint32_t i = start_index; i < end_index; i += prim_len) {
+        f4 point = s.gprim_points[i];
+        if (has_custom_color) color = to3(s.gprim_points[i + 1]);
+        f3 point3(point.x, point.y, point.z);
+        float pt_radius = point.w;
+        f3 pmp = ray_pos - point3;
+        float b = 2.f * dot(ray_dir, pmp);
+        float c = dot(pmp, pmp) - pt_radius * pt_radius;
+        float D = b * b - 4 * c;
+        if (D >= EPS) {
+          D = std::sqrt(D);
+          float t = -(b + D) * 0.5f;
+          if (t < EPS) t = (-b + D) * 0.5f;
+          if (t > tNear && t < pHit->t) store(t, normalize((ray_pos + t * ray_dir - point3) * 100.f));
+        }
+      }
+      return;
+    }
+    for (uint32_t i = start_index; i < end_index; i += prim_len) {
+      f4 point1 = s.gprim_points[i];
+      f4 point2 = s.gprim_points[i + 1];
+      if (has_custom_color) color = to3(s.gprim_points[i + 2]);
+      f3 point1_3(point1.x, point1.y, point1.z);
+      f3 point2_3(point2.x, point2.y, point2.z);
+      const bool arrow = pt == GP_SEGMENT_DIR || pt == GP_SEGMENT_DIR_COLOR;
+      if (arrow) point2_3 = point2_3 * 0.71f + point1_3 * 0.29f;
+      float ra = point1.w;
+      float t = T_MAX;
+      f3 norm(0.f, 0.f, 0.f);
+      if (pt == GP_LINE || pt == GP_SEGMENT || pt == GP_SEGMENT_DIR || pt == GP_LINE_COLOR || pt == GP_SEGMENT_COLOR ||
+          pt == GP_SEGMENT_DIR_COLOR) {
+        f3 oc = ray_pos - point1_3;
+        f3 ba = point2_3 - point1_3;
+        float baba = dot(ba, ba);
+        float bard = dot(ba, ray_dir);
+        float baoc = dot(ba, oc);
+        float k2 = baba - (bard * bard);
+        float k1 = baba * dot(oc, ray_dir) - baoc * bard;
+        float k0 = baba * dot(oc, oc) - baoc * baoc - ra * ra * baba;
+        float D = k1 * k1 - k2 * k0;
+        if (D > 0.f) {
+          float dist = -(k1 + std::sqrt(D)) / k2;
+          float y = baoc + dist * bard;
+          if (pt == GP_LINE || (y > 0.f && y < baba)) {
+            t = dist;
+            norm = normalize(oc + t * ray_dir - ba * y / baba);
+          } else {
+            dist = ((y < 0.f ? 0.f : baba) - baoc) / bard;
+            if (std::abs(k1 + k2 * dist) < std::sqrt(D)) {
+              t = dist;
+              norm = normalize(ba * lm_sign(y));
+            }
+          }
+        }
+      }
+      if (arrow) {
+        ra = length(point2_3 - point1_3) * 0.15f;
+        point2_3 = f3(point2.x, point2.y, point2.z);
+        f3 pa = point2_3 * 0.7f + point1_3 * 0.3f;
+        f3 ba = point2_3 - pa;
+        f3 oa = ray_pos - pa;
+        float m0 = dot(ba, ba), m1 = dot(oa, ba), m2 = dot(ray_dir, ba), m3 = dot(ray_dir, oa), m5 = dot(oa, oa);
+        if (m1 < 0.f) {
+          f3 tmp1 = oa * m2 - ray_dir * m1;
+          if (dot(tmp1, tmp1) < (ra * ra * m2 * m2)) {
+            t = -m1 / m2;
+            norm = normalize(-1.0f * ba);
+          }
+        }
+        float rr = ra;
+        float hy = m0 + rr * rr;
+        float k2 = m0 * m0 - m2 * m2 * hy;
+        float k1 = m0 * m0 * m3 - m1 * m2 * hy + m0 * ra * (rr * m2);
+        float k0 = m0 * m0 * m5 - m1 * m1 * hy + m0 * ra * (rr * m1 * 2.0f - m0 * ra);
+        float D = k1 * k1 - k2 * k0;
+        if (D > 0.f) {
+          float dist = -(k1 + std::sqrt(D)) / k2;
+          float y = m1 + dist * m2;
+          if (y >= 0.f && y <= m0) {
+            t = dist;
+            norm = normalize(m0 * (m0 * (oa + t * ray_dir) + rr * ba * ra) - ba * hy * y);
+          }
+        }
+      }
+      if (pt == GP_BOX || pt == GP_BOX_COLOR) {
+        f3 lo(std::min(point2_3.x, point1_3.x), std::min(point2_3.y, point1_3.y), std::min(point2_3.z, point1_3.z));
+        f3 hi(std::max(point2_3.x, point1_3.x), std::max(point2_3.y, point1_3.y), std::max(point2_3.z, point1_3.z));
+        float minmax[3][2] = {{lo.x, hi.x}, {lo.y, hi.y}, {lo.z, hi.z}};
+        const int vert_indices[4] = {0, 3, 5, 6};
+        for (int vi = 0; vi < 4; ++vi) {
+          int ax[3] = {int((vert_indices[vi] & 4u) != 0), int((vert_indices[vi] & 2u) != 0), int((vert_indices[vi] & 1u) != 0)};
+          f3 pa(minmax[0][ax[0]], minmax[1][ax[1]], minmax[2][ax[2]]);
+          for (int axis = 0; axis < 3; ++axis) {
+            f3 pb = pa;
+            float& slot = axis == 0 ? pb.x : (axis == 1 ? pb.y : pb.z);
+            slot = minmax[axis][1 - ax[axis]];
+            f4 nd = rayCapsuleIntersect(ray_pos, ray_dir, pa, pb, ra);
+            if (nd.w < t) {
+              t = nd.w;
+              norm = f3(nd.x, nd.y, nd.z);
+            }
+          }
+        }
+      }
+      if (t > tNear && t < pHit->t) store(t, norm);
+    }
   }
 
   // BVHRT::eval_distance_sdf_grid (BVH2Common.cpp:1442-1478), trilinear
@@ -846,6 +1019,7 @@ struct Tracer {
       case TAG_SDF_BRICK: OctreeBrickIntersect(s.geoms[geomId].type, ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
       case TAG_SDF_OCTREE: OctreeIntersect(ray_pos, ray_dir, tNear, instId, geomId, pHit); break;
       case TAG_SDF_GRID: IntersectSDFGrid(ray_pos, ray_dir, tNear, instId, geomId, pHit); break;
+      case TAG_GRAPHICS_PRIM: IntersectGraphicPrims(ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
       default: break;
     }
     if (s.preset.render_mode == 26) pHit->instId = (pHit->instId == 0xFFFFFFFF) ? 1 : pHit->instId + 1;
