@@ -214,6 +214,8 @@ def lib() -> ctypes.CDLL:
             "lrt_gen_icosphere": ([u32, f32, f32, u64, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
             "lrt_gen_geodesic": ([u32, f32, f32, u64, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
             "lrt_sdf_spheres_frame_octree": ([vp, sz, u32, i32, ctypes.POINTER(_Buffer)], i32),
+            "lrt_sdf_spheres_global_octree": ([vp, sz, u32, u32, u32, i32, ctypes.POINTER(_Buffer),
+                                               ctypes.POINTER(_Buffer)], i32),
             "lrt_add_sdf_multi_octree": ([vp, vp, sz, vp, sz, ctypes.POINTER(u32)], i32),
             "lrt_sdf_spheres_multi_octree": ([vp, sz, u32, i32, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
             "lrt_sdf_spheres_sbs": ([vp, sz, u32, vp, u64, i32, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
@@ -249,7 +251,7 @@ def exported_symbols():
             "lrt_sbs_batch_create", "lrt_sbs_batch_run", "lrt_sbs_batch_autotune", "lrt_sbs_batch_configure", "lrt_sbs_batch_destroy",
             "lrt_stack_overflow_count", "lrt_get_array", "lrt_buffer_free",
             "lrt_camera_inverse", "lrt_gen_icosphere", "lrt_gen_geodesic", "lrt_sdf_spheres_frame_octree",
-            "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed", "lrt_resolve_dev", "lrt_tonemap_dev",
+            "lrt_sdf_spheres_global_octree", "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed", "lrt_resolve_dev", "lrt_tonemap_dev",
             "lrt_load_sdf_frame_octree",
             "lrt_save_sdf_frame_octree", "lrt_load_sdf_sbs", "lrt_save_sdf_sbs", "lrt_load_sdf_multi_octree",
             "lrt_save_sdf_multi_octree"]
@@ -635,6 +637,20 @@ def gen_geodesic(n, amplitude=0.0, frequency=4.0, seed=1):
     vb, ib = _Buffer(), _Buffer()
     _check(lib().lrt_gen_geodesic(n, amplitude, frequency, seed, ctypes.byref(vb), ctypes.byref(ib)))
     return _take(vb, np.float32).reshape(-1, 3), _take(ib, np.uint32).reshape(-1, 3)
+
+
+GPU_BUILD = -1   # `threads` value of the SDF generators that builds the global octree on the GPU
+
+
+def sdf_spheres_global_octree(spheres: np.ndarray, depth: int, brick_size: int = 1, brick_pad: int = 0,
+                              threads: int = 0):
+    """(nodes (n, 3) uint32 [type, val_off, offset], values float32) of sdf_to_global_octree
+    (sdf/build/sparse_octree_builder.cpp:106-264) for a sphere union; threads < 0: on the GPU."""
+    s = np.ascontiguousarray(spheres, dtype=np.float32).reshape(-1, 4)
+    nb, vb = _Buffer(), _Buffer()
+    _check(lib().lrt_sdf_spheres_global_octree(_ptr(s), s.shape[0], depth, brick_size, brick_pad, threads,
+                                               ctypes.byref(nb), ctypes.byref(vb)))
+    return _take(nb, np.uint32).reshape(-1, 3), _take(vb, np.float32)
 
 
 def sdf_spheres_frame_octree(spheres: np.ndarray, depth: int, threads: int = 0) -> np.ndarray:

@@ -34,6 +34,14 @@ struct GlobalOctree {
 // sdf_converter::sdf_to_global_octree (sdf/build/sparse_octree_builder.cpp:106-264)
 void sdf_to_global_octree(const SphereUnion& sdf, uint32_t depth, uint32_t brick_size, uint32_t brick_pad,
                           int threads, GlobalOctree& out);
+// The same build on the current HIP device (sdf_build_kernels.hip): level-synchronous growth of
+// the border subtrees, depth-first numbering recovered from subtree sizes; equal to the host
+// build byte for byte. Throws std::runtime_error on a HIP failure.
+void gpu_sdf_to_global_octree(const SphereUnion& sdf, uint32_t depth, uint32_t brick_size, uint32_t brick_pad,
+                              GlobalOctree& out);
+// host (threads >= 0; 0 = all cores) or GPU (threads < 0) build
+void build_global_octree(const SphereUnion& sdf, uint32_t depth, uint32_t brick_size, uint32_t brick_pad,
+                         int threads, GlobalOctree& out);
 // global_octree_to_frame_octree (sdf/simple/sdf_converter.cpp:53-65)
 void global_octree_to_frame_octree(const GlobalOctree& g, std::vector<SdfFrameOctreeNode>& out);
 // global_octree_to_multi_octree (sdf/multi/sdf_multi.cpp:124-180): one node per octree node,

@@ -97,7 +97,7 @@ lrt_status lrt_sdf_spheres_frame_octree(const float* spheres, size_t n_spheres, 
   if (!spheres || n_spheres == 0 || !nodes) return LRT_ERR_INVALID_ARG;
   return guarded([&]() -> lrt_status {
     lrt::GlobalOctree g;
-    lrt::sdf_to_global_octree(lrt::SphereUnion{spheres, n_spheres}, depth, 1, 0, threads, g);
+    lrt::build_global_octree(lrt::SphereUnion{spheres, n_spheres}, depth, 1, 0, threads, g);
     std::vector<lrt::SdfFrameOctreeNode> frame;
     lrt::global_octree_to_frame_octree(g, frame);
     return to_buffer(frame, nodes);
@@ -109,7 +109,7 @@ lrt_status lrt_sdf_spheres_multi_octree(const float* spheres, size_t n_spheres, 
   if (!spheres || n_spheres == 0 || !nodes || !values) return LRT_ERR_INVALID_ARG;
   return guarded([&]() -> lrt_status {
     lrt::GlobalOctree g;
-    lrt::sdf_to_global_octree(lrt::SphereUnion{spheres, n_spheres}, depth, 1, 0, threads, g);
+    lrt::build_global_octree(lrt::SphereUnion{spheres, n_spheres}, depth, 1, 0, threads, g);
     std::vector<lrt::SdfMultiOctreeNode> n;
     std::vector<float> v;
     lrt::global_octree_to_multi_octree(g, n, v);
@@ -126,7 +126,7 @@ lrt_status lrt_sdf_spheres_sbs(const float* spheres, size_t n_spheres, uint32_t 
     const lrt::SdfSBSHeader h = *static_cast<const lrt::SdfSBSHeader*>(header);
     if (h.bytes_per_value != 1 && h.bytes_per_value != 2 && h.bytes_per_value != 4) return LRT_ERR_INVALID_ARG;
     lrt::GlobalOctree g;
-    lrt::sdf_to_global_octree(lrt::SphereUnion{spheres, n_spheres}, depth, h.brick_size, h.brick_pad, threads, g);
+    lrt::build_global_octree(lrt::SphereUnion{spheres, n_spheres}, depth, h.brick_size, h.brick_pad, threads, g);
     std::vector<lrt::SdfSBSNode> n;
     std::vector<uint32_t> v;
     lrt::global_octree_to_sbs_dx(g, h, colour_seed, n, v);
@@ -142,7 +142,7 @@ lrt_status lrt_sdf_spheres_sbs_indexed(const float* spheres, size_t n_spheres, u
   if (!spheres || n_spheres == 0 || !nodes || !values || !values_f) return LRT_ERR_INVALID_ARG;
   return guarded([&]() -> lrt_status {
     lrt::GlobalOctree g;
-    lrt::sdf_to_global_octree(lrt::SphereUnion{spheres, n_spheres}, depth, brick_size, 0, threads, g);
+    lrt::build_global_octree(lrt::SphereUnion{spheres, n_spheres}, depth, brick_size, 0, threads, g);
     std::vector<lrt::SdfSBSNode> n;
     std::vector<uint32_t> v;
     std::vector<float> vf;
@@ -152,6 +152,24 @@ lrt_status lrt_sdf_spheres_sbs_indexed(const float* spheres, size_t n_spheres, u
     st = to_buffer(v, values);
     if (st != LRT_OK) return st;
     return to_buffer(vf, values_f);
+  });
+}
+
+lrt_status lrt_sdf_spheres_global_octree(const float* spheres, size_t n_spheres, uint32_t depth, uint32_t brick_size,
+                                         uint32_t brick_pad, int threads, lrt_buffer* nodes, lrt_buffer* values) {
+  if (!spheres || n_spheres == 0 || !nodes || !values) return LRT_ERR_INVALID_ARG;
+  return guarded([&]() -> lrt_status {
+    lrt::GlobalOctree g;
+    lrt::build_global_octree(lrt::SphereUnion{spheres, n_spheres}, depth, brick_size, brick_pad, threads, g);
+    std::vector<uint32_t> packed(g.nodes.size() * 3);
+    for (size_t i = 0; i < g.nodes.size(); ++i) {
+      packed[3 * i + 0] = g.nodes[i].type;
+      packed[3 * i + 1] = g.nodes[i].val_off;
+      packed[3 * i + 2] = g.nodes[i].offset;
+    }
+    lrt_status st = to_buffer(packed, nodes);
+    if (st != LRT_OK) return st;
+    return to_buffer(g.values, values);
   });
 }
 

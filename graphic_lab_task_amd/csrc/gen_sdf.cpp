@@ -147,6 +147,12 @@ void sdf_to_global_octree(const SphereUnion& sdf, uint32_t depth, uint32_t brick
   }
 }
 
+void build_global_octree(const SphereUnion& sdf, uint32_t depth, uint32_t brick_size, uint32_t brick_pad,
+                         int threads, GlobalOctree& out) {
+  if (threads < 0) gpu_sdf_to_global_octree(sdf, depth, brick_size, brick_pad, out);
+  else sdf_to_global_octree(sdf, depth, brick_size, brick_pad, threads, out);
+}
+
 // global_octree_to_multi_octree (sdf/multi/sdf_multi.cpp:124-180)
 void global_octree_to_multi_octree(const GlobalOctree& g, std::vector<SdfMultiOctreeNode>& nodes,
                                    std::vector<float>& values) {

@@ -41,9 +41,19 @@ lrt_status lrt_gen_icosphere(uint32_t subdiv, float amplitude, float frequency, 
 lrt_status lrt_gen_geodesic(uint32_t n, float amplitude, float frequency, uint64_t seed, lrt_buffer* vpos,
                             lrt_buffer* indices);
 
+/* The SDF generators below build the reference's global sparse octree first
+ * (sdf_to_global_octree, sdf/build/sparse_octree_builder.cpp:106-264): threads >= 0 on the host
+ * (0: all cores), threads < 0 on the current HIP device (same bytes; the converters that follow
+ * run on the host). */
+
+/* The global octree itself: nodes as uint32 triples (type, val_off, offset; global_octree.h
+ * GlobalOctreeNode subset), values v^3 floats per node, v = brick_size + 2 brick_pad + 1. */
+lrt_status lrt_sdf_spheres_global_octree(const float* spheres, size_t n_spheres, uint32_t depth, uint32_t brick_size,
+                                         uint32_t brick_pad, int threads, lrt_buffer* nodes, lrt_buffer* values);
+
 /* sdf_converter::create_sdf_frame_octree (sdf/simple/sdf_converter.cpp:265-273) for the union
  * of spheres (cx,cy,cz,r)[n_spheres]: sdf_to_global_octree with brick size 1, then
- * global_octree_to_frame_octree. nodes: SdfFrameOctreeNode array. threads <= 0: all cores. */
+ * global_octree_to_frame_octree. nodes: SdfFrameOctreeNode array. */
 lrt_status lrt_sdf_spheres_frame_octree(const float* spheres, size_t n_spheres, uint32_t depth, int threads,
                                         lrt_buffer* nodes);
 
