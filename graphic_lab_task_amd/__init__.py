@@ -41,6 +41,39 @@ GPRIM_HEADER_DTYPE = np.dtype([("color", "<f4", (3,)), ("prim_type", "<u4")])   
 (GRAPH_PRIM_POINT, GRAPH_PRIM_LINE, GRAPH_PRIM_LINE_SEGMENT, GRAPH_PRIM_LINE_SEGMENT_DIR, GRAPH_PRIM_BOX,
  GRAPH_PRIM_POINT_COLOR, GRAPH_PRIM_LINE_COLOR, GRAPH_PRIM_LINE_SEGMENT_COLOR, GRAPH_PRIM_LINE_SEGMENT_DIR_COLOR,
  GRAPH_PRIM_BOX_COLOR) = range(10)
+# COctreeV3Header (sdf/scom/scom_shared.h:24-44), 60 B, and its constants (scom_shared.h:4-16)
+COCTREE_HEADER_DTYPE = np.dtype([(n, "<u4") for n in (
+    "brick_size", "brick_pad", "bits_per_value", "uv_size", "sim_compression", "lods", "default_leaf_type",
+    "fallback_leaf_type", "node_pack_mode", "uints_per_child_info", "idx_mask", "idx_sh", "trans_off", "rot_mask",
+    "add_mask")])
+COCTREE_NODE_PACK_MODE_DEFAULT, COCTREE_NODE_PACK_MODE_SIM_COMP_FULL, COCTREE_NODE_PACK_MODE_SIM_COMP_SMALL = 0, 1, 2
+COCTREE_LEAF_TYPE_NOT_A_LEAF, COCTREE_LEAF_TYPE_GRID, COCTREE_LEAF_TYPE_BIT_PACK, COCTREE_LEAF_TYPE_SLICES = 0, 1, 2, 3
+COCTREE_LOD_LEAF_TYPE_SHIFT = 30
+COCTREE_V3_VERSION = 5
+# per pack mode: uints_per_child_info, idx_mask, idx_sh, trans_off, rot_mask, add_mask (scom.h:27-61)
+COCTREE_PACK_MODE_WORDS = {0: (1, 0xFFFFFFFF, 0, 0, 0x0, 0x0), 1: (2, 0xFFFFFFFF, 0, 1, 0x3F, 0x3FFFFF00),
+                           2: (1, 0xFFFFC000, 14, 0, 0x3F, 0x3FC0)}
+COCT_GEOM_DTYPE = np.dtype([(n, "<u4") for n in (
+    "base", "mods", "size", "brick_size", "brick_pad", "bits_per_value", "uv_size", "sim_compression", "lods",
+    "uints_per_child_info", "idx_mask", "idx_sh", "trans_off", "rot_mask", "add_mask", "_pad")])
+
+
+def coctree_header(brick_size=1, brick_pad=0, bits_per_value=8, uv_size=0, sim_compression=0, lods=0,
+                   node_pack_mode=COCTREE_NODE_PACK_MODE_DEFAULT, default_leaf_type=COCTREE_LEAF_TYPE_SLICES,
+                   fallback_leaf_type=COCTREE_LEAF_TYPE_GRID) -> np.ndarray:
+    """A COctreeV3Header with the words node_pack_mode fixes filled in (fill_coctree_v3_header,
+    sdf/scom/scom.h:27-61)."""
+    h = np.zeros((), COCTREE_HEADER_DTYPE)
+    for k, v in dict(brick_size=brick_size, brick_pad=brick_pad, bits_per_value=bits_per_value, uv_size=uv_size,
+                     sim_compression=sim_compression, lods=lods, default_leaf_type=default_leaf_type,
+                     fallback_leaf_type=fallback_leaf_type, node_pack_mode=node_pack_mode).items():
+        h[k] = v
+    upc, idx_mask, idx_sh, trans_off, rot_mask, add_mask = COCTREE_PACK_MODE_WORDS[node_pack_mode]
+    h["uints_per_child_info"], h["idx_mask"], h["idx_sh"] = upc, idx_mask, idx_sh
+    h["trans_off"], h["rot_mask"], h["add_mask"] = trans_off, rot_mask, add_mask
+    return h
+
+
 SVS_NODE_DTYPE = np.dtype([("pos_xy", "<u4"), ("pos_z_lod_size", "<u4"), ("values", "<u4", (2,))])  # SdfSVSNode, 16 B
 SBS_NODE_DTYPE = np.dtype([("pos_xy", "<u4"), ("pos_z_lod_size", "<u4"), ("data_offset", "<u4"),
                            ("_pad", "<u4")])                                       # SdfSBSNode, 16 B
@@ -63,6 +96,7 @@ ARRAY_DTYPES = {
     "preset": PRESET_DTYPE, "sbs_mask": np.uint32, "orig_indices": np.uint32, "svs_nodes": SVS_NODE_DTYPE,
     "svs_roots": np.uint32, "tlas_parent": np.uint32, "grid_data": np.float32, "grid_offsets": np.uint32,
     "grid_sizes": np.uint32, "gprim_headers": GPRIM_HEADER_DTYPE, "gprim_points": np.float32,
+    "coct_data": np.uint32, "coct_geoms": COCT_GEOM_DTYPE, "coct_mods": np.int32,
 }
 LIGHT_DTYPE = np.dtype([("space", "<f4", (3,)), ("type", "<u4"), ("color", "<f4", (3,)), ("_pad", "<u4")])  # Light, 32 B
 LIGHT_TYPE_DIRECT, LIGHT_TYPE_POINT, LIGHT_TYPE_AMBIENT = 0, 1, 2
@@ -174,6 +208,9 @@ def lib() -> ctypes.CDLL:
             "lrt_add_sdf_grid": ([vp, pu32, vp, pu32], i32),
             "lrt_add_graphics_prims": ([vp, vp, vp, sz, pu32], i32),
             "lrt_load_sdf_grid": ([cp, pu32, ctypes.POINTER(_Buffer)], i32),
+            "lrt_add_coctree_v3": ([vp, vp, vp, sz, u32, pu32], i32),
+            "lrt_load_coctree_v3": ([cp, pu32, vp, ctypes.POINTER(_Buffer)], i32),
+            "lrt_save_coctree_v3": ([cp, vp, vp, sz], i32),
             "lrt_save_sdf_grid": ([cp, pu32, vp], i32),
             "lrt_load_sdf_svs": ([cp, ctypes.POINTER(_Buffer)], i32),
             "lrt_save_sdf_svs": ([cp, vp, sz], i32),
@@ -242,7 +279,8 @@ def lib() -> ctypes.CDLL:
 def exported_symbols():
     return ["lrt_last_error", "lrt_version", "lrt_create", "lrt_destroy", "lrt_clear_geom", "lrt_clear_scene",
             "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_svs", "lrt_load_sdf_svs", "lrt_save_sdf_svs",
-            "lrt_add_sdf_grid", "lrt_load_sdf_grid", "lrt_save_sdf_grid", "lrt_add_graphics_prims", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
+            "lrt_add_sdf_grid", "lrt_load_sdf_grid", "lrt_save_sdf_grid", "lrt_add_graphics_prims",
+            "lrt_add_coctree_v3", "lrt_load_coctree_v3", "lrt_save_coctree_v3", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
             "lrt_trace_closest_host", "lrt_trace_closest_noalloc_dev", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
             "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_set_ray_interleave", "lrt_set_persistent",
@@ -363,6 +401,15 @@ class SceneRT:
         out = ctypes.c_uint32()
         _check(lib().lrt_add_sdf_grid(self._h, size.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _ptr(data),
                                       ctypes.byref(out)))
+        return out.value
+
+    def AddGeom_COctreeV3(self, header: np.ndarray, data: np.ndarray, bvh_level: int = 0) -> int:
+        """BVHRT::AddGeom_COctreeV3 (BVH2Common_host.cpp:915-966): a COctreeV3Header record
+        (coctree_header()) and the octree's u32 words; bvh_level is BVHRT::preferredBVHLevel."""
+        h = np.ascontiguousarray(header, dtype=COCTREE_HEADER_DTYPE).reshape(())
+        d = np.ascontiguousarray(data, dtype=np.uint32).ravel()
+        out = ctypes.c_uint32()
+        _check(lib().lrt_add_coctree_v3(self._h, _ptr(h), _ptr(d), d.size, int(bvh_level), ctypes.byref(out)))
         return out.value
 
     def AddGeom_SdfSVS(self, nodes: np.ndarray) -> int:
@@ -763,6 +810,23 @@ def save_sdf_grid(path: str, data: np.ndarray):
     d = np.ascontiguousarray(data, dtype=np.float32)
     size = np.array([d.shape[2], d.shape[1], d.shape[0]], np.uint32)
     _check(lib().lrt_save_sdf_grid(os.fsencode(path), size.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _ptr(d)))
+
+
+def load_coctree_v3(path: str):
+    """load_coctree_v3 (sdf/scom/scom.cpp:97-111): (header, words, version); a version other than
+    5 is read all the same (the reference only warns)."""
+    h = np.zeros((), COCTREE_HEADER_DTYPE)
+    ver = ctypes.c_uint32()
+    b = _Buffer()
+    _check(lib().lrt_load_coctree_v3(os.fsencode(path), ctypes.byref(ver), _ptr(h), ctypes.byref(b)))
+    return h, _take(b, np.uint32), ver.value
+
+
+def save_coctree_v3(path: str, header: np.ndarray, data: np.ndarray):
+    """save_coctree_v3 (sdf/scom/scom.cpp:86-95), version 5."""
+    h = np.ascontiguousarray(header, dtype=COCTREE_HEADER_DTYPE).reshape(())
+    d = np.ascontiguousarray(data, dtype=np.uint32).ravel()
+    _check(lib().lrt_save_coctree_v3(os.fsencode(path), _ptr(h), _ptr(d), d.size))
 
 
 def load_sdf_svs(path: str) -> np.ndarray:

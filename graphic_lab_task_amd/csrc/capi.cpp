@@ -290,6 +290,18 @@ lrt_status lrt_add_sdf_grid(lrt_scene* s, const uint32_t size[3], const float* d
   });
 }
 
+lrt_status lrt_add_coctree_v3(lrt_scene* s, const uint32_t header[15], const uint32_t* data, size_t size,
+                              uint32_t bvh_level, uint32_t* out_geom_id) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !header || !data) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    lrt::COctreeV3Header h;
+    std::memcpy(&h, header, sizeof(h));
+    const uint32_t id = s->s.add_coctree_v3(h, data, size, bvh_level);
+    if (out_geom_id) *out_geom_id = id;
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_add_sdf_svs(lrt_scene* s, const void* nodes, size_t n_nodes, uint32_t* out_geom_id) {
   return guarded([&]() -> lrt_status {
     if (!s || !nodes) return fail(LRT_ERR_INVALID_ARG, "null argument");
@@ -1011,6 +1023,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
     ARR("svs_nodes", c.svsNodes) ARR("svs_roots", c.svsRoots) ARR("tlas_parent", c.tlasParent)
     ARR("grid_data", c.gridData) ARR("grid_offsets", c.gridOffsets) ARR("grid_sizes", c.gridSizes)
     ARR("gprim_headers", c.gprimHeaders) ARR("gprim_points", c.gprimPoints)
+    ARR("coct_data", c.coctData) ARR("coct_geoms", c.coctGeoms) ARR("coct_mods", c.coctMods)
 #undef ARR
     if (n == "preset") { src = &c.preset; sz = sizeof(c.preset); }
     std::vector<uint32_t> mask_copy;   // the committed occupancy masks live on the device only
@@ -1030,7 +1043,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
       static const char* known[] = {"nodes", "tlas", "instances", "geoms", "geom_tags", "start_end",
                                     "prim_id_count", "vert_pos", "vert_norm", "indices", "prim_indices",
                                     "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes", "sbs_data",
-                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices", "svs_nodes", "svs_roots", "tlas_parent", "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points",
+                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices", "svs_nodes", "svs_roots", "tlas_parent", "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points", "coct_data", "coct_geoms", "coct_mods",
                                     "sbs_mask"};
       bool ok = false;
       for (const char* k : known) ok = ok || n == k;

@@ -18,6 +18,7 @@ struct float2 { float x, y; };
 struct float3 { float x, y, z; };
 struct alignas(16) float4 { float x, y, z, w; };
 struct uint2 { uint32_t x, y; };
+struct alignas(16) int4 { int32_t x, y, z, w; };
 
 // Column-major 4x4 matrix, m[c*4 + r] (LiteMath float4x4, LiteMath.h:1931).
 struct alignas(16) float4x4 { float m[16]; };
@@ -99,6 +100,39 @@ LRT_HD uint32_t graphics_prim_len(uint32_t prim_type) {
   return len;
 }
 
+// COctreeV3Header (sdf/scom/scom_shared.h:24-44): 15 words, the layout of the file header and
+// of the ABI. The last six words follow from node_pack_mode (scom.h:27-61).
+struct COctreeV3Header {
+  uint32_t brick_size;       // voxels per brick edge
+  uint32_t brick_pad;        // extra voxels stored on each side of a brick
+  uint32_t bits_per_value;   // quantised distance width
+  uint32_t uv_size;          // 0: untextured, 1: 8 packed 16-bit (u, v) pairs per brick
+  uint32_t sim_compression;  // 1: leaves carry an additive distance shift
+  uint32_t lods;             // 1: every node carries a coarse leaf of its own
+  uint32_t default_leaf_type, fallback_leaf_type;
+  uint32_t node_pack_mode;
+  uint32_t uints_per_child_info, idx_mask, idx_sh, trans_off, rot_mask, add_mask;
+};
+static_assert(sizeof(COctreeV3Header) == 60, "COctreeV3Header layout");
+constexpr uint32_t COCTREE_V3_VERSION = 5;   // scom.h:88
+constexpr uint32_t COCTREE_NODE_PACK_MODE_DEFAULT = 0, COCTREE_NODE_PACK_MODE_SIM_COMP_FULL = 1,
+                   COCTREE_NODE_PACK_MODE_SIM_COMP_SMALL = 2;
+constexpr uint32_t COCTREE_LEAF_TYPE_NOT_A_LEAF = 0, COCTREE_LEAF_TYPE_GRID = 1, COCTREE_LEAF_TYPE_BIT_PACK = 2,
+                   COCTREE_LEAF_TYPE_SLICES = 3;
+constexpr uint32_t COCTREE_LEAF_TYPE_BITS = 2, COCTREE_LEAF_TYPE_MASK = 0x3, COCTREE_LOD_LEAF_TYPE_SHIFT = 30;
+constexpr uint32_t COCTREE_ROT_COUNT = 48;   // axis permutations with signs (BVH2Common.h:286)
+
+// One COctreeV3 geometry as the kernels read it (ours): the header fields the walk uses, where
+// its words start in the scene's COctreeV3 word array and where its 3 x 48 index modifiers start
+// (rotations of the value lattice, of the voxel lattice and of the 2^3 child lattice).
+struct CoctGeom {
+  uint32_t base, mods, size;
+  uint32_t brick_size, brick_pad, bits_per_value, uv_size, sim_compression, lods;
+  uint32_t uints_per_child_info, idx_mask, idx_sh, trans_off, rot_mask, add_mask;
+  uint32_t _pad;
+};
+static_assert(sizeof(CoctGeom) == 64, "CoctGeom is 64 B");
+
 // sdf/simple/sdf_simple_shared.h:39-45
 struct SdfSBSNode {
   uint32_t pos_xy;          // (x << 16) | y, brick position in its LOD grid
@@ -173,6 +207,7 @@ constexpr uint32_t TYPE_SDF_FRAME_OCTREE = 3;
 constexpr uint32_t TYPE_SDF_SVS          = 5;
 constexpr uint32_t TYPE_SDF_SBS          = 6;
 constexpr uint32_t TYPE_SDF_MULTI_OCTREE = 9;
+constexpr uint32_t TYPE_COCTREE_V3       = 20;
 constexpr uint32_t MULTI_OCTREE_FLAG_SURFACE = 1;          // sdf_multi_shared.h:5
 constexpr uint32_t MULTI_RENDER_MODE_MO_VOXEL_COUNT = 24;  // render_settings.h:85-86
 constexpr uint32_t MULTI_RENDER_MODE_MO_FLAGS = 25;
@@ -183,6 +218,7 @@ constexpr uint32_t GEOM_ID_MASK = 0x00FFFFFFu;
 
 constexpr uint32_t TAG_NONE       = 0;
 constexpr uint32_t TAG_TRIANGLE   = 1;
+constexpr uint32_t TAG_COCTREE_BRICKED = 2;
 constexpr uint32_t TAG_SDF_BRICK  = 4;
 constexpr uint32_t TAG_SDF_NODE   = 8;
 constexpr uint32_t TAG_SDF_GRID   = 11;

@@ -147,6 +147,7 @@ void Scene::clear_geom() {
   multiNodes.clear(); multiValues.clear();
   gridData.clear(); gridOffsets.clear(); gridSizes.clear();
   gprimHeaders.clear(); gprimPoints.clear();
+  coctData.clear(); coctGeoms.clear(); coctMods.clear();
   geom_dirty = true;
   clear_scene();
 }
@@ -590,6 +591,7 @@ uint32_t Scene::leaf_kinds() const {
     else if (t == TAG_SDF_OCTREE) k |= LK_MULTI;
     else if (t == TAG_SDF_GRID) k |= LK_GRID;
     else if (t == TAG_GRAPHICS_PRIM) k |= LK_GPRIM;
+    else if (t == TAG_COCTREE_BRICKED) k |= LK_COCT;
   }
   return k;
 }
@@ -835,6 +837,7 @@ hipError_t Scene::commit() {
   UP("sbs_mask_base", sbsMaskBase) UP("orig_indices", origIndices) UP("tlas_parent", tlasParent)
   UP("grid_data", gridData) UP("grid_offsets", gridOffsets) UP("grid_sizes", gridSizes)
   UP("gprim_headers", gprimHeaders) UP("gprim_points", gprimPoints)
+  UP("coct_data", coctData) UP("coct_geoms", coctGeoms) UP("coct_mods", coctMods)
   // MultiRenderer's own m_instanceTransformInvTransposed (MultiRenderer_host.cpp:870-878):
   // transpose(inverse4x4(matrix)), refreshed by UpdateInstance (unlike the BVHRT copy)
   std::vector<float4x4> inst_inv_t(instances.size());
@@ -900,6 +903,9 @@ SceneView Scene::make_view() const {
   v.grid_sizes = (const uint32_t*)P("grid_sizes");
   v.gprim_headers = (const GraphicsPrimHeader*)P("gprim_headers");
   v.gprim_points = (const float4*)P("gprim_points");
+  v.coct_data = (const uint32_t*)P("coct_data");
+  v.coct_geoms = (const CoctGeom*)P("coct_geoms");
+  v.coct_mods = (const int4*)P("coct_mods");
   v.preset = preset;
   v.lod_thr = lod_thr;
   v.leaf_kinds = leaf_kinds();

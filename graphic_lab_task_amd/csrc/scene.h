@@ -67,6 +67,9 @@ struct Scene {
   std::vector<uint32_t> gridOffsets, gridSizes;   // sizes: 3 words per grid
   std::vector<GraphicsPrimHeader> gprimHeaders;
   std::vector<float4> gprimPoints;
+  std::vector<uint32_t> coctData;    // words of every COctreeV3 geometry in turn
+  std::vector<CoctGeom> coctGeoms;
+  std::vector<int4> coctMods;        // 3 x 48 index modifiers per COctreeV3 geometry
   std::vector<InstanceData> instances;
   std::vector<BVHNode> tlas;
 
@@ -149,6 +152,7 @@ struct Scene {
   uint32_t add_graphics_prims(const GraphicsPrimHeader& header, const float4* points, size_t n_points);
   uint32_t add_sdf_sbs(const SdfSBSHeader& header, const SdfSBSNode* nodes, size_t n_nodes, const uint32_t* values,
                        size_t n_values, const float* values_f, size_t n_values_f);
+  uint32_t add_coctree_v3(const COctreeV3Header& h, const uint32_t* data, size_t size, uint32_t bvh_level);
   uint32_t add_sdf_multi_octree(const SdfMultiOctreeNode* nodes, size_t n_nodes, const float* values,
                                 size_t n_values);
   uint32_t add_instance(uint32_t geom_id, const float4x4& m);

@@ -20,6 +20,7 @@ enum LeafKind : uint32_t {
   LK_SBS_SMOOTH = 1u << 5,
   LK_GRID  = 1u << 6,   // TAG_SDF_GRID (dense SDF grid, sphere tracing)
   LK_GPRIM = 1u << 7,   // TAG_GRAPHICS_PRIM (points, lines, segments, arrows, wire boxes)
+  LK_COCT  = 1u << 8,   // TAG_COCTREE_BRICKED (compact octree COctreeV3)
 };
 
 // Frame-octree leaf as the kernels read it: the leaf box, its 8 corner distances and the
@@ -78,6 +79,10 @@ struct SceneView {
   // graphics primitives
   const GraphicsPrimHeader* gprim_headers; // m_GraphicsPrimHeaders
   const float4*             gprim_points;  // m_GraphicsPrimPoints
+  // compact octrees (COctreeV3)
+  const uint32_t*           coct_data;     // m_SdfCompactOctreeV3Data, every geometry's words in turn
+  const CoctGeom*           coct_geoms;    // per COctreeV3 geometry (GeomData.offset.x)
+  const int4*               coct_mods;     // m_SdfCompactOctreeRotModifiers, 3 x 48 per geometry
 
   MultiRenderPreset preset;
   float    lod_thr;                 // m_precomputedLoDThr

@@ -152,6 +152,34 @@ lrt_status lrt_save_sdf_grid(const char* path, const uint32_t size[3], const flo
   });
 }
 
+lrt_status lrt_load_coctree_v3(const char* path, uint32_t* version, uint32_t header[15], lrt_buffer* data) {
+  return io_guarded([&]() -> lrt_status {
+    if (!header || !data) throw std::runtime_error("null output");
+    File fl(path, "rb");
+    size_t left = size_t(fl.size());
+    uint32_t ver = 0;
+    take(fl, &ver, 4, left, "COctreeV3 version");
+    if (version) *version = ver;
+    take(fl, header, sizeof(lrt::COctreeV3Header), left, "COctreeV3 header");
+    std::vector<char> v = counted(fl, 4, left, "COctreeV3 words");
+    return to_buf(v, data);
+  });
+}
+
+lrt_status lrt_save_coctree_v3(const char* path, const uint32_t header[15], const uint32_t* data, size_t size) {
+  return io_guarded([&]() -> lrt_status {
+    if (!header || (!data && size)) throw std::runtime_error("null argument");
+    if (size > 0xFFFFFFFFull) throw std::runtime_error("too many words for a u32 count");
+    const uint32_t ver = lrt::COCTREE_V3_VERSION, n = uint32_t(size);
+    File fl(path, "wb");
+    put(fl, &ver, 4);
+    put(fl, header, sizeof(lrt::COctreeV3Header));
+    put(fl, &n, 4);
+    put(fl, data, size * 4);
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_load_sdf_sbs(const char* path, void* header, lrt_buffer* nodes, lrt_buffer* values,
                             lrt_buffer* values_f) {
   return io_guarded([&]() -> lrt_status {

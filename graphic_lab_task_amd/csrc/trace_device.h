@@ -1209,3 +1209,307 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g,
 }
 
 }  // namespace lrt
+
+namespace lrt {
+
+// ---- compact octree COctreeV3 (BVHRT::COctreeIntersectV3, MultiRenderer/BVH2Common.cpp:1637-2173) ----
+
+// COctreeV3_LoadDistanceValues* (BVH2Common.cpp:1637-1851): the 8 corner values of voxel
+// (vx, vy, vz) of the brick at word `o`, decoded through the leaf's rotation (index modifiers M)
+// and its additive shift. Returns the smallest value (grid leaves), -1 (a present voxel of a
+// bit-pack or slice leaf) or 1e6 (an absent voxel). A voxel outside the padded brick (only the
+// smoothed normal asks for one, and only without padding, where the reference reads outside
+// the brick) counts as absent.
+__device__ __forceinline__ float coct_load_values(const uint32_t* D, const int4* M, const CoctGeom& h,
+                                                  uint32_t leaf_type, uint32_t o, int vx, int vy, int vz,
+                                                  uint32_t v_size, float sz_inv, uint32_t code, float values[8]) {
+  const int pad = int(h.brick_pad);
+  const int p_size = int(h.brick_size + 2 * h.brick_pad);
+  if (vx + pad < 0 || vy + pad < 0 || vz + pad < 0 || vx + pad >= p_size || vy + pad >= p_size || vz + pad >= p_size)
+    return 1e6f;
+  if (leaf_type < COCTREE_LEAF_TYPE_GRID || leaf_type > COCTREE_LEAF_TYPE_SLICES) return 1e6f;
+  const uint32_t rotIdx = code & h.rot_mask;
+  const int4 mv = M[rotIdx];
+  const uint32_t bits = h.bits_per_value;
+  const uint32_t vals_per_int = 32u / bits;
+  const uint32_t max_val = bits == 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+  const float add_transform =
+      h.sim_compression > 0
+          ? 2.0f * 1.73205081f * sz_inv * (2.0f * (float(code & h.add_mask) / float(h.add_mask)) - 1.0f)
+          : 0.0f;
+  uint32_t off_3, off_5;
+  uint32_t off_1 = 0, off_2 = 0, slice_uints = 0;
+  if (leaf_type == COCTREE_LEAF_TYPE_GRID) {
+    off_3 = 0;
+  } else {
+    const uint32_t pfs = (uint32_t(p_size * p_size * p_size) + 31u) / 32u;
+    const int4 mp = M[COCTREE_ROT_COUNT + rotIdx];
+    const uint32_t pflag = uint32_t(mp.x * (vx + pad) + mp.y * (vy + pad) + mp.z * (vz + pad) + mp.w);
+    if ((D[o + pflag / 32u] & (1u << (pflag % 32u))) == 0u) return 1e6f;
+    off_1 = pfs;
+    if (leaf_type == COCTREE_LEAF_TYPE_BIT_PACK) {
+      off_3 = off_1 + (v_size * v_size * v_size + 31u) / 32u;
+    } else {
+      slice_uints = (v_size * v_size + 31u) / 32u;
+      off_2 = off_1 + v_size * slice_uints;
+      off_3 = off_2 + (v_size + 1u) / 2u;
+    }
+  }
+  off_5 = off_3 + 2u + 8u * h.uv_size;
+  const float min_val = -float(D[o + off_3]) / float(0xFFFFFFFFu) + add_transform;
+  const float range = (float(D[o + off_3 + 1]) / float(0xFFFFFFFFu)) / float(max_val);
+  float vmin = 1e6f;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int cx = vx + pad + ((i & 4) >> 2), cy = vy + pad + ((i & 2) >> 1), cz = vz + pad + (i & 1);
+    const uint32_t lid = uint32_t(mv.x * cx + mv.y * cy + mv.z * cz + mv.w);
+    uint32_t vId;
+    if (leaf_type == COCTREE_LEAF_TYPE_GRID) {
+      vId = lid;
+    } else {
+      uint32_t localId = lid, base = 0, w = off_1;
+      if (leaf_type == COCTREE_LEAF_TYPE_SLICES) {
+        const uint32_t sliceId = lid / (v_size * v_size);
+        localId = lid % (v_size * v_size);
+        w = off_1 + slice_uints * sliceId;
+        base = (D[o + off_2 + sliceId / 2u] >> (16u * (sliceId % 2u))) & 0xFFFFu;
+      }
+      uint32_t b0 = D[o + w];
+      b0 = localId > 31u ? b0 : (b0 & ((1u << localId) - 1u));
+      const uint32_t b1 = localId > 32u ? (D[o + w + 1] & ((1u << ((localId - 32u) & 31u)) - 1u)) : 0u;
+      vId = base + uint32_t(__popc(b0)) + uint32_t(__popc(b1));
+    }
+    const uint32_t dist = (D[o + off_5 + vId / vals_per_int] >> (bits * (vId % vals_per_int))) & max_val;
+    values[i] = min_val + range * float(dist);
+    vmin = fmin_ref(values[i], vmin);
+  }
+  return leaf_type == COCTREE_LEAF_TYPE_GRID ? vmin : -1.0f;
+}
+
+// COctreeV3_BrickIntersect (BVH2Common.cpp:1853-2003): marches the voxels of one leaf brick
+// (position p at level size sz) along the ray; intersects voxels with surface like a frame-octree
+// node (the hit reports TYPE_SDF_FRAME_OCTREE and the brick's word offset as primId), then the
+// smoothed normal and the texture coordinates. The march ends after 4 bs + 8 voxels, which only
+// cuts marches whose 1e-5 advance is lost to rounding (the reference would not end them).
+__device__ __forceinline__ void coct_brick_intersect(const SceneView& s, const uint32_t* D, const int4* M,
+                                                     const CoctGeom& h, float3 ray_pos, float3 ray_dir, float tNear,
+                                                     uint32_t instId, uint32_t geomId, uint32_t o, float3 p, float sz,
+                                                     uint32_t info, CRT_Hit& hit) {
+  float values[8];
+  const uint32_t v_size = h.brick_size + 2 * h.brick_pad + 1;
+  const uint32_t leaf_type = info & COCTREE_LEAF_TYPE_MASK;
+  const uint32_t code = info >> COCTREE_LEAF_TYPE_BITS;
+  const float bs = float(h.brick_size);
+  const float sz_inv = 2.0f / sz;
+  const float d = 2.0f / (sz * bs);
+  const float3 brick_min_pos = make3(-1, -1, -1) + sz_inv * p;
+  const float3 brick_max_pos = brick_min_pos + sz_inv * make3(1, 1, 1);
+  const float3 inv = safe_inverse(ray_dir);
+  float2 bnf = ray_box2(ray_pos, inv, brick_min_pos, brick_max_pos);
+  bnf.x = fmax_ref(tNear, bnf.x);
+  const float old_t = hit.t;
+  const float scale = 0.5f * sz * bs;
+  const float hi = bs - 1e-6f;
+  const uint32_t max_steps = 4 * h.brick_size + 8;
+  uint32_t steps = 0;
+  while (bnf.x < bnf.y && hit.t == old_t && steps < max_steps) {
+    steps++;
+    const float3 hit_pos = ray_pos + bnf.x * ray_dir;
+    const float3 local_pos = (hit_pos - brick_min_pos) * scale;
+    const float3 voxelPos = make3(floorf(clamp_ref(local_pos.x, 1e-6f, hi)), floorf(clamp_ref(local_pos.y, 1e-6f, hi)),
+                                  floorf(clamp_ref(local_pos.z, 1e-6f, hi)));
+    const float3 min_pos = brick_min_pos + d * voxelPos;
+    const float3 max_pos = min_pos + d * make3(1, 1, 1);
+    const int vx = int(voxelPos.x), vy = int(voxelPos.y), vz = int(voxelPos.z);
+    const float vmin = coct_load_values(D, M, h, leaf_type, o, vx, vy, vz, v_size, sz_inv, code, values);
+    const float2 fNearFar = ray_box2(ray_pos, inv, min_pos, max_pos);
+    if (tNear < fNearFar.x && vmin <= 0.0f) {
+      const float3 start_pos = ray_pos + fNearFar.x * ray_dir;
+      const float3 start_q = (start_pos - min_pos) * scale;
+      const float qFar = (fNearFar.y - fNearFar.x) * scale;
+      local_surface_intersection(s, TYPE_SDF_FRAME_OCTREE, ray_dir, instId, geomId, values, o, d, 0.0f, qFar, fNearFar,
+                                 start_q, hit);
+      if (s.preset.normal_mode == NORMAL_MODE_SDF_SMOOTHED && preset_needs_normal(s.preset.render_mode) &&
+          hit.t < old_t) {
+        const float beta = 0.5f;
+        const float3 dp = start_q + ((hit.t - fNearFar.x) / d) * ray_dir;
+        float3 normals[8];
+        float values_n[8];
+        float3 nmq;
+        nmq.x = dp.x < 0.5f ? step_ref(-beta, beta, dp.x) : step_ref(-beta, beta, dp.x - 1.0f);
+        nmq.y = dp.y < 0.5f ? step_ref(-beta, beta, dp.y) : step_ref(-beta, beta, dp.y - 1.0f);
+        nmq.z = dp.z < 0.5f ? step_ref(-beta, beta, dp.z) : step_ref(-beta, beta, dp.z - 1.0f);
+        const int x0 = dp.x < 0.5f ? vx - 1 : vx, y0 = dp.y < 0.5f ? vy - 1 : vy, z0 = dp.z < 0.5f ? vz - 1 : vz;
+        for (int i = 0; i < 8; i++) {
+          normals[i] = make3(0, 0, 0);
+          const int xi = x0 + ((i & 4) >> 2), yi = y0 + ((i & 2) >> 1), zi = z0 + (i & 1);
+          const float3 dVoxelPos = make3(float(xi), float(yi), float(zi)) - voxelPos;
+          const float vmin_n = coct_load_values(D, M, h, leaf_type, o, xi, yi, zi, v_size, sz_inv, code, values_n);
+          if (vmin_n <= 0.0f) normals[i] = normalize3(eval_dist_trilinear_diff(values_n, dp - dVoxelPos));
+        }
+        float3 sm = ((1 - nmq.x) * (1 - nmq.y) * (1 - nmq.z)) * normals[0];
+        sm = sm + ((1 - nmq.x) * (1 - nmq.y) * (nmq.z)) * normals[1];
+        sm = sm + ((1 - nmq.x) * (nmq.y) * (1 - nmq.z)) * normals[2];
+        sm = sm + ((1 - nmq.x) * (nmq.y) * (nmq.z)) * normals[3];
+        sm = sm + ((nmq.x) * (1 - nmq.y) * (1 - nmq.z)) * normals[4];
+        sm = sm + ((nmq.x) * (1 - nmq.y) * (nmq.z)) * normals[5];
+        sm = sm + ((nmq.x) * (nmq.y) * (1 - nmq.z)) * normals[6];
+        sm = sm + ((nmq.x) * (nmq.y) * (nmq.z)) * normals[7];
+        float m[16];
+        load_mat(s.instances[instId].transformInvTransposed, m);
+        const float2 e = encode_normal(normalize3(matmul4x3(m, sm)));
+        hit.coords[2] = e.x;
+        hit.coords[3] = e.y;
+      }
+    }
+    bnf.x += fmax_ref(0.0f, fNearFar.y - bnf.x) + 1e-5f;
+  }
+  if (hit.t < old_t && h.uv_size > 0) {
+    // texture coordinates sit where a slice leaf keeps them, whatever the leaf type
+    const uint32_t p_size = h.brick_size + 2 * h.brick_pad;
+    const uint32_t slice_uints = (v_size * v_size + 31u) / 32u;
+    const uint32_t off_4 = (p_size * p_size * p_size + 31u) / 32u + v_size * slice_uints + (v_size + 1u) / 2u + 2u;
+    const float3 pos = ray_pos + hit.t * ray_dir;
+    const float3 dp = (pos - brick_min_pos) * (0.5f * sz);
+    const float w[8] = {(1 - dp.x) * (1 - dp.y) * (1 - dp.z), (1 - dp.x) * (1 - dp.y) * (dp.z),
+                        (1 - dp.x) * (dp.y) * (1 - dp.z),     (1 - dp.x) * (dp.y) * (dp.z),
+                        (dp.x) * (1 - dp.y) * (1 - dp.z),     (dp.x) * (1 - dp.y) * (dp.z),
+                        (dp.x) * (dp.y) * (1 - dp.z),         (dp.x) * (dp.y) * (dp.z)};
+    float u = 0, v = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const uint32_t c = D[o + off_4 + i];
+      const float tu = w[i] * (1.5259022e-5f * float(c >> 16));
+      const float tv = w[i] * (1.5259022e-5f * float(c & 0xFFFF));
+      u = (i == 0) ? tu : u + tu;
+      v = (i == 0) ? tv : v + tv;
+    }
+    hit.coords[0] = u;
+    hit.coords[1] = v;
+  }
+}
+
+// COctreeIntersectV3 (BVH2Common.cpp:2005-2173): front-to-back walk of the octree below the BLAS
+// box `bvhNodeId` (its node, level size 2 / box width and position), with mirrored coordinates
+// (octant mask a), optional LOD leaves and the first hitting leaf ending the walk. Stack entries
+// are the reference's (node, info, packed position and level size); entries past 32 are dropped.
+__device__ __forceinline__ void coct_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir,
+                                            float tNear, uint32_t instId, uint32_t geomId, uint32_t bvhNodeId,
+                                            CRT_Hit& hit) {
+  const CoctGeom h = s.coct_geoms[g.offset.x];
+  const uint32_t* D = s.coct_data + h.base;
+  const int4* M = s.coct_mods + h.mods;
+  float3 pos_ray_pos = ray_pos;
+  float3 pos_ray_dir = ray_dir;
+  uint32_t a = 0;
+  if (ray_dir.x < 0) { pos_ray_pos.x *= -1; pos_ray_dir.x *= -1; a |= 4; }
+  if (ray_dir.y < 0) { pos_ray_pos.y *= -1; pos_ray_dir.y *= -1; a |= 2; }
+  if (ray_dir.z < 0) { pos_ray_pos.z *= -1; pos_ray_dir.z *= -1; a |= 1; }
+  const BVHNode box = load_node(s.orig_nodes + g.offset.y + bvhNodeId);
+  const float3 min_pos = box.boxMin;
+  const float3 max_pos = box.boxMax;
+  const uint32_t start_sz = uint32_t(2.0f / (max_pos.x - min_pos.x));
+  const float3 sp = float(start_sz) * 0.5f * (min_pos - make3(-1, -1, -1));
+  const uint32_t spx = uint32_t(sp.x), spy = uint32_t(sp.y), spz = uint32_t(sp.z);
+  uint32_t p_mask = start_sz - 1;
+  const uint32_t sx = (a & 4) ? (~spx & p_mask) : spx;
+  const uint32_t sy = (a & 2) ? (~spy & p_mask) : spy;
+  const uint32_t sz0 = (a & 1) ? (~spz & p_mask) : spz;
+  pos_ray_pos = -1.0f * pos_ray_pos;
+  const float3 inv = safe_inverse(pos_ray_dir);
+  const float3 t0_ = pos_ray_pos * inv - inv;
+  const float3 t1_ = pos_ray_pos * inv + inv;
+  const float3 l_ = t1_ - t0_;
+  // nn_indices of the reference, packed as x | y << 4 | z << 8
+  const uint32_t nn[8] = {0x124u, 0x835u, 0x386u, 0x887u, 0x568u, 0x878u, 0x788u, 0x888u};
+  constexpr int COCT_STACK = 32;
+  uint32_t st_node[COCT_STACK], st_info[COCT_STACK], st_pxy[COCT_STACK], st_pzs[COCT_STACK];
+  int top = 0;
+  const float old_t = hit.t;
+  st_node[0] = box.leftOffset;
+  st_info[0] = 0;
+  st_pxy[0] = (sx << 16) | sy;
+  st_pzs[0] = (sz0 << 16) | start_sz;
+  while (top >= 0) {
+    uint32_t nodeId = st_node[top], info = st_info[top];
+    const uint32_t pxy = st_pxy[top], pzs = st_pzs[top];
+    const uint32_t level_sz = pzs & 0xFFFFu;
+    const uint32_t px = pxy >> 16, py = pxy & 0xFFFFu, pz = pzs >> 16;
+    const float d = 1.0f / float(level_sz);
+    const float3 p_f = make3(float(px), float(py), float(pz));
+    const float3 t0 = t0_ + d * p_f * l_;
+    const float3 t1 = t0_ + d * (p_f + 1.0f) * l_;
+    const float t_out = fmin_ref(fmin_ref(t1.x, t1.y), t1.z);
+    if (t_out < tNear) {
+      top--;
+      continue;
+    }
+    bool is_leaf = (info & COCTREE_LEAF_TYPE_MASK) != COCTREE_LEAF_TYPE_NOT_A_LEAF;
+    if (h.lods > 0 && !is_leaf) {
+      const float t = s.preset.fixed_lod > 0 ? 1.0f : fmax_ref(t0.x, fmax_ref(t0.y, t0.z));
+      const float target_lod_size = (powf(2.0f, s.preset.level_of_detail) - 0.01f) / t;
+      if (float(level_sz * h.brick_size) > target_lod_size) {
+        const uint32_t lodLeafId = D[nodeId + 1];
+        const uint32_t type = (D[nodeId] >> COCTREE_LOD_LEAF_TYPE_SHIFT) & COCTREE_LEAF_TYPE_MASK;
+        const uint32_t infoOffset = nodeId + 1 + h.trans_off;
+        info = (D[infoOffset] << COCTREE_LEAF_TYPE_BITS) | type;
+        nodeId = lodLeafId;
+        is_leaf = true;
+        if (type == COCTREE_LEAF_TYPE_NOT_A_LEAF) {
+          top--;
+          continue;
+        }
+      }
+    }
+    if (is_leaf) {
+      p_mask = level_sz - 1;
+      const uint32_t rx = (a & 4) ? (~px & p_mask) : px;
+      const uint32_t ry = (a & 2) ? (~py & p_mask) : py;
+      const uint32_t rz = (a & 1) ? (~pz & p_mask) : pz;
+      const uint32_t leafOffset = (nodeId & h.idx_mask) >> h.idx_sh;
+      coct_brick_intersect(s, D, M, h, ray_pos, ray_dir, tNear, instId, geomId, leafOffset,
+                           make3(float(rx), float(ry), float(rz)), float(level_sz), info, hit);
+      if (hit.t < old_t) {
+        top = -1;
+        if (s.preset.render_mode == MULTI_RENDER_MODE_LOD) hit.primId = uint32_t(log2f(float(level_sz * h.brick_size)));
+      } else {
+        top--;
+      }
+    } else {
+      uint32_t b_node[4], b_info[4], b_pxy[4], b_pzs[4];
+      int buf_top = 0;
+      const float3 tm = 0.5f * (t0 + t1);
+      int curr = ot_first_node(t0, tm);
+      const uint32_t childrenInfo = D[nodeId];
+      do {
+        const uint32_t childNode = uint32_t(curr) ^ a;
+        if ((childrenInfo & (1u << childNode)) != 0u && buf_top < 4) {
+          const uint32_t localChildPos = h.lods + uint32_t(__popc(childrenInfo & ((1u << childNode) - 1u)));
+          const uint32_t childPos = 1 + h.uints_per_child_info * localChildPos;
+          const uint32_t childType = (childrenInfo >> (8 + COCTREE_LEAF_TYPE_BITS * childNode)) & COCTREE_LEAF_TYPE_MASK;
+          b_node[buf_top] = D[nodeId + childPos];
+          b_info[buf_top] = (D[nodeId + childPos + h.trans_off] << COCTREE_LEAF_TYPE_BITS) | childType;
+          b_pxy[buf_top] = (pxy << 1) | (((uint32_t(curr) & 4u) << 14) | ((uint32_t(curr) & 2u) >> 1));
+          b_pzs[buf_top] = (pzs << 1) | ((uint32_t(curr) & 1u) << 16);
+          buf_top++;
+        }
+        const uint32_t ni = nn[curr];
+        curr = ot_new_node((curr & 4) ? t1.x : tm.x, int(ni & 0xF), (curr & 2) ? t1.y : tm.y, int((ni >> 4) & 0xF),
+                           (curr & 1) ? t1.z : tm.z, int(ni >> 8));
+      } while (curr < 8);
+      for (int i = 0; i < buf_top; i++) {
+        const int k = top + i;
+        if (k < COCT_STACK) {
+          st_node[k] = b_node[buf_top - i - 1];
+          st_info[k] = b_info[buf_top - i - 1];
+          st_pxy[k] = b_pxy[buf_top - i - 1];
+          st_pzs[k] = b_pzs[buf_top - i - 1];
+        }
+      }
+      top += buf_top - 1;
+      if (top >= COCT_STACK) top = COCT_STACK - 1;
+    }
+  }
+}
+
+}  // namespace lrt

@@ -83,6 +83,19 @@ lrt_status lrt_add_sdf_frame_octree(lrt_scene* s, const void* nodes, size_t n_no
 lrt_status lrt_add_graphics_prims(lrt_scene* s, const void* header, const float* points, size_t n_points,
                                   uint32_t* out_geom_id);
 
+/* Replaces BVHRT::AddGeom_COctreeV3 (BVH2Common_host.cpp:915-966). header: COctreeV3Header
+ * (15 u32, sdf/scom/scom_shared.h:24-44) whose last six words match node_pack_mode (scom.h:27-61);
+ * data: the octree's `size` words (node 0 the root). bvh_level: deepest level whose nodes become
+ * BLAS boxes (BVHRT::preferredBVHLevel, 0 by default: one box for the whole octree). The octree
+ * is checked first: every word any ray can make the walk read must lie inside `data` (child and
+ * leaf offsets, brick flags, slice offsets, values and texture coordinates), at most 15 levels,
+ * leaf rotations below 48, bit-pack bricks of at most 64 values and slices of at most 64 values;
+ * LOD leaves take bvh_level 0. A failed check returns LRT_ERR_INVALID_ARG with the reason.
+ * Hits report TYPE_SDF_FRAME_OCTREE and the leaf brick's word offset as primId, as the reference.
+ * Unlike the reference, which keeps one such octree per scene, several may be added. */
+lrt_status lrt_add_coctree_v3(lrt_scene* s, const uint32_t header[15], const uint32_t* data, size_t size,
+                              uint32_t bvh_level, uint32_t* out_geom_id);
+
 /* Replaces BVHRT::AddGeom_SdfGrid (BVH2Common_host.cpp:466-499): a dense grid of distances over
  * the unit cube, size[0] * size[1] * size[2] floats with x fastest (fewer than 2^28). */
 lrt_status lrt_add_sdf_grid(lrt_scene* s, const uint32_t size[3], const float* data, uint32_t* out_geom_id);

@@ -92,6 +92,12 @@ lrt_status lrt_save_sdf_frame_octree(const char* path, const void* nodes, size_t
 lrt_status lrt_load_sdf_grid(const char* path, uint32_t size[3], lrt_buffer* data);
 lrt_status lrt_save_sdf_grid(const char* path, const uint32_t size[3], const float* data);
 
+/* load_coctree_v3 / save_coctree_v3 (sdf/scom/scom.cpp:86-111): u32 version (5 when written
+ * here; another version is read all the same and returned in *version, as the reference only
+ * warns), the 15-word COctreeV3Header, u32 count, then count u32 words. */
+lrt_status lrt_load_coctree_v3(const char* path, uint32_t* version, uint32_t header[15], lrt_buffer* data);
+lrt_status lrt_save_coctree_v3(const char* path, const uint32_t header[15], const uint32_t* data, size_t size);
+
 /* load_sdf_SVS / save_sdf_SVS (sdf/simple/sdf_simple.cpp:69-94): u32 count, then count SdfSVSNode. */
 lrt_status lrt_load_sdf_svs(const char* path, lrt_buffer* nodes);
 lrt_status lrt_save_sdf_svs(const char* path, const void* nodes, size_t n_nodes);

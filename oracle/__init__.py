@@ -25,6 +25,16 @@ Function -> reference -> how it is pinned (tests/test_oracle_cpu.py unless state
       property (a bijection onto [0, w*h)).
   oracle_camera_inverse LiteMath.h:3315-3360, 2070-2154  pinned by M * inverse(M) = I in
       float64 and the closed-form perspective entries.
+  oracle_trace_closest  COctreeV3 leaves: BVH2Common.cpp:1637-2173 (tests/test_coctree_cpu.py)
+      loaders pinned by round trips of an independent encoder of sphere-union octrees (every leaf
+      type, value width, padding, pack mode, rotation and additive shift) against the source
+      values, presence flags against their definition; the walk by the identical hits of the
+      three leaf types of one octree, by agreement with the frame octree of the same octree and
+      the brick set of the same bricks, and by rotation/shift invariance; the rotation modifiers
+      by the reference's own search (restated in numpy); the BLAS boxes by a restatement of
+      add_border_nodes_rec; the reference's engine-scene COctreeV3 file by its structure and by
+      hit points on the zero set of its decoded distances.
+  oracle_coct_voxel     BVH2Common.cpp:1637-1851   the loaders alone (see above)
   oracle_local_surface  BVH2Common.cpp:345-725 (one voxel)  pinned by float64 root finding on
       the trilinear polynomial along the ray (first sign change), for every intersection mode
       (tests/test_oracle_sdf.py).
@@ -49,7 +59,8 @@ COUNTS_DTYPE = np.dtype([("blas_nodes", "<u4"), ("leaves", "<u4"), ("prims", "<u
 _ARRAYS = ["nodes", "tlas", "instances", "geoms", "geom_tags", "start_end", "prim_id_count", "vert_pos",
            "vert_norm", "indices", "prim_indices", "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes",
            "sbs_data", "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "svs_nodes", "svs_roots",
-           "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points"]
+           "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points", "coct_data", "coct_geoms",
+           "coct_mods"]
 
 
 class OracleScene(ctypes.Structure):
@@ -90,6 +101,8 @@ def load(flavor: str = "exact") -> ctypes.CDLL:
         L.oracle_inverse4x4.restype = None
         L.oracle_local_surface.argtypes = [vp, vp, f32, f32, f32, f32, vp, vp, vp]
         L.oracle_local_surface.restype = i32
+        L.oracle_coct_voxel.argtypes = [ctypes.POINTER(OracleScene), u32, u32, u32, i32, i32, i32, f32, u32, vp]
+        L.oracle_coct_voxel.restype = f32
         _libs[flavor] = L
     return _libs[flavor]
 

@@ -100,6 +100,13 @@ struct SBSHeader { uint32_t brick_size, brick_pad, bytes_per_value, aux_data; };
 struct MultiNode { uint32_t data_offset, children_offset, voxel_count, flags; };   // sdf_multi_shared.h:8-14
 struct SVSNode { uint32_t pos_xy, pos_z_lod_size, values[2]; };                    // sdf_simple_shared.h:30-35
 struct GPrimHeader { float color[3]; uint32_t prim_type; };                         // graphics_primitive_common.h:19-23
+// One COctreeV3 geometry: where its words and 3 x 48 index modifiers start, and the
+// COctreeV3Header fields the walk reads (sdf/scom/scom_shared.h:24-44).
+struct CoctG {
+  uint32_t base, mods, size, brick_size, brick_pad, bits_per_value, uv_size, sim_compression, lods;
+  uint32_t uints_per_child_info, idx_mask, idx_sh, trans_off, rot_mask, add_mask, pad;
+};
+struct I4 { int32_t x, y, z, w; };
 struct Preset {
   uint32_t render_mode, sdf_node_intersect, normal_mode, ray_gen_mode, interpolation_mode, representation_mode, spp,
       fixed_lod;
@@ -113,6 +120,7 @@ constexpr uint32_t LEAF_BIT = 0x80000000u, START_MASK = 0x0FFFFFFFu, SIZE_MASK =
 constexpr int STACK_SIZE = 80;
 constexpr uint32_t TAG_NONE = 0, TAG_TRIANGLE = 1, TAG_SDF_BRICK = 4, TAG_SDF_NODE = 8, TAG_SDF_OCTREE = 15;
 constexpr uint32_t TAG_SDF_GRID = 11, TYPE_SDF_GRID = 1, TAG_GRAPHICS_PRIM = 9, TYPE_GRAPHICS_PRIM = 17;
+constexpr uint32_t TAG_COCTREE_BRICKED = 2;
 // GraphicsPrimType (utils/misc/graphics_primitive_common.h:7-16)
 constexpr uint32_t GP_POINT = 0, GP_LINE = 1, GP_SEGMENT = 2, GP_SEGMENT_DIR = 3, GP_BOX = 4, GP_POINT_COLOR = 5,
                    GP_LINE_COLOR = 6, GP_SEGMENT_COLOR = 7, GP_SEGMENT_DIR_COLOR = 8, GP_BOX_COLOR = 9;
@@ -154,6 +162,9 @@ struct OracleScene {
   const uint32_t* grid_sizes;     // 3 per grid
   const GPrimHeader* gprim_headers;
   const f4* gprim_points;
+  const uint32_t* coct_data;      // m_SdfCompactOctreeV3Data of every COctreeV3 geometry in turn
+  const CoctG* coct_geoms;
+  const I4* coct_mods;            // m_SdfCompactOctreeRotModifiers, 3 x 48 per geometry
   Preset preset;
 };
 
@@ -1004,6 +1015,281 @@ struct Tracer {
     }
   }
 
+  // ---- COctreeV3 (BVH2Common.cpp:1637-2173) ----
+  // COctreeV3_LoadDistanceValues{LeafGrid,LeafBitPack,LeafSlices} (BVH2Common.cpp:1637-1851). A
+  // voxel outside the padded brick (asked for by the smoothed normal without padding, where the
+  // reference reads outside the brick) counts as absent.
+  float COctreeV3_LoadDistanceValues(const CoctG& h, uint32_t leafType, uint32_t brickOffset, int vx, int vy, int vz,
+                                     uint32_t v_size, float sz_inv, uint32_t transform_code, float values[8]) const {
+    const uint32_t* data = s.coct_data + h.base;
+    const I4* mods = s.coct_mods + h.mods;
+    const int pad = int(h.brick_pad), psz = int(h.brick_size + 2 * h.brick_pad);
+    if (vx + pad < 0 || vy + pad < 0 || vz + pad < 0 || vx + pad >= psz || vy + pad >= psz || vz + pad >= psz)
+      return 1e6f;
+    auto dot4 = [](const I4& m, int x, int y, int z) { return m.x * x + m.y * y + m.z * z + m.w * 1; };
+    const uint32_t rotIdx = transform_code & h.rot_mask;
+    const uint32_t vals_per_int = 32 / h.bits_per_value;
+    const uint32_t bits = h.bits_per_value;
+    const uint32_t max_val = h.bits_per_value == 32 ? 0xFFFFFFFF : ((1u << bits) - 1);
+    const float add_transform =
+        h.sim_compression > 0
+            ? 2.0f * 1.73205081f * sz_inv * (2.0f * (float(transform_code & h.add_mask) / float(h.add_mask)) - 1.0f)
+            : 0.0f;
+    if (leafType == 1) {   // COCTREE_LEAF_TYPE_GRID
+      const unsigned off_3 = 0, off_4 = off_3 + 2, off_5 = off_4 + 8 * h.uv_size;
+      const float min_val = -float(data[brickOffset + off_3 + 0]) / float(0xFFFFFFFFu) + add_transform;
+      const float range = (float(data[brickOffset + off_3 + 1]) / float(0xFFFFFFFFu)) / max_val;
+      const I4 m = mods[rotIdx];
+      const uint32_t vId0 = uint32_t(dot4(m, vx + pad, vy + pad, vz + pad));
+      const uint32_t ids[8] = {vId0,
+                               vId0 + m.z,
+                               vId0 + m.y,
+                               vId0 + m.y + m.z,
+                               vId0 + m.x,
+                               vId0 + m.x + m.z,
+                               vId0 + m.x + m.y,
+                               vId0 + m.x + m.y + m.z};
+      float vmin = 1e6f;
+      for (int i = 0; i < 8; i++) {
+        const uint32_t vId = ids[i];
+        const uint32_t dist = (data[brickOffset + off_5 + vId / vals_per_int] >> (bits * (vId % vals_per_int))) & max_val;
+        values[i] = min_val + range * dist;
+        vmin = std::min(values[i], vmin);
+      }
+      return vmin;
+    }
+    if (leafType != 2 && leafType != 3) return 1e6f;
+    const uint32_t p_size = h.brick_size + 2 * h.brick_pad;
+    const uint32_t PFlagPos = uint32_t(dot4(mods[48 + rotIdx], vx + pad, vy + pad, vz + pad));
+    if ((data[brickOffset + PFlagPos / 32] & (1u << (PFlagPos % 32))) == 0) return 1e6f;
+    const uint32_t presence_flags_size_uints = (p_size * p_size * p_size + 32 - 1) / 32;
+    const bool slices = leafType == 3;
+    const uint32_t slice_distance_flags_uints = (v_size * v_size + 32 - 1) / 32;
+    const uint32_t distance_flags_size_uints =
+        slices ? v_size * slice_distance_flags_uints : (v_size * v_size * v_size + 32 - 1) / 32;
+    const uint32_t distance_offsets_size_uints = slices ? (v_size + 2 - 1) / 2 : 0;
+    const unsigned off_0 = 0, off_1 = off_0 + presence_flags_size_uints, off_2 = off_1 + distance_flags_size_uints;
+    const unsigned off_3 = off_2 + distance_offsets_size_uints, off_4 = off_3 + 2, off_5 = off_4 + 8 * h.uv_size;
+    const float min_val = -float(data[brickOffset + off_3 + 0]) / float(0xFFFFFFFFu) + add_transform;
+    const float range = (float(data[brickOffset + off_3 + 1]) / float(0xFFFFFFFFu)) / max_val;
+    for (int i = 0; i < 8; i++) {
+      const int cx = vx + pad + ((i & 4) >> 2), cy = vy + pad + ((i & 2) >> 1), cz = vz + pad + (i & 1);
+      const uint32_t voxelId = uint32_t(dot4(mods[rotIdx], cx, cy, cz));
+      uint32_t sliceId = 0, localId = voxelId, flags = off_1, sliceOffset = 0;
+      if (slices) {
+        sliceId = voxelId / (v_size * v_size);
+        localId = voxelId % (v_size * v_size);
+        flags = off_1 + slice_distance_flags_uints * sliceId;
+        sliceOffset = (data[brickOffset + off_2 + sliceId / 2] >> (16 * (sliceId % 2))) & ((1u << 16) - 1);
+      }
+      uint32_t b0 = data[brickOffset + flags];
+      b0 = localId > 31 ? b0 : b0 & ((1u << localId) - 1);
+      b0 = uint32_t(__builtin_popcount(b0));
+      uint32_t b1 = localId > 32 ? data[brickOffset + flags + 1] & ((1u << (localId - 32)) - 1) : 0;
+      b1 = uint32_t(__builtin_popcount(b1));
+      const uint32_t vId0 = sliceOffset + b0 + b1;
+      const uint32_t dist0 = (data[brickOffset + off_5 + vId0 / vals_per_int] >> (bits * (vId0 % vals_per_int))) & max_val;
+      values[i] = min_val + range * dist0;
+    }
+    return -1.0f;
+  }
+
+  // COctreeV3_BrickIntersect (BVH2Common.cpp:1853-2003); the march stops after 4 bs + 8 voxels
+  // (only marches whose 1e-5 step is lost to rounding get there; the reference would not end).
+  void COctreeV3_BrickIntersect(const CoctG& h, f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
+                                uint32_t brickOffset, f3 p, float sz, uint32_t transform_code_leaf_type, Hit* pHit) {
+    float values[8];
+    const uint32_t v_size = h.brick_size + 2 * h.brick_pad + 1;
+    const uint32_t leaf_type = transform_code_leaf_type & 3;
+    const uint32_t transform_code = transform_code_leaf_type >> 2;
+    const float sz_inv = 2.0f / sz;
+    const float d = 2.0f / (sz * h.brick_size);
+    const f3 brick_min_pos = f3(-1, -1, -1) + sz_inv * p;
+    const f3 brick_max_pos = brick_min_pos + sz_inv * f3(1, 1, 1);
+    f2 brick_fNearFar = RayBoxIntersection2(ray_pos, SafeInverse(ray_dir), brick_min_pos, brick_max_pos);
+    brick_fNearFar.x = std::max(tNear, brick_fNearFar.x);
+    const float old_t = pHit->t;
+    const float hi = h.brick_size - 1e-6f;
+    uint32_t steps = 0;
+    while (brick_fNearFar.x < brick_fNearFar.y && pHit->t == old_t && steps < 4 * h.brick_size + 8) {
+      steps++;
+      const f3 hit_pos = ray_pos + brick_fNearFar.x * ray_dir;
+      const f3 local_pos = (hit_pos - brick_min_pos) * (0.5f * sz * h.brick_size);
+      // LiteMath clamp(u, a, b) = min(max(a, u), b) (LiteMath.h:108)
+      const f3 voxelPos(std::floor(std::min(std::max(1e-6f, local_pos.x), hi)),
+                        std::floor(std::min(std::max(1e-6f, local_pos.y), hi)),
+                        std::floor(std::min(std::max(1e-6f, local_pos.z), hi)));
+      const f3 min_pos = brick_min_pos + d * voxelPos;
+      const f3 max_pos = min_pos + d * f3(1, 1, 1);
+      const int vx = int(voxelPos.x), vy = int(voxelPos.y), vz = int(voxelPos.z);
+      const float vmin = COctreeV3_LoadDistanceValues(h, leaf_type, brickOffset, vx, vy, vz, v_size, sz_inv,
+                                                      transform_code, values);
+      const f2 fNearFar = RayBoxIntersection2(ray_pos, SafeInverse(ray_dir), min_pos, max_pos);
+      if (tNear < fNearFar.x && vmin <= 0.0f) {
+        const f3 start_pos = ray_pos + fNearFar.x * ray_dir;
+        const f3 start_q = (start_pos - min_pos) * (0.5f * sz * h.brick_size);
+        const float qFar = (fNearFar.y - fNearFar.x) * (0.5f * sz * h.brick_size);
+        LocalSurfaceIntersection(TYPE_SDF_FRAME_OCTREE, ray_dir, instId, geomId, values, brickOffset, d, 0.0f, qFar,
+                                 fNearFar, start_q, pHit);
+        if (s.preset.normal_mode == 2 && need_normal() && pHit->t < old_t) {   // NORMAL_MODE_SDF_SMOOTHED
+          const float beta = 0.5f;
+          const f3 dp = start_q + ((pHit->t - fNearFar.x) / d) * ray_dir;
+          f3 normals[8];
+          float values_n[8];
+          f3 nmq(0.5f, 0.5f, 0.5f);
+          nmq.x = dp.x < 0.5f ? step(-beta, beta, dp.x) : step(-beta, beta, dp.x - 1.0f);
+          nmq.y = dp.y < 0.5f ? step(-beta, beta, dp.y) : step(-beta, beta, dp.y - 1.0f);
+          nmq.z = dp.z < 0.5f ? step(-beta, beta, dp.z) : step(-beta, beta, dp.z - 1.0f);
+          const int p0x = int(dp.x < 0.5f ? voxelPos.x - 1 : voxelPos.x);
+          const int p0y = int(dp.y < 0.5f ? voxelPos.y - 1 : voxelPos.y);
+          const int p0z = int(dp.z < 0.5f ? voxelPos.z - 1 : voxelPos.z);
+          for (int i = 0; i < 8; i++) {
+            const int ix = p0x + ((i & 4) >> 2), iy = p0y + ((i & 2) >> 1), iz = p0z + (i & 1);
+            const f3 dVoxelPos = f3(float(ix), float(iy), float(iz)) - voxelPos;
+            const float vmin_n = COctreeV3_LoadDistanceValues(h, leaf_type, brickOffset, ix, iy, iz, v_size, sz_inv,
+                                                              transform_code, values_n);
+            if (vmin_n <= 0.0f) normals[i] = normalize(eval_dist_trilinear_diff(values_n, dp - dVoxelPos));
+          }
+          const f3 smoothed_norm = (1 - nmq.x) * (1 - nmq.y) * (1 - nmq.z) * normals[0] +
+                                   (1 - nmq.x) * (1 - nmq.y) * (nmq.z) * normals[1] +
+                                   (1 - nmq.x) * (nmq.y) * (1 - nmq.z) * normals[2] +
+                                   (1 - nmq.x) * (nmq.y) * (nmq.z) * normals[3] +
+                                   (nmq.x) * (1 - nmq.y) * (1 - nmq.z) * normals[4] +
+                                   (nmq.x) * (1 - nmq.y) * (nmq.z) * normals[5] +
+                                   (nmq.x) * (nmq.y) * (1 - nmq.z) * normals[6] +
+                                   (nmq.x) * (nmq.y) * (nmq.z) * normals[7];
+          const f2 en = encode_normal(normalize(matmul4x3(s.instances[instId].transformInvTransposed, smoothed_norm)));
+          pHit->coords[2] = en.x;
+          pHit->coords[3] = en.y;
+        }
+      }
+      brick_fNearFar.x += std::max(0.0f, fNearFar.y - brick_fNearFar.x) + 1e-5f;
+    }
+    if (pHit->t < old_t && h.uv_size > 0) {
+      const uint32_t* data = s.coct_data + h.base;
+      const uint32_t p_size = h.brick_size + 2 * h.brick_pad;
+      const uint32_t slice_distance_flags_uints = (v_size * v_size + 32 - 1) / 32;
+      const uint32_t off_1 = (p_size * p_size * p_size + 32 - 1) / 32;
+      const uint32_t off_2 = off_1 + v_size * slice_distance_flags_uints;
+      const uint32_t off_3 = off_2 + (v_size + 2 - 1) / 2;
+      const uint32_t off_4 = off_3 + 2;
+      const f3 pos = ray_pos + pHit->t * ray_dir;
+      const f3 dp = (pos - brick_min_pos) * (0.5f * sz);
+      const uint32_t* tc = data + brickOffset + off_4;
+      pHit->coords[0] = (1 - dp.x) * (1 - dp.y) * (1 - dp.z) * (1.5259022e-5f * float(tc[0] >> 16)) +
+                        (1 - dp.x) * (1 - dp.y) * (dp.z) * (1.5259022e-5f * float(tc[1] >> 16)) +
+                        (1 - dp.x) * (dp.y) * (1 - dp.z) * (1.5259022e-5f * float(tc[2] >> 16)) +
+                        (1 - dp.x) * (dp.y) * (dp.z) * (1.5259022e-5f * float(tc[3] >> 16)) +
+                        (dp.x) * (1 - dp.y) * (1 - dp.z) * (1.5259022e-5f * float(tc[4] >> 16)) +
+                        (dp.x) * (1 - dp.y) * (dp.z) * (1.5259022e-5f * float(tc[5] >> 16)) +
+                        (dp.x) * (dp.y) * (1 - dp.z) * (1.5259022e-5f * float(tc[6] >> 16)) +
+                        (dp.x) * (dp.y) * (dp.z) * (1.5259022e-5f * float(tc[7] >> 16));
+      pHit->coords[1] = (1 - dp.x) * (1 - dp.y) * (1 - dp.z) * (1.5259022e-5f * float(tc[0] & 0xFFFF)) +
+                        (1 - dp.x) * (1 - dp.y) * (dp.z) * (1.5259022e-5f * float(tc[1] & 0xFFFF)) +
+                        (1 - dp.x) * (dp.y) * (1 - dp.z) * (1.5259022e-5f * float(tc[2] & 0xFFFF)) +
+                        (1 - dp.x) * (dp.y) * (dp.z) * (1.5259022e-5f * float(tc[3] & 0xFFFF)) +
+                        (dp.x) * (1 - dp.y) * (1 - dp.z) * (1.5259022e-5f * float(tc[4] & 0xFFFF)) +
+                        (dp.x) * (1 - dp.y) * (dp.z) * (1.5259022e-5f * float(tc[5] & 0xFFFF)) +
+                        (dp.x) * (dp.y) * (1 - dp.z) * (1.5259022e-5f * float(tc[6] & 0xFFFF)) +
+                        (dp.x) * (dp.y) * (dp.z) * (1.5259022e-5f * float(tc[7] & 0xFFFF));
+    }
+  }
+
+  // COctreeIntersectV3 (BVH2Common.cpp:2005-2173), starting at the BLAS box's node; pushes past
+  // the 32-entry stack are dropped (the reference writes past its array).
+  void COctreeIntersectV3(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId, uint32_t bvhNodeId,
+                          Hit* pHit) {
+    struct OTStackElement { uint32_t nodeId, info, p_size_x, p_size_y; };
+    const Geom& g = s.geoms[geomId];
+    const CoctG& h = s.coct_geoms[g.offset[0]];
+    const uint32_t* data = s.coct_data + h.base;
+    f3 pos_ray_pos = ray_pos, pos_ray_dir = ray_dir;
+    uint32_t a = 0;
+    if (ray_dir.x < 0) { pos_ray_pos.x *= -1; pos_ray_dir.x *= -1; a |= 4; }
+    if (ray_dir.y < 0) { pos_ray_pos.y *= -1; pos_ray_dir.y *= -1; a |= 2; }
+    if (ray_dir.z < 0) { pos_ray_pos.z *= -1; pos_ray_dir.z *= -1; a |= 1; }
+    const Node& box = s.orig_nodes[g.offset[1] + bvhNodeId];
+    const f3 min_pos = box.boxMin, max_pos = box.boxMax;
+    const uint32_t start_sz = uint32_t(2.0f / (max_pos.x - min_pos.x));
+    const f3 spf = float(start_sz) * 0.5f * (min_pos - f3(-1, -1, -1));
+    const uint32_t spo[3] = {uint32_t(spf.x), uint32_t(spf.y), uint32_t(spf.z)};
+    uint32_t p_mask = start_sz - 1;
+    const uint32_t start_p[3] = {(a & 4) ? (~spo[0] & p_mask) : spo[0], (a & 2) ? (~spo[1] & p_mask) : spo[1],
+                                 (a & 1) ? (~spo[2] & p_mask) : spo[2]};
+    pos_ray_pos = pos_ray_pos * -1.0f;
+    const f3 inv = SafeInverse(pos_ray_dir);
+    const f3 _t0 = pos_ray_pos * inv - inv, _t1 = pos_ray_pos * inv + inv, _l = _t1 - _t0;
+    static const int nn[8][3] = {{4, 2, 1}, {5, 3, 8}, {6, 8, 3}, {7, 8, 8}, {8, 6, 5}, {8, 7, 8}, {8, 8, 7}, {8, 8, 8}};
+    OTStackElement stack[32];
+    int top = 0;
+    const float old_t = pHit->t;
+    stack[0] = OTStackElement{box.leftOffset, 0, (start_p[0] << 16) | start_p[1], (start_p[2] << 16) | start_sz};
+    while (top >= 0) {
+      const uint32_t level_sz = stack[top].p_size_y & 0xFFFF;
+      const uint32_t px = stack[top].p_size_x >> 16, py = stack[top].p_size_x & 0xFFFF, pz = stack[top].p_size_y >> 16;
+      const float d = 1.0f / float(level_sz);
+      const f3 p_f = f3(float(px), float(py), float(pz));
+      const f3 t0 = _t0 + d * p_f * _l;
+      const f3 t1 = _t0 + d * (p_f + 1.0f) * _l;
+      if (std::min(std::min(t1.x, t1.y), t1.z) < tNear) { top--; continue; }
+      bool is_leaf = (stack[top].info & 3) != 0;
+      if (h.lods > 0 && !is_leaf) {
+        const float t = s.preset.fixed_lod > 0 ? 1.0f : std::max(t0.x, std::max(t0.y, t0.z));
+        const float target_lod_size = (std::pow(2.0f, s.preset.level_of_detail) - 0.01f) / t;
+        if (float(level_sz * h.brick_size) > target_lod_size) {
+          const uint32_t lodLeafId = data[stack[top].nodeId + 1];
+          const uint32_t type = (data[stack[top].nodeId + 0] >> 30) & 3;
+          const uint32_t infoOffset = stack[top].nodeId + 1 + h.trans_off;
+          stack[top].nodeId = lodLeafId;
+          stack[top].info = (data[infoOffset] << 2) | type;
+          is_leaf = true;
+          if (type == 0) { top--; continue; }
+        }
+      }
+      if (is_leaf) {
+        p_mask = level_sz - 1;
+        const uint32_t rx = (a & 4) ? (~px & p_mask) : px, ry = (a & 2) ? (~py & p_mask) : py,
+                       rz = (a & 1) ? (~pz & p_mask) : pz;
+        const uint32_t leafOffset = (stack[top].nodeId & h.idx_mask) >> h.idx_sh;
+        COctreeV3_BrickIntersect(h, ray_pos, ray_dir, tNear, instId, geomId, leafOffset,
+                                 f3(float(rx), float(ry), float(rz)), float(level_sz), stack[top].info, pHit);
+        if (pHit->t < old_t) {
+          top = -1;
+          if (s.preset.render_mode == 20) pHit->primId = uint32_t(std::log2(float(level_sz * h.brick_size)));
+        } else {
+          top--;
+        }
+      } else {
+        OTStackElement tmp_buf[4];
+        int buf_top = 0;
+        const f3 tm = 0.5f * (t0 + t1);
+        int currNode = FirstNode(t0, tm);
+        do {
+          const uint32_t childrenInfo = data[stack[top].nodeId + 0];
+          const uint32_t childNode = uint32_t(currNode) ^ a;
+          if ((childrenInfo & (1u << childNode)) > 0 && buf_top < 4) {
+            const uint32_t localChildPos = h.lods + uint32_t(__builtin_popcount(childrenInfo & ((1u << childNode) - 1)));
+            const uint32_t childPos = 1 + h.uints_per_child_info * localChildPos;
+            const uint32_t childOffset = data[stack[top].nodeId + childPos];
+            const uint32_t childType = (childrenInfo >> (8 + 2 * childNode)) & 3;
+            const uint32_t childInfoOffset = stack[top].nodeId + childPos + h.trans_off;
+            const uint32_t childInfo = (data[childInfoOffset] << 2) | childType;
+            tmp_buf[buf_top] = OTStackElement{
+                childOffset, childInfo,
+                (stack[top].p_size_x << 1) | (((uint32_t(currNode) & 4) << (16 - 2)) | ((uint32_t(currNode) & 2) >> 1)),
+                (stack[top].p_size_y << 1) | ((uint32_t(currNode) & 1) << 16)};
+            buf_top++;
+          }
+          currNode = NewNode((currNode & 4) ? t1.x : tm.x, nn[currNode][0], (currNode & 2) ? t1.y : tm.y,
+                             nn[currNode][1], (currNode & 1) ? t1.z : tm.z, nn[currNode][2]);
+        } while (currNode < 8);
+        for (int i = 0; i < buf_top; i++)
+          if (top + i < 32) stack[top + i] = tmp_buf[buf_top - i - 1];
+        top = std::min(top + buf_top - 1, 31);
+      }
+    }
+  }
+
   // BVH2Common.cpp:47-124
   uint32_t IntersectAllPrimitivesInLeaf(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
                                         uint32_t aabbId, Hit* pHit) {
@@ -1020,6 +1306,7 @@ struct Tracer {
       case TAG_SDF_OCTREE: OctreeIntersect(ray_pos, ray_dir, tNear, instId, geomId, pHit); break;
       case TAG_SDF_GRID: IntersectSDFGrid(ray_pos, ray_dir, tNear, instId, geomId, pHit); break;
       case TAG_GRAPHICS_PRIM: IntersectGraphicPrims(ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
+      case TAG_COCTREE_BRICKED: COctreeIntersectV3(ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
       default: break;
     }
     if (s.preset.render_mode == 26) pHit->instId = (pHit->instId == 0xFFFFFFFF) ? 1 : pHit->instId + 1;
@@ -1328,6 +1615,17 @@ int oracle_local_surface(const void* preset, const float* values, float d, float
                               qFar, f2{fNear, 0.0f}, f3(start_q[0], start_q[1], start_q[2]), &h);
   std::memcpy(hit_out, &h, sizeof(h));
   return h.primId != uint32_t(-1) ? 1 : 0;
+}
+
+// The 8 corner values of voxel (vx, vy, vz) of one COctreeV3 leaf (COctreeV3_LoadDistanceValues,
+// BVH2Common.cpp:1637-1851): leaf type, brick word offset and transform code as a child entry
+// gives them, sz the leaf's level size. Returns the loader's result (min value, -1 or 1e6).
+float oracle_coct_voxel(const OracleScene* sc, uint32_t coct, uint32_t leaf_type, uint32_t offset, int vx, int vy,
+                        int vz, float sz, uint32_t code, float* values) {
+  Tracer tr{*sc, nullptr};
+  const CoctG& h = sc->coct_geoms[coct];
+  const uint32_t v_size = h.brick_size + 2 * h.brick_pad + 1;
+  return tr.COctreeV3_LoadDistanceValues(h, leaf_type, offset, vx, vy, vz, v_size, 2.0f / sz, code, values);
 }
 
 }  // extern "C"
