@@ -53,6 +53,26 @@ COCTREE_V3_VERSION = 5
 # per pack mode: uints_per_child_info, idx_mask, idx_sh, trans_off, rot_mask, add_mask (scom.h:27-61)
 COCTREE_PACK_MODE_WORDS = {0: (1, 0xFFFFFFFF, 0, 0, 0x0, 0x0), 1: (2, 0xFFFFFFFF, 0, 1, 0x3F, 0x3FFFFF00),
                            2: (1, 0xFFFFC000, 14, 0, 0x3F, 0x3FC0)}
+# SdfDAG records (sdf/scom2/scom_shared.h:77-119)
+DAG_HEADER_DTYPE = np.dtype([("brick_size", "<u4"), ("brick_pad", "<u4"), ("node_grid_size", "<u4"), ("dim", "<u4"),
+                             ("transform_subgroup", "<u4"), ("user_params", "<f4", (7,)), ("tex_id_off", "<i4"),
+                             ("mat_id_off", "<i4"), ("all_float_tex_id_off", "<i4"), ("all_int_mat_id_off", "<i4")])
+DAG_CHILD_EDGE_DTYPE = np.dtype([("child_offset", "<u4"), ("rotation_id", "<u4")])
+DAG_DATA_EDGE_DTYPE = np.dtype([("data_offset", "<u4"), ("rotation_id", "<u4"), ("type_id", "<u4"), ("add", "<f4")])
+DAG_NODE_DTYPE = np.dtype([("children_edges_offset", "<u4"), ("data_edges_offset", "<u4"),
+                           ("channels_edge", DAG_CHILD_EDGE_DTYPE), ("voxel_count_flags", "<u4"), ("_pad", "<u4")])
+DAG_GEOM_DTYPE = np.dtype([(n, "<u4") for n in ("root", "brick_size", "brick_pad", "node_grid_size", "mods", "_p0",
+                                                 "_p1", "_p2")])
+
+
+def dag_header(brick_size=1, brick_pad=0, node_grid_size=2) -> np.ndarray:
+    """get_default_SdfDAGHeader (sdf/scom2/scom_shared.h:121-146) with the given brick shape."""
+    h = np.zeros((), DAG_HEADER_DTYPE)
+    h["brick_size"], h["brick_pad"], h["node_grid_size"], h["dim"] = brick_size, brick_pad, node_grid_size, 3
+    h["tex_id_off"] = h["mat_id_off"] = h["all_float_tex_id_off"] = h["all_int_mat_id_off"] = -1
+    return h
+
+
 COCT_GEOM_DTYPE = np.dtype([(n, "<u4") for n in (
     "base", "mods", "size", "brick_size", "brick_pad", "bits_per_value", "uv_size", "sim_compression", "lods",
     "uints_per_child_info", "idx_mask", "idx_sh", "trans_off", "rot_mask", "add_mask", "_pad")])
@@ -97,6 +117,8 @@ ARRAY_DTYPES = {
     "svs_roots": np.uint32, "tlas_parent": np.uint32, "grid_data": np.float32, "grid_offsets": np.uint32,
     "grid_sizes": np.uint32, "gprim_headers": GPRIM_HEADER_DTYPE, "gprim_points": np.float32,
     "coct_data": np.uint32, "coct_geoms": COCT_GEOM_DTYPE, "coct_mods": np.int32,
+    "dag_nodes": DAG_NODE_DTYPE, "dag_child_edges": DAG_CHILD_EDGE_DTYPE, "dag_data_edges": DAG_DATA_EDGE_DTYPE,
+    "dag_distances": np.float32, "dag_geoms": DAG_GEOM_DTYPE, "dag_mods": np.int32, "rot_add": np.uint8,
 }
 LIGHT_DTYPE = np.dtype([("space", "<f4", (3,)), ("type", "<u4"), ("color", "<f4", (3,)), ("_pad", "<u4")])  # Light, 32 B
 LIGHT_TYPE_DIRECT, LIGHT_TYPE_POINT, LIGHT_TYPE_AMBIENT = 0, 1, 2
@@ -209,6 +231,9 @@ def lib() -> ctypes.CDLL:
             "lrt_add_graphics_prims": ([vp, vp, vp, sz, pu32], i32),
             "lrt_load_sdf_grid": ([cp, pu32, ctypes.POINTER(_Buffer)], i32),
             "lrt_add_coctree_v3": ([vp, vp, vp, sz, u32, pu32], i32),
+            "lrt_add_sdf_dag": ([vp, vp, vp, sz, vp, sz, vp, sz, vp, sz, pu32], i32),
+            "lrt_load_sdf_dag": ([cp, vp] + [ctypes.POINTER(_Buffer)] * 4, i32),
+            "lrt_save_sdf_dag": ([cp, vp, vp, sz, vp, sz, vp, sz, vp, sz], i32),
             "lrt_load_coctree_v3": ([cp, pu32, vp, ctypes.POINTER(_Buffer)], i32),
             "lrt_save_coctree_v3": ([cp, vp, vp, sz], i32),
             "lrt_save_sdf_grid": ([cp, pu32, vp], i32),
@@ -280,7 +305,8 @@ def exported_symbols():
     return ["lrt_last_error", "lrt_version", "lrt_create", "lrt_destroy", "lrt_clear_geom", "lrt_clear_scene",
             "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_svs", "lrt_load_sdf_svs", "lrt_save_sdf_svs",
             "lrt_add_sdf_grid", "lrt_load_sdf_grid", "lrt_save_sdf_grid", "lrt_add_graphics_prims",
-            "lrt_add_coctree_v3", "lrt_load_coctree_v3", "lrt_save_coctree_v3", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
+            "lrt_add_coctree_v3", "lrt_load_coctree_v3", "lrt_save_coctree_v3",
+            "lrt_add_sdf_dag", "lrt_load_sdf_dag", "lrt_save_sdf_dag", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
             "lrt_trace_closest_host", "lrt_trace_closest_noalloc_dev", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
             "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_set_ray_interleave", "lrt_set_persistent",
@@ -410,6 +436,19 @@ class SceneRT:
         d = np.ascontiguousarray(data, dtype=np.uint32).ravel()
         out = ctypes.c_uint32()
         _check(lib().lrt_add_coctree_v3(self._h, _ptr(h), _ptr(d), d.size, int(bvh_level), ctypes.byref(out)))
+        return out.value
+
+    def AddGeom_SdfDAG(self, header: np.ndarray, nodes: np.ndarray, child_edges: np.ndarray,
+                       data_edges: np.ndarray, distances: np.ndarray) -> int:
+        """BVHRT::AddGeom_SdfDAG (BVH2Common_host.cpp:1020-1131): an SdfDAG's header and arrays."""
+        h = np.ascontiguousarray(header, dtype=DAG_HEADER_DTYPE).reshape(())
+        n = np.ascontiguousarray(nodes, dtype=DAG_NODE_DTYPE)
+        ce = np.ascontiguousarray(child_edges, dtype=DAG_CHILD_EDGE_DTYPE)
+        de = np.ascontiguousarray(data_edges, dtype=DAG_DATA_EDGE_DTYPE)
+        d = np.ascontiguousarray(distances, dtype=np.float32)
+        out = ctypes.c_uint32()
+        _check(lib().lrt_add_sdf_dag(self._h, _ptr(h), _ptr(n), n.size, _ptr(ce), ce.size, _ptr(de), de.size, _ptr(d),
+                                     d.size, ctypes.byref(out)))
         return out.value
 
     def AddGeom_SdfSVS(self, nodes: np.ndarray) -> int:
@@ -827,6 +866,26 @@ def save_coctree_v3(path: str, header: np.ndarray, data: np.ndarray):
     h = np.ascontiguousarray(header, dtype=COCTREE_HEADER_DTYPE).reshape(())
     d = np.ascontiguousarray(data, dtype=np.uint32).ravel()
     _check(lib().lrt_save_coctree_v3(os.fsencode(path), _ptr(h), _ptr(d), d.size))
+
+
+def load_sdf_dag(path: str):
+    """load_sdf_DAG (sdf/scom2/scom_utils.cpp:37-73): (header, nodes, child_edges, data_edges, distances)."""
+    h = np.zeros((), DAG_HEADER_DTYPE)
+    bufs = [_Buffer() for _ in range(4)]
+    _check(lib().lrt_load_sdf_dag(os.fsencode(path), _ptr(h), *[ctypes.byref(b) for b in bufs]))
+    return (h, _take(bufs[0], DAG_NODE_DTYPE), _take(bufs[1], DAG_CHILD_EDGE_DTYPE), _take(bufs[2], DAG_DATA_EDGE_DTYPE),
+            _take(bufs[3], np.float32))
+
+
+def save_sdf_dag(path: str, header, nodes, child_edges, data_edges, distances):
+    """save_sdf_DAG (sdf/scom2/scom_utils.cpp:5-35), without data channels."""
+    h = np.ascontiguousarray(header, dtype=DAG_HEADER_DTYPE).reshape(())
+    n = np.ascontiguousarray(nodes, dtype=DAG_NODE_DTYPE)
+    ce = np.ascontiguousarray(child_edges, dtype=DAG_CHILD_EDGE_DTYPE)
+    de = np.ascontiguousarray(data_edges, dtype=DAG_DATA_EDGE_DTYPE)
+    d = np.ascontiguousarray(distances, dtype=np.float32)
+    _check(lib().lrt_save_sdf_dag(os.fsencode(path), _ptr(h), _ptr(n), n.size, _ptr(ce), ce.size, _ptr(de), de.size,
+                                  _ptr(d), d.size))
 
 
 def load_sdf_svs(path: str) -> np.ndarray:

@@ -302,6 +302,22 @@ lrt_status lrt_add_coctree_v3(lrt_scene* s, const uint32_t header[15], const uin
   });
 }
 
+lrt_status lrt_add_sdf_dag(lrt_scene* s, const void* header, const void* nodes, size_t n_nodes,
+                           const void* child_edges, size_t n_child_edges, const void* data_edges, size_t n_data_edges,
+                           const float* distances, size_t n_distances, uint32_t* out_geom_id) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !header || !nodes || !child_edges || !data_edges || !distances)
+      return fail(LRT_ERR_INVALID_ARG, "null argument");
+    const uint32_t id = s->s.add_sdf_dag(*static_cast<const lrt::SdfDAGHeader*>(header),
+                                         static_cast<const lrt::SdfDAGNode*>(nodes), n_nodes,
+                                         static_cast<const lrt::SdfDAGChildEdge*>(child_edges), n_child_edges,
+                                         static_cast<const lrt::SdfDAGDataEdge*>(data_edges), n_data_edges, distances,
+                                         n_distances);
+    if (out_geom_id) *out_geom_id = id;
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_add_sdf_svs(lrt_scene* s, const void* nodes, size_t n_nodes, uint32_t* out_geom_id) {
   return guarded([&]() -> lrt_status {
     if (!s || !nodes) return fail(LRT_ERR_INVALID_ARG, "null argument");
@@ -1024,6 +1040,9 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
     ARR("grid_data", c.gridData) ARR("grid_offsets", c.gridOffsets) ARR("grid_sizes", c.gridSizes)
     ARR("gprim_headers", c.gprimHeaders) ARR("gprim_points", c.gprimPoints)
     ARR("coct_data", c.coctData) ARR("coct_geoms", c.coctGeoms) ARR("coct_mods", c.coctMods)
+    ARR("dag_nodes", c.dagNodes) ARR("dag_child_edges", c.dagChildEdges) ARR("dag_data_edges", c.dagDataEdges)
+    ARR("dag_distances", c.dagDistances) ARR("dag_geoms", c.dagGeoms) ARR("dag_mods", c.dagMods)
+    if (n == "rot_add") { src = lrt::rot_add_table().data(); sz = lrt::rot_add_table().size(); }
 #undef ARR
     if (n == "preset") { src = &c.preset; sz = sizeof(c.preset); }
     std::vector<uint32_t> mask_copy;   // the committed occupancy masks live on the device only
@@ -1043,7 +1062,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
       static const char* known[] = {"nodes", "tlas", "instances", "geoms", "geom_tags", "start_end",
                                     "prim_id_count", "vert_pos", "vert_norm", "indices", "prim_indices",
                                     "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes", "sbs_data",
-                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices", "svs_nodes", "svs_roots", "tlas_parent", "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points", "coct_data", "coct_geoms", "coct_mods",
+                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices", "svs_nodes", "svs_roots", "tlas_parent", "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points", "coct_data", "coct_geoms", "coct_mods", "dag_nodes", "dag_child_edges", "dag_data_edges", "dag_distances", "dag_geoms", "dag_mods", "rot_add",
                                     "sbs_mask"};
       bool ok = false;
       for (const char* k : known) ok = ok || n == k;

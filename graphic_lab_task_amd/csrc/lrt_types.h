@@ -133,6 +133,39 @@ struct CoctGeom {
 };
 static_assert(sizeof(CoctGeom) == 64, "CoctGeom is 64 B");
 
+// SdfDAG (sdf/scom2/scom.h:56-67, scom_shared.h:77-119): octree nodes whose children and leaf
+// bricks are shared through edges carrying one of the 48 lattice rotations (and an additive
+// shift for bricks).
+struct SdfDAGHeader {
+  uint32_t brick_size, brick_pad, node_grid_size, dim, transform_subgroup;
+  float    user_params[7];
+  int32_t  tex_id_off, mat_id_off, all_float_tex_id_off, all_int_mat_id_off;
+};
+struct SdfDAGChildEdge {
+  uint32_t child_offset;   // node index, 0: no child
+  uint32_t rotation_id;
+};
+struct SdfDAGDataEdge {
+  uint32_t data_offset;    // first distance of the brick
+  uint32_t rotation_id;
+  uint32_t type_id;
+  float    add;
+};
+struct SdfDAGNode {
+  uint32_t children_edges_offset;   // first of node_grid_size^3 child edges, 0: a leaf
+  uint32_t data_edges_offset;       // first data edge, 0: no surface
+  SdfDAGChildEdge channels_edge;
+  uint32_t voxel_count_flags;       // bits 0-15 data edges, bit 16 surface node, bit 17 inside
+  uint32_t _pad;
+};
+static_assert(sizeof(SdfDAGHeader) == 64 && sizeof(SdfDAGNode) == 24 && sizeof(SdfDAGChildEdge) == 8 &&
+              sizeof(SdfDAGDataEdge) == 16, "SdfDAG layouts");
+// One DAG geometry as the kernels read it (ours): its root node, brick shape, node grid size and
+// the first of its 2 x 48 index modifiers (brick value lattice, node grid).
+struct DagGeom {
+  uint32_t root, brick_size, brick_pad, node_grid_size, mods, _pad[3];
+};
+
 // sdf/simple/sdf_simple_shared.h:39-45
 struct SdfSBSNode {
   uint32_t pos_xy;          // (x << 16) | y, brick position in its LOD grid
@@ -208,6 +241,8 @@ constexpr uint32_t TYPE_SDF_SVS          = 5;
 constexpr uint32_t TYPE_SDF_SBS          = 6;
 constexpr uint32_t TYPE_SDF_MULTI_OCTREE = 9;
 constexpr uint32_t TYPE_COCTREE_V3       = 20;
+constexpr uint32_t TYPE_SDF_DAG          = 25;
+constexpr uint32_t TYPE_SCOM2            = 26;
 constexpr uint32_t MULTI_OCTREE_FLAG_SURFACE = 1;          // sdf_multi_shared.h:5
 constexpr uint32_t MULTI_RENDER_MODE_MO_VOXEL_COUNT = 24;  // render_settings.h:85-86
 constexpr uint32_t MULTI_RENDER_MODE_MO_FLAGS = 25;
@@ -224,6 +259,7 @@ constexpr uint32_t TAG_SDF_NODE   = 8;
 constexpr uint32_t TAG_SDF_GRID   = 11;
 constexpr uint32_t TAG_GRAPHICS_PRIM = 9;
 constexpr uint32_t TAG_SDF_OCTREE = 15;
+constexpr uint32_t TAG_SDF_DAG    = 18;
 
 // SBS node layouts (sdf/simple/sdf_simple_shared.h:13-21)
 constexpr uint32_t SDF_SBS_NODE_LAYOUT_UNDEFINED        = 0u << 24;

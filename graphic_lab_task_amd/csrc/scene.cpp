@@ -148,6 +148,7 @@ void Scene::clear_geom() {
   gridData.clear(); gridOffsets.clear(); gridSizes.clear();
   gprimHeaders.clear(); gprimPoints.clear();
   coctData.clear(); coctGeoms.clear(); coctMods.clear();
+  dagNodes.clear(); dagChildEdges.clear(); dagDataEdges.clear(); dagDistances.clear(); dagGeoms.clear(); dagMods.clear();
   geom_dirty = true;
   clear_scene();
 }
@@ -592,6 +593,7 @@ uint32_t Scene::leaf_kinds() const {
     else if (t == TAG_SDF_GRID) k |= LK_GRID;
     else if (t == TAG_GRAPHICS_PRIM) k |= LK_GPRIM;
     else if (t == TAG_COCTREE_BRICKED) k |= LK_COCT;
+    else if (t == TAG_SDF_DAG) k |= LK_DAG;
   }
   return k;
 }
@@ -838,6 +840,9 @@ hipError_t Scene::commit() {
   UP("grid_data", gridData) UP("grid_offsets", gridOffsets) UP("grid_sizes", gridSizes)
   UP("gprim_headers", gprimHeaders) UP("gprim_points", gprimPoints)
   UP("coct_data", coctData) UP("coct_geoms", coctGeoms) UP("coct_mods", coctMods)
+  UP("dag_nodes", dagNodes) UP("dag_child_edges", dagChildEdges) UP("dag_data_edges", dagDataEdges)
+  UP("dag_distances", dagDistances) UP("dag_geoms", dagGeoms) UP("dag_mods", dagMods)
+  if (!dagGeoms.empty()) { UP("rot_add", rot_add_table()) }
   // MultiRenderer's own m_instanceTransformInvTransposed (MultiRenderer_host.cpp:870-878):
   // transpose(inverse4x4(matrix)), refreshed by UpdateInstance (unlike the BVHRT copy)
   std::vector<float4x4> inst_inv_t(instances.size());
@@ -906,6 +911,13 @@ SceneView Scene::make_view() const {
   v.coct_data = (const uint32_t*)P("coct_data");
   v.coct_geoms = (const CoctGeom*)P("coct_geoms");
   v.coct_mods = (const int4*)P("coct_mods");
+  v.dag_nodes = (const SdfDAGNode*)P("dag_nodes");
+  v.dag_child_edges = (const SdfDAGChildEdge*)P("dag_child_edges");
+  v.dag_data_edges = (const SdfDAGDataEdge*)P("dag_data_edges");
+  v.dag_distances = (const float*)P("dag_distances");
+  v.dag_geoms = (const DagGeom*)P("dag_geoms");
+  v.dag_mods = (const int4*)P("dag_mods");
+  v.rot_add = (const uint8_t*)P("rot_add");
   v.preset = preset;
   v.lod_thr = lod_thr;
   v.leaf_kinds = leaf_kinds();

@@ -14,6 +14,9 @@
 
 namespace lrt {
 
+// m_RotAddTable: 48 x 48 compositions of the lattice rotations (entry i * 48 + j: i, then j)
+const std::vector<uint8_t>& rot_add_table();
+
 class DevBuf {
  public:
   DevBuf() = default;
@@ -70,6 +73,12 @@ struct Scene {
   std::vector<uint32_t> coctData;    // words of every COctreeV3 geometry in turn
   std::vector<CoctGeom> coctGeoms;
   std::vector<int4> coctMods;        // 3 x 48 index modifiers per COctreeV3 geometry
+  std::vector<SdfDAGNode> dagNodes;
+  std::vector<SdfDAGChildEdge> dagChildEdges;
+  std::vector<SdfDAGDataEdge> dagDataEdges;
+  std::vector<float> dagDistances;
+  std::vector<DagGeom> dagGeoms;
+  std::vector<int4> dagMods;         // 2 x 48 index modifiers per DAG geometry
   std::vector<InstanceData> instances;
   std::vector<BVHNode> tlas;
 
@@ -152,6 +161,9 @@ struct Scene {
   uint32_t add_graphics_prims(const GraphicsPrimHeader& header, const float4* points, size_t n_points);
   uint32_t add_sdf_sbs(const SdfSBSHeader& header, const SdfSBSNode* nodes, size_t n_nodes, const uint32_t* values,
                        size_t n_values, const float* values_f, size_t n_values_f);
+  uint32_t add_sdf_dag(const SdfDAGHeader& h, const SdfDAGNode* nodes, size_t n_nodes,
+                       const SdfDAGChildEdge* child_edges, size_t n_child_edges, const SdfDAGDataEdge* data_edges,
+                       size_t n_data_edges, const float* distances, size_t n_distances);
   uint32_t add_coctree_v3(const COctreeV3Header& h, const uint32_t* data, size_t size, uint32_t bvh_level);
   uint32_t add_sdf_multi_octree(const SdfMultiOctreeNode* nodes, size_t n_nodes, const float* values,
                                 size_t n_values);

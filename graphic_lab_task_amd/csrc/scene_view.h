@@ -21,6 +21,7 @@ enum LeafKind : uint32_t {
   LK_GRID  = 1u << 6,   // TAG_SDF_GRID (dense SDF grid, sphere tracing)
   LK_GPRIM = 1u << 7,   // TAG_GRAPHICS_PRIM (points, lines, segments, arrows, wire boxes)
   LK_COCT  = 1u << 8,   // TAG_COCTREE_BRICKED (compact octree COctreeV3)
+  LK_DAG   = 1u << 9,   // TAG_SDF_DAG (octree DAG with rotated shared nodes and bricks)
 };
 
 // Frame-octree leaf as the kernels read it: the leaf box, its 8 corner distances and the
@@ -83,6 +84,14 @@ struct SceneView {
   const uint32_t*           coct_data;     // m_SdfCompactOctreeV3Data, every geometry's words in turn
   const CoctGeom*           coct_geoms;    // per COctreeV3 geometry (GeomData.offset.x)
   const int4*               coct_mods;     // m_SdfCompactOctreeRotModifiers, 3 x 48 per geometry
+  // octree DAGs
+  const SdfDAGNode*         dag_nodes;     // m_SdfDAGNodes
+  const SdfDAGChildEdge*    dag_child_edges;
+  const SdfDAGDataEdge*     dag_data_edges;
+  const float*              dag_distances;
+  const DagGeom*            dag_geoms;     // per DAG geometry (GeomData.offset.y)
+  const int4*               dag_mods;      // 2 x 48 index modifiers per DAG geometry
+  const uint8_t*            rot_add;       // m_RotAddTable: 48 x 48 rotation compositions
 
   MultiRenderPreset preset;
   float    lod_thr;                 // m_precomputedLoDThr

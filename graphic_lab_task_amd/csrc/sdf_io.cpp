@@ -180,6 +180,55 @@ lrt_status lrt_save_coctree_v3(const char* path, const uint32_t header[15], cons
   });
 }
 
+lrt_status lrt_load_sdf_dag(const char* path, void* header, lrt_buffer* nodes, lrt_buffer* child_edges,
+                            lrt_buffer* data_edges, lrt_buffer* distances) {
+  return io_guarded([&]() -> lrt_status {
+    if (!header || !nodes || !child_edges || !data_edges || !distances) throw std::runtime_error("null output");
+    File fl(path, "rb");
+    size_t left = size_t(fl.size());
+    uint32_t counts[6];
+    take(fl, counts, sizeof(counts), left, "SdfDAG counts");
+    if (counts[4] || counts[5])
+      throw std::runtime_error("SdfDAG files with data channels are not read (the intersection does not use them)");
+    take(fl, header, sizeof(lrt::SdfDAGHeader), left, "SdfDAG header");
+    const size_t rec[4] = {sizeof(lrt::SdfDAGNode), sizeof(lrt::SdfDAGChildEdge), sizeof(lrt::SdfDAGDataEdge), 4};
+    lrt_buffer* out[4] = {nodes, child_edges, data_edges, distances};
+    const char* what[4] = {"SdfDAG nodes", "SdfDAG child edges", "SdfDAG data edges", "SdfDAG distances"};
+    std::vector<char> v[4];
+    for (int i = 0; i < 4; ++i) {
+      if (uint64_t(counts[i]) * rec[i] > left)
+        throw std::runtime_error(std::string(what[i]) + ": count " + std::to_string(counts[i]) + " exceeds the file");
+      v[i].resize(size_t(counts[i]) * rec[i]);
+      take(fl, v[i].data(), v[i].size(), left, what[i]);
+    }
+    for (int i = 0; i < 4; ++i) to_buf(v[i], out[i]);
+    return LRT_OK;
+  });
+}
+
+lrt_status lrt_save_sdf_dag(const char* path, const void* header, const void* nodes, size_t n_nodes,
+                            const void* child_edges, size_t n_child_edges, const void* data_edges, size_t n_data_edges,
+                            const float* distances, size_t n_distances) {
+  return io_guarded([&]() -> lrt_status {
+    if (!header || (!nodes && n_nodes) || (!child_edges && n_child_edges) || (!data_edges && n_data_edges) ||
+        (!distances && n_distances))
+      throw std::runtime_error("null argument");
+    if (n_nodes > 0xFFFFFFFFull || n_child_edges > 0xFFFFFFFFull || n_data_edges > 0xFFFFFFFFull ||
+        n_distances > 0xFFFFFFFFull)
+      throw std::runtime_error("too many records for a u32 count");
+    const uint32_t counts[6] = {uint32_t(n_nodes), uint32_t(n_child_edges), uint32_t(n_data_edges),
+                                uint32_t(n_distances), 0, 0};
+    File fl(path, "wb");
+    put(fl, counts, sizeof(counts));
+    put(fl, header, sizeof(lrt::SdfDAGHeader));
+    put(fl, nodes, n_nodes * sizeof(lrt::SdfDAGNode));
+    put(fl, child_edges, n_child_edges * sizeof(lrt::SdfDAGChildEdge));
+    put(fl, data_edges, n_data_edges * sizeof(lrt::SdfDAGDataEdge));
+    put(fl, distances, n_distances * 4);
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_load_sdf_sbs(const char* path, void* header, lrt_buffer* nodes, lrt_buffer* values,
                             lrt_buffer* values_f) {
   return io_guarded([&]() -> lrt_status {

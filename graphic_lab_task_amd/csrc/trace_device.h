@@ -207,7 +207,9 @@ __device__ bool local_surface_intersection(const SceneView& s, uint32_t type, fl
                                            uint32_t geomId, const float values[8], uint32_t primId, float d,
                                            float qNear, float qFar, float2 fNearFar, float3 start_q,
                                            CRT_Hit& hit, float* t_local = nullptr, bool surface_node = false) {
-  const float EPS = 1e-6f;
+  // surface nodes of the compact formats stop at a distance scaled with the voxel (BVH2Common.cpp:340-351)
+  const float EPS = (surface_node && (type == TYPE_SDF_DAG || type == TYPE_SCOM2))
+                        ? fmax_ref(1e-6f, s.preset.compact_sdf_eps * d) : 1e-6f;
   const float d_inv = 1.0f / d;
   float t = qNear;
   bool early_hit = false;
@@ -393,6 +395,17 @@ __device__ bool local_surface_intersection(const SceneView& s, uint32_t type, fl
     hit.coords[2] = en.x;
     hit.coords[3] = en.y;
     if (s.preset.render_mode == MULTI_RENDER_MODE_ST_ITERATIONS) hit.primId = iter;
+    // the channel render modes keep the voxel-local hit point (16 bits per axis) instead of the
+    // texture coordinates (BVH2Common.cpp:714-723)
+    if ((type == TYPE_SDF_MULTI_OCTREE || type == TYPE_SDF_DAG || type == TYPE_SCOM2) &&
+        (s.preset.render_mode == MULTI_RENDER_MODE_CHANNEL || s.preset.render_mode == MULTI_RENDER_MODE_CHANNEL_LAMBERT)) {
+      const float3 dq = start_q + t * ray_dir;
+      const int qx = int(clamp_ref(65535.0f * dq.x, 0.0f, 65535.0f));
+      const int qy = int(clamp_ref(65535.0f * dq.y, 0.0f, 65535.0f));
+      const int qz = int(clamp_ref(65535.0f * dq.z, 0.0f, 65535.0f));
+      hit.coords[0] = __int_as_float((qy << 16) | qx);
+      hit.coords[1] = __int_as_float(qz);
+    }
     if (t_local) *t_local = t;
     return true;
   }
@@ -1508,6 +1521,145 @@ __device__ __forceinline__ void coct_intersect(const SceneView& s, const GeomRef
       }
       top += buf_top - 1;
       if (top >= COCT_STACK) top = COCT_STACK - 1;
+    }
+  }
+}
+
+}  // namespace lrt
+
+namespace lrt {
+
+// ---- octree DAG (BVHRT::DAGIntersect, MultiRenderer/BVH2Common.cpp:2365-2557) ----
+// Depth-first walk from the root; each stack entry carries the cumulative rotation of its node
+// (composed through the rotation table along the path). Children are pushed far to near by
+// marching the node's child grid backwards along the ray, so the nearest ends on top; leaves
+// (and inner nodes coarser than the level of detail) march their brick and intersect every
+// surface the node lists. Stack pushes past 32 entries are dropped (the reference writes past
+// its array); a brick march stops after 4 bs + 8 voxels and a child march after 256 cells per
+// grid step, which only cuts marches whose advance is lost to rounding.
+__device__ __forceinline__ void dag_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir,
+                                              float tNear, uint32_t instId, uint32_t geomId, CRT_Hit& hit) {
+  const DagGeom dg = s.dag_geoms[g.offset.y];
+  const int4* Mb = s.dag_mods + dg.mods;                      // brick value lattice
+  const int4* Mg = s.dag_mods + dg.mods + COCTREE_ROT_COUNT;  // node grid
+  const float3 ray_dir_inv = safe_inverse(ray_dir);
+  constexpr int DAG_STACK = 32;
+  uint32_t st_node[DAG_STACK], st_info[DAG_STACK], st_pxy[DAG_STACK], st_pzs[DAG_STACK];
+  int top = 0;
+  const float old_t = hit.t;
+  const uint32_t type = g.type;
+  const uint32_t bs = dg.brick_size;
+  const uint32_t v_size = dg.brick_size + 2 * dg.brick_pad + 1;
+  const int v_sz = int(dg.node_grid_size);
+  float2 fNearFar = float2{0.0f, 0.0f};
+  st_node[0] = dg.root;
+  st_info[0] = 0;
+  st_pxy[0] = 0;
+  st_pzs[0] = 1;
+  while (top >= 0) {
+    const uint32_t nodeId = st_node[top], info = st_info[top], pxy = st_pxy[top], pzs = st_pzs[top];
+    top--;
+    const SdfDAGNode node = s.dag_nodes[nodeId];
+    const uint32_t level_sz = pzs & 0xFFFFu;
+    const float level_sz_f = float(level_sz);
+    const float3 p = make3(float(pxy >> 16), float(pxy & 0xFFFFu), float(pzs >> 16));
+    const float sz_inv = 2.0f / level_sz_f;
+    const float3 brick_min_pos = make3(-1, -1, -1) + sz_inv * p;
+    const float3 brick_max_pos = brick_min_pos + sz_inv * make3(1, 1, 1);
+    float2 bnf = ray_box2(ray_pos, ray_dir_inv, brick_min_pos, brick_max_pos);
+    bnf.x = fmax_ref(tNear, bnf.x);
+    if (bnf.x >= bnf.y) continue;
+    bool is_leaf = node.children_edges_offset == 0;
+    if (!is_leaf) {
+      const float t = s.preset.fixed_lod > 0 ? 1.0f : bnf.x;
+      const float target_lod_size = (powf(2.0f, s.preset.level_of_detail) - 0.01f) / t;
+      is_leaf = float(level_sz) > target_lod_size;
+    }
+    if (is_leaf) {
+      const uint32_t surface_count = node.voxel_count_flags & 0xFFFFu;
+      if (surface_count == 0 || node.data_edges_offset == 0) continue;
+      const float sz = level_sz_f;
+      const float d = 2.0f / (sz * float(bs));
+      const bool is_surface = ((node.voxel_count_flags >> 16) & 1u) != 0;
+      const uint32_t primId = node.channels_edge.child_offset;
+      const float scale = 0.5f * sz * float(bs);
+      const float hi = float(bs) - 1e-6f;
+      const uint32_t max_steps = 4 * bs + 8;
+      uint32_t steps = 0;
+      while (bnf.x < bnf.y && hit.t == old_t && steps < max_steps) {
+        steps++;
+        const float3 hit_pos = ray_pos + bnf.x * ray_dir;
+        const float3 local_pos = (hit_pos - brick_min_pos) * scale;
+        const float3 voxelPos = make3(floorf(clamp_ref(local_pos.x, 1e-6f, hi)), floorf(clamp_ref(local_pos.y, 1e-6f, hi)),
+                                      floorf(clamp_ref(local_pos.z, 1e-6f, hi)));
+        const float3 min_pos = brick_min_pos + d * voxelPos;
+        const float3 max_pos = min_pos + d * make3(1, 1, 1);
+        const int vx = int(voxelPos.x), vy = int(voxelPos.y), vz = int(voxelPos.z);
+        for (uint32_t s_id = 0; s_id < surface_count && hit.t == old_t; ++s_id) {
+          const SdfDAGDataEdge e = s.dag_data_edges[node.data_edges_offset + s_id];
+          const int4 m = Mb[s.rot_add[info * COCTREE_ROT_COUNT + e.rotation_id]];
+          float values[8];
+          float vmin = 1.0f;
+          float vmax = is_surface ? -1.0f : 1.0f;
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            const int cx = vx + ((i & 4) >> 2), cy = vy + ((i & 2) >> 1), cz = vz + (i & 1);
+            const uint32_t vId0 = uint32_t(m.x * cx + m.y * cy + m.z * cz + m.w);
+            const float val = s.dag_distances[e.data_offset + vId0] + e.add;
+            values[i] = val - s.preset.compact_sdf_eps * d;
+            vmin = fmin_ref(vmin, val);
+            vmax = fmax_ref(vmax, val);
+          }
+          fNearFar = ray_box2(ray_pos, safe_inverse(ray_dir), min_pos, max_pos);
+          if (fNearFar.x < fNearFar.y && vmin <= 0.0f && vmax >= 0.0f) {
+            const float3 start_pos = ray_pos + fNearFar.x * ray_dir;
+            const float vox_sz = 0.5f * sz * float(bs);
+            const float3 start_q = vox_sz * (start_pos - min_pos);
+            const float qNear = fmax_ref(0.0f, vox_sz * (tNear - fNearFar.x));
+            const float qFar = vox_sz * (fNearFar.y - fNearFar.x);
+            local_surface_intersection(s, type, ray_dir, instId, geomId, values, primId, d, qNear, qFar, fNearFar,
+                                       start_q, hit, nullptr, is_surface);
+          }
+        }
+        bnf.x += fmax_ref(0.0f, fNearFar.y - bnf.x) + 1e-5f;
+        if (hit.t < old_t) top = -1;
+      }
+    } else {
+      // children, pushed far to near by marching the node's grid backwards along the ray
+      const float t_neg = bnf.y + tNear + 1;
+      const float3 neg_pos = ray_pos + t_neg * ray_dir;
+      const float3 neg_dir_inv = make3(-ray_dir_inv.x, -ray_dir_inv.y, -ray_dir_inv.z);
+      const float3 neg_dir = make3(-ray_dir.x, -ray_dir.y, -ray_dir.z);
+      float2 nbnf = float2{t_neg - bnf.y, t_neg - bnf.x};
+      const float d = 2.0f / (level_sz_f * float(v_sz));
+      const float hi = float(v_sz) - 1e-6f;
+      const float scale = 0.5f * level_sz_f * float(v_sz);
+      uint32_t steps = 0;
+      // every step advances at least 0.01 of a cell, so a march ends within ~180 v_sz steps; the cap
+      // only ends marches whose advance is lost to rounding
+      while (nbnf.x < nbnf.y && steps < uint32_t(256 * v_sz)) {
+        steps++;
+        const float3 hit_pos = neg_pos + nbnf.x * neg_dir;
+        const float3 local_pos = (hit_pos - brick_min_pos) * scale;
+        const float3 voxelPos = make3(floorf(clamp_ref(local_pos.x, 1e-6f, hi)), floorf(clamp_ref(local_pos.y, 1e-6f, hi)),
+                                      floorf(clamp_ref(local_pos.z, 1e-6f, hi)));
+        const int4 m = Mg[info];
+        const uint32_t voxId = uint32_t(m.x * int(voxelPos.x) + m.y * int(voxelPos.y) + m.z * int(voxelPos.z) + m.w);
+        const float3 min_pos = brick_min_pos + d * voxelPos;
+        const float3 max_pos = min_pos + d * make3(1, 1, 1);
+        const float2 cnf = ray_box2(neg_pos, neg_dir_inv, min_pos, max_pos);
+        const SdfDAGChildEdge edge = s.dag_child_edges[node.children_edges_offset + voxId];
+        if (tNear < cnf.x && edge.child_offset > 0) {
+          if (top + 1 < DAG_STACK) {
+            top++;
+            st_node[top] = edge.child_offset;
+            st_info[top] = s.rot_add[info * COCTREE_ROT_COUNT + edge.rotation_id];
+            st_pxy[top] = pxy * uint32_t(v_sz) + ((uint32_t(voxelPos.x) << 16) | uint32_t(voxelPos.y));
+            st_pzs[top] = pzs * uint32_t(v_sz) + (uint32_t(voxelPos.z) << 16);
+          }
+        }
+        nbnf.x += fmax_ref(0.0f, cnf.y - nbnf.x) + 0.01f * d;
+      }
     }
   }
 }

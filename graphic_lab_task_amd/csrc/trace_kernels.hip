@@ -88,6 +88,8 @@ hipError_t launch_trace(const SceneView& s, const TraceArgs& a, hipStream_t stre
     e = launch_trace_multi(s, a, stream);
   else if (k == LK_COCT)
     e = launch_trace_coct(s, a, stream);
+  else if (k == LK_DAG)
+    e = launch_trace_dag(s, a, stream);
   else
     e = launch_trace_mixed(s, a, stream);
   // a call whose second pass did not launch must not leave queued rays to the next call

@@ -336,6 +336,7 @@ hipError_t launch_trace_sbs(const SceneView& s, const TraceArgs& a, hipStream_t 
 hipError_t launch_trace_multi(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_trace_mixed(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_trace_coct(const SceneView& s, const TraceArgs& a, hipStream_t stream);
+hipError_t launch_trace_dag(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_trace_sbs_smooth(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_trace_mixed_smooth(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 

@@ -96,6 +96,19 @@ lrt_status lrt_add_graphics_prims(lrt_scene* s, const void* header, const float*
 lrt_status lrt_add_coctree_v3(lrt_scene* s, const uint32_t header[15], const uint32_t* data, size_t size,
                               uint32_t bvh_level, uint32_t* out_geom_id);
 
+/* Replaces BVHRT::AddGeom_SdfDAG (BVH2Common_host.cpp:1020-1131). header: SdfDAGHeader (64 B,
+ * sdf/scom2/scom_shared.h:105-119: dim 3, the default transform subgroup, node_grid_size 2..4);
+ * nodes: SdfDAGNode[n_nodes] (24 B, :95-103), node 0 the root; child_edges: SdfDAGChildEdge
+ * (8 B: child node, rotation); data_edges: SdfDAGDataEdge (16 B: first distance, rotation, type,
+ * additive shift); distances: (brick_size + 2 brick_pad + 1)^3 floats per brick. Checked when
+ * added: every node, edge and brick a ray can reach lies inside the arrays, rotations are below
+ * 48 and the DAG is at most 15 levels deep (for a 2^3 grid); a failed check returns
+ * LRT_ERR_INVALID_ARG with the reason. Hits report TYPE_SDF_DAG and the node's channel offset as
+ * primId, as the reference. */
+lrt_status lrt_add_sdf_dag(lrt_scene* s, const void* header, const void* nodes, size_t n_nodes,
+                           const void* child_edges, size_t n_child_edges, const void* data_edges, size_t n_data_edges,
+                           const float* distances, size_t n_distances, uint32_t* out_geom_id);
+
 /* Replaces BVHRT::AddGeom_SdfGrid (BVH2Common_host.cpp:466-499): a dense grid of distances over
  * the unit cube, size[0] * size[1] * size[2] floats with x fastest (fewer than 2^28). */
 lrt_status lrt_add_sdf_grid(lrt_scene* s, const uint32_t size[3], const float* data, uint32_t* out_geom_id);

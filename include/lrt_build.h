@@ -98,6 +98,16 @@ lrt_status lrt_save_sdf_grid(const char* path, const uint32_t size[3], const flo
 lrt_status lrt_load_coctree_v3(const char* path, uint32_t* version, uint32_t header[15], lrt_buffer* data);
 lrt_status lrt_save_coctree_v3(const char* path, const uint32_t header[15], const uint32_t* data, size_t size);
 
+/* load_sdf_DAG / save_sdf_DAG (sdf/scom2/scom_utils.cpp:5-73): u32 counts of nodes, child edges,
+ * data edges, distances, point channels and voxel channels, the 64-B SdfDAGHeader, then the four
+ * arrays (SdfDAGNode 24 B, SdfDAGChildEdge 8 B, SdfDAGDataEdge 16 B, float). Files with data
+ * channels are refused (the ray query does not read them); the saver writes none. */
+lrt_status lrt_load_sdf_dag(const char* path, void* header, lrt_buffer* nodes, lrt_buffer* child_edges,
+                            lrt_buffer* data_edges, lrt_buffer* distances);
+lrt_status lrt_save_sdf_dag(const char* path, const void* header, const void* nodes, size_t n_nodes,
+                            const void* child_edges, size_t n_child_edges, const void* data_edges, size_t n_data_edges,
+                            const float* distances, size_t n_distances);
+
 /* load_sdf_SVS / save_sdf_SVS (sdf/simple/sdf_simple.cpp:69-94): u32 count, then count SdfSVSNode. */
 lrt_status lrt_load_sdf_svs(const char* path, lrt_buffer* nodes);
 lrt_status lrt_save_sdf_svs(const char* path, const void* nodes, size_t n_nodes);

@@ -60,7 +60,8 @@ _ARRAYS = ["nodes", "tlas", "instances", "geoms", "geom_tags", "start_end", "pri
            "vert_norm", "indices", "prim_indices", "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes",
            "sbs_data", "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "svs_nodes", "svs_roots",
            "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points", "coct_data", "coct_geoms",
-           "coct_mods"]
+           "coct_mods", "dag_nodes", "dag_child_edges", "dag_data_edges", "dag_distances", "dag_geoms", "dag_mods",
+           "rot_add"]
 
 
 class OracleScene(ctypes.Structure):

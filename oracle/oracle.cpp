@@ -107,6 +107,12 @@ struct CoctG {
   uint32_t uints_per_child_info, idx_mask, idx_sh, trans_off, rot_mask, add_mask, pad;
 };
 struct I4 { int32_t x, y, z, w; };
+// SdfDAG records (sdf/scom2/scom_shared.h:77-103) and one DAG geometry: root node, brick shape,
+// node grid size, first of its 2 x 48 index modifiers.
+struct DagChildEdge { uint32_t child_offset, rotation_id; };
+struct DagDataEdge { uint32_t data_offset, rotation_id, type_id; float add; };
+struct DagNode { uint32_t children_edges_offset, data_edges_offset; DagChildEdge channels_edge; uint32_t voxel_count_flags, pad; };
+struct DagG { uint32_t root, brick_size, brick_pad, node_grid_size, mods, pad[3]; };
 struct Preset {
   uint32_t render_mode, sdf_node_intersect, normal_mode, ray_gen_mode, interpolation_mode, representation_mode, spp,
       fixed_lod;
@@ -120,7 +126,7 @@ constexpr uint32_t LEAF_BIT = 0x80000000u, START_MASK = 0x0FFFFFFFu, SIZE_MASK =
 constexpr int STACK_SIZE = 80;
 constexpr uint32_t TAG_NONE = 0, TAG_TRIANGLE = 1, TAG_SDF_BRICK = 4, TAG_SDF_NODE = 8, TAG_SDF_OCTREE = 15;
 constexpr uint32_t TAG_SDF_GRID = 11, TYPE_SDF_GRID = 1, TAG_GRAPHICS_PRIM = 9, TYPE_GRAPHICS_PRIM = 17;
-constexpr uint32_t TAG_COCTREE_BRICKED = 2;
+constexpr uint32_t TAG_COCTREE_BRICKED = 2, TAG_SDF_DAG = 18, TYPE_SDF_DAG = 25, TYPE_SCOM2 = 26;
 // GraphicsPrimType (utils/misc/graphics_primitive_common.h:7-16)
 constexpr uint32_t GP_POINT = 0, GP_LINE = 1, GP_SEGMENT = 2, GP_SEGMENT_DIR = 3, GP_BOX = 4, GP_POINT_COLOR = 5,
                    GP_LINE_COLOR = 6, GP_SEGMENT_COLOR = 7, GP_SEGMENT_DIR_COLOR = 8, GP_BOX_COLOR = 9;
@@ -165,6 +171,13 @@ struct OracleScene {
   const uint32_t* coct_data;      // m_SdfCompactOctreeV3Data of every COctreeV3 geometry in turn
   const CoctG* coct_geoms;
   const I4* coct_mods;            // m_SdfCompactOctreeRotModifiers, 3 x 48 per geometry
+  const DagNode* dag_nodes;       // m_SdfDAGNodes and their edges and distances
+  const DagChildEdge* dag_child_edges;
+  const DagDataEdge* dag_data_edges;
+  const float* dag_distances;
+  const DagG* dag_geoms;
+  const I4* dag_mods;             // 2 x 48 index modifiers per DAG (brick lattice, node grid)
+  const uint8_t* rot_add;         // m_RotAddTable, 48 x 48
   Preset preset;
 };
 
@@ -268,11 +281,14 @@ struct Tracer {
            (dp.x) * (dp.y) * (1 - dp.z) * v[6] + (dp.x) * (dp.y) * (dp.z) * v[7];
   }
 
-  // BVH2Common.cpp:345-725 (non-compact SDF types, no cutting plane, no tex types)
+  // BVH2Common.cpp:345-725 (no cutting plane, no tex types); the compact types (DAG, SCom2,
+  // :340-343) widen EPS on surface nodes
   void LocalSurfaceIntersection(uint32_t type, f3 ray_dir, uint32_t instId, uint32_t geomId, const float values[8],
                                 uint32_t primId, float d, float qNear, float qFar, f2 fNearFar, f3 start_q, Hit* pHit,
                                 bool is_surface_node = false) {
-    const float EPS = 1e-6f;
+    const float BASE_EPS = 1e-6f;
+    const bool compact = type == TYPE_SDF_DAG || type == TYPE_SCOM2;
+    const float EPS = (is_surface_node && compact) ? std::max(BASE_EPS, s.preset.compact_sdf_eps * d) : BASE_EPS;
     const float d_inv = 1.0f / d;
     float t = qNear;
     bool early_hit = false, hit = false;
@@ -432,6 +448,18 @@ struct Tracer {
       pHit->coords[2] = en.x;
       pHit->coords[3] = en.y;
       if (s.preset.render_mode == 10) pHit->primId = iter;
+      // channel render modes: interpolation quotients instead of texture coordinates (:714-723)
+      if ((type == 9 || type == TYPE_SDF_DAG || type == TYPE_SCOM2) &&
+          (s.preset.render_mode == 23 || s.preset.render_mode == 27)) {
+        f3 dp = start_q + t * ray_dir;
+        dp = f3(std::min(std::max(0.0f, float(0xFFFFu) * dp.x), float(0xFFFFu)),
+                std::min(std::max(0.0f, float(0xFFFFu) * dp.y), float(0xFFFFu)),
+                std::min(std::max(0.0f, float(0xFFFFu) * dp.z), float(0xFFFFu)));
+        const int ux = int(dp.x), uy = int(dp.y), uz = int(dp.z);
+        const uint32_t c0 = uint32_t((uy << 16) | ux), c1 = uint32_t(uz);
+        std::memcpy(&pHit->coords[0], &c0, 4);
+        std::memcpy(&pHit->coords[1], &c1, 4);
+      }
     }
   }
 
@@ -1290,6 +1318,129 @@ struct Tracer {
     }
   }
 
+  // DAGIntersect (BVH2Common.cpp:2365-2557). The transpositions of the brick and node-grid
+  // lattices (m_SdfDAGTranspositions) are the index maps of the rotations, read through the
+  // modifiers: T_r[x v^2 + y v + z] = dot(M_r, (x, y, z, 1)). Pushes past 32 entries are dropped;
+  // a brick march stops after 4 bs + 8 voxels, a child march after 256 cells per grid step.
+  void DAGIntersect(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId, Hit* pHit) {
+    struct OTStackElement { uint32_t nodeId, info, p_size_x, p_size_y; };
+    const Geom& gm = s.geoms[geomId];
+    const DagG& header = s.dag_geoms[gm.offset[1]];
+    const I4* bricks_T = s.dag_mods + header.mods;
+    const I4* branch_T = s.dag_mods + header.mods + 48;
+    const uint32_t type = TYPE_SDF_DAG;
+    const f3 ray_dir_inv = SafeInverse(ray_dir);
+    OTStackElement stack[32];
+    int top = 0;
+    f2 fNearFar{0.0f, 0.0f};
+    float old_t = pHit->t;
+    stack[top] = OTStackElement{gm.offset[0], 0, 0, 1};
+    while (top >= 0) {
+      const OTStackElement curElem = stack[top];
+      top--;
+      const DagNode node = s.dag_nodes[curElem.nodeId];
+      const uint32_t level_sz = curElem.p_size_y & 0xFFFF;
+      const float level_sz_f = float(level_sz);
+      const f3 p(float(curElem.p_size_x >> 16), float(curElem.p_size_x & 0xFFFF), float(curElem.p_size_y >> 16));
+      const float sz_inv = 2.0f / level_sz_f;
+      const f3 brick_min_pos = f3(-1, -1, -1) + sz_inv * p;
+      const f3 brick_max_pos = brick_min_pos + sz_inv * f3(1, 1, 1);
+      f2 brick_fNearFar = RayBoxIntersection2(ray_pos, ray_dir_inv, brick_min_pos, brick_max_pos);
+      brick_fNearFar.x = std::max(tNear, brick_fNearFar.x);
+      if (brick_fNearFar.x >= brick_fNearFar.y) continue;
+      bool is_leaf = node.children_edges_offset == 0;
+      if (!is_leaf) {
+        const float t = s.preset.fixed_lod > 0 ? 1.0f : brick_fNearFar.x;
+        const float target_lod_size = (std::pow(2.0f, s.preset.level_of_detail) - 0.01f) / t;
+        is_leaf = float(level_sz) > target_lod_size;
+      }
+      if (is_leaf) {
+        if ((node.voxel_count_flags & 0xFFFFu) != 0 && node.data_edges_offset > 0) {
+          const float sz = level_sz_f;
+          const float d = 2.0f / (sz * header.brick_size);
+          const bool is_surface = ((node.voxel_count_flags >> 16) & 0x1) > 0;
+          const uint32_t surface_count = node.voxel_count_flags & 0xFFFFu;
+          const uint32_t primId = node.channels_edge.child_offset;
+          uint32_t steps = 0;
+          while (brick_fNearFar.x < brick_fNearFar.y && pHit->t == old_t && steps < 4 * header.brick_size + 8) {
+            steps++;
+            const f3 hit_pos = ray_pos + brick_fNearFar.x * ray_dir;
+            const f3 local_pos = (hit_pos - brick_min_pos) * (0.5f * sz * header.brick_size);
+            const float hi = header.brick_size - 1e-6f;
+            const f3 voxelPos(std::floor(std::min(std::max(1e-6f, local_pos.x), hi)),
+                              std::floor(std::min(std::max(1e-6f, local_pos.y), hi)),
+                              std::floor(std::min(std::max(1e-6f, local_pos.z), hi)));
+            const f3 min_pos = brick_min_pos + d * voxelPos;
+            const f3 max_pos = min_pos + d * f3(1, 1, 1);
+            uint32_t s_id = 0;
+            while (s_id < surface_count && pHit->t == old_t) {
+              const DagDataEdge& e = s.dag_data_edges[node.data_edges_offset + s_id];
+              const uint32_t rotIdx = s.rot_add[curElem.info * 48 + e.rotation_id];
+              const I4& T = bricks_T[rotIdx];
+              float values[8];
+              float vmin = 1.0f;
+              float vmax = is_surface ? -1.0f : 1.0f;
+              for (int i = 0; i < 8; i++) {
+                const int px = int(voxelPos.x) + ((i & 4) >> 2), py = int(voxelPos.y) + ((i & 2) >> 1),
+                          pz = int(voxelPos.z) + (i & 1);
+                const uint32_t vId0 = uint32_t(T.x * px + T.y * py + T.z * pz + T.w);
+                const float val = s.dag_distances[e.data_offset + vId0] + e.add;
+                values[i] = val - s.preset.compact_sdf_eps * d;
+                vmin = std::min(vmin, val);
+                vmax = std::max(vmax, val);
+              }
+              fNearFar = RayBoxIntersection2(ray_pos, SafeInverse(ray_dir), min_pos, max_pos);
+              if (fNearFar.x < fNearFar.y && vmin <= 0.0f && vmax >= 0.0f) {
+                const f3 start_pos = ray_pos + fNearFar.x * ray_dir;
+                const float vox_sz = (0.5f * sz * header.brick_size);
+                const f3 start_q = vox_sz * (start_pos - min_pos);
+                const float qNear = std::max(0.0f, vox_sz * (tNear - fNearFar.x));
+                const float qFar = vox_sz * (fNearFar.y - fNearFar.x);
+                LocalSurfaceIntersection(type, ray_dir, instId, geomId, values, primId, d, qNear, qFar, fNearFar,
+                                         start_q, pHit, is_surface);
+              }
+              s_id++;
+            }
+            brick_fNearFar.x += std::max(0.0f, fNearFar.y - brick_fNearFar.x) + 1e-5f;
+            if (pHit->t < old_t) top = -1;
+          }
+        }
+      } else {
+        const int v_sz = int(header.node_grid_size);
+        const float t_neg = (brick_fNearFar.y + tNear + 1);
+        const f3 neg_pos = ray_pos + t_neg * ray_dir;
+        const f3 neg_dir = ray_dir * -1.0f;
+        const f3 neg_dir_inv = ray_dir_inv * -1.0f;
+        f2 neg_brick_fNearFar{t_neg - brick_fNearFar.y, t_neg - brick_fNearFar.x};
+        uint32_t steps = 0;
+        while (neg_brick_fNearFar.x < neg_brick_fNearFar.y && steps < uint32_t(256 * v_sz)) {
+          steps++;
+          const float d = 2.0f / (level_sz_f * v_sz);
+          const f3 hit_pos = neg_pos + neg_brick_fNearFar.x * neg_dir;
+          const f3 local_pos = (hit_pos - brick_min_pos) * (0.5f * level_sz_f * v_sz);
+          const float hi = v_sz - 1e-6f;
+          const f3 voxelPos(std::floor(std::min(std::max(1e-6f, local_pos.x), hi)),
+                            std::floor(std::min(std::max(1e-6f, local_pos.y), hi)),
+                            std::floor(std::min(std::max(1e-6f, local_pos.z), hi)));
+          const I4& T = branch_T[curElem.info];
+          const uint32_t voxId = uint32_t(T.x * int(voxelPos.x) + T.y * int(voxelPos.y) + T.z * int(voxelPos.z) + T.w);
+          const f3 min_pos = brick_min_pos + d * voxelPos;
+          const f3 max_pos = min_pos + d * f3(1, 1, 1);
+          fNearFar = RayBoxIntersection2(neg_pos, neg_dir_inv, min_pos, max_pos);
+          const DagChildEdge edge = s.dag_child_edges[node.children_edges_offset + voxId];
+          if (tNear < fNearFar.x && edge.child_offset > 0 && top + 1 < 32) {
+            top++;
+            stack[top].nodeId = edge.child_offset;
+            stack[top].info = s.rot_add[curElem.info * 48 + edge.rotation_id];
+            stack[top].p_size_x = curElem.p_size_x * v_sz + ((uint32_t(voxelPos.x) << 16) | uint32_t(voxelPos.y));
+            stack[top].p_size_y = curElem.p_size_y * v_sz + ((uint32_t(voxelPos.z) << 16) | 0);
+          }
+          neg_brick_fNearFar.x += std::max(0.0f, fNearFar.y - neg_brick_fNearFar.x) + 0.01f * d;
+        }
+      }
+    }
+  }
+
   // BVH2Common.cpp:47-124
   uint32_t IntersectAllPrimitivesInLeaf(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
                                         uint32_t aabbId, Hit* pHit) {
@@ -1307,6 +1458,7 @@ struct Tracer {
       case TAG_SDF_GRID: IntersectSDFGrid(ray_pos, ray_dir, tNear, instId, geomId, pHit); break;
       case TAG_GRAPHICS_PRIM: IntersectGraphicPrims(ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
       case TAG_COCTREE_BRICKED: COctreeIntersectV3(ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
+      case TAG_SDF_DAG: DAGIntersect(ray_pos, ray_dir, tNear, instId, geomId, pHit); break;
       default: break;
     }
     if (s.preset.render_mode == 26) pHit->instId = (pHit->instId == 0xFFFFFFFF) ? 1 : pHit->instId + 1;

@@ -1,0 +1,149 @@
+"""SdfDAG encoder for tests: turns a global octree (sdf_spheres_global_octree) into the DAG arrays
+the reference's DAGIntersect reads (MultiRenderer/BVH2Common.cpp:2365-2557): one DAG node per
+kept octree node, a block of 8 child edges per inner node (child 0 = none), one data edge per
+brick with a surface (stored through a chosen lattice rotation and additive shift), and the
+distances. Written from the walk, not from the reference's DAG builder (which merges similar
+subtrees; this one shares nothing).
+
+Also restates, for the rotation tables, the generator of the reference's transform subgroup
+(sdf/scom2/clustering/octahedral_groups.cpp:5-81 and symmetric_groups.cpp:186-310).
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+import coct_encode as ce
+
+LEGACY_TO_UNIVERSAL = [0, 24, 32, 1, 25, 33, 8, 16, 40, 9, 17, 41,
+                       2, 26, 34, 3, 27, 35, 10, 18, 42, 11, 19, 43,
+                       4, 28, 36, 5, 29, 37, 12, 20, 44, 13, 21, 45,
+                       6, 30, 38, 7, 31, 39, 14, 22, 46, 15, 23, 47]   # octahedral_groups.h:27-35
+
+
+def universal_transpositions(v):
+    """create_default_subgroup(v, 3): the 48 index maps, each permutation of the axes (in the
+    recursive enumeration order) with each of the 8 sign patterns, then reordered by the legacy
+    remap (symmetric_groups.cpp:294-310)."""
+    perms = list(itertools.permutations(range(3)))      # same order as the recursive enumeration
+    c = v / 2.0 - 0.5
+    maps = []
+    for perm in perms:
+        for sign in range(8):
+            T = np.zeros((4, 4))
+            for d in range(3):
+                T[perm[d], d] = -1.0 if (sign >> d) & 1 else 1.0
+            T[3, 3] = 1.0
+            T1, T2 = np.eye(4), np.eye(4)
+            T1[:3, 3], T2[:3, 3] = c, -c
+            M = T1 @ T @ T2
+            idx = []
+            for i in range(v ** 3):
+                V = np.array([i // (v * v), (i // v) % v, i % v, 1.0])
+                r = M @ V
+                idx.append(int(round(r[0])) * v * v + int(round(r[1])) * v + int(round(r[2])))
+            maps.append(idx)
+    return [maps[LEGACY_TO_UNIVERSAL[i]] for i in range(48)]
+
+
+def rot_add_reference(v=2):
+    """m_RotAddTable from the subgroup: entry i*48+j = cayley[j][i], cayley[a][b] = index of the map
+    x -> e_a[e_b[x]] (compose, symmetric_groups.cpp:32-36; table, BVH2Common_host.cpp:1057-1060)."""
+    maps = universal_transpositions(v)
+    index = {tuple(m): k for k, m in enumerate(maps)}
+    out = np.zeros(48 * 48, np.uint8)
+    for i in range(48):
+        for j in range(48):
+            comp = tuple(maps[j][maps[i][x]] for x in range(v ** 3))   # cayley[j][i]
+            out[i * 48 + j] = index[comp]
+    return out
+
+
+class DagEncoder:
+    def __init__(self, nodes, values, brick_size, rotate=False, add=False, surface=False, lod=False, seed=0):
+        self.gn, self.gv = nodes, values
+        self.bs = brick_size
+        self.v = brick_size + 1
+        self.rng = np.random.default_rng(seed)
+        self.rotate, self.add, self.surface, self.lod = rotate, add, surface, lod
+        self.mods = ce._closed_mods(self.v)
+        self.cache = {}
+
+    def _brick(self, i):
+        v = self.v
+        off = int(self.gn[i][1])
+        return np.asarray(self.gv[off:off + v ** 3], np.float32).reshape(v, v, v)
+
+    def _straddles(self, i):
+        B = self._brick(i)
+        return bool(B.min() <= 0 <= B.max())
+
+    def _active(self, i):
+        if i not in self.cache:
+            off = int(self.gn[i][2])
+            self.cache[i] = self._straddles(i) or (off != 0 and any(self._active(off + c) for c in range(8)))
+        return self.cache[i]
+
+    def _data_edge(self, i):
+        B = self._brick(i)
+        v = self.v
+        r = int(self.rng.integers(0, 48)) if self.rotate else 0
+        a = np.float32(self.rng.uniform(-0.05, 0.05)) if self.add else np.float32(0)
+        m = self.mods[r]
+        stored = np.zeros(v ** 3, np.float32)
+        for x in range(v):
+            for y in range(v):
+                for z in range(v):
+                    stored[m[0] * x + m[1] * y + m[2] * z + m[3]] = np.float32(B[x, y, z] - a)
+        off = len(self.dist)
+        self.dist.extend(stored.tolist())
+        self.de.append((off, r, 0, float(a)))
+        return len(self.de) - 1
+
+    def encode(self):
+        self.nodes, self.ce, self.de, self.dist = [], [(0, 0)], [(0, 0, 0, 0.0)], []
+        order = []   # (global index, dag index) breadth first
+        index = {0: 0}
+        queue = [0]
+        while queue:
+            gi = queue.pop(0)
+            order.append(gi)
+            off = int(self.gn[gi][2])
+            if off:
+                for c in range(8):
+                    if self._active(off + c):
+                        index[off + c] = len(index)
+                        queue.append(off + c)
+        recs = {}
+        for gi in order:
+            off = int(self.gn[gi][2])
+            n = {"ce": 0, "de": 0, "count": 0, "flags": 0}
+            if off:
+                n["ce"] = len(self.ce)
+                for c in range(8):
+                    self.ce.append((index[off + c], 0) if (off + c) in index else (0, 0))
+                if self.lod and self._straddles(gi):
+                    n["de"], n["count"] = self._data_edge(gi), 1
+            elif self._straddles(gi):
+                n["de"], n["count"] = self._data_edge(gi), 1
+            recs[gi] = n
+        nodes = np.zeros(len(order), dtype=[("children_edges_offset", "<u4"), ("data_edges_offset", "<u4"),
+                                            ("channels_edge", [("child_offset", "<u4"), ("rotation_id", "<u4")]),
+                                            ("voxel_count_flags", "<u4"), ("_pad", "<u4")])
+        for gi in order:
+            k = index[gi]
+            n = recs[gi]
+            nodes[k]["children_edges_offset"] = n["ce"]
+            nodes[k]["data_edges_offset"] = n["de"]
+            nodes[k]["voxel_count_flags"] = n["count"] | ((1 << 16) if (self.surface and n["count"]) else 0)
+            nodes[k]["channels_edge"]["child_offset"] = k
+        import graphic_lab_task_amd as g
+        ce_ = np.array(self.ce, dtype=g.DAG_CHILD_EDGE_DTYPE)
+        de_ = np.array(self.de, dtype=g.DAG_DATA_EDGE_DTYPE)
+        return (g.dag_header(self.bs, 0, 2), nodes.astype(g.DAG_NODE_DTYPE), ce_, de_,
+                np.array(self.dist, np.float32))
+
+
+def encode(nodes, values, brick_size, **kw):
+    return DagEncoder(nodes, values, brick_size, **kw).encode()

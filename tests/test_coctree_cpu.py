@@ -180,6 +180,24 @@ def test_reference_file_surface(lrt, orc):
     assert checked > 200
 
 
+def test_reference_file_matches_its_frame_octree(lrt, orc):
+    """The reference's engine scenes hold one object in several formats: its COctreeV3 file (8-bit
+    values) and its frame-octree file (tests/golden/engine_sdf_frame_octree_0.bin, float corners)
+    are hit by the same rays, at t within the 8-bit quantisation."""
+    h, w, _ = lrt.load_coctree_v3(COCT)
+    pos, dir_ = sphere_rays(4000)
+    a = oracle_trace(orc, lrt, coct_scene(lrt, h, w), pos, dir_)
+    fo = lrt.load_sdf_frame_octree(os.path.join(GOLD, "engine_sdf_frame_octree_0.bin"))
+
+    def build(s):
+        s.AddInstance(s.AddGeom_SdfFrameOctree(fo))
+    b = oracle_trace(orc, lrt, host_scene(lrt, build), pos, dir_)
+    ha, hb = a["primId"] != MISS, b["primId"] != MISS
+    assert ha.mean() > 0.5 and (ha != hb).mean() < 2e-3
+    dt = np.abs(a["t"][ha & hb] - b["t"][ha & hb])
+    assert np.median(dt) < 1e-3 and np.percentile(dt, 99) < 5e-3
+
+
 def coct_leaves(words, header):
     """Every leaf of a default-pack COctreeV3 without LODs: (type, offset, code, level size, brick min)."""
     upc = int(header["uints_per_child_info"])

@@ -123,10 +123,12 @@ def test_sdf_intersection_modes(lrt, orc, torch_dev, scenes, rays, kind, mode, r
     ("frame", 10, 0), ("sbs_indexed", 10, 0), ("sbs_dx1", 10, 0), ("multi", 10, 0),
     ("svs", 10, 0), ("svs", 26, 1),
     ("frame", 26, 1), ("sbs_dx1", 26, 1), ("multi", 26, 1), ("frame", 26, 0),
-    ("multi", 24, 1), ("multi", 25, 1), ("multi", 24, 0), ("multi", 25, 3)])
+    ("multi", 24, 1), ("multi", 25, 1), ("multi", 24, 0), ("multi", 25, 3),
+    ("multi", 23, 0), ("multi", 27, 2), ("frame", 23, 0)])
 def test_sdf_debug_render_modes(lrt, orc, torch_dev, scenes, rays, kind, render_mode, mode):
     """ST_ITERATIONS (primId = sphere-tracing steps), INTERSECTION_COUNT (instId = leaf calls),
-    MO_VOXEL_COUNT / MO_FLAGS (primId = the hit node's voxel count / flags)."""
+    MO_VOXEL_COUNT / MO_FLAGS (primId = the hit node's voxel count / flags), CHANNEL /
+    CHANNEL_LAMBERT (multi-octree hits keep the voxel-local hit point in coords[0..1])."""
     _say(f"sdf render mode {render_mode} kind={kind} intersect={MODES[mode]}")
     pos, dirs = rays
     p = lrt.default_preset()
@@ -149,6 +151,8 @@ def test_sdf_debug_render_modes(lrt, orc, torch_dev, scenes, rays, kind, render_
         assert want["instId"][~hit].max() >= 2 and np.all(want["instId"][hit] == 1)
     if render_mode in (24, 25):
         assert np.all(want["primId"][hit] != MISS)
+    if render_mode in (23, 27) and kind == "multi":
+        assert np.any(want["coords"][hit, 0].view(np.uint32) != 0)
 
 
 @pytest.fixture(scope="module")
