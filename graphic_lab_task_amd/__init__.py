@@ -73,6 +73,18 @@ def dag_header(brick_size=1, brick_pad=0, node_grid_size=2) -> np.ndarray:
     return h
 
 
+# SCom2Header (sdf/scom2/scom_shared.h:21-75), 216 B
+SCOM2_HEADER_DTYPE = np.dtype([(n, "<u4") for n in (
+    "brick_size", "v_size", "bits_per_value", "values_per_uint", "value_mask", "bitmask_len", "dimension",
+    "child_rot_shift", "child_rot_mask", "child_add_shift", "child_add_mask", "child_offset_mask", "child_offset_off",
+    "node_offset_mask", "uints_per_link", "unique_brick_prefix", "unique_brick_offset_mask", "children_types_shift",
+    "children_types_mask", "base_reference_shift", "children_active_bits_shift", "children_active_bits_mask",
+    "references_offset", "reference_bits", "reference_mask", "references_per_uint", "links_offset",
+    "max_surface_count", "max_surface_count_per_leaf", "bricks_step", "bricks_arr_offset", "nodes_arr_offset",
+    "root_node_off", "has_channels", "has_surfaces", "has_multi_nodes")] +
+    [(n, "<i4") for n in ("tex_id_off", "mat_id_off", "all_float_tex_id_off", "all_int_mat_id_off")] +
+    [("max_val", "<f4"), ("max_depth", "<u4"), ("user_params", "<f4", (7,)), ("_pad", "<u4", (5,))])
+
 COCT_GEOM_DTYPE = np.dtype([(n, "<u4") for n in (
     "base", "mods", "size", "brick_size", "brick_pad", "bits_per_value", "uv_size", "sim_compression", "lods",
     "uints_per_child_info", "idx_mask", "idx_sh", "trans_off", "rot_mask", "add_mask", "_pad")])
@@ -119,6 +131,7 @@ ARRAY_DTYPES = {
     "coct_data": np.uint32, "coct_geoms": COCT_GEOM_DTYPE, "coct_mods": np.int32,
     "dag_nodes": DAG_NODE_DTYPE, "dag_child_edges": DAG_CHILD_EDGE_DTYPE, "dag_data_edges": DAG_DATA_EDGE_DTYPE,
     "dag_distances": np.float32, "dag_geoms": DAG_GEOM_DTYPE, "dag_mods": np.int32, "rot_add": np.uint8,
+    "scom2_data": np.uint32, "scom2_headers": SCOM2_HEADER_DTYPE, "scom2_mods": np.int32,
 }
 LIGHT_DTYPE = np.dtype([("space", "<f4", (3,)), ("type", "<u4"), ("color", "<f4", (3,)), ("_pad", "<u4")])  # Light, 32 B
 LIGHT_TYPE_DIRECT, LIGHT_TYPE_POINT, LIGHT_TYPE_AMBIENT = 0, 1, 2
@@ -232,6 +245,9 @@ def lib() -> ctypes.CDLL:
             "lrt_load_sdf_grid": ([cp, pu32, ctypes.POINTER(_Buffer)], i32),
             "lrt_add_coctree_v3": ([vp, vp, vp, sz, u32, pu32], i32),
             "lrt_add_sdf_dag": ([vp, vp, vp, sz, vp, sz, vp, sz, vp, sz, pu32], i32),
+            "lrt_add_scom2": ([vp, vp, vp, sz, vp, sz, pu32], i32),
+            "lrt_load_scom2": ([cp, vp, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)], i32),
+            "lrt_save_scom2": ([cp, vp, vp, sz, vp, sz], i32),
             "lrt_load_sdf_dag": ([cp, vp] + [ctypes.POINTER(_Buffer)] * 4, i32),
             "lrt_save_sdf_dag": ([cp, vp, vp, sz, vp, sz, vp, sz, vp, sz], i32),
             "lrt_load_coctree_v3": ([cp, pu32, vp, ctypes.POINTER(_Buffer)], i32),
@@ -306,7 +322,8 @@ def exported_symbols():
             "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_svs", "lrt_load_sdf_svs", "lrt_save_sdf_svs",
             "lrt_add_sdf_grid", "lrt_load_sdf_grid", "lrt_save_sdf_grid", "lrt_add_graphics_prims",
             "lrt_add_coctree_v3", "lrt_load_coctree_v3", "lrt_save_coctree_v3",
-            "lrt_add_sdf_dag", "lrt_load_sdf_dag", "lrt_save_sdf_dag", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
+            "lrt_add_sdf_dag", "lrt_load_sdf_dag", "lrt_save_sdf_dag", "lrt_add_scom2", "lrt_load_scom2",
+            "lrt_save_scom2", "lrt_add_sdf_sbs", "lrt_add_sdf_multi_octree", "lrt_sdf_spheres_multi_octree", "lrt_add_instance",
             "lrt_update_instance", "lrt_commit", "lrt_set_preset", "lrt_trace_closest_dev",
             "lrt_trace_closest_host", "lrt_trace_closest_noalloc_dev", "lrt_trace_any_dev", "lrt_trace_closest_counted_dev", "lrt_query_one", "lrt_eye_rays_dev",
             "lrt_set_launch_config", "lrt_set_tail_config", "lrt_set_leaf_postponement", "lrt_set_ray_interleave", "lrt_set_persistent",
@@ -449,6 +466,15 @@ class SceneRT:
         out = ctypes.c_uint32()
         _check(lib().lrt_add_sdf_dag(self._h, _ptr(h), _ptr(n), n.size, _ptr(ce), ce.size, _ptr(de), de.size, _ptr(d),
                                      d.size, ctypes.byref(out)))
+        return out.value
+
+    def AddGeom_SCom2(self, header: np.ndarray, nodes: np.ndarray, bricks: np.ndarray) -> int:
+        """BVHRT::AddGeom_SCom2 (BVH2Common_host.cpp:1133-1191): an SCom2Tree's header, node and brick words."""
+        h = np.ascontiguousarray(header, dtype=SCOM2_HEADER_DTYPE).reshape(())
+        n = np.ascontiguousarray(nodes, dtype=np.uint32).ravel()
+        b = np.ascontiguousarray(bricks, dtype=np.uint32).ravel()
+        out = ctypes.c_uint32()
+        _check(lib().lrt_add_scom2(self._h, _ptr(h), _ptr(n), n.size, _ptr(b), b.size, ctypes.byref(out)))
         return out.value
 
     def AddGeom_SdfSVS(self, nodes: np.ndarray) -> int:
@@ -886,6 +912,22 @@ def save_sdf_dag(path: str, header, nodes, child_edges, data_edges, distances):
     d = np.ascontiguousarray(distances, dtype=np.float32)
     _check(lib().lrt_save_sdf_dag(os.fsencode(path), _ptr(h), _ptr(n), n.size, _ptr(ce), ce.size, _ptr(de), de.size,
                                   _ptr(d), d.size))
+
+
+def load_scom2(path: str):
+    """load_scom2 (sdf/scom2/scom_utils.cpp:153-195): (header, node words, brick words)."""
+    h = np.zeros((), SCOM2_HEADER_DTYPE)
+    nb, bb = _Buffer(), _Buffer()
+    _check(lib().lrt_load_scom2(os.fsencode(path), _ptr(h), ctypes.byref(nb), ctypes.byref(bb)))
+    return h, _take(nb, np.uint32), _take(bb, np.uint32)
+
+
+def save_scom2(path: str, header, nodes, bricks):
+    """save_scom2 (sdf/scom2/scom_utils.cpp:98-127), version 4 without channels."""
+    h = np.ascontiguousarray(header, dtype=SCOM2_HEADER_DTYPE).reshape(())
+    n = np.ascontiguousarray(nodes, dtype=np.uint32).ravel()
+    b = np.ascontiguousarray(bricks, dtype=np.uint32).ravel()
+    _check(lib().lrt_save_scom2(os.fsencode(path), _ptr(h), _ptr(n), n.size, _ptr(b), b.size))
 
 
 def load_sdf_svs(path: str) -> np.ndarray:

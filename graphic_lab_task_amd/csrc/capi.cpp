@@ -93,10 +93,28 @@ lrt::TraceArgs trace_args(lrt_scene* s, const float* pos, const float* dir, lrt_
   return a;
 }
 
+// SCom2 queries this library does not build: the reference's fast traversal, which it takes for a
+// scene holding one plain SCom2 tree rendered LAMBERT_NO_TEX (BVH2Common_host.cpp:1436-1451,
+// SCom2IntersectFast), and the channel render modes of trees with channels (their quotients are
+// rotated by float matrices). Returns the reason, or null.
+const char* unsupported_scom2(const lrt::Scene& sc) {
+  const uint32_t rm = sc.preset.render_mode;
+  for (const lrt::SCom2Header& h : sc.scom2Headers)
+    if (h.has_channels && (rm == lrt::MULTI_RENDER_MODE_CHANNEL || rm == lrt::MULTI_RENDER_MODE_CHANNEL_LAMBERT))
+      return "channel render modes of SCom2 trees with channels are not built";
+  if (sc.scom2Headers.size() == 1 && rm == lrt::MULTI_RENDER_MODE_LAMBERT_NO_TEX) {
+    const lrt::SCom2Header& h = sc.scom2Headers[0];
+    if (h.has_multi_nodes == 0 && h.has_channels == 0 && h.v_size == 3 && h.links_offset == 1 && h.child_offset_off == 0)
+      return "the fast traversal of a single plain SCom2 tree (LAMBERT_NO_TEX) is not built";
+  }
+  return nullptr;
+}
+
 lrt_status trace_common(lrt_scene* s, const float* pos, const float* dir, lrt_hit* hits, uint32_t* any, uint64_t n,
                         void* stream, uint32_t* counts = nullptr) {
   if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
   if (!s->s.committed) return fail(LRT_ERR_NOT_COMMITTED, "scene is not committed");
+  if (const char* why = unsupported_scom2(s->s)) return fail(LRT_ERR_UNSUPPORTED, why);
   if (n == 0) return LRT_OK;
   if (!pos || !dir || (!hits && !any)) return fail(LRT_ERR_INVALID_ARG, "null ray or hit buffer");
   if (n > 0xFFFFFFFFull) return fail(LRT_ERR_INVALID_ARG, "too many rays in one call");
@@ -318,6 +336,17 @@ lrt_status lrt_add_sdf_dag(lrt_scene* s, const void* header, const void* nodes, 
   });
 }
 
+lrt_status lrt_add_scom2(lrt_scene* s, const void* header, const uint32_t* nodes, size_t n_nodes,
+                         const uint32_t* bricks, size_t n_bricks, uint32_t* out_geom_id) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !header || !nodes || !bricks) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    const uint32_t id =
+        s->s.add_scom2(*static_cast<const lrt::SCom2Header*>(header), nodes, n_nodes, bricks, n_bricks);
+    if (out_geom_id) *out_geom_id = id;
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_add_sdf_svs(lrt_scene* s, const void* nodes, size_t n_nodes, uint32_t* out_geom_id) {
   return guarded([&]() -> lrt_status {
     if (!s || !nodes) return fail(LRT_ERR_INVALID_ARG, "null argument");
@@ -445,6 +474,7 @@ lrt_status lrt_trace_closest_noalloc_dev(lrt_scene* s, const float* pos_near, co
     hipError_t e = hipGetDevice(&device);
     if (e != hipSuccess) return hip_fail(e, "current device");
     if (device != s->s.device) return fail(LRT_ERR_INVALID_ARG, "the current device is not the scene's device");
+    if (const char* why = unsupported_scom2(s->s)) return fail(LRT_ERR_UNSUPPORTED, why);
     const lrt::TraceArgs a = trace_args(s, pos_near, dir_far, hits, nullptr, n, nullptr);
     e = lrt::launch_trace_noalloc(s->s.make_view(), a, device, reinterpret_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "allocation-free trace launch");
@@ -1043,6 +1073,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
     ARR("dag_nodes", c.dagNodes) ARR("dag_child_edges", c.dagChildEdges) ARR("dag_data_edges", c.dagDataEdges)
     ARR("dag_distances", c.dagDistances) ARR("dag_geoms", c.dagGeoms) ARR("dag_mods", c.dagMods)
     if (n == "rot_add") { src = lrt::rot_add_table().data(); sz = lrt::rot_add_table().size(); }
+    ARR("scom2_data", c.scom2Data) ARR("scom2_headers", c.scom2Headers) ARR("scom2_mods", c.scom2Mods)
 #undef ARR
     if (n == "preset") { src = &c.preset; sz = sizeof(c.preset); }
     std::vector<uint32_t> mask_copy;   // the committed occupancy masks live on the device only
@@ -1062,7 +1093,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
       static const char* known[] = {"nodes", "tlas", "instances", "geoms", "geom_tags", "start_end",
                                     "prim_id_count", "vert_pos", "vert_norm", "indices", "prim_indices",
                                     "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes", "sbs_data",
-                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices", "svs_nodes", "svs_roots", "tlas_parent", "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points", "coct_data", "coct_geoms", "coct_mods", "dag_nodes", "dag_child_edges", "dag_data_edges", "dag_distances", "dag_geoms", "dag_mods", "rot_add",
+                                    "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices", "svs_nodes", "svs_roots", "tlas_parent", "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points", "coct_data", "coct_geoms", "coct_mods", "dag_nodes", "dag_child_edges", "dag_data_edges", "dag_distances", "dag_geoms", "dag_mods", "rot_add", "scom2_data", "scom2_headers", "scom2_mods",
                                     "sbs_mask"};
       bool ok = false;
       for (const char* k : known) ok = ok || n == k;

@@ -160,6 +160,28 @@ struct SdfDAGNode {
 };
 static_assert(sizeof(SdfDAGHeader) == 64 && sizeof(SdfDAGNode) == 24 && sizeof(SdfDAGChildEdge) == 8 &&
               sizeof(SdfDAGDataEdge) == 16, "SdfDAG layouts");
+// SCom2Header (sdf/scom2/scom_shared.h:21-75): 54 words describing how an SCom2 tree packs its
+// node heads, links and bricks into one u32 array.
+struct SCom2Header {
+  uint32_t brick_size, v_size, bits_per_value, values_per_uint, value_mask, bitmask_len, dimension;
+  uint32_t child_rot_shift, child_rot_mask, child_add_shift, child_add_mask, child_offset_mask, child_offset_off;
+  uint32_t node_offset_mask, uints_per_link, unique_brick_prefix, unique_brick_offset_mask;
+  uint32_t children_types_shift, children_types_mask, base_reference_shift, children_active_bits_shift;
+  uint32_t children_active_bits_mask, references_offset, reference_bits, reference_mask, references_per_uint;
+  uint32_t links_offset, max_surface_count, max_surface_count_per_leaf;
+  uint32_t bricks_step, bricks_arr_offset, nodes_arr_offset, root_node_off;
+  uint32_t has_channels, has_surfaces, has_multi_nodes;
+  int32_t  tex_id_off, mat_id_off, all_float_tex_id_off, all_int_mat_id_off;
+  float    max_val;
+  uint32_t max_depth;
+  float    user_params[7];
+  uint32_t _pad0, _pad1, _pad2, _pad3, _pad4;
+};
+static_assert(sizeof(SCom2Header) == 216, "SCom2Header layout");
+constexpr uint32_t SCOM2_CHILD_EMPTY = 0, SCOM2_CHILD_LEAF_VOLUME = 1, SCOM2_CHILD_NODE = 2,
+                   SCOM2_CHILD_LEAF_SURFACE = 3, SCOM2_CHILD_TYPE_BITS = 2, SCOM2_CHILD_TYPE_MASK = 3;
+constexpr uint32_t SCOM2_MAGIC_NUMBER = 0xffffdefau, SCOM2_VERSION = 4;
+
 // One DAG geometry as the kernels read it (ours): its root node, brick shape, node grid size and
 // the first of its 2 x 48 index modifiers (brick value lattice, node grid).
 struct DagGeom {
@@ -260,6 +282,7 @@ constexpr uint32_t TAG_SDF_GRID   = 11;
 constexpr uint32_t TAG_GRAPHICS_PRIM = 9;
 constexpr uint32_t TAG_SDF_OCTREE = 15;
 constexpr uint32_t TAG_SDF_DAG    = 18;
+constexpr uint32_t TAG_SCOM2      = 19;
 
 // SBS node layouts (sdf/simple/sdf_simple_shared.h:13-21)
 constexpr uint32_t SDF_SBS_NODE_LAYOUT_UNDEFINED        = 0u << 24;

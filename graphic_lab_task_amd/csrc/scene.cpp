@@ -149,6 +149,7 @@ void Scene::clear_geom() {
   gprimHeaders.clear(); gprimPoints.clear();
   coctData.clear(); coctGeoms.clear(); coctMods.clear();
   dagNodes.clear(); dagChildEdges.clear(); dagDataEdges.clear(); dagDistances.clear(); dagGeoms.clear(); dagMods.clear();
+  scom2Data.clear(); scom2Headers.clear(); scom2Mods.clear();
   geom_dirty = true;
   clear_scene();
 }
@@ -594,6 +595,7 @@ uint32_t Scene::leaf_kinds() const {
     else if (t == TAG_GRAPHICS_PRIM) k |= LK_GPRIM;
     else if (t == TAG_COCTREE_BRICKED) k |= LK_COCT;
     else if (t == TAG_SDF_DAG) k |= LK_DAG;
+    else if (t == TAG_SCOM2) k |= LK_SCOM2;
   }
   return k;
 }
@@ -842,7 +844,8 @@ hipError_t Scene::commit() {
   UP("coct_data", coctData) UP("coct_geoms", coctGeoms) UP("coct_mods", coctMods)
   UP("dag_nodes", dagNodes) UP("dag_child_edges", dagChildEdges) UP("dag_data_edges", dagDataEdges)
   UP("dag_distances", dagDistances) UP("dag_geoms", dagGeoms) UP("dag_mods", dagMods)
-  if (!dagGeoms.empty()) { UP("rot_add", rot_add_table()) }
+  UP("scom2_data", scom2Data) UP("scom2_headers", scom2Headers) UP("scom2_mods", scom2Mods)
+  if (!dagGeoms.empty() || !scom2Headers.empty()) { UP("rot_add", rot_add_table()) }
   // MultiRenderer's own m_instanceTransformInvTransposed (MultiRenderer_host.cpp:870-878):
   // transpose(inverse4x4(matrix)), refreshed by UpdateInstance (unlike the BVHRT copy)
   std::vector<float4x4> inst_inv_t(instances.size());
@@ -918,6 +921,9 @@ SceneView Scene::make_view() const {
   v.dag_geoms = (const DagGeom*)P("dag_geoms");
   v.dag_mods = (const int4*)P("dag_mods");
   v.rot_add = (const uint8_t*)P("rot_add");
+  v.scom2_data = (const uint32_t*)P("scom2_data");
+  v.scom2_headers = (const SCom2Header*)P("scom2_headers");
+  v.scom2_mods = (const int4*)P("scom2_mods");
   v.preset = preset;
   v.lod_thr = lod_thr;
   v.leaf_kinds = leaf_kinds();

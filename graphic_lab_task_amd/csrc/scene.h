@@ -79,6 +79,9 @@ struct Scene {
   std::vector<float> dagDistances;
   std::vector<DagGeom> dagGeoms;
   std::vector<int4> dagMods;         // 2 x 48 index modifiers per DAG geometry
+  std::vector<uint32_t> scom2Data;   // each SCom2 tree's nodes then bricks
+  std::vector<SCom2Header> scom2Headers;
+  std::vector<int4> scom2Mods;       // 3 x 48 index modifiers per SCom2 tree
   std::vector<InstanceData> instances;
   std::vector<BVHNode> tlas;
 
@@ -164,6 +167,8 @@ struct Scene {
   uint32_t add_sdf_dag(const SdfDAGHeader& h, const SdfDAGNode* nodes, size_t n_nodes,
                        const SdfDAGChildEdge* child_edges, size_t n_child_edges, const SdfDAGDataEdge* data_edges,
                        size_t n_data_edges, const float* distances, size_t n_distances);
+  uint32_t add_scom2(const SCom2Header& header, const uint32_t* nodes, size_t n_nodes, const uint32_t* bricks,
+                     size_t n_bricks);
   uint32_t add_coctree_v3(const COctreeV3Header& h, const uint32_t* data, size_t size, uint32_t bvh_level);
   uint32_t add_sdf_multi_octree(const SdfMultiOctreeNode* nodes, size_t n_nodes, const float* values,
                                 size_t n_values);

@@ -22,6 +22,7 @@ enum LeafKind : uint32_t {
   LK_GPRIM = 1u << 7,   // TAG_GRAPHICS_PRIM (points, lines, segments, arrows, wire boxes)
   LK_COCT  = 1u << 8,   // TAG_COCTREE_BRICKED (compact octree COctreeV3)
   LK_DAG   = 1u << 9,   // TAG_SDF_DAG (octree DAG with rotated shared nodes and bricks)
+  LK_SCOM2 = 1u << 10,  // TAG_SCOM2 (packed SCom2 tree)
 };
 
 // Frame-octree leaf as the kernels read it: the leaf box, its 8 corner distances and the
@@ -92,6 +93,10 @@ struct SceneView {
   const DagGeom*            dag_geoms;     // per DAG geometry (GeomData.offset.y)
   const int4*               dag_mods;      // 2 x 48 index modifiers per DAG geometry
   const uint8_t*            rot_add;       // m_RotAddTable: 48 x 48 rotation compositions
+  // SCom2 trees
+  const uint32_t*           scom2_data;    // m_SCom2Data: each tree's nodes then bricks
+  const SCom2Header*        scom2_headers; // m_SCom2Headers (GeomData.offset.y), offsets made global
+  const int4*               scom2_mods;    // 3 x 48 index modifiers per tree
 
   MultiRenderPreset preset;
   float    lod_thr;                 // m_precomputedLoDThr

@@ -229,6 +229,47 @@ lrt_status lrt_save_sdf_dag(const char* path, const void* header, const void* no
   });
 }
 
+lrt_status lrt_load_scom2(const char* path, void* header, lrt_buffer* nodes, lrt_buffer* bricks) {
+  return io_guarded([&]() -> lrt_status {
+    if (!header || !nodes || !bricks) throw std::runtime_error("null output");
+    File fl(path, "rb");
+    size_t left = size_t(fl.size());
+    uint32_t head[6];
+    take(fl, head, 8, left, "SCom2 magic and version");
+    if (head[0] != lrt::SCOM2_MAGIC_NUMBER)
+      throw std::runtime_error("not an SCom2 file of the current format (no magic number; legacy files are not read)");
+    if (head[1] != lrt::SCOM2_VERSION)
+      throw std::runtime_error("SCom2 version " + std::to_string(head[1]) + " (the reference reads version 4 only)");
+    take(fl, head + 2, 16, left, "SCom2 counts");
+    if (head[4] || head[5])
+      throw std::runtime_error("SCom2 files with data channels are not read (the intersection does not use them)");
+    take(fl, header, sizeof(lrt::SCom2Header), left, "SCom2 header");
+    if (uint64_t(head[2]) * 4 > left) throw std::runtime_error("SCom2 nodes exceed the file");
+    std::vector<char> n(size_t(head[2]) * 4);
+    take(fl, n.data(), n.size(), left, "SCom2 nodes");
+    if (uint64_t(head[3]) * 4 > left) throw std::runtime_error("SCom2 bricks exceed the file");
+    std::vector<char> b(size_t(head[3]) * 4);
+    take(fl, b.data(), b.size(), left, "SCom2 bricks");
+    to_buf(n, nodes);
+    return to_buf(b, bricks);
+  });
+}
+
+lrt_status lrt_save_scom2(const char* path, const void* header, const uint32_t* nodes, size_t n_nodes,
+                          const uint32_t* bricks, size_t n_bricks) {
+  return io_guarded([&]() -> lrt_status {
+    if (!header || (!nodes && n_nodes) || (!bricks && n_bricks)) throw std::runtime_error("null argument");
+    if (n_nodes > 0xFFFFFFFFull || n_bricks > 0xFFFFFFFFull) throw std::runtime_error("too many words for a u32 count");
+    const uint32_t head[6] = {lrt::SCOM2_MAGIC_NUMBER, lrt::SCOM2_VERSION, uint32_t(n_nodes), uint32_t(n_bricks), 0, 0};
+    File fl(path, "wb");
+    put(fl, head, sizeof(head));
+    put(fl, header, sizeof(lrt::SCom2Header));
+    put(fl, nodes, n_nodes * 4);
+    put(fl, bricks, n_bricks * 4);
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_load_sdf_sbs(const char* path, void* header, lrt_buffer* nodes, lrt_buffer* values,
                             lrt_buffer* values_f) {
   return io_guarded([&]() -> lrt_status {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include "lrt_math.h"
 #include "scene_view.h"
+#include "scom2_unpack.h"
 
 namespace lrt {
 
@@ -1660,6 +1661,190 @@ __device__ __forceinline__ void dag_intersect(const SceneView& s, const GeomRef&
         }
         nbnf.x += fmax_ref(0.0f, cnf.y - nbnf.x) + 0.01f * d;
       }
+    }
+  }
+}
+
+}  // namespace lrt
+
+namespace lrt {
+
+// ---- SCom2 tree (BVHRT::SCom2Intersect, MultiRenderer/BVH2Common.cpp:2560-2818) ----
+// Front-to-back octree walk over the packed node heads with mirrored coordinates (octant mask a);
+// each entry carries its node's cumulative rotation (8 bits) and, for leaves, the surface count
+// and surface flag. A leaf marches its brick's voxels and intersects every surface link's brick
+// (decoded through the composed rotation and the edge's shift); inner nodes coarser than the level
+// of detail are taken as leaves through their own links. Stack pushes past 32 entries are dropped
+// (the reference writes past its array); a brick march stops after 4 bs + 8 voxels, which only
+// cuts marches whose advance is lost to rounding.
+__device__ __forceinline__ void scom2_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir,
+                                                float tNear, uint32_t instId, uint32_t geomId, CRT_Hit& hit) {
+  const SCom2Header* H = s.scom2_headers + g.offset.y;
+  const SCom2Header& h = *H;
+  const uint32_t* D = s.scom2_data;
+  const int4* M = s.scom2_mods + g.offset.y * (3 * COCTREE_ROT_COUNT);
+  float3 pos_ray_pos = ray_pos;
+  float3 pos_ray_dir = ray_dir;
+  uint32_t a = 0;
+  if (ray_dir.x < 0) { pos_ray_pos.x *= -1; pos_ray_dir.x *= -1; a |= 4; }
+  if (ray_dir.y < 0) { pos_ray_pos.y *= -1; pos_ray_dir.y *= -1; a |= 2; }
+  if (ray_dir.z < 0) { pos_ray_pos.z *= -1; pos_ray_dir.z *= -1; a |= 1; }
+  pos_ray_pos = -1.0f * pos_ray_pos;
+  const float3 inv = safe_inverse(pos_ray_dir);
+  const float3 t0_ = pos_ray_pos * inv - inv;
+  const float3 t1_ = pos_ray_pos * inv + inv;
+  const float3 l_ = t1_ - t0_;
+  const uint32_t nn[8] = {0x124u, 0x835u, 0x386u, 0x887u, 0x568u, 0x878u, 0x788u, 0x888u};
+  constexpr int SC_STACK = 32;
+  uint32_t st_node[SC_STACK], st_info[SC_STACK], st_pxy[SC_STACK], st_pzs[SC_STACK];
+  int top = 0;
+  const uint32_t type = g.type;
+  const uint32_t bs = h.brick_size;
+  const uint32_t has_ch = h.has_channels;
+  const uint32_t upl = h.uints_per_link, coo = h.child_offset_off, lo = h.links_offset;
+  st_node[0] = h.nodes_arr_offset + h.root_node_off;
+  st_info[0] = 0;
+  st_pxy[0] = 0;
+  st_pzs[0] = 1;
+  while (top >= 0) {
+    uint32_t nodeId = st_node[top];
+    const uint32_t info = st_info[top], pxy = st_pxy[top], pzs = st_pzs[top];
+    top--;
+    const uint32_t level_sz = pzs & 0xFFFFu;
+    const float level_sz_f = float(level_sz);
+    const uint32_t px = pxy >> 16, py = pxy & 0xFFFFu, pz = pzs >> 16;
+    float d = 1.0f / float(level_sz);
+    const float3 p_f = make3(float(px), float(py), float(pz));
+    const float3 t0 = t0_ + d * p_f * l_;
+    const float3 t1 = t0_ + d * (p_f + 1.0f) * l_;
+    if (fmin_ref(t1.x, fmin_ref(t1.y, t1.z)) < tNear) continue;
+    bool is_leaf = info > 0xFFu;
+    bool is_surface = info > 0xFFFFFFu;
+    uint32_t surface_count = (info >> 8) & 0xFFFFu;
+    if (!is_leaf) {
+      const float t = s.preset.fixed_lod > 0 ? 1.0f : fmax_ref(t0.x, fmax_ref(t0.y, t0.z));
+      const float target_lod_size = (powf(2.0f, s.preset.level_of_detail) - 0.01f) / t;
+      is_leaf = float(level_sz) > target_lod_size;
+      if (is_leaf) {
+        const SCom2NodeHead node = scom2_node_head(h, D[nodeId], D[nodeId + lo - 1]);
+        if (node.base_type == SCOM2_CHILD_EMPTY) continue;
+        is_surface = (node.base_type & SCOM2_CHILD_TYPE_MASK) == SCOM2_CHILD_LEAF_SURFACE;
+        nodeId = nodeId + lo;
+        surface_count = node.base_links_end - has_ch;
+      }
+    }
+    if (is_leaf) {
+      const float sz_inv = 2.0f / level_sz_f;
+      const uint32_t p_mask = level_sz - 1;
+      const uint32_t rx = (a & 4) ? (~px & p_mask) : px;
+      const uint32_t ry = (a & 2) ? (~py & p_mask) : py;
+      const uint32_t rz = (a & 1) ? (~pz & p_mask) : pz;
+      const float3 brick_min_pos = make3(-1, -1, -1) + sz_inv * make3(float(rx), float(ry), float(rz));
+      float tmin = fmax_ref(t0.x, fmax_ref(t0.y, t0.z));
+      const float tmax = fmin_ref(t1.x, fmin_ref(t1.y, t1.z));
+      const uint32_t rotId = info & 0xFFu;
+      const float max_val = scom2_max_sdf_val(level_sz_f);
+      d = 2.0f / (level_sz_f * float(bs));
+      const float old_t = hit.t;
+      const float scale = 0.5f * level_sz_f * float(bs);
+      const float hi = float(bs) - 1e-6f;
+      uint32_t steps = 0;
+      while (tmin < tmax && hit.t == old_t && steps < 4 * bs + 8) {
+        steps++;
+        const float3 hit_pos = ray_pos + tmin * ray_dir;
+        const float3 local_pos = (hit_pos - brick_min_pos) * scale;
+        const float3 voxelPos = make3(floorf(clamp_ref(local_pos.x, 1e-6f, hi)), floorf(clamp_ref(local_pos.y, 1e-6f, hi)),
+                                      floorf(clamp_ref(local_pos.z, 1e-6f, hi)));
+        const float3 min_pos = brick_min_pos + d * voxelPos;
+        const float3 max_pos = min_pos + d * make3(1, 1, 1);
+        const float2 fNearFar = ray_box2(ray_pos, safe_inverse(ray_dir), min_pos, max_pos);
+        const int vx = int(voxelPos.x), vy = int(voxelPos.y), vz = int(voxelPos.z);
+        for (uint32_t s_id = 0; fNearFar.x < fNearFar.y && s_id < surface_count && hit.t == old_t; ++s_id) {
+          const uint32_t link_0 = nodeId + s_id * upl;
+          const SCom2DataEdge de = scom2_data_edge(h, max_val, D[link_0], D[link_0 + coo]);
+          const uint32_t offset = h.bricks_arr_offset + h.bricks_step * de.data_offset;
+          const int4 m = M[s.rot_add[rotId * COCTREE_ROT_COUNT + de.rotation_id]];
+          float values[8];
+          float vmin = 1.0f;
+          float vmax = is_surface ? -1.0f : 1.0f;
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            const int cx = vx + ((i & 4) >> 2), cy = vy + ((i & 2) >> 1), cz = vz + (i & 1);
+            const uint32_t vId0 = uint32_t(m.x * cx + m.y * cy + m.z * cz + m.w);
+            const uint32_t p_val = D[offset + vId0 / h.values_per_uint];
+            const uint32_t p_off = (vId0 % h.values_per_uint) * h.bits_per_value;
+            const float val =
+                max_val * (2.0f * float((p_val >> p_off) & h.value_mask) / float(h.value_mask) - 1.0f) + de.add;
+            values[i] = val - s.preset.compact_sdf_eps * d;
+            vmin = fmin_ref(vmin, val);
+            vmax = fmax_ref(vmax, val);
+          }
+          if (vmin <= 0.0f && vmax >= 0.0f) {
+            const float3 start_pos = ray_pos + fNearFar.x * ray_dir;
+            const float vox_sz = 0.5f * level_sz_f * float(bs);
+            const float3 start_q = vox_sz * (start_pos - min_pos);
+            const float qNear = fmax_ref(0.0f, vox_sz * (tNear - fNearFar.x));
+            const float qFar = vox_sz * (fNearFar.y - fNearFar.x);
+            local_surface_intersection(s, type, ray_dir, instId, geomId, values, nodeId, d, qNear, qFar, fNearFar,
+                                       start_q, hit, nullptr, is_surface);
+          }
+          // with channels the hit names the leaf's channel link (MULTI_RENDER_MODE_PRIMITIVE excepted)
+          if (hit.t != old_t && has_ch > 0 && s.preset.render_mode != MULTI_RENDER_MODE_PRIMITIVE) {
+            const uint32_t ch_off = nodeId + surface_count * upl;
+            hit.primId = scom2_child_edge(h, D[ch_off], D[ch_off + coo]).child_offset;
+          }
+        }
+        tmin += fmax_ref(0.0f, fNearFar.y - tmin) + 1e-5f;
+        if (hit.t < old_t) {
+          top = -1;
+          if (s.preset.render_mode == MULTI_RENDER_MODE_LOD) hit.primId = uint32_t(log2f(float(level_sz * bs)));
+        }
+      }
+    } else {
+      const SCom2NodeHead node = scom2_node_head(h, D[nodeId], D[nodeId + lo - 1]);
+      const uint32_t node1 = D[nodeId + h.references_offset];
+      const uint32_t node2 = D[nodeId + lo - 1];
+      uint32_t b_node[4], b_info[4], b_pxy[4], b_pzs[4];
+      int buf_top = 0;
+      const float3 tm = 0.5f * (t0 + t1);
+      int curr = ot_first_node(t0, tm);
+      const int4 mc = M[2 * COCTREE_ROT_COUNT + info];
+      do {
+        const uint32_t raw = uint32_t(curr) ^ a;
+        const uint32_t voxId = uint32_t(mc.x * int((raw & 4) >> 2) + mc.y * int((raw & 2) >> 1) + mc.z * int(raw & 1) + mc.w);
+        const uint32_t ch_type = (node.children_types >> (SCOM2_CHILD_TYPE_BITS * voxId)) & SCOM2_CHILD_TYPE_MASK;
+        const uint32_t ch_start = voxId == 0 ? node.base_links_end : scom2_links_end(h, voxId - 1, node1, node2);
+        const uint32_t ch_end = scom2_links_end(h, voxId, node1, node2);
+        if (ch_type != SCOM2_CHILD_EMPTY && buf_top < 4) {
+          b_pxy[buf_top] = (pxy << 1) | (((uint32_t(curr) & 4u) << 14) | ((uint32_t(curr) & 2u) >> 1));
+          b_pzs[buf_top] = (pzs << 1) | ((uint32_t(curr) & 1u) << 16);
+          const uint32_t link_off = nodeId + lo + ch_start * upl;
+          if (ch_type == SCOM2_CHILD_NODE) {
+            const SCom2ChildEdge ce = scom2_child_edge(h, D[link_off], D[link_off + coo]);
+            b_node[buf_top] = h.nodes_arr_offset + ce.child_offset;
+            b_info[buf_top] = s.rot_add[info * COCTREE_ROT_COUNT + ce.rotation_id];
+          } else {
+            const uint32_t surf_count = ch_end - ch_start - has_ch;
+            b_node[buf_top] = link_off;
+            b_info[buf_top] = (info & 0xFFu) | (surf_count << 8) | (ch_type == SCOM2_CHILD_LEAF_SURFACE ? (1u << 24) : 0u);
+          }
+          buf_top++;
+        }
+        const uint32_t ni = nn[curr];
+        curr = ot_new_node((curr & 4) ? t1.x : tm.x, int(ni & 0xF), (curr & 2) ? t1.y : tm.y, int((ni >> 4) & 0xF),
+                           (curr & 1) ? t1.z : tm.z, int(ni >> 8));
+      } while (curr < 8);
+      for (int i = 1; i <= buf_top; i++) {
+        const int k = top + i;
+        if (k < SC_STACK) {
+          st_node[k] = b_node[buf_top - i];
+          st_info[k] = b_info[buf_top - i];
+          st_pxy[k] = b_pxy[buf_top - i];
+          st_pzs[k] = b_pzs[buf_top - i];
+        }
+      }
+      top += buf_top;
+      if (top >= SC_STACK) top = SC_STACK - 1;
     }
   }
 }

@@ -90,6 +90,8 @@ hipError_t launch_trace(const SceneView& s, const TraceArgs& a, hipStream_t stre
     e = launch_trace_coct(s, a, stream);
   else if (k == LK_DAG)
     e = launch_trace_dag(s, a, stream);
+  else if (k == LK_SCOM2)
+    e = launch_trace_scom2(s, a, stream);
   else
     e = launch_trace_mixed(s, a, stream);
   // a call whose second pass did not launch must not leave queued rays to the next call

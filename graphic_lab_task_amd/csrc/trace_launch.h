@@ -337,6 +337,7 @@ hipError_t launch_trace_multi(const SceneView& s, const TraceArgs& a, hipStream_
 hipError_t launch_trace_mixed(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_trace_coct(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_trace_dag(const SceneView& s, const TraceArgs& a, hipStream_t stream);
+hipError_t launch_trace_scom2(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_trace_sbs_smooth(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_trace_mixed_smooth(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 

@@ -56,6 +56,8 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
       coct_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
     } else if ((KINDS & LK_DAG) && tag == TAG_SDF_DAG) {
       dag_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, hit);
+    } else if ((KINDS & LK_SCOM2) && tag == TAG_SCOM2) {
+      scom2_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, hit);
     }
     if (s.preset.render_mode == MULTI_RENDER_MODE_INTERSECTION_COUNT)
       hit.instId = (hit.instId == 0xFFFFFFFFu) ? 1 : hit.instId + 1;
@@ -74,10 +76,10 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
   // box entry, past the final hit, so the result only differs where a triangle's t rounds below
   // its own box entry by a few ulp and ties the final hit that closely.
   const bool postpone = !COUNT && (((KINDS & LK_TRI_POSTPONE) != 0) ||
-                                   ((KINDS & (LK_FRAME | LK_SBS | LK_MULTI | LK_GRID | LK_GPRIM | LK_COCT | LK_DAG)) != 0 &&
+                                   ((KINDS & (LK_FRAME | LK_SBS | LK_MULTI | LK_GRID | LK_GPRIM | LK_COCT | LK_DAG | LK_SCOM2)) != 0 &&
                                     tag != TAG_TRIANGLE));
   uint32_t postponed = 0;   // the noted leaf word (leaf words always have LEAF_BIT set)
-  constexpr bool LEAF_HALF = (KINDS & (LK_SBS | LK_MULTI | LK_GRID | LK_COCT | LK_DAG)) == 0;
+  constexpr bool LEAF_HALF = (KINDS & (LK_SBS | LK_MULTI | LK_GRID | LK_COCT | LK_DAG | LK_SCOM2)) == 0;
   auto note_leaf = [&]() {
     if (leftNodeOffset != 0xFFFFFFFFu) postponed = leftNodeOffset;
     top--;

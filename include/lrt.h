@@ -109,6 +109,17 @@ lrt_status lrt_add_sdf_dag(lrt_scene* s, const void* header, const void* nodes, 
                            const void* child_edges, size_t n_child_edges, const void* data_edges, size_t n_data_edges,
                            const float* distances, size_t n_distances, uint32_t* out_geom_id);
 
+/* Replaces BVHRT::AddGeom_SCom2 (BVH2Common_host.cpp:1133-1191). header: SCom2Header (216 B,
+ * sdf/scom2/scom_shared.h:21-75); nodes, bricks: the tree's u32 words (the header's
+ * nodes_arr_offset and bricks_arr_offset are replaced by where the words land, as the reference
+ * does). Checked when added: every word a ray can make the walk read lies inside the tree,
+ * rotations are below 48 and the tree is at most 15 levels deep; a failed check returns
+ * LRT_ERR_INVALID_ARG with the reason. Trace calls return LRT_ERR_UNSUPPORTED where the
+ * reference would take its fast traversal (one plain SCom2 tree, LAMBERT_NO_TEX) and for the
+ * channel render modes of trees with channels: neither is built. Hits report TYPE_SCOM2. */
+lrt_status lrt_add_scom2(lrt_scene* s, const void* header, const uint32_t* nodes, size_t n_nodes,
+                         const uint32_t* bricks, size_t n_bricks, uint32_t* out_geom_id);
+
 /* Replaces BVHRT::AddGeom_SdfGrid (BVH2Common_host.cpp:466-499): a dense grid of distances over
  * the unit cube, size[0] * size[1] * size[2] floats with x fastest (fewer than 2^28). */
 lrt_status lrt_add_sdf_grid(lrt_scene* s, const uint32_t size[3], const float* data, uint32_t* out_geom_id);

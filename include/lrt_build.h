@@ -108,6 +108,14 @@ lrt_status lrt_save_sdf_dag(const char* path, const void* header, const void* no
                             const void* child_edges, size_t n_child_edges, const void* data_edges, size_t n_data_edges,
                             const float* distances, size_t n_distances);
 
+/* load_scom2 / save_scom2 (sdf/scom2/scom_utils.cpp:98-195): u32 magic 0xffffdefa, version 4,
+ * counts of node words, brick words, voxel and point channels, the 216-B SCom2Header, then the
+ * node and brick words. Legacy files without the magic number, other versions and files with
+ * data channels are refused; the saver writes version 4 without channels. */
+lrt_status lrt_load_scom2(const char* path, void* header, lrt_buffer* nodes, lrt_buffer* bricks);
+lrt_status lrt_save_scom2(const char* path, const void* header, const uint32_t* nodes, size_t n_nodes,
+                          const uint32_t* bricks, size_t n_bricks);
+
 /* load_sdf_SVS / save_sdf_SVS (sdf/simple/sdf_simple.cpp:69-94): u32 count, then count SdfSVSNode. */
 lrt_status lrt_load_sdf_svs(const char* path, lrt_buffer* nodes);
 lrt_status lrt_save_sdf_svs(const char* path, const void* nodes, size_t n_nodes);

@@ -61,7 +61,7 @@ _ARRAYS = ["nodes", "tlas", "instances", "geoms", "geom_tags", "start_end", "pri
            "sbs_data", "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "svs_nodes", "svs_roots",
            "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points", "coct_data", "coct_geoms",
            "coct_mods", "dag_nodes", "dag_child_edges", "dag_data_edges", "dag_distances", "dag_geoms", "dag_mods",
-           "rot_add"]
+           "rot_add", "scom2_data", "scom2_headers", "scom2_mods"]
 
 
 class OracleScene(ctypes.Structure):

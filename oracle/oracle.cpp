@@ -113,6 +113,23 @@ struct DagChildEdge { uint32_t child_offset, rotation_id; };
 struct DagDataEdge { uint32_t data_offset, rotation_id, type_id; float add; };
 struct DagNode { uint32_t children_edges_offset, data_edges_offset; DagChildEdge channels_edge; uint32_t voxel_count_flags, pad; };
 struct DagG { uint32_t root, brick_size, brick_pad, node_grid_size, mods, pad[3]; };
+// SCom2Header (sdf/scom2/scom_shared.h:21-75)
+struct SC2Header {
+  uint32_t brick_size, v_size, bits_per_value, values_per_uint, value_mask, bitmask_len, dimension;
+  uint32_t child_rot_shift, child_rot_mask, child_add_shift, child_add_mask, child_offset_mask, child_offset_off;
+  uint32_t node_offset_mask, uints_per_link, unique_brick_prefix, unique_brick_offset_mask;
+  uint32_t children_types_shift, children_types_mask, base_reference_shift, children_active_bits_shift;
+  uint32_t children_active_bits_mask, references_offset, reference_bits, reference_mask, references_per_uint;
+  uint32_t links_offset, max_surface_count, max_surface_count_per_leaf;
+  uint32_t bricks_step, bricks_arr_offset, nodes_arr_offset, root_node_off;
+  uint32_t has_channels, has_surfaces, has_multi_nodes;
+  int32_t tex_id_off, mat_id_off, all_float_tex_id_off, all_int_mat_id_off;
+  float max_val;
+  uint32_t max_depth;
+  float user_params[7];
+  uint32_t pad[5];
+};
+static_assert(sizeof(SC2Header) == 216, "layout");
 struct Preset {
   uint32_t render_mode, sdf_node_intersect, normal_mode, ray_gen_mode, interpolation_mode, representation_mode, spp,
       fixed_lod;
@@ -126,7 +143,7 @@ constexpr uint32_t LEAF_BIT = 0x80000000u, START_MASK = 0x0FFFFFFFu, SIZE_MASK =
 constexpr int STACK_SIZE = 80;
 constexpr uint32_t TAG_NONE = 0, TAG_TRIANGLE = 1, TAG_SDF_BRICK = 4, TAG_SDF_NODE = 8, TAG_SDF_OCTREE = 15;
 constexpr uint32_t TAG_SDF_GRID = 11, TYPE_SDF_GRID = 1, TAG_GRAPHICS_PRIM = 9, TYPE_GRAPHICS_PRIM = 17;
-constexpr uint32_t TAG_COCTREE_BRICKED = 2, TAG_SDF_DAG = 18, TYPE_SDF_DAG = 25, TYPE_SCOM2 = 26;
+constexpr uint32_t TAG_COCTREE_BRICKED = 2, TAG_SDF_DAG = 18, TYPE_SDF_DAG = 25, TYPE_SCOM2 = 26, TAG_SCOM2 = 19;
 // GraphicsPrimType (utils/misc/graphics_primitive_common.h:7-16)
 constexpr uint32_t GP_POINT = 0, GP_LINE = 1, GP_SEGMENT = 2, GP_SEGMENT_DIR = 3, GP_BOX = 4, GP_POINT_COLOR = 5,
                    GP_LINE_COLOR = 6, GP_SEGMENT_COLOR = 7, GP_SEGMENT_DIR_COLOR = 8, GP_BOX_COLOR = 9;
@@ -178,6 +195,9 @@ struct OracleScene {
   const DagG* dag_geoms;
   const I4* dag_mods;             // 2 x 48 index modifiers per DAG (brick lattice, node grid)
   const uint8_t* rot_add;         // m_RotAddTable, 48 x 48
+  const uint32_t* scom2_data;     // m_SCom2Data
+  const SC2Header* scom2_headers; // m_SCom2Headers (offsets into scom2_data)
+  const I4* scom2_mods;           // 3 x 48 index modifiers per tree
   Preset preset;
 };
 
@@ -1441,6 +1461,207 @@ struct Tracer {
     }
   }
 
+  // ---- SCom2 (BVH2Common.cpp:2560-2818, sdf/scom2/scom_shared.h:183-258) ----
+  struct NodeHead { uint32_t base_type, children_types, base_links_end, children_active; };
+  static NodeHead unpack_node_head(const SC2Header& h, uint32_t node0, uint32_t node1) {
+    NodeHead r;
+    r.base_type = node0 & 0xFFu;
+    if (h.has_multi_nodes != 0) r.children_types = (node0 >> h.children_types_shift) & h.children_types_mask;
+    else r.children_types = (node1 >> h.children_types_shift) & h.children_types_mask;
+    if (h.has_multi_nodes != 0) r.base_links_end = (node0 >> h.base_reference_shift) & h.reference_mask;
+    else r.base_links_end = (r.base_type == 0 ? 0 : (h.has_surfaces == 0 ? h.has_channels : 1 + h.has_channels));
+    r.children_active = (node0 >> h.children_active_bits_shift) & h.children_active_bits_mask;
+    return r;
+  }
+  struct DataEdge { uint32_t data_offset, rotation_id; float add; };
+  static DataEdge unpack_data_edge(const SC2Header& h, float max_val, uint32_t edge0, uint32_t edge1) {
+    DataEdge e;
+    const bool is_unique = (edge0 & h.unique_brick_prefix) == h.unique_brick_prefix;
+    const bool no_add = is_unique || (h.child_add_mask == 0);
+    if (h.max_val > 0)
+      e.add = no_add ? 0.0f : h.max_val * (2 * ((edge0 >> h.child_add_shift) & h.child_add_mask) / float(h.child_add_mask) - 1);
+    else
+      e.add = no_add ? 0.0f : max_val * (2 * ((edge0 >> h.child_add_shift) & h.child_add_mask) / float(h.child_add_mask) - 1);
+    e.rotation_id = is_unique ? 0 : (edge0 >> h.child_rot_shift) & h.child_rot_mask;
+    e.data_offset = is_unique ? edge1 & h.unique_brick_offset_mask : edge1 & h.child_offset_mask;
+    return e;
+  }
+  struct ChildEdge { uint32_t child_offset, rotation_id; };
+  static ChildEdge unpack_child_edge(const SC2Header& h, uint32_t edge0, uint32_t edge1) {
+    return ChildEdge{edge1 & h.node_offset_mask, (edge0 >> h.child_rot_shift) & h.child_rot_mask};
+  }
+  static uint32_t get_links_end(const SC2Header& h, uint32_t child_n, uint32_t node1, uint32_t node2) {
+    if (h.has_multi_nodes != 0) {
+      const uint32_t sh = child_n * h.reference_bits;
+      return sh >= 32 ? ((node2 >> sh % 32) & h.reference_mask) : ((node1 >> sh) & h.reference_mask);
+    } else if (h.has_surfaces > 0) {
+      const uint32_t base_links_end = ((node1 & 0xFFu) == 0 ? 0 : 1);
+      const uint32_t ch_active = (node1 >> h.children_active_bits_shift) & h.children_active_bits_mask;
+      const uint32_t ch_n = uint32_t(__builtin_popcount(ch_active & ((1u << (1 + child_n)) - 1)));
+      return (1 + h.has_channels) * (base_links_end) + ch_n;
+    } else {
+      const uint32_t base_links_end = ((node1 & 0xFFu) == 0 ? 0 : 1);
+      const uint32_t ch_active = (node1 >> h.children_active_bits_shift) & h.children_active_bits_mask;
+      const uint32_t ch_n = uint32_t(__builtin_popcount(ch_active & ((1u << (1 + child_n)) - 1)));
+      return (h.has_channels) * (base_links_end + ch_n);
+    }
+  }
+
+  // SCom2Intersect. Pushes past the 32-entry stack are dropped; the brick march stops after
+  // 4 bs + 8 voxels (only marches whose 1e-5 step is lost to rounding get there).
+  void SCom2Intersect(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId, Hit* pHit) {
+    struct OTStackElement { uint32_t nodeId, info, p_size_x, p_size_y; };
+    const uint32_t type = TYPE_SCOM2;
+    const uint32_t hidx = s.geoms[geomId].offset[1];
+    const SC2Header header = s.scom2_headers[hidx];
+    const uint32_t* data = s.scom2_data;
+    const I4* mods = s.scom2_mods + hidx * 144;
+    f3 pos_ray_pos = ray_pos, pos_ray_dir = ray_dir;
+    uint32_t a = 0;
+    if (ray_dir.x < 0) { pos_ray_pos.x *= -1; pos_ray_dir.x *= -1; a |= 4; }
+    if (ray_dir.y < 0) { pos_ray_pos.y *= -1; pos_ray_dir.y *= -1; a |= 2; }
+    if (ray_dir.z < 0) { pos_ray_pos.z *= -1; pos_ray_dir.z *= -1; a |= 1; }
+    pos_ray_pos = pos_ray_pos * -1.0f;
+    const f3 pos_ray_dir_inv = SafeInverse(pos_ray_dir);
+    const f3 _t0 = pos_ray_pos * pos_ray_dir_inv - pos_ray_dir_inv;
+    const f3 _t1 = pos_ray_pos * pos_ray_dir_inv + pos_ray_dir_inv;
+    const f3 _l = _t1 - _t0;
+    static const int nn[8][3] = {{4, 2, 1}, {5, 3, 8}, {6, 8, 3}, {7, 8, 8}, {8, 6, 5}, {8, 7, 8}, {8, 8, 7}, {8, 8, 8}};
+    OTStackElement stack[32];
+    int top = 0;
+    stack[0] = OTStackElement{header.nodes_arr_offset + header.root_node_off, 0, 0, 1};
+    while (top >= 0) {
+      OTStackElement curElem = stack[top];
+      top--;
+      const uint32_t level_sz = curElem.p_size_y & 0xFFFF;
+      const float level_sz_f = float(level_sz);
+      const uint32_t px = curElem.p_size_x >> 16, py = curElem.p_size_x & 0xFFFF, pz = curElem.p_size_y >> 16;
+      float d = 1.0f / float(level_sz);
+      const f3 p_f = f3(float(px), float(py), float(pz));
+      const f3 t0 = _t0 + d * p_f * _l;
+      const f3 t1 = _t0 + d * (p_f + 1.0f) * _l;
+      if (std::min(t1.x, std::min(t1.y, t1.z)) < tNear) continue;
+      bool is_leaf = curElem.info > 0xFF;
+      bool is_surface = curElem.info > 0xFFFFFFu;
+      uint32_t surface_count = (curElem.info >> 8) & 0xFFFF;
+      if (!is_leaf) {
+        const float t = s.preset.fixed_lod > 0 ? 1.0f : std::max(t0.x, std::max(t0.y, t0.z));
+        const float target_lod_size = (std::pow(2.0f, s.preset.level_of_detail) - 0.01f) / t;
+        is_leaf = float(level_sz) > target_lod_size;
+        if (is_leaf) {
+          const NodeHead node = unpack_node_head(header, data[curElem.nodeId], data[curElem.nodeId + header.links_offset - 1]);
+          if (node.base_type == 0) continue;
+          is_surface = (node.base_type & 3) == 3;
+          curElem.nodeId = curElem.nodeId + header.links_offset;
+          surface_count = node.base_links_end - header.has_channels;
+        }
+      }
+      if (is_leaf) {
+        const float sz_inv = 2.0f / level_sz_f;
+        const uint32_t p_mask = level_sz - 1;
+        const uint32_t rp[3] = {(a & 4) ? (~px & p_mask) : px, (a & 2) ? (~py & p_mask) : py, (a & 1) ? (~pz & p_mask) : pz};
+        const f3 brick_min_pos = f3(-1, -1, -1) + sz_inv * f3(float(rp[0]), float(rp[1]), float(rp[2]));
+        float tmin = std::max(t0.x, std::max(t0.y, t0.z));
+        const float tmax = std::min(t1.x, std::min(t1.y, t1.z));
+        const uint32_t rotId = curElem.info & 0xFF;
+        const float max_val = 1.7320508f * 2.0f / level_sz_f;   // get_max_sdf_val
+        d = 2.0f / (level_sz_f * header.brick_size);
+        const float old_t = pHit->t;
+        uint32_t steps = 0;
+        while (tmin < tmax && pHit->t == old_t && steps < 4 * header.brick_size + 8) {
+          steps++;
+          const f3 hit_pos = ray_pos + tmin * ray_dir;
+          const f3 local_pos = (hit_pos - brick_min_pos) * (0.5f * level_sz_f * header.brick_size);
+          const float hi = header.brick_size - 1e-6f;
+          const f3 voxelPos(std::floor(std::min(std::max(1e-6f, local_pos.x), hi)),
+                            std::floor(std::min(std::max(1e-6f, local_pos.y), hi)),
+                            std::floor(std::min(std::max(1e-6f, local_pos.z), hi)));
+          const f3 min_pos = brick_min_pos + d * voxelPos;
+          const f3 max_pos = min_pos + d * f3(1, 1, 1);
+          uint32_t s_id = 0;
+          const f2 fNearFar = RayBoxIntersection2(ray_pos, SafeInverse(ray_dir), min_pos, max_pos);
+          while (fNearFar.x < fNearFar.y && s_id < surface_count && pHit->t == old_t) {
+            const uint32_t link_0 = curElem.nodeId + s_id * header.uints_per_link;
+            const uint32_t link_1 = link_0 + header.child_offset_off;
+            const DataEdge de = unpack_data_edge(header, max_val, data[link_0], data[link_1]);
+            const uint32_t offset = header.bricks_arr_offset + header.bricks_step * de.data_offset;
+            const uint32_t rotIdx = s.rot_add[rotId * 48 + de.rotation_id];
+            float vmin = 1.0f;
+            float vmax = is_surface ? -1.0f : 1.0f;
+            float values[8];
+            for (int i = 0; i < 8; i++) {
+              const int pI[3] = {int(voxelPos.x) + ((i & 4) >> 2), int(voxelPos.y) + ((i & 2) >> 1), int(voxelPos.z) + (i & 1)};
+              const I4& m = mods[rotIdx];
+              const uint32_t vId0 = uint32_t(m.x * pI[0] + m.y * pI[1] + m.z * pI[2] + m.w * 1);
+              const uint32_t p_val = data[offset + vId0 / header.values_per_uint];
+              const uint32_t p_off = (vId0 % header.values_per_uint) * header.bits_per_value;
+              const float val = max_val * (2.0f * ((p_val >> p_off) & header.value_mask) / float(header.value_mask) - 1) + de.add;
+              values[i] = val - s.preset.compact_sdf_eps * d;
+              vmin = std::min(vmin, val);
+              vmax = std::max(vmax, val);
+            }
+            if (vmin <= 0.0f && vmax >= 0.0f) {
+              const f3 start_pos = ray_pos + fNearFar.x * ray_dir;
+              const float vox_sz = (0.5f * level_sz_f * header.brick_size);
+              const f3 start_q = vox_sz * (start_pos - min_pos);
+              const float qNear = std::max(0.0f, vox_sz * (tNear - fNearFar.x));
+              const float qFar = vox_sz * (fNearFar.y - fNearFar.x);
+              LocalSurfaceIntersection(type, ray_dir, instId, geomId, values, curElem.nodeId, d, qNear, qFar, fNearFar,
+                                       start_q, pHit, is_surface);
+            }
+            if (pHit->t != old_t && header.has_channels > 0 && s.preset.render_mode != 5) {
+              const uint32_t ch_off = curElem.nodeId + surface_count * header.uints_per_link;
+              pHit->primId = unpack_child_edge(header, data[ch_off], data[ch_off + header.child_offset_off]).child_offset;
+            }
+            s_id++;
+          }
+          tmin += std::max(0.0f, fNearFar.y - tmin) + 1e-5f;
+          if (pHit->t < old_t) {
+            top = -1;
+            if (s.preset.render_mode == 20) pHit->primId = uint32_t(std::log2(float(level_sz * header.brick_size)));
+          }
+        }
+      } else {
+        const NodeHead node = unpack_node_head(header, data[curElem.nodeId], data[curElem.nodeId + header.links_offset - 1]);
+        const uint32_t node1 = data[curElem.nodeId + header.references_offset];
+        const uint32_t node2 = data[curElem.nodeId + header.links_offset - 1];
+        OTStackElement tmp_buf[4];
+        int buf_top = 0;
+        const f3 tm = 0.5f * (t0 + t1);
+        int currNode = FirstNode(t0, tm);
+        do {
+          const uint32_t rawChildNode = uint32_t(currNode) ^ a;
+          const int vp[3] = {int((rawChildNode & 4) >> 2), int((rawChildNode & 2) >> 1), int(rawChildNode & 1)};
+          const I4& m = mods[2 * 48 + curElem.info];
+          const uint32_t voxId = uint32_t(m.x * vp[0] + m.y * vp[1] + m.z * vp[2] + m.w * 1);
+          const uint32_t ch_type = (node.children_types >> 2 * voxId) & 3;
+          const uint32_t ch_start_link = voxId == 0 ? node.base_links_end : get_links_end(header, voxId - 1, node1, node2);
+          const uint32_t ch_end_link = get_links_end(header, voxId, node1, node2);
+          if (ch_type != 0 && buf_top < 4) {
+            tmp_buf[buf_top].p_size_x = (curElem.p_size_x << 1) | (((uint32_t(currNode) & 4) << (16 - 2)) | ((uint32_t(currNode) & 2) >> 1));
+            tmp_buf[buf_top].p_size_y = (curElem.p_size_y << 1) | ((uint32_t(currNode) & 1) << 16);
+            const uint32_t link_off = curElem.nodeId + header.links_offset + ch_start_link * header.uints_per_link;
+            if (ch_type == 2) {
+              const ChildEdge ce = unpack_child_edge(header, data[link_off], data[link_off + header.child_offset_off]);
+              tmp_buf[buf_top].nodeId = header.nodes_arr_offset + ce.child_offset;
+              tmp_buf[buf_top].info = s.rot_add[curElem.info * 48 + ce.rotation_id];
+            } else {
+              const uint32_t surf_count = ch_end_link - ch_start_link - header.has_channels;
+              tmp_buf[buf_top].nodeId = link_off;
+              tmp_buf[buf_top].info = (curElem.info & 0xFF) | (surf_count << 8) | (ch_type == 3 ? (1u << 24) : 0);
+            }
+            buf_top++;
+          }
+          currNode = NewNode((currNode & 4) ? t1.x : tm.x, nn[currNode][0], (currNode & 2) ? t1.y : tm.y, nn[currNode][1],
+                             (currNode & 1) ? t1.z : tm.z, nn[currNode][2]);
+        } while (currNode < 8);
+        for (int i = 1; i <= buf_top; i++)
+          if (top + i < 32) stack[top + i] = tmp_buf[buf_top - i];
+        top = std::min(top + buf_top, 31);
+      }
+    }
+  }
+
   // BVH2Common.cpp:47-124
   uint32_t IntersectAllPrimitivesInLeaf(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
                                         uint32_t aabbId, Hit* pHit) {
@@ -1459,6 +1680,7 @@ struct Tracer {
       case TAG_GRAPHICS_PRIM: IntersectGraphicPrims(ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
       case TAG_COCTREE_BRICKED: COctreeIntersectV3(ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
       case TAG_SDF_DAG: DAGIntersect(ray_pos, ray_dir, tNear, instId, geomId, pHit); break;
+      case TAG_SCOM2: SCom2Intersect(ray_pos, ray_dir, tNear, instId, geomId, pHit); break;
       default: break;
     }
     if (s.preset.render_mode == 26) pHit->instId = (pHit->instId == 0xFFFFFFFF) ? 1 : pHit->instId + 1;

@@ -1,0 +1,111 @@
+"""SCom2 leaves on the CPU: the oracle's restatement of SCom2Intersect
+(MultiRenderer/BVH2Common.cpp:2560-2818, with the unpacking of sdf/scom2/scom_shared.h:183-258)
+pinned by the reference's own engine-scene SCom2 file (scenes/06_engine_scenes/scom_tree_2/
+scom2_0.bin, copied under tests/golden/): it holds the object of the engine scene's DAG file
+(same depth, one with bricks shared through rotations and shifts), so both trace alike, and a
+planted rotation error breaks that. Host checks of AddGeom_SCom2 (BVH2Common_host.cpp:1133-1191)
+and the file format (sdf/scom2/scom_utils.cpp:98-195)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import MISS, host_scene
+from test_coctree_cpu import oracle_trace, sphere_rays
+from test_dag_cpu import dag_scene, no_eps
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SCOM2 = os.path.join(GOLD, "engine_scom2_0.bin")
+
+
+def scom2_scene(lrt, arrays, preset=None):
+    def build(s):
+        s.AddInstance(s.AddGeom_SCom2(*arrays))
+        if preset is not None:
+            s.SetPreset(preset)
+    return host_scene(lrt, build)
+
+
+def test_reference_file_layout(lrt, tmp_path):
+    h, n, b = lrt.load_scom2(SCOM2)
+    assert (int(h["brick_size"]), int(h["v_size"]), int(h["bits_per_value"]), int(h["has_multi_nodes"]),
+            int(h["has_surfaces"]), int(h["has_channels"])) == (1, 2, 8, 1, 1, 0)
+    assert n.size == 32510   # scene.xml: num_primitives
+    out = tmp_path / "s.bin"
+    lrt.save_scom2(str(out), h, n, b)
+    assert open(out, "rb").read() == open(SCOM2, "rb").read()
+    for bad in (b"\x00" * 8, open(SCOM2, "rb").read()[:-4]):
+        p = tmp_path / "bad.bin"
+        p.write_bytes(bad)
+        with pytest.raises(lrt.LrtError):
+            lrt.load_scom2(str(p))
+
+
+def test_reference_file_matches_the_dag_file(lrt, orc):
+    """The SCom2 tree and the DAG of the engine scene (both 6 levels, 8-bit values against floats)
+    are hit by the same rays at t within the 8-bit quantisation; against the 5-level frame octree
+    the differences are those of the coarser voxels."""
+    pos, dir_ = sphere_rays(4000)
+    a = oracle_trace(orc, lrt, scom2_scene(lrt, lrt.load_scom2(SCOM2), no_eps(lrt)), pos, dir_)
+    b = oracle_trace(orc, lrt, dag_scene(lrt, lrt.load_sdf_dag(os.path.join(GOLD, "engine_sdf_dag_0.bin")),
+                                         no_eps(lrt)), pos, dir_)
+    ha, hb = a["primId"] != MISS, b["primId"] != MISS
+    assert ha.mean() > 0.5 and (ha != hb).mean() < 2e-3
+    dt = np.abs(a["t"][ha & hb] - b["t"][ha & hb])
+    assert np.median(dt) < 1e-3 and np.percentile(dt, 99) < 5e-3
+    assert np.all((a["geomId"][ha] >> 24) == 26)   # TYPE_SCOM2
+
+
+def test_planted_rotation_error_is_caught(lrt, orc):
+    """The tree shares bricks through rotations: decoding every brick through the next rotation
+    of the table instead moves the hits away from the DAG's."""
+    h, n, b = lrt.load_scom2(SCOM2)
+    sc = scom2_scene(lrt, (h, n, b), no_eps(lrt))
+    o = orc.Scene.from_scene(sc)
+    mods = sc.get_array("scom2_mods").reshape(3, 48, 4).copy()
+    mods[0] = np.roll(mods[0], 1, axis=0)
+    bad = np.ascontiguousarray(mods.ravel())
+    o._keep["scom2_mods"] = bad
+    o.view.scom2_mods = bad.ctypes.data
+    pos, dir_ = sphere_rays(4000)
+    w = orc.trace_closest(o, pos, dir_)[0]
+    good = oracle_trace(orc, lrt, sc, pos, dir_)
+    hw, hg = w["primId"] != MISS, good["primId"] != MISS
+    differ = (hw != hg).mean() + np.mean(np.abs(w["t"][hw & hg] - good["t"][hw & hg]) > 1e-3)
+    assert differ > 0.02
+
+
+def test_host_checks_reject_malformed_trees(lrt):
+    h, n, b = lrt.load_scom2(SCOM2)
+    scom2_scene(lrt, (h, n, b))
+
+    def rejects(arrays):
+        with pytest.raises(lrt.LrtError):
+            scom2_scene(lrt, arrays)
+
+    rejects((h, n, b[: b.size // 2]))                       # bricks cut short
+    h2 = h.copy()
+    h2["root_node_off"] = n.size + b.size + 10              # root beyond the words
+    rejects((h2, n, b))
+    h3 = h.copy()
+    h3["values_per_uint"], h3["bits_per_value"] = 8, 8      # 64 bits per word
+    rejects((h3, n, b))
+    h4 = h.copy()
+    h4["dimension"] = 2
+    rejects((h4, n, b))
+    # a child edge pointing back at the root: walk the root's first inner child link
+    root = int(h["root_node_off"])
+    n5 = n.copy()
+    lo, upl, coo = int(h["links_offset"]), int(h["uints_per_link"]), int(h["child_offset_off"])
+    node0 = int(n[root])
+    types = (node0 >> int(h["children_types_shift"])) & int(h["children_types_mask"])
+    base = (node0 >> int(h["base_reference_shift"])) & int(h["reference_mask"])
+    for c in range(8):
+        if ((types >> (2 * c)) & 3) == 2:
+            ref = int(n[root + int(h["references_offset"])])
+            nodeL = int(n[root + lo - 1])
+            sh = (c - 1) * int(h["reference_bits"])
+            start = base if c == 0 else (((nodeL >> (sh % 32)) if sh >= 32 else (ref >> sh)) & int(h["reference_mask"]))
+            n5[root + lo + start * upl + coo] = (n5[root + lo + start * upl + coo] & ~np.uint32(h["node_offset_mask"])) | root
+            rejects((h, n5, b))
+            break
