@@ -93,20 +93,14 @@ lrt::TraceArgs trace_args(lrt_scene* s, const float* pos, const float* dir, lrt_
   return a;
 }
 
-// SCom2 queries this library does not build: the reference's fast traversal, which it takes for a
-// scene holding one plain SCom2 tree rendered LAMBERT_NO_TEX (BVH2Common_host.cpp:1436-1451,
-// SCom2IntersectFast), and the channel render modes of trees with channels (their quotients are
-// rotated by float matrices). Returns the reason, or null.
+// SCom2 queries this library does not build: the channel render modes of trees with channels
+// (their hit-point quotients are rotated by float matrices, BVH2Common.cpp:2697-2711). Returns the
+// reason, or null.
 const char* unsupported_scom2(const lrt::Scene& sc) {
   const uint32_t rm = sc.preset.render_mode;
   for (const lrt::SCom2Header& h : sc.scom2Headers)
     if (h.has_channels && (rm == lrt::MULTI_RENDER_MODE_CHANNEL || rm == lrt::MULTI_RENDER_MODE_CHANNEL_LAMBERT))
       return "channel render modes of SCom2 trees with channels are not built";
-  if (sc.scom2Headers.size() == 1 && rm == lrt::MULTI_RENDER_MODE_LAMBERT_NO_TEX) {
-    const lrt::SCom2Header& h = sc.scom2Headers[0];
-    if (h.has_multi_nodes == 0 && h.has_channels == 0 && h.v_size == 3 && h.links_offset == 1 && h.child_offset_off == 0)
-      return "the fast traversal of a single plain SCom2 tree (LAMBERT_NO_TEX) is not built";
-  }
   return nullptr;
 }
 

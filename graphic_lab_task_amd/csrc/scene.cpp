@@ -924,6 +924,9 @@ SceneView Scene::make_view() const {
   v.scom2_data = (const uint32_t*)P("scom2_data");
   v.scom2_headers = (const SCom2Header*)P("scom2_headers");
   v.scom2_mods = (const int4*)P("scom2_mods");
+  // BVHRT::SetPreset (BVH2Common_host.cpp:1436-1451): one plain SCom2 tree rendered LAMBERT_NO_TEX
+  v.scom2_fast = (scom2Headers.size() == 1 && preset.render_mode == MULTI_RENDER_MODE_LAMBERT_NO_TEX &&
+                  scom2_plain_tree(scom2Headers[0])) ? 1u : 0u;
   v.preset = preset;
   v.lod_thr = lod_thr;
   v.leaf_kinds = leaf_kinds();

@@ -16,6 +16,9 @@ namespace lrt {
 
 // m_RotAddTable: 48 x 48 compositions of the lattice rotations (entry i * 48 + j: i, then j)
 const std::vector<uint8_t>& rot_add_table();
+// An SCom2 tree of the shape the reference's fast traversal takes (no multi-nodes or channels,
+// 3^3 values, one-word links)
+bool scom2_plain_tree(const SCom2Header& h);
 
 class DevBuf {
  public:

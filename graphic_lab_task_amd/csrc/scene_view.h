@@ -97,6 +97,7 @@ struct SceneView {
   const uint32_t*           scom2_data;    // m_SCom2Data: each tree's nodes then bricks
   const SCom2Header*        scom2_headers; // m_SCom2Headers (GeomData.offset.y), offsets made global
   const int4*               scom2_mods;    // 3 x 48 index modifiers per tree
+  uint32_t                  scom2_fast;    // 1: SCom2 leaves take the fast traversal (support_fast_scom_traversal)
 
   MultiRenderPreset preset;
   float    lod_thr;                 // m_precomputedLoDThr

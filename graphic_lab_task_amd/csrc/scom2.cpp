@@ -101,7 +101,62 @@ struct SCom2Checker {
   }
 };
 
+// Follows SCom2IntersectFast (BVH2Common.cpp:2988-3231) for a tree it may be used on (no
+// multi-nodes, no channels, 3^3 values, one-word links): a node's head word, one link per child
+// that has data (its rank among them), the child node's head or the leaf's brick.
+struct SCom2FastChecker {
+  const SCom2Header& h;
+  const uint32_t* D;
+  uint64_t total, bricks_at;
+  uint32_t max_depth = 15;
+  std::unordered_map<uint32_t, uint32_t> height;
+
+  [[noreturn]] void bad(const std::string& what) const { throw std::invalid_argument("SCom2: " + what); }
+  uint32_t word(uint64_t i) const {
+    if (i >= total) bad("word index out of range");
+    return D[i];
+  }
+  uint32_t check_node(uint64_t n, uint32_t depth) {
+    if (depth > max_depth) bad("deeper than 15 levels (or a cycle)");
+    auto it = height.find(uint32_t(n));
+    if (it != height.end()) {
+      if (depth + it->second > max_depth) bad("deeper than 15 levels");
+      return it->second;
+    }
+    const uint32_t node0 = word(n);
+    const SCom2NodeHead head = scom2_node_head(h, node0, node0);
+    const uint64_t links = n + h.links_offset + (head.base_type == SCOM2_CHILD_EMPTY ? 0 : 1);
+    uint32_t rank = 0, hgt = 0;
+    for (uint32_t c = 0; c < 8; ++c) {
+      const uint32_t type = (head.children_types >> (2 * c)) & 3u;
+      if (type == SCOM2_CHILD_EMPTY) continue;
+      const uint32_t link = word(links + rank++);
+      if (type & 1u) {
+        const SCom2DataEdge e = scom2_data_edge(h, 1.0f, link, link);
+        if (e.rotation_id >= COCTREE_ROT_COUNT) bad("brick rotation beyond the 48 rotations");
+        const uint64_t rel = uint64_t(h.bricks_step) * e.data_offset;
+        if (rel >= total || bricks_at + rel + (27 - 1) / h.values_per_uint >= total) bad("brick values out of range");
+        hgt = std::max(hgt, 1u);
+      } else {
+        const SCom2ChildEdge ce = scom2_child_edge(h, link, link);
+        if (ce.rotation_id >= COCTREE_ROT_COUNT) bad("child rotation beyond the 48 rotations");
+        hgt = std::max(hgt, 1 + check_node(ce.child_offset, depth + 1));
+      }
+    }
+    height[uint32_t(n)] = hgt;
+    return hgt;
+  }
+};
+
 }  // namespace
+
+// The tree shape for which the reference may take SCom2IntersectFast (its preset-dependent switch,
+// BVH2Common_host.cpp:1436-1451, also needs the tree to be the scene's only one and
+// LAMBERT_NO_TEX rendering).
+bool scom2_plain_tree(const SCom2Header& h) {
+  return h.has_multi_nodes == 0 && h.has_channels == 0 && h.v_size == 3 && h.links_offset == 1 &&
+         h.child_offset_off == 0;
+}
 
 // BVHRT::AddGeom_SCom2 (BVH2Common_host.cpp:1133-1191): nodes and bricks appended to the scene's
 // word array, the header's nodes_arr_offset / bricks_arr_offset set to where they landed (the
@@ -125,6 +180,11 @@ uint32_t Scene::add_scom2(const SCom2Header& header, const uint32_t* nodes, size
   const uint32_t v = h.brick_size + 1;
   SCom2Checker ck{lh, local.data(), local.size(), n_nodes, v * v * v};
   ck.check_node(uint64_t(h.root_node_off), 0);
+  if (scom2_plain_tree(h)) {
+    if (h.brick_size != 2) throw std::invalid_argument("SCom2: a plain tree with 3^3 values needs brick_size 2");
+    SCom2FastChecker fk{lh, local.data(), local.size(), n_nodes};
+    fk.check_node(uint64_t(h.root_node_off), 0);
+  }
 
   const uint32_t base = uint32_t(scom2Data.size());
   if (uint64_t(base) + local.size() >= (1ull << 32)) throw std::invalid_argument("SCom2: scene words exceed 2^32");

@@ -126,7 +126,8 @@ __device__ __forceinline__ void coop_leaf(const SceneView& s, float3 ray_pos, fl
   } else if ((KINDS & LK_DAG) && tag == TAG_SDF_DAG) {
     dag_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, h);
   } else if ((KINDS & LK_SCOM2) && tag == TAG_SCOM2) {
-    scom2_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, h);
+    if (s.scom2_fast) scom2_intersect_fast(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, h);
+    else scom2_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, h);
   }
   if (s.preset.render_mode == MULTI_RENDER_MODE_INTERSECTION_COUNT)
     h.instId = (h.instId == 0xFFFFFFFFu) ? 1 : h.instId + 1;

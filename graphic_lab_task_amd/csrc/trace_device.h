@@ -199,17 +199,19 @@ __device__ __forceinline__ float3 eval_grad_trilinear(const float v[8], float3 q
   return make3(gx, gy, gz);
 }
 
-// BVHRT::LocalSurfaceIntersection (BVH2Common.cpp:345-725) for the non-compact SDF types
-// used here (frame octree, SBS, multi octree). `is_surface_node` is always false for them.
-// Returns true when the hit record was updated; then *t_local (if given) receives the hit
-// parameter in voxel units along ray_dir from start_q. `surface_node` (multi-octree nodes
-// flagged as surface) makes the start sign follow the start distance, as SURFACE mode does.
+// BVHRT::LocalSurfaceIntersection (BVH2Common.cpp:345-725). Returns true when the hit record was
+// updated; then *t_local (if given) receives the hit parameter in voxel units along ray_dir from
+// start_q. `surface_node` (multi-octree, DAG and SCom2 nodes flagged as surface) makes the start
+// sign follow the start distance, as SURFACE mode does. OCTREE_TYPES compiles in what only the
+// multi-octree, DAG and SCom2 types use: the compact types' wider stopping distance on surface
+// nodes and the channel render modes' hit-point quotients (the other callers stay free of it).
+template <bool OCTREE_TYPES = false>
 __device__ bool local_surface_intersection(const SceneView& s, uint32_t type, float3 ray_dir, uint32_t instId,
                                            uint32_t geomId, const float values[8], uint32_t primId, float d,
                                            float qNear, float qFar, float2 fNearFar, float3 start_q,
                                            CRT_Hit& hit, float* t_local = nullptr, bool surface_node = false) {
   // surface nodes of the compact formats stop at a distance scaled with the voxel (BVH2Common.cpp:340-351)
-  const float EPS = (surface_node && (type == TYPE_SDF_DAG || type == TYPE_SCOM2))
+  const float EPS = (OCTREE_TYPES && surface_node && (type == TYPE_SDF_DAG || type == TYPE_SCOM2))
                         ? fmax_ref(1e-6f, s.preset.compact_sdf_eps * d) : 1e-6f;
   const float d_inv = 1.0f / d;
   float t = qNear;
@@ -398,7 +400,7 @@ __device__ bool local_surface_intersection(const SceneView& s, uint32_t type, fl
     if (s.preset.render_mode == MULTI_RENDER_MODE_ST_ITERATIONS) hit.primId = iter;
     // the channel render modes keep the voxel-local hit point (16 bits per axis) instead of the
     // texture coordinates (BVH2Common.cpp:714-723)
-    if ((type == TYPE_SDF_MULTI_OCTREE || type == TYPE_SDF_DAG || type == TYPE_SCOM2) &&
+    if (OCTREE_TYPES && (type == TYPE_SDF_MULTI_OCTREE || type == TYPE_SDF_DAG || type == TYPE_SCOM2) &&
         (s.preset.render_mode == MULTI_RENDER_MODE_CHANNEL || s.preset.render_mode == MULTI_RENDER_MODE_CHANNEL_LAMBERT)) {
       const float3 dq = start_q + t * ray_dir;
       const int qx = int(clamp_ref(65535.0f * dq.x, 0.0f, 65535.0f));
@@ -1175,7 +1177,7 @@ __device__ uint32_t multi_octree_intersect(const SceneView& s, const GeomRef& g,
         float values[8];
 #pragma unroll
         for (int i = 0; i < 8; i++) values[i] = s.multi_values[info + 8 * v + i];
-        local_surface_intersection(s, TYPE_SDF_MULTI_OCTREE, ray_dir, instId, geomId, values, primId, d, qNear, qFar,
+        local_surface_intersection<true>(s, TYPE_SDF_MULTI_OCTREE, ray_dir, instId, geomId, values, primId, d, qNear, qFar,
                                    fNearFar, start_q, hit, nullptr, surface);
       }
       if (s.preset.render_mode == MULTI_RENDER_MODE_MO_VOXEL_COUNT) {
@@ -1618,7 +1620,7 @@ __device__ __forceinline__ void dag_intersect(const SceneView& s, const GeomRef&
             const float3 start_q = vox_sz * (start_pos - min_pos);
             const float qNear = fmax_ref(0.0f, vox_sz * (tNear - fNearFar.x));
             const float qFar = vox_sz * (fNearFar.y - fNearFar.x);
-            local_surface_intersection(s, type, ray_dir, instId, geomId, values, primId, d, qNear, qFar, fNearFar,
+            local_surface_intersection<true>(s, type, ray_dir, instId, geomId, values, primId, d, qNear, qFar, fNearFar,
                                        start_q, hit, nullptr, is_surface);
           }
         }
@@ -1785,7 +1787,7 @@ __device__ __forceinline__ void scom2_intersect(const SceneView& s, const GeomRe
             const float3 start_q = vox_sz * (start_pos - min_pos);
             const float qNear = fmax_ref(0.0f, vox_sz * (tNear - fNearFar.x));
             const float qFar = vox_sz * (fNearFar.y - fNearFar.x);
-            local_surface_intersection(s, type, ray_dir, instId, geomId, values, nodeId, d, qNear, qFar, fNearFar,
+            local_surface_intersection<true>(s, type, ray_dir, instId, geomId, values, nodeId, d, qNear, qFar, fNearFar,
                                        start_q, hit, nullptr, is_surface);
           }
           // with channels the hit names the leaf's channel link (MULTI_RENDER_MODE_PRIMITIVE excepted)
@@ -1845,6 +1847,266 @@ __device__ __forceinline__ void scom2_intersect(const SceneView& s, const GeomRe
       }
       top += buf_top;
       if (top >= SC_STACK) top = SC_STACK - 1;
+    }
+  }
+}
+
+}  // namespace lrt
+
+namespace lrt {
+
+// LocalSurfaceIntersectionFast (BVH2Common.cpp:2820-2986): the analytic root of the trilinear
+// cubic along the ray (or the box in BBOX mode); the normal from central differences, normalised
+// without the instance transform; no iteration count or channel quotients.
+__device__ __forceinline__ void local_surface_fast(const SceneView& s, uint32_t type, float3 ray_dir, uint32_t instId,
+                                                   uint32_t geomId, const float values[8], uint32_t primId, float d,
+                                                   float qNear, float qFar, float2 fNearFar, float3 start_q,
+                                                   CRT_Hit& hit) {
+  const float EPS = 1e-6f;
+  const float d_inv = 1.0f / d;
+  float t = qNear;
+  bool hitp = false;
+  const float start_dist = eval_dist_trilinear(values, start_q + t * ray_dir);
+  if (start_dist <= EPS || s.preset.sdf_node_intersect == SDF_OCTREE_NODE_INTERSECT_BBOX) {
+    hitp = true;
+  } else {
+    const float s000 = values[0] * d_inv, s001 = values[1] * d_inv, s010 = values[2] * d_inv,
+                s011 = values[3] * d_inv, s100 = values[4] * d_inv, s101 = values[5] * d_inv,
+                s110 = values[6] * d_inv, s111 = values[7] * d_inv;
+    const float tmp = s101 - s001;
+    const float k0 = s000, k1 = s100 - s000, k2 = s010 - s000, k3 = s110 - s010 - k1, k4 = k0 - s001;
+    const float k5 = k1 - tmp, k6 = k2 - (s011 - s001), k7 = k3 - (s111 - s011 - tmp);
+    const float3 o = start_q, d3 = ray_dir;
+    const float m0 = o.x * o.y, m1 = d3.x * d3.y, m2 = o.x * d3.y + o.y * d3.x;
+    const float m3 = k5 * o.z - k1, m4 = k6 * o.z - k2, m5 = k7 * o.z - k3;
+    const float c0 = (k4 * o.z - k0) + o.x * m3 + o.y * m4 + m0 * m5;
+    const float c1 = d3.x * m3 + d3.y * m4 + m2 * m5 + d3.z * (k4 + k5 * o.x + k6 * o.y + k7 * m0);
+    const float c2 = m1 * m5 + d3.z * (k5 * d3.x + k6 * d3.y + k7 * m2);
+    const float c3 = k7 * m1 * d3.z;
+    float x1 = 1000, x2 = 1000, x3 = 1000;
+    if (fabs_ref(c3) > 1e-5f) {
+      const float a = c2 / c3, b = c1 / c3, c = c0 / c3;
+      const float Q = (a * a - 3 * b) / 9;
+      const float R = (2 * a * a * a - 9 * a * b + 27 * c) / 54;
+      const float Q3 = Q * Q * Q;
+      if (R * R < Q3) {
+        const float PI = 3.14159265358979323846f;
+        const float theta = acosf(R / sqrtf(Q3));
+        x1 = -2 * sqrtf(Q) * cosf(theta / 3) - a / 3;
+        x2 = -2 * sqrtf(Q) * cosf((theta + 2 * PI) / 3) - a / 3;
+        x3 = -2 * sqrtf(Q) * cosf((theta - 2 * PI) / 3) - a / 3;
+      } else {
+        const float A = -sign_ref(R) * powf(fabs_ref(R) + sqrtf(R * R - Q3), 1.0f / 3.0f);
+        const float B = fabs_ref(A) > EPS ? Q / A : 0;
+        x1 = A + B - a / 3;
+      }
+    } else if (fabs_ref(c2) > 1e-7f) {
+      const float a = c2, b = c1, c = c0;
+      const float D = b * b - 4 * a * c;
+      if (D > 0) {
+        const float q = -0.5f * (b + sign_ref(b) * sqrtf(D));
+        x1 = q / a;
+        if (fabs_ref(q) > EPS) x2 = c / q;
+      }
+    } else if (fabs_ref(c1) > EPS) {
+      x1 = -c0 / c1;
+    }
+    x1 = x1 < 0 ? 1000 : x1;
+    x2 = x2 < 0 ? 1000 : x2;
+    x3 = x3 < 0 ? 1000 : x3;
+    t = fmin_ref(x1, fmin_ref(x2, x3));
+    hitp = (t >= 0 && t <= qFar);
+  }
+  const float tReal = fNearFar.x + d * t;
+  if (t <= qFar && hitp && tReal < hit.t) {
+    const float3 p0 = start_q + t * ray_dir;
+    const float h = 0.001f;
+    const float ddx = (eval_dist_trilinear(values, p0 + make3(h, 0, 0)) - eval_dist_trilinear(values, p0 + make3(-h, 0, 0))) / (2 * h);
+    const float ddy = (eval_dist_trilinear(values, p0 + make3(0, h, 0)) - eval_dist_trilinear(values, p0 + make3(0, -h, 0))) / (2 * h);
+    const float ddz = (eval_dist_trilinear(values, p0 + make3(0, 0, h)) - eval_dist_trilinear(values, p0 + make3(0, 0, -h))) / (2 * h);
+    const float2 en = encode_normal(normalize3(make3(ddx, ddy, ddz)));
+    hit.t = tReal;
+    hit.primId = primId;
+    hit.instId = instId;
+    hit.geomId = geomId | (type << SH_TYPE);
+    hit.coords[0] = 0;
+    hit.coords[1] = 0;
+    hit.coords[2] = en.x;
+    hit.coords[3] = en.y;
+  }
+}
+
+// new_node(i, txm, tym, tzm) (utils/raytracing/intersection_common.h:87-93): the next child of a
+// node after child i, 8 or more when the ray leaves the node
+__device__ __forceinline__ uint32_t ot_new_node_i(uint32_t i, float txm, float tym, float tzm) {
+  const uint32_t x = 4 + i, y = 2 + (i & 0x5) + ((i & 0x2) << 3), z = 1 + (i & 0x6) + ((i & 0x1) << 3);
+  return (txm < tym) ? (txm < tzm ? x : z) : (tym < tzm ? y : z);
+}
+
+// SCom2IntersectFast (BVH2Common.cpp:2988-3231), taken instead of scom2_intersect for a scene whose
+// only SCom2 tree is plain when it renders LAMBERT_NO_TEX: the walk keeps the current node's next
+// child in its entry and steps one child at a time, pushing a node only while it has children
+// left (16 entries; a push past them is dropped), and walks a leaf's 2^3 voxels front to back.
+// Child offsets are taken as absolute word indices, as the reference does (the tree is the
+// scene's only one, so its words start at 0).
+__device__ __forceinline__ void scom2_intersect_fast(const SceneView& s, const GeomRef& g, float3 ray_pos,
+                                                     float3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
+                                                     CRT_Hit& hit) {
+  const SCom2Header& h = s.scom2_headers[g.offset.y];
+  const uint32_t* D = s.scom2_data;
+  const int4* M = s.scom2_mods + g.offset.y * (3 * COCTREE_ROT_COUNT);
+  const uint32_t type = g.type;
+  float3 pos_ray_pos = ray_pos;
+  float3 pos_ray_dir = ray_dir;
+  uint32_t a = 0;
+  if (ray_dir.x < 0) { pos_ray_pos.x *= -1; pos_ray_dir.x *= -1; a |= 4; }
+  if (ray_dir.y < 0) { pos_ray_pos.y *= -1; pos_ray_dir.y *= -1; a |= 2; }
+  if (ray_dir.z < 0) { pos_ray_pos.z *= -1; pos_ray_dir.z *= -1; a |= 1; }
+  pos_ray_pos = -1.0f * pos_ray_pos;
+  const float3 inv = safe_inverse(pos_ray_dir);
+  const float3 t0_ = pos_ray_pos * inv - inv;
+  const float3 t1_ = pos_ray_pos * inv + inv;
+  const float3 l_ = t1_ - t0_;
+  constexpr int FS = 16;
+  uint32_t st_links[FS], st_info[FS], st_tr[FS], st_pxy[FS], st_pzs[FS];
+  int top = 0;
+  float d = 1.0f;
+  float3 t0 = t0_;
+  float3 tm = t0 + 0.5f * l_;
+  uint32_t links, info, tr, pxy, pzs;
+  {
+    const uint32_t n0 = D[h.root_node_off];
+    const SCom2NodeHead node = scom2_node_head(h, n0, n0);
+    const uint32_t curr = uint32_t(ot_first_node(t0, tm));
+    links = h.root_node_off + h.links_offset + (node.base_type == SCOM2_CHILD_EMPTY ? 0u : 1u);
+    info = (node.children_types << 8) | curr;
+    tr = 0;
+    pxy = 0;
+    pzs = 1;
+  }
+  st_links[0] = 0xFFFFFFFFu;
+  st_info[0] = 0;
+  st_tr[0] = 0;
+  st_pxy[0] = 0;
+  st_pzs[0] = 0;
+  while (top >= 0) {
+    const uint32_t currNode = info & 0x7u;
+    const uint32_t raw = currNode ^ a;
+    const int4 mc = M[2 * COCTREE_ROT_COUNT + tr];
+    const uint32_t childN = uint32_t(mc.x * int((raw & 4) >> 2) + mc.y * int((raw & 2) >> 1) + mc.z * int(raw & 1) + mc.w);
+    const uint32_t active = (((info & 0x00AAAA00u) >> 1) | (info & 0x00555500u)) >> 8;
+    const uint32_t childLink = links + uint32_t(__popc(active & ((1u << (2 * childN)) - 1u)));
+    const uint32_t childHasData = active & (1u << (2 * childN));
+    const uint32_t childIsLeaf = info & (1u << (8 + 2 * childN));
+    const uint32_t rotId = info >> 24;
+    d = 1.0f / float(pzs & 0xFFFFu);
+    const float3 pf2 = make3(float(pxy >> 16) + ((currNode & 4) ? 1.0f : 0.5f),
+                             float(pxy & 0xFFFFu) + ((currNode & 2) ? 1.0f : 0.5f),
+                             float(pzs >> 16) + ((currNode & 1) ? 1.0f : 0.5f));
+    const float3 tl = t0_ + d * pf2 * l_;
+    bool advance = false;
+    if (fmin_ref(fmin_ref(tl.x, tl.y), tl.z) < tNear || childHasData == 0) {
+      advance = true;
+    } else if (childIsLeaf > 0) {
+      const uint32_t level_sz = 2 * (pzs & 0xFFFFu);
+      const uint32_t p0 = 2 * (pxy >> 16) + ((currNode & 4) >> 2);
+      const uint32_t p1 = 2 * (pxy & 0xFFFFu) + ((currNode & 2) >> 1);
+      const uint32_t p2 = 2 * (pzs >> 16) + (currNode & 1);
+      d = 2.0f / float(level_sz);
+      const float3 p_f = make3(float(p0), float(p1), float(p2));
+      t0 = t0_ + 0.5f * d * p_f * l_;
+      const float3 t1 = t0_ + 0.5f * d * (p_f + 1.0f) * l_;
+      const float max_val = scom2_max_sdf_val(float(level_sz));
+      const bool childIsSurface = (info & (1u << (8 + 2 * childN + 1))) != 0;
+      const uint32_t link0 = D[childLink];
+      const SCom2DataEdge de = scom2_data_edge(h, max_val, link0, link0);
+      const uint32_t offset = h.bricks_arr_offset + h.bricks_step * de.data_offset;
+      const int4 m = M[s.rot_add[rotId * COCTREE_ROT_COUNT + de.rotation_id]];
+      const float3 vox_tm = 0.5f * (t0 + t1);
+      uint32_t vox = uint32_t(ot_first_node(t0, vox_tm));
+      while (vox < 8) {
+        const float3 vox_tl = make3((vox & 4) ? t1.x : vox_tm.x, (vox & 2) ? t1.y : vox_tm.y, (vox & 1) ? t1.z : vox_tm.z);
+        const uint32_t voxId = vox ^ a;
+        float vmin = 1.0f;
+        float vmax = childIsSurface ? -1.0f : 1.0f;
+        float values[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const int cx = int((voxId & 4) >> 2) + ((i & 4) >> 2), cy = int((voxId & 2) >> 1) + ((i & 2) >> 1),
+                    cz = int(voxId & 1) + (i & 1);
+          const uint32_t vId0 = uint32_t(m.x * cx + m.y * cy + m.z * cz + m.w);
+          const uint32_t p_val = D[offset + vId0 / h.values_per_uint];
+          const uint32_t p_off = (vId0 % h.values_per_uint) * h.bits_per_value;
+          const float val = max_val * (2.0f * float((p_val >> p_off) & h.value_mask) / float(h.value_mask) - 1.0f) + de.add;
+          values[i] = val - 0.5f * s.preset.compact_sdf_eps * d;
+          vmin = fmin_ref(vmin, val);
+          vmax = fmax_ref(vmax, val);
+        }
+        if (vmin <= 0.0f && vmax >= 0.0f) {
+          const uint32_t p_mask = level_sz - 1;
+          const uint32_t r0 = (a & 4) ? (~p0 & p_mask) : p0, r1 = (a & 2) ? (~p1 & p_mask) : p1,
+                         r2 = (a & 1) ? (~p2 & p_mask) : p2;
+          const float3 vox_t0 = make3((vox & 4) ? vox_tm.x : t0.x, (vox & 2) ? vox_tm.y : t0.y, (vox & 1) ? vox_tm.z : t0.z);
+          const float3 vox_t1 = make3((vox & 4) ? t1.x : vox_tm.x, (vox & 2) ? t1.y : vox_tm.y, (vox & 1) ? t1.z : vox_tm.z);
+          const float tmin = fmax_ref(vox_t0.x, fmax_ref(vox_t0.y, vox_t0.z));
+          const float tmax = fmin_ref(vox_t1.x, fmin_ref(vox_t1.y, vox_t1.z));
+          const float3 start_pos = ray_pos + tmin * ray_dir;
+          const float3 min_pos = make3(-1, -1, -1) + d * make3(float(r0), float(r1), float(r2)) +
+                                 0.5f * d * make3(float((voxId & 4) >> 2), float((voxId & 2) >> 1), float(voxId & 1));
+          const float vox_sz = float(level_sz);
+          const float3 start_q = vox_sz * (start_pos - min_pos);
+          const float qNear = fmax_ref(0.0f, vox_sz * (tNear - tmin));
+          const float qFar = vox_sz * (tmax - tmin);
+          const float last_t = hit.t;
+          local_surface_fast(s, type, ray_dir, instId, geomId, values, childLink, 0.5f * d, qNear, qFar,
+                             float2{tmin, tmax}, start_q, hit);
+          if (hit.t < last_t) return;
+        }
+        vox = ot_new_node_i(vox, vox_tl.x, vox_tl.y, vox_tl.z);
+      }
+      advance = true;
+    } else {
+      const uint32_t next = ot_new_node_i(currNode, tl.x, tl.y, tl.z);
+      if (next < 8) {
+        info = next | (info & 0xFFFFFF00u);
+        if (top + 1 < FS) {
+          ++top;
+          st_links[top] = links;
+          st_info[top] = info;
+          st_tr[top] = tr;
+          st_pxy[top] = pxy;
+          st_pzs[top] = pzs;
+        }
+      }
+      d = 0.5f / float(pzs & 0xFFFFu);
+      const float3 p_f = make3(float(2 * (pxy >> 16) + ((currNode & 4) >> 2)), float(2 * (pxy & 0xFFFFu) + ((currNode & 2) >> 1)),
+                               float(2 * (pzs >> 16) + (currNode & 1)));
+      t0 = t0_ + d * p_f * l_;
+      tm = t0 + 0.5f * d * l_;
+      const uint32_t link0 = D[childLink];
+      const SCom2ChildEdge ce = scom2_child_edge(h, link0, link0);
+      const uint32_t n0 = D[ce.child_offset];
+      const SCom2NodeHead node = scom2_node_head(h, n0, n0);
+      const uint32_t nextNode = uint32_t(ot_first_node(t0, tm));
+      const uint32_t rotIdx = s.rot_add[rotId * COCTREE_ROT_COUNT + ce.rotation_id];
+      links = ce.child_offset + h.links_offset + (node.base_type == SCOM2_CHILD_EMPTY ? 0u : 1u);
+      tr = s.rot_add[tr * COCTREE_ROT_COUNT + ce.rotation_id];
+      info = (rotIdx << 24) | (node.children_types << 8) | nextNode;
+      pxy = (pxy << 1) | (((currNode & 4) << 14) | ((currNode & 2) >> 1));
+      pzs = (pzs << 1) | ((currNode & 1) << 16);
+    }
+    if (advance) {
+      const uint32_t next = ot_new_node_i(currNode, tl.x, tl.y, tl.z);
+      if (next >= 8) {
+        links = st_links[top];
+        info = st_info[top];
+        tr = st_tr[top];
+        pxy = st_pxy[top];
+        pzs = st_pzs[top];
+        top--;
+      } else {
+        info = next | (info & 0xFFFFFF00u);
+      }
     }
   }
 }

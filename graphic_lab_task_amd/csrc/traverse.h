@@ -57,7 +57,8 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
     } else if ((KINDS & LK_DAG) && tag == TAG_SDF_DAG) {
       dag_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, hit);
     } else if ((KINDS & LK_SCOM2) && tag == TAG_SCOM2) {
-      scom2_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, hit);
+      if (s.scom2_fast) scom2_intersect_fast(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, hit);
+      else scom2_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, hit);
     }
     if (s.preset.render_mode == MULTI_RENDER_MODE_INTERSECTION_COUNT)
       hit.instId = (hit.instId == 0xFFFFFFFFu) ? 1 : hit.instId + 1;

@@ -114,9 +114,11 @@ lrt_status lrt_add_sdf_dag(lrt_scene* s, const void* header, const void* nodes, 
  * nodes_arr_offset and bricks_arr_offset are replaced by where the words land, as the reference
  * does). Checked when added: every word a ray can make the walk read lies inside the tree,
  * rotations are below 48 and the tree is at most 15 levels deep; a failed check returns
- * LRT_ERR_INVALID_ARG with the reason. Trace calls return LRT_ERR_UNSUPPORTED where the
- * reference would take its fast traversal (one plain SCom2 tree, LAMBERT_NO_TEX) and for the
- * channel render modes of trees with channels: neither is built. Hits report TYPE_SCOM2. */
+ * LRT_ERR_INVALID_ARG with the reason. Where the reference takes its fast traversal
+ * (SCom2IntersectFast: the scene's only SCom2 tree is plain -- no multi-nodes, no channels, 3^3
+ * values, one-word links -- and the preset renders LAMBERT_NO_TEX) so does this library; a plain
+ * tree must then have brick_size 2. Trace calls return LRT_ERR_UNSUPPORTED for the channel render
+ * modes of trees with channels (not built). Hits report TYPE_SCOM2. */
 lrt_status lrt_add_scom2(lrt_scene* s, const void* header, const uint32_t* nodes, size_t n_nodes,
                          const uint32_t* bricks, size_t n_bricks, uint32_t* out_geom_id);
 

@@ -65,7 +65,7 @@ _ARRAYS = ["nodes", "tlas", "instances", "geoms", "geom_tags", "start_end", "pri
 
 
 class OracleScene(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in _ARRAYS] + [("preset", ctypes.c_uint8 * 60)]
+    _fields_ = [(n, ctypes.c_void_p) for n in _ARRAYS] + [("preset", ctypes.c_uint8 * 60), ("n_scom2", ctypes.c_uint32)]
 
 
 _libs = {}
@@ -129,6 +129,8 @@ class Scene:
         raw = np.ascontiguousarray(preset).tobytes()
         assert len(raw) == 60, "MultiRenderPreset is 60 bytes"
         ctypes.memmove(self.view.preset, raw, 60)
+        hs = arrays.get("scom2_headers")
+        self.view.n_scom2 = 0 if hs is None else int(np.asarray(hs).view(np.uint8).size // 216)
 
     @classmethod
     def from_scene(cls, scene) -> "Scene":

@@ -199,6 +199,7 @@ struct OracleScene {
   const SC2Header* scom2_headers; // m_SCom2Headers (offsets into scom2_data)
   const I4* scom2_mods;           // 3 x 48 index modifiers per tree
   Preset preset;
+  uint32_t n_scom2;               // SCom2 trees in the scene
 };
 
 // Per-ray traversal counters (the RayInfo idea, BVH2Common.cpp:3403-3571) for the
@@ -229,6 +230,13 @@ struct Tracer {
   OracleCounts* cnt;
   VoxelRec* rec = nullptr;     // records the voxel of each brick-set hit (test support)
   float last_t_local = 0.0f;   // local hit parameter of the latest LocalSurfaceIntersection hit
+
+  // support_fast_scom_traversal (BVH2Common_host.cpp:1436-1451): one plain SCom2 tree, LAMBERT_NO_TEX
+  bool scom2_fast() const {
+    if (s.n_scom2 != 1 || s.preset.render_mode != 1) return false;
+    const SC2Header& h = s.scom2_headers[0];
+    return h.has_multi_nodes == 0 && h.has_channels == 0 && h.v_size == 3 && h.links_offset == 1 && h.child_offset_off == 0;
+  }
 
   // BVH2Common.cpp:19-32
   bool need_normal() const {
@@ -1662,6 +1670,237 @@ struct Tracer {
     }
   }
 
+  // LocalSurfaceIntersectionFast (BVH2Common.cpp:2820-2986): the analytic root (or the box in
+  // BBOX mode), the normal from central differences without the instance transform
+  void LocalSurfaceIntersectionFast(uint32_t type, f3 ray_dir, uint32_t instId, uint32_t geomId, const float values[8],
+                                    uint32_t primId, float d, float qNear, float qFar, f2 fNearFar, f3 start_q,
+                                    Hit* pHit) {
+    const float EPS = 1e-6f;
+    const float d_inv = 1.0f / d;
+    float t = qNear;
+    bool hit = false;
+    const float start_dist = eval_dist_trilinear(values, start_q + t * ray_dir);
+    if (start_dist <= EPS || s.preset.sdf_node_intersect == 3) {
+      hit = true;
+    } else {
+      const float s000 = values[0] * d_inv, s001 = values[1] * d_inv, s010 = values[2] * d_inv, s011 = values[3] * d_inv;
+      const float s100 = values[4] * d_inv, s101 = values[5] * d_inv, s110 = values[6] * d_inv, s111 = values[7] * d_inv;
+      const float tmp = s101 - s001;
+      const float k0 = s000, k1 = s100 - s000, k2 = s010 - s000, k3 = s110 - s010 - k1, k4 = k0 - s001;
+      const float k5 = k1 - tmp, k6 = k2 - (s011 - s001), k7 = k3 - (s111 - s011 - tmp);
+      const f3 o = start_q, d3 = ray_dir;
+      const float m0 = o.x * o.y, m1 = d3.x * d3.y, m2 = o.x * d3.y + o.y * d3.x;
+      const float m3 = k5 * o.z - k1, m4 = k6 * o.z - k2, m5 = k7 * o.z - k3;
+      const float c0 = (k4 * o.z - k0) + o.x * m3 + o.y * m4 + m0 * m5;
+      const float c1 = d3.x * m3 + d3.y * m4 + m2 * m5 + d3.z * (k4 + k5 * o.x + k6 * o.y + k7 * m0);
+      const float c2 = m1 * m5 + d3.z * (k5 * d3.x + k6 * d3.y + k7 * m2);
+      const float c3 = k7 * m1 * d3.z;
+      float x1 = 1000, x2 = 1000, x3 = 1000;
+      const float PI = 3.14159265358979323846f;
+      if (std::abs(c3) > 1e-5f) {
+        float a = c2 / c3, b = c1 / c3, c = c0 / c3;
+        float Q = (a * a - 3 * b) / 9;
+        float R = (2 * a * a * a - 9 * a * b + 27 * c) / 54;
+        float Q3 = Q * Q * Q;
+        if (R * R < Q3) {
+          float theta = std::acos(R / std::sqrt(Q3));
+          x1 = -2 * std::sqrt(Q) * std::cos(theta / 3) - a / 3;
+          x2 = -2 * std::sqrt(Q) * std::cos((theta + 2 * PI) / 3) - a / 3;
+          x3 = -2 * std::sqrt(Q) * std::cos((theta - 2 * PI) / 3) - a / 3;
+        } else {
+          float A = -lm_sign(R) * std::pow(std::abs(R) + std::sqrt(R * R - Q3), 1.0f / 3.0f);
+          float B = std::abs(A) > EPS ? Q / A : 0;
+          x1 = A + B - a / 3;
+        }
+      } else if (std::abs(c2) > 1e-7f) {
+        float a = c2, b = c1, c = c0;
+        float D = b * b - 4 * a * c;
+        if (D > 0) {
+          float q = -0.5f * (b + lm_sign(b) * std::sqrt(D));
+          x1 = q / a;
+          if (std::abs(q) > EPS) x2 = c / q;
+        }
+      } else if (std::abs(c1) > EPS) {
+        x1 = -c0 / c1;
+      }
+      x1 = x1 < 0 ? 1000 : x1;
+      x2 = x2 < 0 ? 1000 : x2;
+      x3 = x3 < 0 ? 1000 : x3;
+      t = std::min(x1, std::min(x2, x3));
+      hit = (t >= 0 && t <= qFar);
+    }
+    const float tReal = fNearFar.x + d * t;
+    if (t <= qFar && hit && tReal < pHit->t) {
+      const f3 p0 = start_q + t * ray_dir;
+      const float h = 0.001f;
+      const float ddx = (eval_dist_trilinear(values, p0 + f3(h, 0, 0)) - eval_dist_trilinear(values, p0 + f3(-h, 0, 0))) / (2 * h);
+      const float ddy = (eval_dist_trilinear(values, p0 + f3(0, h, 0)) - eval_dist_trilinear(values, p0 + f3(0, -h, 0))) / (2 * h);
+      const float ddz = (eval_dist_trilinear(values, p0 + f3(0, 0, h)) - eval_dist_trilinear(values, p0 + f3(0, 0, -h))) / (2 * h);
+      const f2 en = encode_normal(normalize(f3(ddx, ddy, ddz)));
+      pHit->t = tReal;
+      pHit->primId = primId;
+      pHit->instId = instId;
+      pHit->geomId = geomId | (type << SH_TYPE);
+      pHit->coords[0] = 0;
+      pHit->coords[1] = 0;
+      pHit->coords[2] = en.x;
+      pHit->coords[3] = en.y;
+    }
+  }
+
+  // new_node(i, txm, tym, tzm) (utils/raytracing/intersection_common.h:87-93)
+  static uint32_t NewNodeI(uint32_t i, float txm, float tym, float tzm) {
+    const uint32_t x = 4 + i, y = 2 + (i & 0x5) + ((i & 0x2) << 3), z = 1 + (i & 0x6) + ((i & 0x1) << 3);
+    return (txm < tym) ? (txm < tzm ? x : z) : (tym < tzm ? y : z);
+  }
+
+  // SCom2IntersectFast (BVH2Common.cpp:2988-3231), taken for a scene whose only SCom2 tree is plain
+  // (no multi-nodes, no channels, 3^3 values, one-word links) when the preset renders
+  // LAMBERT_NO_TEX: one child per step, a 16-entry stack of the nodes still to finish, leaves
+  // walked voxel by voxel in front-to-back order.
+  void SCom2IntersectFast(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId, Hit* pHit) {
+    struct Ext { uint32_t linksOff, info, transform, p_size_x, p_size_y; };
+    const uint32_t type = TYPE_SCOM2;
+    const uint32_t hidx = s.geoms[geomId].offset[1];
+    const SC2Header header = s.scom2_headers[hidx];
+    const uint32_t* data = s.scom2_data;
+    const I4* mods = s.scom2_mods + hidx * 144;
+    f3 pos_ray_pos = ray_pos, pos_ray_dir = ray_dir;
+    uint32_t a = 0;
+    if (ray_dir.x < 0) { pos_ray_pos.x *= -1; pos_ray_dir.x *= -1; a |= 4; }
+    if (ray_dir.y < 0) { pos_ray_pos.y *= -1; pos_ray_dir.y *= -1; a |= 2; }
+    if (ray_dir.z < 0) { pos_ray_pos.z *= -1; pos_ray_dir.z *= -1; a |= 1; }
+    pos_ray_pos = pos_ray_pos * -1.0f;
+    const f3 pos_ray_dir_inv = SafeInverse(pos_ray_dir);
+    const f3 _t0 = pos_ray_pos * pos_ray_dir_inv - pos_ray_dir_inv;
+    const f3 _t1 = pos_ray_pos * pos_ray_dir_inv + pos_ray_dir_inv;
+    const f3 _l = _t1 - _t0;
+    Ext stack[16];
+    int top = 0;
+    float d = 1.0f;
+    f3 t0 = _t0;
+    f3 tm = t0 + 0.5f * _l;
+    Ext cur;
+    {
+      const uint32_t nodeData0 = data[header.root_node_off];
+      const NodeHead node = unpack_node_head(header, nodeData0, nodeData0);
+      const uint32_t currNode = uint32_t(FirstNode(t0, tm));
+      cur.linksOff = header.root_node_off + header.links_offset + (node.base_type == 0 ? 0 : 1);
+      cur.info = (0u << 24) | (node.children_types << 8) | currNode;
+      cur.transform = 0;
+      cur.p_size_x = 0;
+      cur.p_size_y = 1;
+    }
+    stack[0] = Ext{0xFFFFFFFFu, 0, 0, 0, 0};
+    while (top >= 0) {
+      const uint32_t currNode = cur.info & 0x7;
+      const uint32_t rawChildNode = currNode ^ a;
+      const int vp[3] = {int((rawChildNode & 4) >> 2), int((rawChildNode & 2) >> 1), int(rawChildNode & 1)};
+      const I4& mc = mods[2 * 48 + cur.transform];
+      const uint32_t childN = uint32_t(mc.x * vp[0] + mc.y * vp[1] + mc.z * vp[2] + mc.w * 1);
+      const uint32_t activeChildrenMask = (((cur.info & 0x00AAAA00u) >> 1) | (cur.info & 0x00555500u)) >> 8;
+      const uint32_t childLink = cur.linksOff + uint32_t(__builtin_popcount(activeChildrenMask & ((1u << (2 * childN)) - 1)));
+      const uint32_t childHasData = activeChildrenMask & (1u << (2 * childN));
+      const uint32_t childIsLeaf = cur.info & (1u << (8 + 2 * childN));
+      const uint32_t rotId = cur.info >> 24;
+      d = 1.0f / float(cur.p_size_y & 0xFFFF);
+      const f3 pf2 = f3(float(cur.p_size_x >> 16) + (((currNode & 4) > 0) ? 1.0f : 0.5f),
+                        float(cur.p_size_x & 0xFFFF) + (((currNode & 2) > 0) ? 1.0f : 0.5f),
+                        float(cur.p_size_y >> 16) + (((currNode & 1) > 0) ? 1.0f : 0.5f));
+      const f3 tl = _t0 + d * pf2 * _l;
+      if (std::min(std::min(tl.x, tl.y), tl.z) < tNear || childHasData == 0) {
+        const uint32_t next = NewNodeI(currNode, tl.x, tl.y, tl.z);
+        if (next >= 8) cur = stack[top--];
+        else cur.info = next | (cur.info & 0xFFFFFF00u);
+      } else if (childIsLeaf > 0) {
+        const uint32_t level_sz = 2 * (cur.p_size_y & 0xFFFF);
+        const uint32_t p[3] = {2 * (cur.p_size_x >> 16) + ((currNode & 4) >> 2), 2 * (cur.p_size_x & 0xFFFF) + ((currNode & 2) >> 1),
+                               2 * (cur.p_size_y >> 16) + (currNode & 1)};
+        d = 2.0f / float(level_sz);
+        const f3 p_f = f3(float(p[0]), float(p[1]), float(p[2]));
+        t0 = _t0 + 0.5f * d * p_f * _l;
+        const f3 t1 = _t0 + 0.5f * d * (p_f + 1.0f) * _l;
+        const float max_val = 1.7320508f * 2.0f / float(level_sz);
+        const bool childIsSurface = (cur.info & (1u << (8 + 2 * childN + 1))) > 0;
+        const uint32_t linkData0 = data[childLink];
+        const DataEdge de = unpack_data_edge(header, max_val, linkData0, linkData0);
+        const uint32_t offset = header.bricks_arr_offset + header.bricks_step * de.data_offset;
+        const uint32_t rotIdx = s.rot_add[rotId * 48 + de.rotation_id];
+        const float add = de.add;
+        const f3 vox_tm = 0.5f * (t0 + t1);
+        uint32_t vox_currNode = uint32_t(FirstNode(t0, vox_tm));
+        while (vox_currNode < 8) {
+          const f3 vox_tl((vox_currNode & 4) > 0 ? t1.x : vox_tm.x, (vox_currNode & 2) > 0 ? t1.y : vox_tm.y,
+                          (vox_currNode & 1) > 0 ? t1.z : vox_tm.z);
+          const uint32_t voxId = vox_currNode ^ a;
+          float vmin = 1.0f;
+          float vmax = childIsSurface ? -1.0f : 1.0f;
+          float values[8];
+          for (int i = 0; i < 8; i++) {
+            const int pI[3] = {int((voxId & 4) >> 2) + ((i & 4) >> 2), int((voxId & 2) >> 1) + ((i & 2) >> 1),
+                               int(voxId & 1) + (i & 1)};
+            const I4& m = mods[rotIdx];
+            const uint32_t vId0 = uint32_t(m.x * pI[0] + m.y * pI[1] + m.z * pI[2] + m.w * 1);
+            const uint32_t p_val = data[offset + vId0 / header.values_per_uint];
+            const uint32_t p_off = (vId0 % header.values_per_uint) * header.bits_per_value;
+            const float val = max_val * (2.0f * ((p_val >> p_off) & header.value_mask) / float(header.value_mask) - 1) + add;
+            values[i] = val - 0.5f * s.preset.compact_sdf_eps * d;
+            vmin = std::min(vmin, val);
+            vmax = std::max(vmax, val);
+          }
+          if (vmin <= 0.0f && vmax >= 0.0f) {
+            const uint32_t p_mask = level_sz - 1;
+            const uint32_t rp[3] = {(a & 4) ? (~p[0] & p_mask) : p[0], (a & 2) ? (~p[1] & p_mask) : p[1],
+                                    (a & 1) ? (~p[2] & p_mask) : p[2]};
+            const f3 vox_t0((vox_currNode & 4) > 0 ? vox_tm.x : t0.x, (vox_currNode & 2) > 0 ? vox_tm.y : t0.y,
+                            (vox_currNode & 1) > 0 ? vox_tm.z : t0.z);
+            const f3 vox_t1((vox_currNode & 4) > 0 ? t1.x : vox_tm.x, (vox_currNode & 2) > 0 ? t1.y : vox_tm.y,
+                            (vox_currNode & 1) > 0 ? t1.z : vox_tm.z);
+            const float tmin = std::max(vox_t0.x, std::max(vox_t0.y, vox_t0.z));
+            const float tmax = std::min(vox_t1.x, std::min(vox_t1.y, vox_t1.z));
+            const f3 start_pos = ray_pos + tmin * ray_dir;
+            const f3 min_pos = f3(-1, -1, -1) + d * f3(float(rp[0]), float(rp[1]), float(rp[2])) +
+                               0.5f * d * f3(float((voxId & 4) >> 2), float((voxId & 2) >> 1), float(voxId & 1));
+            const float vox_sz = float(level_sz);
+            const f3 start_q = vox_sz * (start_pos - min_pos);
+            const float qNear = std::max(0.0f, vox_sz * (tNear - tmin));
+            const float qFar = vox_sz * (tmax - tmin);
+            const float last_t = pHit->t;
+            LocalSurfaceIntersectionFast(type, ray_dir, instId, geomId, values, childLink, 0.5f * d, qNear, qFar,
+                                         f2{tmin, tmax}, start_q, pHit);
+            if (pHit->t < last_t) return;
+          }
+          vox_currNode = NewNodeI(vox_currNode, vox_tl.x, vox_tl.y, vox_tl.z);
+        }
+        const uint32_t next = NewNodeI(currNode, tl.x, tl.y, tl.z);
+        if (next >= 8) cur = stack[top--];
+        else cur.info = next | (cur.info & 0xFFFFFF00u);
+      } else {
+        const uint32_t next = NewNodeI(currNode, tl.x, tl.y, tl.z);
+        if (next < 8) {
+          cur.info = next | (cur.info & 0xFFFFFF00u);
+          if (top + 1 < 16) stack[++top] = cur;
+        }
+        d = 0.5f / float(cur.p_size_y & 0xFFFF);
+        const f3 p_f = f3(float(2 * (cur.p_size_x >> 16) + ((currNode & 4) >> 2)), float(2 * (cur.p_size_x & 0xFFFF) + ((currNode & 2) >> 1)),
+                          float(2 * (cur.p_size_y >> 16) + (currNode & 1)));
+        t0 = _t0 + d * p_f * _l;
+        tm = t0 + 0.5f * d * _l;
+        const uint32_t linkData0 = data[childLink];
+        const ChildEdge ce = unpack_child_edge(header, linkData0, linkData0);
+        const uint32_t nodeData0 = data[ce.child_offset];
+        const NodeHead node = unpack_node_head(header, nodeData0, nodeData0);
+        const uint32_t nextNode = uint32_t(FirstNode(t0, tm));
+        const uint32_t rotIdx = s.rot_add[rotId * 48 + ce.rotation_id];
+        cur.linksOff = ce.child_offset + header.links_offset + (node.base_type == 0 ? 0 : 1);
+        cur.transform = s.rot_add[cur.transform * 48 + ce.rotation_id];
+        cur.info = (rotIdx << 24) | (node.children_types << 8) | nextNode;
+        cur.p_size_x = (cur.p_size_x << 1) | (((currNode & 4) << (16 - 2)) | ((currNode & 2) >> 1));
+        cur.p_size_y = (cur.p_size_y << 1) | ((currNode & 1) << 16);
+      }
+    }
+  }
+
   // BVH2Common.cpp:47-124
   uint32_t IntersectAllPrimitivesInLeaf(f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
                                         uint32_t aabbId, Hit* pHit) {
@@ -1680,7 +1919,10 @@ struct Tracer {
       case TAG_GRAPHICS_PRIM: IntersectGraphicPrims(ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
       case TAG_COCTREE_BRICKED: COctreeIntersectV3(ray_pos, ray_dir, tNear, instId, geomId, a_start, pHit); break;
       case TAG_SDF_DAG: DAGIntersect(ray_pos, ray_dir, tNear, instId, geomId, pHit); break;
-      case TAG_SCOM2: SCom2Intersect(ray_pos, ray_dir, tNear, instId, geomId, pHit); break;
+      case TAG_SCOM2:
+        if (scom2_fast()) SCom2IntersectFast(ray_pos, ray_dir, tNear, instId, geomId, pHit);
+        else SCom2Intersect(ray_pos, ray_dir, tNear, instId, geomId, pHit);
+        break;
       default: break;
     }
     if (s.preset.render_mode == 26) pHit->instId = (pHit->instId == 0xFFFFFFFF) ? 1 : pHit->instId + 1;

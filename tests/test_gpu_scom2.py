@@ -3,8 +3,9 @@ the oracle on the same arrays: the reference's engine-scene SCom2 file (bricks s
 rotations and shifts) in every intersection mode, with and without the compact surface offset,
 and the LOD, iteration, intersection-count, channel and normal render modes; instances beside a
 DAG, a compact octree and a mesh (the mixed kernel), the persistent kernel and any-hit; the
-queries the library refuses (the reference's fast traversal of one plain tree). Records are
-bit-identical except under the analytic cubic, held to the SDF bar of conftest.sdf_check.
+reference's fast traversal of one plain tree (encoded by tests/scom2_encode.py). Records are
+bit-identical except under the analytic cubic (which the fast walk always solves, BBOX aside),
+held to the SDF bar of conftest.sdf_check.
 """
 import os
 import sys
@@ -12,7 +13,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import MISS, raw_equal
+from conftest import MISS, assert_sdf_parity, raw_equal, sdf_check
 from test_gpu_coctree import _check, _preset, _rays
 
 pytestmark = pytest.mark.gpu
@@ -105,27 +106,80 @@ def test_scom2_instances_mixed_persistent_any(lrt, orc, torch_dev, ref_tree):
     assert np.array_equal(flags.cpu().numpy().astype(np.uint32), want)
 
 
-def test_fast_traversal_case_is_refused(lrt, torch_dev):
-    """One plain SCom2 tree (no multi-nodes, no channels, 3^3 values, one-word links) rendered
-    LAMBERT_NO_TEX is where the reference switches to SCom2IntersectFast, which is not built: the
-    call fails with LRT_ERR_UNSUPPORTED instead of answering differently; other render modes run."""
-    g = lrt
-    torch = torch_dev
-    _say("scom2 fast traversal case")
-    h = np.zeros((), g.SCOM2_HEADER_DTYPE)
-    h["brick_size"], h["v_size"], h["bits_per_value"], h["values_per_uint"], h["value_mask"] = 2, 3, 8, 4, 0xFF
-    h["dimension"], h["uints_per_link"], h["links_offset"], h["child_offset_off"] = 3, 1, 1, 0
-    h["children_types_shift"], h["children_types_mask"] = 8, 0xFFFF
-    s = g.SceneRT()
-    s.AddInstance(s.AddGeom_SCom2(h, np.zeros(1, np.uint32), np.zeros(1, np.uint32)))
-    s.CommitScene()
-    pos, dirs = _rays(torch, g, res=16)
+@pytest.fixture(scope="module")
+def plain_trees(lrt):
+    import scom2_encode as se
+    from test_coctree_cpu import SPHERES
+    nodes, values = lrt.sdf_spheres_global_octree(SPHERES, 5, 2, 0)
+    return {"rotated": se.encode(nodes, values, rotate=True, seed=3),
+            "mixed": se.encode(nodes, values, rotate=True, shift=True, unique_every=5, volume_every=7, seed=4)}
+
+
+def _check_fast(torch, g, orc, s, pos, dirs, preset):
+    """The fast walk's surface step (LocalSurfaceIntersectionFast, BVH2Common.cpp:2820-2986) solves
+    the analytic cubic in every intersection mode but BBOX, so its records are held to the SDF bar
+    (the device's acosf/cosf/powf differ from glibc's in the last ulps); BBOX hits are exact."""
+    if int(preset["sdf_node_intersect"]) == 3:
+        return _check(torch, g, orc, s, pos, dirs, preset)
+    s.SetPreset(preset)
     hits = torch.zeros((pos.shape[0], 8), dtype=torch.int32, device="cuda")
-    s.SetPreset(_preset(g, 0, 1))
-    with pytest.raises(g.LrtError) as e:
-        s.trace_closest(pos, dirs, hits)
-    assert e.value.code == 4   # LRT_ERR_UNSUPPORTED
-    s.SetPreset(_preset(g, 0, 8))
     s.trace_closest(pos, dirs, hits)
     torch.cuda.synchronize()
-    assert np.all(hits.cpu().numpy().view(g.HIT_DTYPE)["primId"] == MISS)
+    got = hits.cpu().numpy().view(g.HIT_DTYPE).reshape(-1)
+    want, ov = orc.trace_closest(orc.Scene.from_scene(s), pos.cpu().numpy(), dirs.cpu().numpy())
+    assert ov == 0
+    assert_sdf_parity(sdf_check(got, want, pos.cpu().numpy(), dirs.cpu().numpy(), colour=True), got, want)
+    return got
+
+
+@pytest.mark.parametrize("key", ["rotated", "mixed"])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("eps", [0.0, 0.05])
+def test_fast_traversal_plain_tree(lrt, orc, torch_dev, plain_trees, key, mode, eps):
+    """One plain SCom2 tree (no multi-nodes, no channels, 3^3 values, one-word links) rendered
+    LAMBERT_NO_TEX takes the fast traversal (SCom2IntersectFast, BVH2Common.cpp:2988-3231) in
+    every intersection mode; NORMAL rendering of the same tree takes the general walk."""
+    g = lrt
+    _say(f"scom2 plain tree {key}, intersection mode {mode}, compact eps {eps}")
+    s = g.SceneRT()
+    s.AddInstance(s.AddGeom_SCom2(*plain_trees[key]))
+    s.CommitScene()
+    pos, dirs = _rays(torch_dev, g)
+    for render in (1, 8):
+        p = _preset(g, mode, render)
+        p["compact_sdf_eps"] = eps
+        got = (_check_fast if render == 1 else _check)(torch_dev, g, orc, s, pos, dirs, p)
+        assert (got["primId"] != MISS).mean() > 0.1
+
+
+def test_fast_traversal_instances_persistent_any(lrt, orc, torch_dev, plain_trees):
+    """Instances of the one plain tree beside a mesh (the mixed kernel) keep the fast walk; the
+    persistent kernel and any-hit agree."""
+    g = lrt
+    torch = torch_dev
+    _say("scom2 plain tree instances beside a mesh")
+    s = g.SceneRT()
+    a = s.AddGeom_SCom2(*plain_trees["mixed"])
+    verts, tris = g.gen_icosphere(3, 0.05, 4.0, 3)
+    v = np.array(verts, np.float32)
+    v[:, :3] *= 0.3
+    m = s.AddGeom_Triangles3f(v, tris)
+    for k, (tx, ty, sc) in enumerate([(-0.8, 0.0, 0.6), (0.7, 0.2, 0.5), (0.1, 0.8, 0.4)]):
+        t = np.eye(4, dtype=np.float32)
+        ang = 0.5 * k
+        t[:3, :3] = sc * np.array([[np.cos(ang), 0, np.sin(ang)], [0, 1, 0], [-np.sin(ang), 0, np.cos(ang)]], np.float32)
+        t[:3, 3] = [tx, ty, -0.2]
+        s.AddInstance(a, t)
+    s.AddInstance(m, np.eye(4, dtype=np.float32))
+    s.CommitScene()
+    pos, dirs = _rays(torch, g, eye=(0.2, 0.6, 3.2))
+    got = _check_fast(torch, g, orc, s, pos, dirs, _preset(g, 0, 1))
+    assert len(set((got["geomId"][got["primId"] != MISS] >> 24).tolist())) >= 2
+    s.set_persistent(1)
+    assert raw_equal(_check_fast(torch, g, orc, s, pos, dirs, _preset(g, 0, 1)), got)
+    s.set_persistent(0)
+    flags = torch.zeros(pos.shape[0], dtype=torch.int32, device="cuda")
+    s.trace_any(pos, dirs, flags)
+    torch.cuda.synchronize()
+    want = orc.trace_any(orc.Scene.from_scene(s), pos.cpu().numpy(), dirs.cpu().numpy())
+    assert np.array_equal(flags.cpu().numpy().astype(np.uint32), want)

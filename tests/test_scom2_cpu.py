@@ -109,3 +109,84 @@ def test_host_checks_reject_malformed_trees(lrt):
             n5[root + lo + start * upl + coo] = (n5[root + lo + start * upl + coo] & ~np.uint32(h["node_offset_mask"])) | root
             rejects((h, n5, b))
             break
+
+
+# ---- plain trees: the fast traversal ----
+@pytest.fixture(scope="module")
+def plain(lrt):
+    import scom2_encode as se
+    from test_coctree_cpu import SPHERES
+    nodes, values = lrt.sdf_spheres_global_octree(SPHERES, 4, 2, 0)
+    return {"go": (nodes, values),
+            "plain": se.encode(nodes, values),
+            "rotated": se.encode(nodes, values, rotate=True, seed=3),
+            "mixed": se.encode(nodes, values, rotate=True, shift=True, unique_every=5, volume_every=7, seed=4)}
+
+
+def lambert(lrt, eps=0.0, mode=1):
+    p = lrt.default_preset()
+    p["render_mode"], p["compact_sdf_eps"], p["sdf_node_intersect"] = 1, eps, mode   # LAMBERT_NO_TEX: fast walk
+    return p
+
+
+def normal_mode(lrt, eps=0.0, mode=1):
+    p = lrt.default_preset()
+    p["render_mode"], p["compact_sdf_eps"], p["sdf_node_intersect"] = 8, eps, mode   # NORMAL: the general walk
+    return p
+
+
+def test_plain_tree_rotations_are_transparent(lrt, orc, plain):
+    """Subtrees and bricks stored through random rotations (composed down the tree) decode to the
+    same values: both walks give bit-identical records for the rotated and the plain encoding."""
+    pos, dir_ = sphere_rays(3000)
+    for preset in (lambert(lrt), normal_mode(lrt), lambert(lrt, 0.05, 3), normal_mode(lrt, 0.05, 0)):
+        a = oracle_trace(orc, lrt, scom2_scene(lrt, plain["plain"], preset), pos, dir_)
+        b = oracle_trace(orc, lrt, scom2_scene(lrt, plain["rotated"], preset), pos, dir_)
+        assert (a["primId"] != MISS).mean() > 0.2
+        assert np.array_equal(np.delete(a.view(np.uint32).reshape(-1, 8), 1, 1),
+                              np.delete(b.view(np.uint32).reshape(-1, 8), 1, 1))   # primId: link words differ
+
+
+def test_fast_walk_agrees_with_general_walk(lrt, orc, plain):
+    """The fast traversal (LAMBERT_NO_TEX) and the general walk (other render modes) find the same
+    hits on plain trees, at t within rounding of their different voxel set-ups, with and without
+    shared, shifted, unique and volume bricks."""
+    pos, dir_ = sphere_rays(3000)
+    for key in ("plain", "mixed"):
+        a = oracle_trace(orc, lrt, scom2_scene(lrt, plain[key], lambert(lrt)), pos, dir_)
+        b = oracle_trace(orc, lrt, scom2_scene(lrt, plain[key], normal_mode(lrt)), pos, dir_)
+        ha, hb = a["primId"] != MISS, b["primId"] != MISS
+        assert ha.mean() > 0.2 and (ha != hb).mean() < 2e-3
+        dt = np.abs(a["t"][ha & hb] - b["t"][ha & hb])
+        assert np.percentile(dt, 99) < 1e-4
+
+
+def test_plain_tree_matches_brick_set(lrt, orc, plain):
+    """A plain tree of the sphere-union octree (brick size 2, 8-bit values) is hit like the brick
+    set of the same octree."""
+    pos, dir_ = sphere_rays(3000)
+    a = oracle_trace(orc, lrt, scom2_scene(lrt, plain["plain"], lambert(lrt)), pos, dir_)
+    hdr = np.zeros((), lrt.SBS_HEADER_DTYPE)
+    hdr["brick_size"], hdr["brick_pad"], hdr["bytes_per_value"], hdr["aux_data"] = 2, 0, 4, 1 << 24
+    from test_coctree_cpu import SPHERES
+    sn, sv = lrt.sdf_spheres_sbs(SPHERES, 4, hdr)
+
+    def build(s):
+        s.AddInstance(s.AddGeom_SdfSBS(hdr, sn, sv))
+        s.SetPreset(lambert(lrt))
+    b = oracle_trace(orc, lrt, host_scene(lrt, build), pos, dir_)
+    ha, hb = a["primId"] != MISS, b["primId"] != MISS
+    assert ha.mean() > 0.2 and (ha != hb).mean() < 5e-3
+    assert np.median(np.abs(a["t"][ha & hb] - b["t"][ha & hb])) < 2e-3
+
+
+def test_plain_tree_host_checks(lrt, plain):
+    h, n, b = plain["plain"]
+    h2 = h.copy()
+    h2["brick_size"] = 1                      # 3^3 values but brick size 1
+    with pytest.raises(lrt.LrtError):
+        scom2_scene(lrt, (h2, n, b))
+    n2 = n.copy()
+    n2[1] = (n2[1] & np.uint32(0xFC000000)) | np.uint32(n.size + 100)   # first link points past the words
+    with pytest.raises(lrt.LrtError):
+        scom2_scene(lrt, (h, n2, b))
