@@ -35,6 +35,8 @@ INSTANCE_DTYPE = np.dtype([("boxMin", "<f4", (4,)), ("boxMax", "<f4", (4,)), ("g
                            ("transformInv", "<f4", (16,)),
                            ("transformInvTransposed", "<f4", (16,))])              # InstanceData, 256 B
 FRAME_NODE_DTYPE = np.dtype([("values", "<f4", (8,)), ("offset", "<u4")])         # SdfFrameOctreeNode, 36 B
+FRAME_TEX_NODE_DTYPE = np.dtype([("tex_coords", "<f4", (16,)), ("values", "<f4", (8,)), ("offset", "<u4"),
+                                 ("material_id", "<u4")])                           # SdfFrameOctreeTexNode, 104 B
 MULTI_NODE_DTYPE = np.dtype([("data_offset", "<u4"), ("children_offset", "<u4"), ("voxel_count", "<u4"),
                              ("flags", "<u4")])                                       # SdfMultiOctreeNode, 16 B
 GPRIM_HEADER_DTYPE = np.dtype([("color", "<f4", (3,)), ("prim_type", "<u4")])   # GraphicsPrimHeader, 16 B
@@ -123,6 +125,7 @@ ARRAY_DTYPES = {
     "geom_tags": np.uint32, "start_end": np.uint32, "prim_id_count": np.uint32,
     "vert_pos": np.float32, "vert_norm": np.float32, "indices": np.uint32, "prim_indices": np.uint32,
     "orig_nodes": NODE_DTYPE, "frame_nodes": FRAME_NODE_DTYPE, "frame_roots": np.uint32,
+    "frame_tex_nodes": FRAME_TEX_NODE_DTYPE, "frame_tex_roots": np.uint32,
     "sbs_nodes": SBS_NODE_DTYPE, "sbs_data": np.uint32, "sbs_data_f": np.float32, "sbs_roots": np.uint32,
     "sbs_headers": SBS_HEADER_DTYPE, "multi_nodes": MULTI_NODE_DTYPE, "multi_values": np.float32,
     "preset": PRESET_DTYPE, "sbs_mask": np.uint32, "orig_indices": np.uint32, "svs_nodes": SVS_NODE_DTYPE,
@@ -150,6 +153,7 @@ def default_lights() -> np.ndarray:
 
 # enums (core/render_settings.h, BVH2Common.h:30-43, sdf_simple_shared.h:13-21)
 TYPE_MESH_TRIANGLE, TYPE_SDF_FRAME_OCTREE, TYPE_SDF_SBS, TYPE_SDF_SBS_COL = 0, 3, 6, 11
+TYPE_SDF_FRAME_OCTREE_TEX = 8
 SH_TYPE, GEOM_ID_MASK = 24, 0x00FFFFFF
 MISS = 0xFFFFFFFF
 NOALLOC_MAX_RAYS = 65536   # LRT_NOALLOC_MAX_RAYS (include/lrt.h)
@@ -239,6 +243,7 @@ def lib() -> ctypes.CDLL:
             "lrt_destroy": ([vp], i32), "lrt_clear_geom": ([vp], i32), "lrt_clear_scene": ([vp], i32),
             "lrt_add_triangles": ([vp, vp, sz, sz, vp, vp, sz, u32, pu32], i32),
             "lrt_add_sdf_frame_octree": ([vp, vp, sz, pu32], i32),
+            "lrt_add_sdf_frame_octree_tex": ([vp, vp, sz, pu32], i32),
             "lrt_add_sdf_svs": ([vp, vp, sz, pu32], i32),
             "lrt_add_sdf_grid": ([vp, pu32, vp, pu32], i32),
             "lrt_add_graphics_prims": ([vp, vp, vp, sz, pu32], i32),
@@ -303,6 +308,8 @@ def lib() -> ctypes.CDLL:
             "lrt_tonemap_dev": ([vp, u64, u32, vp, vp], i32),
             "lrt_load_sdf_frame_octree": ([cp, ctypes.POINTER(_Buffer)], i32),
             "lrt_save_sdf_frame_octree": ([cp, vp, sz], i32),
+            "lrt_load_sdf_frame_octree_tex": ([cp, ctypes.POINTER(_Buffer)], i32),
+            "lrt_save_sdf_frame_octree_tex": ([cp, vp, sz], i32),
             "lrt_load_sdf_sbs": ([cp, vp, ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer), ctypes.POINTER(_Buffer)],
                                  i32),
             "lrt_save_sdf_sbs": ([cp, vp, vp, sz, vp, sz, vp, sz], i32),
@@ -319,7 +326,7 @@ def lib() -> ctypes.CDLL:
 
 def exported_symbols():
     return ["lrt_last_error", "lrt_version", "lrt_create", "lrt_destroy", "lrt_clear_geom", "lrt_clear_scene",
-            "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_svs", "lrt_load_sdf_svs", "lrt_save_sdf_svs",
+            "lrt_add_triangles", "lrt_add_sdf_frame_octree", "lrt_add_sdf_frame_octree_tex", "lrt_add_sdf_svs", "lrt_load_sdf_svs", "lrt_save_sdf_svs",
             "lrt_add_sdf_grid", "lrt_load_sdf_grid", "lrt_save_sdf_grid", "lrt_add_graphics_prims",
             "lrt_add_coctree_v3", "lrt_load_coctree_v3", "lrt_save_coctree_v3",
             "lrt_add_sdf_dag", "lrt_load_sdf_dag", "lrt_save_sdf_dag", "lrt_add_scom2", "lrt_load_scom2",
@@ -334,7 +341,7 @@ def exported_symbols():
             "lrt_camera_inverse", "lrt_gen_icosphere", "lrt_gen_geodesic", "lrt_sdf_spheres_frame_octree",
             "lrt_sdf_spheres_global_octree", "lrt_sdf_spheres_sbs", "lrt_sdf_spheres_sbs_indexed", "lrt_resolve_dev", "lrt_tonemap_dev",
             "lrt_load_sdf_frame_octree",
-            "lrt_save_sdf_frame_octree", "lrt_load_sdf_sbs", "lrt_save_sdf_sbs", "lrt_load_sdf_multi_octree",
+            "lrt_save_sdf_frame_octree", "lrt_load_sdf_frame_octree_tex", "lrt_save_sdf_frame_octree_tex", "lrt_load_sdf_sbs", "lrt_save_sdf_sbs", "lrt_load_sdf_multi_octree",
             "lrt_save_sdf_multi_octree"]
 
 
@@ -422,6 +429,15 @@ class SceneRT:
         nodes = np.ascontiguousarray(nodes, dtype=FRAME_NODE_DTYPE)
         out = ctypes.c_uint32()
         _check(lib().lrt_add_sdf_frame_octree(self._h, _ptr(nodes), nodes.size, ctypes.byref(out)))
+        return out.value
+
+    def AddGeom_SdfFrameOctreeTex(self, nodes: np.ndarray) -> int:
+        """BVHRT::AddGeom_SdfFrameOctreeTex (BVH2Common_host.cpp:697-731): a frame octree whose nodes
+        carry 16 corner texture coordinates; hits report TYPE_SDF_FRAME_OCTREE_TEX and the
+        coordinates interpolated at the hit point in coords[0..1]."""
+        nodes = np.ascontiguousarray(nodes, dtype=FRAME_TEX_NODE_DTYPE)
+        out = ctypes.c_uint32()
+        _check(lib().lrt_add_sdf_frame_octree_tex(self._h, _ptr(nodes), nodes.size, ctypes.byref(out)))
         return out.value
 
     def AddGeom_GraphicsPrim(self, prim_type: int, points: np.ndarray, color=(1.0, 1.0, 1.0)) -> int:
@@ -860,6 +876,18 @@ def load_sdf_frame_octree(path: str) -> np.ndarray:
 def save_sdf_frame_octree(path: str, nodes: np.ndarray):
     n = np.ascontiguousarray(nodes, dtype=FRAME_NODE_DTYPE)
     _check(lib().lrt_save_sdf_frame_octree(os.fsencode(path), _ptr(n), n.size))
+
+
+def load_sdf_frame_octree_tex(path: str) -> np.ndarray:
+    """load_sdf_frame_octree_tex (sdf/simple/sdf_simple.cpp:166-174): SdfFrameOctreeTexNode array."""
+    b = _Buffer()
+    _check(lib().lrt_load_sdf_frame_octree_tex(os.fsencode(path), ctypes.byref(b)))
+    return _take(b, FRAME_TEX_NODE_DTYPE)
+
+
+def save_sdf_frame_octree_tex(path: str, nodes: np.ndarray):
+    n = np.ascontiguousarray(nodes, dtype=FRAME_TEX_NODE_DTYPE)
+    _check(lib().lrt_save_sdf_frame_octree_tex(os.fsencode(path), _ptr(n), n.size))
 
 
 def load_sdf_grid(path: str) -> np.ndarray:

@@ -282,6 +282,15 @@ lrt_status lrt_add_sdf_frame_octree(lrt_scene* s, const void* nodes, size_t n_no
   });
 }
 
+lrt_status lrt_add_sdf_frame_octree_tex(lrt_scene* s, const void* nodes, size_t n_nodes, uint32_t* out_geom_id) {
+  return guarded([&]() -> lrt_status {
+    if (!s || !nodes) return fail(LRT_ERR_INVALID_ARG, "null argument");
+    const uint32_t id = s->s.add_sdf_frame_octree_tex(static_cast<const lrt::SdfFrameOctreeTexNode*>(nodes), n_nodes);
+    if (out_geom_id) *out_geom_id = id;
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_add_graphics_prims(lrt_scene* s, const void* header, const float* points, size_t n_points,
                                   uint32_t* out_geom_id) {
   return guarded([&]() -> lrt_status {
@@ -1057,7 +1066,7 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
     ARR("geom_tags", c.geomTags) ARR("start_end", c.startEnd) ARR("prim_id_count", c.primIdCount)
     ARR("vert_pos", c.vertPos) ARR("vert_norm", c.vertNorm) ARR("indices", c.indices)
     ARR("prim_indices", c.primIndices) ARR("orig_nodes", c.origNodes) ARR("frame_nodes", c.frameNodes)
-    ARR("frame_roots", c.frameRoots) ARR("sbs_nodes", c.sbsNodes) ARR("sbs_data", c.sbsData)
+    ARR("frame_roots", c.frameRoots) ARR("frame_tex_nodes", c.frameTexNodes) ARR("frame_tex_roots", c.frameTexRoots) ARR("sbs_nodes", c.sbsNodes) ARR("sbs_data", c.sbsData)
     ARR("sbs_data_f", c.sbsDataF) ARR("sbs_roots", c.sbsRoots) ARR("sbs_headers", c.sbsHeaders)
     ARR("multi_nodes", c.multiNodes) ARR("multi_values", c.multiValues) ARR("orig_indices", c.origIndices)
     ARR("svs_nodes", c.svsNodes) ARR("svs_roots", c.svsRoots) ARR("tlas_parent", c.tlasParent)
@@ -1086,7 +1095,8 @@ lrt_status lrt_get_array(lrt_scene* s, const char* name, void* dst, size_t* byte
     if (!src && sz == 0) {
       static const char* known[] = {"nodes", "tlas", "instances", "geoms", "geom_tags", "start_end",
                                     "prim_id_count", "vert_pos", "vert_norm", "indices", "prim_indices",
-                                    "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes", "sbs_data",
+                                    "orig_nodes", "frame_nodes", "frame_roots", "frame_tex_nodes", "frame_tex_roots",
+                                    "sbs_nodes", "sbs_data",
                                     "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "preset", "orig_indices", "svs_nodes", "svs_roots", "tlas_parent", "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points", "coct_data", "coct_geoms", "coct_mods", "dag_nodes", "dag_child_edges", "dag_data_edges", "dag_distances", "dag_geoms", "dag_mods", "rot_add", "scom2_data", "scom2_headers", "scom2_mods",
                                     "sbs_mask"};
       bool ok = false;

@@ -73,6 +73,15 @@ struct SdfFrameOctreeNode {
   float    values[8];
   uint32_t offset;     // children block offset, 0 = leaf
 };
+// SdfFrameOctreeTexNode (sdf/simple/sdf_simple_shared.h:57-63): a frame-octree node with a
+// texture coordinate pair (u, v) per corner, corner j = (x << 2 | y << 1 | z) at [2j, 2j+1], 104 B
+struct SdfFrameOctreeTexNode {
+  float    tex_coords[16];
+  float    values[8];
+  uint32_t offset;       // children block offset, 0 = leaf
+  uint32_t material_id;  // read by the material resolve only
+};
+static_assert(sizeof(SdfFrameOctreeTexNode) == 104, "SdfFrameOctreeTexNode is 104 B");
 // SdfSVSNode (sdf/simple/sdf_simple_shared.h:30-35): one voxel of a sparse voxel set, 16 B
 struct SdfSVSNode {
   uint32_t pos_xy;           // x << 16 | y, position of the voxel in its level
@@ -259,6 +268,7 @@ constexpr uint32_t TYPE_MESH_TRIANGLE    = 0;
 constexpr uint32_t TYPE_SDF_GRID         = 1;
 constexpr uint32_t TYPE_GRAPHICS_PRIM    = 17;
 constexpr uint32_t TYPE_SDF_FRAME_OCTREE = 3;
+constexpr uint32_t TYPE_SDF_FRAME_OCTREE_TEX = 8;
 constexpr uint32_t TYPE_SDF_SVS          = 5;
 constexpr uint32_t TYPE_SDF_SBS          = 6;
 constexpr uint32_t TYPE_SDF_MULTI_OCTREE = 9;

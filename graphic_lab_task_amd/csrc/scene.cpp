@@ -98,8 +98,11 @@ void include4(Box4& b, float4 p) {
 }
 
 // add_border_nodes_rec for frame octrees (BVH2Common_host.cpp:1348-1383): one box per leaf
-// whose corner values change sign; leftOffset keeps the octree node index.
-void add_border_nodes_rec(const SdfFrameOctreeNode* nodes, size_t n_nodes, std::vector<BVHNode>& out, uint32_t idx,
+// whose corner values change sign; leftOffset keeps the octree node index. The textured octree's
+// version (:1392-1427) tests a positive value against -sqrt(3)*d instead of 0, which every
+// positive value passes, so both node types share this walk.
+template <class NodeT>
+void add_border_nodes_rec(const NodeT* nodes, size_t n_nodes, std::vector<BVHNode>& out, uint32_t idx,
                           float3 p, float d) {
   const uint32_t ofs = nodes[idx].offset;
   if (ofs != 0 && size_t(ofs) + 8 > n_nodes) throw std::invalid_argument("frame octree child offset out of range");
@@ -142,7 +145,7 @@ void Scene::clear_geom() {
   geomData.clear(); geomTags.clear(); startEnd.clear();
   vertPos.clear(); vertNorm.clear(); indices.clear(); primIndices.clear(); primIdCount.clear(); origIndices.clear();
   nodePairs.clear(); origNodes.clear();
-  frameNodes.clear(); frameRoots.clear(); svsNodes.clear(); svsRoots.clear();
+  frameNodes.clear(); frameRoots.clear(); frameTexNodes.clear(); frameTexRoots.clear(); svsNodes.clear(); svsRoots.clear();
   sbsNodes.clear(); sbsData.clear(); sbsDataF.clear(); sbsRoots.clear(); sbsHeaders.clear(); sbsFSlice.clear(); sbsDiffChecked.clear();
   multiNodes.clear(); multiValues.clear();
   gridData.clear(); gridOffsets.clear(); gridSizes.clear();
@@ -237,6 +240,30 @@ uint32_t Scene::add_sdf_frame_octree(const SdfFrameOctreeNode* nodes, size_t n) 
   frameRoots.push_back(n_offset);
   frameNodes.insert(frameNodes.end(), nodes, nodes + n);
   for (size_t i = n_offset; i < frameNodes.size(); i++) frameNodes[i].offset += n_offset;
+  origNodes.insert(origNodes.end(), boxes.begin(), boxes.end());
+  return add_aabb(TAG_SDF_NODE, boxes.data(), boxes.size());
+}
+
+// BVHRT::AddGeom_SdfFrameOctreeTex (BVH2Common_host.cpp:697-731): a frame octree whose nodes also
+// carry texture coordinates; traced as the frame octree, hits report TYPE_SDF_FRAME_OCTREE_TEX and
+// the coordinates interpolated at the hit point. Leaf boxes are kept per octree as for frame
+// octrees.
+uint32_t Scene::add_sdf_frame_octree_tex(const SdfFrameOctreeTexNode* nodes, size_t n) {
+  if (n == 0 || n >= (1u << 28)) throw std::invalid_argument("textured frame octree size out of range");
+  std::vector<BVHNode> boxes;
+  add_border_nodes_rec(nodes, n, boxes, 0, make3(0, 0, 0), 1.0f);
+  if (boxes.empty()) throw std::invalid_argument("textured frame octree has no surface leaves");
+  GeomData g{};
+  g.boxMin = make4(-1, -1, -1, 1);
+  g.boxMax = make4(1, 1, 1, 1);
+  g.offset = uint2{uint32_t(frameTexRoots.size()), uint32_t(origNodes.size())};
+  g.bvhOffset = uint32_t(nodePairs.size());
+  g.type = TYPE_SDF_FRAME_OCTREE_TEX;
+  geomData.push_back(g);
+  const uint32_t n_offset = uint32_t(frameTexNodes.size());
+  frameTexRoots.push_back(n_offset);
+  frameTexNodes.insert(frameTexNodes.end(), nodes, nodes + n);
+  for (size_t i = n_offset; i < frameTexNodes.size(); i++) frameTexNodes[i].offset += n_offset;
   origNodes.insert(origNodes.end(), boxes.begin(), boxes.end());
   return add_aabb(TAG_SDF_NODE, boxes.data(), boxes.size());
 }
@@ -761,19 +788,28 @@ void Scene::build_device_layout() {
       triSoup[k] = r;
     }
   }
-  // frame-octree leaf records
+  // frame-octree leaf records (textured octrees: values from their nodes, and the texture
+  // coordinates of each leaf node gathered into frameTex)
   frameLeaves.assign(origNodes.size(), FrameLeaf{});
+  frameTex.clear();
   for (size_t g = 0; g < geomData.size(); ++g) {
-    if (geomTags[g] != TAG_SDF_NODE || geomData[g].type != TYPE_SDF_FRAME_OCTREE) continue;
-    const uint32_t root = frameRoots[geomData[g].offset.x], ob = geomData[g].offset.y;
+    const bool tex = geomData[g].type == TYPE_SDF_FRAME_OCTREE_TEX;
+    if (geomTags[g] != TAG_SDF_NODE || (geomData[g].type != TYPE_SDF_FRAME_OCTREE && !tex)) continue;
+    const uint32_t root = (tex ? frameTexRoots : frameRoots)[geomData[g].offset.x], ob = geomData[g].offset.y;
     const uint32_t nb = startEnd[g].y - startEnd[g].x;
     for (uint32_t k = 0; k < nb; ++k) {
       const BVHNode& on = origNodes[ob + k];
       FrameLeaf& fl = frameLeaves[ob + k];
       fl.boxMin = on.boxMin;
       fl.boxMax = on.boxMax;
-      for (int j = 0; j < 8; ++j) fl.values[j] = frameNodes[on.leftOffset + root].values[j];
+      const float* vals = tex ? frameTexNodes[on.leftOffset + root].values : frameNodes[on.leftOffset + root].values;
+      for (int j = 0; j < 8; ++j) fl.values[j] = vals[j];
       fl.primId = on.leftOffset;
+      if (tex) {
+        fl.tex = uint32_t(frameTex.size() / 4);
+        const float* tc = frameTexNodes[on.leftOffset + root].tex_coords;
+        for (int j = 0; j < 4; ++j) frameTex.push_back(make4(tc[4 * j], tc[4 * j + 1], tc[4 * j + 2], tc[4 * j + 3]));
+      }
     }
   }
   // sparse-voxel-set leaf records: the voxel's box and its 8 byte-quantised values decoded as
@@ -833,7 +869,7 @@ hipError_t Scene::commit() {
 #define UP(name, vec)                                                        \
   e = dev[name].upload(vec.data(), vec.size() * sizeof(vec[0]));             \
   if (e != hipSuccess) return e;
-  UP("nodes", devNodes) UP("tri_soup", triSoup) UP("frame_leaves", frameLeaves) UP("tlas", tlas) UP("instances", instances) UP("geoms", geomData) UP("geom_tags", geomTags)
+  UP("nodes", devNodes) UP("tri_soup", triSoup) UP("frame_leaves", frameLeaves) UP("frame_tex", frameTex) UP("tlas", tlas) UP("instances", instances) UP("geoms", geomData) UP("geom_tags", geomTags)
   UP("start_end", startEnd) UP("prim_id_count", primIdCount) UP("vert_pos", vertPos) UP("vert_norm", vertNorm)
   UP("indices", indices) UP("prim_indices", primIndices) UP("orig_nodes", origNodes) UP("frame_nodes", frameNodes)
   UP("frame_roots", frameRoots) UP("sbs_nodes", sbsNodes) UP("sbs_data", sbsData) UP("sbs_data_f", sbsDataF)
@@ -885,6 +921,7 @@ SceneView Scene::make_view() const {
   v.tlas = (const BVHNode*)P("tlas");
   v.tri_soup = (const float4*)P("tri_soup");
   v.frame_leaves = (const FrameLeaf*)P("frame_leaves");
+  v.frame_tex = (const float4*)P("frame_tex");
   v.instances = (const InstanceData*)P("instances");
   v.geoms = (const GeomData*)P("geoms");
   v.geom_tags = (const uint32_t*)P("geom_tags");

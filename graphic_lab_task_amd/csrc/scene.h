@@ -55,6 +55,8 @@ struct Scene {
   std::vector<BVHNode> origNodes;
   std::vector<SdfFrameOctreeNode> frameNodes;
   std::vector<uint32_t> frameRoots;
+  std::vector<SdfFrameOctreeTexNode> frameTexNodes;   // m_SdfFrameOctreeTexNodes
+  std::vector<uint32_t> frameTexRoots;                // m_SdfFrameOctreeTexRoots
   std::vector<SdfSVSNode> svsNodes;
   std::vector<uint32_t> svsRoots;
   std::vector<SdfSBSNode> sbsNodes;
@@ -92,6 +94,7 @@ struct Scene {
   std::vector<BVHNodePair> devNodes;
   std::vector<float4> triSoup;
   std::vector<FrameLeaf> frameLeaves;
+  std::vector<float4> frameTex;      // 4 float4 per textured frame-octree leaf
   uint32_t cacheGeom = 0xFFFFFFFFu, cachePairs = 0;
 
   bool committed = false;
@@ -162,6 +165,7 @@ struct Scene {
                          const uint32_t* idx, size_t n_idx);
   uint32_t add_aabb(uint32_t tag, const BVHNode* boxes, size_t n);
   uint32_t add_sdf_frame_octree(const SdfFrameOctreeNode* nodes, size_t n);
+  uint32_t add_sdf_frame_octree_tex(const SdfFrameOctreeTexNode* nodes, size_t n);
   uint32_t add_sdf_svs(const SdfSVSNode* nodes, size_t n);
   uint32_t add_sdf_grid(const uint32_t size[3], const float* data);
   uint32_t add_graphics_prims(const GraphicsPrimHeader& header, const float4* points, size_t n_points);

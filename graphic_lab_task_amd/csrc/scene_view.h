@@ -26,13 +26,14 @@ enum LeafKind : uint32_t {
 };
 
 // Frame-octree leaf as the kernels read it: the leaf box, its 8 corner distances and the
-// octree node index, gathered at commit into one 64-byte record per BLAS box.
+// octree node index, gathered at commit into one 64-byte record per BLAS box. Leaves of textured
+// frame octrees also name their 16 texture coordinates in frame_tex (4 float4 from 4 * tex).
 struct alignas(16) FrameLeaf {
   float3   boxMin;
   float3   boxMax;
   float    values[8];
   uint32_t primId;
-  uint32_t _pad;
+  uint32_t tex;
 };
 static_assert(sizeof(FrameLeaf) == 64, "FrameLeaf is 64 B");
 
@@ -60,6 +61,7 @@ struct SceneView {
   const BVHNode*            orig_nodes;   // m_origNodes
   const SdfFrameOctreeNode* frame_nodes;  // m_SdfFrameOctreeNodes
   const uint32_t*           frame_roots;  // m_SdfFrameOctreeRoots
+  const float4*             frame_tex;    // texture coordinates of textured frame-octree leaves, 4 float4 each
   // sparse brick sets
   const SdfSBSNode*   sbs_nodes;    // m_SdfSBSNodes
   const uint32_t*     sbs_data;     // m_SdfSBSData

@@ -105,6 +105,28 @@ lrt_status lrt_save_sdf_frame_octree(const char* path, const void* nodes, size_t
   });
 }
 
+lrt_status lrt_load_sdf_frame_octree_tex(const char* path, lrt_buffer* nodes) {
+  return io_guarded([&]() -> lrt_status {
+    if (!nodes) throw std::runtime_error("null output buffer");
+    File fl(path, "rb");
+    size_t left = size_t(fl.size());
+    std::vector<char> v = counted(fl, sizeof(lrt::SdfFrameOctreeTexNode), left, "textured frame-octree nodes");
+    return to_buf(v, nodes);
+  });
+}
+
+lrt_status lrt_save_sdf_frame_octree_tex(const char* path, const void* nodes, size_t n_nodes) {
+  return io_guarded([&]() -> lrt_status {
+    if (!nodes && n_nodes) throw std::runtime_error("null nodes");
+    check_u32(n_nodes, "textured frame-octree nodes");
+    File fl(path, "wb");
+    const uint32_t n = uint32_t(n_nodes);
+    put(fl, &n, 4);
+    put(fl, nodes, n_nodes * sizeof(lrt::SdfFrameOctreeTexNode));
+    return LRT_OK;
+  });
+}
+
 lrt_status lrt_load_sdf_svs(const char* path, lrt_buffer* nodes) {
   return io_guarded([&]() -> lrt_status {
     if (!nodes) throw std::runtime_error("null output buffer");

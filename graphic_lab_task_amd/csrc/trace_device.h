@@ -415,14 +415,24 @@ __device__ bool local_surface_intersection(const SceneView& s, uint32_t type, fl
   return false;
 }
 
+// Texture coordinate c (0: u, 1: v) of a textured frame-octree hit: the corner pairs weighted
+// trilinearly at the voxel-local hit point dp (BVH2Common.cpp:689-710, same order of operations).
+__device__ __forceinline__ float frame_tex_interp(const float tc[16], float3 dp, int c) {
+  const float x0 = 1 - dp.x, y0 = 1 - dp.y, z0 = 1 - dp.z;
+  return x0 * y0 * z0 * tc[c] + x0 * y0 * dp.z * tc[2 + c] + x0 * dp.y * z0 * tc[4 + c] + x0 * dp.y * dp.z * tc[6 + c] +
+         dp.x * y0 * z0 * tc[8 + c] + dp.x * y0 * dp.z * tc[10 + c] + dp.x * dp.y * z0 * tc[12 + c] +
+         dp.x * dp.y * dp.z * tc[14 + c];
+}
+
 // Frame-octree leaf: BVHRT::OctreeNodeIntersect + RayNodeIntersection (BVH2Common.cpp:727-749,
-// 141-167), reading the leaf's box, corner values and node index from one FrameLeaf record.
+// 141-188), reading the leaf's box, corner values and node index from one FrameLeaf record; a
+// textured octree's hit also takes its texture coordinates (LocalSurfaceIntersection, :689-710).
 __device__ __forceinline__ void frame_node_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir,
                                                      float tNear, uint32_t instId, uint32_t geomId,
                                                      uint32_t bvhNodeId, CRT_Hit& hit) {
   const uint32_t type = g.type;
   // frame octrees and sparse voxel sets (BVH2Common.cpp:141-218): one leaf record per BLAS leaf
-  if (type != TYPE_SDF_FRAME_OCTREE && type != TYPE_SDF_SVS) return;
+  if (type != TYPE_SDF_FRAME_OCTREE && type != TYPE_SDF_SVS && type != TYPE_SDF_FRAME_OCTREE_TEX) return;
   const uint2 go = g.offset;   // y: first leaf record of this geometry
   const float4* q = reinterpret_cast<const float4*>(s.frame_leaves + go.y + bvhNodeId);
   const float4 r0 = q[0], r1 = q[1], r2 = q[2], r3 = q[3];
@@ -438,7 +448,17 @@ __device__ __forceinline__ void frame_node_intersect(const SceneView& s, const G
   const float qFar = (fNearFar.y - fNearFar.x) / d;
   const float qNear = tNear > fNearFar.x ? (tNear - fNearFar.x) / d : 0.0f;
   if (qNear > 0.0f) return;
-  local_surface_intersection(s, type, ray_dir, instId, geomId, values, primId, d, qNear, qFar, fNearFar, start_q, hit);
+  float t_local = 0.0f;
+  if (local_surface_intersection(s, type, ray_dir, instId, geomId, values, primId, d, qNear, qFar, fNearFar, start_q, hit,
+                                 &t_local) &&
+      type == TYPE_SDF_FRAME_OCTREE_TEX) {
+    const float4* tq = s.frame_tex + 4 * size_t(__float_as_uint(r3.w));
+    const float4 a = tq[0], b = tq[1], c = tq[2], e = tq[3];
+    const float tc[16] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, e.x, e.y, e.z, e.w};
+    const float3 dp = start_q + t_local * ray_dir;
+    hit.coords[0] = frame_tex_interp(tc, dp, 0);
+    hit.coords[1] = frame_tex_interp(tc, dp, 1);
+  }
 }
 
 // BVHRT::load_distance_values (BVH2Common.cpp:846-882); returns the minimum value.

@@ -87,6 +87,11 @@ lrt_status lrt_sdf_spheres_sbs_indexed(const float* spheres, size_t n_spheres, u
 lrt_status lrt_load_sdf_frame_octree(const char* path, lrt_buffer* nodes);
 lrt_status lrt_save_sdf_frame_octree(const char* path, const void* nodes, size_t n_nodes);
 
+/* load_sdf_frame_octree_tex / save_sdf_frame_octree_tex (sdf/simple/sdf_simple.cpp:157-174):
+ * u32 count, then count SdfFrameOctreeTexNode (104 B). */
+lrt_status lrt_load_sdf_frame_octree_tex(const char* path, lrt_buffer* nodes);
+lrt_status lrt_save_sdf_frame_octree_tex(const char* path, const void* nodes, size_t n_nodes);
+
 /* load_sdf_grid / save_sdf_grid (sdf/simple/sdf_simple.cpp:20-36): 3 x u32 size, then
  * size.x * size.y * size.z floats (x fastest). */
 lrt_status lrt_load_sdf_grid(const char* path, uint32_t size[3], lrt_buffer* data);

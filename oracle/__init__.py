@@ -34,6 +34,9 @@ Function -> reference -> how it is pinned (tests/test_oracle_cpu.py unless state
       by the reference's own search (restated in numpy); the BLAS boxes by a restatement of
       add_border_nodes_rec; the reference's engine-scene COctreeV3 file by its structure and by
       hit points on the zero set of its decoded distances.
+  oracle_trace_closest  textured frame octrees: BVH2Common.cpp:168-188, 689-710
+      (tests/test_frame_tex_cpu.py) pinned by bit-identical hits with the frame octree of the same
+      nodes and by the closed form of texture coordinates that are a linear map of position.
   oracle_coct_voxel     BVH2Common.cpp:1637-1851   the loaders alone (see above)
   oracle_local_surface  BVH2Common.cpp:345-725 (one voxel)  pinned by float64 root finding on
       the trilinear polynomial along the ray (first sign change), for every intersection mode
@@ -57,7 +60,8 @@ COUNTS_DTYPE = np.dtype([("blas_nodes", "<u4"), ("leaves", "<u4"), ("prims", "<u
                          ("instances", "<u4"), ("sdf_voxels", "<u4")])
 
 _ARRAYS = ["nodes", "tlas", "instances", "geoms", "geom_tags", "start_end", "prim_id_count", "vert_pos",
-           "vert_norm", "indices", "prim_indices", "orig_nodes", "frame_nodes", "frame_roots", "sbs_nodes",
+           "vert_norm", "indices", "prim_indices", "orig_nodes", "frame_nodes", "frame_roots", "frame_tex_nodes",
+           "frame_tex_roots", "sbs_nodes",
            "sbs_data", "sbs_data_f", "sbs_roots", "sbs_headers", "multi_nodes", "multi_values", "svs_nodes", "svs_roots",
            "grid_data", "grid_offsets", "grid_sizes", "gprim_headers", "gprim_points", "coct_data", "coct_geoms",
            "coct_mods", "dag_nodes", "dag_child_edges", "dag_data_edges", "dag_distances", "dag_geoms", "dag_mods",

@@ -95,6 +95,7 @@ struct NodePair { Node left, right; };
 struct Geom { f4 boxMin, boxMax; uint32_t bvhOffset, type; uint32_t offset[2]; uint32_t hasChannels, pad[3]; };
 struct Inst { f4 boxMin, boxMax; uint32_t geomId, pad[7]; float transform[16], transformInv[16], transformInvTransposed[16]; };
 struct FrameNode { float values[8]; uint32_t offset; };
+struct FrameTexNode { float tex_coords[16]; float values[8]; uint32_t offset, material_id; };   // sdf_simple_shared.h:57-63
 struct SBSNode { uint32_t pos_xy, pos_z_lod_size, data_offset, pad; };
 struct SBSHeader { uint32_t brick_size, brick_pad, bytes_per_value, aux_data; };
 struct MultiNode { uint32_t data_offset, children_offset, voxel_count, flags; };   // sdf_multi_shared.h:8-14
@@ -148,6 +149,7 @@ constexpr uint32_t TAG_COCTREE_BRICKED = 2, TAG_SDF_DAG = 18, TYPE_SDF_DAG = 25,
 constexpr uint32_t GP_POINT = 0, GP_LINE = 1, GP_SEGMENT = 2, GP_SEGMENT_DIR = 3, GP_BOX = 4, GP_POINT_COLOR = 5,
                    GP_LINE_COLOR = 6, GP_SEGMENT_COLOR = 7, GP_SEGMENT_DIR_COLOR = 8, GP_BOX_COLOR = 9;
 constexpr uint32_t TYPE_SDF_SVS = 5;
+constexpr uint32_t TYPE_SDF_FRAME_OCTREE_TEX = 8;
 constexpr uint32_t TYPE_MESH_TRIANGLE = 0, TYPE_SDF_FRAME_OCTREE = 3, SH_TYPE = 24, GEOM_ID_MASK = 0x00FFFFFFu;
 constexpr uint32_t LAYOUT_DX_UV16 = 2u << 24, LAYOUT_DX_RGB8 = 3u << 24, LAYOUT_ID32F = 4u << 24, LAYOUT_ID32F_IN = 5u << 24;
 
@@ -171,6 +173,8 @@ struct OracleScene {
   const Node* orig_nodes;
   const FrameNode* frame_nodes;
   const uint32_t* frame_roots;
+  const FrameTexNode* frame_tex_nodes;   // m_SdfFrameOctreeTexNodes
+  const uint32_t* frame_tex_roots;
   const SBSNode* sbs_nodes;
   const uint32_t* sbs_data;
   const float* sbs_data_f;
@@ -230,6 +234,7 @@ struct Tracer {
   OracleCounts* cnt;
   VoxelRec* rec = nullptr;     // records the voxel of each brick-set hit (test support)
   float last_t_local = 0.0f;   // local hit parameter of the latest LocalSurfaceIntersection hit
+  const FrameTexNode* tex_node = nullptr;   // the textured frame-octree node being intersected
 
   // support_fast_scom_traversal (BVH2Common_host.cpp:1436-1451): one plain SCom2 tree, LAMBERT_NO_TEX
   bool scom2_fast() const {
@@ -309,8 +314,8 @@ struct Tracer {
            (dp.x) * (dp.y) * (1 - dp.z) * v[6] + (dp.x) * (dp.y) * (dp.z) * v[7];
   }
 
-  // BVH2Common.cpp:345-725 (no cutting plane, no tex types); the compact types (DAG, SCom2,
-  // :340-343) widen EPS on surface nodes
+  // BVH2Common.cpp:345-725 (no cutting plane); textured frame octrees take their texture
+  // coordinates (:689-710); the compact types (DAG, SCom2, :340-343) widen EPS on surface nodes
   void LocalSurfaceIntersection(uint32_t type, f3 ray_dir, uint32_t instId, uint32_t geomId, const float values[8],
                                 uint32_t primId, float d, float qNear, float qFar, f2 fNearFar, f3 start_q, Hit* pHit,
                                 bool is_surface_node = false) {
@@ -476,6 +481,19 @@ struct Tracer {
       pHit->coords[2] = en.x;
       pHit->coords[3] = en.y;
       if (s.preset.render_mode == 10) pHit->primId = iter;
+      // textured frame octrees: corner texture coordinates at the hit point (:689-710)
+      if (type == TYPE_SDF_FRAME_OCTREE_TEX && tex_node) {
+        const f3 dp = start_q + t * ray_dir;
+        const float* tc = tex_node->tex_coords;
+        pHit->coords[0] = (1 - dp.x) * (1 - dp.y) * (1 - dp.z) * tc[0] + (1 - dp.x) * (1 - dp.y) * (dp.z) * tc[2] +
+                          (1 - dp.x) * (dp.y) * (1 - dp.z) * tc[4] + (1 - dp.x) * (dp.y) * (dp.z) * tc[6] +
+                          (dp.x) * (1 - dp.y) * (1 - dp.z) * tc[8] + (dp.x) * (1 - dp.y) * (dp.z) * tc[10] +
+                          (dp.x) * (dp.y) * (1 - dp.z) * tc[12] + (dp.x) * (dp.y) * (dp.z) * tc[14];
+        pHit->coords[1] = (1 - dp.x) * (1 - dp.y) * (1 - dp.z) * tc[1] + (1 - dp.x) * (1 - dp.y) * (dp.z) * tc[3] +
+                          (1 - dp.x) * (dp.y) * (1 - dp.z) * tc[5] + (1 - dp.x) * (dp.y) * (dp.z) * tc[7] +
+                          (dp.x) * (1 - dp.y) * (1 - dp.z) * tc[9] + (dp.x) * (1 - dp.y) * (dp.z) * tc[11] +
+                          (dp.x) * (dp.y) * (1 - dp.z) * tc[13] + (dp.x) * (dp.y) * (dp.z) * tc[15];
+      }
       // channel render modes: interpolation quotients instead of texture coordinates (:714-723)
       if ((type == 9 || type == TYPE_SDF_DAG || type == TYPE_SCOM2) &&
           (s.preset.render_mode == 23 || s.preset.render_mode == 27)) {
@@ -492,21 +510,29 @@ struct Tracer {
   }
 
   // BVH2Common.cpp:727-749 + 141-167. Leaf boxes of geometry g start at geoms[g].offset[1].
-  // SVS: BVH2Common.cpp:190-215 (box from the voxel's position, byte values decoded)
+  // Textured frame octrees: :168-188, their nodes in frame_tex_nodes. SVS: BVH2Common.cpp:190-215
+  // (box from the voxel's position, byte values decoded)
   void OctreeNodeIntersect(uint32_t type, f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
                            uint32_t a_start, Hit* pHit) {
-    if (type != TYPE_SDF_FRAME_OCTREE && type != TYPE_SDF_SVS) return;
+    if (type != TYPE_SDF_FRAME_OCTREE && type != TYPE_SDF_SVS && type != TYPE_SDF_FRAME_OCTREE_TEX) return;
     const uint32_t sdfId = s.geoms[geomId].offset[0];
     uint32_t primId = 0;
     f3 min_pos, max_pos;
     float values[8];
-    if (type == TYPE_SDF_FRAME_OCTREE) {
+    tex_node = nullptr;
+    if (type == TYPE_SDF_FRAME_OCTREE || type == TYPE_SDF_FRAME_OCTREE_TEX) {
       const Node& on = s.orig_nodes[s.geoms[geomId].offset[1] + a_start];
       primId = on.leftOffset;
-      const uint32_t nodeId = primId + s.frame_roots[sdfId];
       min_pos = on.boxMin;
       max_pos = on.boxMax;
-      for (int i = 0; i < 8; i++) values[i] = s.frame_nodes[nodeId].values[i];
+      if (type == TYPE_SDF_FRAME_OCTREE) {
+        const uint32_t nodeId = primId + s.frame_roots[sdfId];
+        for (int i = 0; i < 8; i++) values[i] = s.frame_nodes[nodeId].values[i];
+      } else {
+        const uint32_t nodeId = primId + s.frame_tex_roots[sdfId];
+        tex_node = &s.frame_tex_nodes[nodeId];
+        for (int i = 0; i < 8; i++) values[i] = tex_node->values[i];
+      }
     } else {
       primId = a_start;
       const SVSNode& nd = s.svs_nodes[primId + s.svs_roots[sdfId]];
