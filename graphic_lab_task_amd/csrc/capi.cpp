@@ -93,14 +93,19 @@ lrt::TraceArgs trace_args(lrt_scene* s, const float* pos, const float* dir, lrt_
   return a;
 }
 
-// SCom2 queries this library does not build: the channel render modes of trees with channels
-// (their hit-point quotients are rotated by float matrices, BVH2Common.cpp:2697-2711). Returns the
-// reason, or null.
-const char* unsupported_scom2(const lrt::Scene& sc) {
+// Queries this library does not answer: the channel render modes of SCom2 trees with channels
+// (their hit-point quotients are rotated by float matrices, BVH2Common.cpp:2697-2711, not built),
+// and tricubic interpolation of a brick set without padding (the 4^3 samples start one voxel
+// before the brick, BVH2Common.cpp:790-844: the reference reads outside the brick there).
+// Returns the reason, or null.
+const char* unsupported_query(const lrt::Scene& sc) {
   const uint32_t rm = sc.preset.render_mode;
   for (const lrt::SCom2Header& h : sc.scom2Headers)
     if (h.has_channels && (rm == lrt::MULTI_RENDER_MODE_CHANNEL || rm == lrt::MULTI_RENDER_MODE_CHANNEL_LAMBERT))
       return "channel render modes of SCom2 trees with channels are not built";
+  if (sc.preset.interpolation_mode == lrt::INTERPOLATION_MODE_TRICUBIC)
+    for (const lrt::SdfSBSHeader& h : sc.sbsHeaders)
+      if (h.brick_pad < 1) return "tricubic interpolation needs brick sets with brick_pad >= 1";
   return nullptr;
 }
 
@@ -108,7 +113,7 @@ lrt_status trace_common(lrt_scene* s, const float* pos, const float* dir, lrt_hi
                         void* stream, uint32_t* counts = nullptr) {
   if (!s) return fail(LRT_ERR_INVALID_ARG, "null scene");
   if (!s->s.committed) return fail(LRT_ERR_NOT_COMMITTED, "scene is not committed");
-  if (const char* why = unsupported_scom2(s->s)) return fail(LRT_ERR_UNSUPPORTED, why);
+  if (const char* why = unsupported_query(s->s)) return fail(LRT_ERR_UNSUPPORTED, why);
   if (n == 0) return LRT_OK;
   if (!pos || !dir || (!hits && !any)) return fail(LRT_ERR_INVALID_ARG, "null ray or hit buffer");
   if (n > 0xFFFFFFFFull) return fail(LRT_ERR_INVALID_ARG, "too many rays in one call");
@@ -135,6 +140,8 @@ lrt_status diff_geometry(lrt_scene* s, uint32_t geom_id, uint32_t* f_offset, uin
   // skipped silently (an occluding mesh rendered invisible, its shadowed gradients wrong)
   if (sc.leaf_kinds() != lrt::LK_SBS)
     return fail(LRT_ERR_UNSUPPORTED, "differentiable calls need a scene whose geometries are all sparse brick sets");
+  if (sc.preset.interpolation_mode != lrt::INTERPOLATION_MODE_TRILINEAR)
+    return fail(LRT_ERR_UNSUPPORTED, "differentiable calls interpolate trilinearly (interpolation_mode 0)");
   const uint32_t sdf = sc.geomData[geom_id].offset.x;
   const lrt::uint2 sl = sc.sbsFSlice[sdf];
   if (sl.y == 0 || sl.y % 4 != 0)
@@ -450,8 +457,8 @@ lrt_status lrt_set_preset(lrt_scene* s, const void* preset) {
   return guarded([&]() -> lrt_status {
     if (!s || !preset) return fail(LRT_ERR_INVALID_ARG, "null argument");
     const lrt::MultiRenderPreset& p = *static_cast<const lrt::MultiRenderPreset*>(preset);
-    if (p.interpolation_mode != lrt::INTERPOLATION_MODE_TRILINEAR)
-      return fail(LRT_ERR_UNSUPPORTED, "only trilinear interpolation (interpolation_mode 0) is built");
+    if (p.interpolation_mode != lrt::INTERPOLATION_MODE_TRILINEAR && p.interpolation_mode != lrt::INTERPOLATION_MODE_TRICUBIC)
+      return fail(LRT_ERR_UNSUPPORTED, "interpolation_mode must be 0 (trilinear) or 1 (tricubic)");
     s->s.set_preset(p);
     return LRT_OK;
   });
@@ -477,7 +484,7 @@ lrt_status lrt_trace_closest_noalloc_dev(lrt_scene* s, const float* pos_near, co
     hipError_t e = hipGetDevice(&device);
     if (e != hipSuccess) return hip_fail(e, "current device");
     if (device != s->s.device) return fail(LRT_ERR_INVALID_ARG, "the current device is not the scene's device");
-    if (const char* why = unsupported_scom2(s->s)) return fail(LRT_ERR_UNSUPPORTED, why);
+    if (const char* why = unsupported_query(s->s)) return fail(LRT_ERR_UNSUPPORTED, why);
     const lrt::TraceArgs a = trace_args(s, pos_near, dir_far, hits, nullptr, n, nullptr);
     e = lrt::launch_trace_noalloc(s->s.make_view(), a, device, reinterpret_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "allocation-free trace launch");

@@ -341,6 +341,7 @@ constexpr uint32_t NORMAL_MODE_GEOMETRY     = 0;
 constexpr uint32_t NORMAL_MODE_VERTEX       = 1;
 constexpr uint32_t NORMAL_MODE_SDF_SMOOTHED = 2;
 constexpr uint32_t INTERPOLATION_MODE_TRILINEAR = 0;
+constexpr uint32_t INTERPOLATION_MODE_TRICUBIC  = 1;
 constexpr uint32_t REPRESENTATION_MODE_SURFACE  = 0;
 constexpr uint32_t REPRESENTATION_MODE_VOLUME   = 1;
 

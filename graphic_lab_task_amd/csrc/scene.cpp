@@ -974,6 +974,9 @@ SceneView Scene::make_view() const {
     for (const SdfSBSHeader& h : sbsHeaders)
       if (h.aux_data == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN) v.leaf_kinds |= LK_SBS_SMOOTH;
   }
+  // tricubic interpolation (BVH2Common.cpp:933-955, 1478-1499) is read by brick sets and grids only
+  if ((v.leaf_kinds & (LK_SBS | LK_GRID)) && preset.interpolation_mode == INTERPOLATION_MODE_TRICUBIC)
+    v.leaf_kinds |= LK_TRICUBIC;
   v.n_instances = uint32_t(instances.size());
   v.cache_geom = cacheGeom;
   v.cache_pairs = cachePairs;

@@ -23,6 +23,9 @@ enum LeafKind : uint32_t {
   LK_COCT  = 1u << 8,   // TAG_COCTREE_BRICKED (compact octree COctreeV3)
   LK_DAG   = 1u << 9,   // TAG_SDF_DAG (octree DAG with rotated shared nodes and bricks)
   LK_SCOM2 = 1u << 10,  // TAG_SCOM2 (packed SCom2 tree)
+  // Walk option set in SceneView::leaf_kinds when the preset interpolates tricubically and the
+  // scene holds brick sets or grids: selects the kernels with the 4^3-sample interpolant.
+  LK_TRICUBIC = 1u << 11,
 };
 
 // Frame-octree leaf as the kernels read it: the leaf box, its 8 corner distances and the

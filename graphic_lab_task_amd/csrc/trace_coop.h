@@ -114,11 +114,11 @@ __device__ __forceinline__ void coop_leaf(const SceneView& s, float3 ray_pos, fl
   } else if ((KINDS & LK_FRAME) && tag == TAG_SDF_NODE) {
     frame_node_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, h);
   } else if ((KINDS & LK_SBS) && tag == TAG_SDF_BRICK) {
-    sbs_brick_intersect<(KINDS & LK_SBS_SMOOTH) != 0>(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, h);
+    sbs_brick_intersect<(KINDS & LK_SBS_SMOOTH) != 0, (KINDS & LK_TRICUBIC) != 0>(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, h);
   } else if ((KINDS & LK_MULTI) && tag == TAG_SDF_OCTREE) {
     multi_octree_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, h);
   } else if ((KINDS & LK_GRID) && tag == TAG_SDF_GRID) {
-    sdf_grid_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, h);
+    sdf_grid_intersect<(KINDS & LK_TRICUBIC) != 0>(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, h);
   } else if ((KINDS & LK_GPRIM) && tag == TAG_GRAPHICS_PRIM) {
     gprim_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, h);
   } else if ((KINDS & LK_COCT) && tag == TAG_COCTREE_BRICKED) {

@@ -569,8 +569,123 @@ __device__ __forceinline__ bool sbs_voxel_occupied(const SceneView& s, const Sdf
 // The voxel whose surface produced a brick-set hit: the value indices of its 8 corners (into
 // the scene's values_f, indexed layouts), the entry point in voxel units, and the hit
 // parameter along the ray direction from there.
-// BVHRT::eval_distance_sdf_grid (BVH2Common.cpp:1442-1478), trilinear: the grid spans the unit
-// cube; outside the last cell the nearest sample is returned.
+// tricubic_spline (BVH2Common.cpp:1526-1529): the Catmull-Rom cubic through p0..p3, at x in [0, 1]
+// between p1 and p2 (same order of operations).
+__device__ __forceinline__ float tricubic_spline(float p0, float p1, float p2, float p3, float x) {
+  return p1 + 0.5f * x * (p2 - p0 + x * (2.0f * p0 - 5.0f * p1 + 4.0f * p2 - p3 + x * (3.0f * (p1 - p2) + p3 - p0)));
+}
+
+// BVHRT::tricubicInterpolation (BVH2Common.cpp:1531-1569): g[16 z + 4 y + x] holds the 4^3 samples
+// around the cell, dp the position in the cell; splines along x, then y, then z.
+__device__ __forceinline__ float tricubic_interp(const float g[64], float3 dp) {
+  float yz[16];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      yz[4 * j + k] = tricubic_spline(g[16 * k + 4 * j], g[16 * k + 4 * j + 1], g[16 * k + 4 * j + 2], g[16 * k + 4 * j + 3],
+                                      dp.x);
+  float z4[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) z4[k] = tricubic_spline(yz[k], yz[4 + k], yz[8 + k], yz[12 + k], dp.y);
+  return tricubic_spline(z4[0], z4[1], z4[2], z4[3], dp.z);
+}
+
+// BVHRT::load_tricubic_distance_values (BVH2Common.cpp:790-844): the 4^3 values from one voxel
+// before the voxel's corner to two after, at 16 z + 4 y + x; returns their minimum. Needs
+// brick_pad >= 1 (trace calls refuse tricubic brick sets without padding).
+__device__ __forceinline__ float sbs_load_tricubic(const SceneView& s, const SdfSBSHeader& h, uint32_t nodeId,
+                                                   float3 voxelPos, uint32_t v_size, float sz_inv, float values[64]) {
+  float vmin = 1e6f;
+  const uint32_t v_off = s.sbs_nodes[nodeId].data_offset;
+  const uint32_t bx = uint32_t(voxelPos.x) + h.brick_pad - 1, by = uint32_t(voxelPos.y) + h.brick_pad - 1,
+                 bz = uint32_t(voxelPos.z) + h.brick_pad - 1;
+  const bool indexed = h.aux_data == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F || h.aux_data == SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN;
+  const uint32_t vals_per_int = 4 / h.bytes_per_value;
+  const uint32_t bits = 8 * h.bytes_per_value;
+  const uint32_t max_val = h.bytes_per_value == 4 ? 0xFFFFFFFFu : ((1u << bits) - 1);
+  const float d_max = 1.73205081f * sz_inv;
+  const float mult = 2 * d_max / float(max_val);
+  for (uint32_t x = 0; x < 4; x++)
+    for (uint32_t y = 0; y < 4; y++)
+#pragma unroll
+      for (uint32_t z = 0; z < 4; z++) {
+        const uint32_t vId = (bx + x) * v_size * v_size + (by + y) * v_size + (bz + z);
+        float v;
+        if (indexed) {
+          v = s.sbs_data_f[s.sbs_data[v_off + vId]];
+        } else {
+          const uint32_t word = s.sbs_data[v_off + vId / vals_per_int];
+          const uint32_t sh = bits * (vId % vals_per_int);
+          v = -d_max + mult * float((sh < 32 ? (word >> sh) : 0u) & max_val);
+        }
+        values[16 * z + 4 * y + x] = v;
+        vmin = fmin_ref(vmin, v);
+      }
+  return vmin;
+}
+
+// BVHRT::TricubicLocalSurfaceIntersection (BVH2Common.cpp:220-335): the start sample, then sphere
+// tracing in ST mode (other modes hit only where the ray starts inside, or at once in BBOX mode);
+// central-difference normal of the interpolant, turned by the instance's inverse transpose and
+// then signed. Returns true when the hit record was updated.
+__device__ bool tricubic_local_surface(const SceneView& s, uint32_t type, float3 ray_dir, uint32_t instId,
+                                       uint32_t geomId, const float values[64], uint32_t primId, float d, float qNear,
+                                       float qFar, float2 fNearFar, float3 start_q, CRT_Hit& hit) {
+  const float EPS = 1e-6f;
+  const float d_inv = 1.0f / d;
+  float t = qNear;
+  bool hitp = false;
+  unsigned iter = 0;
+  float start_sign = 1;
+  float start_dist = tricubic_interp(values, start_q + t * ray_dir);
+  if (s.preset.representation_mode == REPRESENTATION_MODE_SURFACE) start_sign = sign_ref(start_dist);
+  start_dist *= start_sign;
+  if (start_dist <= EPS || s.preset.sdf_node_intersect == SDF_OCTREE_NODE_INTERSECT_BBOX) {
+    hitp = true;
+  } else if (s.preset.sdf_node_intersect == SDF_OCTREE_NODE_INTERSECT_ST) {
+    const unsigned ST_max_iters = 256;
+    float dist = start_dist;
+    while (t < qFar && dist > EPS && iter < ST_max_iters) {
+      t += dist * d_inv;
+      dist = tricubic_interp(values, start_q + t * ray_dir);
+      iter++;
+    }
+    hitp = (dist <= EPS);
+  }
+  const float tReal = fNearFar.x + d * t;
+  if (!(t <= qFar && hitp && tReal < hit.t)) return false;
+  float3 norm = make3(0, 0, 1);
+  if (preset_needs_normal(s.preset.render_mode)) {
+    const float3 p0 = start_q + t * ray_dir;
+    const float h = 0.001f;
+    const float ddx = (tricubic_interp(values, make3(p0.x + h, p0.y, p0.z)) -
+                       tricubic_interp(values, make3(p0.x + -h, p0.y, p0.z))) / (2 * h);
+    const float ddy = (tricubic_interp(values, make3(p0.x, p0.y + h, p0.z)) -
+                       tricubic_interp(values, make3(p0.x, p0.y + -h, p0.z))) / (2 * h);
+    const float ddz = (tricubic_interp(values, make3(p0.x, p0.y, p0.z + h)) -
+                       tricubic_interp(values, make3(p0.x, p0.y, p0.z + -h))) / (2 * h);
+    float m[16];
+    load_mat(s.instances[instId].transformInvTransposed, m);
+    norm = start_sign * normalize3(matmul4x3(m, make3(ddx, ddy, ddz)));
+  }
+  const float2 en = encode_normal(norm);
+  hit.t = tReal;
+  hit.primId = primId;
+  hit.instId = instId;
+  hit.geomId = geomId | (type << SH_TYPE);
+  hit.coords[0] = 0;
+  hit.coords[1] = 0;
+  hit.coords[2] = en.x;
+  hit.coords[3] = en.y;
+  if (s.preset.render_mode == MULTI_RENDER_MODE_ST_ITERATIONS) hit.primId = iter;
+  return true;
+}
+
+// BVHRT::eval_distance_sdf_grid (BVH2Common.cpp:1442-1520): the grid spans the unit cube;
+// trilinear, or with TRICUBIC the 4^3 samples around interior cells (trilinear next to the faces);
+// outside the last cell the nearest sample is returned.
+template <bool TRICUBIC = false>
 __device__ __forceinline__ float sdf_grid_eval(const SceneView& s, uint32_t grid_id, float3 pos) {
   const uint32_t off = s.grid_offsets[grid_id];
   const uint32_t sx = s.grid_sizes[3 * grid_id], sy = s.grid_sizes[3 * grid_id + 1], sz = s.grid_sizes[3 * grid_id + 2];
@@ -581,7 +696,15 @@ __device__ __forceinline__ float sdf_grid_eval(const SceneView& s, uint32_t grid
   const float3 dp = vox_f - make3(float(ux), float(uy), float(uz));
   const float* data = s.grid_data + off;
   float res = 0.0f;
-  if (ux < sx - 1 && uy < sy - 1 && uz < sz - 1) {
+  if (TRICUBIC && ux < sx - 2 && uy < sy - 2 && uz < sz - 2 && ux > 0 && uy > 0 && uz > 0) {
+    float g[64];
+    for (uint32_t x = 0; x < 4; ++x)
+      for (uint32_t y = 0; y < 4; ++y)
+#pragma unroll
+        for (uint32_t z = 0; z < 4; ++z)
+          g[16 * z + 4 * y + x] = data[(uz - 1 + z) * sx * sy + (uy - 1 + y) * sx + (ux - 1 + x)];
+    res = tricubic_interp(g, dp);
+  } else if (ux < sx - 1 && uy < sy - 1 && uz < sz - 1) {
     for (uint32_t i = 0; i < 2; i++)
       for (uint32_t j = 0; j < 2; j++)
         for (uint32_t k = 0; k < 2; k++) {
@@ -612,6 +735,8 @@ __device__ __forceinline__ float2 box_intersects_ref(float3 min_pos, float3 max_
 
 // BVHRT::IntersectSDFGrid (BVH2Common.cpp:1114-1182): sphere tracing along the normalised
 // direction from the grid box's entry (at most 1000 steps, EPS 1e-5), central-difference normal.
+// TRICUBIC: the preset's tricubic interpolation.
+template <bool TRICUBIC = false>
 __device__ __forceinline__ void sdf_grid_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir,
                                                    float tNear, uint32_t instId, uint32_t geomId, CRT_Hit& hit) {
   const uint32_t sdfId = g.offset.x;
@@ -623,10 +748,10 @@ __device__ __forceinline__ void sdf_grid_intersect(const SceneView& s, const Geo
   const float tFar = tt.y;
   if (t > tFar) return;
   uint32_t iter = 0;
-  float d = sdf_grid_eval(s, sdfId, ray_pos + t * dir);
+  float d = sdf_grid_eval<TRICUBIC>(s, sdfId, ray_pos + t * dir);
   while (iter < 1000 && d > EPS && t < tFar) {
     t += d + EPS;
-    d = sdf_grid_eval(s, sdfId, ray_pos + t * dir);
+    d = sdf_grid_eval<TRICUBIC>(s, sdfId, ray_pos + t * dir);
     iter++;
   }
   if (d > EPS) return;
@@ -635,9 +760,9 @@ __device__ __forceinline__ void sdf_grid_intersect(const SceneView& s, const Geo
     float3 norm = make3(0, 0, 1);
     if (preset_needs_normal(s.preset.render_mode)) {
       const float h = 0.001f;
-      const float ddx = (sdf_grid_eval(s, sdfId, p0 + make3(h, 0, 0)) - sdf_grid_eval(s, sdfId, p0 + make3(-h, 0, 0))) / (2 * h);
-      const float ddy = (sdf_grid_eval(s, sdfId, p0 + make3(0, h, 0)) - sdf_grid_eval(s, sdfId, p0 + make3(0, -h, 0))) / (2 * h);
-      const float ddz = (sdf_grid_eval(s, sdfId, p0 + make3(0, 0, h)) - sdf_grid_eval(s, sdfId, p0 + make3(0, 0, -h))) / (2 * h);
+      const float ddx = (sdf_grid_eval<TRICUBIC>(s, sdfId, p0 + make3(h, 0, 0)) - sdf_grid_eval<TRICUBIC>(s, sdfId, p0 + make3(-h, 0, 0))) / (2 * h);
+      const float ddy = (sdf_grid_eval<TRICUBIC>(s, sdfId, p0 + make3(0, h, 0)) - sdf_grid_eval<TRICUBIC>(s, sdfId, p0 + make3(0, -h, 0))) / (2 * h);
+      const float ddz = (sdf_grid_eval<TRICUBIC>(s, sdfId, p0 + make3(0, 0, h)) - sdf_grid_eval<TRICUBIC>(s, sdfId, p0 + make3(0, 0, -h))) / (2 * h);
       norm = normalize3(make3(ddx, ddy, ddz));
     }
     float m[16];
@@ -892,7 +1017,9 @@ struct SbsVoxelRec {
 // Returns the number of voxels whose values were loaded (all voxels stepped without an
 // occupancy mask). `rec` (indexed layouts only) records the hit voxel. SMOOTH compiles in the
 // smoothed normals of the neighbour-index layout (applied when the preset asks for them).
-template <bool SMOOTH = false>
+// TRICUBIC takes the preset's tricubic interpolation instead: each voxel's 4^3 neighbourhood and
+// the tricubic surface step; the occupancy mask (which knows only the 8 corners) is not used.
+template <bool SMOOTH = false, bool TRICUBIC = false>
 __device__ uint32_t sbs_brick_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir, float tNear,
                                         uint32_t instId, uint32_t geomId, uint32_t bvhNodeId, CRT_Hit& hit,
                                         SbsVoxelRec* rec = nullptr) {
@@ -918,7 +1045,7 @@ __device__ uint32_t sbs_brick_intersect(const SceneView& s, const GeomRef& g, fl
   const float old_t = hit.t;
   const float scale = 0.5f * sz * bs;
   const uint32_t* bmask = nullptr;
-  if (s.sbs_mask) {
+  if (!TRICUBIC && s.sbs_mask) {
     const uint32_t bs_u = header.brick_size;
     bmask = s.sbs_mask + s.sbs_mask_base[sdfId] + primId * (1u + (bs_u * bs_u * bs_u + 31u) / 32u);
     if (bmask[0] == 0u) return 0;   // no occupied voxel: the march could not change the hit
@@ -937,6 +1064,19 @@ __device__ uint32_t sbs_brick_intersect(const SceneView& s, const GeomRef& g, fl
     const float3 min_pos = brick_min_pos + d * voxelPos;
     const float3 max_pos = min_pos + d * make3(1, 1, 1);
     const float2 fNearFar = ray_box2(ray_pos, inv, min_pos, max_pos);
+    if (TRICUBIC) {
+      float tv[64];
+      loaded++;
+      const float vmin = sbs_load_tricubic(s, header, nodeId, voxelPos, v_size, sz_inv, tv);
+      if (tNear < fNearFar.x && vmin <= 0.0f) {
+        const float3 start_pos = ray_pos + fNearFar.x * ray_dir;
+        const float3 start_q = (start_pos - min_pos) * scale;
+        const float qFar = (fNearFar.y - fNearFar.x) * scale;
+        tricubic_local_surface(s, type, ray_dir, instId, geomId, tv, primId, d, 0.0f, qFar, fNearFar, start_q, hit);
+      }
+      brick_fNearFar.x += fmax_ref(0.0f, fNearFar.y - brick_fNearFar.x) + 1e-6f;
+      continue;
+    }
     bool occupied = true;
     if (bmask) {
       const uint32_t vlin = (uint32_t(voxelPos.x) * header.brick_size + uint32_t(voxelPos.y)) * header.brick_size +

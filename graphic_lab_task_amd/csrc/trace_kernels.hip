@@ -74,9 +74,11 @@ __global__ void k_eye_rays(EyeRayParams prm, const float2* __restrict__ jitter, 
 }
 
 hipError_t launch_trace(const SceneView& s, const TraceArgs& a, hipStream_t stream) {
-  const uint32_t k = s.leaf_kinds & ~LK_SBS_SMOOTH;
+  const uint32_t k = s.leaf_kinds & ~(LK_SBS_SMOOTH | LK_TRICUBIC);
   hipError_t e;
-  if (s.leaf_kinds & LK_SBS_SMOOTH)
+  if (s.leaf_kinds & LK_TRICUBIC)
+    e = k == LK_SBS ? launch_trace_sbs_tricubic(s, a, stream) : launch_trace_mixed_tricubic(s, a, stream);
+  else if (s.leaf_kinds & LK_SBS_SMOOTH)
     e = k == LK_SBS ? launch_trace_sbs_smooth(s, a, stream) : launch_trace_mixed_smooth(s, a, stream);
   else if (k == LK_TRI)
     e = a.postpone_tri && !a.counts ? launch_trace_tri_postponed(s, a, stream) : launch_trace_tri(s, a, stream);

@@ -340,5 +340,7 @@ hipError_t launch_trace_dag(const SceneView& s, const TraceArgs& a, hipStream_t 
 hipError_t launch_trace_scom2(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_trace_sbs_smooth(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_trace_mixed_smooth(const SceneView& s, const TraceArgs& a, hipStream_t stream);
+hipError_t launch_trace_sbs_tricubic(const SceneView& s, const TraceArgs& a, hipStream_t stream);
+hipError_t launch_trace_mixed_tricubic(const SceneView& s, const TraceArgs& a, hipStream_t stream);
 
 }  // namespace lrt

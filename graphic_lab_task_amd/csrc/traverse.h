@@ -44,12 +44,12 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
     } else if ((KINDS & LK_FRAME) && tag == TAG_SDF_NODE) {
       frame_node_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
     } else if ((KINDS & LK_SBS) && tag == TAG_SDF_BRICK) {
-      cnt.sdf_voxels += sbs_brick_intersect<(KINDS & LK_SBS_SMOOTH) != 0>(s, g, ray_pos, ray_dir, tNearSdf, instId,
+      cnt.sdf_voxels += sbs_brick_intersect<(KINDS & LK_SBS_SMOOTH) != 0, (KINDS & LK_TRICUBIC) != 0>(s, g, ray_pos, ray_dir, tNearSdf, instId,
                                                                           geomId, a_start, hit, rec);
     } else if ((KINDS & LK_MULTI) && tag == TAG_SDF_OCTREE) {
       cnt.sdf_voxels += multi_octree_intersect<BLOCK>(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, hit, ot_col);
     } else if ((KINDS & LK_GRID) && tag == TAG_SDF_GRID) {
-      sdf_grid_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, hit);
+      sdf_grid_intersect<(KINDS & LK_TRICUBIC) != 0>(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, hit);
     } else if ((KINDS & LK_GPRIM) && tag == TAG_GRAPHICS_PRIM) {
       gprim_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
     } else if ((KINDS & LK_COCT) && tag == TAG_COCTREE_BRICKED) {

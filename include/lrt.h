@@ -186,8 +186,11 @@ typedef struct lrt_commit_info {
 lrt_status lrt_get_commit_info(lrt_scene* s, lrt_commit_info* out);
 
 /* Replaces BVHRT::SetPreset (BVH2Common_host.cpp:1437-1454). preset: MultiRenderPreset
- * (core/render_settings.h:116-136, 60 B). Takes effect on the next trace call. Only trilinear
- * interpolation is built: another interpolation_mode fails with LRT_ERR_UNSUPPORTED. */
+ * (core/render_settings.h:116-136, 60 B). Takes effect on the next trace call. interpolation_mode
+ * 0 (trilinear) or 1 (tricubic, read by brick sets and dense grids as in the reference); another
+ * value fails with LRT_ERR_UNSUPPORTED. Trace calls refuse tricubic interpolation of a brick set
+ * with brick_pad 0 (LRT_ERR_UNSUPPORTED: the reference reads outside the brick there), and the
+ * differentiable calls refuse it (they are trilinear). */
 lrt_status lrt_set_preset(lrt_scene* s, const void* preset);
 
 /* Batched RayQuery_NearestHit (BVH2Common.cpp:3322-3392) over device buffers:

@@ -37,6 +37,10 @@ Function -> reference -> how it is pinned (tests/test_oracle_cpu.py unless state
   oracle_trace_closest  textured frame octrees: BVH2Common.cpp:168-188, 689-710
       (tests/test_frame_tex_cpu.py) pinned by bit-identical hits with the frame octree of the same
       nodes and by the closed form of texture coordinates that are a linear map of position.
+  oracle_trace_closest  tricubic interpolation: BVH2Common.cpp:220-335, 790-844, 884-955, 1478-1569
+      (tests/test_tricubic_cpu.py) pinned by the spline's exact reproduction of quadratics (hits
+      on the sphere |p-c| = r from samples of |p-c|^2 - r^2, radial normals, for brick sets and
+      grids), the non-marching modes' entry-face hits and the representation modes' normal signs.
   oracle_coct_voxel     BVH2Common.cpp:1637-1851   the loaders alone (see above)
   oracle_local_surface  BVH2Common.cpp:345-725 (one voxel)  pinned by float64 root finding on
       the trilinear polynomial along the ray (first sign change), for every intersection mode

@@ -580,6 +580,112 @@ struct Tracer {
     return vmin;
   }
 
+  // BVH2Common.cpp:790-844: 4^3 samples from one voxel before the voxel's corner, stored at
+  // 16 z + 4 y + x. The reference converts the float position (which can be -1) to an unsigned
+  // index before adding the padding; with brick_pad >= 1 (trace calls refuse less) that is the
+  // integer arithmetic below.
+  float load_tricubic_distance_values(uint32_t nodeId, f3 voxelPos, uint32_t v_size, float sz_inv, const SBSHeader& h,
+                                      float values[64]) {
+    float vmin = 1e6f;
+    const uint32_t v_off = s.sbs_nodes[nodeId].data_offset;
+    const int p0x = int(voxelPos.x) - 1, p0y = int(voxelPos.y) - 1, p0z = int(voxelPos.z) - 1;
+    for (int x = 0; x < 4; x++)
+      for (int y = 0; y < 4; y++)
+        for (int z = 0; z < 4; z++) {
+          const uint32_t vId = uint32_t(p0x + x + int(h.brick_pad)) * v_size * v_size +
+                               uint32_t(p0y + y + int(h.brick_pad)) * v_size + uint32_t(p0z + z + int(h.brick_pad));
+          float v;
+          if (h.aux_data == LAYOUT_ID32F || h.aux_data == LAYOUT_ID32F_IN) {
+            v = s.sbs_data_f[s.sbs_data[v_off + vId]];
+          } else {
+            const uint32_t vpi = 4 / h.bytes_per_value, bits = 8 * h.bytes_per_value;
+            const uint32_t max_val = h.bytes_per_value == 4 ? 0xFFFFFFFFu : ((1u << bits) - 1);
+            const float d_max = 1.73205081f * sz_inv;
+            const float mult = 2 * d_max / max_val;
+            const uint32_t sh = bits * (vId % vpi);
+            v = -d_max + mult * ((sh < 32 ? s.sbs_data[v_off + vId / vpi] >> sh : 0u) & max_val);
+          }
+          values[16 * z + 4 * y + x] = v;
+          vmin = std::min(vmin, v);
+        }
+    return vmin;
+  }
+
+  // tricubic_spline (BVH2Common.cpp:1526-1529): Catmull-Rom through p0..p3 at x in [0, 1]
+  static float tricubic_spline(float p0, float p1, float p2, float p3, float x) {
+    return p1 + 0.5f * x * (p2 - p0 + x * (2.0f * p0 - 5.0f * p1 + 4.0f * p2 - p3 + x * (3.0f * (p1 - p2) + p3 - p0)));
+  }
+
+  // BVHRT::tricubicInterpolation (BVH2Common.cpp:1531-1569): along x for each (y, z) row, then
+  // along y, then along z
+  static float tricubicInterpolation(const float grid[64], const float dp[3]) {
+    float values_yz[16], values_z[4];
+    for (uint32_t j = 0; j < 4; ++j)
+      for (uint32_t k = 0; k < 4; ++k)
+        values_yz[4 * j + k] = tricubic_spline(grid[k * 16 + j * 4 + 0], grid[k * 16 + j * 4 + 1], grid[k * 16 + j * 4 + 2],
+                                               grid[k * 16 + j * 4 + 3], dp[0]);
+    for (uint32_t k = 0; k < 4; ++k)
+      values_z[k] = tricubic_spline(values_yz[k], values_yz[4 + k], values_yz[8 + k], values_yz[12 + k], dp[1]);
+    return tricubic_spline(values_z[0], values_z[1], values_z[2], values_z[3], dp[2]);
+  }
+  static float tricubic_at(const float grid[64], f3 p) {
+    const float dp[3] = {p.x, p.y, p.z};
+    return tricubicInterpolation(grid, dp);
+  }
+
+  // BVHRT::TricubicLocalSurfaceIntersection (BVH2Common.cpp:220-335): only ST mode marches; other
+  // modes hit where the ray starts inside (BBOX at once)
+  void TricubicLocalSurfaceIntersection(uint32_t type, f3 ray_dir, uint32_t instId, uint32_t geomId,
+                                        const float values[64], uint32_t primId, float d, float qNear, float qFar,
+                                        f2 fNearFar, f3 start_q, Hit* pHit) {
+    const float EPS = 1e-6f;
+    const float d_inv = 1.0f / d;
+    float t = qNear;
+    bool hit = false;
+    unsigned iter = 0;
+    float start_sign = 1;
+    float start_dist = tricubic_at(values, start_q + t * ray_dir);
+    if (s.preset.representation_mode == 0) start_sign = lm_sign(start_dist);   // REPRESENTATION_MODE_SURFACE
+    start_dist *= start_sign;
+    if (start_dist <= EPS || s.preset.sdf_node_intersect == 3) {
+      hit = true;
+    } else if (s.preset.sdf_node_intersect == 0) {
+      const unsigned ST_max_iters = 256;
+      float dist = start_dist;
+      while (t < qFar && dist > EPS && iter < ST_max_iters) {
+        t += dist * d_inv;
+        dist = tricubic_at(values, start_q + t * ray_dir);
+        iter++;
+      }
+      hit = (dist <= EPS);
+    }
+    const float tReal = fNearFar.x + d * t;
+    if (t <= qFar && hit && tReal < pHit->t) {
+      f3 norm(0, 0, 1);
+      if (need_normal()) {
+        const f3 p0 = start_q + t * ray_dir;
+        const float h = 0.001f;
+        const float ddx = (tricubic_at(values, f3((p0 + f3(h, 0, 0)).x, p0.y, p0.z)) -
+                           tricubic_at(values, f3((p0 + f3(-h, 0, 0)).x, p0.y, p0.z))) / (2 * h);
+        const float ddy = (tricubic_at(values, f3(p0.x, (p0 + f3(0, h, 0)).y, p0.z)) -
+                           tricubic_at(values, f3(p0.x, (p0 + f3(0, -h, 0)).y, p0.z))) / (2 * h);
+        const float ddz = (tricubic_at(values, f3(p0.x, p0.y, (p0 + f3(0, 0, h)).z)) -
+                           tricubic_at(values, f3(p0.x, p0.y, (p0 + f3(0, 0, -h)).z))) / (2 * h);
+        norm = start_sign * normalize(matmul4x3(s.instances[instId].transformInvTransposed, f3(ddx, ddy, ddz)));
+      }
+      const f2 en = encode_normal(norm);
+      pHit->t = tReal;
+      pHit->primId = primId;
+      pHit->instId = instId;
+      pHit->geomId = geomId | (type << SH_TYPE);
+      pHit->coords[0] = 0;
+      pHit->coords[1] = 0;
+      pHit->coords[2] = en.x;
+      pHit->coords[3] = en.y;
+      if (s.preset.render_mode == 10) pHit->primId = iter;   // MULTI_RENDER_MODE_ST_ITERATIONS
+    }
+  }
+
   // rayCapsuleIntersect (BVH2Common.cpp:1184-1223)
   static f4 rayCapsuleIntersect(f3 ray_pos, f3 ray_dir, f3 pa, f3 pb, float ra) {
     f3 ba = pb - pa;
@@ -747,7 +853,8 @@ struct Tracer {
     }
   }
 
-  // BVHRT::eval_distance_sdf_grid (BVH2Common.cpp:1442-1478), trilinear
+  // BVHRT::eval_distance_sdf_grid (BVH2Common.cpp:1442-1520): trilinear, or tricubic where the
+  // 4^3 samples fit inside the grid (trilinear next to the faces)
   float eval_distance_sdf_grid(uint32_t grid_id, f3 pos) const {
     const uint32_t off = s.grid_offsets[grid_id];
     const uint32_t sx = s.grid_sizes[3 * grid_id], sy = s.grid_sizes[3 * grid_id + 1], sz = s.grid_sizes[3 * grid_id + 2];
@@ -758,7 +865,16 @@ struct Tracer {
     const uint32_t vx = uint32_t(vox_f.x), vy = uint32_t(vox_f.y), vz = uint32_t(vox_f.z);
     const f3 dp = vox_f - f3(float(vx), float(vy), float(vz));
     float res = 0.0f;
-    if (vx < sx - 1 && vy < sy - 1 && vz < sz - 1) {
+    if (s.preset.interpolation_mode == 1 && vx < sx - 2 && vy < sy - 2 && vz < sz - 2 && vx > 0 && vy > 0 && vz > 0) {
+      const uint32_t x0 = vx - 1u, y0 = vy - 1u, z0 = vz - 1u;
+      const float f_dp[3] = {dp.x, dp.y, dp.z};
+      float grid_part[64];
+      for (uint32_t x = 0; x < 4; ++x)
+        for (uint32_t y = 0; y < 4; ++y)
+          for (uint32_t z = 0; z < 4; ++z)
+            grid_part[z * 16 + y * 4 + x] = s.grid_data[off + (z0 + z) * sx * sy + (y0 + y) * sx + (x0 + x)];
+      res = tricubicInterpolation(grid_part, f_dp);
+    } else if (vx < sx - 1 && vy < sy - 1 && vz < sz - 1) {
       for (uint32_t i = 0; i < 2; i++)
         for (uint32_t j = 0; j < 2; j++)
           for (uint32_t k = 0; k < 2; k++) {
@@ -897,7 +1013,7 @@ struct Tracer {
     pHit->coords[3] = e.y;
   }
 
-  // BVH2Common.cpp:884-1112 (trilinear)
+  // BVH2Common.cpp:884-1112 (trilinear, or tricubic when the preset asks for it)
   void OctreeBrickIntersect(uint32_t type, f3 ray_pos, f3 ray_dir, float tNear, uint32_t instId, uint32_t geomId,
                             uint32_t bvhNodeId, Hit* pHit) {
     const uint32_t sdfId = s.geoms[geomId].offset[0];
@@ -928,6 +1044,20 @@ struct Tracer {
       f3 min_pos = brick_min_pos + d * voxelPos;
       f3 max_pos = min_pos + d * f3(1, 1, 1);
       float values[8];
+      if (s.preset.interpolation_mode != 0) {
+        float tricub_values[64];
+        const float vmin3 = load_tricubic_distance_values(nodeId, voxelPos, v_size, sz_inv, header, tricub_values);
+        const f2 fNF = RayBoxIntersection2(ray_pos, SafeInverse(ray_dir), min_pos, max_pos);
+        if (tNear < fNF.x && vmin3 <= 0.0f) {
+          const f3 start_pos = ray_pos + fNF.x * ray_dir;
+          const f3 start_q = (start_pos - min_pos) * (0.5f * sz * header.brick_size);
+          const float qFar = (fNF.y - fNF.x) * (0.5f * sz * header.brick_size);
+          TricubicLocalSurfaceIntersection(type, ray_dir, instId, geomId, tricub_values, primId, d, 0.0f, qFar, fNF,
+                                           start_q, pHit);
+        }
+        brick_fNearFar.x += std::max(0.0f, fNF.y - brick_fNearFar.x) + 1e-6f;
+        continue;
+      }
       float vmin = load_distance_values(nodeId, voxelPos, v_size, sz_inv, header, values);
       f2 fNearFar = RayBoxIntersection2(ray_pos, SafeInverse(ray_dir), min_pos, max_pos);
       if (tNear < fNearFar.x && vmin <= 0.0f) {

@@ -90,12 +90,3 @@ def test_grid_file_round_trip_and_errors(lrt, tmp_path):
     (tmp_path / "cut.bin").write_bytes(raw[:-4])
     with pytest.raises(lrt.LrtError):
         lrt.load_sdf_grid(str(tmp_path / "cut.bin"))
-
-
-def test_only_trilinear_interpolation_is_accepted(lrt):
-    s = lrt.SceneRT()
-    p = lrt.default_preset()
-    p["interpolation_mode"] = 1
-    with pytest.raises(lrt.LrtError) as e:
-        s.SetPreset(p)
-    assert e.value.code == 4 or "trilinear" in str(e.value)
