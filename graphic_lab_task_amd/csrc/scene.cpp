@@ -613,9 +613,10 @@ void Scene::set_preset(const MultiRenderPreset& p) {
 
 uint32_t Scene::leaf_kinds() const {
   uint32_t k = 0;
-  for (uint32_t t : geomTags) {
+  for (size_t g = 0; g < geomTags.size(); ++g) {
+    const uint32_t t = geomTags[g];
     if (t == TAG_TRIANGLE) k |= LK_TRI;
-    else if (t == TAG_SDF_NODE) k |= LK_FRAME;
+    else if (t == TAG_SDF_NODE) k |= geomData[g].type == TYPE_SDF_FRAME_OCTREE_TEX ? (LK_FRAME | LK_FRAME_TEX) : LK_FRAME;
     else if (t == TAG_SDF_BRICK) k |= LK_SBS;
     else if (t == TAG_SDF_OCTREE) k |= LK_MULTI;
     else if (t == TAG_SDF_GRID) k |= LK_GRID;

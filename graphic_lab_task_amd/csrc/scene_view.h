@@ -26,6 +26,10 @@ enum LeafKind : uint32_t {
   // Walk option set in SceneView::leaf_kinds when the preset interpolates tricubically and the
   // scene holds brick sets or grids: selects the kernels with the 4^3-sample interpolant.
   LK_TRICUBIC = 1u << 11,
+  // TAG_SDF_NODE over a textured frame octree (always with LK_FRAME): its texture-coordinate step
+  // is compiled only into the kernels that take this kind (the mixed ones), so the frame-octree
+  // kernel keeps its registers.
+  LK_FRAME_TEX = 1u << 12,
 };
 
 // Frame-octree leaf as the kernels read it: the leaf box, its 8 corner distances and the

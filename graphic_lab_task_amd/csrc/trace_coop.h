@@ -112,7 +112,7 @@ __device__ __forceinline__ void coop_leaf(const SceneView& s, float3 ray_pos, fl
     prims += a_count;
     intersect_tris(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, a_count, h);
   } else if ((KINDS & LK_FRAME) && tag == TAG_SDF_NODE) {
-    frame_node_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, h);
+    frame_node_intersect<(KINDS & LK_FRAME_TEX) != 0>(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, h);
   } else if ((KINDS & LK_SBS) && tag == TAG_SDF_BRICK) {
     sbs_brick_intersect<(KINDS & LK_SBS_SMOOTH) != 0, (KINDS & LK_TRICUBIC) != 0>(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, h);
   } else if ((KINDS & LK_MULTI) && tag == TAG_SDF_OCTREE) {

@@ -425,14 +425,16 @@ __device__ __forceinline__ float frame_tex_interp(const float tc[16], float3 dp,
 }
 
 // Frame-octree leaf: BVHRT::OctreeNodeIntersect + RayNodeIntersection (BVH2Common.cpp:727-749,
-// 141-188), reading the leaf's box, corner values and node index from one FrameLeaf record; a
-// textured octree's hit also takes its texture coordinates (LocalSurfaceIntersection, :689-710).
+// 141-188), reading the leaf's box, corner values and node index from one FrameLeaf record; with
+// TEX (kernels taking LK_FRAME_TEX) a textured octree's hit also takes its texture coordinates
+// (LocalSurfaceIntersection, :689-710).
+template <bool TEX = false>
 __device__ __forceinline__ void frame_node_intersect(const SceneView& s, const GeomRef& g, float3 ray_pos, float3 ray_dir,
                                                      float tNear, uint32_t instId, uint32_t geomId,
                                                      uint32_t bvhNodeId, CRT_Hit& hit) {
   const uint32_t type = g.type;
   // frame octrees and sparse voxel sets (BVH2Common.cpp:141-218): one leaf record per BLAS leaf
-  if (type != TYPE_SDF_FRAME_OCTREE && type != TYPE_SDF_SVS && type != TYPE_SDF_FRAME_OCTREE_TEX) return;
+  if (type != TYPE_SDF_FRAME_OCTREE && type != TYPE_SDF_SVS && (!TEX || type != TYPE_SDF_FRAME_OCTREE_TEX)) return;
   const uint2 go = g.offset;   // y: first leaf record of this geometry
   const float4* q = reinterpret_cast<const float4*>(s.frame_leaves + go.y + bvhNodeId);
   const float4 r0 = q[0], r1 = q[1], r2 = q[2], r3 = q[3];
@@ -448,6 +450,10 @@ __device__ __forceinline__ void frame_node_intersect(const SceneView& s, const G
   const float qFar = (fNearFar.y - fNearFar.x) / d;
   const float qNear = tNear > fNearFar.x ? (tNear - fNearFar.x) / d : 0.0f;
   if (qNear > 0.0f) return;
+  if (!TEX) {
+    local_surface_intersection(s, type, ray_dir, instId, geomId, values, primId, d, qNear, qFar, fNearFar, start_q, hit);
+    return;
+  }
   float t_local = 0.0f;
   if (local_surface_intersection(s, type, ray_dir, instId, geomId, values, primId, d, qNear, qFar, fNearFar, start_q, hit,
                                  &t_local) &&

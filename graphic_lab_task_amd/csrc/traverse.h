@@ -42,7 +42,7 @@ __device__ __forceinline__ void blas_traverse(const SceneView& s, float3 ray_pos
     if ((KINDS & LK_TRI) && tag == TAG_TRIANGLE) {
       intersect_tris(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, a_count, hit);
     } else if ((KINDS & LK_FRAME) && tag == TAG_SDF_NODE) {
-      frame_node_intersect(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
+      frame_node_intersect<(KINDS & LK_FRAME_TEX) != 0>(s, g, ray_pos, ray_dir, tNearSdf, instId, geomId, a_start, hit);
     } else if ((KINDS & LK_SBS) && tag == TAG_SDF_BRICK) {
       cnt.sdf_voxels += sbs_brick_intersect<(KINDS & LK_SBS_SMOOTH) != 0, (KINDS & LK_TRICUBIC) != 0>(s, g, ray_pos, ray_dir, tNearSdf, instId,
                                                                           geomId, a_start, hit, rec);

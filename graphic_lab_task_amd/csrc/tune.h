@@ -26,7 +26,8 @@ struct TuneResult {
 // minimum of `reps`): (1) block size x LDS stack entries x ray interleaving (off, groups of 8
 // or 32 lanes) with the single-pass walk, and block size x LDS stack entries as the persistent
 // kernel, (2) step budget x tail waves of the two-pass scheme on the three best grid shapes from (1); both stages also try triangle leaf postponement off and on when
-// `tune_postpone`.
+// `tune_postpone`; (3) the four fastest distinct candidates timed again in turns over seven
+// rounds, the lowest median time winning (best_ms is that median).
 hipError_t autotune(const std::function<hipError_t(const TuneConfig&)>& launch, hipStream_t stream,
                     const TuneConfig& start, bool tune_postpone, int reps, TuneResult* out);
 

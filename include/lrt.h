@@ -393,7 +393,7 @@ lrt_status lrt_sbs_recompute_misses(lrt_scene* s, uint64_t* out, int reset);
 typedef struct lrt_tune_result {
   int32_t block, lds_stack, xcd_remap;
   uint32_t budget, tail_waves;
-  float best_ms;           /* fastest time of one closest-hit call on the tuning batch */
+  float best_ms;           /* median time of one closest-hit call with the chosen configuration */
   uint32_t configs_tried;
   int32_t leaf_postpone;   /* triangle leaf postponement chosen (0/1; tuned on triangle-only scenes) */
   uint32_t ray_interleave; /* lanes per group chosen (lrt_set_ray_interleave; 0 = off) */
@@ -409,7 +409,8 @@ lrt_status lrt_set_persistent(lrt_scene* s, int on);
 /* Chooses the launch configuration (block size, LDS stack entries, ray interleaving, step
  * budget, tail waves, persistent or grid launch, and on triangle-only scenes the leaf postponement) by timing closest-hit calls on the
  * caller's device ray batch (`hits` is overwritten with the batch's hit records), and keeps
- * the fastest on the scene. Every configuration returns identical records, up to the ulp-level
+ * the fastest on the scene: the four fastest candidates are timed again in turns over seven
+ * rounds and the lowest median wins, so drift between candidates' turns cannot decide. Every configuration returns identical records, up to the ulp-level
  * ties noted at lrt_set_leaf_postponement. No counterpart in the reference (its CPU loop has
  * nothing to tune). */
 lrt_status lrt_autotune(lrt_scene* s, const float* pos_near, const float* dir_far, lrt_hit* hits, uint64_t n,
