@@ -93,16 +93,10 @@ lrt::TraceArgs trace_args(lrt_scene* s, const float* pos, const float* dir, lrt_
   return a;
 }
 
-// Queries this library does not answer: the channel render modes of SCom2 trees with channels
-// (their hit-point quotients are rotated by float matrices, BVH2Common.cpp:2697-2711, not built),
-// and tricubic interpolation of a brick set without padding (the 4^3 samples start one voxel
-// before the brick, BVH2Common.cpp:790-844: the reference reads outside the brick there).
-// Returns the reason, or null.
+// Queries this library does not answer: tricubic interpolation of a brick set without padding
+// (the 4^3 samples start one voxel before the brick, BVH2Common.cpp:790-844: the reference reads
+// outside the brick there). Returns the reason, or null.
 const char* unsupported_query(const lrt::Scene& sc) {
-  const uint32_t rm = sc.preset.render_mode;
-  for (const lrt::SCom2Header& h : sc.scom2Headers)
-    if (h.has_channels && (rm == lrt::MULTI_RENDER_MODE_CHANNEL || rm == lrt::MULTI_RENDER_MODE_CHANNEL_LAMBERT))
-      return "channel render modes of SCom2 trees with channels are not built";
   if (sc.preset.interpolation_mode == lrt::INTERPOLATION_MODE_TRICUBIC)
     for (const lrt::SdfSBSHeader& h : sc.sbsHeaders)
       if (h.brick_pad < 1) return "tricubic interpolation needs brick sets with brick_pad >= 1";

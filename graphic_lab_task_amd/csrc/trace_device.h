@@ -1698,6 +1698,36 @@ __device__ __forceinline__ void coct_intersect(const SceneView& s, const GeomRef
 
 namespace lrt {
 
+// m_SdfCompactOctreeRotTransforms[r] * (p, 1) (BVH2Common.cpp:2501, 2742; the matrices of
+// octahedral_groups.cpp:37-101): rotation r turns the lattice [0, vt - 1]^3 about its centre. Read
+// off the modifier of rotation r on a lattice of v points per axis: input axis i with coefficient
+// +-v^k lands on output axis 2 - k, and a mirrored axis maps x to (vt - 1) - x. Each row of the
+// reference's matrix has one non-zero term besides the translation, so the float result does not
+// depend on the order of its sum.
+__device__ __forceinline__ float3 rot_transform_point(int4 m, int v, float vt1, float3 p) {
+  const float in[3] = {p.x, p.y, p.z};
+  const int c[3] = {m.x, m.y, m.z};
+  float out[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int a = c[i] < 0 ? -c[i] : c[i];
+    const int o = a == 1 ? 2 : (a == v ? 1 : 0);
+    out[o] = c[i] < 0 ? vt1 - in[i] : in[i];
+  }
+  return make3(out[0], out[1], out[2]);
+}
+
+// Channel render modes' quotients of a hit point dp turned by a rotation (BVH2Common.cpp:2492-2505,
+// 2731-2747): clamp(65535 dp_rot) as (y << 16 | x, z).
+__device__ __forceinline__ void channel_quotients_rotated(CRT_Hit& hit, int4 m, int v, float vt1, float3 dp) {
+  float3 r = rot_transform_point(m, v, vt1, dp);
+  r = make3(clamp_ref(65535.0f * r.x, 0.0f, 65535.0f), clamp_ref(65535.0f * r.y, 0.0f, 65535.0f),
+            clamp_ref(65535.0f * r.z, 0.0f, 65535.0f));
+  const int qx = int(r.x), qy = int(r.y), qz = int(r.z);
+  hit.coords[0] = __int_as_float((qy << 16) | qx);
+  hit.coords[1] = __int_as_float(qz);
+}
+
 // ---- octree DAG (BVHRT::DAGIntersect, MultiRenderer/BVH2Common.cpp:2365-2557) ----
 // Depth-first walk from the root; each stack entry carries the cumulative rotation of its node
 // (composed through the rotation table along the path). Children are pushed far to near by
@@ -1780,14 +1810,23 @@ __device__ __forceinline__ void dag_intersect(const SceneView& s, const GeomRef&
             vmax = fmax_ref(vmax, val);
           }
           fNearFar = ray_box2(ray_pos, safe_inverse(ray_dir), min_pos, max_pos);
+          float3 start_q = make3(0.0f, 0.0f, 0.0f);
           if (fNearFar.x < fNearFar.y && vmin <= 0.0f && vmax >= 0.0f) {
             const float3 start_pos = ray_pos + fNearFar.x * ray_dir;
             const float vox_sz = 0.5f * sz * float(bs);
-            const float3 start_q = vox_sz * (start_pos - min_pos);
+            start_q = vox_sz * (start_pos - min_pos);
             const float qNear = fmax_ref(0.0f, vox_sz * (tNear - fNearFar.x));
             const float qFar = vox_sz * (fNearFar.y - fNearFar.x);
             local_surface_intersection<true>(s, type, ray_dir, instId, geomId, values, primId, d, qNear, qFar, fNearFar,
                                        start_q, hit, nullptr, is_surface);
+          }
+          // channel render modes: the quotients turned by the node's channel rotation, so that the
+          // resolve reads the channel data as stored (BVH2Common.cpp:2492-2505)
+          if (hit.t != old_t && (s.preset.render_mode == MULTI_RENDER_MODE_CHANNEL ||
+                                 s.preset.render_mode == MULTI_RENDER_MODE_CHANNEL_LAMBERT)) {
+            const uint32_t crot = s.rot_add[info * COCTREE_ROT_COUNT + node.channels_edge.rotation_id];
+            const float tl = (hit.t - fNearFar.x) / d;
+            channel_quotients_rotated(hit, Mb[crot], int(v_size), float(bs), start_q + tl * ray_dir);
           }
         }
         bnf.x += fmax_ref(0.0f, fNearFar.y - bnf.x) + 1e-5f;
@@ -1947,19 +1986,27 @@ __device__ __forceinline__ void scom2_intersect(const SceneView& s, const GeomRe
             vmin = fmin_ref(vmin, val);
             vmax = fmax_ref(vmax, val);
           }
+          float3 start_q = make3(0.0f, 0.0f, 0.0f);
           if (vmin <= 0.0f && vmax >= 0.0f) {
             const float3 start_pos = ray_pos + fNearFar.x * ray_dir;
             const float vox_sz = 0.5f * level_sz_f * float(bs);
-            const float3 start_q = vox_sz * (start_pos - min_pos);
+            start_q = vox_sz * (start_pos - min_pos);
             const float qNear = fmax_ref(0.0f, vox_sz * (tNear - fNearFar.x));
             const float qFar = vox_sz * (fNearFar.y - fNearFar.x);
             local_surface_intersection<true>(s, type, ray_dir, instId, geomId, values, nodeId, d, qNear, qFar, fNearFar,
                                        start_q, hit, nullptr, is_surface);
           }
-          // with channels the hit names the leaf's channel link (MULTI_RENDER_MODE_PRIMITIVE excepted)
+          // with channels the hit names the leaf's channel link (MULTI_RENDER_MODE_PRIMITIVE excepted);
+          // the channel modes turn the quotients by the channel's rotation (BVH2Common.cpp:2731-2747)
           if (hit.t != old_t && has_ch > 0 && s.preset.render_mode != MULTI_RENDER_MODE_PRIMITIVE) {
             const uint32_t ch_off = nodeId + surface_count * upl;
-            hit.primId = scom2_child_edge(h, D[ch_off], D[ch_off + coo]).child_offset;
+            const SCom2ChildEdge ce = scom2_child_edge(h, D[ch_off], D[ch_off + coo]);
+            hit.primId = ce.child_offset;
+            if (s.preset.render_mode == MULTI_RENDER_MODE_CHANNEL || s.preset.render_mode == MULTI_RENDER_MODE_CHANNEL_LAMBERT) {
+              const uint32_t crot = s.rot_add[rotId * COCTREE_ROT_COUNT + ce.rotation_id];
+              const float tl = (hit.t - fNearFar.x) / d;
+              channel_quotients_rotated(hit, M[crot], int(bs + 1), float(bs), start_q + tl * ray_dir);
+            }
           }
         }
         tmin += fmax_ref(0.0f, fNearFar.y - tmin) + 1e-5f;

@@ -124,8 +124,9 @@ lrt_status lrt_add_sdf_dag(lrt_scene* s, const void* header, const void* nodes, 
  * LRT_ERR_INVALID_ARG with the reason. Where the reference takes its fast traversal
  * (SCom2IntersectFast: the scene's only SCom2 tree is plain -- no multi-nodes, no channels, 3^3
  * values, one-word links -- and the preset renders LAMBERT_NO_TEX) so does this library; a plain
- * tree must then have brick_size 2. Trace calls return LRT_ERR_UNSUPPORTED for the channel render
- * modes of trees with channels (not built). Hits report TYPE_SCOM2. */
+ * tree must then have brick_size 2. Trees with channels report the leaf's channel link as primId;
+ * the channel render modes turn the hit's voxel-local quotients by the channel's rotation
+ * (BVH2Common.cpp:2731-2747). Hits report TYPE_SCOM2. */
 lrt_status lrt_add_scom2(lrt_scene* s, const void* header, const uint32_t* nodes, size_t n_nodes,
                          const uint32_t* bricks, size_t n_bricks, uint32_t* out_geom_id);
 

@@ -229,6 +229,62 @@ struct VoxelRec {
 };
 static_assert(sizeof(VoxelRec) == 72, "layout");
 
+// scom::initialize_rot_transforms (sdf/scom2/clustering/octahedral_groups.cpp:37-101) in three
+// dimensions: the axis permutations in lexicographic order (the recursive enumeration's order),
+// each with the 8 sign patterns (bit d negates axis d), as a matrix R with R[perm[d]][d] = sign_d,
+// turned about the lattice centre c = v/2 - 0.5 (T1 R T2, translations by +c and -c), then taken
+// in the legacy order of legacy_to_universal_id_remap (octahedral_groups.h:27-35).
+struct RotTransforms {
+  float M[48][4][4];
+  explicit RotTransforms(int v) {
+    static const int legacy_to_universal[48] = {0, 24, 32, 1, 25, 33, 8, 16, 40, 9, 17, 41,
+                                                2, 26, 34, 3, 27, 35, 10, 18, 42, 11, 19, 43,
+                                                4, 28, 36, 5, 29, 37, 12, 20, 44, 13, 21, 45,
+                                                6, 30, 38, 7, 31, 39, 14, 22, 46, 15, 23, 47};
+    float U[48][4][4];
+    int perm[3] = {0, 1, 2};
+    int t_id = 0;
+    const float c = v / 2.0f - 0.5f;
+    do {
+      for (int sign = 0; sign < 8; ++sign) {
+        float R[4][4] = {}, T1[4][4] = {}, T2[4][4] = {}, A[4][4] = {};
+        for (int d = 0; d < 3; ++d) R[perm[d]][d] = ((sign >> d) % 2 == 0) ? 1.0f : -1.0f;
+        R[3][3] = 1.0f;
+        for (int d = 0; d < 4; ++d) T1[d][d] = T2[d][d] = 1.0f;
+        for (int d = 0; d < 3; ++d) { T1[d][3] = c; T2[d][3] = -c; }
+        for (int a = 0; a < 4; ++a)
+          for (int b = 0; b < 4; ++b)
+            for (int k = 0; k < 4; ++k) A[a][b] += R[a][k] * T2[k][b];
+        float* out = &U[t_id * 8 + sign][0][0];
+        for (int a = 0; a < 4; ++a)
+          for (int b = 0; b < 4; ++b) {
+            float acc = 0.0f;
+            for (int k = 0; k < 4; ++k) acc += T1[a][k] * A[k][b];
+            out[a * 4 + b] = acc;
+          }
+      }
+      ++t_id;
+    } while (std::next_permutation(perm, perm + 3));
+    for (int i = 0; i < 48; ++i) std::memcpy(M[i], U[legacy_to_universal[i]], sizeof(M[i]));
+  }
+  // to_float3(M * float4(p, 1))
+  f3 apply(int r, f3 p) const {
+    const float (*m)[4] = M[r];
+    return f3(m[0][0] * p.x + m[0][1] * p.y + m[0][2] * p.z + m[0][3], m[1][0] * p.x + m[1][1] * p.y + m[1][2] * p.z + m[1][3],
+              m[2][0] * p.x + m[2][1] * p.y + m[2][2] * p.z + m[2][3]);
+  }
+};
+
+// one table per lattice size, built on first use
+inline const RotTransforms& rot_transforms(int v) {
+  static RotTransforms* tables[33] = {};
+  #pragma omp critical(oracle_rot_transforms)
+  {
+    if (!tables[v]) tables[v] = new RotTransforms(v);
+  }
+  return *tables[v];
+}
+
 struct Tracer {
   const OracleScene& s;
   OracleCounts* cnt;
@@ -1574,14 +1630,31 @@ struct Tracer {
                 vmax = std::max(vmax, val);
               }
               fNearFar = RayBoxIntersection2(ray_pos, SafeInverse(ray_dir), min_pos, max_pos);
+              f3 start_q(0, 0, 0);
               if (fNearFar.x < fNearFar.y && vmin <= 0.0f && vmax >= 0.0f) {
                 const f3 start_pos = ray_pos + fNearFar.x * ray_dir;
                 const float vox_sz = (0.5f * sz * header.brick_size);
-                const f3 start_q = vox_sz * (start_pos - min_pos);
+                start_q = vox_sz * (start_pos - min_pos);
                 const float qNear = std::max(0.0f, vox_sz * (tNear - fNearFar.x));
                 const float qFar = vox_sz * (fNearFar.y - fNearFar.x);
                 LocalSurfaceIntersection(type, ray_dir, instId, geomId, values, primId, d, qNear, qFar, fNearFar,
                                          start_q, pHit, is_surface);
+              }
+              // channel render modes: quotients turned by the channel rotation (:2492-2505), with the
+              // transforms of the brick_size + 1 lattice (BVH2Common_host.cpp:1027)
+              if (pHit->t != old_t && (s.preset.render_mode == 23 || s.preset.render_mode == 27)) {
+                const RotTransforms& rt = rot_transforms(header.brick_size + 1);
+                const uint32_t channelRotId = s.rot_add[curElem.info * 48 + node.channels_edge.rotation_id];
+                const float t = (pHit->t - fNearFar.x) / d;
+                const f3 dp = start_q + t * ray_dir;
+                f3 dp_rot = rt.apply(int(channelRotId), dp);
+                dp_rot = f3(std::min(std::max(65535.0f * dp_rot.x, 0.0f), 65535.0f),
+                            std::min(std::max(65535.0f * dp_rot.y, 0.0f), 65535.0f),
+                            std::min(std::max(65535.0f * dp_rot.z, 0.0f), 65535.0f));
+                const int qx = int(dp_rot.x), qy = int(dp_rot.y), qz = int(dp_rot.z);
+                const uint32_t w0 = uint32_t((qy << 16) | qx), w1 = uint32_t(qz);
+                std::memcpy(&pHit->coords[0], &w0, 4);
+                std::memcpy(&pHit->coords[1], &w1, 4);
               }
               s_id++;
             }
@@ -1764,10 +1837,11 @@ struct Tracer {
               vmin = std::min(vmin, val);
               vmax = std::max(vmax, val);
             }
+            f3 start_q(0, 0, 0);
             if (vmin <= 0.0f && vmax >= 0.0f) {
               const f3 start_pos = ray_pos + fNearFar.x * ray_dir;
               const float vox_sz = (0.5f * level_sz_f * header.brick_size);
-              const f3 start_q = vox_sz * (start_pos - min_pos);
+              start_q = vox_sz * (start_pos - min_pos);
               const float qNear = std::max(0.0f, vox_sz * (tNear - fNearFar.x));
               const float qFar = vox_sz * (fNearFar.y - fNearFar.x);
               LocalSurfaceIntersection(type, ray_dir, instId, geomId, values, curElem.nodeId, d, qNear, qFar, fNearFar,
@@ -1775,7 +1849,24 @@ struct Tracer {
             }
             if (pHit->t != old_t && header.has_channels > 0 && s.preset.render_mode != 5) {
               const uint32_t ch_off = curElem.nodeId + surface_count * header.uints_per_link;
-              pHit->primId = unpack_child_edge(header, data[ch_off], data[ch_off + header.child_offset_off]).child_offset;
+              const ChildEdge ce = unpack_child_edge(header, data[ch_off], data[ch_off + header.child_offset_off]);
+              pHit->primId = ce.child_offset;
+              // channel render modes: quotients turned by the channel rotation (:2731-2747), with the
+              // transforms of the brick_size + 1 lattice (BVH2Common_host.cpp:1141)
+              if (s.preset.render_mode == 23 || s.preset.render_mode == 27) {
+                const RotTransforms& rt = rot_transforms(header.brick_size + 1);
+                const uint32_t channelRotId = s.rot_add[rotId * 48 + ce.rotation_id];
+                const float t = (pHit->t - fNearFar.x) / d;
+                const f3 dp = start_q + t * ray_dir;
+                f3 dp_rot = rt.apply(int(channelRotId), dp);
+                dp_rot = f3(std::min(std::max(65535.0f * dp_rot.x, 0.0f), 65535.0f),
+                            std::min(std::max(65535.0f * dp_rot.y, 0.0f), 65535.0f),
+                            std::min(std::max(65535.0f * dp_rot.z, 0.0f), 65535.0f));
+                const int qx = int(dp_rot.x), qy = int(dp_rot.y), qz = int(dp_rot.z);
+                const uint32_t w0 = uint32_t((qy << 16) | qx), w1 = uint32_t(qz);
+                std::memcpy(&pHit->coords[0], &w0, 4);
+                std::memcpy(&pHit->coords[1], &w1, 4);
+              }
             }
             s_id++;
           }

@@ -1,11 +1,17 @@
-"""Plain SCom2 encoder for tests: packs a global octree of brick size 2 (sdf_spheres_global_octree)
-into the SCom2 layout both of the reference's walks read (SCom2Intersect and SCom2IntersectFast,
-MultiRenderer/BVH2Common.cpp:2560-3231) when a tree has no multi-nodes and no channels: per node
-one head word (bits 0-7 own brick type, 8-23 child types, 24-31 the children that have links)
-followed by one link word per such child in child order; a child node's link is its rotation
-(bits 26-31) and node offset, a leaf's link its brick rotation, additive shift code (bits 18-25)
-and brick index, or a unique-brick marker (rotation bits 0x30) and index; bricks hold 27 values of
-8 bits, 7 words each. Written from the walks, not from the reference's packer.
+"""SCom2 encoder for tests: packs a global octree (sdf_spheres_global_octree) into the SCom2
+layouts the reference's walks read (SCom2Intersect and SCom2IntersectFast,
+MultiRenderer/BVH2Common.cpp:2560-3231).
+
+Plain trees (no multi-nodes, no channels): per node one head word (bits 0-7 own brick type, 8-23
+child types, 24-31 the children that have links) followed by one link word per such child in
+child order. Multi-node trees (the field positions of the reference's engine-scene file): a head
+word (bits 0-7 own brick type, 8-23 child types, 24-31 the end of the node's own links), two
+words of 8-bit cumulative link ends per child slot (slots 0-3, 4-7), then the links; with channels
+a leaf's links are its brick link followed by a channel link (rotation bits 26-31, channel offset).
+A child node's link is its rotation (bits 26-31) and node offset, a leaf's link its brick
+rotation, additive shift code (bits 18-25) and brick index, or a unique-brick marker (rotation bits
+0x30) and index; bricks hold (bs+1)^3 values of 8 bits. Written from the walks, not from the
+reference's packer.
 """
 from __future__ import annotations
 
@@ -17,17 +23,19 @@ import dag_encode as de
 EMPTY, VOLUME, NODE, SURFACE = 0, 1, 2, 3
 
 
-def plain_header(root_node_off=0, shift=True):
+def plain_header(root_node_off=0, shift=True, brick_size=2, multi=False, channels=False):
     import graphic_lab_task_amd as g
     h = np.zeros((), g.SCOM2_HEADER_DTYPE)
-    vals = dict(brick_size=2, v_size=3, bits_per_value=8, values_per_uint=4, value_mask=0xFF, dimension=3,
+    v = brick_size + 1
+    vals = dict(brick_size=brick_size, v_size=v, bits_per_value=8, values_per_uint=4, value_mask=0xFF, dimension=3,
                 child_rot_shift=26, child_rot_mask=0x3F, child_add_shift=18, child_add_mask=0xFF,
                 child_offset_mask=0x3FFFF, child_offset_off=0, node_offset_mask=0x3FFFFFF, uints_per_link=1,
                 unique_brick_prefix=0xC0000000, unique_brick_offset_mask=0x3FFFFFFF, children_types_shift=8,
                 children_types_mask=0xFFFF, base_reference_shift=24, children_active_bits_shift=24,
-                children_active_bits_mask=0xFF, references_offset=0, reference_bits=8, reference_mask=0xFF,
-                references_per_uint=4, links_offset=1, max_surface_count=1, max_surface_count_per_leaf=1,
-                bricks_step=7, root_node_off=root_node_off, has_channels=0, has_surfaces=1, has_multi_nodes=0,
+                children_active_bits_mask=0xFF, reference_bits=8, reference_mask=0xFF,
+                references_per_uint=4, links_offset=3 if multi else 1, references_offset=1 if multi else 0,
+                max_surface_count=1, max_surface_count_per_leaf=1, bricks_step=(v ** 3 + 3) // 4,
+                root_node_off=root_node_off, has_channels=int(channels), has_surfaces=1, has_multi_nodes=int(multi),
                 max_depth=8)
     for k, v in vals.items():
         h[k] = v
@@ -40,11 +48,16 @@ def plain_header(root_node_off=0, shift=True):
 
 
 class Scom2Encoder:
-    def __init__(self, nodes, values, rotate=False, shift=False, unique_every=0, volume_every=0, seed=0):
+    def __init__(self, nodes, values, rotate=False, shift=False, unique_every=0, volume_every=0, seed=0,
+                 brick_size=2, multi=False, channels=False, channel_rotate=False):
         self.gn, self.gv = nodes, values
         self.rng = np.random.default_rng(seed)
+        self.crng = np.random.default_rng(seed + 1000)   # channel rotations: the bricks do not depend on them
         self.rotate, self.shift, self.unique_every, self.volume_every = rotate, shift, unique_every, volume_every
-        self.mv = ce._closed_mods(3)
+        self.bs, self.v = brick_size, brick_size + 1
+        self.multi, self.channels, self.channel_rotate = multi or channels, channels, channel_rotate
+        self.channel_info = []   # per channel offset: (cumulative rotation of the leaf's node, channel rotation)
+        self.mv = ce._closed_mods(self.v)
         self.mg = ce._closed_mods(2)
         self.rot_add = de.rot_add_reference(2).reshape(48, 48)
         self.words: list[int] = []
@@ -53,8 +66,8 @@ class Scom2Encoder:
         self.n_leaves = 0
 
     def _brick(self, i):
-        off = int(self.gn[i][1])
-        return np.asarray(self.gv[off:off + 27], np.float32).reshape(3, 3, 3)
+        off, v = int(self.gn[i][1]), self.v
+        return np.asarray(self.gv[off:off + v ** 3], np.float32).reshape(v, v, v)
 
     def _straddles(self, i):
         B = self._brick(i)
@@ -81,26 +94,33 @@ class Scom2Encoder:
             add = np.float32(max_val * frac) if self.shift else np.float32(0)
         rot = int(self.rot_add[q, r_edge]) if not unique else int(self.rot_add[q, 0])
         m = self.mv[rot]
-        qv = np.zeros(27, np.uint32)
-        for x in range(3):
-            for y in range(3):
-                for z in range(3):
+        n = self.v
+        qv = np.zeros(n ** 3, np.uint32)
+        for x in range(n):
+            for y in range(n):
+                for z in range(n):
                     v = (float(B[x, y, z]) - float(add)) / float(max_val)
                     qv[m[0] * x + m[1] * y + m[2] * z + m[3]] = int(np.clip(np.rint((v + 1.0) * 0.5 * 255.0), 0, 255))
-        idx = len(self.bricks) // 7
-        words = [0] * 7
-        for k in range(27):
+        step = (n ** 3 + 3) // 4
+        idx = len(self.bricks) // step
+        words = [0] * step
+        for k in range(n ** 3):
             words[k // 4] |= int(qv[k]) << (8 * (k % 4))
         self.bricks.extend(words)
         if unique:
             return 0xC0000000 | idx
         return (r_edge << 26) | (code << 18) | idx
 
+    def _channel_link(self, q):
+        r = int(self.crng.integers(0, 48)) if self.channel_rotate else 0
+        self.channel_info.append((q, r))
+        return (r << 26) | (len(self.channel_info) - 1)
+
     def _node(self, gi, q, level_sz):
         """Appends the node of global node gi under cumulative rotation q; returns its word index."""
         off = int(self.gn[gi][2])
         at = len(self.words)
-        self.words.append(0)
+        self.words.extend([0, 0, 0] if self.multi else [0])
         types = [EMPTY] * 8
         stored = {}   # stored child slot -> world child
         for c in range(8):
@@ -123,26 +143,41 @@ class Scom2Encoder:
             else:
                 types[slot] = NODE
                 links.append(("node", child))
-        self.words.extend([0] * len(links))
+        per_leaf = 2 if self.channels else 1
+        counts = [0 if types[slot] == EMPTY else (1 if types[slot] == NODE else per_leaf) for slot in range(8)]
+        first = at + (3 if self.multi else 1)
+        self.words.extend([0] * sum(counts))
         active = 0
         ctypes_ = 0
         for slot in range(8):
             ctypes_ |= types[slot] << (2 * slot)
             if types[slot] != EMPTY:
                 active |= 1 << slot
-        self.words[at] = (ctypes_ << 8) | (active << 24)
-        for k, (kind, child) in enumerate(links):
+        if self.multi:
+            ends = np.cumsum(counts)   # own links: none (the node's base type is empty)
+            self.words[at] = ctypes_ << 8
+            self.words[at + 1] = sum(int(ends[c]) << (8 * c) for c in range(4))
+            self.words[at + 2] = sum(int(ends[c]) << (8 * (c - 4)) for c in range(4, 8))
+        else:
+            self.words[at] = (ctypes_ << 8) | (active << 24)
+        w = first
+        for kind, child in links:
             if kind == "leaf":
-                self.words[at + 1 + k] = self._leaf_link(child, q, 2 * level_sz)
+                self.words[w] = self._leaf_link(child, q, 2 * level_sz)
+                if self.channels:
+                    self.words[w + 1] = self._channel_link(q)
+                w += per_leaf
             else:
                 r = int(self.rng.integers(0, 48)) if self.rotate else 0
                 qc = int(self.rot_add[q, r])
-                self.words[at + 1 + k] = (r << 26) | self._node(child, qc, 2 * level_sz)
+                self.words[w] = (r << 26) | self._node(child, qc, 2 * level_sz)
+                w += 1
         return at
 
     def encode(self):
         root = self._node(0, 0, 1)
-        return plain_header(root, self.shift), np.array(self.words, np.uint32), np.array(self.bricks, np.uint32)
+        return (plain_header(root, self.shift, self.bs, self.multi, self.channels), np.array(self.words, np.uint32),
+                np.array(self.bricks, np.uint32))
 
 
 def encode(nodes, values, **kw):

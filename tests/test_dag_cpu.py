@@ -219,3 +219,59 @@ def test_channel_modes_keep_the_voxel_local_hit_point(lrt, orc, gotrees):
     p = no_eps(lrt)
     h = oracle_trace(orc, lrt, dag_scene(lrt, de.encode(*gotrees[1], 1), p), pos, dir_)
     assert not np.any(h["coords"][h["primId"] != MISS, :2])
+
+
+def _axis_maps(v):
+    """Per rotation r of the reference's subgroup (restated in numpy from its generator): for each
+    input axis, the output axis it lands on and its sign, read off the index map of lattice v."""
+    out = []
+    for m in de.universal_transpositions(v):
+        def pt(i):
+            return np.array([i // (v * v), (i // v) % v, i % v])
+        o = pt(m[0])
+        axes = []
+        for a in range(3):
+            e = [0, 0, 0]
+            e[a] = 1
+            dvec = pt(m[e[0] * v * v + e[1] * v + e[2]]) - o
+            k = int(np.nonzero(dvec)[0][0])
+            axes.append((k, int(dvec[k])))
+        out.append(axes)
+    return out
+
+
+@pytest.mark.parametrize("bs", [1, 2])
+def test_channel_modes_turn_the_quotients_by_the_channel_rotation(lrt, orc, gotrees, bs):
+    """A node's channel edge names the rotation its channel data is stored in; the channel modes
+    turn the voxel-local hit point by it (BVH2Common.cpp:2492-2505) with the transforms of the
+    brick_size + 1 lattice: input axis i lands on the rotation's output axis, mirrored axes as
+    bs - x. Checked against the unrotated quotients of the same hits and the axis maps of the
+    subgroup generator restated in numpy."""
+    pos, dir_ = sphere_rays(2000)
+    p = no_eps(lrt)
+    p["render_mode"] = 23
+    tree = gotrees[1] if bs == 1 else gotrees[2]
+    plain = oracle_trace(orc, lrt, dag_scene(lrt, de.encode(*tree, bs), p), pos, dir_)
+    arrays = de.encode(*tree, bs, channel_rotate=True, seed=11)
+    rot = oracle_trace(orc, lrt, dag_scene(lrt, arrays, p), pos, dir_)
+    hit = plain["primId"] != MISS
+    assert hit.mean() > 0.2 and np.array_equal(hit, rot["primId"] != MISS)
+    assert np.array_equal(plain["t"].view(np.uint32), rot["t"].view(np.uint32))
+
+    def q(h):
+        c0, c1 = h["coords"][hit, 0].view(np.uint32), h["coords"][hit, 1].view(np.uint32)
+        return np.stack([c0 & 0xFFFF, c0 >> 16, c1], 1).astype(np.int64)
+    q0, qr = q(plain), q(rot)
+    rid = arrays[1]["channels_edge"]["rotation_id"][rot["primId"][hit]]
+    maps = _axis_maps(bs + 1)
+    n_rot = 0
+    for j in range(q0.shape[0]):
+        for i, (o, sgn) in enumerate(maps[int(rid[j])]):
+            if sgn > 0:
+                assert qr[j, o] == q0[j, i]
+            elif bs == 1:
+                assert abs(qr[j, o] - (65535 - q0[j, i])) <= 1
+            else:
+                assert qr[j, o] == 65535   # (bs - x) leaves the unit cell: clamped
+        n_rot += int(rid[j]) != 0
+    assert n_rot > q0.shape[0] // 2

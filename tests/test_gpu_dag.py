@@ -71,7 +71,7 @@ def test_reference_dag_render_modes(lrt, orc, torch_dev, ref_dag, render):
 
 
 ENCODED = [dict(bs=1), dict(bs=2, rotate=True, add=True), dict(bs=2, rotate=True, surface=True),
-           dict(bs=1, lod=True, surface=True)]
+           dict(bs=1, lod=True, surface=True), dict(bs=1, channel_rotate=True), dict(bs=2, rotate=True, channel_rotate=True)]
 
 
 @pytest.fixture(scope="module")
@@ -100,6 +100,9 @@ def test_encoded_dags(lrt, orc, torch_dev, encoded, case):
     assert (got["primId"] != MISS).mean() > 0.1
     _check(torch_dev, g, orc, s, pos, dirs, _preset(g, 2, 8))
     _check(torch_dev, g, orc, s, pos, dirs, _preset(g, 4, 23))
+    if ENCODED[case].get("channel_rotate"):   # channel modes: quotients turned by the channel rotation
+        for mode, render in ((0, 23), (3, 27), (1, 23)):
+            _check(torch_dev, g, orc, s, pos, dirs, _preset(g, mode, render))
     if ENCODED[case].get("lod"):
         for lod in (2.0, 3.0):
             _check(torch_dev, g, orc, s, pos, dirs, _preset(g, 0, 1, fixed_lod=1, lod=lod))

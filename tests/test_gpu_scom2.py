@@ -183,3 +183,32 @@ def test_fast_traversal_instances_persistent_any(lrt, orc, torch_dev, plain_tree
     torch.cuda.synchronize()
     want = orc.trace_any(orc.Scene.from_scene(s), pos.cpu().numpy(), dirs.cpu().numpy())
     assert np.array_equal(flags.cpu().numpy().astype(np.uint32), want)
+
+
+@pytest.fixture(scope="module")
+def channel_trees(lrt):
+    import scom2_encode as se
+    from test_coctree_cpu import SPHERES
+    out = {}
+    for bs in (1, 2):
+        nodes, values = lrt.sdf_spheres_global_octree(SPHERES, 5 if bs == 1 else 4, bs, 0)
+        out[bs] = se.encode(nodes, values, brick_size=bs, channels=True, rotate=True, channel_rotate=True,
+                            shift=True, unique_every=6, volume_every=9, seed=7)
+    return out
+
+
+@pytest.mark.parametrize("bs", [1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 3, 4])
+def test_multi_node_channel_trees(lrt, orc, torch_dev, channel_trees, bs, mode):
+    """Multi-node trees whose leaves carry channel links (tests/scom2_encode.py): primId is the
+    channel link, and the channel render modes turn the quotients by the composed channel rotation
+    (BVH2Common.cpp:2731-2747)."""
+    g = lrt
+    _say(f"scom2 multi-node channel tree, brick size {bs}, intersection mode {mode}")
+    s = g.SceneRT()
+    s.AddInstance(s.AddGeom_SCom2(*channel_trees[bs]))
+    s.CommitScene()
+    pos, dirs = _rays(torch_dev, g)
+    for render in (23, 27, 8, 5, 1):
+        got = _check(torch_dev, g, orc, s, pos, dirs, _preset(g, mode, render))
+        assert (got["primId"] != MISS).mean() > 0.1

@@ -190,3 +190,68 @@ def test_plain_tree_host_checks(lrt, plain):
     n2[1] = (n2[1] & np.uint32(0xFC000000)) | np.uint32(n.size + 100)   # first link points past the words
     with pytest.raises(lrt.LrtError):
         scom2_scene(lrt, (h, n2, b))
+
+
+# ---- multi-node trees with channels ----
+def _channel_quotients(h, hit):
+    c0, c1 = h["coords"][hit, 0].view(np.uint32), h["coords"][hit, 1].view(np.uint32)
+    return np.stack([c0 & 0xFFFF, c0 >> 16, c1], 1).astype(np.int64)
+
+
+@pytest.mark.parametrize("bs", [1, 2])
+def test_multi_node_channels_turn_the_quotients(lrt, orc, bs):
+    """Multi-node trees (the reference file's field positions) whose leaves carry a channel link:
+    hits name the channel link as primId, and the channel render modes turn the voxel-local hit
+    point by the composed rotation rot_add[node rotation * 48 + channel rotation]
+    (BVH2Common.cpp:2731-2747) with the transforms of the brick_size + 1 lattice. Pinned against
+    the same tree stored without rotations (identity: the point's offset in its voxel) and the
+    axis maps of the subgroup generator restated in numpy; a composition in the other order would
+    turn many hits differently."""
+    import dag_encode as de
+    import scom2_encode as se
+    from test_coctree_cpu import SPHERES
+    from test_dag_cpu import _axis_maps
+    nodes, values = lrt.sdf_spheres_global_octree(SPHERES, 5 if bs == 1 else 4, bs, 0)
+    plain_enc = se.Scom2Encoder(nodes, values, brick_size=bs, channels=True, seed=5)
+    plain = plain_enc.encode()
+    rot_enc = se.Scom2Encoder(nodes, values, brick_size=bs, channels=True, rotate=True, channel_rotate=True, seed=5)
+    rotated = rot_enc.encode()
+    assert int(plain[0]["has_multi_nodes"]) == 1 and int(plain[0]["has_channels"]) == 1
+    pos, dir_ = sphere_rays(2500)
+    p = no_eps(lrt)
+    p["render_mode"] = 23   # MULTI_RENDER_MODE_CHANNEL
+    a = oracle_trace(orc, lrt, scom2_scene(lrt, plain, p), pos, dir_)
+    b = oracle_trace(orc, lrt, scom2_scene(lrt, rotated, p), pos, dir_)
+    hit = a["primId"] != MISS
+    assert hit.mean() > 0.2 and np.array_equal(hit, b["primId"] != MISS)
+    assert np.array_equal(a["t"].view(np.uint32), b["t"].view(np.uint32))
+    assert a["primId"][hit].max() < len(plain_enc.channel_info)
+    # identity: the hit point's offset in its voxel (a point on a voxel face may sit in either)
+    q0 = _channel_quotients(a, hit).astype(np.float64) / 65535.0
+    pt = pos[hit, :3].astype(np.float64) + a["t"][hit, None].astype(np.float64) * dir_[hit, :3]
+    cells = 2 ** (5 if bs == 1 else 4) * bs
+    frac = (pt + 1.0) * cells / 2.0
+    frac -= np.floor(frac)
+    err = np.minimum(np.abs(frac - q0), 1.0 - np.abs(frac - q0))
+    assert np.percentile(err, 99) < 1e-3
+    # rotated: the composed rotation's axis map
+    q0, qr = _channel_quotients(a, hit), _channel_quotients(b, hit)
+    table = de.rot_add_reference(2).reshape(48, 48)
+    maps = _axis_maps(bs + 1)
+    n_turned = 0
+    for j, ch in enumerate(b["primId"][hit]):
+        q, r = rot_enc.channel_info[int(ch)]
+        crot = int(table[q, r])
+        for i, (o, sgn) in enumerate(maps[crot]):
+            if sgn > 0:
+                assert qr[j, o] == q0[j, i]
+            elif bs == 1:
+                assert abs(qr[j, o] - (65535 - q0[j, i])) <= 1
+            else:
+                assert qr[j, o] == 65535
+        n_turned += crot != 0
+    assert n_turned > hit.sum() // 2
+    # other render modes keep the plain records (primId: the channel link)
+    p["render_mode"] = 8
+    c = oracle_trace(orc, lrt, scom2_scene(lrt, rotated, p), pos, dir_)
+    assert np.array_equal(c["primId"][hit], b["primId"][hit]) and not np.any(c["coords"][hit, :2])
