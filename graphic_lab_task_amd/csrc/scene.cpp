@@ -100,12 +100,17 @@ void include4(Box4& b, float4 p) {
 // add_border_nodes_rec for frame octrees (BVH2Common_host.cpp:1348-1383): one box per leaf
 // whose corner values change sign; leftOffset keeps the octree node index. The textured octree's
 // version (:1392-1427) tests a positive value against -sqrt(3)*d instead of 0, which every
-// positive value passes, so both node types share this walk.
+// positive value passes, so both node types share this walk. Children must follow their parent
+// (no cycle can then form) and the octree be at most FRAME_MAX_DEPTH levels deep, which bounds
+// this recursion; the reference walks any offsets it is given.
+constexpr uint32_t FRAME_MAX_DEPTH = 30;
 template <class NodeT>
 void add_border_nodes_rec(const NodeT* nodes, size_t n_nodes, std::vector<BVHNode>& out, uint32_t idx,
-                          float3 p, float d) {
+                          float3 p, float d, uint32_t depth = 0) {
   const uint32_t ofs = nodes[idx].offset;
   if (ofs != 0 && size_t(ofs) + 8 > n_nodes) throw std::invalid_argument("frame octree child offset out of range");
+  if (ofs != 0 && ofs <= idx) throw std::invalid_argument("frame octree children must follow their parent");
+  if (ofs != 0 && depth >= FRAME_MAX_DEPTH) throw std::invalid_argument("frame octree deeper than 30 levels");
   if (ofs == 0) {
     bool less = false, more = false;
     for (int i = 0; i < 8; i++) {
@@ -126,7 +131,7 @@ void add_border_nodes_rec(const NodeT* nodes, size_t n_nodes, std::vector<BVHNod
     for (int i = 0; i < 8; i++) {
       const float ch_d = d / 2;
       const float3 ch_p = 2.0f * p + make3(float((i & 4) >> 2), float((i & 2) >> 1), float(i & 1));
-      add_border_nodes_rec(nodes, n_nodes, out, ofs + i, ch_p, ch_d);
+      add_border_nodes_rec(nodes, n_nodes, out, ofs + i, ch_p, ch_d, depth + 1);
     }
   }
 }

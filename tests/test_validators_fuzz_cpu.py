@@ -1,7 +1,8 @@
-"""The host checks of the compact formats (AddGeom_COctreeV3, AddGeom_SdfDAG, AddGeom_SCom2) stand
-between untrusted arrays and kernels that read without bounds checks: every word a ray can make
-a walk read must lie inside the geometry's arrays. Random corruptions of the reference's
-engine-scene files and of an encoded plain SCom2 tree are either refused with LrtError or traced
+"""The host checks of the SDF formats (AddGeom_SdfFrameOctree, AddGeom_SdfSVS, AddGeom_SdfSBS,
+AddGeom_SdfMultiOctree, AddGeom_COctreeV3, AddGeom_SdfDAG, AddGeom_SCom2) stand between untrusted
+arrays and kernels that read without bounds checks: every word a ray can make a walk read must
+lie inside the geometry's arrays. Random corruptions of the reference's files and of an encoded
+plain SCom2 tree are either refused with LrtError or traced
 by the oracle, whose walks read what the kernels read; under the host-sanitizer build (the
 oracle and library built with AddressSanitizer) a read outside the arrays of an accepted tree is
 reported there. Outside that build the test checks that every corruption is refused or traced
@@ -78,3 +79,35 @@ def test_corrupted_compact_formats_are_refused_or_traced(lrt, orc):
             else:
                 refused += 1
     assert accepted > 20 and refused > 20, (accepted, refused)
+
+
+def test_corrupted_hot_path_formats_are_refused_or_traced(lrt, orc):
+    rng = np.random.default_rng(7)
+    pos, dir_ = sphere_rays(300)
+    frame = lrt.load_sdf_frame_octree(os.path.join(GOLD, "engine_sdf_frame_octree_0.bin"))
+    svs = lrt.load_sdf_svs(os.path.join(GOLD, "test_10_svs.bin"))
+    sbs = lrt.load_sdf_sbs(os.path.join(GOLD, "test_10_sbs.bin"))
+    mo_n, mo_v = lrt.load_sdf_multi_octree(os.path.join(GOLD, "engine_sdf_multi_octree_0.bin"))
+
+    def corrupt_rec(a, flips):
+        return _corrupt(rng, a.view(np.uint32).reshape(-1), flips).view(a.dtype).reshape(a.shape)
+
+    accepted = refused = 0
+    for trial in range(24):
+        flips = 1 + trial % 4
+        fr = corrupt_rec(frame, flips)
+        sv = corrupt_rec(svs, flips)
+        sb_n = corrupt_rec(sbs[1], flips)
+        sb_v = _corrupt(rng, sbs[2], flips)
+        mn = corrupt_rec(mo_n, flips)
+        cases = [lambda s, fr=fr: s.AddInstance(s.AddGeom_SdfFrameOctree(fr)),
+                 lambda s, sv=sv: s.AddInstance(s.AddGeom_SdfSVS(sv)),
+                 lambda s, sb_n=sb_n: s.AddInstance(s.AddGeom_SdfSBS(sbs[0], sb_n, sbs[2], *sbs[3:])),
+                 lambda s, sb_v=sb_v: s.AddInstance(s.AddGeom_SdfSBS(sbs[0], sbs[1], sb_v, *sbs[3:])),
+                 lambda s, mn=mn: s.AddInstance(s.AddGeom_SdfMultiOctree(mn, mo_v))]
+        for build in cases:
+            if _trace(lrt, orc, build, pos, dir_):
+                accepted += 1
+            else:
+                refused += 1
+    assert accepted > 10 and refused > 10, (accepted, refused)
