@@ -74,14 +74,17 @@ lrt_status lrt_add_triangles(lrt_scene* s, const float* vpos, size_t n_verts, si
                              uint32_t* out_geom_id);
 
 /* Replaces BVHRT::AddGeom_SdfFrameOctree (BVH2Common_host.cpp:501-534).
- * nodes: SdfFrameOctreeNode[n_nodes] (36 B each, sdf/simple/sdf_simple_shared.h:23-27). */
+ * nodes: SdfFrameOctreeNode[n_nodes] (36 B each, sdf/simple/sdf_simple_shared.h:23-27). Children
+ * blocks must lie inside the array, after their parent, at most 30 levels deep; otherwise
+ * LRT_ERR_INVALID_ARG with the reason (the reference walks any offsets). */
 lrt_status lrt_add_sdf_frame_octree(lrt_scene* s, const void* nodes, size_t n_nodes, uint32_t* out_geom_id);
 
 /* Replaces BVHRT::AddGeom_SdfFrameOctreeTex (BVH2Common_host.cpp:697-731).
  * nodes: SdfFrameOctreeTexNode[n_nodes] (104 B each: 16 texture coordinates, 8 values, child
  * offset, material id; sdf/simple/sdf_simple_shared.h:57-63). Traced as a frame octree; hits
  * report TYPE_SDF_FRAME_OCTREE_TEX and coords[0..1] = the corner texture coordinates
- * interpolated trilinearly at the hit point (BVH2Common.cpp:689-710). */
+ * interpolated trilinearly at the hit point (BVH2Common.cpp:689-710). Checked as
+ * lrt_add_sdf_frame_octree. */
 lrt_status lrt_add_sdf_frame_octree_tex(lrt_scene* s, const void* nodes, size_t n_nodes, uint32_t* out_geom_id);
 
 /* Replaces BVHRT::AddGeom_GraphicsPrim (BVH2Common_host.cpp:767-903). header: GraphicsPrimHeader
