@@ -172,6 +172,10 @@ uint32_t Scene::add_scom2(const SCom2Header& header, const uint32_t* nodes, size
   if (h.bits_per_value < 1 || h.values_per_uint < 1 || uint64_t(h.bits_per_value) * h.values_per_uint > 32)
     throw std::invalid_argument("SCom2: values_per_uint x bits_per_value must fit one word");
   if (h.uints_per_link < 1 || h.links_offset < 1) throw std::invalid_argument("SCom2: links need at least one word");
+  // the unpacking shifts words by these fields: 32 or more is no shift the walk could mean
+  if (h.child_rot_shift >= 32 || h.child_add_shift >= 32 || h.children_types_shift >= 32 ||
+      h.base_reference_shift >= 32 || h.children_active_bits_shift >= 32 || h.reference_bits >= 32)
+    throw std::invalid_argument("SCom2: header shift fields must be below 32");
   std::vector<uint32_t> local(nodes, nodes + n_nodes);
   local.insert(local.end(), bricks, bricks + n_bricks);
   SCom2Header lh = h;

@@ -73,6 +73,17 @@ def test_corrupted_compact_formats_are_refused_or_traced(lrt, orc):
         cases.append(lambda s, sn=sn: s.AddInstance(s.AddGeom_SCom2(sc2[0], sn, sc2[2])))
         pn = _corrupt(rng, plain[1], flips)
         cases.append(lambda s, pn=pn: s.AddInstance(s.AddGeom_SCom2(plain[0], pn, plain[2])))
+        # headers: shifts, masks, sizes and offsets the walks take as given
+        ch = coct_h.copy()
+        chw = _corrupt(rng, ch.reshape(1).view(np.uint32), 1).view(ch.dtype).reshape(ch.shape)
+        cases.append(lambda s, chw=chw: s.AddInstance(s.AddGeom_COctreeV3(chw, coct_w, 0)))
+        for hsrc, n_, b_ in ((sc2[0], sc2[1], sc2[2]), (plain[0], plain[1], plain[2])):
+            hh = np.array(hsrc).copy()
+            hw = _corrupt(rng, hh.reshape(1).view(np.uint32), 1).view(hh.dtype).reshape(hh.shape)
+            cases.append(lambda s, hw=hw, n_=n_, b_=b_: s.AddInstance(s.AddGeom_SCom2(hw, n_, b_)))
+        dh = np.array(dag[0]).copy()
+        dhw = _corrupt(rng, dh.reshape(1).view(np.uint32), 1).view(dh.dtype).reshape(dh.shape)
+        cases.append(lambda s, dhw=dhw: s.AddInstance(s.AddGeom_SdfDAG(dhw, dag[1], dag[2], dag[3], dag[4])))
         for build in cases:
             if _trace(lrt, orc, build, pos, dir_):
                 accepted += 1
@@ -100,7 +111,11 @@ def test_corrupted_hot_path_formats_are_refused_or_traced(lrt, orc):
         sb_n = corrupt_rec(sbs[1], flips)
         sb_v = _corrupt(rng, sbs[2], flips)
         mn = corrupt_rec(mo_n, flips)
+        tex = np.zeros(frame.size, lrt.FRAME_TEX_NODE_DTYPE)
+        tex["values"], tex["offset"] = frame["values"], frame["offset"]
+        tx = corrupt_rec(tex, flips)
         cases = [lambda s, fr=fr: s.AddInstance(s.AddGeom_SdfFrameOctree(fr)),
+                 lambda s, tx=tx: s.AddInstance(s.AddGeom_SdfFrameOctreeTex(tx)),
                  lambda s, sv=sv: s.AddInstance(s.AddGeom_SdfSVS(sv)),
                  lambda s, sb_n=sb_n: s.AddInstance(s.AddGeom_SdfSBS(sbs[0], sb_n, sbs[2], *sbs[3:])),
                  lambda s, sb_v=sb_v: s.AddInstance(s.AddGeom_SdfSBS(sbs[0], sbs[1], sb_v, *sbs[3:])),
