@@ -123,7 +123,8 @@ lrt_status lrt_add_sdf_dag(lrt_scene* s, const void* header, const void* nodes, 
  * sdf/scom2/scom_shared.h:21-75); nodes, bricks: the tree's u32 words (the header's
  * nodes_arr_offset and bricks_arr_offset are replaced by where the words land, as the reference
  * does). Checked when added: every word a ray can make the walk read lies inside the tree,
- * rotations are below 48 and the tree is at most 15 levels deep; a failed check returns
+ * rotations are below 48, the header's shift fields are below 32 and the tree is at most 15
+ * levels deep; a failed check returns
  * LRT_ERR_INVALID_ARG with the reason. Where the reference takes its fast traversal
  * (SCom2IntersectFast: the scene's only SCom2 tree is plain -- no multi-nodes, no channels, 3^3
  * values, one-word links -- and the preset renders LAMBERT_NO_TEX) so does this library; a plain
