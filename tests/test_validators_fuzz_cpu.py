@@ -119,6 +119,8 @@ def test_corrupted_hot_path_formats_are_refused_or_traced(lrt, orc):
                  lambda s, sv=sv: s.AddInstance(s.AddGeom_SdfSVS(sv)),
                  lambda s, sb_n=sb_n: s.AddInstance(s.AddGeom_SdfSBS(sbs[0], sb_n, sbs[2], *sbs[3:])),
                  lambda s, sb_v=sb_v: s.AddInstance(s.AddGeom_SdfSBS(sbs[0], sbs[1], sb_v, *sbs[3:])),
+                 lambda s, sh=_corrupt(rng, np.array(sbs[0]).reshape(1).view(np.uint32), 1).view(
+                     np.array(sbs[0]).dtype).reshape(()): s.AddInstance(s.AddGeom_SdfSBS(sh, sbs[1], sbs[2], *sbs[3:])),
                  lambda s, mn=mn: s.AddInstance(s.AddGeom_SdfMultiOctree(mn, mo_v))]
         for build in cases:
             if _trace(lrt, orc, build, pos, dir_):
